@@ -1,0 +1,237 @@
+/*
+ * mfa_hip.h -- C ABI of libmfa_hip, the MI355X implementation of MFA_PROG's
+ * per-timestep force-and-integrate loop (md_main.f90:68-184).
+ *
+ * The reference passes every array through `module commons` and its hot-path
+ * subroutines take no arguments (or one mode integer).  Here the same state
+ * lives in an opaque context that owns device memory; each reference
+ * subroutine becomes one entry point below, called in the same order by the
+ * Fortran driver (mfa_amd/fortran/md_main_hip.f90) through iso_c_binding.
+ *
+ * Conventions
+ *   - every entry point returns int32 status: 0 = OK, otherwise an MFA_ERR_*
+ *     code; mfa_last_error(ctx) gives the message (the reference prints and
+ *     `stop`s, e.g. init_system.f90:141-143; the Fortran driver does that).
+ *   - arrays cross the boundary in the reference's layout: Fortran (3,N)
+ *     column-major == C double[N][3]; a_type and mic are int32; types are
+ *     1-based (1..4) like a_type (mfa_common.f90:343).
+ *   - a context is not thread-safe; calls are ordered on the context's HIP
+ *     stream.  The per-subroutine entry points synchronise before returning,
+ *     as the reference's subroutines are synchronous.
+ *   - energy out-pointers may be NULL: the energy reduction is then skipped.
+ */
+#ifndef MFA_HIP_H
+#define MFA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFA_NTYPE 4 /* n_type with WALL 2 (mfa_common.f90:47-64) */
+
+enum {
+  MFA_OK = 0,
+  MFA_ERR_ARG = 1,          /* bad argument / unsupported option */
+  MFA_ERR_HIP = 2,          /* HIP runtime error */
+  MFA_ERR_LIST_OVERFLOW = 3,/* neighbour list capacity exceeded */
+  MFA_ERR_CELL = 4,         /* particle outside the cell grid */
+  MFA_ERR_NOMEM = 5,
+  MFA_ERR_NAN = 6,          /* non-finite force or position */
+  MFA_ERR_IO = 7
+};
+
+/* Precision forms of the force pass. */
+enum {
+  MFA_FP64 = 0,  /* plain FP64 pair/bond math, FP64 accumulation          */
+  MFA_MIXED = 1  /* FP32 pair/bond math, int64 fixed-point accumulation;
+                    FP64 positions, velocities, integration and reductions */
+};
+
+/*
+ * Raw run-time inputs: the numbers system_input, mfa_input and the
+ * compile-time switches of control_simulation.h provide to the reference.
+ * Filled by the Fortran readers (init_system.f90:12-144,
+ * init_params.f90:137-244) or by the Python mirror.
+ * Type tables are indexed [i][j] with 0-based types.
+ */
+typedef struct mfa_inputs {
+  /* system_input */
+  int32_t n_mon, n_chain, n_cell_w_x, n_cell_w_y;  /* init_system.f90:26 */
+  int32_t n_mon_d, n_chain_d;                      /* :32 */
+  int32_t n_mon_e, n_chain_e, inter4;              /* :41-47 (PARTICLE_4) */
+  int32_t pad0;
+  double a_w, sigma_w, a_w4, sigma_w4;             /* :65-81 */
+  double charge[MFA_NTYPE];                        /* :83-91 */
+  double alpha_c;                                  /* :94 */
+  int32_t n_kmax, pad1;                            /* :94 */
+  /* mfa_input */
+  int32_t s_time, n_relax, n_obser, n_safe;        /* init_params.f90:138-141 */
+  double dt, temp;                                 /* :142,145 */
+  double x_space, y_space, z_space_wall;           /* :148-150 */
+  double mass_type[MFA_NTYPE];                     /* :155-162 */
+  double epsil[MFA_NTYPE][MFA_NTYPE];              /* as read (before x4) */
+  double sigma_ii[MFA_NTYPE];
+  double friction[MFA_NTYPE];
+  double k_bend, alpha_eq;                         /* :214-215 (BENDING) */
+  int64_t iseed;                                   /* :223 */
+  double k_chain, r_chain;                         /* :225-226 */
+  double va_spring_twall[3];                       /* :232 */
+  /* control_simulation.h switches, made run-time */
+  int32_t system;       /* SYSTEM: 2 (no brush) or 3 (brush + walls)       */
+  int32_t symmetry;     /* SYMMETRY: 0 channel (PBC x,y), 1 bulk (PBC xyz) */
+  int32_t solvent;      /* SOLVENT: 0 poor, 1 good                          */
+  int32_t thermostat;   /* THERMOSTAT: 0 DPD, -1 none (NVE)                */
+  int32_t particle_4;   /* PARTICLE_4 defined                               */
+  int32_t bending;      /* BENDING defined                                  */
+  double dpd_cut_off;   /* DPD_CUT_OFF value, 0 = undefined                 */
+  /* options the reference hard-codes (documented patches of the survey) */
+  double skin;          /* init_params.f90:800 (0 -> reference 0.4)          */
+  double lj_rc;         /* >0: every LJ range = lj_rc*sigma (P3); 0 = menu   */
+  double rc_coul;       /* >0: real-space Coulomb cutoff; 0 = LJ range (P5)  */
+  int32_t ewald_compat; /* 1: reference's 1/pi real-space factor (bug)      */
+  int32_t force_switch_on; /* FORCE_SWITCH_ON defined: ramp when s_time == 1 */
+} mfa_inputs;
+
+/*
+ * Derived constants (init_params.f90:395-845, init_config.f90:11-22,229-239,
+ * make_binning_boxes.f90:14-64), produced by mfa_derive().  Default-real
+ * literals of the reference are applied as float32 values, as its Makefile
+ * (no -r8) compiles them.
+ */
+typedef struct mfa_params {
+  int32_t n_part, n_mon_tot;
+  int32_t n_mon, n_chain, n_mon_d, n_chain_d, n_mon_e, n_chain_e;
+  int32_t part_init_d, part_init_e;     /* first melt / particle-4 index (0-based) */
+  int32_t system, symmetry, thermostat, dpd_weight;
+  int32_t dpd_cut_off_on, coulomb, bending, ewald_compat;
+  double boundary[3], inv_boundary[3];
+  double dt, dt_2, temp;
+  double mass_type[MFA_NTYPE], inv_mass_type[MFA_NTYPE];
+  double epsil[MFA_NTYPE][MFA_NTYPE];   /* 4*eps */
+  double sigma[MFA_NTYPE][MFA_NTYPE];
+  double sigma_2[MFA_NTYPE][MFA_NTYPE];
+  double range_2[MFA_NTYPE][MFA_NTYPE];
+  double inv_range_2[MFA_NTYPE][MFA_NTYPE];
+  double e_shift[MFA_NTYPE][MFA_NTYPE];
+  double list_2[MFA_NTYPE][MFA_NTYPE];  /* Verlet-list criterion (binning.f90:101-105) */
+  double r_cut_max, r_cut_max_2;
+  double r_cut_dpd, r_cut_dpd_2;
+  double skin, skin_2, skin_half_2;     /* skin_half_2 = (skin/2)^2, check_skin.f90:149 */
+  double friction1, sig;                /* sig = sqrt(2 T gamma / dt), init_params.f90:845 */
+  double k_chain, r_chain_2, inv_r_chain_2;
+  double a_w, sigma_wall[MFA_NTYPE], a_wall[MFA_NTYPE], z_space_wall;
+  double va_spring_twall[3];
+  double k_bend, alpha_eq;
+  int32_t n_bin[3], delta_bin[3];
+  double r_bin[3], inv_r_bin[3];
+  double charge[MFA_NTYPE];
+  double alpha_c, sqrt_alpha, inv_sqrt_pi_real; /* real-space erfc prefactor */
+  double rc_coul_2;                     /* real-space Coulomb cutoff^2 */
+  double v_coul_self;                   /* filled by mfa_create from charges */
+  int32_t n_kmax, pad0;
+  uint64_t seed;
+  /* FORCE_SWITCH_ON ramp (init_force_switch_on.f90:15-26): r_2 >= r_2_min with
+   * r_2_min = (1 - t/fso_time) * fso_r2_init while t = step*dt < fso_time. */
+  double fso_r2_init, fso_time;
+} mfa_params;
+
+/* Observation sums returned by mfa_run_steps / mfa_energies. */
+typedef struct mfa_obs {
+  double v_fluid_fluid;   /* LJ energy (fluid_fluid.f90:110) */
+  double v_fluid_wall;    /* 9-3 wall energy (fluid_wall.f90:98,116) */
+  double v_intra_molec;   /* FENE energy incl. melt (intra_molec.f90:63,537) */
+  double v_bend;          /* bending energy (bending.f90) */
+  double v_coul;          /* Ewald real + k + self + dipolar */
+  double t_kin;           /* 0.5 sum m v^2 (verlet_velocities.f90:155-177) */
+  double virial;          /* sum over pairs r_ij . F_ij (conservative) */
+  double wall_fz_bottom;  /* sum of wall z-forces from the z=0 wall */
+  double wall_fz_top;     /* sum of wall z-forces from the z=Lz wall */
+  double z_moment;        /* sum of z over brush monomers (type 2) */
+  int64_t n_rebuilds;     /* list rebuilds during the call */
+} mfa_obs;
+
+typedef struct mfa_ctx mfa_ctx;
+
+/* ABI self-check: sizes of the structs as compiled into the library. */
+int32_t mfa_abi_sizes(int32_t* sizeof_inputs, int32_t* sizeof_params, int32_t* sizeof_obs);
+
+/* Derived constants.  Replaces the derivation part of init_params
+ * (init_params.f90:395-845), the box of init_config (init_config.f90:11-22)
+ * and make_binning_boxes (make_binning_boxes.f90:14-64).  Host-only. */
+int32_t mfa_derive(const mfa_inputs* in, mfa_params* out);
+
+/* Create/destroy a device context for n_part particles.  Replaces the
+ * allocations of init_system.f90:228-299 and make_binning_boxes.f90:70-78.
+ * precision: MFA_FP64 or MFA_MIXED.  list_capacity: full neighbour-list slots
+ * per particle (0 = sized from the density). */
+int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t list_capacity, mfa_ctx** out);
+int32_t mfa_destroy(mfa_ctx* ctx);
+int32_t mfa_set_precision(mfa_ctx* ctx, int32_t precision);
+/* Lanes that share one particle's neighbour list in the force pass
+ * (1, 2, 4, 8 or 16; default 8).  A tuning knob with no reference
+ * counterpart; results are identical for every value in the mixed form. */
+int32_t mfa_set_threads_per_particle(mfa_ctx* ctx, int32_t tpp);
+const char* mfa_last_error(mfa_ctx* ctx);
+
+/* State upload/download, (3,N) layout.  Replaces the conf_old read
+ * (init_config.f90:110-209) and what store_config reads (store_config.f90:19-61).
+ * mic may be NULL (zeros).  After set_state the next mfa_binning/run call
+ * rebuilds the list, as init_config does (init_config.f90:251-255). */
+int32_t mfa_set_state(mfa_ctx* ctx, const double* r0, const double* v, const double* a,
+                      const int32_t* a_type, const int32_t* mic);
+int32_t mfa_get_state(mfa_ctx* ctx, double* r0, double* v, double* a, double* force, int32_t* mic);
+
+/* verlet_positions (verlet_positions.f90:301-406). */
+int32_t mfa_verlet_positions(mfa_ctx* ctx);
+/* check_skin (check_skin.f90:135-155): *f_skin = 1 if a rebuild is due. */
+int32_t mfa_check_skin(mfa_ctx* ctx, int32_t* f_skin);
+/* binning (binning.f90:11-127): cell list + Verlet list, resets the skin
+ * reference (binning.f90:125-127). */
+int32_t mfa_binning(mfa_ctx* ctx);
+/* Parity exports of the list: cell index per particle (x fastest,
+ * binning.f90:27-29), and the unordered pair set as (i<j), 1-based. */
+int32_t mfa_get_cells(mfa_ctx* ctx, int32_t* cell_xyz /* [N][3] */);
+int32_t mfa_get_pairs(mfa_ctx* ctx, int32_t* pairs /* [cap][2] */, int64_t cap, int64_t* n_pairs);
+/* force(:,:) = 0 (md_main.f90:88). */
+int32_t mfa_zero_force(mfa_ctx* ctx);
+/* fluid_fluid (fluid_fluid.f90:52-240) with dpd_forces (dpd_forces.f90:256-352)
+ * and ewald_real (ewald_real.f90:15-32).  The DPD noise of pair (i,j) at
+ * step `step` is a counter-based normal keyed by (seed, step, min, max). */
+int32_t mfa_fluid_fluid(mfa_ctx* ctx, int64_t step, double* v_ff, double* v_coul);
+/* fluid_wall(2) (fluid_wall.f90:81-123).  Only inter_type 2 is provided. */
+int32_t mfa_fluid_wall(mfa_ctx* ctx, int32_t inter_type, double* v_fw);
+/* intra_molec (intra_molec.f90:16-139): FENE, one thread per bond. */
+int32_t mfa_intra_molec(mfa_ctx* ctx, double* v_intra);
+/* verlet_velocities (verlet_velocities.f90:18-199); *t_kin = 0.5 sum m v^2. */
+int32_t mfa_verlet_velocities(mfa_ctx* ctx, double* t_kin);
+
+/* Whole loop body (md_main.f90:69-155) for n steps, steps first_step ..
+ * first_step+n-1, device resident: fused verlet_velocities + next
+ * verlet_positions, skin test as a wavefront max-reduction, counting-sort
+ * rebuild when due, one fused force pass.  obs (may be NULL) receives the
+ * energies of the last step. */
+int32_t mfa_run_steps(mfa_ctx* ctx, int64_t first_step, int32_t n, mfa_obs* obs);
+
+/* Energies and observation sums of the current state (forces recomputed for
+ * step `step`, the state is not advanced). */
+int32_t mfa_energies(mfa_ctx* ctx, int64_t step, mfa_obs* obs);
+
+/* Kernel timing of the force pass: average duration (ms) over the launches
+ * since the last reset, measured with HIP events on the context's stream. */
+int32_t mfa_timing(mfa_ctx* ctx, int32_t enable, double* force_ms_avg, int64_t* force_launches,
+                   double* integrate_ms_avg, int64_t* integrate_launches);
+
+/* Seeded input generators (the reference's conf_default/gen_brush/gen_droplet
+ * are replaced by these; conf_default.f90:144-217, gen_brush.f90:101-161).
+ * kind 1: brush + ideal solvent in a channel; kind 2: melt in a periodic box.
+ * Writes r0, v (Maxwell-Boltzmann at temp, COM momentum removed), a_type.
+ * Arrays are sized n_part = p->n_part.  Host-only. */
+int32_t mfa_generate(const mfa_params* p, int32_t kind, uint64_t seed, double head_z,
+                     double bond, double min_dist, double* r0, double* v, int32_t* a_type);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFA_HIP_H */

@@ -1,0 +1,135 @@
+"""Named workloads as raw inputs (the numbers system_input / mfa_input and the
+control_simulation.h switches would hold), plus small parity cases.
+
+Workloads (sizes as stated in the benchmark definitions):
+  brush9k   DPD polymer brush, 10x10x20 box, 100 chains x 30 + 6000 solvent
+  melt100k  Kremer-Grest melt, 2000 chains x 50 at density 0.85, periodic
+The choices the workload text leaves open are fixed here:
+  - heads at z=1.0 with x,y uniform (reference style, gen_brush.f90:9-30);
+  - 9-3 walls A_w=-2, sigma_w=1 (repulsive, as the shipped system_input:11);
+  - solvent is an ideal DPD fluid: epsilon(solvent, .) = 0;
+  - list skin 0.3 (the reference hard-codes 0.4, init_params.f90:800);
+  - the melt's DPD range equals its LJ cut-off 2.5 (no DPD_CUT_OFF);
+  - generated beads keep 0.8 sigma apart (melt) or 0.7 sigma (the brush at
+    grafting density 1 cannot be packed at 0.8).
+"""
+import numpy as np
+
+from . import lib
+
+NT = lib.NTYPE
+
+
+def base_inputs():
+    """mfa_input / system_input defaults: unit masses, sigma and epsilon."""
+    x = lib.Inputs()
+    x.s_time = 2  # > 1: no force switch-on ramp (init_config.f90:264-281)
+    x.n_relax = 0
+    x.n_obser = 0
+    x.n_safe = 1000
+    x.temp = 1.0
+    x.x_space = 1.0
+    x.y_space = 1.0
+    for t in range(NT):
+        x.mass_type[t] = 1.0
+        x.sigma_ii[t] = 1.0
+        x.friction[t] = 1.0
+        for u in range(NT):
+            x.epsil[t][u] = 1.0
+    x.k_chain = 30.0
+    x.r_chain = 1.5
+    x.a_w = -2.0
+    x.sigma_w = 1.0
+    x.a_w4 = -2.0
+    x.sigma_w4 = 1.0
+    x.inter4 = 1
+    x.system = 2
+    x.symmetry = 1
+    x.solvent = 1
+    x.thermostat = 0
+    x.iseed = 1
+    return x
+
+
+def _ideal_solvent(x, solvent_type=3):
+    s = solvent_type - 1
+    for t in range(NT):
+        x.epsil[s][t] = 0.0
+        x.epsil[t][s] = 0.0
+
+
+def brush(n_chain, n_mon, n_solvent, lx, ly, lz, x_space=1.0, dt=0.005, gamma=4.5,
+          dpd_cut_off=1.0, skin=0.3, seed=1001, thermostat=0):
+    """Brush grafted on the z=0 wall in a channel with 9-3 walls (SYSTEM 3,
+    SYMMETRY 0, WALL 2), ideal DPD solvent as single-bead 'melt' chains."""
+    x = base_inputs()
+    x.system = 3
+    x.symmetry = 0
+    x.n_mon, x.n_chain = n_mon, n_chain
+    x.n_mon_d, x.n_chain_d = 1, n_solvent
+    x.x_space = x_space
+    x.y_space = x_space
+    x.n_cell_w_x = int(round(lx / x_space))
+    x.n_cell_w_y = int(round(ly / x_space))
+    x.z_space_wall = lz
+    x.dt = dt
+    x.friction[0] = gamma
+    x.dpd_cut_off = dpd_cut_off
+    x.skin = skin
+    x.iseed = seed
+    x.thermostat = thermostat
+    _ideal_solvent(x)
+    return x
+
+
+def melt(n_chain, n_mon, density, lj_rc=2.5, dt=0.01, gamma=1.0, skin=0.3, seed=2002,
+         thermostat=0, n_cell=None):
+    """Periodic Kremer-Grest melt (SYSTEM 2, SYMMETRY 1), chains as melt
+    ('droplet') chains of type 3, LJ cut-off lj_rc (survey patch P3)."""
+    x = base_inputs()
+    x.system = 2
+    x.symmetry = 1
+    n = n_chain * n_mon
+    L = (n / density) ** (1.0 / 3.0)
+    nc = n_cell if n_cell else max(8, int(L))
+    x.n_cell_w_x = nc
+    x.n_cell_w_y = nc
+    x.x_space = L / nc
+    x.y_space = L / nc
+    x.z_space_wall = x.x_space * nc
+    x.n_mon, x.n_chain = 0, 0
+    x.n_mon_d, x.n_chain_d = n_mon, n_chain
+    x.dt = dt
+    x.friction[0] = gamma
+    x.lj_rc = lj_rc
+    x.skin = skin
+    x.iseed = seed
+    x.thermostat = thermostat
+    return x
+
+
+WORKLOADS = {
+    # plain FP64 DPD polymer brush (CPU-runnable reference case)
+    "brush9k": dict(kind=1, steps=1000, min_dist=0.7,
+                    make=lambda: brush(100, 30, 6000, 10.0, 10.0, 20.0, seed=1001)),
+    # Kremer-Grest melt, the headline workload
+    "melt100k": dict(kind=2, steps=5000, min_dist=0.8,
+                     make=lambda: melt(2000, 50, 0.85, seed=2002)),
+}
+
+
+def small_cases():
+    """Parity cases the oracle finishes in seconds."""
+    return {
+        "brush_small": (1, brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=0.5, seed=11)),
+        "melt_small": (2, melt(30, 20, 0.85, lj_rc=2.5, seed=12, n_cell=16)),
+        "melt_wca": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=13, n_cell=16)),
+    }
+
+
+def build(inputs, kind, seed=None, min_dist=0.8):
+    """Derive constants and generate the seeded initial state."""
+    p = lib.derive(inputs)
+    r, v, t = lib.generate(p, kind, inputs.iseed if seed is None else seed,
+                           head_z=1.0, bond=0.97, min_dist=min_dist)
+    return p, r, v, t

@@ -1,0 +1,1422 @@
+// libmfa_hip device side: state, cell list, neighbour list, force pass and
+// integrator for MI355X (gfx950), plus the C ABI declared in include/mfa_hip.h.
+//
+// Layout in HBM (one context = one system):
+//   r, v, a, f     double[N][3]   the reference's (3,N) arrays, AoS, so a
+//                                 neighbour gather is one 24-byte record
+//   type           int32[N]       a_type (1-based)
+//   mic            int32[N][3]    minimum-image counters (mic_count)
+//   r_ref          double[N][3]   r0_old + mic_old*L at the last rebuild
+//   cell_of        int32[N]       linear cell id (x fastest)
+//   cell_start     int32[C+1]     exclusive scan of cell counts
+//   cell_sorted    int32[N]       particle ids grouped by cell, ascending id
+//                                 inside a cell (the reference's bin_fluid order)
+//   nl, nn         int32[N][cap], int32[N]   full Verlet list (both directions)
+//
+// Everything in this file is compiled with -ffp-contract=off: the FP64 forms
+// reproduce the reference's operation order; the FP32 helpers of the mixed
+// form enable contraction locally.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mfa_hip.h"
+#include "mfa_rng.h"
+
+#define MFA_BLOCK 256
+#define MFA_NTERMS 12
+// term slots of the per-block partial sums
+enum {
+  T_VFF = 0, T_VFW, T_VINTRA, T_VBEND, T_VCOUL, T_TKIN, T_VIR, T_FZB, T_FZT, T_ZMOM, T_SPARE0, T_SPARE1
+};
+// force-pass term mask
+enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16 };
+// device flag slots
+enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_NFLAGS = 8 };
+
+// Fixed-point scale of the mixed form: 2^32 (force resolution 2.3e-10,
+// range +-2.1e9 per particle).
+#define MFA_FIX_SCALE 4294967296.0f
+#define MFA_FIX_INV (1.0 / 4294967296.0)
+
+struct KParams {
+  double L[3], invL[3];
+  double dt, dt_2, half_dt, half_dt_2;
+  double inv_mass_type[MFA_NTYPE], mass_type[MFA_NTYPE];
+  double epsil[MFA_NTYPE][MFA_NTYPE], sigma_2[MFA_NTYPE][MFA_NTYPE];
+  double range_2[MFA_NTYPE][MFA_NTYPE], inv_range_2[MFA_NTYPE][MFA_NTYPE];
+  double e_shift[MFA_NTYPE][MFA_NTYPE], list_2[MFA_NTYPE][MFA_NTYPE];
+  double r_cut_dpd_2, friction1, sig;
+  double k_chain, inv_r_chain_2;
+  double a_w, sigma_wall[MFA_NTYPE], a_wall[MFA_NTYPE], z_space_wall;
+  double va[3];
+  double inv_r_bin[3];
+  double skin_half_2;
+  int nb[3];
+  int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
+  int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
+  uint32_t seed;
+};
+
+struct mfa_ctx {
+  mfa_params p;
+  KParams k;
+  int precision;
+  int n;
+  int cap;
+  int tpp;  // threads per particle in the force pass
+  int n_cells;
+  bool need_rebuild;
+  hipStream_t stream;
+  double *r, *v, *a, *f, *r_ref;
+  int32_t *type, *mic;
+  int32_t *cell_of, *cell_count, *cell_start, *cell_fill, *cell_list, *cell_sorted;
+  int32_t *nl, *nn;
+  int32_t* flags;       // device flags
+  int32_t* h_flags;     // pinned mirror
+  double* part;         // per-block partial sums [blocks][MFA_NTERMS]
+  double* red;          // final sums [MFA_NTERMS]
+  double* h_red;        // pinned mirror
+  int max_blocks;
+  // timing: pairs of events recorded around each timed launch, read back
+  // only when the caller asks (no per-launch synchronisation)
+  bool timing;
+  std::vector<hipEvent_t> ev_force, ev_integ;
+  size_t ev_force_used, ev_integ_used;
+  double force_ms;
+  int64_t force_n;
+  double integ_ms;
+  int64_t integ_n;
+  std::string err;
+};
+
+static thread_local std::string g_noctx_err;
+
+extern "C" void mfa_internal_set_noctx_error(const char* msg) { g_noctx_err = msg; }
+
+static int set_err(mfa_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  else g_noctx_err = buf;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err((c), MFA_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));    \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+
+// PBC as the reference writes it: d - L*int(2*d/L), int() truncating
+// (binning.f90:89-93, fluid_fluid.f90:84-88, intra_molec.f90:25-29).
+__device__ __forceinline__ double pbc(double d, double L, double invL) {
+  const int m = (int)((2.0 * d) * invL);
+  return d - L * (double)m;
+}
+
+__device__ __forceinline__ void min_image(const KParams& k, double& dx, double& dy, double& dz) {
+  dx = pbc(dx, k.L[0], k.invL[0]);
+  dy = pbc(dy, k.L[1], k.invL[1]);
+  if (k.symmetry == 1) dz = pbc(dz, k.L[2], k.invL[2]);
+}
+
+__device__ __forceinline__ int cell_coord(double x, double inv_r_bin, int nb) {
+  // mod(int(r*inv_r_bin), n_bin) (binning.f90:27-29); Fortran mod == C %.
+  return ((int)(x * inv_r_bin)) % nb;
+}
+
+__device__ __forceinline__ bool is_head(const KParams& k, int i) {
+  return k.pinned_heads && i < k.n_brush && (i % k.n_mon) == 0;
+}
+
+// x**9 and friends as the integer-power chains a compiler emits.
+__device__ __forceinline__ double pow3(double x) { return (x * x) * x; }
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block-level sum of MFA_NTERMS doubles held per thread; thread 0 writes the
+// block's partials.  Deterministic (fixed tree).
+__device__ void block_partials(double* vals, double* part_out) {
+  __shared__ double sh[MFA_BLOCK / 64][MFA_NTERMS];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int t = 0; t < MFA_NTERMS; ++t) {
+    double v = vals[t];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) sh[wid][t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < MFA_NTERMS) {
+    double s = 0.0;
+    for (int w = 0; w < MFA_BLOCK / 64; ++w) s += sh[w][threadIdx.x];
+    part_out[(size_t)blockIdx.x * MFA_NTERMS + threadIdx.x] = s;
+  }
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ part, int nblocks,
+                                  double* __restrict__ out) {
+  // one block; thread t sums term t over blocks in index order
+  const int t = threadIdx.x;
+  if (t < MFA_NTERMS) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * MFA_NTERMS + t];
+    out[t] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integrator kernels (FP64).
+
+// verlet_positions (verlet_positions.f90:301-326, wrap 392-406).
+__device__ __forceinline__ void vp_one(const KParams& k, double* r, double* v, const double* a,
+                                       int* mic, bool& wrapped) {
+  for (int d = 0; d < 3; ++d) {
+    r[d] = (r[d] + k.dt * v[d]) + k.half_dt_2 * a[d];
+    v[d] = v[d] + k.half_dt * a[d];
+  }
+  const int nd = k.symmetry == 0 ? 2 : 3;
+  for (int d = 0; d < nd; ++d) {
+    if (r[d] > k.L[d]) {
+      r[d] = r[d] - k.L[d];
+      mic[d] += 1;
+      wrapped = true;
+    } else if (r[d] < 0.0) {
+      r[d] = r[d] + k.L[d];
+      mic[d] -= 1;
+      wrapped = true;
+    }
+  }
+}
+
+// check_skin for one particle (check_skin.f90:139-152).
+__device__ __forceinline__ bool skin_one(const KParams& k, const double* r, const double* rref,
+                                         const int* mic) {
+  double s = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double dd = (rref[d] - r[d]) - (double)mic[d] * k.L[d];
+    s = s + dd * dd;
+  }
+  return s > k.skin_half_2;
+}
+
+// verlet_velocities for one particle (verlet_velocities.f90:25-36,120-133).
+__device__ __forceinline__ void vv_one(const KParams& k, int i, int ti, double* f, double* v,
+                                       double* a) {
+  const double im = k.inv_mass_type[ti];
+  for (int d = 0; d < 3; ++d) {
+    a[d] = f[d] * im;
+    v[d] = v[d] + k.half_dt * a[d];
+  }
+  if (is_head(k, i)) {
+    const int c = i / k.n_mon;
+    const double sgn = c < k.n_chain / 2 ? 1.0 : -1.0;
+    v[0] = sgn * k.va[0];
+    v[1] = sgn * k.va[1];
+    v[2] = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      f[d] = 0.0;
+      a[d] = 0.0;
+    }
+  }
+}
+
+__global__ void k_verlet_positions(KParams k, double* __restrict__ r, double* __restrict__ v,
+                                   const double* __restrict__ a, int* __restrict__ mic,
+                                   const double* __restrict__ r_ref, int* __restrict__ flags,
+                                   int with_skin) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double over = 0.0;
+  if (i < k.n) {
+    double ri[3], vi[3], ai[3];
+    int mi[3];
+    for (int d = 0; d < 3; ++d) {
+      ri[d] = r[3 * (size_t)i + d];
+      vi[d] = v[3 * (size_t)i + d];
+      ai[d] = a[3 * (size_t)i + d];
+      mi[d] = mic[3 * (size_t)i + d];
+    }
+    bool wrapped = false;
+    vp_one(k, ri, vi, ai, mi, wrapped);
+    for (int d = 0; d < 3; ++d) {
+      r[3 * (size_t)i + d] = ri[d];
+      v[3 * (size_t)i + d] = vi[d];
+    }
+    if (wrapped)
+      for (int d = 0; d < 3; ++d) mic[3 * (size_t)i + d] = mi[d];
+    if (with_skin) {
+      double rr[3];
+      for (int d = 0; d < 3; ++d) rr[d] = r_ref[3 * (size_t)i + d];
+      over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
+    }
+  }
+  if (with_skin) {
+    over = wave_max(over);
+    if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[F_REBUILD], 1);
+  }
+}
+
+__global__ void k_check_skin(KParams k, const double* __restrict__ r,
+                             const double* __restrict__ r_ref, const int* __restrict__ mic,
+                             int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double over = 0.0;
+  if (i < k.n) {
+    double ri[3], rr[3];
+    int mi[3];
+    for (int d = 0; d < 3; ++d) {
+      ri[d] = r[3 * (size_t)i + d];
+      rr[d] = r_ref[3 * (size_t)i + d];
+      mi[d] = mic[3 * (size_t)i + d];
+    }
+    over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
+  }
+  over = wave_max(over);
+  if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[F_REBUILD], 1);
+}
+
+// verlet_velocities alone: writes v, a and zeroes head forces; kinetic energy
+// partials when `energy` (verlet_velocities.f90:155-177).
+__global__ void k_verlet_velocities(KParams k, double* __restrict__ f, double* __restrict__ v,
+                                    double* __restrict__ a, const int* __restrict__ type,
+                                    double* __restrict__ part, int energy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+  if (i < k.n) {
+    const int ti = type[i] - 1;
+    double fi[3], vi[3], ai[3];
+    for (int d = 0; d < 3; ++d) {
+      fi[d] = f[3 * (size_t)i + d];
+      vi[d] = v[3 * (size_t)i + d];
+    }
+    vv_one(k, i, ti, fi, vi, ai);
+    for (int d = 0; d < 3; ++d) {
+      v[3 * (size_t)i + d] = vi[d];
+      a[3 * (size_t)i + d] = ai[d];
+    }
+    if (is_head(k, i))
+      for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] = 0.0;
+    vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
+  }
+  if (energy) block_partials(vals, part);
+}
+
+// Fused: verlet_velocities of step s, then verlet_positions and check_skin of
+// step s+1.  Reads F, v, r, mic, r_ref; writes v, r (mic only on a wrap).
+__global__ void k_fused_integrate(KParams k, const double* __restrict__ f, double* __restrict__ v,
+                                  double* __restrict__ r, int* __restrict__ mic,
+                                  const double* __restrict__ r_ref, const int* __restrict__ type,
+                                  int* __restrict__ flags, double* __restrict__ part, int energy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double over = 0.0;
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+  if (i < k.n) {
+    const int ti = type[i] - 1;
+    double fi[3], vi[3], ai[3], ri[3], rr[3];
+    int mi[3];
+    for (int d = 0; d < 3; ++d) {
+      fi[d] = f[3 * (size_t)i + d];
+      vi[d] = v[3 * (size_t)i + d];
+      ri[d] = r[3 * (size_t)i + d];
+      mi[d] = mic[3 * (size_t)i + d];
+      rr[d] = r_ref[3 * (size_t)i + d];
+    }
+    vv_one(k, i, ti, fi, vi, ai);
+    if (energy) vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
+    bool wrapped = false;
+    vp_one(k, ri, vi, ai, mi, wrapped);
+    for (int d = 0; d < 3; ++d) {
+      r[3 * (size_t)i + d] = ri[d];
+      v[3 * (size_t)i + d] = vi[d];
+    }
+    if (wrapped)
+      for (int d = 0; d < 3; ++d) mic[3 * (size_t)i + d] = mi[d];
+    over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
+  }
+  over = wave_max(over);
+  if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[F_REBUILD], 1);
+  if (energy) block_partials(vals, part);
+}
+
+// ---------------------------------------------------------------------------
+// Cell list by counting sort (binning.f90:11-50).  Each rebuild kernel returns
+// at once unless the rebuild flag is set, so a step can enqueue the rebuild
+// unconditionally and let the skin test decide on the device.
+
+__global__ void k_cell_assign(KParams k, const double* __restrict__ r, int* __restrict__ cell_of,
+                              int* __restrict__ cell_count, int* __restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  const int cx = cell_coord(r[3 * (size_t)i + 0], k.inv_r_bin[0], k.nb[0]);
+  const int cy = cell_coord(r[3 * (size_t)i + 1], k.inv_r_bin[1], k.nb[1]);
+  const int cz = cell_coord(r[3 * (size_t)i + 2], k.inv_r_bin[2], k.nb[2]);
+  if (cx < 0 || cy < 0 || cz < 0) {  // the reference would index out of bounds
+    atomicMax(&flags[F_ERROR], MFA_ERR_CELL);
+    cell_of[i] = 0;
+    atomicAdd(&cell_count[0], 1);
+    return;
+  }
+  const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;
+  cell_of[i] = c;
+  atomicAdd(&cell_count[c], 1);
+}
+
+// Exclusive scan of the counts (one block), zeroing counts and fill cursors
+// for the next rebuild.
+__global__ void k_cell_scan(int n_cells, int* __restrict__ cell_count, int* __restrict__ cell_start,
+                            int* __restrict__ cell_fill, const int* __restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  __shared__ int wsum[MFA_BLOCK / 64];
+  __shared__ int carry;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n_cells; base += MFA_BLOCK) {
+    const int c = base + threadIdx.x;
+    const int x = c < n_cells ? cell_count[c] : 0;
+    int incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int off = carry;
+    for (int w = 0; w < wid; ++w) off += wsum[w];
+    if (c < n_cells) {
+      cell_start[c] = off + incl - x;
+      cell_count[c] = 0;
+      cell_fill[c] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == MFA_BLOCK - 1) carry = off + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cell_start[n_cells] = carry;
+}
+
+__global__ void k_cell_scatter(int n, const int* __restrict__ cell_of,
+                               const int* __restrict__ cell_start, int* __restrict__ cell_fill,
+                               int* __restrict__ cell_list, const int* __restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of[i];
+  const int slot = atomicAdd(&cell_fill[c], 1);
+  cell_list[cell_start[c] + slot] = i;
+}
+
+// Order each cell by particle id: the rank of an element is the number of
+// smaller ids in its cell.  The result is the reference's bin_fluid order
+// (particles enter a cell in increasing i_part, binning.f90:25-50).
+__global__ void k_cell_sort(int n, const int* __restrict__ cell_of,
+                            const int* __restrict__ cell_start, const int* __restrict__ cell_list,
+                            int* __restrict__ cell_sorted, const int* __restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = cell_list[p];
+  const int c = cell_of[i];
+  const int b = cell_start[c], e = cell_start[c + 1];
+  int rank = 0;
+  for (int q = b; q < e; ++q) rank += cell_list[q] < i ? 1 : 0;
+  cell_sorted[b + rank] = i;
+}
+
+// Full Verlet list: every j != i in the 27 cells around i with
+// r^2 < list_2(ti,tj) (binning.f90:64-113 with both directions stored).
+// Cells are visited x outer, y, z inner, each in ascending id, so the list
+// order is deterministic.  Also resets the skin reference
+// (binning.f90:125-127).
+__global__ void k_build_list(KParams k, const double* __restrict__ r, const int* __restrict__ type,
+                             const int* __restrict__ mic, const int* __restrict__ cell_of,
+                             const int* __restrict__ cell_start,
+                             const int* __restrict__ cell_sorted, int* __restrict__ nl,
+                             int* __restrict__ nn, int cap, double* __restrict__ r_ref,
+                             int* __restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  const double xi = r[3 * (size_t)i + 0], yi = r[3 * (size_t)i + 1], zi = r[3 * (size_t)i + 2];
+  const int ti = type[i] - 1;
+  const int c = cell_of[i];
+  const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
+  int cnt = 0;
+  int* row = nl + (size_t)i * cap;
+  for (int jx = cx - 1; jx <= cx + 1; ++jx) {
+    const int kx = (jx + k.nb[0]) % k.nb[0];
+    for (int jy = cy - 1; jy <= cy + 1; ++jy) {
+      const int ky = (jy + k.nb[1]) % k.nb[1];
+      for (int jz = cz - 1; jz <= cz + 1; ++jz) {
+        const int kz = (jz + k.nb[2]) % k.nb[2];
+        const int cc = (kz * k.nb[1] + ky) * k.nb[0] + kx;
+        const int b = cell_start[cc], e = cell_start[cc + 1];
+        for (int q = b; q < e; ++q) {
+          const int j = cell_sorted[q];
+          if (j == i) continue;
+          double dx = xi - r[3 * (size_t)j + 0];
+          double dy = yi - r[3 * (size_t)j + 1];
+          double dz = zi - r[3 * (size_t)j + 2];
+          min_image(k, dx, dy, dz);
+          const double r2 = (dx * dx + dy * dy) + dz * dz;
+          const int tj = type[j] - 1;
+          if (r2 < k.list_2[ti][tj]) {
+            if (cnt < cap) row[cnt] = j;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  nn[i] = cnt < cap ? cnt : cap;
+  if (cnt > cap) atomicMax(&flags[F_ERROR], MFA_ERR_LIST_OVERFLOW);
+  atomicMax(&flags[F_MAXNEIGH], cnt);
+  for (int d = 0; d < 3; ++d)
+    r_ref[3 * (size_t)i + d] = r[3 * (size_t)i + d] + (double)mic[3 * (size_t)i + d] * k.L[d];
+  if (i == 0) atomicAdd(&flags[F_NREBUILD], 1);
+}
+
+// ---------------------------------------------------------------------------
+// Force pass.  A lane group of TPP lanes handles one particle i: the lanes
+// split i's neighbour list, then combine their sums with a fixed butterfly,
+// so the result is independent of timing.  Everything acting on i is summed
+// here (pairs, walls, bonds), so no atomics are needed and F is written once.
+//
+// Plain FP64: the reference's arithmetic for LJ (fluid_fluid.f90:94-126), DPD
+// (dpd_forces.f90:273-348), 9-3 walls (fluid_wall.f90:81-121) and FENE
+// (intra_molec.f90:16-139), summed in FP64.
+// Mixed: separations and cut-off tests in FP64 (so the interacting set is the
+// FP64 one), the per-pair force in FP32, summed as int64 fixed point.
+
+struct Acc64 {
+  double x, y, z;
+  __device__ void zero() { x = y = z = 0.0; }
+};
+struct AccFix {
+  long long x, y, z;
+  __device__ void zero() { x = y = z = 0; }
+};
+
+__device__ __forceinline__ long long to_fix(float v) {
+  return (long long)__float2ll_rn(v * MFA_FIX_SCALE);
+}
+
+template <int TPP>
+__device__ __forceinline__ double group_sum(double v) {
+  for (int o = TPP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <int TPP>
+__device__ __forceinline__ long long group_sum(long long v) {
+  for (int o = TPP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Normal deviate of pair (i,j): FP64 and FP32 versions of the same transform.
+__device__ __forceinline__ double pair_normal_f64(uint32_t seed, int64_t step, int i, int j) {
+  uint32_t w0, w1;
+  mfa_pair_words(seed, step, i, j, &w0, &w1);
+  const double u1 = (double)(w0 >> 8) * 0x1p-24 + 0x1p-25;
+  const double u2 = (double)(w1 >> 8) * 0x1p-24;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+__device__ __forceinline__ float pair_normal_f32(uint32_t seed, int64_t step, int i, int j) {
+  uint32_t w0, w1;
+  mfa_pair_words(seed, step, i, j, &w0, &w1);
+  const float u1 = (float)(w0 >> 8) * 0x1p-24f + 0x1p-25f;
+  const float u2 = (float)(w1 >> 8) * 0x1p-24f;
+  // v_log_f32 is log2; v_cos_f32 takes revolutions (cos(2 pi x)).
+  const float l2 = __builtin_amdgcn_logf(u1);
+  return __builtin_sqrtf(-1.3862943611198906f * l2) * __builtin_amdgcn_cosf(u2);
+}
+
+template <int TPP, int MIXED>
+__global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __restrict__ r,
+                                                     const double* __restrict__ v,
+                                                     const int* __restrict__ type,
+                                                     const int* __restrict__ nl,
+                                                     const int* __restrict__ nn, int cap,
+                                                     double* __restrict__ f, int64_t step,
+                                                     double r2min, int mask,
+                                                     double* __restrict__ part,
+                                                     int* __restrict__ flags) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gtid / TPP;
+  const int sub = gtid % TPP;
+  if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  const bool active = i < k.n;
+  const int ii = active ? i : 0;
+  const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
+  const double vxi = v[3 * (size_t)ii + 0], vyi = v[3 * (size_t)ii + 1], vzi = v[3 * (size_t)ii + 2];
+  const int ti = type[ii] - 1;
+  double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0;
+  Acc64 a64;
+  AccFix afx;
+  a64.zero();
+  afx.zero();
+
+  if (active && (mask & TM_PAIR)) {
+    const int cnt = nn[ii];
+    const int* row = nl + (size_t)ii * cap;
+    for (int q = sub; q < cnt; q += TPP) {
+      const int j = row[q];
+      double dx = xi - r[3 * (size_t)j + 0];
+      double dy = yi - r[3 * (size_t)j + 1];
+      double dz = zi - r[3 * (size_t)j + 2];
+      min_image(k, dx, dy, dz);
+      const double r2raw = (dx * dx + dy * dy) + dz * dz;
+      const int tj = type[j] - 1;
+      const bool in_lj = r2raw < k.range_2[ti][tj];
+      // force switch-on clamp (fluid_fluid.f90:96); r2min = 0 leaves r2 as is.
+      // The clamped value also feeds the DPD terms, as in the reference.
+      const double r2 = in_lj ? fmax(r2raw, r2min) : r2raw;
+      bool dpd = false;
+      double w_arg = 0.0;  // r^2 * (1/rc^2) of the DPD weight
+      if (k.dpd_on) {
+        if (!k.dpd_cut_off_on) {
+          dpd = in_lj;
+          w_arg = r2 * k.inv_range_2[ti][tj];
+        } else {
+          const bool excl = k.dpd_head_excl && (ti == 0 || tj == 0);
+          dpd = !excl && r2 < k.r_cut_dpd_2;
+          w_arg = r2 / k.r_cut_dpd_2;
+        }
+      }
+      if (!in_lj && !dpd) continue;
+      if (!MIXED) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const double inv_r_2 = 1.0 / r2;
+        const double inv_sqrt_r_2 = sqrt(1.0 / r2);
+        if (in_lj) {
+          const double eps = k.epsil[ti][tj];
+          const double r61 = k.sigma_2[ti][tj] * inv_r_2;
+          const double r6 = (r61 * r61) * r61;
+          const double r12 = r6 * r6;
+          e_ff += 0.5 * (eps * ((r12 - r6) - k.e_shift[ti][tj]));
+          const double rd = (eps * (-12.0 * r12 + 6.0 * r6)) * inv_r_2;
+          fx = -(rd * dx);
+          fy = -(rd * dy);
+          fz = -(rd * dz);
+          vir += 0.5 * (-(rd * r2raw));
+        }
+        if (dpd) {
+          const double xi_n = pair_normal_f64(k.seed, step, i, j);
+          const double wr = 1.0 - sqrt(w_arg);
+          const double wd = wr * wr;
+          const double ex = dx * inv_sqrt_r_2, ey = dy * inv_sqrt_r_2, ez = dz * inv_sqrt_r_2;
+          const double sr = (k.sig * wr) * xi_n;
+          fx = fx + sr * ex;
+          fy = fy + sr * ey;
+          fz = fz + sr * ez;
+          const double dvx = vxi - v[3 * (size_t)j + 0];
+          const double dvy = vyi - v[3 * (size_t)j + 1];
+          const double dvz = vzi - v[3 * (size_t)j + 2];
+          const double rv = (dvx * ex + dvy * ey) + dvz * ez;
+          const double sd = ((-1.0 * k.friction1) * wd) * rv;
+          fx = fx + sd * ex;
+          fy = fy + sd * ey;
+          fz = fz + sd * ez;
+        }
+        a64.x += fx;
+        a64.y += fy;
+        a64.z += fz;
+      } else {
+#pragma clang fp contract(fast)
+        const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+        const float r2f = (float)r2;
+        const float inv_r_2 = 1.0f / r2f;
+        const float inv_r = __builtin_sqrtf(inv_r_2);
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (in_lj) {
+          const float eps = (float)k.epsil[ti][tj];
+          const float r61 = (float)k.sigma_2[ti][tj] * inv_r_2;
+          const float r6 = r61 * r61 * r61;
+          const float r12 = r6 * r6;
+          e_ff += 0.5 * (double)(eps * ((r12 - r6) - (float)k.e_shift[ti][tj]));
+          const float rd = eps * (6.0f * r6 - 12.0f * r12) * inv_r_2;
+          fx = -rd * dxf;
+          fy = -rd * dyf;
+          fz = -rd * dzf;
+          vir += 0.5 * (double)(-rd * (float)r2raw);
+        }
+        if (dpd) {
+          const float xi_n = pair_normal_f32(k.seed, step, i, j);
+          const float wr = 1.0f - __builtin_sqrtf((float)w_arg);
+          const float wd = wr * wr;
+          const float ex = dxf * inv_r, ey = dyf * inv_r, ez = dzf * inv_r;
+          const float sr = (float)k.sig * wr * xi_n;
+          const float dvx = (float)(vxi - v[3 * (size_t)j + 0]);
+          const float dvy = (float)(vyi - v[3 * (size_t)j + 1]);
+          const float dvz = (float)(vzi - v[3 * (size_t)j + 2]);
+          const float rv = dvx * ex + dvy * ey + dvz * ez;
+          const float s = sr - (float)k.friction1 * wd * rv;
+          fx += s * ex;
+          fy += s * ey;
+          fz += s * ez;
+        }
+        afx.x += to_fix(fx);
+        afx.y += to_fix(fy);
+        afx.z += to_fix(fz);
+      }
+    }
+  }
+
+  // Bonds and walls: lane 0 of the group (at most two bonds per particle).
+  if (active && sub == 0) {
+    if (mask & TM_BOND) {
+      // chain of i: brush [0,n_brush) with n_mon, melt [part_init_d,part_init_e)
+      // with n_mon_d, particle-4 chains [part_init_e,n) with n_mon_e.
+      int base = -1, len = 0;
+      if (i < k.n_brush) {
+        base = (i / k.n_mon) * k.n_mon;
+        len = k.n_mon;
+      } else if (i < k.part_init_e) {
+        base = k.part_init_d + ((i - k.part_init_d) / k.n_mon_d) * k.n_mon_d;
+        len = k.n_mon_d;
+      } else if (k.n_mon_e > 0) {
+        base = k.part_init_e + ((i - k.part_init_e) / k.n_mon_e) * k.n_mon_e;
+        len = k.n_mon_e;
+      }
+      for (int side = 0; side < 2; ++side) {
+        // bond (lo, lo+1): side 0 -> (i-1, i), side 1 -> (i, i+1)
+        const int lo = side == 0 ? i - 1 : i;
+        if (len < 2 || lo < base || lo + 1 >= base + len) continue;
+        const int hi = lo + 1;
+        double dx = r[3 * (size_t)lo + 0] - r[3 * (size_t)hi + 0];
+        double dy = r[3 * (size_t)lo + 1] - r[3 * (size_t)hi + 1];
+        double dz = r[3 * (size_t)lo + 2] - r[3 * (size_t)hi + 2];
+        min_image(k, dx, dy, dz);
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        if (!MIXED) {
+          double x = r2 * k.inv_r_chain_2;
+          const double clamp = (double)0.98f;
+          x = x < clamp ? x : clamp;
+          const double rd = (k.k_chain / (1.0 - x)) * k.inv_r_chain_2;
+          const double sgn = side == 0 ? 1.0 : -1.0;  // +force_loc on hi, -force_loc on lo
+          a64.x += sgn * (rd * dx);
+          a64.y += sgn * (rd * dy);
+          a64.z += sgn * (rd * dz);
+          if (side == 1) {
+            e_bond += -0.5 * log(1.0 - x);
+            vir += -(rd * r2);
+          }
+        } else {
+#pragma clang fp contract(fast)
+          float x = (float)(r2 * k.inv_r_chain_2);
+          x = x < 0.98f ? x : 0.98f;
+          const float rd = (float)k.k_chain / (1.0f - x) * (float)k.inv_r_chain_2;
+          const float sg = side == 0 ? 1.0f : -1.0f;
+          afx.x += to_fix(sg * rd * (float)dx);
+          afx.y += to_fix(sg * rd * (float)dy);
+          afx.z += to_fix(sg * rd * (float)dz);
+          if (side == 1) {
+            e_bond += -0.5 * log(1.0 - (double)x);
+            vir += (double)(-rd * (float)r2);
+          }
+        }
+      }
+    }
+    if ((mask & TM_WALL) && k.symmetry == 0) {
+      // fluid_wall case 2 (fluid_wall.f90:81-121); the energy uses a_w, the
+      // force a_wall(type) (:98,101,116,121).
+      const double sw = k.sigma_wall[ti], aw = k.a_wall[ti], aabs = fabs(aw);
+      const double sw3 = pow3(sw), sw9 = pow3(sw3);
+      double inv_z = 1.0 / zi;
+      double rd = sw * inv_z;
+      double rd3 = pow3(rd);
+      e_fw += aabs * pow3(rd3) - k.a_w * rd3;
+      double iz2 = inv_z * inv_z, iz4 = iz2 * iz2, iz8 = iz4 * iz4;
+      const double fb = (9.0 * aabs) * sw9 * (iz8 * iz2) - (3.0 * aw) * sw3 * iz4;
+      inv_z = 1.0 / (k.z_space_wall - zi);
+      rd = sw * inv_z;
+      rd3 = pow3(rd);
+      e_fw += aabs * pow3(rd3) - k.a_w * rd3;
+      iz2 = inv_z * inv_z;
+      iz4 = iz2 * iz2;
+      iz8 = iz4 * iz4;
+      const double ft = -(9.0 * aabs) * sw9 * (iz8 * iz2) + (3.0 * aw) * sw3 * iz4;
+      fzb = fb;
+      fzt = ft;
+      if (!MIXED) {
+        a64.z += fb;
+        a64.z += ft;
+      } else {
+        afx.z += to_fix((float)fb);
+        afx.z += to_fix((float)ft);
+      }
+    }
+  }
+
+  // Combine the lane group and write F_i.
+  double fx, fy, fz;
+  if (!MIXED) {
+    fx = group_sum<TPP>(a64.x);
+    fy = group_sum<TPP>(a64.y);
+    fz = group_sum<TPP>(a64.z);
+  } else {
+    fx = (double)group_sum<TPP>(afx.x) * MFA_FIX_INV;
+    fy = (double)group_sum<TPP>(afx.y) * MFA_FIX_INV;
+    fz = (double)group_sum<TPP>(afx.z) * MFA_FIX_INV;
+  }
+  if (active && sub == 0) {
+    if (mask & TM_ACCUM) {
+      fx = f[3 * (size_t)i + 0] + fx;
+      fy = f[3 * (size_t)i + 1] + fy;
+      fz = f[3 * (size_t)i + 2] + fz;
+    }
+    f[3 * (size_t)i + 0] = fx;
+    f[3 * (size_t)i + 1] = fy;
+    f[3 * (size_t)i + 2] = fz;
+    if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicMax(&flags[F_ERROR], MFA_ERR_NAN);
+  }
+  if (mask & TM_ENERGY) {
+    double vals[MFA_NTERMS];
+    for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+    if (active) {
+      vals[T_VFF] = e_ff;
+      vals[T_VIR] = vir;
+      vals[T_VFW] = e_fw;
+      vals[T_VINTRA] = e_bond * k.k_chain;
+      vals[T_FZB] = fzb;
+      vals[T_FZT] = fzt;
+      if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
+    }
+    block_partials(vals, part);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+
+static KParams make_kparams(const mfa_params& p) {
+  KParams k;
+  memset(&k, 0, sizeof k);
+  for (int d = 0; d < 3; ++d) {
+    k.L[d] = p.boundary[d];
+    k.invL[d] = p.inv_boundary[d];
+    k.inv_r_bin[d] = p.inv_r_bin[d];
+    k.nb[d] = p.n_bin[d];
+    k.va[d] = p.va_spring_twall[d];
+  }
+  k.dt = p.dt;
+  k.dt_2 = p.dt_2;
+  k.half_dt = 0.5 * p.dt;      // 0.5*dt   (verlet_positions.f90:324)
+  k.half_dt_2 = 0.5 * p.dt_2;  // 0.5*dt_2 (verlet_positions.f90:309)
+  for (int t = 0; t < MFA_NTYPE; ++t) {
+    k.inv_mass_type[t] = p.inv_mass_type[t];
+    k.mass_type[t] = p.mass_type[t];
+    k.sigma_wall[t] = p.sigma_wall[t];
+    k.a_wall[t] = p.a_wall[t];
+    for (int u = 0; u < MFA_NTYPE; ++u) {
+      k.epsil[t][u] = p.epsil[t][u];
+      k.sigma_2[t][u] = p.sigma_2[t][u];
+      k.range_2[t][u] = p.range_2[t][u];
+      k.inv_range_2[t][u] = p.inv_range_2[t][u];
+      k.e_shift[t][u] = p.e_shift[t][u];
+      k.list_2[t][u] = p.list_2[t][u];
+    }
+  }
+  k.r_cut_dpd_2 = p.r_cut_dpd_2;
+  k.friction1 = p.friction1;
+  k.sig = p.sig;
+  k.k_chain = p.k_chain;
+  k.inv_r_chain_2 = p.inv_r_chain_2;
+  k.a_w = p.a_w;
+  k.z_space_wall = p.z_space_wall;
+  k.skin_half_2 = p.skin_half_2;
+  k.n = p.n_part;
+  k.n_mon = p.n_mon > 0 ? p.n_mon : 1;
+  k.n_chain = p.n_chain;
+  k.n_brush = p.n_mon * p.n_chain;
+  k.part_init_d = p.part_init_d;
+  k.n_mon_d = p.n_mon_d > 0 ? p.n_mon_d : 1;
+  k.part_init_e = p.part_init_e;
+  k.n_mon_e = p.n_mon_e;
+  k.symmetry = p.symmetry;
+  // Heads pinned for brush systems in a channel (verlet_velocities.f90:113-134).
+  k.pinned_heads = (p.system == 3 && p.symmetry == 0) ? 1 : 0;
+  k.dpd_on = p.thermostat == 0 ? 1 : 0;
+  k.dpd_cut_off_on = p.dpd_cut_off_on;
+  // DPD_CUT_OFF excludes heads for brush systems in a channel (fluid_fluid.f90:179-185).
+  k.dpd_head_excl = (p.system == 3 && p.symmetry == 0) ? 1 : 0;
+  k.seed = (uint32_t)p.seed;
+  return k;
+}
+
+static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
+
+static int check_flags(mfa_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags, F_NFLAGS * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int e = c->h_flags[F_ERROR];
+  if (e == MFA_ERR_LIST_OVERFLOW)
+    return set_err(c, e, "neighbour list overflow: %d neighbours > capacity %d",
+                   c->h_flags[F_MAXNEIGH], c->cap);
+  if (e == MFA_ERR_CELL) return set_err(c, e, "particle outside the cell grid");
+  if (e == MFA_ERR_NAN) return set_err(c, e, "non-finite force");
+  if (e != 0) return set_err(c, e, "device error %d", e);
+  return MFA_OK;
+}
+
+static int launch_rebuild(mfa_ctx* c) {
+  const int g = grid_of(c->n);
+  hipLaunchKernelGGL(k_cell_assign, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->cell_of,
+                     c->cell_count, c->flags);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->n_cells,
+                     c->cell_count, c->cell_start, c->cell_fill, c->flags);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->n, c->cell_of,
+                     c->cell_start, c->cell_fill, c->cell_list, c->flags);
+  hipLaunchKernelGGL(k_cell_sort, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->n, c->cell_of,
+                     c->cell_start, c->cell_list, c->cell_sorted, c->flags);
+  hipLaunchKernelGGL(k_build_list, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->type,
+                     c->mic, c->cell_of, c->cell_start, c->cell_sorted, c->nl, c->nn, c->cap,
+                     c->r_ref, c->flags);
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
+}
+
+// r_2_min of the force switch-on ramp at `step` (fluid_fluid.f90:44-48),
+// with r_time = step*dt (md_main.f90:69, step = s_time + i_time - 1).
+static double switch_on_r2min(const mfa_params& p, int64_t step) {
+  if (p.fso_time <= 0.0) return 0.0;
+  const double r_time = (double)step * p.dt;
+  return r_time < p.fso_time ? (1.0 - r_time / p.fso_time) * p.fso_r2_init : 0.0;
+}
+
+template <int TPP>
+static void launch_force_t(mfa_ctx* c, int64_t step, int mask) {
+  const int threads = c->n * TPP;
+  const int g = grid_of(threads);
+  const double r2min = switch_on_r2min(c->p, step);
+  if (c->precision == MFA_MIXED)
+    hipLaunchKernelGGL((k_force<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
+                       c->type, c->nl, c->nn, c->cap, c->f, step, r2min, mask, c->part, c->flags);
+  else
+    hipLaunchKernelGGL((k_force<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
+                       c->type, c->nl, c->nn, c->cap, c->f, step, r2min, mask, c->part, c->flags);
+}
+
+static int force_blocks(mfa_ctx* c) { return grid_of(c->n * c->tpp); }
+
+static int timed_begin(mfa_ctx* c, std::vector<hipEvent_t>& pool, size_t& used) {
+  while (pool.size() < used + 2) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    pool.push_back(e);
+  }
+  HIPCHK(c, hipEventRecord(pool[used], c->stream));
+  return MFA_OK;
+}
+static int timed_end(mfa_ctx* c, std::vector<hipEvent_t>& pool, size_t& used) {
+  HIPCHK(c, hipEventRecord(pool[used + 1], c->stream));
+  used += 2;
+  return MFA_OK;
+}
+// Adds the elapsed times of the recorded pairs to (ms, count).
+static int timed_flush(mfa_ctx* c, std::vector<hipEvent_t>& pool, size_t& used, double& ms,
+                       int64_t& count) {
+  if (used == 0) return MFA_OK;
+  HIPCHK(c, hipEventSynchronize(pool[used - 1]));
+  for (size_t k = 0; k < used; k += 2) {
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, pool[k], pool[k + 1]));
+    ms += t;
+    count += 1;
+  }
+  used = 0;
+  return MFA_OK;
+}
+
+static int launch_force(mfa_ctx* c, int64_t step, int mask) {
+  if (c->timing) {
+    if (c->ev_force_used + 2 > 4096) {
+      int rc = timed_flush(c, c->ev_force, c->ev_force_used, c->force_ms, c->force_n);
+      if (rc) return rc;
+    }
+    int rc = timed_begin(c, c->ev_force, c->ev_force_used);
+    if (rc) return rc;
+  }
+  switch (c->tpp) {
+    case 1: launch_force_t<1>(c, step, mask); break;
+    case 2: launch_force_t<2>(c, step, mask); break;
+    case 4: launch_force_t<4>(c, step, mask); break;
+    case 8: launch_force_t<8>(c, step, mask); break;
+    case 16: launch_force_t<16>(c, step, mask); break;
+    default: return set_err(c, MFA_ERR_ARG, "threads per particle %d unsupported", c->tpp);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (c->timing) return timed_end(c, c->ev_force, c->ev_force_used);
+  return MFA_OK;
+}
+
+static int reduce_terms(mfa_ctx* c, int nblocks, double* out) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->part, nblocks,
+                     c->red);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->h_red, c->red, MFA_NTERMS * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < MFA_NTERMS; ++t) out[t] = c->h_red[t];
+  return MFA_OK;
+}
+
+static int set_rebuild_flag(mfa_ctx* c) {
+  const int one = 1;
+  HIPCHK(c, hipMemcpyAsync(c->flags + F_REBUILD, &one, sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+static int ensure_list(mfa_ctx* c) {
+  if (!c->need_rebuild) return MFA_OK;
+  int rc = set_rebuild_flag(c);
+  if (rc) return rc;
+  rc = launch_rebuild(c);
+  if (rc) return rc;
+  // clear the flag (normally done by the force pass)
+  HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
+  c->need_rebuild = false;
+  return check_flags(c);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" const char* mfa_last_error(mfa_ctx* c) {
+  return c ? c->err.c_str() : g_noctx_err.c_str();
+}
+
+extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t list_capacity,
+                              mfa_ctx** out) {
+  if (!p || !out) return set_err(nullptr, MFA_ERR_ARG, "null argument");
+  if (precision != MFA_FP64 && precision != MFA_MIXED)
+    return set_err(nullptr, MFA_ERR_ARG, "unknown precision %d", precision);
+  if (p->n_part <= 0) return set_err(nullptr, MFA_ERR_ARG, "n_part must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(nullptr, MFA_ERR_HIP, "no HIP device");
+  mfa_ctx* c = new mfa_ctx();
+  c->p = *p;
+  c->k = make_kparams(*p);
+  c->precision = precision;
+  c->n = p->n_part;
+  c->n_cells = p->n_bin[0] * p->n_bin[1] * p->n_bin[2];
+  c->tpp = 8;
+  c->need_rebuild = true;
+  c->timing = false;
+  if (list_capacity <= 0) {
+    // expected full-list count at the mean density, with room for fluctuations
+    double vol = p->boundary[0] * p->boundary[1] * p->boundary[2];
+    double rl2 = 0.0;
+    for (int t = 0; t < MFA_NTYPE; ++t)
+      for (int u = 0; u < MFA_NTYPE; ++u) rl2 = p->list_2[t][u] > rl2 ? p->list_2[t][u] : rl2;
+    const double rl = sqrt(rl2);
+    const double expect = (double)p->n_part / vol * 4.18879020478639 * rl * rl * rl;
+    int cap = (int)(2.0 * expect + 48.0);
+    cap = (cap + 31) / 32 * 32;
+    list_capacity = cap;
+  }
+  c->cap = list_capacity;
+  const size_t n = (size_t)c->n;
+  c->max_blocks = grid_of(c->n * 16) + 1;
+#define ALLOC(ptr, bytes)                                                        \
+  do {                                                                           \
+    if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                      \
+      mfa_destroy(c);                                                            \
+      return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc of %zu bytes failed",    \
+                     (size_t)(bytes));                                           \
+    }                                                                            \
+  } while (0)
+  ALLOC(c->r, n * 3 * sizeof(double));
+  ALLOC(c->v, n * 3 * sizeof(double));
+  ALLOC(c->a, n * 3 * sizeof(double));
+  ALLOC(c->f, n * 3 * sizeof(double));
+  ALLOC(c->r_ref, n * 3 * sizeof(double));
+  ALLOC(c->type, n * sizeof(int32_t));
+  ALLOC(c->mic, n * 3 * sizeof(int32_t));
+  ALLOC(c->cell_of, n * sizeof(int32_t));
+  ALLOC(c->cell_list, n * sizeof(int32_t));
+  ALLOC(c->cell_sorted, n * sizeof(int32_t));
+  ALLOC(c->cell_count, (size_t)c->n_cells * sizeof(int32_t));
+  ALLOC(c->cell_fill, (size_t)c->n_cells * sizeof(int32_t));
+  ALLOC(c->cell_start, ((size_t)c->n_cells + 1) * sizeof(int32_t));
+  ALLOC(c->nl, n * (size_t)c->cap * sizeof(int32_t));
+  ALLOC(c->nn, n * sizeof(int32_t));
+  ALLOC(c->flags, F_NFLAGS * sizeof(int32_t));
+  ALLOC(c->part, (size_t)c->max_blocks * MFA_NTERMS * sizeof(double));
+  ALLOC(c->red, MFA_NTERMS * sizeof(double));
+#undef ALLOC
+  if (hipHostMalloc((void**)&c->h_flags, F_NFLAGS * sizeof(int32_t), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&c->h_red, MFA_NTERMS * sizeof(double), hipHostMallocDefault) !=
+          hipSuccess) {
+    mfa_destroy(c);
+    return set_err(nullptr, MFA_ERR_NOMEM, "hipHostMalloc failed");
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    mfa_destroy(c);
+    return set_err(nullptr, MFA_ERR_HIP, "hipStreamCreate failed");
+  }
+  c->ev_force_used = c->ev_integ_used = 0;
+  hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->cell_fill, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->f, 0, n * 3 * sizeof(double), c->stream);
+  hipMemsetAsync(c->a, 0, n * 3 * sizeof(double), c->stream);
+  hipMemsetAsync(c->nn, 0, n * sizeof(int32_t), c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    mfa_destroy(c);
+    return set_err(nullptr, MFA_ERR_HIP, "initial clears failed");
+  }
+  *out = c;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_destroy(mfa_ctx* c) {
+  if (!c) return MFA_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->r, c->v, c->a, c->f, c->r_ref, c->type, c->mic, c->cell_of, c->cell_list,
+                  c->cell_sorted, c->cell_count, c->cell_fill, c->cell_start, c->nl, c->nn,
+                  c->flags, c->part, c->red};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  if (c->h_flags) hipHostFree(c->h_flags);
+  if (c->h_red) hipHostFree(c->h_red);
+  for (hipEvent_t e : c->ev_force) hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_integ) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_set_precision(mfa_ctx* c, int32_t precision) {
+  if (!c) return MFA_ERR_ARG;
+  if (precision != MFA_FP64 && precision != MFA_MIXED)
+    return set_err(c, MFA_ERR_ARG, "unknown precision %d", precision);
+  c->precision = precision;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_set_threads_per_particle(mfa_ctx* c, int32_t tpp) {
+  if (!c) return MFA_ERR_ARG;
+  if (tpp != 1 && tpp != 2 && tpp != 4 && tpp != 8 && tpp != 16)
+    return set_err(c, MFA_ERR_ARG, "threads per particle must be 1,2,4,8 or 16");
+  c->tpp = tpp;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_set_state(mfa_ctx* c, const double* r0, const double* v, const double* a,
+                                 const int32_t* a_type, const int32_t* mic) {
+  if (!c || !r0 || !v || !a || !a_type) return MFA_ERR_ARG;
+  const size_t n = (size_t)c->n;
+  for (size_t i = 0; i < n; ++i)
+    if (a_type[i] < 1 || a_type[i] > MFA_NTYPE)
+      return set_err(c, MFA_ERR_ARG, "a_type(%zu) = %d outside 1..%d", i + 1, a_type[i], MFA_NTYPE);
+  HIPCHK(c, hipMemcpyAsync(c->r, r0, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->v, v, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->a, a, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->type, a_type, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  if (mic)
+    HIPCHK(c, hipMemcpyAsync(c->mic, mic, n * 3 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  else
+    HIPCHK(c, hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->need_rebuild = true;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_get_state(mfa_ctx* c, double* r0, double* v, double* a, double* force,
+                                 int32_t* mic) {
+  if (!c) return MFA_ERR_ARG;
+  const size_t n = (size_t)c->n;
+  if (r0) HIPCHK(c, hipMemcpyAsync(r0, c->r, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (v) HIPCHK(c, hipMemcpyAsync(v, c->v, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (a) HIPCHK(c, hipMemcpyAsync(a, c->a, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (force)
+    HIPCHK(c, hipMemcpyAsync(force, c->f, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (mic)
+    HIPCHK(c, hipMemcpyAsync(mic, c->mic, n * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_verlet_positions(mfa_ctx* c) {
+  if (!c) return MFA_ERR_ARG;
+  hipLaunchKernelGGL(k_verlet_positions, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
+                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_check_skin(mfa_ctx* c, int32_t* f_skin) {
+  if (!c || !f_skin) return MFA_ERR_ARG;
+  if (c->need_rebuild) {
+    *f_skin = 1;
+    return MFA_OK;
+  }
+  HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_check_skin, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r,
+                     c->r_ref, c->mic, c->flags);
+  HIPCHK(c, hipGetLastError());
+  int rc = check_flags(c);
+  if (rc) return rc;
+  *f_skin = c->h_flags[F_REBUILD] ? 1 : 0;
+  HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_binning(mfa_ctx* c) {
+  if (!c) return MFA_ERR_ARG;
+  c->need_rebuild = true;
+  return ensure_list(c);
+}
+
+extern "C" int32_t mfa_get_cells(mfa_ctx* c, int32_t* cell_xyz) {
+  if (!c || !cell_xyz) return MFA_ERR_ARG;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  std::vector<int32_t> co((size_t)c->n);
+  HIPCHK(c, hipMemcpy(co.data(), c->cell_of, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int nx = c->p.n_bin[0], ny = c->p.n_bin[1];
+  for (int i = 0; i < c->n; ++i) {
+    const int cc = co[(size_t)i];
+    cell_xyz[3 * (size_t)i + 0] = cc % nx;
+    cell_xyz[3 * (size_t)i + 1] = (cc / nx) % ny;
+    cell_xyz[3 * (size_t)i + 2] = cc / (nx * ny);
+  }
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_get_pairs(mfa_ctx* c, int32_t* pairs, int64_t cap, int64_t* n_pairs) {
+  if (!c || !n_pairs) return MFA_ERR_ARG;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  std::vector<int32_t> nn((size_t)c->n), nl((size_t)c->n * c->cap);
+  HIPCHK(c, hipMemcpy(nn.data(), c->nn, nn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(nl.data(), c->nl, nl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  int64_t m = 0;
+  for (int i = 0; i < c->n; ++i)
+    for (int q = 0; q < nn[(size_t)i]; ++q) {
+      const int j = nl[(size_t)i * c->cap + q];
+      if (i < j) {
+        if (pairs && m < cap) {
+          pairs[2 * m + 0] = i + 1;
+          pairs[2 * m + 1] = j + 1;
+        }
+        ++m;
+      }
+    }
+  *n_pairs = m;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_zero_force(mfa_ctx* c) {
+  if (!c) return MFA_ERR_ARG;
+  HIPCHK(c, hipMemsetAsync(c->f, 0, (size_t)c->n * 3 * sizeof(double), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+// One force-pass call with the given terms, adding into F, energies optional.
+static int force_terms(mfa_ctx* c, int64_t step, int terms, double* out_terms) {
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  int mask = terms | TM_ACCUM | (out_terms ? TM_ENERGY : 0);
+  rc = launch_force(c, step, mask);
+  if (rc) return rc;
+  if (out_terms) {
+    rc = reduce_terms(c, force_blocks(c), out_terms);
+    if (rc) return rc;
+  }
+  return check_flags(c);
+}
+
+extern "C" int32_t mfa_fluid_fluid(mfa_ctx* c, int64_t step, double* v_ff, double* v_coul) {
+  if (!c) return MFA_ERR_ARG;
+  double t[MFA_NTERMS];
+  int rc = force_terms(c, step, TM_PAIR, (v_ff || v_coul) ? t : nullptr);
+  if (rc) return rc;
+  if (v_ff) *v_ff = t[T_VFF];
+  if (v_coul) *v_coul = t[T_VCOUL];
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_fluid_wall(mfa_ctx* c, int32_t inter_type, double* v_fw) {
+  if (!c) return MFA_ERR_ARG;
+  if (inter_type != 2)
+    return set_err(c, MFA_ERR_ARG, "fluid_wall: only inter_type 2 (9-3 wall) is provided");
+  if (c->p.symmetry != 0) {
+    if (v_fw) *v_fw = 0.0;
+    return MFA_OK;
+  }
+  double t[MFA_NTERMS];
+  int rc = force_terms(c, 0, TM_WALL, v_fw ? t : nullptr);
+  if (rc) return rc;
+  if (v_fw) *v_fw = t[T_VFW];
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_intra_molec(mfa_ctx* c, double* v_intra) {
+  if (!c) return MFA_ERR_ARG;
+  double t[MFA_NTERMS];
+  int rc = force_terms(c, 0, TM_BOND, v_intra ? t : nullptr);
+  if (rc) return rc;
+  if (v_intra) *v_intra = t[T_VINTRA];
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
+  if (!c) return MFA_ERR_ARG;
+  const int g = grid_of(c->n);
+  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,
+                     c->a, c->type, c->part, t_kin ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  if (t_kin) {
+    double t[MFA_NTERMS];
+    int rc = reduce_terms(c, g, t);
+    if (rc) return rc;
+    *t_kin = 0.5 * t[T_TKIN];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+static void fill_obs(const double* t, double tkin_sum, int64_t nreb, mfa_obs* obs) {
+  memset(obs, 0, sizeof *obs);
+  obs->v_fluid_fluid = t[T_VFF];
+  obs->v_fluid_wall = t[T_VFW];
+  obs->v_intra_molec = t[T_VINTRA];
+  obs->v_bend = t[T_VBEND];
+  obs->v_coul = t[T_VCOUL];
+  obs->t_kin = 0.5 * tkin_sum;
+  obs->virial = t[T_VIR];
+  obs->wall_fz_bottom = t[T_FZB];
+  obs->wall_fz_top = t[T_FZT];
+  obs->z_moment = t[T_ZMOM];
+  obs->n_rebuilds = nreb;
+}
+
+extern "C" int32_t mfa_energies(mfa_ctx* c, int64_t step, mfa_obs* obs) {
+  if (!c || !obs) return MFA_ERR_ARG;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  // recompute forces into F (the pass overwrites F), then reduce
+  rc = launch_force(c, step, TM_PAIR | TM_WALL | TM_BOND | TM_ENERGY);
+  if (rc) return rc;
+  double t[MFA_NTERMS];
+  rc = reduce_terms(c, force_blocks(c), t);
+  if (rc) return rc;
+  // kinetic energy of the current velocities
+  std::vector<double> vv((size_t)c->n * 3);
+  std::vector<int32_t> ty((size_t)c->n);
+  HIPCHK(c, hipMemcpy(vv.data(), c->v, vv.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ty.data(), c->type, ty.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  double tk = 0.0;
+  for (int i = 0; i < c->n; ++i) {
+    const double* q = &vv[3 * (size_t)i];
+    tk += c->p.mass_type[ty[(size_t)i] - 1] * ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]);
+  }
+  fill_obs(t, tk, 0, obs);
+  return check_flags(c);
+}
+
+extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_obs* obs) {
+  if (!c || n < 0) return MFA_ERR_ARG;
+  if (n == 0) return MFA_OK;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->flags + F_NREBUILD, 0, sizeof(int), c->stream));
+  const int g = grid_of(c->n);
+  const int terms = TM_PAIR | TM_WALL | TM_BOND;
+  // first half of the first step (verlet_positions + check_skin)
+  hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
+                     c->a, c->mic, c->r_ref, c->flags, 1);
+  HIPCHK(c, hipGetLastError());
+  for (int s = 0; s < n; ++s) {
+    const int64_t step = first_step + s;
+    const bool last = s == n - 1;
+    rc = launch_rebuild(c);
+    if (rc) return rc;
+    rc = launch_force(c, step, terms | ((last && obs) ? TM_ENERGY : 0));
+    if (rc) return rc;
+    if (last) {
+      double t[MFA_NTERMS];
+      if (obs) {
+        rc = reduce_terms(c, force_blocks(c), t);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f,
+                         c->v, c->a, c->type, c->part, obs ? 1 : 0);
+      HIPCHK(c, hipGetLastError());
+      if (obs) {
+        double tk[MFA_NTERMS];
+        rc = reduce_terms(c, g, tk);
+        if (rc) return rc;
+        rc = check_flags(c);
+        if (rc) return rc;
+        fill_obs(t, tk[T_TKIN], c->h_flags[F_NREBUILD], obs);
+      }
+    } else {
+      if (c->timing) {
+        if (c->ev_integ_used + 2 > 4096) {
+          rc = timed_flush(c, c->ev_integ, c->ev_integ_used, c->integ_ms, c->integ_n);
+          if (rc) return rc;
+        }
+        rc = timed_begin(c, c->ev_integ, c->ev_integ_used);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(k_fused_integrate, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f,
+                         c->v, c->r, c->mic, c->r_ref, c->type, c->flags, c->part, 0);
+      HIPCHK(c, hipGetLastError());
+      if (c->timing) {
+        rc = timed_end(c, c->ev_integ, c->ev_integ_used);
+        if (rc) return rc;
+      }
+    }
+  }
+  return check_flags(c);
+}
+
+extern "C" int32_t mfa_timing(mfa_ctx* c, int32_t enable, double* force_ms_avg,
+                              int64_t* force_launches, double* integrate_ms_avg,
+                              int64_t* integrate_launches) {
+  if (!c) return MFA_ERR_ARG;
+  int rc = timed_flush(c, c->ev_force, c->ev_force_used, c->force_ms, c->force_n);
+  if (rc) return rc;
+  rc = timed_flush(c, c->ev_integ, c->ev_integ_used, c->integ_ms, c->integ_n);
+  if (rc) return rc;
+  if (force_ms_avg) *force_ms_avg = c->force_n ? c->force_ms / (double)c->force_n : 0.0;
+  if (force_launches) *force_launches = c->force_n;
+  if (integrate_ms_avg) *integrate_ms_avg = c->integ_n ? c->integ_ms / (double)c->integ_n : 0.0;
+  if (integrate_launches) *integrate_launches = c->integ_n;
+  c->timing = enable != 0;
+  c->force_ms = 0.0;
+  c->force_n = 0;
+  c->integ_ms = 0.0;
+  c->integ_n = 0;
+  return MFA_OK;
+}

@@ -1,0 +1,313 @@
+"""ctypes binding of libmfa_hip (include/mfa_hip.h).
+
+This is the Python side of the C ABI that the Fortran driver also binds
+(mfa_amd/fortran/mfa_hip_bind.f90).  The structures below mirror the C
+structs field by field; `abi_check()` compares their sizes with the library's.
+Loading fails loudly when the library is missing: there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+NTYPE = 4
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfa_hip.so")
+
+MFA_FP64 = 0
+MFA_MIXED = 1
+
+ERRORS = {
+    0: "OK", 1: "MFA_ERR_ARG", 2: "MFA_ERR_HIP", 3: "MFA_ERR_LIST_OVERFLOW",
+    4: "MFA_ERR_CELL", 5: "MFA_ERR_NOMEM", 6: "MFA_ERR_NAN", 7: "MFA_ERR_IO",
+}
+
+_d4 = C.c_double * NTYPE
+_d44 = (C.c_double * NTYPE) * NTYPE
+_d3 = C.c_double * 3
+_i3 = C.c_int32 * 3
+
+
+class Inputs(C.Structure):
+    _fields_ = [
+        ("n_mon", C.c_int32), ("n_chain", C.c_int32), ("n_cell_w_x", C.c_int32),
+        ("n_cell_w_y", C.c_int32), ("n_mon_d", C.c_int32), ("n_chain_d", C.c_int32),
+        ("n_mon_e", C.c_int32), ("n_chain_e", C.c_int32), ("inter4", C.c_int32),
+        ("pad0", C.c_int32),
+        ("a_w", C.c_double), ("sigma_w", C.c_double), ("a_w4", C.c_double),
+        ("sigma_w4", C.c_double), ("charge", _d4), ("alpha_c", C.c_double),
+        ("n_kmax", C.c_int32), ("pad1", C.c_int32),
+        ("s_time", C.c_int32), ("n_relax", C.c_int32), ("n_obser", C.c_int32),
+        ("n_safe", C.c_int32), ("dt", C.c_double), ("temp", C.c_double),
+        ("x_space", C.c_double), ("y_space", C.c_double), ("z_space_wall", C.c_double),
+        ("mass_type", _d4), ("epsil", _d44), ("sigma_ii", _d4), ("friction", _d4),
+        ("k_bend", C.c_double), ("alpha_eq", C.c_double), ("iseed", C.c_int64),
+        ("k_chain", C.c_double), ("r_chain", C.c_double), ("va_spring_twall", _d3),
+        ("system", C.c_int32), ("symmetry", C.c_int32), ("solvent", C.c_int32),
+        ("thermostat", C.c_int32), ("particle_4", C.c_int32), ("bending", C.c_int32),
+        ("dpd_cut_off", C.c_double), ("skin", C.c_double), ("lj_rc", C.c_double),
+        ("rc_coul", C.c_double), ("ewald_compat", C.c_int32), ("force_switch_on", C.c_int32),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("n_part", C.c_int32), ("n_mon_tot", C.c_int32),
+        ("n_mon", C.c_int32), ("n_chain", C.c_int32), ("n_mon_d", C.c_int32),
+        ("n_chain_d", C.c_int32), ("n_mon_e", C.c_int32), ("n_chain_e", C.c_int32),
+        ("part_init_d", C.c_int32), ("part_init_e", C.c_int32),
+        ("system", C.c_int32), ("symmetry", C.c_int32), ("thermostat", C.c_int32),
+        ("dpd_weight", C.c_int32), ("dpd_cut_off_on", C.c_int32), ("coulomb", C.c_int32),
+        ("bending", C.c_int32), ("ewald_compat", C.c_int32),
+        ("boundary", _d3), ("inv_boundary", _d3),
+        ("dt", C.c_double), ("dt_2", C.c_double), ("temp", C.c_double),
+        ("mass_type", _d4), ("inv_mass_type", _d4),
+        ("epsil", _d44), ("sigma", _d44), ("sigma_2", _d44), ("range_2", _d44),
+        ("inv_range_2", _d44), ("e_shift", _d44), ("list_2", _d44),
+        ("r_cut_max", C.c_double), ("r_cut_max_2", C.c_double),
+        ("r_cut_dpd", C.c_double), ("r_cut_dpd_2", C.c_double),
+        ("skin", C.c_double), ("skin_2", C.c_double), ("skin_half_2", C.c_double),
+        ("friction1", C.c_double), ("sig", C.c_double),
+        ("k_chain", C.c_double), ("r_chain_2", C.c_double), ("inv_r_chain_2", C.c_double),
+        ("a_w", C.c_double), ("sigma_wall", _d4), ("a_wall", _d4), ("z_space_wall", C.c_double),
+        ("va_spring_twall", _d3), ("k_bend", C.c_double), ("alpha_eq", C.c_double),
+        ("n_bin", _i3), ("delta_bin", _i3), ("r_bin", _d3), ("inv_r_bin", _d3),
+        ("charge", _d4), ("alpha_c", C.c_double), ("sqrt_alpha", C.c_double),
+        ("inv_sqrt_pi_real", C.c_double), ("rc_coul_2", C.c_double),
+        ("v_coul_self", C.c_double), ("n_kmax", C.c_int32), ("pad0", C.c_int32),
+        ("seed", C.c_uint64), ("fso_r2_init", C.c_double), ("fso_time", C.c_double),
+    ]
+
+
+class Obs(C.Structure):
+    _fields_ = [
+        ("v_fluid_fluid", C.c_double), ("v_fluid_wall", C.c_double),
+        ("v_intra_molec", C.c_double), ("v_bend", C.c_double), ("v_coul", C.c_double),
+        ("t_kin", C.c_double), ("virial", C.c_double), ("wall_fz_bottom", C.c_double),
+        ("wall_fz_top", C.c_double), ("z_moment", C.c_double), ("n_rebuilds", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Every function include/mfa_hip.h declares, with its ctypes signature.
+_P = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+SIGNATURES = {
+    "mfa_abi_sizes": (C.c_int32, [_ip, _ip, _ip]),
+    "mfa_derive": (C.c_int32, [C.POINTER(Inputs), C.POINTER(Params)]),
+    "mfa_create": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "mfa_destroy": (C.c_int32, [_P]),
+    "mfa_set_precision": (C.c_int32, [_P, C.c_int32]),
+    "mfa_set_threads_per_particle": (C.c_int32, [_P, C.c_int32]),
+    "mfa_last_error": (C.c_char_p, [_P]),
+    "mfa_set_state": (C.c_int32, [_P, _dp, _dp, _dp, _ip, _ip]),
+    "mfa_get_state": (C.c_int32, [_P, _dp, _dp, _dp, _dp, _ip]),
+    "mfa_verlet_positions": (C.c_int32, [_P]),
+    "mfa_check_skin": (C.c_int32, [_P, _ip]),
+    "mfa_binning": (C.c_int32, [_P]),
+    "mfa_get_cells": (C.c_int32, [_P, _ip]),
+    "mfa_get_pairs": (C.c_int32, [_P, _ip, C.c_int64, _lp]),
+    "mfa_zero_force": (C.c_int32, [_P]),
+    "mfa_fluid_fluid": (C.c_int32, [_P, C.c_int64, _dp, _dp]),
+    "mfa_fluid_wall": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_intra_molec": (C.c_int32, [_P, _dp]),
+    "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
+    "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
+    "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
+    "mfa_timing": (C.c_int32, [_P, C.c_int32, _dp, _lp, _dp, _lp]),
+    "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
+                                 C.c_double, C.c_double, _dp, _dp, _ip]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libmfa_hip; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libmfa_hip.so not found at %s: build it with `python -m mfa_amd.build` "
+            "(there is no CPU fallback)" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def abi_check():
+    lib = load()
+    a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+    lib.mfa_abi_sizes(C.byref(a), C.byref(b), C.byref(c))
+    return (a.value, b.value, c.value) == (C.sizeof(Inputs), C.sizeof(Params), C.sizeof(Obs))
+
+
+class MfaError(RuntimeError):
+    pass
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+def derive(inputs):
+    lib = load()
+    p = Params()
+    rc = lib.mfa_derive(C.byref(inputs), C.byref(p))
+    if rc:
+        raise MfaError("mfa_derive: %s (%s)" % (lib.mfa_last_error(None).decode(), ERRORS[rc]))
+    return p
+
+
+def generate(params, kind, seed, head_z=1.0, bond=0.97, min_dist=0.8):
+    lib = load()
+    n = params.n_part
+    r = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    t = np.zeros(n, dtype=np.int32)
+    rc = lib.mfa_generate(C.byref(params), kind, seed, head_z, bond, min_dist,
+                          _ptr(r, C.c_double), _ptr(v, C.c_double), _ptr(t, C.c_int32))
+    if rc:
+        raise MfaError("mfa_generate failed (%s)" % ERRORS[rc])
+    return r, v, t
+
+
+class Context:
+    """A device context: the state of one system in HBM."""
+
+    def __init__(self, params, precision=MFA_FP64, list_capacity=0):
+        self.lib = load()
+        self.params = params
+        self.n = params.n_part
+        h = C.c_void_p()
+        rc = self.lib.mfa_create(C.byref(params), precision, list_capacity, C.byref(h))
+        if rc:
+            raise MfaError("mfa_create: %s (%s)" % (
+                self.lib.mfa_last_error(None).decode(), ERRORS.get(rc, rc)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mfa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise MfaError("%s: %s (%s)" % (what, self.lib.mfa_last_error(self.h).decode(),
+                                            ERRORS.get(rc, rc)))
+
+    def set_precision(self, precision):
+        self._chk(self.lib.mfa_set_precision(self.h, precision), "mfa_set_precision")
+
+    def set_tpp(self, tpp):
+        self._chk(self.lib.mfa_set_threads_per_particle(self.h, tpp), "set_threads_per_particle")
+
+    def set_state(self, r, v, a, a_type, mic=None):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        t = np.ascontiguousarray(a_type, dtype=np.int32)
+        assert r.shape == (self.n, 3) and v.shape == (self.n, 3) and a.shape == (self.n, 3)
+        assert t.shape == (self.n,)
+        m = None if mic is None else np.ascontiguousarray(mic, dtype=np.int32)
+        if m is not None:
+            assert m.shape == (self.n, 3)
+        self._chk(self.lib.mfa_set_state(self.h, _ptr(r, C.c_double), _ptr(v, C.c_double),
+                                         _ptr(a, C.c_double), _ptr(t, C.c_int32),
+                                         _ptr(m, C.c_int32)), "mfa_set_state")
+
+    def get_state(self):
+        n = self.n
+        r, v, a, f = (np.zeros((n, 3)) for _ in range(4))
+        mic = np.zeros((n, 3), dtype=np.int32)
+        self._chk(self.lib.mfa_get_state(self.h, _ptr(r, C.c_double), _ptr(v, C.c_double),
+                                         _ptr(a, C.c_double), _ptr(f, C.c_double),
+                                         _ptr(mic, C.c_int32)), "mfa_get_state")
+        return {"r": r, "v": v, "a": a, "f": f, "mic": mic}
+
+    def verlet_positions(self):
+        self._chk(self.lib.mfa_verlet_positions(self.h), "mfa_verlet_positions")
+
+    def check_skin(self):
+        f = C.c_int32()
+        self._chk(self.lib.mfa_check_skin(self.h, C.byref(f)), "mfa_check_skin")
+        return f.value
+
+    def binning(self):
+        self._chk(self.lib.mfa_binning(self.h), "mfa_binning")
+
+    def get_cells(self):
+        c = np.zeros((self.n, 3), dtype=np.int32)
+        self._chk(self.lib.mfa_get_cells(self.h, _ptr(c, C.c_int32)), "mfa_get_cells")
+        return c
+
+    def get_pairs(self):
+        m = C.c_int64()
+        self._chk(self.lib.mfa_get_pairs(self.h, None, 0, C.byref(m)), "mfa_get_pairs")
+        p = np.zeros((m.value, 2), dtype=np.int32)
+        self._chk(self.lib.mfa_get_pairs(self.h, _ptr(p, C.c_int32), m.value, C.byref(m)),
+                  "mfa_get_pairs")
+        return p
+
+    def zero_force(self):
+        self._chk(self.lib.mfa_zero_force(self.h), "mfa_zero_force")
+
+    def fluid_fluid(self, step):
+        e, q = C.c_double(), C.c_double()
+        self._chk(self.lib.mfa_fluid_fluid(self.h, step, C.byref(e), C.byref(q)), "mfa_fluid_fluid")
+        return e.value, q.value
+
+    def fluid_wall(self, inter_type=2):
+        e = C.c_double()
+        self._chk(self.lib.mfa_fluid_wall(self.h, inter_type, C.byref(e)), "mfa_fluid_wall")
+        return e.value
+
+    def intra_molec(self):
+        e = C.c_double()
+        self._chk(self.lib.mfa_intra_molec(self.h, C.byref(e)), "mfa_intra_molec")
+        return e.value
+
+    def verlet_velocities(self):
+        t = C.c_double()
+        self._chk(self.lib.mfa_verlet_velocities(self.h, C.byref(t)), "mfa_verlet_velocities")
+        return t.value
+
+    def run_steps(self, first_step, n, want_obs=True):
+        obs = Obs()
+        self._chk(self.lib.mfa_run_steps(self.h, first_step, n, C.byref(obs) if want_obs else None),
+                  "mfa_run_steps")
+        return obs.as_dict() if want_obs else None
+
+    def energies(self, step):
+        obs = Obs()
+        self._chk(self.lib.mfa_energies(self.h, step, C.byref(obs)), "mfa_energies")
+        return obs.as_dict()
+
+    def timing(self, enable):
+        fm, im = C.c_double(), C.c_double()
+        fn, inn = C.c_int64(), C.c_int64()
+        self._chk(self.lib.mfa_timing(self.h, int(enable), C.byref(fm), C.byref(fn),
+                                      C.byref(im), C.byref(inn)), "mfa_timing")
+        return {"force_ms": fm.value, "force_n": fn.value, "integrate_ms": im.value,
+                "integrate_n": inn.value}

@@ -1,0 +1,709 @@
+/*
+ * mfa_oracle.c -- TEST INFRASTRUCTURE (the oracle), not product code.
+ *
+ * Plain-C FP64 restatement of the reference's per-step routines, in the
+ * reference's order of operations.  Compiled with -O2 -ffp-contract=off so
+ * every FP64 expression rounds as the Fortran source writes it.  Only tests/,
+ * smoke() and bench.py's cpu_baseline load it, as the checker.
+ *
+ * Parity pinning of each function is described next to it; "parity unpinned"
+ * marks what no known-answer or mathematical check covers.
+ */
+#include "mfa_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct orc_sys {
+  orc_params p;
+  int n;
+  int n_neigh;   /* n_neigh_fl (mfa_common.f90:78) */
+  int n_bin_fl;  /* mfa_common.f90:76 */
+  double *r0, *v, *a, *force, *r0_old;
+  int32_t *a_type, *mic, *mic_old;
+  int32_t* ff_list;    /* [N][0..n_neigh]: row 0 is the count, rows hold 1-based ids */
+  int32_t* bin_fluid;  /* [cells][0..n_bin_fl] */
+  int32_t* cell_of;    /* [N][3] */
+  int f_skin;
+  int n_rebuilds;
+};
+
+/* ------------------------------------------------------------------------ */
+/* Threefry-2x32 (Salmon et al., SC11).  Pinned by the Random123 known-answer
+ * vectors in tests/test_oracle.py. */
+static uint32_t rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+void orc_threefry2x32(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1, int rounds,
+                      uint32_t* o0, uint32_t* o1) {
+  static const int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+  uint32_t key[3] = {k0, k1, 0x1BD11BDAu ^ k0 ^ k1};
+  uint32_t x0 = c0 + key[0], x1 = c1 + key[1];
+  int inj = 0;
+  for (int r = 0; r < rounds; ++r) {
+    x0 = x0 + x1;
+    x1 = rotl(x1, R[r % 8]);
+    x1 = x1 ^ x0;
+    if (r % 4 == 3) {
+      ++inj;
+      x0 = x0 + key[inj % 3];
+      x1 = x1 + key[(inj + 1) % 3] + (uint32_t)inj;
+    }
+  }
+  *o0 = x0;
+  *o1 = x1;
+}
+
+/* Normal deviate of the pair (i0, j0) (0-based ids) at `step`: Box-Muller
+ * on the two 24-bit uniforms of Threefry-2x32-20 with counter (min, max) and
+ * key (seed, step).  Pinned by moment tests (mean, variance, kurtosis) and by
+ * the FP64 GPU form on identical keys. */
+double orc_pair_normal(uint32_t seed, int64_t step, int i0, int j0) {
+  uint32_t w0, w1;
+  const uint32_t lo = (uint32_t)(i0 < j0 ? i0 : j0), hi = (uint32_t)(i0 < j0 ? j0 : i0);
+  orc_threefry2x32(lo, hi, seed, (uint32_t)step, 20, &w0, &w1);
+  const double u1 = (double)(w0 >> 8) * (1.0 / 16777216.0) + (1.0 / 33554432.0);
+  const double u2 = (double)(w1 >> 8) * (1.0 / 16777216.0);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Derived constants: init_params.f90:395-845 for SOLVENT 0/1, WALL 2,
+ * SYSTEM 2/3 (with and without PARTICLE_4), make_binning_boxes.f90:14-64 and
+ * init_config.f90:11-22.  Default-real literals are float values.
+ * Pinned by tests/test_oracle.py against hand-evaluated values of the
+ * reference's expressions (2**(1/6), 2.2, 0.4 as float32, e_shift closed
+ * form) and against the library's own derivation. */
+int orc_derive(const mfa_inputs* in, orc_params* p) {
+  memset(p, 0, sizeof *p);
+  const int nt = ORC_NT;
+  const float c6 = powf(2.0f, 1.0f / 6.0f);
+  p->system = in->system;
+  p->symmetry = in->symmetry;
+  p->thermostat = in->thermostat;
+  p->particle_4 = in->particle_4;
+  p->n_mon = in->n_mon;
+  p->n_chain = in->n_chain;
+  p->n_mon_d = in->n_mon_d;
+  p->n_chain_d = in->n_chain_d;
+  p->n_mon_e = in->particle_4 ? in->n_mon_e : 0;
+  p->n_chain_e = in->particle_4 ? in->n_chain_e : 0;
+  p->part_init_d = in->n_mon * in->n_chain;                          /* init_params.f90:871 */
+  p->part_init_e = p->part_init_d + in->n_mon_d * in->n_chain_d;     /* :872 */
+  p->n_part = p->part_init_e + p->n_mon_e * p->n_chain_e;            /* init_system.f90:199-217 */
+  p->boundary[0] = in->x_space * (double)in->n_cell_w_x;             /* init_config.f90:11-19 */
+  p->boundary[1] = in->y_space * (double)in->n_cell_w_y;
+  p->boundary[2] = in->z_space_wall;
+  for (int d = 0; d < 3; ++d) p->inv_boundary[d] = 1. / p->boundary[d];
+  p->dt = in->dt;
+  p->dt_2 = in->dt * in->dt;                                         /* init_params.f90:395 */
+  p->temp = in->temp;
+  p->r_chain_2 = in->r_chain * in->r_chain;                          /* :396 */
+  p->inv_r_chain_2 = 1. / p->r_chain_2;                              /* :397 */
+  p->k_chain = in->k_chain * p->r_chain_2;                           /* :398 */
+  for (int t = 0; t < nt; ++t) p->mass_type[t] = in->mass_type[t];
+  /* :407-420 */
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) p->sigma[i][j] = (i == j) ? in->sigma_ii[i] : 0.0;
+  for (int i = 0; i < nt - 1; ++i)
+    for (int j = i + 1; j < nt; ++j) {
+      p->sigma[i][j] = (p->sigma[i][i] + p->sigma[j][j]) / 2;
+      p->sigma[j][i] = p->sigma[i][j];
+    }
+  if (!in->particle_4)
+    for (int i = 0; i < nt; ++i) p->sigma[i][nt - 1] = p->sigma[nt - 1][i] = 0.;
+  /* :467-472 */
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      p->sigma_2[i][j] = p->sigma[i][j] * p->sigma[i][j];
+      p->epsil[i][j] = 4. * in->epsil[i][j];
+    }
+  /* :502 and :516-517 */
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      double x = (double)c6 * p->sigma[i][j];
+      p->range_2[i][j] = x * x;
+    }
+  {
+    double x = (double)2.2f * p->sigma[0][nt - 1];
+    p->range_2[0][nt - 1] = x * x;
+    x = (double)2.2f * p->sigma[nt - 1][0];
+    p->range_2[nt - 1][0] = x * x;
+  }
+  if (in->solvent == 0) { /* :545-558 */
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) {
+        double x = (double)(2.0f * c6) * p->sigma[i][j];
+        p->range_2[i][j] = x * x;
+      }
+    if (!in->particle_4)
+      for (int i = 0; i < nt; ++i) p->range_2[i][nt - 1] = p->range_2[nt - 1][i] = 0.;
+  }
+  /* :581-595 */
+  p->a_w = in->a_w;
+  for (int t = 0; t < nt; ++t) {
+    p->sigma_wall[t] = (in->sigma_w + p->sigma[t][t]) / 2;
+    p->a_wall[t] = in->a_w;
+  }
+  if (in->particle_4) {
+    p->sigma_wall[3] = (in->sigma_w4 + p->sigma[3][3]) / 2.;
+    p->a_wall[3] = in->a_w4;
+  }
+  p->z_space_wall = in->z_space_wall;
+  if (!in->particle_4) { /* :604-607 */
+    for (int i = 0; i < nt; ++i) p->range_2[i][nt - 1] = p->range_2[nt - 1][i] = 0.;
+  } else { /* :629-676 */
+    const double cc = (double)c6;
+    for (int i = 0; i < 3; ++i) {
+      double f = (in->inter4 == 4) ? (double)(2.0f * c6) : cc;
+      double x = f * p->sigma[i][3];
+      p->range_2[i][3] = x * x;
+      x = f * p->sigma[3][i];
+      p->range_2[3][i] = x * x;
+    }
+    double x = cc * p->sigma[3][3];
+    if (in->inter4 >= 2) x = 2. * x;
+    p->range_2[3][3] = x * x;
+  }
+  if (in->lj_rc > 0.0) /* survey patch P3 (explicit LJ cut-off) */
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j)
+        if (p->range_2[i][j] > 0.0) {
+          double x = in->lj_rc * p->sigma[i][j];
+          p->range_2[i][j] = x * x;
+        }
+  if (in->system == 0 || in->system == 1 || in->system == 3) p->range_2[0][0] = 0.; /* :703-712 */
+  for (int i = 0; i < nt; ++i) /* :713-724 */
+    for (int j = 0; j < nt; ++j) {
+      if (p->range_2[i][j] > 0.0) {
+        double x = (p->sigma[j][i] * p->sigma[j][i]) / p->range_2[j][i];
+        x = x * x * x;
+        p->e_shift[j][i] = x * (x - 1.);
+      } else {
+        p->e_shift[j][i] = 0.0;
+      }
+    }
+  double mx = 0.;
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j)
+      if (p->range_2[i][j] > mx) mx = p->range_2[i][j];
+  p->r_cut_max = sqrt(mx); /* :766 */
+  if (in->dpd_cut_off > 0.0) { /* :770-781 */
+    p->dpd_cut_off_on = 1;
+    p->r_cut_dpd = (double)(float)in->dpd_cut_off;
+    if (p->r_cut_dpd > p->r_cut_max) p->r_cut_max = p->r_cut_dpd;
+    p->r_cut_dpd_2 = p->r_cut_dpd * p->r_cut_dpd;
+    p->r_cut_max_2 = p->r_cut_max * p->r_cut_max;
+  }
+  for (int i = 0; i < nt; ++i) /* :786-795 */
+    for (int j = 0; j < nt; ++j)
+      p->inv_range_2[i][j] = p->range_2[i][j] != 0 ? 1. / p->range_2[i][j] : 0.;
+  p->skin = in->skin > 0.0 ? in->skin : (double)0.4f; /* :800 (P4 when set) */
+  p->skin_2 = (p->r_cut_max + p->skin) * (p->r_cut_max + p->skin) -
+              p->r_cut_max * p->r_cut_max; /* :808 */
+  p->friction1 = in->friction[0];
+  p->sig = sqrt(2. * in->temp * in->friction[0] / in->dt); /* :845 */
+  p->seed = (uint32_t)in->iseed;
+  for (int d = 0; d < 3; ++d) p->va_spring_twall[d] = in->va_spring_twall[d];
+  /* make_binning_boxes.f90:14-37 on the provisional grid of init_system.f90:273-275 */
+  int prov[3] = {(int)(0.4f * (float)in->n_cell_w_x), (int)(0.7f * (float)in->n_cell_w_y), 20};
+  for (int d = 0; d < 3; ++d)
+    p->delta_bin[d] = prov[d] >= 3 ? 1 : (prov[d] == 2 ? 0 : -1);
+  /* make_binning_boxes.f90:42-64 */
+  const double binning_box_skin = (double)0.4f;
+  for (int d = 0; d < 3; ++d) {
+    p->n_bin[d] = (int)(p->boundary[d] / (p->r_cut_max + binning_box_skin));
+    if (p->n_bin[d] < 1) return 1;
+    p->r_bin[d] = p->boundary[d] / (double)p->n_bin[d];
+    p->inv_r_bin[d] = 1. / p->r_bin[d];
+  }
+  /* init_config.f90:264-281 and init_force_switch_on.f90:15-26 */
+  if (in->force_switch_on && in->s_time == 1) {
+    double maxsig = 0.;
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j)
+        if (p->sigma[i][j] > maxsig) maxsig = p->sigma[i][j];
+    p->r_2_min_init = (maxsig / 0.9f) * (maxsig / 0.9f);
+    p->r_2_min_time = (double)in->n_relax * in->dt < 40. ? 40. : (double)in->n_relax * in->dt;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+orc_sys* orc_create(const orc_params* p, int n_neigh) {
+  orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
+  s->p = *p;
+  s->n = p->n_part;
+  s->n_neigh = n_neigh > 0 ? n_neigh : 256;
+  s->n_bin_fl = 300;
+  const size_t n = (size_t)s->n;
+  s->r0 = (double*)calloc(3 * n, sizeof(double));
+  s->v = (double*)calloc(3 * n, sizeof(double));
+  s->a = (double*)calloc(3 * n, sizeof(double));
+  s->force = (double*)calloc(3 * n, sizeof(double));
+  s->r0_old = (double*)calloc(3 * n, sizeof(double));
+  s->a_type = (int32_t*)calloc(n, sizeof(int32_t));
+  s->mic = (int32_t*)calloc(3 * n, sizeof(int32_t));
+  s->mic_old = (int32_t*)calloc(3 * n, sizeof(int32_t));
+  s->ff_list = (int32_t*)calloc(n * (size_t)(s->n_neigh + 1), sizeof(int32_t));
+  const size_t nc = (size_t)p->n_bin[0] * p->n_bin[1] * p->n_bin[2];
+  s->bin_fluid = (int32_t*)calloc(nc * (size_t)(s->n_bin_fl + 1), sizeof(int32_t));
+  s->cell_of = (int32_t*)calloc(3 * n, sizeof(int32_t));
+  return s;
+}
+
+void orc_destroy(orc_sys* s) {
+  if (!s) return;
+  free(s->r0); free(s->v); free(s->a); free(s->force); free(s->r0_old);
+  free(s->a_type); free(s->mic); free(s->mic_old); free(s->ff_list); free(s->bin_fluid);
+  free(s->cell_of);
+  free(s);
+}
+
+void orc_set_state(orc_sys* s, const double* r0, const double* v, const double* a,
+                   const int32_t* a_type, const int32_t* mic) {
+  const size_t n = (size_t)s->n;
+  memcpy(s->r0, r0, 3 * n * sizeof(double));
+  memcpy(s->v, v, 3 * n * sizeof(double));
+  memcpy(s->a, a, 3 * n * sizeof(double));
+  memcpy(s->a_type, a_type, n * sizeof(int32_t));
+  if (mic) memcpy(s->mic, mic, 3 * n * sizeof(int32_t));
+  else memset(s->mic, 0, 3 * n * sizeof(int32_t));
+  s->f_skin = 1;
+}
+
+void orc_get_state(orc_sys* s, double* r0, double* v, double* a, double* force, int32_t* mic) {
+  const size_t n = (size_t)s->n;
+  if (r0) memcpy(r0, s->r0, 3 * n * sizeof(double));
+  if (v) memcpy(v, s->v, 3 * n * sizeof(double));
+  if (a) memcpy(a, s->a, 3 * n * sizeof(double));
+  if (force) memcpy(force, s->force, 3 * n * sizeof(double));
+  if (mic) memcpy(mic, s->mic, 3 * n * sizeof(int32_t));
+}
+
+#define R(i, d) s->r0[3 * (size_t)(i) + (d)]
+#define V(i, d) s->v[3 * (size_t)(i) + (d)]
+#define A(i, d) s->a[3 * (size_t)(i) + (d)]
+#define F(i, d) s->force[3 * (size_t)(i) + (d)]
+#define MIC(i, d) s->mic[3 * (size_t)(i) + (d)]
+
+/* verlet_positions.f90:301-326 (update) and :392-406 (wrap).  Pinned by
+ * closed forms: free flight and constant acceleration, tests/test_oracle.py. */
+void orc_verlet_positions(orc_sys* s) {
+  const orc_params* p = &s->p;
+  for (int i = 0; i < s->n; ++i) {
+    for (int d = 0; d < 3; ++d) R(i, d) = R(i, d) + p->dt * V(i, d) + 0.5 * p->dt_2 * A(i, d);
+    for (int d = 0; d < 3; ++d) V(i, d) = V(i, d) + 0.5 * p->dt * A(i, d);
+  }
+  const int nd = p->symmetry == 0 ? 2 : 3;
+  for (int i = 0; i < s->n; ++i)
+    for (int d = 0; d < nd; ++d) {
+      if (R(i, d) > p->boundary[d]) {
+        R(i, d) = R(i, d) - p->boundary[d];
+        MIC(i, d) += 1;
+      } else if (R(i, d) < 0.) {
+        R(i, d) = R(i, d) + p->boundary[d];
+        MIC(i, d) -= 1;
+      }
+    }
+}
+
+/* check_skin.f90:139-152.  Pinned by a closed-form displacement test. */
+int orc_check_skin(orc_sys* s) {
+  const orc_params* p = &s->p;
+  for (int i = 0; i < s->n; ++i) {
+    double dr[3], r_dummy = 0.;
+    for (int d = 0; d < 3; ++d)
+      dr[d] = s->r0_old[3 * (size_t)i + d] + (double)s->mic_old[3 * (size_t)i + d] * p->boundary[d] -
+              R(i, d) - (double)MIC(i, d) * p->boundary[d];
+    for (int d = 0; d < 3; ++d) r_dummy = r_dummy + dr[d] * dr[d];
+    if (r_dummy > (p->skin / 2.) * (p->skin / 2.)) s->f_skin = 1;
+  }
+  return s->f_skin;
+}
+
+static double pbc(double d, double L, double invL) { return d - L * (double)(int)(2 * d * invL); }
+
+static double list_crit(const orc_params* p, int ti, int tj) {
+  return p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : p->range_2[ti][tj] + p->skin_2;
+}
+
+/* binning.f90:11-127: cell fill in particle order, then the 27-cell scan with
+ * i<j and the balanced insertion into the shorter list.  The pair set is
+ * pinned by orc_brute_pairs (tests/test_oracle.py). */
+int orc_binning(orc_sys* s) {
+  const orc_params* p = &s->p;
+  const int nx = p->n_bin[0], ny = p->n_bin[1], nz = p->n_bin[2];
+  const int stride = s->n_bin_fl + 1;
+  const size_t nc = (size_t)nx * ny * nz;
+  int err = 0;
+  memset(s->bin_fluid, 0, nc * (size_t)stride * sizeof(int32_t));
+#define BIN(x, y, z, k) s->bin_fluid[((((size_t)(z) * ny + (y)) * nx + (x)) * stride) + (k)]
+  for (int i = 0; i < s->n; ++i) {
+    const int ix = ((int)(R(i, 0) * p->inv_r_bin[0])) % nx;
+    const int iy = ((int)(R(i, 1) * p->inv_r_bin[1])) % ny;
+    const int iz = ((int)(R(i, 2) * p->inv_r_bin[2])) % nz;
+    if (ix < 0 || iy < 0 || iz < 0) return MFA_ERR_CELL;
+    s->cell_of[3 * (size_t)i + 0] = ix;
+    s->cell_of[3 * (size_t)i + 1] = iy;
+    s->cell_of[3 * (size_t)i + 2] = iz;
+    const int c = ++BIN(ix, iy, iz, 0);
+    if (c > s->n_bin_fl) return MFA_ERR_LIST_OVERFLOW;
+    BIN(ix, iy, iz, c) = i + 1;
+  }
+  const int ls = s->n_neigh + 1;
+#define FF(i, k) s->ff_list[(size_t)(i) * ls + (k)]
+  for (int i = 0; i < s->n; ++i) FF(i, 0) = 0;
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const int n1 = BIN(ix, iy, iz, 0);
+        for (int k1 = 1; k1 <= n1; ++k1) {
+          const int ip = BIN(ix, iy, iz, k1) - 1;
+          const int ti = s->a_type[ip] - 1;
+          for (int jx = ix - 1; jx <= ix + p->delta_bin[0]; ++jx) {
+            const int kx = (jx + nx) % nx;
+            for (int jy = iy - 1; jy <= iy + p->delta_bin[1]; ++jy) {
+              const int ky = (jy + ny) % ny;
+              for (int jz = iz - 1; jz <= iz + p->delta_bin[2]; ++jz) {
+                const int kz = (jz + nz) % nz;
+                const int n2 = BIN(kx, ky, kz, 0);
+                for (int k2 = 1; k2 <= n2; ++k2) {
+                  const int jp = BIN(kx, ky, kz, k2) - 1;
+                  if (ip < jp) {
+                    const int tj = s->a_type[jp] - 1;
+                    double dr[3];
+                    for (int d = 0; d < 3; ++d) dr[d] = R(ip, d) - R(jp, d);
+                    dr[0] = pbc(dr[0], p->boundary[0], p->inv_boundary[0]);
+                    dr[1] = pbc(dr[1], p->boundary[1], p->inv_boundary[1]);
+                    if (p->symmetry == 1) dr[2] = pbc(dr[2], p->boundary[2], p->inv_boundary[2]);
+                    double r_2 = 0.;
+                    r_2 = r_2 + dr[0] * dr[0];
+                    r_2 = r_2 + dr[1] * dr[1];
+                    r_2 = r_2 + dr[2] * dr[2];
+                    if (r_2 < list_crit(p, ti, tj)) {
+                      if (FF(ip, 0) <= FF(jp, 0)) {
+                        if (FF(ip, 0) >= s->n_neigh) err = MFA_ERR_LIST_OVERFLOW;
+                        else { FF(ip, 0) += 1; FF(ip, FF(ip, 0)) = jp + 1; }
+                      } else {
+                        if (FF(jp, 0) >= s->n_neigh) err = MFA_ERR_LIST_OVERFLOW;
+                        else { FF(jp, 0) += 1; FF(jp, FF(jp, 0)) = ip + 1; }
+                      }
+                    }
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+  s->f_skin = 0; /* :125-127 */
+  memcpy(s->r0_old, s->r0, 3 * (size_t)s->n * sizeof(double));
+  memcpy(s->mic_old, s->mic, 3 * (size_t)s->n * sizeof(int32_t));
+  s->n_rebuilds += 1;
+  return err;
+#undef BIN
+}
+
+void orc_get_cells(orc_sys* s, int32_t* cells) {
+  memcpy(cells, s->cell_of, 3 * (size_t)s->n * sizeof(int32_t));
+}
+
+static int cmp_pair(const void* x, const void* y) {
+  const int32_t* a = (const int32_t*)x;
+  const int32_t* b = (const int32_t*)y;
+  if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+  if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
+  return 0;
+}
+
+/* Unordered pair set of ff_list as sorted (min,max), 1-based. */
+int64_t orc_get_pairs(orc_sys* s, int32_t* pairs, int64_t cap) {
+  const int ls = s->n_neigh + 1;
+  int64_t m = 0;
+  for (int i = 0; i < s->n; ++i)
+    for (int k = 1; k <= FF(i, 0); ++k) {
+      const int j = FF(i, k) - 1;
+      if (pairs && m < cap) {
+        pairs[2 * m] = (i < j ? i : j) + 1;
+        pairs[2 * m + 1] = (i < j ? j : i) + 1;
+      }
+      ++m;
+    }
+  if (pairs && m <= cap) qsort(pairs, (size_t)m, 2 * sizeof(int32_t), cmp_pair);
+  return m;
+}
+
+/* O(N^2) version of the list criterion of binning.f90:84-105 over all i<j,
+ * independent of the cell grid. */
+int64_t orc_brute_pairs(orc_sys* s, int32_t* pairs, int64_t cap) {
+  const orc_params* p = &s->p;
+  int64_t m = 0;
+  for (int i = 0; i < s->n; ++i)
+    for (int j = i + 1; j < s->n; ++j) {
+      double dr[3];
+      for (int d = 0; d < 3; ++d) dr[d] = R(i, d) - R(j, d);
+      dr[0] = pbc(dr[0], p->boundary[0], p->inv_boundary[0]);
+      dr[1] = pbc(dr[1], p->boundary[1], p->inv_boundary[1]);
+      if (p->symmetry == 1) dr[2] = pbc(dr[2], p->boundary[2], p->inv_boundary[2]);
+      const double r_2 = (dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2];
+      if (r_2 < list_crit(p, s->a_type[i] - 1, s->a_type[j] - 1)) {
+        if (pairs && m < cap) {
+          pairs[2 * m] = i + 1;
+          pairs[2 * m + 1] = j + 1;
+        }
+        ++m;
+      }
+    }
+  return m;
+}
+
+void orc_zero_force(orc_sys* s) { memset(s->force, 0, 3 * (size_t)s->n * sizeof(double)); }
+
+/* fluid_fluid.f90:52-223 with dpd_forces.f90:256-352 (DPD_WEIGHT 0,
+ * BIN_TYPE 0).  LJ is pinned by closed forms (minimum at 2^(1/6) sigma,
+ * shifted energy zero at the cut-off) and energy/force finite differences;
+ * the pair loop by antisymmetry (total force zero with DPD off); DPD by its
+ * fluctuation-dissipation moments. */
+void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
+  const orc_params* p = &s->p;
+  const int ls = s->n_neigh + 1;
+  const int head_excl = p->symmetry == 0 && (p->system == 0 || p->system == 1 || p->system == 3);
+  double v_ff = 0., vir = 0.;
+  /* force switch-on (fluid_fluid.f90:42-49), r_time = step*dt (md_main.f90:69) */
+  double r_2_min = 0.;
+  const double r_time = (double)step * p->dt;
+  if (p->r_2_min_time > 0. && r_time < p->r_2_min_time)
+    r_2_min = (1. - r_time / p->r_2_min_time) * p->r_2_min_init;
+  for (int ip = 0; ip < s->n; ++ip) {
+    const int ti = s->a_type[ip] - 1;
+    double f_ipart[3] = {0., 0., 0.};
+    for (int k = 1; k <= FF(ip, 0); ++k) {
+      const int jp = FF(ip, k) - 1;
+      const int tj = s->a_type[jp] - 1;
+      const double r_cut2 = p->range_2[ti][tj];
+      double dr[3];
+      for (int d = 0; d < 3; ++d) dr[d] = R(ip, d) - R(jp, d);
+      dr[0] = pbc(dr[0], p->boundary[0], p->inv_boundary[0]);
+      dr[1] = pbc(dr[1], p->boundary[1], p->inv_boundary[1]);
+      if (p->symmetry == 1) dr[2] = pbc(dr[2], p->boundary[2], p->inv_boundary[2]);
+      double r_2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+      const double r_2_raw = r_2;
+      int do_dpd = 0;
+      double w_arg = 0.;
+      double inv_sqrt_r_2 = sqrt(1. / r_2);
+      if (r_2 < r_cut2) {
+        if (r_2 < r_2_min) r_2 = r_2_min; /* r_2 = max(r_2, r_2_min) (:96) */
+        const double inv_r_2 = 1. / r_2;
+        inv_sqrt_r_2 = sqrt(1. / r_2);
+        const double l_eps = p->epsil[ti][tj];
+        const double r_61 = p->sigma_2[ti][tj] * inv_r_2;
+        const double r_6 = r_61 * r_61 * r_61;
+        const double r_12 = r_6 * r_6;
+        const double pot_loc = (r_12 - r_6) - p->e_shift[ti][tj];
+        v_ff = v_ff + l_eps * pot_loc;
+        const double r_dummy = l_eps * (-12 * r_12 + 6 * r_6) * inv_r_2;
+        vir = vir + (-r_dummy * r_2_raw);
+        for (int d = 0; d < 3; ++d) {
+          const double fl = r_dummy * dr[d];
+          f_ipart[d] = f_ipart[d] - fl;
+          F(jp, d) = F(jp, d) + fl;
+        }
+        if (p->thermostat == 0 && !p->dpd_cut_off_on) {
+          do_dpd = 1;
+          w_arg = r_2 * p->inv_range_2[ti][tj];
+        }
+      }
+      if (p->thermostat == 0 && p->dpd_cut_off_on) {
+        if (head_excl && (ti == 0 || tj == 0)) continue;
+        if (r_2 < p->r_cut_dpd_2) {
+          do_dpd = 1;
+          w_arg = r_2 / p->r_cut_dpd_2;
+        }
+      }
+      if (do_dpd) { /* dpd_forces.f90:273-348 */
+        const double g_rand = orc_pair_normal(p->seed, step, ip, jp);
+        double w_d = 1. - sqrt(w_arg);
+        const double w_r = w_d;
+        w_d = w_d * w_d;
+        double rv[3], vd[3], dv[3];
+        for (int d = 0; d < 3; ++d) rv[d] = dr[d] * inv_sqrt_r_2;
+        for (int d = 0; d < 3; ++d) vd[d] = p->sig * w_r * g_rand * rv[d];
+        for (int d = 0; d < 3; ++d) {
+          F(ip, d) = F(ip, d) + vd[d];
+          F(jp, d) = F(jp, d) - vd[d];
+        }
+        for (int d = 0; d < 3; ++d) dv[d] = V(ip, d) - V(jp, d);
+        const double rdum = dv[0] * rv[0] + dv[1] * rv[1] + dv[2] * rv[2];
+        for (int d = 0; d < 3; ++d) vd[d] = -1. * p->friction1 * w_d * rdum * rv[d];
+        for (int d = 0; d < 3; ++d) {
+          F(ip, d) = F(ip, d) + vd[d];
+          F(jp, d) = F(jp, d) - vd[d];
+        }
+      }
+    }
+    for (int d = 0; d < 3; ++d) F(ip, d) = F(ip, d) + f_ipart[d];
+  }
+  out[0] = v_ff;
+  out[1] = vir;
+}
+
+static double ipow(double x, int n) { /* square-and-multiply, lowest bit first */
+  double res = 1.0, sq = x;
+  int first = 1;
+  while (n) {
+    if (n & 1) {
+      res = first ? sq : res * sq;
+      first = 0;
+    }
+    n >>= 1;
+    if (n) sq = sq * sq;
+  }
+  return res;
+}
+
+/* fluid_wall.f90:81-121 (WALL 2).  Pinned by the closed form of the 9-3
+ * potential (force = -dV/dz by finite differences, zero of the force). */
+void orc_fluid_wall(orc_sys* s, double* out) {
+  const orc_params* p = &s->p;
+  double v_fw = 0., fzb = 0., fzt = 0.;
+  for (int i = 0; i < s->n; ++i) {
+    const int t = s->a_type[i] - 1;
+    const double sw = p->sigma_wall[t], aw = p->a_wall[t];
+    double inv_z = 1. / R(i, 2);
+    double r_dummy = sw * inv_z;
+    v_fw = v_fw + fabs(aw) * ipow(r_dummy, 9) - p->a_w * ipow(r_dummy, 3);
+    double f1 = 9. * fabs(aw) * ipow(sw, 9) * ipow(inv_z, 10);
+    double f2 = 3. * aw * ipow(sw, 3) * ipow(inv_z, 4);
+    F(i, 2) = F(i, 2) + f1;
+    F(i, 2) = F(i, 2) - f2;
+    fzb += f1 - f2;
+    inv_z = 1. / (p->z_space_wall - R(i, 2));
+    r_dummy = sw * inv_z;
+    v_fw = v_fw + fabs(aw) * ipow(r_dummy, 9) - p->a_w * ipow(r_dummy, 3);
+    f1 = 9. * fabs(aw) * ipow(sw, 9) * ipow(inv_z, 10);
+    f2 = 3 * aw * ipow(sw, 3) * ipow(inv_z, 4);
+    F(i, 2) = F(i, 2) - f1;
+    F(i, 2) = F(i, 2) + f2;
+    fzt += -f1 + f2;
+  }
+  out[0] = v_fw;
+  out[1] = fzb;
+  out[2] = fzt;
+}
+
+static void fene_bond(orc_sys* s, int ip, int jp, double* pot, double* vir) {
+  const orc_params* p = &s->p;
+  double dr[3];
+  for (int d = 0; d < 3; ++d) dr[d] = R(ip, d) - R(jp, d);
+  dr[0] = pbc(dr[0], p->boundary[0], p->inv_boundary[0]);
+  dr[1] = pbc(dr[1], p->boundary[1], p->inv_boundary[1]);
+  if (p->symmetry == 1) dr[2] = pbc(dr[2], p->boundary[2], p->inv_boundary[2]);
+  double r_2 = 0.;
+  for (int d = 0; d < 3; ++d) r_2 = r_2 + dr[d] * dr[d];
+  double r_dummy = r_2 * p->inv_r_chain_2;
+  if (r_dummy > (double)0.98f) r_dummy = (double)0.98f;
+  *pot = *pot + (-0.5 * log(1. - r_dummy));
+  r_dummy = p->k_chain / (1. - r_dummy) * p->inv_r_chain_2;
+  *vir = *vir + (-r_dummy * r_2);
+  for (int d = 0; d < 3; ++d) {
+    const double fl = r_dummy * dr[d];
+    F(ip, d) = F(ip, d) - fl;
+    F(jp, d) = F(jp, d) + fl;
+  }
+}
+
+/* intra_molec.f90:16-63 (brushes), :101-139 and :270 (melt), :280-446
+ * (particle-4 chains).  FENE pinned by its closed form and finite
+ * differences of the energy. */
+void orc_intra_molec(orc_sys* s, double* out) {
+  const orc_params* p = &s->p;
+  double vb = 0., vd = 0., ve = 0., vir = 0.;
+  for (int c = 0; c < p->n_chain; ++c)
+    for (int m = 0; m < p->n_mon - 1; ++m) {
+      const int ip = c * p->n_mon + m;
+      fene_bond(s, ip, ip + 1, &vb, &vir);
+    }
+  vb = p->k_chain * vb;
+  for (int c = 0; c < p->n_chain_d; ++c)
+    for (int m = 0; m < p->n_mon_d - 1; ++m) {
+      const int ip = p->part_init_d + c * p->n_mon_d + m;
+      fene_bond(s, ip, ip + 1, &vd, &vir);
+    }
+  vd = p->k_chain * vd;
+  for (int c = 0; c < p->n_chain_e; ++c)
+    for (int m = 0; m < p->n_mon_e - 1; ++m) {
+      const int ip = p->part_init_e + c * p->n_mon_e + m;
+      fene_bond(s, ip, ip + 1, &ve, &vir);
+    }
+  ve = p->k_chain * ve;
+  out[0] = vb + vd + ve;
+  out[1] = vir;
+}
+
+/* verlet_velocities.f90:25-36, head pinning :113-134, kinetic energy
+ * :155-177.  Pinned by closed forms (constant force, pinned heads). */
+double orc_verlet_velocities(orc_sys* s) {
+  const orc_params* p = &s->p;
+  for (int i = 0; i < s->n; ++i) {
+    const double inv_mass = 1. / p->mass_type[s->a_type[i] - 1];
+    for (int d = 0; d < 3; ++d) A(i, d) = F(i, d) * inv_mass;
+    for (int d = 0; d < 3; ++d) V(i, d) = V(i, d) + 0.5 * p->dt * A(i, d);
+  }
+  if (p->symmetry == 0 && (p->system == 0 || p->system == 3)) {
+    for (int c = 0; c < p->n_chain / 2; ++c) {
+      const int h = p->n_mon * c;
+      V(h, 0) = p->va_spring_twall[0];
+      V(h, 1) = p->va_spring_twall[1];
+      V(h, 2) = 0.;
+      for (int d = 0; d < 3; ++d) F(h, d) = A(h, d) = 0.;
+    }
+    for (int c = p->n_chain / 2; c < p->n_chain; ++c) {
+      const int h = p->n_mon * c;
+      V(h, 0) = -p->va_spring_twall[0];
+      V(h, 1) = -p->va_spring_twall[1];
+      V(h, 2) = 0.;
+      for (int d = 0; d < 3; ++d) F(h, d) = A(h, d) = 0.;
+    }
+  }
+  double t_inst = 0.;
+  for (int i = 0; i < s->n; ++i) {
+    const double m = p->mass_type[s->a_type[i] - 1];
+    t_inst = t_inst + m * (V(i, 0) * V(i, 0) + V(i, 1) * V(i, 1) + V(i, 2) * V(i, 2));
+  }
+  return 0.5 * t_inst;
+}
+
+/* md_main.f90:69-155 for SYSTEM 2/3, WALL 2, THERMOSTAT 0 or none, BIN_TYPE 0. */
+int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
+  int rc = 0;
+  if (s->f_skin) { /* init_config.f90:251-255: bin before the loop */
+    rc = orc_binning(s);
+    if (rc) return rc;
+  }
+  const int reb0 = s->n_rebuilds; /* count the rebuilds inside the loop only */
+  double e_ff[2] = {0, 0}, e_fw[3] = {0, 0, 0}, e_in[2] = {0, 0}, tk = 0.;
+  for (int it = 0; it < n; ++it) {
+    const int64_t step = first_step + it;
+    orc_verlet_positions(s);
+    if (orc_check_skin(s)) {
+      rc = orc_binning(s);
+      if (rc) return rc;
+    }
+    orc_zero_force(s);
+    orc_fluid_fluid(s, step, e_ff);
+    if (s->p.symmetry == 0) orc_fluid_wall(s, e_fw);
+    orc_intra_molec(s, e_in);
+    tk = orc_verlet_velocities(s);
+  }
+  if (obs) {
+    obs[0] = e_ff[0];
+    obs[1] = e_fw[0];
+    obs[2] = e_in[0];
+    obs[3] = tk;
+    obs[4] = e_ff[1] + e_in[1];
+    obs[5] = (double)(s->n_rebuilds - reb0);
+    obs[6] = e_fw[1];
+  }
+  return 0;
+}

@@ -1,0 +1,221 @@
+"""ctypes binding of the oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  It never stands in for the HIP library.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+NT = 4
+
+_d4 = C.c_double * NT
+_d44 = (C.c_double * NT) * NT
+_d3 = C.c_double * 3
+_i3 = C.c_int * 3
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("n_part", C.c_int), ("n_mon", C.c_int), ("n_chain", C.c_int), ("n_mon_d", C.c_int),
+        ("n_chain_d", C.c_int), ("n_mon_e", C.c_int), ("n_chain_e", C.c_int),
+        ("part_init_d", C.c_int), ("part_init_e", C.c_int),
+        ("system", C.c_int), ("symmetry", C.c_int), ("thermostat", C.c_int),
+        ("dpd_cut_off_on", C.c_int), ("particle_4", C.c_int),
+        ("boundary", _d3), ("inv_boundary", _d3),
+        ("dt", C.c_double), ("dt_2", C.c_double), ("temp", C.c_double),
+        ("mass_type", _d4),
+        ("sigma", _d44), ("sigma_2", _d44), ("epsil", _d44),
+        ("range_2", _d44), ("inv_range_2", _d44), ("e_shift", _d44),
+        ("r_cut_max", C.c_double), ("r_cut_max_2", C.c_double),
+        ("r_cut_dpd", C.c_double), ("r_cut_dpd_2", C.c_double),
+        ("skin", C.c_double), ("skin_2", C.c_double),
+        ("friction1", C.c_double), ("sig", C.c_double),
+        ("k_chain", C.c_double), ("r_chain_2", C.c_double), ("inv_r_chain_2", C.c_double),
+        ("a_w", C.c_double), ("sigma_wall", _d4), ("a_wall", _d4), ("z_space_wall", C.c_double),
+        ("va_spring_twall", _d3),
+        ("n_bin", _i3), ("delta_bin", _i3), ("r_bin", _d3), ("inv_r_bin", _d3),
+        ("seed", C.c_uint32), ("r_2_min_init", C.c_double), ("r_2_min_time", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    u32 = C.c_uint32
+    lib.orc_threefry2x32.argtypes = [u32, u32, u32, u32, C.c_int, C.POINTER(u32), C.POINTER(u32)]
+    lib.orc_threefry2x32.restype = None
+    lib.orc_pair_normal.argtypes = [u32, C.c_int64, C.c_int, C.c_int]
+    lib.orc_pair_normal.restype = C.c_double
+    lib.orc_derive.argtypes = [C.c_void_p, C.POINTER(OrcParams)]
+    lib.orc_derive.restype = C.c_int
+    lib.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int]
+    lib.orc_create.restype = P
+    lib.orc_destroy.argtypes = [P]
+    lib.orc_set_state.argtypes = [P, dp, dp, dp, ip, ip]
+    lib.orc_get_state.argtypes = [P, dp, dp, dp, dp, ip]
+    lib.orc_verlet_positions.argtypes = [P]
+    lib.orc_check_skin.argtypes = [P]
+    lib.orc_check_skin.restype = C.c_int
+    lib.orc_binning.argtypes = [P]
+    lib.orc_binning.restype = C.c_int
+    lib.orc_get_cells.argtypes = [P, ip]
+    lib.orc_get_pairs.argtypes = [P, ip, C.c_int64]
+    lib.orc_get_pairs.restype = C.c_int64
+    lib.orc_brute_pairs.argtypes = [P, ip, C.c_int64]
+    lib.orc_brute_pairs.restype = C.c_int64
+    lib.orc_zero_force.argtypes = [P]
+    lib.orc_fluid_fluid.argtypes = [P, C.c_int64, dp]
+    lib.orc_fluid_wall.argtypes = [P, dp]
+    lib.orc_intra_molec.argtypes = [P, dp]
+    lib.orc_verlet_velocities.argtypes = [P]
+    lib.orc_verlet_velocities.restype = C.c_double
+    lib.orc_run_steps.argtypes = [P, C.c_int64, C.c_int, dp]
+    lib.orc_run_steps.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def threefry2x32(c0, c1, k0, k1, rounds=20):
+    lib = load()
+    a, b = C.c_uint32(), C.c_uint32()
+    lib.orc_threefry2x32(c0, c1, k0, k1, rounds, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def pair_normal(seed, step, i, j):
+    return load().orc_pair_normal(seed, step, i, j)
+
+
+def derive(inputs):
+    lib = load()
+    p = OrcParams()
+    rc = lib.orc_derive(C.byref(inputs), C.byref(p))
+    if rc:
+        raise RuntimeError("orc_derive failed")
+    return p
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+class System:
+    """The reference's state and routines on the CPU (FP64, one thread)."""
+
+    def __init__(self, inputs, n_neigh=256):
+        self.lib = load()
+        self.p = derive(inputs)
+        self.n = self.p.n_part
+        self.h = self.lib.orc_create(C.byref(self.p), n_neigh)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, r, v, a, t, mic=None):
+        self._keep = [np.ascontiguousarray(r, dtype=np.float64),
+                      np.ascontiguousarray(v, dtype=np.float64),
+                      np.ascontiguousarray(a, dtype=np.float64),
+                      np.ascontiguousarray(t, dtype=np.int32),
+                      None if mic is None else np.ascontiguousarray(mic, dtype=np.int32)]
+        r, v, a, t, m = self._keep
+        self.lib.orc_set_state(self.h, _dp(r), _dp(v), _dp(a), _ip(t), _ip(m))
+
+    def get_state(self):
+        n = self.n
+        r, v, a, f = (np.zeros((n, 3)) for _ in range(4))
+        mic = np.zeros((n, 3), dtype=np.int32)
+        self.lib.orc_get_state(self.h, _dp(r), _dp(v), _dp(a), _dp(f), _ip(mic))
+        return {"r": r, "v": v, "a": a, "f": f, "mic": mic}
+
+    def verlet_positions(self):
+        self.lib.orc_verlet_positions(self.h)
+
+    def check_skin(self):
+        return self.lib.orc_check_skin(self.h)
+
+    def binning(self):
+        rc = self.lib.orc_binning(self.h)
+        if rc:
+            raise RuntimeError("orc_binning error %d" % rc)
+
+    def get_cells(self):
+        c = np.zeros((self.n, 3), dtype=np.int32)
+        self.lib.orc_get_cells(self.h, _ip(c))
+        return c
+
+    def get_pairs(self):
+        m = self.lib.orc_get_pairs(self.h, None, 0)
+        p = np.zeros((m, 2), dtype=np.int32)
+        self.lib.orc_get_pairs(self.h, _ip(p), m)
+        return p
+
+    def brute_pairs(self):
+        m = self.lib.orc_brute_pairs(self.h, None, 0)
+        p = np.zeros((m, 2), dtype=np.int32)
+        self.lib.orc_brute_pairs(self.h, _ip(p), m)
+        return p
+
+    def zero_force(self):
+        self.lib.orc_zero_force(self.h)
+
+    def fluid_fluid(self, step):
+        out = np.zeros(2)
+        self.lib.orc_fluid_fluid(self.h, step, _dp(out))
+        return out
+
+    def fluid_wall(self):
+        out = np.zeros(3)
+        self.lib.orc_fluid_wall(self.h, _dp(out))
+        return out
+
+    def intra_molec(self):
+        out = np.zeros(2)
+        self.lib.orc_intra_molec(self.h, _dp(out))
+        return out
+
+    def verlet_velocities(self):
+        return self.lib.orc_verlet_velocities(self.h)
+
+    def run_steps(self, first_step, n):
+        obs = np.zeros(8)
+        rc = self.lib.orc_run_steps(self.h, first_step, n, _dp(obs))
+        if rc:
+            raise RuntimeError("orc_run_steps error %d" % rc)
+        return {"v_fluid_fluid": obs[0], "v_fluid_wall": obs[1], "v_intra_molec": obs[2],
+                "t_kin": obs[3], "virial": obs[4], "n_rebuilds": obs[5], "wall_fz_bottom": obs[6]}
+
+    def forces(self, step):
+        """Total force of one step's force pass at the current state
+        (md_main.f90:88-106 without integrating)."""
+        self.zero_force()
+        e_ff = self.fluid_fluid(step)
+        e_fw = self.fluid_wall() if self.p.symmetry == 0 else np.zeros(3)
+        e_in = self.intra_molec()
+        return self.get_state()["f"], {"v_fluid_fluid": e_ff[0], "v_fluid_wall": e_fw[0],
+                                       "v_intra_molec": e_in[0], "virial": e_ff[1] + e_in[1],
+                                       "wall_fz_bottom": e_fw[1], "wall_fz_top": e_fw[2]}
