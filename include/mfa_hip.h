@@ -218,6 +218,11 @@ int32_t mfa_run_steps(mfa_ctx* ctx, int64_t first_step, int32_t n, mfa_obs* obs)
  * step `step`, the state is not advanced). */
 int32_t mfa_energies(mfa_ctx* ctx, int64_t step, mfa_obs* obs);
 
+/* List statistics: out[0] = largest neighbour count seen since create,
+ * out[1] = list capacity, out[2] = device error code, out[3] = rebuilds in the
+ * current run_steps call. */
+int32_t mfa_stats(mfa_ctx* ctx, int64_t* out /* [4] */);
+
 /* Kernel timing of the force pass: average duration (ms) over the launches
  * since the last reset, measured with HIP events on the context's stream. */
 int32_t mfa_timing(mfa_ctx* ctx, int32_t enable, double* force_ms_avg, int64_t* force_launches,

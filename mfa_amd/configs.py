@@ -83,7 +83,7 @@ def brush(n_chain, n_mon, n_solvent, lx, ly, lz, x_space=1.0, dt=0.005, gamma=4.
 
 
 def melt(n_chain, n_mon, density, lj_rc=2.5, dt=0.01, gamma=1.0, skin=0.3, seed=2002,
-         thermostat=0, n_cell=None):
+         thermostat=0, n_cell=None, switch_on=False):
     """Periodic Kremer-Grest melt (SYSTEM 2, SYMMETRY 1), chains as melt
     ('droplet') chains of type 3, LJ cut-off lj_rc (survey patch P3)."""
     x = base_inputs()
@@ -105,6 +105,11 @@ def melt(n_chain, n_mon, density, lj_rc=2.5, dt=0.01, gamma=1.0, skin=0.3, seed=
     x.skin = skin
     x.iseed = seed
     x.thermostat = thermostat
+    if switch_on:
+        # fresh start with FORCE_SWITCH_ON: the reference ramps the minimum
+        # pair distance over 40 tau (init_force_switch_on.f90:15-26)
+        x.s_time = 1
+        x.force_switch_on = 1
     return x
 
 
@@ -114,7 +119,7 @@ WORKLOADS = {
                     make=lambda: brush(100, 30, 6000, 10.0, 10.0, 20.0, seed=1001)),
     # Kremer-Grest melt, the headline workload
     "melt100k": dict(kind=2, steps=5000, min_dist=0.8,
-                     make=lambda: melt(2000, 50, 0.85, seed=2002)),
+                     make=lambda: melt(2000, 50, 0.85, seed=2002, switch_on=True)),
 }
 
 
@@ -124,6 +129,7 @@ def small_cases():
         "brush_small": (1, brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=0.5, seed=11)),
         "melt_small": (2, melt(30, 20, 0.85, lj_rc=2.5, seed=12, n_cell=16)),
         "melt_wca": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=13, n_cell=16)),
+        "melt_switch_on": (2, melt(30, 20, 0.85, lj_rc=2.5, seed=14, n_cell=16, switch_on=True)),
     }
 
 

@@ -323,6 +323,7 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
                                   double* __restrict__ r, int* __restrict__ mic,
                                   const double* __restrict__ r_ref, const int* __restrict__ type,
                                   int* __restrict__ flags, double* __restrict__ part, int energy) {
+  if (flags[F_ERROR] != 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double over = 0.0;
   double vals[MFA_NTERMS];
@@ -362,14 +363,14 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
 
 __global__ void k_cell_assign(KParams k, const double* __restrict__ r, int* __restrict__ cell_of,
                               int* __restrict__ cell_count, int* __restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
   const int cx = cell_coord(r[3 * (size_t)i + 0], k.inv_r_bin[0], k.nb[0]);
   const int cy = cell_coord(r[3 * (size_t)i + 1], k.inv_r_bin[1], k.nb[1]);
   const int cz = cell_coord(r[3 * (size_t)i + 2], k.inv_r_bin[2], k.nb[2]);
   if (cx < 0 || cy < 0 || cz < 0) {  // the reference would index out of bounds
-    atomicMax(&flags[F_ERROR], MFA_ERR_CELL);
+    atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL);
     cell_of[i] = 0;
     atomicAdd(&cell_count[0], 1);
     return;
@@ -487,7 +488,7 @@ __global__ void k_build_list(KParams k, const double* __restrict__ r, const int*
     }
   }
   nn[i] = cnt < cap ? cnt : cap;
-  if (cnt > cap) atomicMax(&flags[F_ERROR], MFA_ERR_LIST_OVERFLOW);
+  if (cnt > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
   atomicMax(&flags[F_MAXNEIGH], cnt);
   for (int d = 0; d < 3; ++d)
     r_ref[3 * (size_t)i + d] = r[3 * (size_t)i + d] + (double)mic[3 * (size_t)i + d] * k.L[d];
@@ -562,6 +563,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
   const int i = gtid / TPP;
   const int sub = gtid % TPP;
   if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  if (flags[F_ERROR] != 0) return;      // state already invalid: stop, the host reports it
   const bool active = i < k.n;
   const int ii = active ? i : 0;
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
@@ -785,7 +787,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
     f[3 * (size_t)i + 0] = fx;
     f[3 * (size_t)i + 1] = fy;
     f[3 * (size_t)i + 2] = fz;
-    if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicMax(&flags[F_ERROR], MFA_ERR_NAN);
+    if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
   }
   if (mask & TM_ENERGY) {
     double vals[MFA_NTERMS];
@@ -1399,6 +1401,18 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
     }
   }
   return check_flags(c);
+}
+
+extern "C" int32_t mfa_stats(mfa_ctx* c, int64_t* out) {
+  if (!c || !out) return MFA_ERR_ARG;
+  HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags, F_NFLAGS * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = c->h_flags[F_MAXNEIGH];
+  out[1] = c->cap;
+  out[2] = c->h_flags[F_ERROR];
+  out[3] = c->h_flags[F_NREBUILD];
+  return MFA_OK;
 }
 
 extern "C" int32_t mfa_timing(mfa_ctx* c, int32_t enable, double* force_ms_avg,

@@ -119,6 +119,7 @@ SIGNATURES = {
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
     "mfa_timing": (C.c_int32, [_P, C.c_int32, _dp, _lp, _dp, _lp]),
+    "mfa_stats": (C.c_int32, [_P, _lp]),
     "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, _ip]),
 }
@@ -303,6 +304,12 @@ class Context:
         obs = Obs()
         self._chk(self.lib.mfa_energies(self.h, step, C.byref(obs)), "mfa_energies")
         return obs.as_dict()
+
+    def stats(self):
+        out = np.zeros(4, dtype=np.int64)
+        self._chk(self.lib.mfa_stats(self.h, out.ctypes.data_as(_lp)), "mfa_stats")
+        return {"max_neigh": int(out[0]), "capacity": int(out[1]), "error": int(out[2]),
+                "rebuilds": int(out[3])}
 
     def timing(self, enable):
         fm, im = C.c_double(), C.c_double()

@@ -143,6 +143,9 @@ class System:
                       np.ascontiguousarray(t, dtype=np.int32),
                       None if mic is None else np.ascontiguousarray(mic, dtype=np.int32)]
         r, v, a, t, m = self._keep
+        n = self.n
+        assert r.shape == (n, 3) and v.shape == (n, 3) and a.shape == (n, 3) and t.shape == (n,)
+        assert m is None or m.shape == (n, 3)
         self.lib.orc_set_state(self.h, _dp(r), _dp(v), _dp(a), _ip(t), _ip(m))
 
     def get_state(self):

@@ -1,0 +1,336 @@
+"""CPU tests: pin the oracle against known answers and mathematics, and check
+the host side of the library (ABI, derived constants).  No GPU needed.
+
+The reference ships no tests, golden vectors or fixtures for this path, so the
+oracle is pinned by:
+  - Random123's published known-answer vectors for Threefry-2x32;
+  - hand-evaluated reference expressions with float32 literals;
+  - brute force (O(N^2) pair criterion) for the cell scan;
+  - closed forms (LJ, FENE, 9-3 wall, free flight, constant force);
+  - energy/force consistency by central finite differences;
+  - invariants (Newton's third law, momentum, pinned heads).
+Each check is built so that a plausible slip (dropped term, wrong sign or
+index, transposed operand, float vs double literal) fails it.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mfa_amd import configs, lib
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+C6 = float(np.float32(2.0) ** np.float32(np.float32(1.0) / np.float32(6.0)))  # 2.**(1./6.) in real(4)
+
+
+# ---------------------------------------------------------------- RNG pins
+
+# Random123 kat_vectors: threefry2x32 R ctr0 ctr1 key0 key1 -> out0 out1
+THREEFRY_KAT = [
+    (13, (0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x9d1c5ec6, 0x8bd50731)),
+    (13, (0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0xfd36d048, 0x2d17272c)),
+    (13, (0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xba3e4725, 0xf27d669e)),
+    (20, (0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x6b200159, 0x99ba4efe)),
+    (20, (0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0x1cb996fc, 0xbb002be7)),
+    (20, (0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xc4923a9c, 0x483df7a0)),
+]
+
+
+@pytest.mark.parametrize("rounds,inp,out", THREEFRY_KAT)
+def test_threefry_known_answers(rounds, inp, out):
+    assert orc.threefry2x32(*inp, rounds=rounds) == out
+
+
+def test_pair_normal_moments_and_symmetry():
+    xs = np.array([orc.pair_normal(1234, 7, i, i + 1 + (i % 13)) for i in range(200000)])
+    assert abs(xs.mean()) < 0.01
+    assert abs(xs.var() - 1.0) < 0.01
+    k = ((xs - xs.mean()) ** 4).mean() / xs.var() ** 2
+    assert abs(k - 3.0) < 0.05
+    # same number from either side of the pair; new numbers per step
+    assert orc.pair_normal(5, 3, 10, 20) == orc.pair_normal(5, 3, 20, 10)
+    assert orc.pair_normal(5, 3, 10, 20) != orc.pair_normal(5, 4, 10, 20)
+
+
+# ---------------------------------------------------------------- constants
+
+def test_derived_constants_float32_literals():
+    inp = configs.brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=0.5, skin=0.0)
+    p = orc.derive(inp)
+    # WCA range from the float32 2**(1/6) (init_params.f90:502), squared in FP64
+    assert C6 == 1.1224620342254639
+    assert p.range_2[1][1] == C6 * C6
+    assert p.range_2[0][0] == 0.0            # head-head (init_params.f90:703-712)
+    assert p.range_2[1][3] == 0.0            # no PARTICLE_4 (init_params.f90:604-607)
+    assert p.skin == float(np.float32(0.4))  # skin = 0.4 (init_params.f90:800)
+    x = (1.0 / (C6 * C6)) ** 3
+    assert abs(p.e_shift[1][1] - x * (x - 1.0)) < 1e-15
+    assert abs(p.e_shift[1][1] + 0.25) < 1e-6  # WCA shift is -1/4 (x^3 ~ 1/2)
+    assert p.epsil[1][1] == 4.0
+    assert p.k_chain == 30.0 * 1.5 ** 2       # k_chain*R0^2 (init_params.f90:398)
+    # DPD cut-off macro literal and r_cut_max (init_params.f90:770-781)
+    assert p.r_cut_dpd == 1.0 and p.r_cut_max == C6
+    assert p.skin_2 == (C6 + p.skin) ** 2 - C6 ** 2
+    # binning grid: int(L/(r_cut_max + 0.4f)) (make_binning_boxes.f90:44-46)
+    assert list(p.n_bin) == [int(6.0 / (C6 + float(np.float32(0.4)))),
+                             int(6.0 / (C6 + float(np.float32(0.4)))),
+                             int(8.0 / (C6 + float(np.float32(0.4))))]
+    assert abs(p.sig - np.sqrt(2 * 1.0 * 4.5 / 0.005)) < 1e-12
+
+
+def test_poor_solvent_and_particle4_ranges():
+    inp = configs.brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=0.5)
+    inp.solvent = 0
+    p = orc.derive(inp)
+    c2 = float(np.float32(2.0) * np.float32(C6))
+    assert p.range_2[1][2] == c2 * c2 and p.range_2[3][1] == 0.0
+    inp.solvent = 1
+    inp.particle_4 = 1
+    inp.inter4 = 2
+    p = orc.derive(inp)
+    assert p.range_2[3][3] == (2.0 * C6) ** 2 and p.range_2[1][3] == C6 * C6
+
+
+@pytest.mark.parametrize("case", ["brush_small", "melt_small", "melt_wca", "melt_switch_on"])
+def test_library_derive_matches_oracle(case):
+    """Host-side derivation of libmfa_hip (mfa_derive) vs the oracle's."""
+    kind, inp = configs.small_cases()[case]
+    p = lib.derive(inp)
+    q = orc.derive(inp)
+    for f in ("range_2", "inv_range_2", "e_shift", "sigma_2", "epsil"):
+        assert np.array_equal(np.array(getattr(p, f)), np.array(getattr(q, f))), f
+    for f in ("r_cut_max", "skin", "skin_2", "sig", "k_chain", "inv_r_chain_2", "dt_2",
+              "r_cut_dpd_2", "friction1", "z_space_wall", "a_w"):
+        assert getattr(p, f) == getattr(q, f), f
+    assert list(p.n_bin) == list(q.n_bin) and list(p.r_bin) == list(q.r_bin)
+    assert list(p.sigma_wall) == list(q.sigma_wall) and list(p.a_wall) == list(q.a_wall)
+    assert p.fso_r2_init == q.r_2_min_init and p.fso_time == q.r_2_min_time
+    assert p.n_part == q.n_part and p.part_init_d == q.part_init_d
+
+
+def test_derive_refuses_grids_that_miss_pairs():
+    inp = configs.brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=1.0)  # int(0.4*6)=2 -> delta_bin 0
+    with pytest.raises(lib.MfaError):
+        lib.derive(inp)
+
+
+# ---------------------------------------------------------------- cell scan
+
+@pytest.mark.parametrize("case", ["brush_small", "melt_small", "melt_wca"])
+def test_oracle_pair_set_equals_brute_force(case):
+    kind, inp = configs.small_cases()[case]
+    p, r, v, t = configs.build(inp, kind)
+    s = orc.System(inp)
+    s.set_state(r, v, np.zeros_like(r), t)
+    s.binning()
+    assert np.array_equal(s.get_pairs(), s.brute_pairs())
+    # cells by the closed form mod(int(r*inv_r_bin), n_bin)
+    cells = s.get_cells()
+    q = s.p
+    exp = np.stack([np.mod((r[:, d] * q.inv_r_bin[d]).astype(np.int64), q.n_bin[d])
+                    for d in range(3)], axis=1)
+    assert np.array_equal(cells, exp)
+
+
+# ---------------------------------------------------------------- closed forms
+
+def _two_body(r12, symmetry=1, types=(3, 3), thermostat=-1, lj_rc=2.5, box=12.0):
+    inp = configs.melt(1, 2, 0.001, lj_rc=lj_rc, thermostat=thermostat, n_cell=16)
+    inp.x_space = inp.y_space = box / 16
+    inp.z_space_wall = box
+    inp.n_mon_d, inp.n_chain_d = 1, 2  # two free beads, no bond
+    s = orc.System(inp)
+    r = np.array([[5.0, 5.0, 5.0], [5.0 + r12, 5.0, 5.0]])
+    z = np.zeros((2, 3))
+    s.set_state(r, z, z, np.array(types, dtype=np.int32))
+    return s
+
+
+@pytest.mark.parametrize("rr", [0.95, 1.0, 1.12, 1.5, 2.2, 2.49])
+def test_lj_closed_form(rr):
+    s = _two_body(rr)
+    s.binning()
+    f, e = s.forces(step=0)
+    fmag = 24.0 * (2.0 / rr ** 13 - 1.0 / rr ** 7)
+    assert abs(f[1, 0] - fmag) < 1e-12 * max(1.0, abs(fmag))
+    assert f[0, 0] == -f[1, 0]
+    shift = 4.0 * (1 / 2.5 ** 12 - 1 / 2.5 ** 6)
+    ev = 4.0 * (1 / rr ** 12 - 1 / rr ** 6) - shift
+    assert abs(e["v_fluid_fluid"] - ev) < 1e-12
+
+
+def test_lj_outside_cutoff_is_zero():
+    s = _two_body(2.51)
+    s.binning()
+    f, e = s.forces(step=0)
+    assert np.all(f == 0.0) and e["v_fluid_fluid"] == 0.0
+
+
+def test_fene_closed_form():
+    inp = configs.melt(1, 2, 0.001, lj_rc=2.5, thermostat=-1, n_cell=16)
+    inp.x_space = inp.y_space = 12.0 / 16
+    inp.z_space_wall = 12.0
+    for t_eps in range(4):
+        for u in range(4):
+            inp.epsil[t_eps][u] = 0.0
+    s = orc.System(inp)
+    rr = 1.1
+    r = np.array([[5.0, 5.0, 5.0], [5.0, 5.0 + rr, 5.0]])
+    z = np.zeros((2, 3))
+    s.set_state(r, z, z, np.array([3, 3], dtype=np.int32))
+    s.binning()
+    f, e = s.forces(step=0)
+    k, R0 = 30.0, 1.5
+    assert abs(f[0, 1] - k * rr / (1 - (rr / R0) ** 2)) < 1e-11   # pulled toward bead 2
+    assert abs(e["v_intra_molec"] + 0.5 * k * R0 ** 2 * np.log(1 - (rr / R0) ** 2)) < 1e-11
+
+
+def test_wall_closed_form_and_zero_force_point():
+    inp = configs.brush(2, 1, 2, 6.0, 6.0, 8.0, x_space=0.5, thermostat=-1)
+    s = orc.System(inp)
+    r = np.array([[1.0, 1.0, 1.0], [2.0, 3.0, 2.0], [4.0, 1.0, 1.3], [3.0, 4.0, 4.0]])
+    z = np.zeros((4, 3))
+    s.set_state(r, z, z, np.array([1, 1, 3, 3], dtype=np.int32))
+    s.zero_force()
+    out = s.fluid_wall()
+    f = s.get_state()["f"]
+    A, sw, L = 2.0, 1.0, 8.0   # |a_w| with a_w = -2; sigma_wall = (1+1)/2
+
+    def fz(zz):
+        return 9 * A * sw ** 9 / zz ** 10 + 3 * A * sw ** 3 / zz ** 4 \
+            - (9 * A * sw ** 9 / (L - zz) ** 10 + 3 * A * sw ** 3 / (L - zz) ** 4)
+
+    def vz(zz):
+        return A * (sw / zz) ** 9 + A * (sw / zz) ** 3 + A * (sw / (L - zz)) ** 9 + A * (sw / (L - zz)) ** 3
+    for i, zz in enumerate(r[:, 2]):
+        assert abs(f[i, 2] - fz(zz)) < 1e-11 * max(1, abs(fz(zz)))
+    assert abs(out[0] - sum(vz(zz) for zz in r[:, 2])) < 1e-10
+    # midplane: forces of the two walls cancel
+    assert abs(fz(4.0)) < 1e-15 and abs(f[3, 2]) < 1e-15
+
+
+def test_energy_force_consistency_fd():
+    """F = -grad V by central differences for LJ + FENE + walls (no DPD)."""
+    kind, inp = configs.small_cases()["brush_small"]
+    inp.thermostat = -1
+    p, r, v, t = configs.build(inp, kind)
+    s = orc.System(inp)
+    s.set_state(r, v, np.zeros_like(r), t)
+    s.binning()
+    f, _ = s.forces(step=0)
+
+    def energy(rr):
+        s2 = orc.System(inp)
+        s2.set_state(rr, v, np.zeros_like(rr), t)
+        s2.binning()
+        _, e = s2.forces(step=0)
+        return e["v_fluid_fluid"] + e["v_intra_molec"] + e["v_fluid_wall"]
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    for i in rng.choice(p.n_part, 12, replace=False):
+        for d in range(3):
+            rp, rm = r.copy(), r.copy()
+            rp[i, d] += h
+            rm[i, d] -= h
+            g = -(energy(rp) - energy(rm)) / (2 * h)
+            assert abs(g - f[i, d]) < 1e-5 * max(1.0, abs(f[i, d])), (i, d, g, f[i, d])
+
+
+def test_newton_third_law_and_dpd_momentum():
+    kind, inp = configs.small_cases()["melt_small"]
+    p, r, v, t = configs.build(inp, kind)
+    s = orc.System(inp)
+    s.set_state(r, v, np.zeros_like(r), t)
+    s.binning()
+    f, _ = s.forces(step=3)
+    frms = np.sqrt((f ** 2).sum(1).mean())
+    assert np.abs(f.sum(0)).max() < 1e-10 * frms * p.n_part
+
+
+def test_verlet_free_flight_and_constant_acceleration():
+    inp = configs.melt(1, 2, 0.001, lj_rc=2.5, thermostat=-1, n_cell=16)
+    inp.x_space = inp.y_space = 12.0 / 16
+    inp.z_space_wall = 12.0
+    inp.n_mon_d, inp.n_chain_d = 1, 2
+    s = orc.System(inp)
+    r = np.array([[1.0, 2.0, 3.0], [7.0, 8.0, 9.0]])
+    v = np.array([[0.5, -0.25, 1.0], [-1.0, 0.0, 0.75]])
+    a = np.array([[0.1, 0.0, -0.2], [0.0, 0.3, 0.0]])
+    s.set_state(r, v, a, np.array([3, 3], dtype=np.int32))
+    dt = s.p.dt
+    s.verlet_positions()
+    st = s.get_state()
+    assert np.allclose(st["r"], r + dt * v + 0.5 * dt * dt * a, atol=1e-15)
+    assert np.allclose(st["v"], v + 0.5 * dt * a, atol=1e-15)
+    # periodic wrap with the image counter (verlet_positions.f90:398-404)
+    s.set_state(np.array([[11.999, 2.0, 0.001], [7.0, 8.0, 9.0]]),
+                np.array([[1.0, 0, -1.0], [0, 0, 0]]), np.zeros((2, 3)),
+                np.array([3, 3], dtype=np.int32))
+    s.verlet_positions()
+    st = s.get_state()
+    assert st["mic"][0].tolist() == [1, 0, -1]
+    assert 0.0 <= st["r"][0, 0] < 0.01 and 11.99 < st["r"][0, 2] <= 12.0
+
+
+def test_check_skin_threshold():
+    """check_skin.f90:139-152: rebuild once a displacement exceeds skin/2."""
+    inp = configs.melt(1, 2, 0.001, lj_rc=2.5, thermostat=-1, n_cell=16)
+    inp.x_space = inp.y_space = 12.0 / 16
+    inp.z_space_wall = 12.0
+    inp.n_mon_d, inp.n_chain_d = 1, 2
+    s = orc.System(inp)
+    r = np.array([[1.0, 2.0, 3.0], [7.0, 8.0, 9.0]])
+    z = np.zeros((2, 3))
+    half = s.p.skin / 2
+    v = np.array([[0.9 * half / s.p.dt, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    s.set_state(r, v, z, np.array([3, 3], dtype=np.int32))
+    s.binning()
+    s.verlet_positions()           # moved 0.9 * skin/2
+    assert s.check_skin() == 0
+    s.verlet_positions()           # moved 1.8 * skin/2
+    assert s.check_skin() == 1
+
+
+def test_pinned_heads():
+    kind, inp = configs.small_cases()["brush_small"]
+    p, r, v, t = configs.build(inp, kind)
+    s = orc.System(inp)
+    s.set_state(r, v, np.zeros_like(r), t)
+    s.run_steps(1, 5)
+    st = s.get_state()
+    heads = np.arange(0, p.n_mon * p.n_chain, p.n_mon)
+    assert np.array_equal(st["r"][heads], r[heads])
+    assert np.all(st["v"][heads] == 0.0) and np.all(st["f"][heads] == 0.0)
+
+
+# ---------------------------------------------------------------- library ABI
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "mfa_hip.h")).read()
+    names = sorted(set(re.findall(r"\b(mfa_[a-z_0-9]+)\s*\(", hdr)))
+    dll = C.CDLL(lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(dll, n)]
+    assert not missing, missing
+    assert set(names) <= set(lib.SIGNATURES), set(names) - set(lib.SIGNATURES)
+
+
+def test_struct_layouts_match_library():
+    assert lib.abi_check()
+
+
+def test_generators_are_seeded_and_valid():
+    kind, inp = configs.small_cases()["melt_small"]
+    p, r1, v1, t1 = configs.build(inp, kind)
+    _, r2, v2, t2 = configs.build(inp, kind)
+    assert np.array_equal(r1, r2) and np.array_equal(v1, v2)
+    L = p.boundary[0]
+    assert r1.min() >= 0 and r1.max() < L
+    d = r1[1:] - r1[:-1]
+    d -= L * np.rint(d / L)
+    bonds = np.linalg.norm(d, axis=1).reshape(-1)
+    chain_bonds = bonds[(np.arange(1, p.n_part) % p.n_mon_d) != 0]
+    assert np.allclose(chain_bonds, 0.97, atol=1e-12)
+    assert np.abs(v1.sum(0)).max() < 1e-9
