@@ -119,7 +119,7 @@ WORKLOADS = {
                     make=lambda: brush(100, 30, 6000, 10.0, 10.0, 20.0, seed=1001)),
     # Kremer-Grest melt, the headline workload
     "melt100k": dict(kind=2, steps=5000, min_dist=0.8,
-                     make=lambda: melt(2000, 50, 0.85, seed=2002, switch_on=True)),
+                     make=lambda: melt(2000, 50, 0.85, seed=2002)),
 }
 
 
@@ -129,8 +129,57 @@ def small_cases():
         "brush_small": (1, brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=0.5, seed=11)),
         "melt_small": (2, melt(30, 20, 0.85, lj_rc=2.5, seed=12, n_cell=16)),
         "melt_wca": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=13, n_cell=16)),
-        "melt_switch_on": (2, melt(30, 20, 0.85, lj_rc=2.5, seed=14, n_cell=16, switch_on=True)),
+        "melt_switch_on": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=14, n_cell=16,
+                                   switch_on=True)),
     }
+
+
+def warmup_inputs(inputs, dt=0.002):
+    """Start-up form of a melt: the same system with the purely repulsive
+    WCA cut-off and FORCE_SWITCH_ON on a fresh start (s_time = 1), at a
+    smaller time step.  The switch-on clamp r >= 1.11 sigma
+    (fluid_fluid.f90:96) would remove the core but keep the attraction of an
+    rc = 2.5 potential and collapse the melt, and at dt = 0.01 the re-expanding
+    bonds heat the system until it blows up; relaxed under WCA at dt = 0.002
+    over the 40 tau ramp, the melt then runs at the production cut-off and
+    time step at T = 1."""
+    w = lib.Inputs.from_buffer_copy(inputs)
+    w.lj_rc = 0.0
+    w.s_time = 1
+    w.force_switch_on = 1
+    w.dt = dt
+    return w
+
+
+def equilibrate(inputs, kind, min_dist=0.8, cool_steps=2000):
+    """Seeded start, then on the GPU (melts only): the switch-on warm-up
+    under WCA, and `cool_steps` of the production potential at dt = 0.002,
+    whose DPD range (rc = 2.5) brings the warm-up's T ~ 5 back to kT before
+    the production time step.  Returns (params, state dict: r, v, a, mic, t)."""
+    p, r, v, t = build(inputs, kind, min_dist=min_dist)
+    a = np.zeros_like(r)
+    mic = np.zeros((p.n_part, 3), dtype=np.int32)
+    if kind == 2:
+        wi = warmup_inputs(inputs)
+        wp = lib.derive(wi)
+        with lib.Context(wp, lib.MFA_MIXED) as c:
+            c.set_state(r, v, a, t)
+            c.run_steps(1, ramp_steps(wp), want_obs=False)
+            st = c.get_state()
+        ci = lib.Inputs.from_buffer_copy(inputs)
+        ci.dt = wi.dt
+        with lib.Context(lib.derive(ci), lib.MFA_MIXED) as c:
+            c.set_state(st["r"], st["v"], st["a"], t, st["mic"])
+            c.run_steps(1, cool_steps, want_obs=False)
+            st = c.get_state()
+        r, v, mic = st["r"], st["v"], st["mic"]
+        a = st["a"]
+    return p, {"r": r, "v": v, "a": a, "mic": mic, "t": t}
+
+
+def ramp_steps(params):
+    """Steps until the switch-on clamp is gone (r_time = step*dt >= fso_time)."""
+    return int(params.fso_time / params.dt) + 1
 
 
 def build(inputs, kind, seed=None, min_dist=0.8):

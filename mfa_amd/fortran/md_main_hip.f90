@@ -1,0 +1,176 @@
+! Driver of the force-and-integrate loop on MI355X: md_main.f90's structure
+! (init, main time loop md_main.f90:68-184, final output) with every hot-path
+! subroutine executed by libmfa_hip through iso_c_binding.
+!
+! Run in a directory holding system_input, mfa_input, hip_options and
+! (optionally) conf_old.  Writes fort.60 every 10 steps
+! (verlet_velocities.f90:206-227), conf_new (+ conf_new.bin) every n_safe
+! steps and at the end, and the wall time.
+module md_hip_state
+  use iso_c_binding
+  use mfa_hip_bind
+  implicit none
+  type(c_ptr) :: ctx = c_null_ptr
+  type(mfa_inputs_t) :: inp
+  type(mfa_params_t) :: par
+  integer :: n_part
+  real(c_double), allocatable :: r0(:,:), v(:,:), a(:,:), force(:,:)
+  integer(c_int32_t), allocatable :: a_type(:), mic_count(:,:)
+  integer(c_int32_t) :: f_skin
+  real(c_double) :: v_fluid_fluid, v_coul, v_fluid_wall, v_intra_molec, t_fluid
+
+contains
+
+  ! Thin wrappers with the reference's names (md_main.f90:73-147).
+  subroutine verlet_positions()
+    call mfa_check(mfa_verlet_positions(ctx), "verlet_positions", ctx)
+  end subroutine
+  subroutine check_skin()
+    call mfa_check(mfa_check_skin(ctx, f_skin), "check_skin", ctx)
+  end subroutine
+  subroutine binning()
+    call mfa_check(mfa_binning(ctx), "binning", ctx)
+    f_skin = 0
+  end subroutine
+  subroutine zero_force()
+    call mfa_check(mfa_zero_force(ctx), "force = 0", ctx)
+  end subroutine
+  subroutine fluid_fluid(step)
+    integer(c_int64_t), intent(in) :: step
+    call mfa_check(mfa_fluid_fluid(ctx, step, v_fluid_fluid, v_coul), "fluid_fluid", ctx)
+  end subroutine
+  subroutine fluid_wall(inter_type)
+    integer, intent(in) :: inter_type
+    call mfa_check(mfa_fluid_wall(ctx, int(inter_type, c_int32_t), v_fluid_wall), "fluid_wall", ctx)
+  end subroutine
+  subroutine intra_molec()
+    call mfa_check(mfa_intra_molec(ctx, v_intra_molec), "intra_molec", ctx)
+  end subroutine
+  subroutine verlet_velocities()
+    call mfa_check(mfa_verlet_velocities(ctx, t_fluid), "verlet_velocities", ctx)
+  end subroutine
+
+  subroutine fetch_state()
+    call mfa_check(mfa_get_state(ctx, r0, v, a, force, mic_count), "get_state", ctx)
+  end subroutine
+
+end module md_hip_state
+
+program md_pb_hip
+  use iso_c_binding
+  use mfa_hip_bind
+  use mfa_inputs_mod
+  use md_hip_state
+  implicit none
+  integer :: i_time, tot_time, i, u60, nchunk
+  integer(c_int64_t) :: step
+  integer(c_int32_t) :: rc
+  logical :: have_conf
+  real(8) :: inv_N, v_total, e_total, vcm(3), mtot
+  integer(8) :: c0, c1, crate
+  type(mfa_obs_t) :: obs
+
+  call system_clock(c0, crate)
+  print '(/a/)', " *** MFA_PROG on MI355X (libmfa_hip) ***"
+  call mfa_check_abi()
+
+  ! ---- initialisation (md_main.f90:51-53)
+  call read_hip_options(inp, "hip_options")
+  call read_system_input(inp)
+  call read_mfa_input(inp)
+  rc = mfa_derive(inp, par)
+  call mfa_check(rc, "init_params (derived constants)", c_null_ptr)
+  n_part = par%n_part
+  allocate(r0(3, n_part), v(3, n_part), a(3, n_part), force(3, n_part))
+  allocate(a_type(n_part), mic_count(3, n_part))
+  mic_count = 0
+  inquire(file="conf_old", exist=have_conf)
+  if (have_conf) then
+    call read_conf_old(n_part, a_type, r0, v, a)
+  else
+    print '(/a/)', " *   'conf_old' file not found. Using the seeded generator ..."
+    rc = mfa_generate(par, merge(1_c_int32_t, 2_c_int32_t, inp%symmetry == 0), inp%iseed, &
+                      1d0, 0.97d0, 0.8d0, r0, v, a_type)
+    call mfa_check(rc, "generate", c_null_ptr)
+    a = 0d0
+  end if
+  ! SYSTEM 2/3 in bulk: remove the centre-of-mass velocity (init_config.f90:336-356)
+  if (inp%symmetry == 1) then
+    vcm = 0d0
+    mtot = 0d0
+    do i = 1, n_part
+      vcm(:) = vcm(:) + par%mass_type(a_type(i)) * v(:, i)
+      mtot = mtot + par%mass_type(a_type(i))
+    end do
+    vcm(:) = vcm(:) / mtot
+    do i = 1, n_part
+      v(1, i) = v(1, i) - vcm(1)
+      v(2, i) = v(2, i) - vcm(2)
+      v(3, i) = v(3, i) - vcm(3)
+    end do
+  end if
+  rc = mfa_create(par, precision, list_capacity, ctx)
+  call mfa_check(rc, "create", c_null_ptr)
+  call mfa_check(mfa_set_state(ctx, r0, v, a, a_type, mic_count), "set_state", ctx)
+  call binning()          ! init_config.f90:251-255
+
+  if (inp%system == 3 .and. inp%symmetry == 0) then
+    inv_N = 1d0 / dble(par%n_mon_tot - par%n_chain)   ! init_params.f90:481-486
+  else
+    inv_N = 1d0 / dble(par%n_mon_tot)
+  end if
+  open(newunit=u60, file="fort.60", status="replace", action="write")
+
+  ! ---- main time loop (md_main.f90:68-184)
+  tot_time = inp%n_relax + inp%n_obser
+  if (trim(run_mode) == "fused") then
+    i_time = 0
+    do while (i_time < tot_time)
+      nchunk = min(10, tot_time - i_time)
+      step = int(inp%s_time + i_time, c_int64_t)
+      call mfa_check(mfa_run_steps(ctx, step, int(nchunk, c_int32_t), obs), "run_steps", ctx)
+      i_time = i_time + nchunk
+      if (nchunk == 10) call write_fort60(i_time, obs%v_fluid_wall, obs%v_fluid_fluid, &
+                                          obs%v_intra_molec, obs%t_kin, obs%v_coul)
+      if (mod(i_time, inp%n_safe) == 0) call store()
+    end do
+  else
+    do i_time = 1, tot_time
+      step = int(inp%s_time + i_time - 1, c_int64_t)
+      call verlet_positions()
+      call check_skin()
+      if (f_skin == 1) call binning()
+      call zero_force()
+      call fluid_fluid(step)
+      if (inp%symmetry == 0) call fluid_wall(2)
+      call intra_molec()
+      call verlet_velocities()
+      if (mod(i_time, 10) == 0) call write_fort60(i_time, v_fluid_wall, v_fluid_fluid, &
+                                                   v_intra_molec, t_fluid, v_coul)
+      if (mod(i_time, inp%n_safe) == 0) call store()
+    end do
+  end if
+  call store()
+  close(u60)
+  rc = mfa_destroy(ctx)
+  call system_clock(c1)
+  print *, ' WALL TIME (s)= ', dble(c1 - c0) / dble(crate)
+
+contains
+
+  ! fort.60 line for SYSTEM 2/3 (verlet_velocities.f90:185-227).
+  subroutine write_fort60(it, vfw, vff, vin, tkin, vc)
+    integer, intent(in) :: it
+    real(c_double), intent(in) :: vfw, vff, vin, tkin, vc
+    v_total = vfw + vff + vin
+    e_total = v_total + tkin
+    v_total = v_total + vc
+    write(u60, '(i7,4g17.5)') it, v_total * inv_N, e_total * inv_N, tkin * inv_N, vc * inv_N
+  end subroutine
+
+  subroutine store()
+    call fetch_state()
+    call write_conf_new(inp, n_part, a_type, r0, v, a, mic_count)
+  end subroutine
+
+end program md_pb_hip
