@@ -39,7 +39,8 @@ enum {
   MFA_ERR_CELL = 4,         /* particle outside the cell grid */
   MFA_ERR_NOMEM = 5,
   MFA_ERR_NAN = 6,          /* non-finite force or position */
-  MFA_ERR_IO = 7
+  MFA_ERR_IO = 7,
+  MFA_ERR_CELL_OVERFLOW = 8 /* more particles in a cell than its capacity */
 };
 
 /* Precision forms of the force pass. */
@@ -173,6 +174,9 @@ int32_t mfa_set_precision(mfa_ctx* ctx, int32_t precision);
  * (1, 2, 4, 8 or 16; default 8).  A tuning knob with no reference
  * counterpart; results are identical for every value in the mixed form. */
 int32_t mfa_set_threads_per_particle(mfa_ctx* ctx, int32_t tpp);
+/* Steps per captured hipGraph block in mfa_run_steps (default 16, 0 = launch
+ * every kernel directly).  No reference counterpart; results are identical. */
+int32_t mfa_set_graph_steps(mfa_ctx* ctx, int32_t steps);
 const char* mfa_last_error(mfa_ctx* ctx);
 
 /* State upload/download, (3,N) layout.  Replaces the conf_old read
@@ -222,6 +226,11 @@ int32_t mfa_energies(mfa_ctx* ctx, int64_t step, mfa_obs* obs);
  * out[1] = list capacity, out[2] = device error code, out[3] = rebuilds in the
  * current run_steps call. */
 int32_t mfa_stats(mfa_ctx* ctx, int64_t* out /* [4] */);
+
+/* Average duration (ms) of the force pass at the current state: `reps`
+ * back-to-back launches captured in one graph, bracketed by two HIP events on
+ * the context's stream.  F is recomputed (same values). */
+int32_t mfa_time_force(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
 /* Kernel timing of the force pass: average duration (ms) over the launches
  * since the last reset, measured with HIP events on the context's stream. */

@@ -8,10 +8,14 @@
 //   mic            int32[N][3]    minimum-image counters (mic_count)
 //   r_ref          double[N][3]   r0_old + mic_old*L at the last rebuild
 //   cell_of        int32[N]       linear cell id (x fastest)
-//   cell_start     int32[C+1]     exclusive scan of cell counts
-//   cell_sorted    int32[N]       particle ids grouped by cell, ascending id
-//                                 inside a cell (the reference's bin_fluid order)
+//   cell_count     int32[C]       particles per cell (zeroed by every integrate
+//                                 kernel, filled by a rebuild)
+//   cell_slots     int32[C][ccap] particle ids of a cell in arrival order; the
+//                                 list build orders each cell by id in LDS (the
+//                                 reference's bin_fluid order)
 //   nl, nn         int32[N][cap], int32[N]   full Verlet list (both directions)
+//   step           int64          current step (RNG key, switch-on ramp); lives
+//                                 on the device so captured graphs replay
 //
 // Everything in this file is compiled with -ffp-contract=off: the FP64 forms
 // reproduce the reference's operation order; the FP32 helpers of the mixed
@@ -58,7 +62,10 @@ struct KParams {
   double va[3];
   double inv_r_bin[3];
   double skin_half_2;
+  double fso_r2_init, fso_time;  // force switch-on ramp (0 = off)
+  float pf_margin;               // bound on |r^2(FP32) - r^2(FP64)| in the list build
   int nb[3];
+  int n_cells, cell_cap;
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
   int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
   uint32_t seed;
@@ -76,8 +83,15 @@ struct mfa_ctx {
   hipStream_t stream;
   double *r, *v, *a, *f, *r_ref;
   int32_t *type, *mic;
-  int32_t *cell_of, *cell_count, *cell_start, *cell_fill, *cell_list, *cell_sorted;
+  int32_t *cell_of, *cell_count, *cell_slots;
+  int cell_cap;
   int32_t *nl, *nn;
+  long long* d_step;    // device step counter
+  // captured step block: graph_steps steps of [rebuild, force, integrate]
+  int graph_steps;
+  hipGraphExec_t gexec;
+  int gexec_precision, gexec_tpp;
+  bool capturing;
   int32_t* flags;       // device flags
   int32_t* h_flags;     // pinned mirror
   double* part;         // per-block partial sums [blocks][MFA_NTERMS]
@@ -169,15 +183,22 @@ __device__ void block_partials(double* vals, double* part_out) {
   }
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ part, int nblocks,
-                                  double* __restrict__ out) {
-  // one block; thread t sums term t over blocks in index order
-  const int t = threadIdx.x;
-  if (t < MFA_NTERMS) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * MFA_NTERMS + t];
-    out[t] = s;
+// Final sums: one block per term; each thread adds a fixed strided subset
+// of the block partials, then a fixed tree combines the threads.
+__global__ __launch_bounds__(MFA_BLOCK) void k_reduce_partials(const double* __restrict__ part,
+                                                               int nblocks,
+                                                               double* __restrict__ out) {
+  __shared__ double sh[MFA_BLOCK];
+  const int t = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += MFA_BLOCK) s += part[(size_t)b * MFA_NTERMS + t];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[t] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -236,10 +257,17 @@ __device__ __forceinline__ void vv_one(const KParams& k, int i, int ti, double* 
   }
 }
 
+// Clears the cell counts for the rebuild that may follow (grid-stride).
+__device__ __forceinline__ void zero_counts(const KParams& k, int* __restrict__ cell_count) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k.n_cells; c += gridDim.x * blockDim.x)
+    cell_count[c] = 0;
+}
+
 __global__ void k_verlet_positions(KParams k, double* __restrict__ r, double* __restrict__ v,
                                    const double* __restrict__ a, int* __restrict__ mic,
                                    const double* __restrict__ r_ref, int* __restrict__ flags,
-                                   int with_skin) {
+                                   int with_skin, int* __restrict__ cell_count) {
+  zero_counts(k, cell_count);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double over = 0.0;
   if (i < k.n) {
@@ -322,9 +350,12 @@ __global__ void k_verlet_velocities(KParams k, double* __restrict__ f, double* _
 __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, double* __restrict__ v,
                                   double* __restrict__ r, int* __restrict__ mic,
                                   const double* __restrict__ r_ref, const int* __restrict__ type,
-                                  int* __restrict__ flags, double* __restrict__ part, int energy) {
+                                  int* __restrict__ flags, double* __restrict__ part, int energy,
+                                  int* __restrict__ cell_count, long long* __restrict__ d_step) {
   if (flags[F_ERROR] != 0) return;
+  zero_counts(k, cell_count);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *d_step += 1;  // next step; read by the next force pass only
   double over = 0.0;
   double vals[MFA_NTERMS];
   for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
@@ -357,12 +388,19 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
 }
 
 // ---------------------------------------------------------------------------
-// Cell list by counting sort (binning.f90:11-50).  Each rebuild kernel returns
-// at once unless the rebuild flag is set, so a step can enqueue the rebuild
-// unconditionally and let the skin test decide on the device.
+// Cell list (binning.f90:11-50) in fixed-capacity cells.  Each rebuild kernel
+// returns at once unless the rebuild flag is set, so a step can enqueue the
+// rebuild unconditionally and let the skin test decide on the device.
 
-__global__ void k_cell_assign(KParams k, const double* __restrict__ r, int* __restrict__ cell_of,
-                              int* __restrict__ cell_count, int* __restrict__ flags) {
+__global__ void k_zero_counts(KParams k, int* __restrict__ cell_count) { zero_counts(k, cell_count); }
+
+__global__ void k_set_step(long long* __restrict__ d_step, long long step) { *d_step = step; }
+
+// Cell of each particle, mod(int(r*inv_r_bin), n_bin) per axis, and a slot in
+// that cell (binning.f90:25-48).
+__global__ void k_cell_fill(KParams k, const double* __restrict__ r, int* __restrict__ cell_of,
+                            int* __restrict__ cell_count, int* __restrict__ cell_slots,
+                            int* __restrict__ flags) {
   if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
@@ -372,127 +410,182 @@ __global__ void k_cell_assign(KParams k, const double* __restrict__ r, int* __re
   if (cx < 0 || cy < 0 || cz < 0) {  // the reference would index out of bounds
     atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL);
     cell_of[i] = 0;
-    atomicAdd(&cell_count[0], 1);
     return;
   }
   const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;
   cell_of[i] = c;
-  atomicAdd(&cell_count[c], 1);
-}
-
-// Exclusive scan of the counts (one block), zeroing counts and fill cursors
-// for the next rebuild.
-__global__ void k_cell_scan(int n_cells, int* __restrict__ cell_count, int* __restrict__ cell_start,
-                            int* __restrict__ cell_fill, const int* __restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  __shared__ int wsum[MFA_BLOCK / 64];
-  __shared__ int carry;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < n_cells; base += MFA_BLOCK) {
-    const int c = base + threadIdx.x;
-    const int x = c < n_cells ? cell_count[c] : 0;
-    int incl = x;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += y;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    int off = carry;
-    for (int w = 0; w < wid; ++w) off += wsum[w];
-    if (c < n_cells) {
-      cell_start[c] = off + incl - x;
-      cell_count[c] = 0;
-      cell_fill[c] = 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == MFA_BLOCK - 1) carry = off + incl;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) cell_start[n_cells] = carry;
-}
-
-__global__ void k_cell_scatter(int n, const int* __restrict__ cell_of,
-                               const int* __restrict__ cell_start, int* __restrict__ cell_fill,
-                               int* __restrict__ cell_list, const int* __restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = cell_of[i];
-  const int slot = atomicAdd(&cell_fill[c], 1);
-  cell_list[cell_start[c] + slot] = i;
-}
-
-// Order each cell by particle id: the rank of an element is the number of
-// smaller ids in its cell.  The result is the reference's bin_fluid order
-// (particles enter a cell in increasing i_part, binning.f90:25-50).
-__global__ void k_cell_sort(int n, const int* __restrict__ cell_of,
-                            const int* __restrict__ cell_start, const int* __restrict__ cell_list,
-                            int* __restrict__ cell_sorted, const int* __restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int i = cell_list[p];
-  const int c = cell_of[i];
-  const int b = cell_start[c], e = cell_start[c + 1];
-  int rank = 0;
-  for (int q = b; q < e; ++q) rank += cell_list[q] < i ? 1 : 0;
-  cell_sorted[b + rank] = i;
+  const int slot = atomicAdd(&cell_count[c], 1);
+  if (slot < k.cell_cap)
+    cell_slots[(size_t)c * k.cell_cap + slot] = i;
+  else
+    atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
 }
 
 // Full Verlet list: every j != i in the 27 cells around i with
-// r^2 < list_2(ti,tj) (binning.f90:64-113 with both directions stored).
-// Cells are visited x outer, y, z inner, each in ascending id, so the list
-// order is deterministic.  Also resets the skin reference
-// (binning.f90:125-127).
-__global__ void k_build_list(KParams k, const double* __restrict__ r, const int* __restrict__ type,
-                             const int* __restrict__ mic, const int* __restrict__ cell_of,
-                             const int* __restrict__ cell_start,
-                             const int* __restrict__ cell_sorted, int* __restrict__ nl,
-                             int* __restrict__ nn, int cap, double* __restrict__ r_ref,
-                             int* __restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.n) return;
-  const double xi = r[3 * (size_t)i + 0], yi = r[3 * (size_t)i + 1], zi = r[3 * (size_t)i + 2];
-  const int ti = type[i] - 1;
-  const int c = cell_of[i];
+// r^2 < list_2(ti,tj) (binning.f90:64-113, both directions stored).
+// One workgroup per cell.  The neighbourhood is staged in LDS one x-plane of
+// 9 cells at a time (cells x outer, y, z inner, each in ascending id) as
+// (id, float x, y, z shifted by the cell's periodic image, type).  A group of
+// LTPP lanes handles one particle and tests LTPP consecutive candidates at a
+// time: an FP32 distance decides when it is farther than its rounding bound
+// from the radius, the FP64 criterion with the reference's image convention
+// decides the rest, and an ordered ballot appends the hits, so each list
+// holds its neighbours in the order of a sequential scan.  Also resets the
+// skin reference (binning.f90:125-127).
+#define LTPP 8
+__global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
+    KParams k, const double* __restrict__ r, const int* __restrict__ type,
+    const int* __restrict__ mic, const int* __restrict__ cell_count,
+    const int* __restrict__ cell_slots, int* __restrict__ nl, int* __restrict__ nn, int cap,
+    double* __restrict__ r_ref, int* __restrict__ flags) {
+  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the grid
+  extern __shared__ unsigned char sh_raw[];
+  float4* cpos = (float4*)sh_raw;             // [9*cell_cap]
+  int* cid = (int*)(cpos + 9 * k.cell_cap);   // [9*cell_cap]
+  int* tmp = cid + 9 * k.cell_cap;            // [9*cell_cap]
+  __shared__ int off[10];
+  __shared__ int cell_base[9];
+  __shared__ float shift[9][3];
+  __shared__ int own[MFA_BLOCK / LTPP];
+  __shared__ int m_own_s;
+  __shared__ int blk_max;
+  const int c = blockIdx.x;
   const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
-  int cnt = 0;
-  int* row = nl + (size_t)i * cap;
-  for (int jx = cx - 1; jx <= cx + 1; ++jx) {
-    const int kx = (jx + k.nb[0]) % k.nb[0];
-    for (int jy = cy - 1; jy <= cy + 1; ++jy) {
-      const int ky = (jy + k.nb[1]) % k.nb[1];
-      for (int jz = cz - 1; jz <= cz + 1; ++jz) {
+  const float Lf[3] = {(float)k.L[0], (float)k.L[1], (float)k.L[2]};
+  const int lane = threadIdx.x & 63;
+  const int grp = threadIdx.x / LTPP, sub = threadIdx.x % LTPP;
+  const int gbase = lane & ~(LTPP - 1);
+  const unsigned long long below = (1ull << sub) - 1ull;
+  // the own cell's particles, ascending id
+  if (threadIdx.x == 0) {
+    blk_max = 0;
+    m_own_s = min(cell_count[c], k.cell_cap);
+  }
+  __syncthreads();
+  const int m_own = m_own_s;
+  int my_max = 0;
+  // passes of MFA_BLOCK/LTPP particles (one lane group each)
+  for (int p0 = 0; p0 < m_own; p0 += MFA_BLOCK / LTPP) {
+    const int np = min(MFA_BLOCK / LTPP, m_own - p0);
+    if (threadIdx.x < m_own) tmp[threadIdx.x] = cell_slots[(size_t)c * k.cell_cap + threadIdx.x];
+    for (int t = threadIdx.x + MFA_BLOCK; t < m_own; t += MFA_BLOCK)
+      tmp[t] = cell_slots[(size_t)c * k.cell_cap + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < m_own; t += MFA_BLOCK) {
+      const int id = tmp[t];
+      int rank = 0;
+      for (int u = 0; u < m_own; ++u) rank += tmp[u] < id ? 1 : 0;
+      if (rank >= p0 && rank < p0 + np) own[rank - p0] = id;
+    }
+    __syncthreads();
+    // particle state of the groups of this pass
+    const bool active = grp < np;
+    int i = active ? own[grp] : 0;
+    const double xi = r[3 * (size_t)i + 0], yi = r[3 * (size_t)i + 1], zi = r[3 * (size_t)i + 2];
+    const float xf = (float)(xi >= k.L[0] ? xi - k.L[0] : xi);
+    const float yf = (float)(yi >= k.L[1] ? yi - k.L[1] : yi);
+    const float zf = (float)(k.symmetry == 1 && zi >= k.L[2] ? zi - k.L[2] : zi);
+    const int ti = type[i] - 1;
+    float l2f[MFA_NTYPE];
+    for (int t = 0; t < MFA_NTYPE; ++t) l2f[t] = (float)k.list_2[ti][t];
+    int* row = nl + (size_t)i * cap;
+    int cnt = 0;
+    for (int sx = 0; sx < 3; ++sx) {
+      __syncthreads();  // the previous plane is no longer read
+      if (threadIdx.x < 9) {
+        const int q = threadIdx.x;
+        const int jx = cx + sx - 1, jy = cy + q / 3 - 1, jz = cz + q % 3 - 1;
+        const int kx = (jx + k.nb[0]) % k.nb[0];
+        const int ky = (jy + k.nb[1]) % k.nb[1];
         const int kz = (jz + k.nb[2]) % k.nb[2];
         const int cc = (kz * k.nb[1] + ky) * k.nb[0] + kx;
-        const int b = cell_start[cc], e = cell_start[cc + 1];
-        for (int q = b; q < e; ++q) {
-          const int j = cell_sorted[q];
-          if (j == i) continue;
-          double dx = xi - r[3 * (size_t)j + 0];
-          double dy = yi - r[3 * (size_t)j + 1];
-          double dz = zi - r[3 * (size_t)j + 2];
-          min_image(k, dx, dy, dz);
-          const double r2 = (dx * dx + dy * dy) + dz * dz;
-          const int tj = type[j] - 1;
-          if (r2 < k.list_2[ti][tj]) {
-            if (cnt < cap) row[cnt] = j;
-            ++cnt;
+        cell_base[q] = cc;
+        off[q + 1] = min(cell_count[cc], k.cell_cap);
+        // periodic image of the neighbour cell as seen from cell c
+        shift[q][0] = jx < 0 ? -Lf[0] : (jx >= k.nb[0] ? Lf[0] : 0.0f);
+        shift[q][1] = jy < 0 ? -Lf[1] : (jy >= k.nb[1] ? Lf[1] : 0.0f);
+        shift[q][2] = k.symmetry == 1 ? (jz < 0 ? -Lf[2] : (jz >= k.nb[2] ? Lf[2] : 0.0f)) : 0.0f;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        off[0] = 0;
+        for (int q = 0; q < 9; ++q) off[q + 1] += off[q];
+      }
+      __syncthreads();
+      const int total = off[9];
+      for (int e = threadIdx.x; e < total; e += MFA_BLOCK) {
+        int q = 0;
+        while (off[q + 1] <= e) ++q;
+        tmp[e] = cell_slots[(size_t)cell_base[q] * k.cell_cap + (e - off[q])];
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < total; e += MFA_BLOCK) {
+        int q = 0;
+        while (off[q + 1] <= e) ++q;
+        const int id = tmp[e];
+        int rank = 0;
+        for (int u = off[q]; u < off[q + 1]; ++u) rank += tmp[u] < id ? 1 : 0;
+        // a coordinate equal to L belongs to cell 0 (mod), i.e. to the image at 0
+        double px = r[3 * (size_t)id + 0], py = r[3 * (size_t)id + 1], pz = r[3 * (size_t)id + 2];
+        if (px >= k.L[0]) px -= k.L[0];
+        if (py >= k.L[1]) py -= k.L[1];
+        if (k.symmetry == 1 && pz >= k.L[2]) pz -= k.L[2];
+        cpos[off[q] + rank] = make_float4((float)px + shift[q][0], (float)py + shift[q][1],
+                                          (float)pz + shift[q][2], (float)type[id]);
+        cid[off[q] + rank] = id;
+      }
+      __syncthreads();
+      if (active) {
+        for (int q0 = 0; q0 < total; q0 += LTPP) {
+          const int q = q0 + sub;
+          bool hit = false;
+          int j = -1;
+          if (q < total) {
+            j = cid[q];
+            const float4 pj = cpos[q];
+            const float dx = xf - pj.x, dy = yf - pj.y, dz = zf - pj.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const int tj = (int)pj.w - 1;
+            const float l2 = tj == 0 ? l2f[0] : (tj == 1 ? l2f[1] : (tj == 2 ? l2f[2] : l2f[3]));
+            if (j != i) {
+              if (d2 < l2 - k.pf_margin) {
+                hit = true;  // inside by more than the FP32 error: the FP64 test agrees
+              } else if (d2 < l2 + k.pf_margin) {
+                // within the FP32 error of the radius: the criterion in FP64
+                // with the reference's image convention (binning.f90:84-105)
+                double ex = xi - r[3 * (size_t)j + 0];
+                double ey = yi - r[3 * (size_t)j + 1];
+                double ez = zi - r[3 * (size_t)j + 2];
+                min_image(k, ex, ey, ez);
+                const double r2 = (ex * ex + ey * ey) + ez * ez;
+                hit = r2 < k.list_2[ti][tj];
+              }
+            }
           }
+          const unsigned long long mask = (__ballot(hit) >> gbase) & ((1ull << LTPP) - 1ull);
+          if (hit) {
+            const int pos = cnt + __popcll(mask & below);
+            if (pos < cap) row[pos] = j;
+          }
+          cnt += __popcll(mask);
         }
       }
     }
+    if (active && sub == 0) {
+      nn[i] = cnt < cap ? cnt : cap;
+      if (cnt > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
+      for (int d = 0; d < 3; ++d)
+        r_ref[3 * (size_t)i + d] = r[3 * (size_t)i + d] + (double)mic[3 * (size_t)i + d] * k.L[d];
+    }
+    if (active) my_max = max(my_max, cnt);
+    __syncthreads();  // own[] and tmp[] are reused by the next pass
   }
-  nn[i] = cnt < cap ? cnt : cap;
-  if (cnt > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
-  atomicMax(&flags[F_MAXNEIGH], cnt);
-  for (int d = 0; d < 3; ++d)
-    r_ref[3 * (size_t)i + d] = r[3 * (size_t)i + d] + (double)mic[3 * (size_t)i + d] * k.L[d];
-  if (i == 0) atomicAdd(&flags[F_NREBUILD], 1);
+  atomicMax(&blk_max, my_max);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (blk_max > 0) atomicMax(&flags[F_MAXNEIGH], blk_max);
+    if (c == 0) atomicAdd(&flags[F_NREBUILD], 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -555,15 +648,23 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
                                                      const int* __restrict__ type,
                                                      const int* __restrict__ nl,
                                                      const int* __restrict__ nn, int cap,
-                                                     double* __restrict__ f, int64_t step,
-                                                     double r2min, int mask,
-                                                     double* __restrict__ part,
+                                                     double* __restrict__ f,
+                                                     const long long* __restrict__ d_step,
+                                                     int mask, double* __restrict__ part,
                                                      int* __restrict__ flags) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gtid / TPP;
   const int sub = gtid % TPP;
   if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   if (flags[F_ERROR] != 0) return;      // state already invalid: stop, the host reports it
+  const int64_t step = *d_step;
+  // force switch-on ramp (fluid_fluid.f90:42-49) with r_time = step*dt
+  // (md_main.f90:69, step = s_time + i_time - 1); r2min = 0 once it is over
+  double r2min = 0.0;
+  if (k.fso_time > 0.0) {
+    const double r_time = (double)step * k.dt;
+    if (r_time < k.fso_time) r2min = (1.0 - r_time / k.fso_time) * k.fso_r2_init;
+  }
   const bool active = i < k.n;
   const int ii = active ? i : 0;
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
@@ -644,8 +745,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
 #pragma clang fp contract(fast)
         const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
         const float r2f = (float)r2;
-        const float inv_r_2 = 1.0f / r2f;
-        const float inv_r = __builtin_sqrtf(inv_r_2);
+        const float inv_r = __builtin_amdgcn_rsqf(r2f);  // v_rsq_f32, ~1 ulp
+        const float inv_r_2 = inv_r * inv_r;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         if (in_lj) {
           const float eps = (float)k.epsil[ti][tj];
@@ -844,6 +945,24 @@ static KParams make_kparams(const mfa_params& p) {
   k.a_w = p.a_w;
   k.z_space_wall = p.z_space_wall;
   k.skin_half_2 = p.skin_half_2;
+  k.fso_r2_init = p.fso_r2_init;
+  k.fso_time = p.fso_time;
+  {
+    // FP32 separation from FP32 positions: the two coordinates, the raw
+    // difference (|d| <= L before the image shift) and the FP32 box edge each
+    // carry at most ulp(L)/2 <= 2 L 2^-24, the shifted result |d| <= r_list
+    // one more rounding.  Four times the resulting bound on r^2, plus the
+    // rounding of the sum of squares and of list_2 to FP32.
+    double lmax = 0.0, l2max = 0.0;
+    for (int d = 0; d < 3; ++d) lmax = p.boundary[d] > lmax ? p.boundary[d] : lmax;
+    for (int t = 0; t < MFA_NTYPE; ++t)
+      for (int u = 0; u < MFA_NTYPE; ++u) l2max = p.list_2[t][u] > l2max ? p.list_2[t][u] : l2max;
+    const double u = ldexp(1.0, -24);
+    const double dcoord = 8.0 * lmax * u + 2.0 * sqrt(l2max) * u;
+    k.pf_margin = (float)(4.0 * (3.0 * 2.0 * sqrt(l2max) * dcoord + 3.0 * dcoord * dcoord) +
+                          8.0 * l2max * u);
+  }
+  k.n_cells = p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
   k.n = p.n_part;
   k.n_mon = p.n_mon > 0 ? p.n_mon : 1;
   k.n_chain = p.n_chain;
@@ -865,56 +984,30 @@ static KParams make_kparams(const mfa_params& p) {
 
 static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
 
-static int check_flags(mfa_ctx* c) {
-  HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags, F_NFLAGS * sizeof(int32_t), hipMemcpyDeviceToHost,
-                           c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const int e = c->h_flags[F_ERROR];
-  if (e == MFA_ERR_LIST_OVERFLOW)
-    return set_err(c, e, "neighbour list overflow: %d neighbours > capacity %d",
-                   c->h_flags[F_MAXNEIGH], c->cap);
-  if (e == MFA_ERR_CELL) return set_err(c, e, "particle outside the cell grid");
-  if (e == MFA_ERR_NAN) return set_err(c, e, "non-finite force");
-  if (e != 0) return set_err(c, e, "device error %d", e);
-  return MFA_OK;
+static size_t list_lds_bytes(int cell_cap) {
+  return (size_t)9 * cell_cap * (sizeof(float4) + 2 * sizeof(int));
 }
 
 static int launch_rebuild(mfa_ctx* c) {
   const int g = grid_of(c->n);
-  hipLaunchKernelGGL(k_cell_assign, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->cell_of,
-                     c->cell_count, c->flags);
-  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->n_cells,
-                     c->cell_count, c->cell_start, c->cell_fill, c->flags);
-  hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->n, c->cell_of,
-                     c->cell_start, c->cell_fill, c->cell_list, c->flags);
-  hipLaunchKernelGGL(k_cell_sort, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->n, c->cell_of,
-                     c->cell_start, c->cell_list, c->cell_sorted, c->flags);
-  hipLaunchKernelGGL(k_build_list, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->type,
-                     c->mic, c->cell_of, c->cell_start, c->cell_sorted, c->nl, c->nn, c->cap,
-                     c->r_ref, c->flags);
+  hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->cell_of,
+                     c->cell_count, c->cell_slots, c->flags);
+  hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c->cell_cap),
+                     c->stream, c->k, c->r, c->type, c->mic, c->cell_count, c->cell_slots, c->nl,
+                     c->nn, c->cap, c->r_ref, c->flags);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
 
-// r_2_min of the force switch-on ramp at `step` (fluid_fluid.f90:44-48),
-// with r_time = step*dt (md_main.f90:69, step = s_time + i_time - 1).
-static double switch_on_r2min(const mfa_params& p, int64_t step) {
-  if (p.fso_time <= 0.0) return 0.0;
-  const double r_time = (double)step * p.dt;
-  return r_time < p.fso_time ? (1.0 - r_time / p.fso_time) * p.fso_r2_init : 0.0;
-}
-
 template <int TPP>
-static void launch_force_t(mfa_ctx* c, int64_t step, int mask) {
-  const int threads = c->n * TPP;
-  const int g = grid_of(threads);
-  const double r2min = switch_on_r2min(c->p, step);
+static void launch_force_t(mfa_ctx* c, int mask) {
+  const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED)
     hipLaunchKernelGGL((k_force<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                       c->type, c->nl, c->nn, c->cap, c->f, step, r2min, mask, c->part, c->flags);
+                       c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part, c->flags);
   else
     hipLaunchKernelGGL((k_force<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                       c->type, c->nl, c->nn, c->cap, c->f, step, r2min, mask, c->part, c->flags);
+                       c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part, c->flags);
 }
 
 static int force_blocks(mfa_ctx* c) { return grid_of(c->n * c->tpp); }
@@ -938,9 +1031,9 @@ static int timed_flush(mfa_ctx* c, std::vector<hipEvent_t>& pool, size_t& used, 
                        int64_t& count) {
   if (used == 0) return MFA_OK;
   HIPCHK(c, hipEventSynchronize(pool[used - 1]));
-  for (size_t k = 0; k < used; k += 2) {
+  for (size_t q = 0; q < used; q += 2) {
     float t = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&t, pool[k], pool[k + 1]));
+    HIPCHK(c, hipEventElapsedTime(&t, pool[q], pool[q + 1]));
     ms += t;
     count += 1;
   }
@@ -948,8 +1041,9 @@ static int timed_flush(mfa_ctx* c, std::vector<hipEvent_t>& pool, size_t& used, 
   return MFA_OK;
 }
 
-static int launch_force(mfa_ctx* c, int64_t step, int mask) {
-  if (c->timing) {
+static int launch_force(mfa_ctx* c, int mask) {
+  const bool timed = c->timing && !c->capturing;
+  if (timed) {
     if (c->ev_force_used + 2 > 4096) {
       int rc = timed_flush(c, c->ev_force, c->ev_force_used, c->force_ms, c->force_n);
       if (rc) return rc;
@@ -958,21 +1052,46 @@ static int launch_force(mfa_ctx* c, int64_t step, int mask) {
     if (rc) return rc;
   }
   switch (c->tpp) {
-    case 1: launch_force_t<1>(c, step, mask); break;
-    case 2: launch_force_t<2>(c, step, mask); break;
-    case 4: launch_force_t<4>(c, step, mask); break;
-    case 8: launch_force_t<8>(c, step, mask); break;
-    case 16: launch_force_t<16>(c, step, mask); break;
+    case 1: launch_force_t<1>(c, mask); break;
+    case 2: launch_force_t<2>(c, mask); break;
+    case 4: launch_force_t<4>(c, mask); break;
+    case 8: launch_force_t<8>(c, mask); break;
+    case 16: launch_force_t<16>(c, mask); break;
     default: return set_err(c, MFA_ERR_ARG, "threads per particle %d unsupported", c->tpp);
   }
   HIPCHK(c, hipGetLastError());
-  if (c->timing) return timed_end(c, c->ev_force, c->ev_force_used);
+  if (timed) return timed_end(c, c->ev_force, c->ev_force_used);
+  return MFA_OK;
+}
+
+static int launch_fused(mfa_ctx* c, int energy) {
+  const bool timed = c->timing && !c->capturing;
+  int rc;
+  if (timed) {
+    if (c->ev_integ_used + 2 > 4096) {
+      rc = timed_flush(c, c->ev_integ, c->ev_integ_used, c->integ_ms, c->integ_n);
+      if (rc) return rc;
+    }
+    rc = timed_begin(c, c->ev_integ, c->ev_integ_used);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_fused_integrate, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
+                     c->f, c->v, c->r, c->mic, c->r_ref, c->type, c->flags, c->part, energy,
+                     c->cell_count, c->d_step);
+  HIPCHK(c, hipGetLastError());
+  if (timed) return timed_end(c, c->ev_integ, c->ev_integ_used);
+  return MFA_OK;
+}
+
+static int set_step(mfa_ctx* c, int64_t step) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, c->d_step, (long long)step);
+  HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
 
 static int reduce_terms(mfa_ctx* c, int nblocks, double* out) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->part, nblocks,
-                     c->red);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(MFA_NTERMS), dim3(MFA_BLOCK), 0, c->stream, c->part,
+                     nblocks, c->red);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->h_red, c->red, MFA_NTERMS * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
@@ -981,24 +1100,84 @@ static int reduce_terms(mfa_ctx* c, int nblocks, double* out) {
   return MFA_OK;
 }
 
-static int set_rebuild_flag(mfa_ctx* c) {
-  const int one = 1;
-  HIPCHK(c, hipMemcpyAsync(c->flags + F_REBUILD, &one, sizeof(int), hipMemcpyHostToDevice,
+static int check_flags(mfa_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags, F_NFLAGS * sizeof(int32_t), hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int e = c->h_flags[F_ERROR];
+  if (e == MFA_ERR_LIST_OVERFLOW)
+    return set_err(c, e, "neighbour list overflow: %d neighbours > capacity %d",
+                   c->h_flags[F_MAXNEIGH], c->cap);
+  if (e == MFA_ERR_CELL_OVERFLOW)
+    return set_err(c, e, "binning cell overflow: more than %d particles in a cell", c->cell_cap);
+  if (e == MFA_ERR_CELL) return set_err(c, e, "particle outside the cell grid");
+  if (e == MFA_ERR_NAN) return set_err(c, e, "non-finite force");
+  if (e != 0) return set_err(c, e, "device error %d", e);
   return MFA_OK;
 }
 
+// A rebuild ordered by the host (init_config's binning, mfa_binning, a new
+// state): clear the counts, raise the flag, rebuild, lower the flag.
 static int ensure_list(mfa_ctx* c) {
   if (!c->need_rebuild) return MFA_OK;
-  int rc = set_rebuild_flag(c);
+  hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream, c->k,
+                     c->cell_count);
+  const int one = 1;
+  HIPCHK(c, hipMemcpyAsync(c->flags + F_REBUILD, &one, sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+  int rc = launch_rebuild(c);
   if (rc) return rc;
-  rc = launch_rebuild(c);
-  if (rc) return rc;
-  // clear the flag (normally done by the force pass)
   HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
   c->need_rebuild = false;
   return check_flags(c);
+}
+
+static void drop_graph(mfa_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+}
+
+// One step after its verlet_positions: rebuild when due, force pass, and
+// verlet_velocities fused with the next step's verlet_positions.
+static int enqueue_middle_step(mfa_ctx* c) {
+  int rc = launch_rebuild(c);
+  if (rc) return rc;
+  rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
+  if (rc) return rc;
+  return launch_fused(c, 0);
+}
+
+// Captures graph_steps middle steps once per (precision, threads per particle).
+static int ensure_graph(mfa_ctx* c) {
+  if (c->gexec && c->gexec_precision == c->precision && c->gexec_tpp == c->tpp) return MFA_OK;
+  drop_graph(c);
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  int rc = MFA_OK;
+  for (int s = 0; s < c->graph_steps && rc == MFA_OK; ++s) rc = enqueue_middle_step(c);
+  c->capturing = false;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(c, MFA_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return set_err(c, MFA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  c->gexec_precision = c->precision;
+  c->gexec_tpp = c->tpp;
+  return MFA_OK;
+}
+
+static KParams make_kparams_ctx(mfa_ctx* c) {
+  KParams k = make_kparams(c->p);
+  k.cell_cap = c->cell_cap;
+  return k;
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,27 +1198,53 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
     return set_err(nullptr, MFA_ERR_HIP, "no HIP device");
   mfa_ctx* c = new mfa_ctx();
   c->p = *p;
-  c->k = make_kparams(*p);
   c->precision = precision;
   c->n = p->n_part;
   c->n_cells = p->n_bin[0] * p->n_bin[1] * p->n_bin[2];
   c->tpp = 8;
   c->need_rebuild = true;
   c->timing = false;
+  c->capturing = false;
+  c->graph_steps = 16;
+  if (const char* e = getenv("MFA_GRAPH_STEPS")) c->graph_steps = atoi(e);  // 0: direct launches
+  if (c->graph_steps < 0 || c->graph_steps > 1024) c->graph_steps = 16;
+  c->gexec = nullptr;
+  const double vol = p->boundary[0] * p->boundary[1] * p->boundary[2];
+  const double dens = (double)p->n_part / vol;
   if (list_capacity <= 0) {
     // expected full-list count at the mean density, with room for fluctuations
-    double vol = p->boundary[0] * p->boundary[1] * p->boundary[2];
     double rl2 = 0.0;
     for (int t = 0; t < MFA_NTYPE; ++t)
       for (int u = 0; u < MFA_NTYPE; ++u) rl2 = p->list_2[t][u] > rl2 ? p->list_2[t][u] : rl2;
     const double rl = sqrt(rl2);
-    const double expect = (double)p->n_part / vol * 4.18879020478639 * rl * rl * rl;
+    const double expect = dens * 4.18879020478639 * rl * rl * rl;
     int cap = (int)(2.0 * expect + 48.0);
-    cap = (cap + 31) / 32 * 32;
-    list_capacity = cap;
+    list_capacity = (cap + 31) / 32 * 32;
   }
   c->cap = list_capacity;
+  {
+    // cell capacity: twice the mean occupancy plus 32 (the reference allows 300)
+    const double occ = dens * p->r_bin[0] * p->r_bin[1] * p->r_bin[2];
+    int cc = (int)(2.0 * occ + 32.0);
+    c->cell_cap = (cc + 31) / 32 * 32;
+  }
+  c->k = make_kparams_ctx(c);
+  {
+    // LDS of the list build: the 27-cell neighbourhood as (float4, id)
+    const size_t lds = list_lds_bytes(c->cell_cap);
+    if (lds > 150 * 1024) {
+      delete c;
+      return set_err(nullptr, MFA_ERR_ARG,
+                     "binning cells too crowded for the list build (capacity %d)", c->cell_cap);
+    }
+    if (hipFuncSetAttribute((const void*)k_build_list, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess) {
+      delete c;
+      return set_err(nullptr, MFA_ERR_HIP, "cannot reserve %zu bytes of LDS", lds);
+    }
+  }
   const size_t n = (size_t)c->n;
+  const size_t ncs = (size_t)c->n_cells * c->cell_cap;
   c->max_blocks = grid_of(c->n * 16) + 1;
 #define ALLOC(ptr, bytes)                                                        \
   do {                                                                           \
@@ -1057,14 +1262,12 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   ALLOC(c->type, n * sizeof(int32_t));
   ALLOC(c->mic, n * 3 * sizeof(int32_t));
   ALLOC(c->cell_of, n * sizeof(int32_t));
-  ALLOC(c->cell_list, n * sizeof(int32_t));
-  ALLOC(c->cell_sorted, n * sizeof(int32_t));
   ALLOC(c->cell_count, (size_t)c->n_cells * sizeof(int32_t));
-  ALLOC(c->cell_fill, (size_t)c->n_cells * sizeof(int32_t));
-  ALLOC(c->cell_start, ((size_t)c->n_cells + 1) * sizeof(int32_t));
+  ALLOC(c->cell_slots, ncs * sizeof(int32_t));
   ALLOC(c->nl, n * (size_t)c->cap * sizeof(int32_t));
   ALLOC(c->nn, n * sizeof(int32_t));
   ALLOC(c->flags, F_NFLAGS * sizeof(int32_t));
+  ALLOC(c->d_step, sizeof(long long));
   ALLOC(c->part, (size_t)c->max_blocks * MFA_NTERMS * sizeof(double));
   ALLOC(c->red, MFA_NTERMS * sizeof(double));
 #undef ALLOC
@@ -1081,8 +1284,8 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   }
   c->ev_force_used = c->ev_integ_used = 0;
   hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
-  hipMemsetAsync(c->cell_fill, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->d_step, 0, sizeof(long long), c->stream);
   hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->f, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->a, 0, n * 3 * sizeof(double), c->stream);
@@ -1098,9 +1301,9 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
 extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   if (!c) return MFA_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->r, c->v, c->a, c->f, c->r_ref, c->type, c->mic, c->cell_of, c->cell_list,
-                  c->cell_sorted, c->cell_count, c->cell_fill, c->cell_start, c->nl, c->nn,
-                  c->flags, c->part, c->red};
+  drop_graph(c);
+  void* ptrs[] = {c->r, c->v, c->a, c->f, c->r_ref, c->type, c->mic, c->cell_of, c->cell_count,
+                  c->cell_slots, c->nl, c->nn, c->flags, c->d_step, c->part, c->red};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->h_flags) hipHostFree(c->h_flags);
@@ -1125,6 +1328,13 @@ extern "C" int32_t mfa_set_threads_per_particle(mfa_ctx* c, int32_t tpp) {
   if (tpp != 1 && tpp != 2 && tpp != 4 && tpp != 8 && tpp != 16)
     return set_err(c, MFA_ERR_ARG, "threads per particle must be 1,2,4,8 or 16");
   c->tpp = tpp;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_set_graph_steps(mfa_ctx* c, int32_t steps) {
+  if (!c || steps < 0 || steps > 1024) return MFA_ERR_ARG;
+  if (steps != c->graph_steps) drop_graph(c);
+  c->graph_steps = steps;
   return MFA_OK;
 }
 
@@ -1167,7 +1377,7 @@ extern "C" int32_t mfa_get_state(mfa_ctx* c, double* r0, double* v, double* a, d
 extern "C" int32_t mfa_verlet_positions(mfa_ctx* c) {
   if (!c) return MFA_ERR_ARG;
   hipLaunchKernelGGL(k_verlet_positions, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0);
+                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0, c->cell_count);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFA_OK;
@@ -1247,8 +1457,9 @@ extern "C" int32_t mfa_zero_force(mfa_ctx* c) {
 static int force_terms(mfa_ctx* c, int64_t step, int terms, double* out_terms) {
   int rc = ensure_list(c);
   if (rc) return rc;
-  int mask = terms | TM_ACCUM | (out_terms ? TM_ENERGY : 0);
-  rc = launch_force(c, step, mask);
+  rc = set_step(c, step);
+  if (rc) return rc;
+  rc = launch_force(c, terms | TM_ACCUM | (out_terms ? TM_ENERGY : 0));
   if (rc) return rc;
   if (out_terms) {
     rc = reduce_terms(c, force_blocks(c), out_terms);
@@ -1326,8 +1537,10 @@ extern "C" int32_t mfa_energies(mfa_ctx* c, int64_t step, mfa_obs* obs) {
   if (!c || !obs) return MFA_ERR_ARG;
   int rc = ensure_list(c);
   if (rc) return rc;
+  rc = set_step(c, step);
+  if (rc) return rc;
   // recompute forces into F (the pass overwrites F), then reduce
-  rc = launch_force(c, step, TM_PAIR | TM_WALL | TM_BOND | TM_ENERGY);
+  rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND | TM_ENERGY);
   if (rc) return rc;
   double t[MFA_NTERMS];
   rc = reduce_terms(c, force_blocks(c), t);
@@ -1352,54 +1565,94 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   int rc = ensure_list(c);
   if (rc) return rc;
   HIPCHK(c, hipMemsetAsync(c->flags + F_NREBUILD, 0, sizeof(int), c->stream));
+  rc = set_step(c, first_step);
+  if (rc) return rc;
   const int g = grid_of(c->n);
-  const int terms = TM_PAIR | TM_WALL | TM_BOND;
   // first half of the first step (verlet_positions + check_skin)
   hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                     c->a, c->mic, c->r_ref, c->flags, 1);
+                     c->a, c->mic, c->r_ref, c->flags, 1, c->cell_count);
   HIPCHK(c, hipGetLastError());
-  for (int s = 0; s < n; ++s) {
-    const int64_t step = first_step + s;
-    const bool last = s == n - 1;
-    rc = launch_rebuild(c);
+  // steps 1 .. n-1: captured blocks of graph_steps steps, then single steps
+  int middle = n - 1;
+  if (c->graph_steps > 0 && !c->timing && middle >= c->graph_steps) {
+    rc = ensure_graph(c);
     if (rc) return rc;
-    rc = launch_force(c, step, terms | ((last && obs) ? TM_ENERGY : 0));
-    if (rc) return rc;
-    if (last) {
-      double t[MFA_NTERMS];
-      if (obs) {
-        rc = reduce_terms(c, force_blocks(c), t);
-        if (rc) return rc;
-      }
-      hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f,
-                         c->v, c->a, c->type, c->part, obs ? 1 : 0);
-      HIPCHK(c, hipGetLastError());
-      if (obs) {
-        double tk[MFA_NTERMS];
-        rc = reduce_terms(c, g, tk);
-        if (rc) return rc;
-        rc = check_flags(c);
-        if (rc) return rc;
-        fill_obs(t, tk[T_TKIN], c->h_flags[F_NREBUILD], obs);
-      }
-    } else {
-      if (c->timing) {
-        if (c->ev_integ_used + 2 > 4096) {
-          rc = timed_flush(c, c->ev_integ, c->ev_integ_used, c->integ_ms, c->integ_n);
-          if (rc) return rc;
-        }
-        rc = timed_begin(c, c->ev_integ, c->ev_integ_used);
-        if (rc) return rc;
-      }
-      hipLaunchKernelGGL(k_fused_integrate, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f,
-                         c->v, c->r, c->mic, c->r_ref, c->type, c->flags, c->part, 0);
-      HIPCHK(c, hipGetLastError());
-      if (c->timing) {
-        rc = timed_end(c, c->ev_integ, c->ev_integ_used);
-        if (rc) return rc;
-      }
+    while (middle >= c->graph_steps) {
+      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      middle -= c->graph_steps;
     }
   }
+  for (; middle > 0; --middle) {
+    rc = enqueue_middle_step(c);
+    if (rc) return rc;
+  }
+  // last step: force pass with the energies, verlet_velocities alone
+  rc = launch_rebuild(c);
+  if (rc) return rc;
+  rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND | (obs ? TM_ENERGY : 0));
+  if (rc) return rc;
+  double t[MFA_NTERMS], tk[MFA_NTERMS];
+  if (obs) {
+    rc = reduce_terms(c, force_blocks(c), t);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,
+                     c->a, c->type, c->part, obs ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  if (obs) {
+    rc = reduce_terms(c, g, tk);
+    if (rc) return rc;
+  }
+  rc = check_flags(c);
+  if (rc) return rc;
+  if (obs) fill_obs(t, tk[T_TKIN], c->h_flags[F_NREBUILD], obs);
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
+  if (!c || !ms_avg || reps < 1 || reps > 4096) return MFA_ERR_ARG;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  hipGraphExec_t ge = nullptr;
+  if (c->graph_steps > 0) {  // back-to-back launches captured in one graph
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
+    c->capturing = false;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_err(c, MFA_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(c, MFA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  for (int pass = 0; pass < 2 && rc == MFA_OK; ++pass) {  // pass 0 warms up
+    if (pass == 1) HIPCHK(c, hipEventRecord(e0, c->stream));
+    if (ge) {
+      HIPCHK(c, hipGraphLaunch(ge, c->stream));
+    } else {
+      for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
+    }
+  }
+  c->timing = timing;
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ge) hipGraphExecDestroy(ge);
+  if (rc) return rc;
+  *ms_avg = (double)ms / (double)reps;
   return check_flags(c);
 }
 

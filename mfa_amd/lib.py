@@ -120,6 +120,8 @@ SIGNATURES = {
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
     "mfa_timing": (C.c_int32, [_P, C.c_int32, _dp, _lp, _dp, _lp]),
     "mfa_stats": (C.c_int32, [_P, _lp]),
+    "mfa_set_graph_steps": (C.c_int32, [_P, C.c_int32]),
+    "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, _ip]),
 }
@@ -223,6 +225,14 @@ class Context:
 
     def set_tpp(self, tpp):
         self._chk(self.lib.mfa_set_threads_per_particle(self.h, tpp), "set_threads_per_particle")
+
+    def set_graph_steps(self, steps):
+        self._chk(self.lib.mfa_set_graph_steps(self.h, steps), "mfa_set_graph_steps")
+
+    def time_force(self, reps=100):
+        ms = C.c_double()
+        self._chk(self.lib.mfa_time_force(self.h, reps, C.byref(ms)), "mfa_time_force")
+        return ms.value
 
     def set_state(self, r, v, a, a_type, mic=None):
         r = np.ascontiguousarray(r, dtype=np.float64)

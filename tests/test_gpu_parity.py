@@ -13,7 +13,7 @@ from tests.util import force_error, gpu_ctx, make_case, oracle_sys, rel
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["brush_small", "melt_small", "melt_wca"]
+CASES = ["brush_small", "melt_small", "melt_wca", "melt_switch_on"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -175,3 +175,51 @@ def test_compat_loop_matches_oracle():
     assert np.abs(st["r"] - st_ref["r"]).max() < 1e-10
     assert rel(tk, ref["t_kin"]) < 1e-10
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [lib.MFA_FP64, lib.MFA_MIXED])
+def test_graph_replays_match_direct_launches(precision):
+    """Captured step blocks give bit-identical results to direct launches; the
+    graph is replayed many times with eager work and synchronisation between
+    the replays, and every chunk is compared."""
+    inputs, p, r, v, t, a = make_case("melt_small")
+    eager = gpu_ctx(p, r, v, a, t, precision=precision)
+    eager.set_graph_steps(0)
+    graph = gpu_ctx(p, r, v, a, t, precision=precision)
+    graph.set_graph_steps(4)
+    step = 1
+    for chunk in range(5):
+        oe = eager.run_steps(step, 13)
+        og = graph.run_steps(step, 13)
+        step += 13
+        se, sg = eager.get_state(), graph.get_state()
+        for k in ("r", "v", "f", "a", "mic"):
+            assert np.array_equal(se[k], sg[k]), (chunk, k)
+        assert oe["n_rebuilds"] == og["n_rebuilds"]
+        assert oe["v_fluid_fluid"] == og["v_fluid_fluid"]
+    eager.close()
+    graph.close()
+
+
+def test_pairs_bit_exact_melt100k():
+    """Full-size pair set (100k beads, 16^3 cells of about 24 and up to ~45
+    particles) against the oracle's binning."""
+    from mfa_amd import configs
+    from oracle import oracle as orc
+    w = configs.WORKLOADS["melt100k"]
+    inputs = w["make"]()
+    p, r, v, t = configs.build(inputs, w["kind"], min_dist=w["min_dist"])
+    a = np.zeros_like(r)
+    print("melt100k pairs", flush=True)
+    ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    ctx.binning()
+    gp = ctx.get_pairs()
+    gp = gp[np.lexsort((gp[:, 1], gp[:, 0]))]
+    cells = ctx.get_cells()
+    ctx.close()
+    s = orc.System(inputs)
+    s.set_state(r, v, a, t)
+    s.binning()
+    assert np.array_equal(cells, s.get_cells())
+    op = s.get_pairs()
+    assert gp.shape == op.shape and np.array_equal(gp, op)
