@@ -32,6 +32,10 @@ HBM_PEAK_GBS = 8000.0
 # v (24) and the type (4), write F (24).
 FORCE_BYTES_PER_PARTICLE = 76
 STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
+# HBM bytes per force pass of the mixed form on this workload, from the
+# FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
+# each scaled by the same counter's reading for a copy of known size.
+FORCE_TRAFFIC_BYTES = 198.959e6 + 2.490e6
 
 
 def log(*a):
@@ -163,7 +167,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": None,
+            "traffic": FORCE_TRAFFIC_BYTES,
             "force_ms_avg": tm_m["force_ms"],
             "force_ms_avg_fp64": tm_d["force_ms"],
         },

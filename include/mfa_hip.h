@@ -232,6 +232,10 @@ int32_t mfa_stats(mfa_ctx* ctx, int64_t* out /* [4] */);
  * the context's stream.  F is recomputed (same values). */
 int32_t mfa_time_force(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
+/* Device-to-device copy of `bytes` (read `bytes`, write `bytes`) timed with
+ * HIP events: the reference point for HBM counters and bandwidth. */
+int32_t mfa_copy_probe(int64_t bytes, double* ms);
+
 /* Kernel timing of the force pass: average duration (ms) over the launches
  * since the last reset, measured with HIP events on the context's stream. */
 int32_t mfa_timing(mfa_ctx* ctx, int32_t enable, double* force_ms_avg, int64_t* force_launches,

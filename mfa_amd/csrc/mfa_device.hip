@@ -396,6 +396,8 @@ __global__ void k_zero_counts(KParams k, int* __restrict__ cell_count) { zero_co
 
 __global__ void k_set_step(long long* __restrict__ d_step, long long step) { *d_step = step; }
 
+__global__ void k_set_flag(int* __restrict__ flags, int idx, int value) { flags[idx] = value; }
+
 // Cell of each particle, mod(int(r*inv_r_bin), n_bin) per axis, and a slot in
 // that cell (binning.f90:25-48).
 __global__ void k_cell_fill(KParams k, const double* __restrict__ r, int* __restrict__ cell_of,
@@ -1122,9 +1124,7 @@ static int ensure_list(mfa_ctx* c) {
   if (!c->need_rebuild) return MFA_OK;
   hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream, c->k,
                      c->cell_count);
-  const int one = 1;
-  HIPCHK(c, hipMemcpyAsync(c->flags + F_REBUILD, &one, sizeof(int), hipMemcpyHostToDevice,
-                           c->stream));
+  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->stream, c->flags, (int)F_REBUILD, 1);
   int rc = launch_rebuild(c);
   if (rc) return rc;
   HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
@@ -1654,6 +1654,35 @@ extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
   if (rc) return rc;
   *ms_avg = (double)ms / (double)reps;
   return check_flags(c);
+}
+
+extern "C" int32_t mfa_copy_probe(int64_t bytes, double* ms) {
+  if (bytes <= 0 || !ms) return set_err(nullptr, MFA_ERR_ARG, "bad copy size");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MFA_OK;
+  if (hipMalloc(&a, (size_t)bytes) != hipSuccess || hipMalloc(&b, (size_t)bytes) != hipSuccess ||
+      hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = set_err(nullptr, MFA_ERR_NOMEM, "copy probe allocation failed");
+  } else if (hipMemsetAsync(a, 1, (size_t)bytes, st) != hipSuccess ||
+             hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+             hipEventRecord(e0, st) != hipSuccess ||
+             hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+             hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+    rc = set_err(nullptr, MFA_ERR_HIP, "copy probe failed");
+  } else {
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    *ms = t;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
 }
 
 extern "C" int32_t mfa_stats(mfa_ctx* c, int64_t* out) {

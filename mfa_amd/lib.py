@@ -122,6 +122,7 @@ SIGNATURES = {
     "mfa_stats": (C.c_int32, [_P, _lp]),
     "mfa_set_graph_steps": (C.c_int32, [_P, C.c_int32]),
     "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
     "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, _ip]),
 }
@@ -156,6 +157,16 @@ def abi_check():
 
 class MfaError(RuntimeError):
     pass
+
+
+def copy_probe(nbytes):
+    """Time a device-to-device copy of nbytes; returns milliseconds."""
+    l = load()
+    ms = C.c_double()
+    rc = l.mfa_copy_probe(nbytes, C.byref(ms))
+    if rc:
+        raise MfaError("mfa_copy_probe: %s" % l.mfa_last_error(None).decode())
+    return ms.value
 
 
 def _ptr(a, ct):

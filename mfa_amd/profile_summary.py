@@ -1,0 +1,79 @@
+"""Summaries of rocprofv3 csv output for the hot kernels.
+
+--trace run_kernel_trace.csv : per-kernel average durations in the timed
+    window of a bench.py run (the launches after the last k_force<*,1> warm-up
+    block); rebuild kernels are split into no-op launches and real rebuilds.
+--pmc run_counter_collection.csv : FETCH_SIZE / WRITE_SIZE per dispatch of the
+    force pass, scaled by the reading of the calibration copy of known size.
+"""
+import argparse
+import csv
+import statistics
+from collections import defaultdict
+
+COPY_BYTES = 256 * 1024 * 1024
+
+
+def trace(path, n_timed):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    names = [r["Kernel_Name"] for r in rows]
+    try:
+        end = next(i for i, n in enumerate(names) if n.startswith("void k_force<8, 0>"))
+    except StopIteration:
+        end = len(rows)
+    idx = [i for i in range(end) if names[i].startswith("void k_force<8, 1>")]
+    seg = rows[idx[-n_timed]:end]
+    d = defaultdict(list)
+    for r in seg:
+        d[r["Kernel_Name"].split("(")[0]].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = []
+    for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
+        busy = [x for x in v if x > 8.0]
+        out.append("%-28s launches %5d  avg %8.2f us  median %8.2f us  (launches > 8 us: %d, avg %.2f us)"
+                   % (k, len(v), sum(v) / len(v), statistics.median(v), len(busy),
+                      sum(busy) / len(busy) if busy else 0.0))
+    t0 = int(seg[0]["Start_Timestamp"])
+    t1 = int(seg[-1]["End_Timestamp"])
+    busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in seg)
+    out.append("window %.2f ms, kernels busy %.2f ms (%.0f%%)" % ((t1 - t0) / 1e6, busy / 1e6,
+                                                                 100.0 * busy / (t1 - t0)))
+    return "\n".join(out)
+
+
+def pmc(path):
+    rows = list(csv.DictReader(open(path)))
+    by = defaultdict(lambda: defaultdict(list))
+    for r in rows:
+        by[r["Counter_Name"]][r["Kernel_Name"].split("(")[0]].append(float(r["Counter_Value"]))
+    out = []
+    for counter, kern in by.items():
+        copy = [v for k, v in kern.items() if "copybuffer" in k.lower()]
+        force = kern.get("void k_force<8, 1>") or kern.get("void k_force<8, 0>") or []
+        if not copy or not force:
+            out.append("%s: calibration copy or force pass not found (%s)" % (counter, list(kern)))
+            continue
+        cal = max(max(v) for v in copy)  # KiB read or written by the 256 MiB copy (2nd dispatch)
+        scale = COPY_BYTES / (cal * 1024.0)
+        f = statistics.median(force)
+        out.append("%s: force pass %.1f KiB/dispatch (median of %d), copy %.1f KiB for %d B "
+                   "-> scale %.4f -> %.3f MB per force pass"
+                   % (counter, f, len(force), cal, COPY_BYTES, scale, f * 1024.0 * scale / 1e6))
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace")
+    ap.add_argument("--timed", type=int, default=2400)
+    ap.add_argument("--pmc")
+    a = ap.parse_args()
+    if a.trace:
+        print(trace(a.trace, a.timed))
+    if a.pmc:
+        print(pmc(a.pmc))
+
+
+if __name__ == "__main__":
+    main()
