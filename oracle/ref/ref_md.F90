@@ -1,0 +1,101 @@
+! Test infrastructure, not product code.  A driver of the reference's own
+! hot-path subroutines (init_system, init_params, init_config, binning,
+! check_skin, verlet_positions, fluid_fluid, fluid_wall, intra_molec,
+! verlet_velocities), linked against objects that build_ref.sh compiles from
+! the reference's Fortran sources into oracle/_ref/.  The tests use it only
+! to pin the C oracle against the real reference; nothing ships it.
+!
+! Run in a directory holding system_input, mfa_input and conf_old:
+!   ref_md <force_step> <n_steps>
+! writes
+!   ref_init.bin   the state, the cell of every particle, the pair list of
+!                  the first binning, and one force pass at r_time =
+!                  force_step*dt (md_main.f90:84-106 without integrating);
+!   ref_final.bin  the state after n_steps of the main loop
+!                  (md_main.f90:68-147, observation and storage left out).
+! Record layout (stream, native endianness): int32 n_part, int32 n_rebuilds,
+! float64 v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, then
+! r0, v, a, force as float64 (3, n_part) and mic_count as int32 (3, n_part);
+! ref_init.bin adds the cells int32 (3, n_part), int32 n_pairs and the pairs
+! int32 (2, n_pairs), each pair as (min, max).
+program ref_md
+#include "control_simulation.h"
+  use commons
+  implicit none
+  character(len=32) :: arg
+  integer :: force_step, n_steps, n_rebuilds, n_pairs, k, j, u
+  integer, allocatable :: cells(:,:)
+
+  call get_command_argument(1, arg)
+  read(arg, *) force_step
+  call get_command_argument(2, arg)
+  read(arg, *) n_steps
+
+  call init_system()
+  call init_params()
+  call init_config()      ! ends with a binning (init_config.f90:251-255)
+
+  ! cells of the binning just made, by binning.f90:27-29's formula
+  allocate(cells(3, n_part))
+  do k = 1, n_part
+    cells(1, k) = mod(int(r0(1, k) * inv_r_bin_x), n_bin_x)
+    cells(2, k) = mod(int(r0(2, k) * inv_r_bin_y), n_bin_y)
+    cells(3, k) = mod(int(r0(3, k) * inv_r_bin_z), n_bin_z)
+  end do
+  n_pairs = 0
+  do k = 1, n_part
+    n_pairs = n_pairs + ff_list(0, k)
+  end do
+
+  i_time = 0
+  r_time = dble(force_step) * dt
+  force(:,:) = 0.0
+  call fluid_fluid()
+#if SYMMETRY == 0
+  call fluid_wall(wall_flag)
+#endif
+  call intra_molec()
+  open(newunit=u, file="ref_init.bin", access="stream", form="unformatted", status="replace")
+  call put_state(u, 0)
+  write(u) cells
+  write(u) n_pairs
+  do k = 1, n_part
+    do j = 1, ff_list(0, k)
+      write(u) min(k, ff_list(j, k)), max(k, ff_list(j, k))
+    end do
+  end do
+  close(u)
+
+  ! main loop, md_main.f90:68-147
+  n_rebuilds = 0
+  do i_time = 1, n_steps
+    r_time = dble(s_time + i_time - 1) * dt
+    call verlet_positions()
+    call check_skin
+    if (f_skin .eq. 1) then
+      call binning
+      n_rebuilds = n_rebuilds + 1
+    end if
+    force(:,:) = 0.0
+    call fluid_fluid()
+#if SYMMETRY == 0
+    call fluid_wall(wall_flag)
+#endif
+    call intra_molec
+    call verlet_velocities()
+  end do
+  open(newunit=u, file="ref_final.bin", access="stream", form="unformatted", status="replace")
+  call put_state(u, n_rebuilds)
+  close(u)
+
+contains
+
+  subroutine put_state(unit, nreb)
+    integer, intent(in) :: unit, nreb
+    write(unit) n_part, nreb
+    write(unit) v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid
+    write(unit) r0(:, 1:n_part), v(:, 1:n_part), a(:, 1:n_part), force(:, 1:n_part)
+    write(unit) mic_count(:, 1:n_part)
+  end subroutine put_state
+
+end program ref_md

@@ -10,26 +10,11 @@ import pytest
 
 from mfa_amd import configs, inputs as io
 from oracle import oracle as orc
+from tests.util import remove_vcm
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROG = os.path.join(ROOT, "mfa_amd", "mfa_hip_prog")
-
-
-def _remove_vcm(r, v, t, mass_type):
-    # init_config.f90:336-356, same summation order as the driver
-    vc = [0.0, 0.0, 0.0]
-    mt = 0.0
-    for i in range(len(t)):
-        m = mass_type[t[i] - 1]
-        for d in range(3):
-            vc[d] = vc[d] + m * v[i, d]
-        mt = mt + m
-    vc = [x / mt for x in vc]
-    v = v.copy()
-    for d in range(3):
-        v[:, d] = v[:, d] - vc[d]
-    return v
 
 
 @pytest.mark.parametrize("case,mode,nsteps", [("brush_small", "compat", 30),
@@ -49,7 +34,7 @@ def test_fortran_driver_matches_oracle(tmp_path, case, mode, nsteps):
     # oracle from the same text
     t2, r2, v2, a2 = io.read_conf(os.path.join(d, "conf_old"), p.n_part)
     if inp.symmetry == 1:
-        v2 = _remove_vcm(r2, v2, t2, list(inp.mass_type))
+        v2 = remove_vcm(v2, t2, list(inp.mass_type))
     s = orc.System(inp)
     s.set_state(r2, v2, a2, t2)
     ref = s.run_steps(inp.s_time, nsteps)

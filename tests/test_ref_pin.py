@@ -1,0 +1,135 @@
+"""The C oracle against the reference itself: the reference's Fortran hot
+path, compiled from its own sources into oracle/_ref/ by
+oracle/ref/build_ref.sh, run from the same system_input / mfa_input /
+conf_old text.  Skipped where those programs are not built (they need the
+reference's sources at build time).
+
+Inputs keep the reference's own constants (list skin 0.4, the WCA cut-off
+its menu offers) and set the temperature in mfa_input to 0: the random DPD
+force is then sig * w * g with sig = sqrt(2 T gamma / dt) = 0
+(init_params.f90:845, dpd_forces.f90:58), so the reference's ziggurat stream
+drops out while its dissipative DPD force stays in.
+
+Bars: cells and the pair set bit-exact; forces and energies of one force
+pass within 1e-12; after 20 (brush) / 30 (melt) steps positions within
+1e-10, velocities within 1e-9, the same wrap counts and rebuild count.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mfa_amd import configs, inputs as io
+from oracle import oracle as orc
+from tests.util import force_error, rel, remove_vcm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+CASES = {
+    # case -> (small case, program, force step, steps)
+    "brush": ("brush_small", "ref_md_brush", 3, 20),
+    "melt": ("melt_wca", "ref_md_melt", 3, 30),
+    "melt_switch_on": ("melt_switch_on", "ref_md_melt_fso", 3, 30),
+}
+
+
+def _read(path, n, init):
+    raw = open(path, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a
+
+    n_part, nreb = take(np.int32, 2)
+    assert n_part == n
+    e = take(np.float64, 4)
+    out = {"n_rebuilds": int(nreb), "v_fluid_fluid": e[0], "v_fluid_wall": e[1],
+           "v_intra_molec": e[2], "t_kin": e[3]}
+    for k in ("r", "v", "a", "f"):
+        out[k] = take(np.float64, 3 * n).reshape(n, 3).copy()
+    out["mic"] = take(np.int32, 3 * n).reshape(n, 3).copy()
+    if init:
+        out["cells"] = take(np.int32, 3 * n).reshape(n, 3).copy()
+        m = int(take(np.int32, 1)[0])
+        pr = take(np.int32, 2 * m).reshape(m, 2)
+        out["pairs"] = pr[np.lexsort((pr[:, 1], pr[:, 0]))]
+    return out
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def run(request, tmp_path_factory):
+    case, prog, fstep, nsteps = CASES[request.param]
+    exe = os.path.join(REF, prog)
+    if not os.path.exists(exe):
+        pytest.skip("reference build absent (oracle/ref/build_ref.sh needs the reference sources)")
+    kind, inp = configs.small_cases()[case]
+    inp.skin = 0.0      # the reference's hard-coded 0.4 (init_params.f90:800)
+    inp.temp = 0.0
+    p, r, v, t = configs.build(inp, kind)
+    d = str(tmp_path_factory.mktemp(request.param))
+    io.write_run_dir(d, inp, t, r, v, np.zeros_like(r))
+    res = subprocess.run([exe, str(fstep), str(nsteps)], cwd=d, capture_output=True,
+                         text=True, timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    # the oracle from the same conf_old text
+    t2, r2, v2, a2 = io.read_conf(os.path.join(d, "conf_old"), p.n_part)
+    if inp.symmetry == 1:
+        v2 = remove_vcm(v2, t2, list(inp.mass_type))
+    init = _read(os.path.join(d, "ref_init.bin"), p.n_part, True)
+    final = _read(os.path.join(d, "ref_final.bin"), p.n_part, False)
+    return dict(inp=inp, p=p, state=(r2, v2, a2, t2), fstep=fstep, nsteps=nsteps,
+                init=init, final=final)
+
+
+def _system(run):
+    r, v, a, t = run["state"]
+    s = orc.System(run["inp"])
+    s.set_state(r, v, a, t)
+    return s
+
+
+def test_reference_reads_the_same_state(run):
+    r, v, a, t = run["state"]
+    assert np.array_equal(run["init"]["r"], r)
+    assert np.array_equal(run["init"]["v"], v)
+
+
+def test_cells_and_pairs_match_reference(run):
+    s = _system(run)
+    s.binning()
+    assert np.array_equal(s.get_cells(), run["init"]["cells"])
+    op = s.get_pairs()
+    rp = run["init"]["pairs"]
+    assert op.shape == rp.shape and np.array_equal(op, rp)
+
+
+def test_force_pass_matches_reference(run):
+    s = _system(run)
+    s.binning()
+    f, e = s.forces(step=run["fstep"])
+    ref = run["init"]
+    per, rms = force_error(f, ref["f"])
+    assert per < 1e-12, (per, rms)
+    for k in ("v_fluid_fluid", "v_intra_molec"):
+        assert rel(e[k], ref[k]) < 1e-12, (k, e[k], ref[k])
+    if run["p"].symmetry == 0:
+        assert rel(e["v_fluid_wall"], ref["v_fluid_wall"]) < 1e-12
+
+
+def test_trajectory_matches_reference(run):
+    s = _system(run)
+    s.binning()
+    obs = s.run_steps(first_step=run["inp"].s_time, n=run["nsteps"])
+    st = s.get_state()
+    ref = run["final"]
+    assert np.array_equal(st["mic"], ref["mic"])
+    assert np.abs(st["r"] - ref["r"]).max() < 1e-10
+    assert np.abs(st["v"] - ref["v"]).max() < 1e-9
+    assert obs["n_rebuilds"] == ref["n_rebuilds"]
+    assert rel(obs["t_kin"], ref["t_kin"]) < 1e-10
+    assert rel(obs["v_fluid_fluid"], ref["v_fluid_fluid"]) < 1e-10

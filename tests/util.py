@@ -40,3 +40,20 @@ def force_error(f, ref):
 
 def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
+
+
+def remove_vcm(v, t, mass_type):
+    """Centre-of-mass velocity removal of SYSTEM 2/3 bulk runs
+    (init_config.f90:336-356), in the reference's summation order."""
+    vc = [0.0, 0.0, 0.0]
+    mt = 0.0
+    for i in range(len(t)):
+        m = mass_type[t[i] - 1]
+        for d in range(3):
+            vc[d] = vc[d] + m * v[i, d]
+        mt = mt + m
+    vc = [x / mt for x in vc]
+    v = v.copy()
+    for d in range(3):
+        v[:, d] = v[:, d] - vc[d]
+    return v
