@@ -34,8 +34,9 @@ FORCE_BYTES_PER_PARTICLE = 76
 STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
 # HBM bytes per force pass of the mixed form on this workload, from the
 # FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
-# each scaled by the same counter's reading for a copy of known size.
-FORCE_TRAFFIC_BYTES = 198.959e6 + 2.490e6
+# each scaled by the same counter's reading for a copy of known size; None
+# until the counters have been read for the current kernel.
+FORCE_TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -162,7 +163,7 @@ def main():
         "step_roofline_frac": value * STEP_BYTES_PER_PARTICLE / (HBM_PEAK_GBS * 1e9),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_force<8,1>",
+            "kernel": "k_force_mixed<8,0>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -13,7 +13,14 @@
 //   cell_slots     int32[C][ccap] particle ids of a cell in arrival order; the
 //                                 list build orders each cell by id in LDS (the
 //                                 reference's bin_fluid order)
-//   nl, nn         int32[N][cap], int32[N]   full Verlet list (both directions)
+//   nl, nn         uint32[N][cap], int32[N]  full Verlet list (both directions);
+//                                 an entry is j | code << 27, code (0..26) the
+//                                 offset (dx,dy,dz) in {-1,0,1}^3 of j's cell from
+//                                 i's cell at the rebuild, 9(dx+1)+3(dy+1)+(dz+1)
+//   pos4, vel4     float4[N]      FP32 view of the mixed form: position relative
+//                                 to the origin of the particle's cell at the
+//                                 last rebuild (.w = type-1 as int bits), and
+//                                 velocity; rewritten whenever r or v change
 //   step           int64          current step (RNG key, switch-on ramp); lives
 //                                 on the device so captured graphs replay
 //
@@ -44,10 +51,13 @@ enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16 };
 // device flag slots
 enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_NFLAGS = 8 };
 
-// Fixed-point scale of the mixed form: 2^32 (force resolution 2.3e-10,
-// range +-2.1e9 per particle).
-#define MFA_FIX_SCALE 4294967296.0f
-#define MFA_FIX_INV (1.0 / 4294967296.0)
+// Fixed-point scale of the mixed form: 2^22 per unit force (resolution
+// 2.4e-7; a pair term converts through int32 up to |F| < 512, through int64
+// beyond; both truncate, so the two routes agree).
+#define MFA_FIX_SCALE 4194304.0f
+#define MFA_FIX_INV (1.0 / 4194304.0)
+#define MFA_JBITS 27
+#define MFA_JMASK ((1u << MFA_JBITS) - 1u)
 
 struct KParams {
   double L[3], invL[3];
@@ -60,10 +70,11 @@ struct KParams {
   double k_chain, inv_r_chain_2;
   double a_w, sigma_wall[MFA_NTYPE], a_wall[MFA_NTYPE], z_space_wall;
   double va[3];
-  double inv_r_bin[3];
+  double inv_r_bin[3], r_bin[3];
   double skin_half_2;
   double fso_r2_init, fso_time;  // force switch-on ramp (0 = off)
   float pf_margin;               // bound on |r^2(FP32) - r^2(FP64)| in the list build
+  float ff_margin;               // the same for the cell-relative FP32 view of the force pass
   int nb[3];
   int n_cells, cell_cap;
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
@@ -85,7 +96,9 @@ struct mfa_ctx {
   int32_t *type, *mic;
   int32_t *cell_of, *cell_count, *cell_slots;
   int cell_cap;
-  int32_t *nl, *nn;
+  unsigned* nl;
+  int32_t* nn;
+  float4 *pos4, *vel4;  // FP32 view of the mixed form
   long long* d_step;    // device step counter
   // captured step block: graph_steps steps of [rebuild, force, integrate]
   int graph_steps;
@@ -263,10 +276,29 @@ __device__ __forceinline__ void zero_counts(const KParams& k, int* __restrict__ 
     cell_count[c] = 0;
 }
 
+// FP32 view of particle i: position relative to the origin of `cell` (its
+// cell at the last rebuild), wrapped back across periodic faces it crossed
+// since, and velocity.
+__device__ __forceinline__ void write_view(const KParams& k, int i, int cell, int ti,
+                                           const double* ri, const double* vi,
+                                           float4* __restrict__ pos4, float4* __restrict__ vel4) {
+  const int cx = cell % k.nb[0], cy = (cell / k.nb[0]) % k.nb[1], cz = cell / (k.nb[0] * k.nb[1]);
+  double lx = ri[0] - (double)cx * k.r_bin[0];
+  double ly = ri[1] - (double)cy * k.r_bin[1];
+  double lz = ri[2] - (double)cz * k.r_bin[2];
+  lx -= k.L[0] * rint(lx * k.invL[0]);
+  ly -= k.L[1] * rint(ly * k.invL[1]);
+  if (k.symmetry == 1) lz -= k.L[2] * rint(lz * k.invL[2]);
+  pos4[i] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float(ti));
+  vel4[i] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
+}
+
 __global__ void k_verlet_positions(KParams k, double* __restrict__ r, double* __restrict__ v,
                                    const double* __restrict__ a, int* __restrict__ mic,
                                    const double* __restrict__ r_ref, int* __restrict__ flags,
-                                   int with_skin, int* __restrict__ cell_count) {
+                                   int with_skin, int* __restrict__ cell_count,
+                                   const int* __restrict__ type, const int* __restrict__ cell_of,
+                                   float4* __restrict__ pos4, float4* __restrict__ vel4) {
   zero_counts(k, cell_count);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double over = 0.0;
@@ -287,6 +319,7 @@ __global__ void k_verlet_positions(KParams k, double* __restrict__ r, double* __
     }
     if (wrapped)
       for (int d = 0; d < 3; ++d) mic[3 * (size_t)i + d] = mi[d];
+    write_view(k, i, cell_of[i], type[i] - 1, ri, vi, pos4, vel4);
     if (with_skin) {
       double rr[3];
       for (int d = 0; d < 3; ++d) rr[d] = r_ref[3 * (size_t)i + d];
@@ -322,7 +355,8 @@ __global__ void k_check_skin(KParams k, const double* __restrict__ r,
 // partials when `energy` (verlet_velocities.f90:155-177).
 __global__ void k_verlet_velocities(KParams k, double* __restrict__ f, double* __restrict__ v,
                                     double* __restrict__ a, const int* __restrict__ type,
-                                    double* __restrict__ part, int energy) {
+                                    double* __restrict__ part, int energy,
+                                    float4* __restrict__ vel4) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double vals[MFA_NTERMS];
   for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
@@ -338,6 +372,7 @@ __global__ void k_verlet_velocities(KParams k, double* __restrict__ f, double* _
       v[3 * (size_t)i + d] = vi[d];
       a[3 * (size_t)i + d] = ai[d];
     }
+    vel4[i] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
     if (is_head(k, i))
       for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] = 0.0;
     vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
@@ -351,7 +386,9 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
                                   double* __restrict__ r, int* __restrict__ mic,
                                   const double* __restrict__ r_ref, const int* __restrict__ type,
                                   int* __restrict__ flags, double* __restrict__ part, int energy,
-                                  int* __restrict__ cell_count, long long* __restrict__ d_step) {
+                                  int* __restrict__ cell_count, long long* __restrict__ d_step,
+                                  const int* __restrict__ cell_of, float4* __restrict__ pos4,
+                                  float4* __restrict__ vel4) {
   if (flags[F_ERROR] != 0) return;
   zero_counts(k, cell_count);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,6 +417,7 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
     }
     if (wrapped)
       for (int d = 0; d < 3; ++d) mic[3 * (size_t)i + d] = mi[d];
+    write_view(k, i, cell_of[i], ti, ri, vi, pos4, vel4);
     over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
   }
   over = wave_max(over);
@@ -432,14 +470,19 @@ __global__ void k_cell_fill(KParams k, const double* __restrict__ r, int* __rest
 // time: an FP32 distance decides when it is farther than its rounding bound
 // from the radius, the FP64 criterion with the reference's image convention
 // decides the rest, and an ordered ballot appends the hits, so each list
-// holds its neighbours in the order of a sequential scan.  Also resets the
-// skin reference (binning.f90:125-127).
+// holds its neighbours in the order of a sequential scan.  Each entry carries
+// the offset code of the neighbour's cell.  With walls (no periodicity in z)
+// the cells beyond the z faces hold no neighbours (their particles are more
+// than a cell width away without the image) and are not staged.  Also resets
+// the skin reference (binning.f90:125-127) and writes the FP32 view of the
+// cell's own particles relative to this cell.
 #define LTPP 8
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
-    KParams k, const double* __restrict__ r, const int* __restrict__ type,
-    const int* __restrict__ mic, const int* __restrict__ cell_count,
-    const int* __restrict__ cell_slots, int* __restrict__ nl, int* __restrict__ nn, int cap,
-    double* __restrict__ r_ref, int* __restrict__ flags) {
+    KParams k, const double* __restrict__ r, const double* __restrict__ v,
+    const int* __restrict__ type, const int* __restrict__ mic,
+    const int* __restrict__ cell_count, const int* __restrict__ cell_slots,
+    unsigned* __restrict__ nl, int* __restrict__ nn, int cap, double* __restrict__ r_ref,
+    int* __restrict__ flags, float4* __restrict__ pos4, float4* __restrict__ vel4) {
   if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the grid
   extern __shared__ unsigned char sh_raw[];
   float4* cpos = (float4*)sh_raw;             // [9*cell_cap]
@@ -490,7 +533,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const int ti = type[i] - 1;
     float l2f[MFA_NTYPE];
     for (int t = 0; t < MFA_NTYPE; ++t) l2f[t] = (float)k.list_2[ti][t];
-    int* row = nl + (size_t)i * cap;
+    unsigned* row = nl + (size_t)i * cap;
     int cnt = 0;
     for (int sx = 0; sx < 3; ++sx) {
       __syncthreads();  // the previous plane is no longer read
@@ -502,7 +545,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
         const int kz = (jz + k.nb[2]) % k.nb[2];
         const int cc = (kz * k.nb[1] + ky) * k.nb[0] + kx;
         cell_base[q] = cc;
-        off[q + 1] = min(cell_count[cc], k.cell_cap);
+        off[q + 1] = (k.symmetry == 0 && (jz < 0 || jz >= k.nb[2])) ? 0
+                                                                    : min(cell_count[cc], k.cell_cap);
         // periodic image of the neighbour cell as seen from cell c
         shift[q][0] = jx < 0 ? -Lf[0] : (jx >= k.nb[0] ? Lf[0] : 0.0f);
         shift[q][1] = jy < 0 ? -Lf[1] : (jy >= k.nb[1] ? Lf[1] : 0.0f);
@@ -534,7 +578,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
         if (k.symmetry == 1 && pz >= k.L[2]) pz -= k.L[2];
         cpos[off[q] + rank] = make_float4((float)px + shift[q][0], (float)py + shift[q][1],
                                           (float)pz + shift[q][2], (float)type[id]);
-        cid[off[q] + rank] = id;
+        cid[off[q] + rank] = id | (q << MFA_JBITS);
       }
       __syncthreads();
       if (active) {
@@ -542,8 +586,11 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
           const int q = q0 + sub;
           bool hit = false;
           int j = -1;
+          unsigned entry = 0u;
           if (q < total) {
-            j = cid[q];
+            const unsigned tagged = (unsigned)cid[q];
+            j = (int)(tagged & MFA_JMASK);
+            entry = (unsigned)j | ((unsigned)(9 * sx + (int)(tagged >> MFA_JBITS)) << MFA_JBITS);
             const float4 pj = cpos[q];
             const float dx = xf - pj.x, dy = yf - pj.y, dz = zf - pj.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
@@ -567,7 +614,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
           const unsigned long long mask = (__ballot(hit) >> gbase) & ((1ull << LTPP) - 1ull);
           if (hit) {
             const int pos = cnt + __popcll(mask & below);
-            if (pos < cap) row[pos] = j;
+            if (pos < cap) row[pos] = entry;
           }
           cnt += __popcll(mask);
         }
@@ -576,8 +623,11 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     if (active && sub == 0) {
       nn[i] = cnt < cap ? cnt : cap;
       if (cnt > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
+      const double ri[3] = {xi, yi, zi};
+      const double vi[3] = {v[3 * (size_t)i + 0], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]};
       for (int d = 0; d < 3; ++d)
-        r_ref[3 * (size_t)i + d] = r[3 * (size_t)i + d] + (double)mic[3 * (size_t)i + d] * k.L[d];
+        r_ref[3 * (size_t)i + d] = ri[d] + (double)mic[3 * (size_t)i + d] * k.L[d];
+      write_view(k, i, c, ti, ri, vi, pos4, vel4);
     }
     if (active) my_max = max(my_max, cnt);
     __syncthreads();  // own[] and tmp[] are reused by the next pass
@@ -596,23 +646,39 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
 // so the result is independent of timing.  Everything acting on i is summed
 // here (pairs, walls, bonds), so no atomics are needed and F is written once.
 //
-// Plain FP64: the reference's arithmetic for LJ (fluid_fluid.f90:94-126), DPD
-// (dpd_forces.f90:273-348), 9-3 walls (fluid_wall.f90:81-121) and FENE
-// (intra_molec.f90:16-139), summed in FP64.
-// Mixed: separations and cut-off tests in FP64 (so the interacting set is the
-// FP64 one), the per-pair force in FP32, summed as int64 fixed point.
-
-struct Acc64 {
-  double x, y, z;
-  __device__ void zero() { x = y = z = 0.0; }
-};
-struct AccFix {
-  long long x, y, z;
-  __device__ void zero() { x = y = z = 0; }
-};
+// Plain FP64 (k_force64): the reference's arithmetic for LJ
+// (fluid_fluid.f90:94-126), DPD (dpd_forces.f90:273-348), 9-3 walls
+// (fluid_wall.f90:81-121) and FENE (intra_molec.f90:16-139), summed in FP64.
+//
+// Mixed (k_force_mixed): separations from the FP32 cell-relative view plus
+// the FP32 offset of the neighbour's cell (no periodic arithmetic per pair);
+// a separation whose r^2 lies within the view's error bound of a cut-off is
+// decided again in FP64 with the reference's image convention, so the
+// interacting set is the FP64 one.  Pair math in FP32; each pair term enters
+// an int64 fixed-point sum.
 
 __device__ __forceinline__ long long to_fix(float v) {
-  return (long long)__float2ll_rn(v * MFA_FIX_SCALE);
+  return (long long)__float2ll_rz(v * MFA_FIX_SCALE);
+}
+
+// Adds a pair/bond/wall term to the fixed-point sums.  Terms below 512 in
+// every component take the int32 conversion; both routes truncate, so the
+// sum does not depend on which one ran.
+__device__ __forceinline__ void fix_add3(long long& ax, long long& ay, long long& az, float fx,
+                                         float fy, float fz, bool& bad) {
+  const float sx = fx * MFA_FIX_SCALE, sy = fy * MFA_FIX_SCALE, sz = fz * MFA_FIX_SCALE;
+  const float lim = 2147483520.0f;  // largest float below 2^31
+  if (fabsf(sx) < lim && fabsf(sy) < lim && fabsf(sz) < lim) {
+    ax += (long long)(int)sx;
+    ay += (long long)(int)sy;
+    az += (long long)(int)sz;
+  } else {
+    // large or non-finite term (a NaN fails every comparison above)
+    bad = bad || !(fabsf(sx) <= 3.0e38f && fabsf(sy) <= 3.0e38f && fabsf(sz) <= 3.0e38f);
+    ax += (long long)__float2ll_rz(sx);
+    ay += (long long)__float2ll_rz(sy);
+    az += (long long)__float2ll_rz(sz);
+  }
 }
 
 template <int TPP>
@@ -641,48 +707,92 @@ __device__ __forceinline__ float pair_normal_f32(uint32_t seed, int64_t step, in
   const float u2 = (float)(w1 >> 8) * 0x1p-24f;
   // v_log_f32 is log2; v_cos_f32 takes revolutions (cos(2 pi x)).
   const float l2 = __builtin_amdgcn_logf(u1);
-  return __builtin_sqrtf(-1.3862943611198906f * l2) * __builtin_amdgcn_cosf(u2);
+  return __builtin_amdgcn_sqrtf(-1.3862943611198906f * l2) * __builtin_amdgcn_cosf(u2);
 }
 
-template <int TPP, int MIXED>
-__global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __restrict__ r,
-                                                     const double* __restrict__ v,
-                                                     const int* __restrict__ type,
-                                                     const int* __restrict__ nl,
-                                                     const int* __restrict__ nn, int cap,
-                                                     double* __restrict__ f,
-                                                     const long long* __restrict__ d_step,
-                                                     int mask, double* __restrict__ part,
-                                                     int* __restrict__ flags) {
+// Switch-on clamp of r^2 (fluid_fluid.f90:42-49) with r_time = step*dt
+// (md_main.f90:69, step = s_time + i_time - 1); 0 once the ramp is over.
+__device__ __forceinline__ double switch_on_r2min(const KParams& k, int64_t step) {
+  if (k.fso_time > 0.0) {
+    const double r_time = (double)step * k.dt;
+    if (r_time < k.fso_time) return (1.0 - r_time / k.fso_time) * k.fso_r2_init;
+  }
+  return 0.0;
+}
+
+// The chain of particle i and its length: brush [0,n_brush) with n_mon, melt
+// [part_init_d,part_init_e) with n_mon_d, particle-4 chains [part_init_e,n)
+// with n_mon_e (intra_molec.f90:16-139).
+__device__ __forceinline__ void chain_of(const KParams& k, int i, int& base, int& len) {
+  base = -1;
+  len = 0;
+  if (i < k.n_brush) {
+    base = (i / k.n_mon) * k.n_mon;
+    len = k.n_mon;
+  } else if (i < k.part_init_e) {
+    base = k.part_init_d + ((i - k.part_init_d) / k.n_mon_d) * k.n_mon_d;
+    len = k.n_mon_d;
+  } else if (k.n_mon_e > 0) {
+    base = k.part_init_e + ((i - k.part_init_e) / k.n_mon_e) * k.n_mon_e;
+    len = k.n_mon_e;
+  }
+}
+
+// 9-3 walls at z = 0 and z = z_space_wall (fluid_wall.f90:81-121); the energy
+// uses a_w, the force a_wall(type) (:98,101,116,121).
+__device__ __forceinline__ void wall_93(const KParams& k, int ti, double zi, double& fb,
+                                        double& ft, double& e_fw) {
+  const double sw = k.sigma_wall[ti], aw = k.a_wall[ti], aabs = fabs(aw);
+  const double sw3 = pow3(sw), sw9 = pow3(sw3);
+  double inv_z = 1.0 / zi;
+  double rd = sw * inv_z;
+  double rd3 = pow3(rd);
+  e_fw += aabs * pow3(rd3) - k.a_w * rd3;
+  double iz2 = inv_z * inv_z, iz4 = iz2 * iz2, iz8 = iz4 * iz4;
+  fb = (9.0 * aabs) * sw9 * (iz8 * iz2) - (3.0 * aw) * sw3 * iz4;
+  inv_z = 1.0 / (k.z_space_wall - zi);
+  rd = sw * inv_z;
+  rd3 = pow3(rd);
+  e_fw += aabs * pow3(rd3) - k.a_w * rd3;
+  iz2 = inv_z * inv_z;
+  iz4 = iz2 * iz2;
+  iz8 = iz4 * iz4;
+  ft = -(9.0 * aabs) * sw9 * (iz8 * iz2) + (3.0 * aw) * sw3 * iz4;
+}
+
+template <int TPP>
+__global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* __restrict__ r,
+                                                       const double* __restrict__ v,
+                                                       const int* __restrict__ type,
+                                                       const unsigned* __restrict__ nl,
+                                                       const int* __restrict__ nn, int cap,
+                                                       double* __restrict__ f,
+                                                       const long long* __restrict__ d_step,
+                                                       int mask, double* __restrict__ part,
+                                                       int* __restrict__ flags) {
+  __shared__ int err_s;
+  if (threadIdx.x == 0) err_s = flags[F_ERROR];
+  __syncthreads();
+  if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gtid / TPP;
   const int sub = gtid % TPP;
   if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
-  if (flags[F_ERROR] != 0) return;      // state already invalid: stop, the host reports it
   const int64_t step = *d_step;
-  // force switch-on ramp (fluid_fluid.f90:42-49) with r_time = step*dt
-  // (md_main.f90:69, step = s_time + i_time - 1); r2min = 0 once it is over
-  double r2min = 0.0;
-  if (k.fso_time > 0.0) {
-    const double r_time = (double)step * k.dt;
-    if (r_time < k.fso_time) r2min = (1.0 - r_time / k.fso_time) * k.fso_r2_init;
-  }
+  const double r2min = switch_on_r2min(k, step);
   const bool active = i < k.n;
   const int ii = active ? i : 0;
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
   const double vxi = v[3 * (size_t)ii + 0], vyi = v[3 * (size_t)ii + 1], vzi = v[3 * (size_t)ii + 2];
   const int ti = type[ii] - 1;
   double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0;
-  Acc64 a64;
-  AccFix afx;
-  a64.zero();
-  afx.zero();
+  double ax = 0.0, ay = 0.0, az = 0.0;
 
   if (active && (mask & TM_PAIR)) {
     const int cnt = nn[ii];
-    const int* row = nl + (size_t)ii * cap;
+    const unsigned* row = nl + (size_t)ii * cap;
     for (int q = sub; q < cnt; q += TPP) {
-      const int j = row[q];
+      const int j = (int)(row[q] & MFA_JMASK);
       double dx = xi - r[3 * (size_t)j + 0];
       double dy = yi - r[3 * (size_t)j + 1];
       double dz = zi - r[3 * (size_t)j + 2];
@@ -706,100 +816,50 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
         }
       }
       if (!in_lj && !dpd) continue;
-      if (!MIXED) {
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        const double inv_r_2 = 1.0 / r2;
-        const double inv_sqrt_r_2 = sqrt(1.0 / r2);
-        if (in_lj) {
-          const double eps = k.epsil[ti][tj];
-          const double r61 = k.sigma_2[ti][tj] * inv_r_2;
-          const double r6 = (r61 * r61) * r61;
-          const double r12 = r6 * r6;
-          e_ff += 0.5 * (eps * ((r12 - r6) - k.e_shift[ti][tj]));
-          const double rd = (eps * (-12.0 * r12 + 6.0 * r6)) * inv_r_2;
-          fx = -(rd * dx);
-          fy = -(rd * dy);
-          fz = -(rd * dz);
-          vir += 0.5 * (-(rd * r2raw));
-        }
-        if (dpd) {
-          const double xi_n = pair_normal_f64(k.seed, step, i, j);
-          const double wr = 1.0 - sqrt(w_arg);
-          const double wd = wr * wr;
-          const double ex = dx * inv_sqrt_r_2, ey = dy * inv_sqrt_r_2, ez = dz * inv_sqrt_r_2;
-          const double sr = (k.sig * wr) * xi_n;
-          fx = fx + sr * ex;
-          fy = fy + sr * ey;
-          fz = fz + sr * ez;
-          const double dvx = vxi - v[3 * (size_t)j + 0];
-          const double dvy = vyi - v[3 * (size_t)j + 1];
-          const double dvz = vzi - v[3 * (size_t)j + 2];
-          const double rv = (dvx * ex + dvy * ey) + dvz * ez;
-          const double sd = ((-1.0 * k.friction1) * wd) * rv;
-          fx = fx + sd * ex;
-          fy = fy + sd * ey;
-          fz = fz + sd * ez;
-        }
-        a64.x += fx;
-        a64.y += fy;
-        a64.z += fz;
-      } else {
-#pragma clang fp contract(fast)
-        const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
-        const float r2f = (float)r2;
-        const float inv_r = __builtin_amdgcn_rsqf(r2f);  // v_rsq_f32, ~1 ulp
-        const float inv_r_2 = inv_r * inv_r;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (in_lj) {
-          const float eps = (float)k.epsil[ti][tj];
-          const float r61 = (float)k.sigma_2[ti][tj] * inv_r_2;
-          const float r6 = r61 * r61 * r61;
-          const float r12 = r6 * r6;
-          e_ff += 0.5 * (double)(eps * ((r12 - r6) - (float)k.e_shift[ti][tj]));
-          const float rd = eps * (6.0f * r6 - 12.0f * r12) * inv_r_2;
-          fx = -rd * dxf;
-          fy = -rd * dyf;
-          fz = -rd * dzf;
-          vir += 0.5 * (double)(-rd * (float)r2raw);
-        }
-        if (dpd) {
-          const float xi_n = pair_normal_f32(k.seed, step, i, j);
-          const float wr = 1.0f - __builtin_sqrtf((float)w_arg);
-          const float wd = wr * wr;
-          const float ex = dxf * inv_r, ey = dyf * inv_r, ez = dzf * inv_r;
-          const float sr = (float)k.sig * wr * xi_n;
-          const float dvx = (float)(vxi - v[3 * (size_t)j + 0]);
-          const float dvy = (float)(vyi - v[3 * (size_t)j + 1]);
-          const float dvz = (float)(vzi - v[3 * (size_t)j + 2]);
-          const float rv = dvx * ex + dvy * ey + dvz * ez;
-          const float s = sr - (float)k.friction1 * wd * rv;
-          fx += s * ex;
-          fy += s * ey;
-          fz += s * ez;
-        }
-        afx.x += to_fix(fx);
-        afx.y += to_fix(fy);
-        afx.z += to_fix(fz);
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      const double inv_r_2 = 1.0 / r2;
+      const double inv_sqrt_r_2 = sqrt(1.0 / r2);
+      if (in_lj) {
+        const double eps = k.epsil[ti][tj];
+        const double r61 = k.sigma_2[ti][tj] * inv_r_2;
+        const double r6 = (r61 * r61) * r61;
+        const double r12 = r6 * r6;
+        e_ff += 0.5 * (eps * ((r12 - r6) - k.e_shift[ti][tj]));
+        const double rd = (eps * (-12.0 * r12 + 6.0 * r6)) * inv_r_2;
+        fx = -(rd * dx);
+        fy = -(rd * dy);
+        fz = -(rd * dz);
+        vir += 0.5 * (-(rd * r2raw));
       }
+      if (dpd) {
+        const double xi_n = pair_normal_f64(k.seed, step, i, j);
+        const double wr = 1.0 - sqrt(w_arg);
+        const double wd = wr * wr;
+        const double ex = dx * inv_sqrt_r_2, ey = dy * inv_sqrt_r_2, ez = dz * inv_sqrt_r_2;
+        const double sr = (k.sig * wr) * xi_n;
+        fx = fx + sr * ex;
+        fy = fy + sr * ey;
+        fz = fz + sr * ez;
+        const double dvx = vxi - v[3 * (size_t)j + 0];
+        const double dvy = vyi - v[3 * (size_t)j + 1];
+        const double dvz = vzi - v[3 * (size_t)j + 2];
+        const double rv = (dvx * ex + dvy * ey) + dvz * ez;
+        const double sd = ((-1.0 * k.friction1) * wd) * rv;
+        fx = fx + sd * ex;
+        fy = fy + sd * ey;
+        fz = fz + sd * ez;
+      }
+      ax += fx;
+      ay += fy;
+      az += fz;
     }
   }
 
   // Bonds and walls: lane 0 of the group (at most two bonds per particle).
   if (active && sub == 0) {
     if (mask & TM_BOND) {
-      // chain of i: brush [0,n_brush) with n_mon, melt [part_init_d,part_init_e)
-      // with n_mon_d, particle-4 chains [part_init_e,n) with n_mon_e.
-      int base = -1, len = 0;
-      if (i < k.n_brush) {
-        base = (i / k.n_mon) * k.n_mon;
-        len = k.n_mon;
-      } else if (i < k.part_init_e) {
-        base = k.part_init_d + ((i - k.part_init_d) / k.n_mon_d) * k.n_mon_d;
-        len = k.n_mon_d;
-      } else if (k.n_mon_e > 0) {
-        base = k.part_init_e + ((i - k.part_init_e) / k.n_mon_e) * k.n_mon_e;
-        len = k.n_mon_e;
-      }
+      int base, len;
+      chain_of(k, i, base, len);
       for (int side = 0; side < 2; ++side) {
         // bond (lo, lo+1): side 0 -> (i-1, i), side 1 -> (i, i+1)
         const int lo = side == 0 ? i - 1 : i;
@@ -810,77 +870,29 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
         double dz = r[3 * (size_t)lo + 2] - r[3 * (size_t)hi + 2];
         min_image(k, dx, dy, dz);
         const double r2 = (dx * dx + dy * dy) + dz * dz;
-        if (!MIXED) {
-          double x = r2 * k.inv_r_chain_2;
-          const double clamp = (double)0.98f;
-          x = x < clamp ? x : clamp;
-          const double rd = (k.k_chain / (1.0 - x)) * k.inv_r_chain_2;
-          const double sgn = side == 0 ? 1.0 : -1.0;  // +force_loc on hi, -force_loc on lo
-          a64.x += sgn * (rd * dx);
-          a64.y += sgn * (rd * dy);
-          a64.z += sgn * (rd * dz);
-          if (side == 1) {
-            e_bond += -0.5 * log(1.0 - x);
-            vir += -(rd * r2);
-          }
-        } else {
-#pragma clang fp contract(fast)
-          float x = (float)(r2 * k.inv_r_chain_2);
-          x = x < 0.98f ? x : 0.98f;
-          const float rd = (float)k.k_chain / (1.0f - x) * (float)k.inv_r_chain_2;
-          const float sg = side == 0 ? 1.0f : -1.0f;
-          afx.x += to_fix(sg * rd * (float)dx);
-          afx.y += to_fix(sg * rd * (float)dy);
-          afx.z += to_fix(sg * rd * (float)dz);
-          if (side == 1) {
-            e_bond += -0.5 * log(1.0 - (double)x);
-            vir += (double)(-rd * (float)r2);
-          }
+        double x = r2 * k.inv_r_chain_2;
+        const double clamp = (double)0.98f;
+        x = x < clamp ? x : clamp;
+        const double rd = (k.k_chain / (1.0 - x)) * k.inv_r_chain_2;
+        const double sgn = side == 0 ? 1.0 : -1.0;  // +force_loc on hi, -force_loc on lo
+        ax += sgn * (rd * dx);
+        ay += sgn * (rd * dy);
+        az += sgn * (rd * dz);
+        if (side == 1) {
+          e_bond += -0.5 * log(1.0 - x);
+          vir += -(rd * r2);
         }
       }
     }
     if ((mask & TM_WALL) && k.symmetry == 0) {
-      // fluid_wall case 2 (fluid_wall.f90:81-121); the energy uses a_w, the
-      // force a_wall(type) (:98,101,116,121).
-      const double sw = k.sigma_wall[ti], aw = k.a_wall[ti], aabs = fabs(aw);
-      const double sw3 = pow3(sw), sw9 = pow3(sw3);
-      double inv_z = 1.0 / zi;
-      double rd = sw * inv_z;
-      double rd3 = pow3(rd);
-      e_fw += aabs * pow3(rd3) - k.a_w * rd3;
-      double iz2 = inv_z * inv_z, iz4 = iz2 * iz2, iz8 = iz4 * iz4;
-      const double fb = (9.0 * aabs) * sw9 * (iz8 * iz2) - (3.0 * aw) * sw3 * iz4;
-      inv_z = 1.0 / (k.z_space_wall - zi);
-      rd = sw * inv_z;
-      rd3 = pow3(rd);
-      e_fw += aabs * pow3(rd3) - k.a_w * rd3;
-      iz2 = inv_z * inv_z;
-      iz4 = iz2 * iz2;
-      iz8 = iz4 * iz4;
-      const double ft = -(9.0 * aabs) * sw9 * (iz8 * iz2) + (3.0 * aw) * sw3 * iz4;
-      fzb = fb;
-      fzt = ft;
-      if (!MIXED) {
-        a64.z += fb;
-        a64.z += ft;
-      } else {
-        afx.z += to_fix((float)fb);
-        afx.z += to_fix((float)ft);
-      }
+      wall_93(k, ti, zi, fzb, fzt, e_fw);
+      az += fzb;
+      az += fzt;
     }
   }
 
   // Combine the lane group and write F_i.
-  double fx, fy, fz;
-  if (!MIXED) {
-    fx = group_sum<TPP>(a64.x);
-    fy = group_sum<TPP>(a64.y);
-    fz = group_sum<TPP>(a64.z);
-  } else {
-    fx = (double)group_sum<TPP>(afx.x) * MFA_FIX_INV;
-    fy = (double)group_sum<TPP>(afx.y) * MFA_FIX_INV;
-    fz = (double)group_sum<TPP>(afx.z) * MFA_FIX_INV;
-  }
+  double fx = group_sum<TPP>(ax), fy = group_sum<TPP>(ay), fz = group_sum<TPP>(az);
   if (active && sub == 0) {
     if (mask & TM_ACCUM) {
       fx = f[3 * (size_t)i + 0] + fx;
@@ -908,6 +920,172 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force(KParams k, const double* __
   }
 }
 
+template <int TPP, int ENERGY>
+__global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
+    KParams k, const double* __restrict__ r, const float4* __restrict__ pos4,
+    const float4* __restrict__ vel4, const unsigned* __restrict__ nl, const int* __restrict__ nn,
+    int cap, double* __restrict__ f, const long long* __restrict__ d_step, int mask,
+    double* __restrict__ part, int* __restrict__ flags) {
+#pragma clang fp contract(fast)
+  // cell offsets -(d)*r_bin of the 27 codes, and the FP32 pair tables
+  __shared__ float4 s_off[27];
+  __shared__ float4 s_tab[MFA_NTYPE * MFA_NTYPE];  // range_2, 4 eps, sigma_2, e_shift
+  __shared__ float s_inv_range_2[MFA_NTYPE * MFA_NTYPE];
+  __shared__ int err_s;
+  if (threadIdx.x < 27) {
+    const int c = threadIdx.x;
+    const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
+    s_off[c] = make_float4(-(float)dx * (float)k.r_bin[0], -(float)dy * (float)k.r_bin[1],
+                           -(float)dz * (float)k.r_bin[2], 0.0f);
+  }
+  if (threadIdx.x >= 32 && threadIdx.x < 32 + MFA_NTYPE * MFA_NTYPE) {
+    const int q = threadIdx.x - 32, a = q / MFA_NTYPE, b = q % MFA_NTYPE;
+    s_tab[q] = make_float4((float)k.range_2[a][b], (float)k.epsil[a][b], (float)k.sigma_2[a][b],
+                           (float)k.e_shift[a][b]);
+    s_inv_range_2[q] = (float)k.inv_range_2[a][b];
+  }
+  if (threadIdx.x == 64) err_s = flags[F_ERROR];
+  __syncthreads();
+  if (err_s != 0) return;  // state already invalid: stop, the host reports it
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gtid / TPP;
+  const int sub = gtid % TPP;
+  if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  const int64_t step = *d_step;
+  const float r2min = (float)switch_on_r2min(k, step);
+  const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd2 = (float)(1.0 / k.r_cut_dpd_2);
+  const float sigf = (float)k.sig, gam = (float)k.friction1, margin = k.ff_margin;
+  const bool active = i < k.n;
+  const int ii = active ? i : 0;
+  const float4 pi4 = pos4[ii];
+  const float4 vi4 = vel4[ii];
+  const int ti = __float_as_int(pi4.w);
+  double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0, zi = 0.0;
+  long long ax = 0, ay = 0, az = 0;
+  bool bad = false;
+
+  if (active && (mask & TM_PAIR)) {
+    const int cnt = nn[ii];
+    const unsigned* row = nl + (size_t)ii * cap;
+    for (int q = sub; q < cnt; q += TPP) {
+      const unsigned e = row[q];
+      const int j = (int)(e & MFA_JMASK);
+      const float4 o = s_off[e >> MFA_JBITS];
+      const float4 pj = pos4[j];
+      const float dx = (pi4.x - pj.x) + o.x;
+      const float dy = (pi4.y - pj.y) + o.y;
+      const float dz = (pi4.z - pj.z) + o.z;
+      const float r2raw = dx * dx + dy * dy + dz * dz;
+      const int tj = __float_as_int(pj.w);
+      const float4 tb = s_tab[ti * MFA_NTYPE + tj];
+      bool in_lj = r2raw < tb.x;
+      bool in_dpd = k.dpd_cut_off_on ? r2raw < rdpd2 : in_lj;
+      bool near = fabsf(r2raw - tb.x) < margin;
+      if (k.dpd_cut_off_on) near = near || fabsf(r2raw - rdpd2) < margin;
+      if (near) {
+        // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
+        double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+        double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+        double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+        min_image(k, ex, ey, ez);
+        const double r2d = (ex * ex + ey * ey) + ez * ez;
+        in_lj = r2d < k.range_2[ti][tj];
+        in_dpd = k.dpd_cut_off_on ? r2d < k.r_cut_dpd_2 : in_lj;
+      }
+      const bool dpd = k.dpd_on && in_dpd && !(k.dpd_head_excl && k.dpd_cut_off_on &&
+                                               (ti == 0 || tj == 0));
+      if (!in_lj && !dpd) continue;
+      const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
+      const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
+      const float inv_r_2 = inv_r * inv_r;
+      float s = 0.0f;  // F_ij = s * (dx, dy, dz)
+      if (in_lj) {
+        const float r61 = tb.z * inv_r_2;
+        const float r6 = r61 * r61 * r61;
+        const float r12 = r6 * r6;
+        const float rd = tb.y * (6.0f * r6 - 12.0f * r12) * inv_r_2;
+        s = -rd;
+        if (ENERGY) {
+          e_ff += 0.5 * (double)(tb.y * ((r12 - r6) - tb.w));
+          vir += 0.5 * (double)(-rd * r2raw);
+        }
+      }
+      if (dpd) {
+        const float xi_n = pair_normal_f32(k.seed, step, i, j);
+        const float w_arg = k.dpd_cut_off_on ? r2 * inv_rdpd2 : r2 * s_inv_range_2[ti * MFA_NTYPE + tj];
+        const float wr = 1.0f - __builtin_amdgcn_sqrtf(w_arg);  // v_sqrt_f32, ~1 ulp
+        const float4 vj = vel4[j];
+        const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
+        s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
+      }
+      fix_add3(ax, ay, az, s * dx, s * dy, s * dz, bad);
+    }
+  }
+
+  // Bonds and walls: lane 0 of the group, from the FP64 positions.
+  if (active && sub == 0 && (ENERGY || (mask & (TM_BOND | TM_WALL)))) zi = r[3 * (size_t)i + 2];
+  if (active && sub == 0 && (mask & (TM_BOND | TM_WALL))) {
+    if (mask & TM_BOND) {
+      int base, len;
+      chain_of(k, i, base, len);
+      for (int side = 0; side < 2; ++side) {
+        const int lo = side == 0 ? i - 1 : i;
+        if (len < 2 || lo < base || lo + 1 >= base + len) continue;
+        const int hi = lo + 1;
+        double dx = r[3 * (size_t)lo + 0] - r[3 * (size_t)hi + 0];
+        double dy = r[3 * (size_t)lo + 1] - r[3 * (size_t)hi + 1];
+        double dz = r[3 * (size_t)lo + 2] - r[3 * (size_t)hi + 2];
+        min_image(k, dx, dy, dz);
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        float x = (float)(r2 * k.inv_r_chain_2);
+        x = x < 0.98f ? x : 0.98f;
+        const float rd = (float)k.k_chain / (1.0f - x) * (float)k.inv_r_chain_2;
+        const float sg = side == 0 ? rd : -rd;
+        fix_add3(ax, ay, az, sg * (float)dx, sg * (float)dy, sg * (float)dz, bad);
+        if (ENERGY && side == 1) {
+          e_bond += -0.5 * log(1.0 - (double)x);
+          vir += (double)(-rd * (float)r2);
+        }
+      }
+    }
+    if ((mask & TM_WALL) && k.symmetry == 0) {
+      wall_93(k, ti, zi, fzb, fzt, e_fw);
+      fix_add3(ax, ay, az, 0.0f, 0.0f, (float)fzb, bad);
+      fix_add3(ax, ay, az, 0.0f, 0.0f, (float)fzt, bad);
+    }
+  }
+
+  // Combine the lane group and write F_i.
+  double fx = (double)group_sum<TPP>(ax) * MFA_FIX_INV;
+  double fy = (double)group_sum<TPP>(ay) * MFA_FIX_INV;
+  double fz = (double)group_sum<TPP>(az) * MFA_FIX_INV;
+  if (active && sub == 0) {
+    if (mask & TM_ACCUM) {
+      fx = f[3 * (size_t)i + 0] + fx;
+      fy = f[3 * (size_t)i + 1] + fy;
+      fz = f[3 * (size_t)i + 2] + fz;
+    }
+    f[3 * (size_t)i + 0] = fx;
+    f[3 * (size_t)i + 1] = fy;
+    f[3 * (size_t)i + 2] = fz;
+  }
+  if (bad) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
+  if (ENERGY) {
+    double vals[MFA_NTERMS];
+    for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+    if (active) {
+      vals[T_VFF] = e_ff;
+      vals[T_VIR] = vir;
+      vals[T_VFW] = e_fw;
+      vals[T_VINTRA] = e_bond * k.k_chain;
+      vals[T_FZB] = fzb;
+      vals[T_FZT] = fzt;
+      if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
+    }
+    block_partials(vals, part);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 
@@ -918,6 +1096,7 @@ static KParams make_kparams(const mfa_params& p) {
     k.L[d] = p.boundary[d];
     k.invL[d] = p.inv_boundary[d];
     k.inv_r_bin[d] = p.inv_r_bin[d];
+    k.r_bin[d] = p.r_bin[d];
     k.nb[d] = p.n_bin[d];
     k.va[d] = p.va_spring_twall[d];
   }
@@ -963,6 +1142,21 @@ static KParams make_kparams(const mfa_params& p) {
     const double dcoord = 8.0 * lmax * u + 2.0 * sqrt(l2max) * u;
     k.pf_margin = (float)(4.0 * (3.0 * 2.0 * sqrt(l2max) * dcoord + 3.0 * dcoord * dcoord) +
                           8.0 * l2max * u);
+    // Force pass: coordinates relative to the cell origin at the rebuild stay
+    // within [-skin/2, r_bin + skin/2] (a particle that moves farther triggers
+    // a rebuild); each of the two coordinates, their difference, the FP32 cell
+    // offset and the final sum carry one rounding.
+    double bmax = 0.0, rbmax = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      bmax = p.r_bin[d] + p.skin > bmax ? p.r_bin[d] + p.skin : bmax;
+      rbmax = p.r_bin[d] > rbmax ? p.r_bin[d] : rbmax;
+    }
+    const double dsep = (6.0 * bmax + 2.0 * rbmax) * u;
+    double c2max = p.r_cut_dpd_2;
+    for (int t = 0; t < MFA_NTYPE; ++t)
+      for (int q = 0; q < MFA_NTYPE; ++q) c2max = p.range_2[t][q] > c2max ? p.range_2[t][q] : c2max;
+    k.ff_margin = (float)(4.0 * (2.0 * sqrt(3.0 * c2max) * dsep + 3.0 * dsep * dsep +
+                                 6.0 * c2max * u));
   }
   k.n_cells = p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
   k.n = p.n_part;
@@ -995,8 +1189,8 @@ static int launch_rebuild(mfa_ctx* c) {
   hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->cell_of,
                      c->cell_count, c->cell_slots, c->flags);
   hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c->cell_cap),
-                     c->stream, c->k, c->r, c->type, c->mic, c->cell_count, c->cell_slots, c->nl,
-                     c->nn, c->cap, c->r_ref, c->flags);
+                     c->stream, c->k, c->r, c->v, c->type, c->mic, c->cell_count, c->cell_slots,
+                     c->nl, c->nn, c->cap, c->r_ref, c->flags, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -1004,12 +1198,19 @@ static int launch_rebuild(mfa_ctx* c) {
 template <int TPP>
 static void launch_force_t(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
-  if (c->precision == MFA_MIXED)
-    hipLaunchKernelGGL((k_force<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
+  if (c->precision == MFA_MIXED) {
+    if (mask & TM_ENERGY)
+      hipLaunchKernelGGL((k_force_mixed<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k,
+                         c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step, mask,
+                         c->part, c->flags);
+    else
+      hipLaunchKernelGGL((k_force_mixed<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k,
+                         c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step, mask,
+                         c->part, c->flags);
+  } else {
+    hipLaunchKernelGGL((k_force64<TPP>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
                        c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part, c->flags);
-  else
-    hipLaunchKernelGGL((k_force<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                       c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part, c->flags);
+  }
 }
 
 static int force_blocks(mfa_ctx* c) { return grid_of(c->n * c->tpp); }
@@ -1079,7 +1280,7 @@ static int launch_fused(mfa_ctx* c, int energy) {
   }
   hipLaunchKernelGGL(k_fused_integrate, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
                      c->f, c->v, c->r, c->mic, c->r_ref, c->type, c->flags, c->part, energy,
-                     c->cell_count, c->d_step);
+                     c->cell_count, c->d_step, c->cell_of, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (timed) return timed_end(c, c->ev_integ, c->ev_integ_used);
   return MFA_OK;
@@ -1193,6 +1394,8 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   if (precision != MFA_FP64 && precision != MFA_MIXED)
     return set_err(nullptr, MFA_ERR_ARG, "unknown precision %d", precision);
   if (p->n_part <= 0) return set_err(nullptr, MFA_ERR_ARG, "n_part must be positive");
+  if ((unsigned)p->n_part > MFA_JMASK)
+    return set_err(nullptr, MFA_ERR_ARG, "at most %u particles per context", MFA_JMASK);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, MFA_ERR_HIP, "no HIP device");
@@ -1264,8 +1467,10 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   ALLOC(c->cell_of, n * sizeof(int32_t));
   ALLOC(c->cell_count, (size_t)c->n_cells * sizeof(int32_t));
   ALLOC(c->cell_slots, ncs * sizeof(int32_t));
-  ALLOC(c->nl, n * (size_t)c->cap * sizeof(int32_t));
+  ALLOC(c->nl, n * (size_t)c->cap * sizeof(unsigned));
   ALLOC(c->nn, n * sizeof(int32_t));
+  ALLOC(c->pos4, n * sizeof(float4));
+  ALLOC(c->vel4, n * sizeof(float4));
   ALLOC(c->flags, F_NFLAGS * sizeof(int32_t));
   ALLOC(c->d_step, sizeof(long long));
   ALLOC(c->part, (size_t)c->max_blocks * MFA_NTERMS * sizeof(double));
@@ -1290,6 +1495,7 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   hipMemsetAsync(c->f, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->a, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->nn, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->cell_of, 0, n * sizeof(int32_t), c->stream);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     mfa_destroy(c);
     return set_err(nullptr, MFA_ERR_HIP, "initial clears failed");
@@ -1302,8 +1508,9 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   if (!c) return MFA_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void* ptrs[] = {c->r, c->v, c->a, c->f, c->r_ref, c->type, c->mic, c->cell_of, c->cell_count,
-                  c->cell_slots, c->nl, c->nn, c->flags, c->d_step, c->part, c->red};
+  void* ptrs[] = {c->r,  c->v,          c->a,     c->f,     c->r_ref,  c->type,
+                  c->mic, c->cell_of,   c->cell_count, c->cell_slots, c->nl, c->nn,
+                  c->pos4, c->vel4,     c->flags, c->d_step, c->part,   c->red};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->h_flags) hipHostFree(c->h_flags);
@@ -1377,7 +1584,8 @@ extern "C" int32_t mfa_get_state(mfa_ctx* c, double* r0, double* v, double* a, d
 extern "C" int32_t mfa_verlet_positions(mfa_ctx* c) {
   if (!c) return MFA_ERR_ARG;
   hipLaunchKernelGGL(k_verlet_positions, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0, c->cell_count);
+                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0, c->cell_count, c->type,
+                     c->cell_of, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFA_OK;
@@ -1427,13 +1635,14 @@ extern "C" int32_t mfa_get_pairs(mfa_ctx* c, int32_t* pairs, int64_t cap, int64_
   if (!c || !n_pairs) return MFA_ERR_ARG;
   int rc = ensure_list(c);
   if (rc) return rc;
-  std::vector<int32_t> nn((size_t)c->n), nl((size_t)c->n * c->cap);
+  std::vector<int32_t> nn((size_t)c->n);
+  std::vector<unsigned> nl((size_t)c->n * c->cap);
   HIPCHK(c, hipMemcpy(nn.data(), c->nn, nn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(nl.data(), c->nl, nl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(nl.data(), c->nl, nl.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
   int64_t m = 0;
   for (int i = 0; i < c->n; ++i)
     for (int q = 0; q < nn[(size_t)i]; ++q) {
-      const int j = nl[(size_t)i * c->cap + q];
+      const int j = (int)(nl[(size_t)i * c->cap + q] & MFA_JMASK);
       if (i < j) {
         if (pairs && m < cap) {
           pairs[2 * m + 0] = i + 1;
@@ -1506,7 +1715,7 @@ extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
   if (!c) return MFA_ERR_ARG;
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,
-                     c->a, c->type, c->part, t_kin ? 1 : 0);
+                     c->a, c->type, c->part, t_kin ? 1 : 0, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (t_kin) {
     double t[MFA_NTERMS];
@@ -1570,7 +1779,8 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   const int g = grid_of(c->n);
   // first half of the first step (verlet_positions + check_skin)
   hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                     c->a, c->mic, c->r_ref, c->flags, 1, c->cell_count);
+                     c->a, c->mic, c->r_ref, c->flags, 1, c->cell_count, c->type, c->cell_of,
+                     c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   // steps 1 .. n-1: captured blocks of graph_steps steps, then single steps
   int middle = n - 1;
@@ -1597,7 +1807,7 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,
-                     c->a, c->type, c->part, obs ? 1 : 0);
+                     c->a, c->type, c->part, obs ? 1 : 0, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (obs) {
     rc = reduce_terms(c, g, tk);
