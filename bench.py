@@ -34,9 +34,8 @@ FORCE_BYTES_PER_PARTICLE = 76
 STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
 # HBM bytes per force pass of the mixed form on this workload, from the
 # FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
-# each scaled by the same counter's reading for a copy of known size; None
-# until the counters have been read for the current kernel.
-FORCE_TRAFFIC_BYTES = None
+# each scaled by the same counter's reading for a copy of known size.
+FORCE_TRAFFIC_BYTES = 79.433e6 + 2.443e6
 
 
 def log(*a):

@@ -1,10 +1,12 @@
 """Summaries of rocprofv3 csv output for the hot kernels.
 
 --trace run_kernel_trace.csv : per-kernel average durations in the timed
-    window of a bench.py run (the launches after the last k_force<*,1> warm-up
+    window of a bench.py run (the launches after the last mixed warm-up
     block); rebuild kernels are split into no-op launches and real rebuilds.
 --pmc run_counter_collection.csv : FETCH_SIZE / WRITE_SIZE per dispatch of the
     force pass, scaled by the reading of the calibration copy of known size.
+--counters run_counter_collection.csv : median of every counter per dispatch
+    of each kernel.
 """
 import argparse
 import csv
@@ -12,6 +14,8 @@ import statistics
 from collections import defaultdict
 
 COPY_BYTES = 256 * 1024 * 1024
+MIXED_FORCE = "void k_force_mixed<8, 0>"
+FP64_FORCE = "void k_force64<8>"
 
 
 def trace(path, n_timed):
@@ -19,10 +23,10 @@ def trace(path, n_timed):
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     names = [r["Kernel_Name"] for r in rows]
     try:
-        end = next(i for i, n in enumerate(names) if n.startswith("void k_force<8, 0>"))
+        end = next(i for i, n in enumerate(names) if n.startswith(FP64_FORCE))
     except StopIteration:
         end = len(rows)
-    idx = [i for i in range(end) if names[i].startswith("void k_force<8, 1>")]
+    idx = [i for i in range(end) if names[i].startswith(MIXED_FORCE)]
     seg = rows[idx[-n_timed]:end]
     d = defaultdict(list)
     for r in seg:
@@ -50,7 +54,7 @@ def pmc(path):
     out = []
     for counter, kern in by.items():
         copy = [v for k, v in kern.items() if "copybuffer" in k.lower()]
-        force = kern.get("void k_force<8, 1>") or kern.get("void k_force<8, 0>") or []
+        force = kern.get(MIXED_FORCE) or kern.get(FP64_FORCE) or []
         if not copy or not force:
             out.append("%s: calibration copy or force pass not found (%s)" % (counter, list(kern)))
             continue
@@ -63,16 +67,33 @@ def pmc(path):
     return "\n".join(out)
 
 
+def counters(path):
+    rows = list(csv.DictReader(open(path)))
+    by = defaultdict(lambda: defaultdict(list))
+    for r in rows:
+        by[r["Kernel_Name"].split("(")[0]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = []
+    for kern, cs in sorted(by.items()):
+        n = max(len(v) for v in cs.values())
+        out.append("%s (%d dispatches)" % (kern, n))
+        for cname, v in sorted(cs.items()):
+            out.append("    %-24s median %16.1f" % (cname, statistics.median(v)))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace")
     ap.add_argument("--timed", type=int, default=2400)
     ap.add_argument("--pmc")
+    ap.add_argument("--counters")
     a = ap.parse_args()
     if a.trace:
         print(trace(a.trace, a.timed))
     if a.pmc:
         print(pmc(a.pmc))
+    if a.counters:
+        print(counters(a.counters))
 
 
 if __name__ == "__main__":

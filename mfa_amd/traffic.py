@@ -1,9 +1,10 @@
-"""Workload for the HBM-traffic counters of the force pass.
+"""Workload for the counter runs of the force pass and the list build.
 
 Builds the seeded 100k-bead melt (density 0.85, the production cut-off, so
-the neighbour structure of the timed runs), bins it and runs the force pass
-`--reps` times with direct launches, then a plain copy of a known size (the
-calibration for the counters).  No trajectory is run, which keeps the number
+the neighbour structure of the timed runs), bins it `--rebuilds` times (each
+a cell fill and a list build), runs the force pass `--reps` times with direct
+launches, then a plain copy of a known size (the calibration for the
+counters).  No trajectory is run, which keeps the number
 of dispatches small for the counter runs.  Meant to run under rocprofv3 with one counter per run:
 
     MFA_GRAPH_STEPS=0 rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run \
@@ -27,6 +28,7 @@ COPY_BYTES = 256 * 1024 * 1024  # bytes read and bytes written by the calibratio
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--rebuilds", type=int, default=1)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     args = ap.parse_args()
     from mfa_amd import configs, lib
@@ -37,6 +39,8 @@ def main():
     with lib.Context(p, prec) as c:
         c.set_graph_steps(0)
         c.set_state(r, v, np.zeros_like(r), t)
+        for _ in range(args.rebuilds):
+            c.binning()
         ms = c.time_force(args.reps)
     cms = lib.copy_probe(COPY_BYTES)
     print("force pass %.4f ms; calibration copy of %d bytes in %.4f ms (%.0f GB/s read+write)"
