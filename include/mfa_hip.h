@@ -187,6 +187,16 @@ int32_t mfa_set_state(mfa_ctx* ctx, const double* r0, const double* v, const dou
                       const int32_t* a_type, const int32_t* mic);
 int32_t mfa_get_state(mfa_ctx* ctx, double* r0, double* v, double* a, double* force, int32_t* mic);
 
+/* ewald_k(mode) (ewald_k.f90:22-202; called at md_main.f90:129, mode 0 at
+ * init_config.f90:244): mode 0 is a no-op here (mfa_create builds the k
+ * grid); mode 1 adds the k-space forces of the charged particles to F and
+ * returns u_cou_k in *v_coul.  No-op without charges. */
+int32_t mfa_ewald_k(mfa_ctx* ctx, int32_t mode, double* v_coul);
+/* dipolar_correction() (dipolar_correction.f90:20-33, md_main.f90:131): adds
+ * -4 pi q M_z / V to F_z of the charged particles (walls only) and returns
+ * 2 pi M_z^2 / V in *v_coul. */
+int32_t mfa_dipolar_correction(mfa_ctx* ctx, double* v_coul);
+
 /* verlet_positions (verlet_positions.f90:301-406). */
 int32_t mfa_verlet_positions(mfa_ctx* ctx);
 /* check_skin (check_skin.f90:135-155): *f_skin = 1 if a rebuild is due. */
@@ -243,7 +253,9 @@ int32_t mfa_timing(mfa_ctx* ctx, int32_t enable, double* force_ms_avg, int64_t* 
 
 /* Seeded input generators (the reference's conf_default/gen_brush/gen_droplet
  * are replaced by these; conf_default.f90:144-217, gen_brush.f90:101-161).
- * kind 1: brush + ideal solvent in a channel; kind 2: melt in a periodic box.
+ * kind 1: brush + ideal solvent in a channel; kind 2: melt in a periodic box;
+ * kind 3: polyelectrolyte brush (every third bead type 4), one counterion
+ * (type 3) per charged bead and neutral solvent (type 2) in a channel.
  * Writes r0, v (Maxwell-Boltzmann at temp, COM momentum removed), a_type.
  * Arrays are sized n_part = p->n_part.  Host-only. */
 int32_t mfa_generate(const mfa_params* p, int32_t kind, uint64_t seed, double head_z,

@@ -113,6 +113,31 @@ def melt(n_chain, n_mon, density, lj_rc=2.5, dt=0.01, gamma=1.0, skin=0.3, seed=
     return x
 
 
+def pe_brush(n_chain, n_mon, n_solvent, lx, ly, lz, x_space=1.0, dt=0.005, gamma=4.5,
+             alpha_c=0.81, n_kmax=10, rc_coul=3.0, compat=False, skin=0.3, seed=3003):
+    """Polyelectrolyte brush with Ewald (SYSTEM 3, PARTICLE_4): brush monomers
+    type 2, every third bead type 4 with charge -1, counterions type 3 (+1)
+    and neutral solvent type 2 as single beads; WCA everywhere, DPD cut-off
+    1.0.  alpha_c is the reference's alpha_c (erfc(sqrt(alpha_c) r)); with
+    `compat` the real space runs inside the LJ range with the reference's
+    1/pi factor (ewald_real.f90, init_config.f90:230)."""
+    x = brush(n_chain, n_mon, 0, lx, ly, lz, x_space=x_space, dt=dt, gamma=gamma, skin=skin,
+              seed=seed)
+    n_ci = n_chain * sum(1 for m in range(1, n_mon) if (m + 1) % 3 == 0)
+    x.n_mon_d, x.n_chain_d = 1, n_ci + n_solvent
+    x.particle_4 = 1
+    x.inter4 = 1
+    for t in range(NT):
+        for u in range(NT):
+            x.epsil[t][u] = 1.0
+    x.charge[0], x.charge[1], x.charge[2], x.charge[3] = 0.0, 0.0, 1.0, -1.0
+    x.alpha_c = alpha_c
+    x.n_kmax = n_kmax
+    x.ewald_compat = 1 if compat else 0
+    x.rc_coul = 0.0 if compat else rc_coul
+    return x
+
+
 WORKLOADS = {
     # configs[0]: plain FP64 DPD polymer brush (CPU-runnable reference case)
     "brush9k": dict(kind=1, steps=1000, min_dist=0.7,
@@ -131,6 +156,12 @@ def small_cases():
         "melt_wca": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=13, n_cell=16)),
         "melt_switch_on": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=14, n_cell=16,
                                    switch_on=True)),
+        # charged brush, the reference's own Ewald form (compat) and the
+        # corrected one with a real-space cut-off of 3
+        "pe_brush_compat": (3, pe_brush(8, 12, 60, 6.0, 6.0, 8.0, x_space=0.5, n_kmax=4,
+                                        compat=True, seed=15)),
+        "pe_brush_small": (3, pe_brush(8, 12, 60, 12.0, 12.0, 14.0, x_space=0.5, n_kmax=4,
+                                       rc_coul=3.0, seed=16)),
     }
 
 

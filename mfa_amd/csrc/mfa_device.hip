@@ -73,6 +73,11 @@ struct KParams {
   double inv_r_bin[3], r_bin[3];
   double skin_half_2;
   double fso_r2_init, fso_time;  // force switch-on ramp (0 = off)
+  // Coulomb (SYSTEM 2/3): charge per type, erfc(sqrt(alpha_c) r) real space
+  // with the prefactor inv_sqrt_pi (1/pi in the compat form,
+  // init_config.f90:230), inside rc_coul or (rc_coul_2 = 0) inside the LJ range
+  double q_type[MFA_NTYPE], alpha_c, sqrt_alpha, inv_sqrt_pi, rc_coul_2;
+  int coulomb;
   float pf_margin;               // bound on |r^2(FP32) - r^2(FP64)| in the list build
   float ff_margin;               // the same for the cell-relative FP32 view of the force pass
   int nb[3];
@@ -80,6 +85,15 @@ struct KParams {
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
   int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
   uint32_t seed;
+};
+
+// Ewald k-space launch constants (the k grid lives in device arrays).
+struct EwaldK {
+  double dk[3];       // delta_k (ewald_k.f90:44-52)
+  double lz_half;     // init_config.f90:246
+  int nk, nruns, nq;
+  int n_pw, runs_per_rgroup;   // structure factor: particle waves x run groups
+  int runs_per_kgroup;         // forces: run groups
 };
 
 struct mfa_ctx {
@@ -99,6 +113,19 @@ struct mfa_ctx {
   unsigned* nl;
   int32_t* nn;
   float4 *pos4, *vel4;  // FP32 view of the mixed form
+  // Ewald k space (charged systems)
+  bool ewald;
+  EwaldK ek;
+  double ew_inv_V;
+  int ew_n_rgroups, ew_n_kgroups, ew_sum_blocks;
+  double4* d_kvec;
+  int4* d_runs;
+  int* d_run_k0;
+  int* d_qidx;
+  double* d_qval;
+  double2 *d_rho_part, *d_grho;
+  double *d_uk_part, *d_mz_part, *d_mz, *d_fk_part, *d_ek;
+  double v_coul_self;
   long long* d_step;    // device step counter
   // captured step block: graph_steps steps of [rebuild, force, integrate]
   int graph_steps;
@@ -785,7 +812,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
   const double vxi = v[3 * (size_t)ii + 0], vyi = v[3 * (size_t)ii + 1], vzi = v[3 * (size_t)ii + 2];
   const int ti = type[ii] - 1;
-  double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0;
+  double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0, e_coul = 0.0;
   double ax = 0.0, ay = 0.0, az = 0.0;
 
   if (active && (mask & TM_PAIR)) {
@@ -815,7 +842,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
           w_arg = r2 / k.r_cut_dpd_2;
         }
       }
-      if (!in_lj && !dpd) continue;
+      // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
+      const bool coul = k.coulomb && (k.rc_coul_2 > 0.0 ? r2raw < k.rc_coul_2 : in_lj);
+      if (!in_lj && !dpd && !coul) continue;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       const double inv_r_2 = 1.0 / r2;
       const double inv_sqrt_r_2 = sqrt(1.0 / r2);
@@ -848,6 +877,18 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
         fx = fx + sd * ex;
         fy = fy + sd * ey;
         fz = fz + sd * ez;
+      }
+      if (coul) {  // ewald_real.f90:15-32 with the clamped r_2
+        const double qq = k.q_type[ti] * k.q_type[tj];
+        const double rr = sqrt(r2);
+        const double inv_r = 1.0 / rr;
+        const double ec = erfc(k.sqrt_alpha * rr);
+        e_coul += 0.5 * (qq * ec * inv_r);
+        const double f_scal = qq * k.sqrt_alpha * k.inv_sqrt_pi * 2.0 * exp(-k.alpha_c * r2) * inv_r +
+                              qq * ec * inv_r * inv_r;
+        fx = fx + f_scal * dx * inv_r;
+        fy = fy + f_scal * dy * inv_r;
+        fz = fz + f_scal * dz * inv_r;
       }
       ax += fx;
       ay += fy;
@@ -912,6 +953,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
       vals[T_VIR] = vir;
       vals[T_VFW] = e_fw;
       vals[T_VINTRA] = e_bond * k.k_chain;
+      vals[T_VCOUL] = e_coul;
       vals[T_FZB] = fzb;
       vals[T_FZT] = fzt;
       if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
@@ -961,6 +1003,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const float4 vi4 = vel4[ii];
   const int ti = __float_as_int(pi4.w);
   double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0, zi = 0.0;
+  double e_coul = 0.0;
+  const float rc2f = (float)k.rc_coul_2, sa = (float)k.sqrt_alpha, alf = (float)k.alpha_c;
+  const float pref = (float)(k.sqrt_alpha * k.inv_sqrt_pi * 2.0);
   long long ax = 0, ay = 0, az = 0;
   bool bad = false;
 
@@ -980,8 +1025,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       const float4 tb = s_tab[ti * MFA_NTYPE + tj];
       bool in_lj = r2raw < tb.x;
       bool in_dpd = k.dpd_cut_off_on ? r2raw < rdpd2 : in_lj;
+      bool in_coul = k.rc_coul_2 > 0.0 && r2raw < rc2f;
       bool near = fabsf(r2raw - tb.x) < margin;
       if (k.dpd_cut_off_on) near = near || fabsf(r2raw - rdpd2) < margin;
+      if (k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
       if (near) {
         // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
         double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
@@ -991,10 +1038,13 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const double r2d = (ex * ex + ey * ey) + ez * ez;
         in_lj = r2d < k.range_2[ti][tj];
         in_dpd = k.dpd_cut_off_on ? r2d < k.r_cut_dpd_2 : in_lj;
+        in_coul = k.rc_coul_2 > 0.0 && r2d < k.rc_coul_2;
       }
       const bool dpd = k.dpd_on && in_dpd && !(k.dpd_head_excl && k.dpd_cut_off_on &&
                                                (ti == 0 || tj == 0));
-      if (!in_lj && !dpd) continue;
+      // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
+      const bool coul = k.coulomb && (k.rc_coul_2 > 0.0 ? in_coul : in_lj);
+      if (!in_lj && !dpd && !coul) continue;
       const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
       const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
       const float inv_r_2 = inv_r * inv_r;
@@ -1017,6 +1067,13 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const float4 vj = vel4[j];
         const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
         s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
+      }
+      if (coul) {  // ewald_real.f90:15-32 with the clamped r_2
+        const float qq = (float)k.q_type[ti] * (float)k.q_type[tj];
+        const float rr = r2 * inv_r;
+        const float ec = erfcf(sa * rr);
+        s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
+        if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
       }
       fix_add3(ax, ay, az, s * dx, s * dy, s * dz, bad);
     }
@@ -1078,12 +1135,213 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       vals[T_VIR] = vir;
       vals[T_VFW] = e_fw;
       vals[T_VINTRA] = e_bond * k.k_chain;
+      vals[T_VCOUL] = e_coul;
       vals[T_FZB] = fzb;
       vals[T_FZT] = fzt;
       if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
     }
     block_partials(vals, part);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Ewald sum in k space (ewald_k.f90:150-199) and the slab dipole correction
+// (dipolar_correction.f90:20-33), FP64.  The k vectors of the half space
+// (ewald_k.f90:104-124) come in runs of consecutive nx at fixed (ny, nz); a
+// thread evaluates exp(i k.r) once per run with sincos and steps along the
+// run by the factor exp(i dkx x), so a run of m vectors costs one sincos and
+// m complex products.  Charged particles only (q = 0 contributes nothing).
+#define EW_QPT 4  // charged particles per thread in the structure factor
+
+// Partial structure factor of one wave of 64*EW_QPT charged particles over
+// one group of runs, reduced across the wave in a fixed tree; and (run group
+// 0) the partial dipole moment sum q (z - lz_half).
+__global__ __launch_bounds__(64) void k_ewald_rho(EwaldK e, const double* __restrict__ r,
+                                                  const int* __restrict__ qidx,
+                                                  const double* __restrict__ qval,
+                                                  const int4* __restrict__ runs,
+                                                  const int* __restrict__ run_k0,
+                                                  double2* __restrict__ rho_part,
+                                                  double* __restrict__ mz_part) {
+  const int pw = blockIdx.x, rg = blockIdx.y, lane = threadIdx.x;
+  double x[EW_QPT], y[EW_QPT], z[EW_QPT], q[EW_QPT], cx[EW_QPT], sx[EW_QPT];
+  double mz = 0.0;
+  for (int u = 0; u < EW_QPT; ++u) {
+    const int idx = (pw * EW_QPT + u) * 64 + lane;
+    if (idx < e.nq) {
+      const int i = qidx[idx];
+      x[u] = r[3 * (size_t)i + 0];
+      y[u] = r[3 * (size_t)i + 1];
+      z[u] = r[3 * (size_t)i + 2];
+      q[u] = qval[idx];
+    } else {
+      x[u] = y[u] = z[u] = q[u] = 0.0;
+    }
+    sincos(e.dk[0] * x[u], &sx[u], &cx[u]);
+    mz += q[u] * (z[u] - e.lz_half);
+  }
+  if (rg == 0) {
+    for (int o = 32; o > 0; o >>= 1) mz += __shfl_down(mz, o, 64);
+    if (lane == 0) mz_part[pw] = mz;
+  }
+  const int r0 = rg * e.runs_per_rgroup, r1 = min(e.nruns, r0 + e.runs_per_rgroup);
+  for (int run = r0; run < r1; ++run) {
+    const int4 rn = runs[run];  // (ny, nz, nx_lo, nx_hi)
+    const double kx0 = e.dk[0] * (double)rn.z, ky = e.dk[1] * (double)rn.x,
+                 kz = e.dk[2] * (double)rn.y;
+    double c[EW_QPT], s[EW_QPT];
+    for (int u = 0; u < EW_QPT; ++u) sincos(kx0 * x[u] + ky * y[u] + kz * z[u], &s[u], &c[u]);
+    int kidx = run_k0[run];
+    for (int nx = rn.z; nx <= rn.w; ++nx, ++kidx) {
+      double re = 0.0, im = 0.0;
+      for (int u = 0; u < EW_QPT; ++u) {
+        re += q[u] * c[u];
+        im += q[u] * s[u];
+        const double cn = c[u] * cx[u] - s[u] * sx[u];
+        s[u] = s[u] * cx[u] + c[u] * sx[u];
+        c[u] = cn;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        re += __shfl_down(re, o, 64);
+        im += __shfl_down(im, o, 64);
+      }
+      if (lane == 0) rho_part[(size_t)pw * e.nk + kidx] = make_double2(re, im);
+    }
+  }
+}
+
+// rho(k) = sum of the wave partials (fixed order), stored weighted by
+// g_k = exp(-k^2/(4 alpha_c))/k^2; per-block sums of g_k |rho(k)|^2 for the
+// energy; block 0 also sums the dipole moment.
+__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_sum(EwaldK e,
+                                                         const double2* __restrict__ rho_part,
+                                                         const double4* __restrict__ kvec,
+                                                         double2* __restrict__ grho,
+                                                         double* __restrict__ uk_part,
+                                                         const double* __restrict__ mz_part,
+                                                         double* __restrict__ mz) {
+  __shared__ double sh[MFA_BLOCK];
+  const int kk = blockIdx.x * MFA_BLOCK + threadIdx.x;
+  double u = 0.0;
+  if (kk < e.nk) {
+    double re = 0.0, im = 0.0;
+    for (int pw = 0; pw < e.n_pw; ++pw) {
+      const double2 t = rho_part[(size_t)pw * e.nk + kk];
+      re += t.x;
+      im += t.y;
+    }
+    const double g = kvec[kk].w;
+    grho[kk] = make_double2(g * re, g * im);
+    u = g * (re * re + im * im);
+  }
+  sh[threadIdx.x] = u;
+  __syncthreads();
+  for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    uk_part[blockIdx.x] = sh[0];
+    if (blockIdx.x == 0) {
+      double m = 0.0;
+      for (int pw = 0; pw < e.n_pw; ++pw) m += mz_part[pw];
+      *mz = m;
+    }
+  }
+}
+
+// Sum over one group of runs of k g_k (cos(k.r) Im rho - sin(k.r) Re rho) for
+// each charged particle (ewald_k.f90:177-192 without the prefactor).
+__global__ __launch_bounds__(64) void k_ewald_force(EwaldK e, const double* __restrict__ r,
+                                                    const int* __restrict__ qidx,
+                                                    const int4* __restrict__ runs,
+                                                    const int* __restrict__ run_k0,
+                                                    const double2* __restrict__ grho,
+                                                    double* __restrict__ fk_part) {
+  const int idx = blockIdx.x * 64 + threadIdx.x, kg = blockIdx.y;
+  if (idx >= e.nq) return;
+  const int i = qidx[idx];
+  const double x = r[3 * (size_t)i + 0], y = r[3 * (size_t)i + 1], z = r[3 * (size_t)i + 2];
+  double sx, cx;
+  sincos(e.dk[0] * x, &sx, &cx);
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  const int r0 = kg * e.runs_per_kgroup, r1 = min(e.nruns, r0 + e.runs_per_kgroup);
+  for (int run = r0; run < r1; ++run) {
+    const int4 rn = runs[run];
+    const double kx0 = e.dk[0] * (double)rn.z, ky = e.dk[1] * (double)rn.x,
+                 kz = e.dk[2] * (double)rn.y;
+    double s, c;
+    sincos(kx0 * x + ky * y + kz * z, &s, &c);
+    int kidx = run_k0[run];
+    double ty = 0.0;  // sum of t over the run (ky, kz are constant along it)
+    for (int nx = rn.z; nx <= rn.w; ++nx, ++kidx) {
+      const double2 g = grho[kidx];
+      const double t = c * g.y - s * g.x;
+      fx += (e.dk[0] * (double)nx) * t;
+      ty += t;
+      const double cn = c * cx - s * sx;
+      s = s * cx + c * sx;
+      c = cn;
+    }
+    fy += ky * ty;
+    fz += kz * ty;
+  }
+  double* o = fk_part + 3 * ((size_t)kg * e.nq + idx);
+  o[0] = fx;
+  o[1] = fy;
+  o[2] = fz;
+}
+
+// Dipole moment of the charged particles, one block (the per-subroutine
+// entry point; the step loop takes it from k_ewald_rho/k_ewald_sum).
+__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_mz(EwaldK e, const double* __restrict__ r,
+                                                        const int* __restrict__ qidx,
+                                                        const double* __restrict__ qval,
+                                                        double* __restrict__ mz) {
+  __shared__ double sh[MFA_BLOCK];
+  double m = 0.0;
+  for (int idx = threadIdx.x; idx < e.nq; idx += MFA_BLOCK)
+    m += qval[idx] * (r[3 * (size_t)qidx[idx] + 2] - e.lz_half);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *mz = sh[0];
+}
+
+// Adds -8 pi q f inv_V (ewald_k.f90:195-196) and the slab dipole force
+// -4 pi q M_z inv_V (dipolar_correction.f90:29) to F.
+__global__ void k_ewald_apply(EwaldK e, const int* __restrict__ qidx,
+                              const double* __restrict__ qval, const double* __restrict__ fk_part,
+                              int n_kgroups, const double* __restrict__ mz, double inv_V,
+                              int add_k, int add_dip, double* __restrict__ f) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= e.nq) return;
+  const int i = qidx[idx];
+  const double q = qval[idx];
+  const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
+  if (add_k) {
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int kg = 0; kg < n_kgroups; ++kg)
+      for (int d = 0; d < 3; ++d) s[d] += fk_part[3 * ((size_t)kg * e.nq + idx) + d];
+    for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] += -8.0 * pi_f * q * s[d] * inv_V;
+  }
+  if (add_dip) f[3 * (size_t)i + 2] -= 4.0 * pi_f * q * (*mz) * inv_V;
+}
+
+// u_cou_k = 4 pi inv_V sum g_k |rho_k|^2 (ewald_k.f90:160-170) and the dipole
+// energy 2 pi M_z^2 inv_V (dipolar_correction.f90:32).
+__global__ void k_ewald_energy(const double* __restrict__ uk_part, int nparts,
+                               const double* __restrict__ mz, double inv_V,
+                               double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double pi_f = (double)3.1415926536f;
+  double u = 0.0;
+  for (int b = 0; b < nparts; ++b) u += uk_part[b];
+  out[0] = u * 4.0 * pi_f * inv_V;
+  out[1] = 2.0 * pi_f * (*mz) * (*mz) * inv_V;
 }
 
 // ---------------------------------------------------------------------------
@@ -1128,6 +1386,12 @@ static KParams make_kparams(const mfa_params& p) {
   k.skin_half_2 = p.skin_half_2;
   k.fso_r2_init = p.fso_r2_init;
   k.fso_time = p.fso_time;
+  k.coulomb = p.coulomb;
+  for (int t = 0; t < MFA_NTYPE; ++t) k.q_type[t] = p.charge[t];
+  k.alpha_c = p.alpha_c;
+  k.sqrt_alpha = p.sqrt_alpha;
+  k.inv_sqrt_pi = p.inv_sqrt_pi_real;
+  k.rc_coul_2 = p.coulomb ? p.rc_coul_2 : 0.0;
   {
     // FP32 separation from FP32 positions: the two coordinates, the raw
     // difference (|d| <= L before the image shift) and the FP32 box edge each
@@ -1152,7 +1416,7 @@ static KParams make_kparams(const mfa_params& p) {
       rbmax = p.r_bin[d] > rbmax ? p.r_bin[d] : rbmax;
     }
     const double dsep = (6.0 * bmax + 2.0 * rbmax) * u;
-    double c2max = p.r_cut_dpd_2;
+    double c2max = p.r_cut_dpd_2 > p.rc_coul_2 ? p.r_cut_dpd_2 : p.rc_coul_2;
     for (int t = 0; t < MFA_NTYPE; ++t)
       for (int q = 0; q < MFA_NTYPE; ++q) c2max = p.range_2[t][q] > c2max ? p.range_2[t][q] : c2max;
     k.ff_margin = (float)(4.0 * (2.0 * sqrt(3.0 * c2max) * dsep + 3.0 * dsep * dsep +
@@ -1179,6 +1443,7 @@ static KParams make_kparams(const mfa_params& p) {
 }
 
 static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
+static int launch_ewald(mfa_ctx* c, int add_dip);
 
 static size_t list_lds_bytes(int cell_cap) {
   return (size_t)9 * cell_cap * (sizeof(float4) + 2 * sizeof(int));
@@ -1345,6 +1610,8 @@ static int enqueue_middle_step(mfa_ctx* c) {
   if (rc) return rc;
   rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
   if (rc) return rc;
+  rc = launch_ewald(c, c->p.symmetry == 0);  // md_main.f90:128-132
+  if (rc) return rc;
   return launch_fused(c, 0);
 }
 
@@ -1372,6 +1639,157 @@ static int ensure_graph(mfa_ctx* c) {
   }
   c->gexec_precision = c->precision;
   c->gexec_tpp = c->tpp;
+  return MFA_OK;
+}
+
+// ewald_k(0) (ewald_k.f90:35-146): the half-space k grid inside |k| <= k_c,
+// without k = 0, the lower triangle of the plane kz = 0 and half of its
+// diagonal, in the reference's order (kz, ky outer, kx inner), grouped into
+// runs of consecutive kx; slab period 3 Lz with walls.  Device arrays hold
+// (kx, ky, kz, exp(-k^2/(4 alpha_c))/k^2) per vector and the runs.
+static int ewald_setup(mfa_ctx* c) {
+  const mfa_params& p = c->p;
+  const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
+  double dk[3], z_slab;
+  if (p.symmetry == 1) {
+    for (int d = 0; d < 3; ++d) dk[d] = 2. * pi_f * p.inv_boundary[d];
+    z_slab = 0.;
+  } else {
+    z_slab = 2.;
+    dk[0] = 2. * pi_f * p.inv_boundary[0];
+    dk[1] = 2. * pi_f * p.inv_boundary[1];
+    dk[2] = 2. * pi_f / (p.boundary[2] + z_slab * p.boundary[2]);
+  }
+  const double k_c = (dk[0] + dk[1]) * (double)p.n_kmax / 2.;
+  const int nxm = (int)(k_c / dk[0]), nym = (int)(k_c / dk[1]), nzm = (int)(k_c / dk[2]);
+  const double fac_alpha = 1. / (4. * p.alpha_c);
+  std::vector<double4> kv;
+  std::vector<int4> runs;
+  std::vector<int> run_k0;
+  for (int nkz = 0; nkz <= nzm; ++nkz)
+    for (int nky = -nym; nky <= nym; ++nky) {
+      int open = -1;  // index of the open run
+      for (int nkx = -nxm; nkx <= nxm; ++nkx) {
+        const double ax = (double)nkx * dk[0], ay = (double)nky * dk[1], az = (double)nkz * dk[2];
+        const double rk_2 = ax * ax + ay * ay + az * az;
+        bool take = !(rk_2 > k_c * k_c);
+        if (nkz == 0 && nky == 0 && nkx == 0) take = false;
+        if (nkz == 0 && nkx + nky < 0) take = false;
+        if (nkz == 0 && -nkx == nky && nky > 0) take = false;
+        if (!take) {
+          open = -1;
+          continue;
+        }
+        const double kx = dk[0] * (double)nkx, ky = dk[1] * (double)nky, kz = dk[2] * (double)nkz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        kv.push_back(make_double4(kx, ky, kz, (1. / k2) * exp(-k2 * fac_alpha)));
+        if (open < 0) {
+          runs.push_back(make_int4(nky, nkz, nkx, nkx));
+          run_k0.push_back((int)kv.size() - 1);
+          open = (int)runs.size() - 1;
+        } else {
+          runs[(size_t)open].w = nkx;
+        }
+      }
+    }
+  EwaldK& e = c->ek;
+  memset(&e, 0, sizeof e);
+  for (int d = 0; d < 3; ++d) e.dk[d] = dk[d];
+  e.lz_half = p.boundary[2] / 2.;
+  e.nk = (int)kv.size();
+  e.nruns = (int)runs.size();
+  c->ew_inv_V = p.inv_boundary[0] * p.inv_boundary[1] / (p.boundary[2] * (1. + z_slab));
+  c->ew_sum_blocks = (e.nk + MFA_BLOCK - 1) / MFA_BLOCK;
+  const size_t n = (size_t)c->n;
+  const size_t n_pw_max = (n + 64 * EW_QPT - 1) / (64 * EW_QPT);
+  const size_t nk = (size_t)(e.nk > 0 ? e.nk : 1), nr = (size_t)(e.nruns > 0 ? e.nruns : 1);
+  void** bufs[] = {(void**)&c->d_kvec, (void**)&c->d_runs, (void**)&c->d_run_k0,
+                   (void**)&c->d_qidx, (void**)&c->d_qval, (void**)&c->d_rho_part,
+                   (void**)&c->d_grho, (void**)&c->d_uk_part, (void**)&c->d_mz_part,
+                   (void**)&c->d_mz, (void**)&c->d_fk_part, (void**)&c->d_ek};
+  const size_t bytes[] = {nk * sizeof(double4), nr * sizeof(int4), nr * sizeof(int),
+                          n * sizeof(int), n * sizeof(double), n_pw_max * nk * sizeof(double2),
+                          nk * sizeof(double2), (size_t)(c->ew_sum_blocks + 1) * sizeof(double),
+                          n_pw_max * sizeof(double), sizeof(double),
+                          (2048 * 64 + n) * 3 * sizeof(double), 2 * sizeof(double)};
+  for (int b = 0; b < 12; ++b)
+    if (hipMalloc(bufs[b], bytes[b]) != hipSuccess)
+      return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (Ewald) failed", bytes[b]);
+  if (e.nk > 0) {
+    HIPCHK(nullptr, hipMemcpy(c->d_kvec, kv.data(), kv.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_runs, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_run_k0, run_k0.data(), run_k0.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIPCHK(nullptr, hipMemset(c->d_mz, 0, sizeof(double)));
+  c->ewald = true;
+  return MFA_OK;
+}
+
+// Charged particles of a new state (q(i) = charge(a_type(i)),
+// init_config.f90:221-226), the self term (:235-239) and launch shapes.
+static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
+  const double pi_f = (double)3.1415926536f;
+  std::vector<int> qi;
+  std::vector<double> qv;
+  double self = 0.;
+  for (int i = 0; i < c->n; ++i) {
+    const double q = c->p.charge[a_type[i] - 1];
+    self = self - q * q;
+    if (q != 0.0) {
+      qi.push_back(i);
+      qv.push_back(q);
+    }
+  }
+  c->v_coul_self = self * c->p.sqrt_alpha / sqrt(pi_f);
+  EwaldK& e = c->ek;
+  e.nq = (int)qi.size();
+  if (e.nq == 0 || e.nruns == 0) return MFA_OK;
+  HIPCHK(c, hipMemcpy(c->d_qidx, qi.data(), qi.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_qval, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice));
+  e.n_pw = (e.nq + 64 * EW_QPT - 1) / (64 * EW_QPT);
+  int ng = 2048 / e.n_pw;
+  ng = ng < 1 ? 1 : (ng > e.nruns ? e.nruns : ng);
+  e.runs_per_rgroup = (e.nruns + ng - 1) / ng;
+  c->ew_n_rgroups = (e.nruns + e.runs_per_rgroup - 1) / e.runs_per_rgroup;
+  const int qb = (e.nq + 63) / 64;
+  int kg = 2048 / qb;
+  kg = kg < 1 ? 1 : (kg > e.nruns ? e.nruns : kg);
+  e.runs_per_kgroup = (e.nruns + kg - 1) / kg;
+  c->ew_n_kgroups = (e.nruns + e.runs_per_kgroup - 1) / e.runs_per_kgroup;
+  return MFA_OK;
+}
+
+static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ek.nq > 0 && c->ek.nruns > 0; }
+
+// k-space forces (and, with add_dip, the slab dipole forces) added to F.
+static int launch_ewald(mfa_ctx* c, int add_dip) {
+  if (!ewald_active(c)) return MFA_OK;
+  const EwaldK& e = c->ek;
+  hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_pw, c->ew_n_rgroups), dim3(64), 0, c->stream, e, c->r,
+                     c->d_qidx, c->d_qval, c->d_runs, c->d_run_k0, c->d_rho_part, c->d_mz_part);
+  hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(MFA_BLOCK), 0, c->stream, e,
+                     c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
+  hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups), dim3(64), 0, c->stream,
+                     e, c->r, c->d_qidx, c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
+  hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
+                     c->ew_inv_V, 1, add_dip, c->f);
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
+}
+
+// u_cou_k and the dipole energy of the last launch_ewald / dipole update.
+static int ewald_energies(mfa_ctx* c, double* uk, double* udip) {
+  *uk = *udip = 0.0;
+  if (!ewald_active(c)) return MFA_OK;
+  hipLaunchKernelGGL(k_ewald_energy, dim3(1), dim3(64), 0, c->stream, c->d_uk_part, c->ew_sum_blocks,
+                     c->d_mz, c->ew_inv_V, c->d_ek);
+  HIPCHK(c, hipGetLastError());
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, c->d_ek, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *uk = h[0];
+  *udip = c->p.symmetry == 0 ? h[1] : 0.0;
   return MFA_OK;
 }
 
@@ -1488,6 +1906,13 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
     return set_err(nullptr, MFA_ERR_HIP, "hipStreamCreate failed");
   }
   c->ev_force_used = c->ev_integ_used = 0;
+  if (p->coulomb) {
+    const int rc = ewald_setup(c);
+    if (rc) {
+      mfa_destroy(c);
+      return rc;
+    }
+  }
   hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->d_step, 0, sizeof(long long), c->stream);
@@ -1512,6 +1937,10 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
                   c->mic, c->cell_of,   c->cell_count, c->cell_slots, c->nl, c->nn,
                   c->pos4, c->vel4,     c->flags, c->d_step, c->part,   c->red};
   for (void* q : ptrs)
+    if (q) hipFree(q);
+  void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
+                   c->d_grho,    c->d_uk_part, c->d_mz_part, c->d_mz,  c->d_fk_part, c->d_ek};
+  for (void* q : eptrs)
     if (q) hipFree(q);
   if (c->h_flags) hipHostFree(c->h_flags);
   if (c->h_red) hipHostFree(c->h_red);
@@ -1563,6 +1992,10 @@ extern "C" int32_t mfa_set_state(mfa_ctx* c, const double* r0, const double* v, 
   HIPCHK(c, hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->need_rebuild = true;
+  if (c->ewald) {
+    drop_graph(c);  // captured launches carry the charged-particle count
+    return ewald_set_charges(c, a_type);
+  }
   return MFA_OK;
 }
 
@@ -1683,7 +2116,8 @@ extern "C" int32_t mfa_fluid_fluid(mfa_ctx* c, int64_t step, double* v_ff, doubl
   int rc = force_terms(c, step, TM_PAIR, (v_ff || v_coul) ? t : nullptr);
   if (rc) return rc;
   if (v_ff) *v_ff = t[T_VFF];
-  if (v_coul) *v_coul = t[T_VCOUL];
+  // real space plus the self term (fluid_fluid.f90:238-240)
+  if (v_coul) *v_coul = c->p.coulomb ? t[T_VCOUL] + c->v_coul_self : 0.0;
   return MFA_OK;
 }
 
@@ -1742,6 +2176,53 @@ static void fill_obs(const double* t, double tkin_sum, int64_t nreb, mfa_obs* ob
   obs->n_rebuilds = nreb;
 }
 
+// v_coul of a full force pass: real space + self term + k space + slab dipole.
+static int add_coulomb_energy(mfa_ctx* c, double* t) {
+  if (!c->p.coulomb) {
+    t[T_VCOUL] = 0.0;
+    return MFA_OK;
+  }
+  double uk, udip;
+  const int rc = ewald_energies(c, &uk, &udip);
+  if (rc) return rc;
+  t[T_VCOUL] = t[T_VCOUL] + c->v_coul_self + uk + udip;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_ewald_k(mfa_ctx* c, int32_t mode, double* v_coul) {
+  if (!c) return MFA_ERR_ARG;
+  if (v_coul) *v_coul = 0.0;
+  if (mode == 0 || !ewald_active(c)) return MFA_OK;  // the k grid is set up by mfa_create
+  if (mode != 1) return set_err(c, MFA_ERR_ARG, "ewald_k: mode must be 0 or 1");
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  rc = launch_ewald(c, 0);
+  if (rc) return rc;
+  double uk, udip;
+  rc = ewald_energies(c, &uk, &udip);
+  if (rc) return rc;
+  if (v_coul) *v_coul = uk;
+  return check_flags(c);
+}
+
+extern "C" int32_t mfa_dipolar_correction(mfa_ctx* c, double* v_coul) {
+  if (!c) return MFA_ERR_ARG;
+  if (v_coul) *v_coul = 0.0;
+  if (!ewald_active(c) || c->p.symmetry != 0) return MFA_OK;
+  const EwaldK& e = c->ek;
+  hipLaunchKernelGGL(k_ewald_mz, dim3(1), dim3(MFA_BLOCK), 0, c->stream, e, c->r, c->d_qidx,
+                     c->d_qval, c->d_mz);
+  hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
+                     c->ew_inv_V, 0, 1, c->f);
+  HIPCHK(c, hipGetLastError());
+  double uk, udip;
+  int rc = ewald_energies(c, &uk, &udip);
+  if (rc) return rc;
+  if (v_coul) *v_coul = udip;
+  return check_flags(c);
+}
+
 extern "C" int32_t mfa_energies(mfa_ctx* c, int64_t step, mfa_obs* obs) {
   if (!c || !obs) return MFA_ERR_ARG;
   int rc = ensure_list(c);
@@ -1751,8 +2232,12 @@ extern "C" int32_t mfa_energies(mfa_ctx* c, int64_t step, mfa_obs* obs) {
   // recompute forces into F (the pass overwrites F), then reduce
   rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND | TM_ENERGY);
   if (rc) return rc;
+  rc = launch_ewald(c, c->p.symmetry == 0);
+  if (rc) return rc;
   double t[MFA_NTERMS];
   rc = reduce_terms(c, force_blocks(c), t);
+  if (rc) return rc;
+  rc = add_coulomb_energy(c, t);
   if (rc) return rc;
   // kinetic energy of the current velocities
   std::vector<double> vv((size_t)c->n * 3);
@@ -1801,9 +2286,13 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   if (rc) return rc;
   rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND | (obs ? TM_ENERGY : 0));
   if (rc) return rc;
+  rc = launch_ewald(c, c->p.symmetry == 0);
+  if (rc) return rc;
   double t[MFA_NTERMS], tk[MFA_NTERMS];
   if (obs) {
     rc = reduce_terms(c, force_blocks(c), t);
+    if (rc) return rc;
+    rc = add_coulomb_energy(c, t);
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,

@@ -234,6 +234,14 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   } else {
     p->r_cut_max_2 = p->r_cut_max * p->r_cut_max;
   }
+  // A real-space Coulomb cut-off beyond the LJ range (outside the compat
+  // form) widens the list radius and the cells.
+  int charged_any = 0;
+  for (int t = 0; t < nt; ++t) charged_any |= in->charge[t] != 0.0;
+  if (charged_any && !in->ewald_compat && in->rc_coul > p->r_cut_max) {
+    p->r_cut_max = in->rc_coul;
+    p->r_cut_max_2 = p->r_cut_max * p->r_cut_max;
+  }
   // inv_range_2 (init_params.f90:786-795).
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < nt; ++j)
@@ -246,6 +254,11 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < nt; ++j)
       p->list_2[i][j] = p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : range_2[i][j] + p->skin_2;
+  if (charged_any && !in->ewald_compat && in->rc_coul > 0.0) {
+    const double lc = in->rc_coul * in->rc_coul + p->skin_2;
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) p->list_2[i][j] = p->list_2[i][j] > lc ? p->list_2[i][j] : lc;
+  }
   // DPD (init_params.f90:845).
   p->friction1 = in->friction[0];
   p->sig = sqrt(2.0 * in->temp * in->friction[0] / in->dt);
@@ -478,7 +491,17 @@ extern "C" int32_t mfa_generate(const mfa_params* p, int32_t kind, uint64_t seed
     return true;
   };
 
-  if (kind == 1) {
+  // Polyelectrolyte brush (kind 3): every third bead of a chain (1-based
+  // positions 3, 6, ...) is a charged monomer of type 4; the single-bead
+  // species holds first one counterion (type 3) per charged monomer, then
+  // neutral solvent (type 2).
+  int n_counterions = 0;
+  if (kind == 3) {
+    for (int m = 1; m < p->n_mon; ++m) n_counterions += ((m + 1) % 3 == 0) ? 1 : 0;
+    n_counterions *= p->n_chain;
+    if (p->n_mon_d != 1 || p->n_chain_d < n_counterions) return MFA_ERR_ARG;
+  }
+  if (kind == 1 || kind == 3) {
     // Brush grafted at z = head_z with heads uniform in x,y (gen_brush.f90:9-30
     // style), then solvent/melt as single beads uniform in the channel.
     if (!walls) return MFA_ERR_ARG;
@@ -503,13 +526,14 @@ extern "C" int32_t mfa_generate(const mfa_params* p, int32_t kind, uint64_t seed
           r[0] = wrap(r[0], L[0]);
           r[1] = wrap(r[1], L[1]);
           if (place_ok(r, min_dist - 0.05 * (double)(tries / 1000))) {
-            put(c * p->n_mon + m, r, 2);
+            put(c * p->n_mon + m, r, (kind == 3 && (m + 1) % 3 == 0) ? 4 : 2);
             ok = true;
           }
         }
         if (!ok) return MFA_ERR_ARG;
       }
-    check_overlap = p->n_mon_d > 1;  // single-bead solvent is ideal (epsilon 0)
+    // single-bead solvent is ideal (epsilon 0); counterions keep their distance
+    check_overlap = p->n_mon_d > 1 || kind == 3;
     for (int c = 0; c < p->n_chain_d; ++c) {
       const int first = p->part_init_d + c * p->n_mon_d;
       int attempt = 0;
@@ -517,7 +541,8 @@ extern "C" int32_t mfa_generate(const mfa_params* p, int32_t kind, uint64_t seed
         if (attempt > 100000) return MFA_ERR_ARG;
         double h[3] = {s.uniform() * L[0], s.uniform() * L[1], zlo + s.uniform() * (zhi - zlo)};
         if (!place_ok(h, min_dist)) continue;
-        if (grow(first, p->n_mon_d, h, 3, 3, false)) break;
+        const int t = kind == 3 ? (c < n_counterions ? 3 : 2) : 3;
+        if (grow(first, p->n_mon_d, h, t, t, false)) break;
       }
     }
   } else if (kind == 2) {

@@ -46,6 +46,17 @@ contains
   subroutine intra_molec()
     call mfa_check(mfa_intra_molec(ctx, v_intra_molec), "intra_molec", ctx)
   end subroutine
+  subroutine ewald_k(mode)
+    integer, intent(in) :: mode
+    real(c_double) :: dv
+    call mfa_check(mfa_ewald_k(ctx, int(mode, c_int32_t), dv), "ewald_k", ctx)
+    v_coul = v_coul + dv
+  end subroutine
+  subroutine dipolar_correction()
+    real(c_double) :: dv
+    call mfa_check(mfa_dipolar_correction(ctx, dv), "dipolar_correction", ctx)
+    v_coul = v_coul + dv
+  end subroutine
   subroutine verlet_velocities()
     call mfa_check(mfa_verlet_velocities(ctx, t_fluid), "verlet_velocities", ctx)
   end subroutine
@@ -144,6 +155,10 @@ program md_pb_hip
       call fluid_fluid(step)
       if (inp%symmetry == 0) call fluid_wall(2)
       call intra_molec()
+      if (inp%system == 2 .or. inp%system == 3) then   ! md_main.f90:128-132
+        call ewald_k(1)
+        if (inp%symmetry == 0) call dipolar_correction()
+      end if
       call verlet_velocities()
       if (mod(i_time, 10) == 0) call write_fort60(i_time, v_fluid_wall, v_fluid_fluid, &
                                                    v_intra_molec, t_fluid, v_coul)

@@ -128,6 +128,17 @@ module mfa_hip_bind
       integer(c_int64_t), value :: step
       real(c_double) :: v_ff, v_coul
     end function
+    integer(c_int32_t) function mfa_ewald_k(ctx, mode, v_coul) bind(C, name="mfa_ewald_k")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+      real(c_double) :: v_coul
+    end function
+    integer(c_int32_t) function mfa_dipolar_correction(ctx, v_coul) bind(C, name="mfa_dipolar_correction")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: v_coul
+    end function
     integer(c_int32_t) function mfa_fluid_wall(ctx, inter_type, v_fw) bind(C, name="mfa_fluid_wall")
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

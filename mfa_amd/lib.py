@@ -115,6 +115,8 @@ SIGNATURES = {
     "mfa_fluid_fluid": (C.c_int32, [_P, C.c_int64, _dp, _dp]),
     "mfa_fluid_wall": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_intra_molec": (C.c_int32, [_P, _dp]),
+    "mfa_ewald_k": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_dipolar_correction": (C.c_int32, [_P, _dp]),
     "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
@@ -308,6 +310,16 @@ class Context:
     def intra_molec(self):
         e = C.c_double()
         self._chk(self.lib.mfa_intra_molec(self.h, C.byref(e)), "mfa_intra_molec")
+        return e.value
+
+    def ewald_k(self, mode=1):
+        e = C.c_double()
+        self._chk(self.lib.mfa_ewald_k(self.h, mode, C.byref(e)), "mfa_ewald_k")
+        return e.value
+
+    def dipolar_correction(self):
+        e = C.c_double()
+        self._chk(self.lib.mfa_dipolar_correction(self.h, C.byref(e)), "mfa_dipolar_correction")
         return e.value
 
     def verlet_velocities(self):

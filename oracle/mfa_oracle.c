@@ -27,7 +27,16 @@ struct orc_sys {
   int32_t* cell_of;    /* [N][3] */
   int f_skin;
   int n_rebuilds;
+  /* Coulomb: q(:) (init_config.f90:221-226), self term (:235-239) and the
+   * k-space tables of ewald_k(0) (ewald_k.f90:35-146) */
+  double* q;
+  double v_coul_self;
+  int nk;
+  double *k_vecs, *k_vecs_2, *inv_k_vecs_2, *re_rho_k, *im_rho_k;
+  double inv_V, lz_half, fac_alpha;
 };
+
+static const double PI_F = (double)3.1415926536f; /* mfa_common.f90:332, a default-real literal */
 
 /* ------------------------------------------------------------------------ */
 /* Threefry-2x32 (Salmon et al., SC11).  Pinned by the Random123 known-answer
@@ -195,6 +204,14 @@ int orc_derive(const mfa_inputs* in, orc_params* p) {
     p->r_cut_dpd_2 = p->r_cut_dpd * p->r_cut_dpd;
     p->r_cut_max_2 = p->r_cut_max * p->r_cut_max;
   }
+  { /* a real-space Coulomb cut-off beyond the LJ range widens list and cells */
+    int charged = 0;
+    for (int t = 0; t < nt; ++t) charged |= in->charge[t] != 0.0;
+    if (charged && !in->ewald_compat && in->rc_coul > p->r_cut_max) {
+      p->r_cut_max = in->rc_coul;
+      p->r_cut_max_2 = p->r_cut_max * p->r_cut_max;
+    }
+  }
   for (int i = 0; i < nt; ++i) /* :786-795 */
     for (int j = 0; j < nt; ++j)
       p->inv_range_2[i][j] = p->range_2[i][j] != 0 ? 1. / p->range_2[i][j] : 0.;
@@ -217,6 +234,21 @@ int orc_derive(const mfa_inputs* in, orc_params* p) {
     p->r_bin[d] = p->boundary[d] / (double)p->n_bin[d];
     p->inv_r_bin[d] = 1. / p->r_bin[d];
   }
+  /* Coulomb constants (init_system.f90:83-98, init_config.f90:221-239).  The
+   * reference's real-space force uses inv_sqrt_pi = 1./pi (:230); the compat
+   * form keeps it, the default uses 1/sqrt(pi).  The reference evaluates the
+   * real space only inside the LJ range (fluid_fluid.f90:94,166); without
+   * ewald_compat and with rc_coul set, it is evaluated within rc_coul. */
+  for (int t = 0; t < nt; ++t) {
+    p->charge[t] = in->charge[t];
+    if (in->charge[t] != 0.0) p->coulomb = 1;
+  }
+  p->ewald_compat = in->ewald_compat;
+  p->alpha_c = in->alpha_c;
+  p->sqrt_alpha = sqrt(in->alpha_c);
+  p->inv_sqrt_pi = in->ewald_compat ? 1. / PI_F : 1. / sqrt(3.14159265358979323846);
+  p->n_kmax = in->n_kmax;
+  p->rc_coul_2 = (!in->ewald_compat && in->rc_coul > 0.0) ? in->rc_coul * in->rc_coul : 0.0;
   /* init_config.f90:264-281 and init_force_switch_on.f90:15-26 */
   if (in->force_switch_on && in->s_time == 1) {
     double maxsig = 0.;
@@ -230,6 +262,69 @@ int orc_derive(const mfa_inputs* in, orc_params* p) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* ewald_k(0) (ewald_k.f90:35-146): the k grid of the half space kz >= 0
+ * inside the sphere |k| <= k_c, without k = 0, the lower triangle of the
+ * plane kz = 0 and half of its diagonal; with walls (SYMMETRY 0) the z
+ * period is the slab 3 Lz (z_slab = 2).  Also inv_V (:126) and lz_half
+ * (init_config.f90:246).  Pinned by the compiled reference (charged brush),
+ * by the count of a brute enumeration and by Madelung sums. */
+static void ewald_init(orc_sys* s) {
+  const orc_params* p = &s->p;
+  double delta_k[3], z_slab;
+  if (p->symmetry == 1) {
+    for (int d = 0; d < 3; ++d) delta_k[d] = 2. * PI_F * p->inv_boundary[d];
+    z_slab = 0.;
+  } else {
+    z_slab = 2.;
+    delta_k[0] = 2. * PI_F * p->inv_boundary[0];
+    delta_k[1] = 2. * PI_F * p->inv_boundary[1];
+    delta_k[2] = 2. * PI_F / (p->boundary[2] + z_slab * p->boundary[2]);
+  }
+  const double k_c = (delta_k[0] + delta_k[1]) * (double)p->n_kmax / 2.;
+  const int nkx_max = (int)(k_c / delta_k[0]);
+  const int nky_max = (int)(k_c / delta_k[1]);
+  const int nkz_max = (int)(k_c / delta_k[2]);
+  for (int pass = 0; pass < 2; ++pass) {
+    int k_index = 0;
+    for (int nkz = 0; nkz <= nkz_max; ++nkz)
+      for (int nky = -nky_max; nky <= nky_max; ++nky)
+        for (int nkx = -nkx_max; nkx <= nkx_max; ++nkx) {
+          const double ax = (double)nkx * delta_k[0], ay = (double)nky * delta_k[1],
+                       az = (double)nkz * delta_k[2];
+          const double rk_2 = ax * ax + ay * ay + az * az;
+          if (rk_2 > k_c * k_c) continue;
+          if (nkz == 0 && nky == 0 && nkx == 0) continue;
+          if (nkz == 0 && nkx + nky < 0) continue;
+          if (nkz == 0 && -nkx == nky && nky > 0) continue;
+          if (pass == 1) {
+            double* kv = s->k_vecs + 3 * (size_t)k_index;
+            kv[0] = delta_k[0] * (double)nkx;
+            kv[1] = delta_k[1] * (double)nky;
+            kv[2] = delta_k[2] * (double)nkz;
+            s->k_vecs_2[k_index] = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+            s->inv_k_vecs_2[k_index] = 1. / s->k_vecs_2[k_index];
+          }
+          ++k_index;
+        }
+    if (pass == 0) {
+      s->nk = k_index;
+      s->k_vecs = (double*)calloc(3 * (size_t)k_index + 3, sizeof(double));
+      s->k_vecs_2 = (double*)calloc((size_t)k_index + 1, sizeof(double));
+      s->inv_k_vecs_2 = (double*)calloc((size_t)k_index + 1, sizeof(double));
+      s->re_rho_k = (double*)calloc((size_t)k_index + 1, sizeof(double));
+      s->im_rho_k = (double*)calloc((size_t)k_index + 1, sizeof(double));
+    }
+  }
+  s->fac_alpha = 1. / (4. * p->alpha_c);
+  s->inv_V = p->inv_boundary[0] * p->inv_boundary[1] / (p->boundary[2] * (1. + z_slab));
+  s->lz_half = p->boundary[2] / 2.;
+}
+
+int orc_ewald_nk(orc_sys* s, double* kvecs) {
+  if (kvecs && s->nk) memcpy(kvecs, s->k_vecs, 3 * (size_t)s->nk * sizeof(double));
+  return s->nk;
+}
+
 orc_sys* orc_create(const orc_params* p, int n_neigh) {
   orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
   s->p = *p;
@@ -249,6 +344,8 @@ orc_sys* orc_create(const orc_params* p, int n_neigh) {
   const size_t nc = (size_t)p->n_bin[0] * p->n_bin[1] * p->n_bin[2];
   s->bin_fluid = (int32_t*)calloc(nc * (size_t)(s->n_bin_fl + 1), sizeof(int32_t));
   s->cell_of = (int32_t*)calloc(3 * n, sizeof(int32_t));
+  s->q = (double*)calloc(n, sizeof(double));
+  if (p->coulomb) ewald_init(s);
   return s;
 }
 
@@ -257,6 +354,8 @@ void orc_destroy(orc_sys* s) {
   free(s->r0); free(s->v); free(s->a); free(s->force); free(s->r0_old);
   free(s->a_type); free(s->mic); free(s->mic_old); free(s->ff_list); free(s->bin_fluid);
   free(s->cell_of);
+  free(s->q);
+  free(s->k_vecs); free(s->k_vecs_2); free(s->inv_k_vecs_2); free(s->re_rho_k); free(s->im_rho_k);
   free(s);
 }
 
@@ -270,6 +369,13 @@ void orc_set_state(orc_sys* s, const double* r0, const double* v, const double* 
   if (mic) memcpy(s->mic, mic, 3 * n * sizeof(int32_t));
   else memset(s->mic, 0, 3 * n * sizeof(int32_t));
   s->f_skin = 1;
+  /* q(i) = charge(a_type(i)) and the self term (init_config.f90:221-239) */
+  s->v_coul_self = 0.;
+  for (size_t i = 0; i < n; ++i) {
+    s->q[i] = s->p.coulomb ? s->p.charge[a_type[i] - 1] : 0.;
+    s->v_coul_self = s->v_coul_self - s->q[i] * s->q[i];
+  }
+  s->v_coul_self = s->v_coul_self * s->p.sqrt_alpha / sqrt(PI_F);
 }
 
 void orc_get_state(orc_sys* s, double* r0, double* v, double* a, double* force, int32_t* mic) {
@@ -325,7 +431,9 @@ int orc_check_skin(orc_sys* s) {
 static double pbc(double d, double L, double invL) { return d - L * (double)(int)(2 * d * invL); }
 
 static double list_crit(const orc_params* p, int ti, int tj) {
-  return p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : p->range_2[ti][tj] + p->skin_2;
+  double l2 = p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : p->range_2[ti][tj] + p->skin_2;
+  if (p->coulomb && p->rc_coul_2 + p->skin_2 > l2) l2 = p->rc_coul_2 + p->skin_2;
+  return l2;
 }
 
 /* binning.f90:11-127: cell fill in particle order, then the 27-cell scan with
@@ -460,6 +568,25 @@ int64_t orc_brute_pairs(orc_sys* s, int32_t* pairs, int64_t cap) {
 
 void orc_zero_force(orc_sys* s) { memset(s->force, 0, 3 * (size_t)s->n * sizeof(double)); }
 
+/* ewald_real.f90:15-32 for the pair (ip, jp) at r_2 (after the switch-on
+ * clamp) and separation dr, in the reference's operation order.  Pinned by
+ * the compiled reference (charged brush, compat form) and by the Madelung
+ * constant of a rock-salt crystal (correct form, with ewald_k). */
+static void ewald_real(orc_sys* s, int ip, int jp, double r_2, const double* dr, double* v_coul) {
+  const orc_params* p = &s->p;
+  const double r = sqrt(r_2);
+  const double inv_r = 1. / r;
+  const double qq = s->q[ip] * s->q[jp];
+  *v_coul = *v_coul + qq * erfc(p->sqrt_alpha * r) * inv_r;
+  const double f_scal = qq * p->sqrt_alpha * p->inv_sqrt_pi * 2. * exp(-p->alpha_c * r_2) * inv_r +
+                        qq * erfc(p->sqrt_alpha * r) * inv_r * inv_r;
+  for (int d = 0; d < 3; ++d) {
+    const double fc = f_scal * dr[d] * inv_r;
+    F(ip, d) = F(ip, d) + fc;
+    F(jp, d) = F(jp, d) - fc;
+  }
+}
+
 /* fluid_fluid.f90:52-223 with dpd_forces.f90:256-352 (DPD_WEIGHT 0,
  * BIN_TYPE 0).  LJ is pinned by closed forms (minimum at 2^(1/6) sigma,
  * shifted energy zero at the cut-off) and energy/force finite differences;
@@ -469,7 +596,7 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
   const orc_params* p = &s->p;
   const int ls = s->n_neigh + 1;
   const int head_excl = p->symmetry == 0 && (p->system == 0 || p->system == 1 || p->system == 3);
-  double v_ff = 0., vir = 0.;
+  double v_ff = 0., vir = 0., v_coul = 0.;
   /* force switch-on (fluid_fluid.f90:42-49), r_time = step*dt (md_main.f90:69) */
   double r_2_min = 0.;
   const double r_time = (double)step * p->dt;
@@ -514,6 +641,10 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
           w_arg = r_2 * p->inv_range_2[ti][tj];
         }
       }
+      /* ewald_real (fluid_fluid.f90:166, ewald_real.f90:15-32): inside the LJ
+       * range with the clamped r_2 (compat), or within rc_coul */
+      const int real_space = p->coulomb && (p->rc_coul_2 > 0. ? r_2_raw < p->rc_coul_2 : r_2 < r_cut2);
+      if (real_space && !do_dpd) ewald_real(s, ip, jp, r_2, dr, &v_coul);
       if (p->thermostat == 0 && p->dpd_cut_off_on) {
         if (head_excl && (ti == 0 || tj == 0)) continue;
         if (r_2 < p->r_cut_dpd_2) {
@@ -540,12 +671,59 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
           F(ip, d) = F(ip, d) + vd[d];
           F(jp, d) = F(jp, d) - vd[d];
         }
+        /* the reference calls ewald_real after dpd_forces inside the LJ branch */
+        if (real_space && !p->dpd_cut_off_on) ewald_real(s, ip, jp, r_2, dr, &v_coul);
       }
     }
     for (int d = 0; d < 3; ++d) F(ip, d) = F(ip, d) + f_ipart[d];
   }
   out[0] = v_ff;
   out[1] = vir;
+  out[2] = p->coulomb ? v_coul + s->v_coul_self : 0.; /* fluid_fluid.f90:238-240 */
+}
+
+/* ewald_k(1) (ewald_k.f90:150-199): structure factor over the charged
+ * particles, u_cou_k, and the k-space forces added to F.  Pinned with
+ * ewald_init. */
+double orc_ewald_k(orc_sys* s) {
+  const orc_params* p = &s->p;
+  if (!p->coulomb || s->nk == 0) return 0.;
+  const int nk = s->nk;
+  for (int k = 0; k < nk; ++k) s->re_rho_k[k] = s->im_rho_k[k] = 0.;
+  for (int i = 0; i < s->n; ++i)
+    for (int k = 0; k < nk; ++k) {
+      const double* kv = s->k_vecs + 3 * (size_t)k;
+      const double lphase = kv[0] * R(i, 0) + kv[1] * R(i, 1) + kv[2] * R(i, 2);
+      s->re_rho_k[k] = s->re_rho_k[k] + s->q[i] * cos(lphase);
+      s->im_rho_k[k] = s->im_rho_k[k] + s->q[i] * sin(lphase);
+    }
+  double u_cou_k = 0.;
+  for (int k = 0; k < nk; ++k)
+    u_cou_k = u_cou_k + s->inv_k_vecs_2[k] * exp(-s->k_vecs_2[k] * s->fac_alpha) *
+                            (s->re_rho_k[k] * s->re_rho_k[k] + s->im_rho_k[k] * s->im_rho_k[k]);
+  u_cou_k = u_cou_k * 4. * PI_F * s->inv_V;
+  for (int i = 0; i < s->n; ++i) {
+    double f_k[3] = {0., 0., 0.};
+    for (int k = 0; k < nk; ++k) {
+      const double* kv = s->k_vecs + 3 * (size_t)k;
+      const double cphase = kv[0] * R(i, 0) + kv[1] * R(i, 1) + kv[2] * R(i, 2);
+      const double t = cos(cphase) * s->im_rho_k[k] - sin(cphase) * s->re_rho_k[k];
+      for (int d = 0; d < 3; ++d) f_k[d] = f_k[d] + kv[d] * s->inv_k_vecs_2[k] *
+                                                     exp(-s->k_vecs_2[k] * s->fac_alpha) * t;
+    }
+    for (int d = 0; d < 3; ++d) F(i, d) = F(i, d) + -8. * PI_F * s->q[i] * f_k[d] * s->inv_V;
+  }
+  return u_cou_k;
+}
+
+/* dipolar_correction.f90:20-33 (slab geometry). */
+double orc_dipolar(orc_sys* s) {
+  const orc_params* p = &s->p;
+  if (!p->coulomb || p->symmetry != 0) return 0.;
+  double m_z = 0.;
+  for (int i = 0; i < s->n; ++i) m_z = m_z + s->q[i] * (R(i, 2) - s->lz_half);
+  for (int i = 0; i < s->n; ++i) F(i, 2) = F(i, 2) - 4. * PI_F * s->q[i] * m_z * s->inv_V;
+  return 2. * PI_F * m_z * m_z * s->inv_V;
 }
 
 static double ipow(double x, int n) { /* square-and-multiply, lowest bit first */
@@ -682,7 +860,7 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
     if (rc) return rc;
   }
   const int reb0 = s->n_rebuilds; /* count the rebuilds inside the loop only */
-  double e_ff[2] = {0, 0}, e_fw[3] = {0, 0, 0}, e_in[2] = {0, 0}, tk = 0.;
+  double e_ff[3] = {0, 0, 0}, e_fw[3] = {0, 0, 0}, e_in[2] = {0, 0}, tk = 0., v_coul = 0.;
   for (int it = 0; it < n; ++it) {
     const int64_t step = first_step + it;
     orc_verlet_positions(s);
@@ -694,6 +872,11 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
     orc_fluid_fluid(s, step, e_ff);
     if (s->p.symmetry == 0) orc_fluid_wall(s, e_fw);
     orc_intra_molec(s, e_in);
+    v_coul = e_ff[2];
+    if (s->p.coulomb) { /* md_main.f90:128-132 */
+      v_coul = v_coul + orc_ewald_k(s);
+      if (s->p.symmetry == 0) v_coul = v_coul + orc_dipolar(s);
+    }
     tk = orc_verlet_velocities(s);
   }
   if (obs) {
@@ -704,6 +887,7 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
     obs[4] = e_ff[1] + e_in[1];
     obs[5] = (double)(s->n_rebuilds - reb0);
     obs[6] = e_fw[1];
+    obs[7] = v_coul;
   }
   return 0;
 }

@@ -43,6 +43,9 @@ typedef struct orc_params {
   double r_bin[3], inv_r_bin[3];
   uint32_t seed;
   double r_2_min_init, r_2_min_time; /* FORCE_SWITCH_ON ramp, 0 when off */
+  /* Coulomb (SYSTEM 2/3): init_system.f90:83-98, init_config.f90:221-239 */
+  int coulomb, ewald_compat, n_kmax;
+  double charge[ORC_NT], alpha_c, sqrt_alpha, inv_sqrt_pi, rc_coul_2;
 } orc_params;
 
 typedef struct orc_sys orc_sys;
@@ -65,15 +68,22 @@ int orc_binning(orc_sys* s); /* 0 or MFA_ERR_LIST_OVERFLOW / MFA_ERR_CELL */
 void orc_get_cells(orc_sys* s, int32_t* cells);
 int64_t orc_get_pairs(orc_sys* s, int32_t* pairs, int64_t cap);
 void orc_zero_force(orc_sys* s);
-/* energies: out[0]=v_fluid_fluid, out[1]=virial contribution */
+/* energies: out[0]=v_fluid_fluid, out[1]=virial contribution,
+ * out[2]=v_coul (real space + self term) */
 void orc_fluid_fluid(orc_sys* s, int64_t step, double* out);
+/* ewald_k(1): adds the k-space forces; returns u_cou_k.  orc_ewald_nk gives
+ * the number of k vectors and (optionally) copies them (nk x 3). */
+double orc_ewald_k(orc_sys* s);
+int orc_ewald_nk(orc_sys* s, double* kvecs);
+/* dipolar_correction: adds the slab forces; returns its energy. */
+double orc_dipolar(orc_sys* s);
 /* out[0]=v_fluid_wall, out[1]=sum Fz bottom, out[2]=sum Fz top */
 void orc_fluid_wall(orc_sys* s, double* out);
 /* out[0]=v_intra_molec, out[1]=virial contribution */
 void orc_intra_molec(orc_sys* s, double* out);
 double orc_verlet_velocities(orc_sys* s);
-/* Loop body of md_main.f90:69-155 for n steps.  obs[0..6]: v_ff, v_fw,
- * v_intra, t_kin, virial, n_rebuilds, wall Fz bottom of the last step. */
+/* Loop body of md_main.f90:69-155 for n steps.  obs[0..7]: v_ff, v_fw,
+ * v_intra, t_kin, virial, n_rebuilds, wall Fz bottom, v_coul of the last step. */
 int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs);
 /* Brute-force O(N^2) list criterion (for pinning the cell scan). */
 int64_t orc_brute_pairs(orc_sys* s, int32_t* pairs, int64_t cap);

@@ -40,6 +40,9 @@ class OrcParams(C.Structure):
         ("va_spring_twall", _d3),
         ("n_bin", _i3), ("delta_bin", _i3), ("r_bin", _d3), ("inv_r_bin", _d3),
         ("seed", C.c_uint32), ("r_2_min_init", C.c_double), ("r_2_min_time", C.c_double),
+        ("coulomb", C.c_int), ("ewald_compat", C.c_int), ("n_kmax", C.c_int),
+        ("charge", _d4), ("alpha_c", C.c_double), ("sqrt_alpha", C.c_double),
+        ("inv_sqrt_pi", C.c_double), ("rc_coul_2", C.c_double),
     ]
 
 
@@ -90,6 +93,12 @@ def load():
     lib.orc_verlet_velocities.restype = C.c_double
     lib.orc_run_steps.argtypes = [P, C.c_int64, C.c_int, dp]
     lib.orc_run_steps.restype = C.c_int
+    lib.orc_ewald_k.argtypes = [P]
+    lib.orc_ewald_k.restype = C.c_double
+    lib.orc_ewald_nk.argtypes = [P, dp]
+    lib.orc_ewald_nk.restype = C.c_int
+    lib.orc_dipolar.argtypes = [P]
+    lib.orc_dipolar.restype = C.c_double
     _lib = lib
     return lib
 
@@ -187,7 +196,7 @@ class System:
         self.lib.orc_zero_force(self.h)
 
     def fluid_fluid(self, step):
-        out = np.zeros(2)
+        out = np.zeros(3)
         self.lib.orc_fluid_fluid(self.h, step, _dp(out))
         return out
 
@@ -210,7 +219,21 @@ class System:
         if rc:
             raise RuntimeError("orc_run_steps error %d" % rc)
         return {"v_fluid_fluid": obs[0], "v_fluid_wall": obs[1], "v_intra_molec": obs[2],
-                "t_kin": obs[3], "virial": obs[4], "n_rebuilds": obs[5], "wall_fz_bottom": obs[6]}
+                "t_kin": obs[3], "virial": obs[4], "n_rebuilds": obs[5], "wall_fz_bottom": obs[6],
+                "v_coul": obs[7]}
+
+    def ewald_k(self):
+        return self.lib.orc_ewald_k(self.h)
+
+    def dipolar(self):
+        return self.lib.orc_dipolar(self.h)
+
+    def k_vectors(self):
+        nk = self.lib.orc_ewald_nk(self.h, None)
+        kv = np.zeros((nk, 3))
+        if nk:
+            self.lib.orc_ewald_nk(self.h, _dp(kv))
+        return kv
 
     def forces(self, step):
         """Total force of one step's force pass at the current state
@@ -219,6 +242,12 @@ class System:
         e_ff = self.fluid_fluid(step)
         e_fw = self.fluid_wall() if self.p.symmetry == 0 else np.zeros(3)
         e_in = self.intra_molec()
+        v_coul = e_ff[2]
+        if self.p.coulomb:  # md_main.f90:128-132
+            v_coul += self.ewald_k()
+            if self.p.symmetry == 0:
+                v_coul += self.dipolar()
         return self.get_state()["f"], {"v_fluid_fluid": e_ff[0], "v_fluid_wall": e_fw[0],
                                        "v_intra_molec": e_in[0], "virial": e_ff[1] + e_in[1],
-                                       "wall_fz_bottom": e_fw[1], "wall_fz_top": e_fw[2]}
+                                       "wall_fz_bottom": e_fw[1], "wall_fz_top": e_fw[2],
+                                       "v_coul": v_coul}

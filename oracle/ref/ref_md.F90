@@ -10,11 +10,11 @@
 ! writes
 !   ref_init.bin   the state, the cell of every particle, the pair list of
 !                  the first binning, and one force pass at r_time =
-!                  force_step*dt (md_main.f90:84-106 without integrating);
+!                  force_step*dt (md_main.f90:84-132 without integrating);
 !   ref_final.bin  the state after n_steps of the main loop
 !                  (md_main.f90:68-147, observation and storage left out).
 ! Record layout (stream, native endianness): int32 n_part, int32 n_rebuilds,
-! float64 v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, then
+! float64 v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, v_coul, then
 ! r0, v, a, force as float64 (3, n_part) and mic_count as int32 (3, n_part);
 ! ref_init.bin adds the cells int32 (3, n_part), int32 n_pairs and the pairs
 ! int32 (2, n_pairs), each pair as (min, max).
@@ -55,6 +55,7 @@ program ref_md
   call fluid_wall(wall_flag)
 #endif
   call intra_molec()
+  call coulomb_k()
   open(newunit=u, file="ref_init.bin", access="stream", form="unformatted", status="replace")
   call put_state(u, 0)
   write(u) cells
@@ -82,6 +83,7 @@ program ref_md
     call fluid_wall(wall_flag)
 #endif
     call intra_molec
+    call coulomb_k()
     call verlet_velocities()
   end do
   open(newunit=u, file="ref_final.bin", access="stream", form="unformatted", status="replace")
@@ -90,10 +92,20 @@ program ref_md
 
 contains
 
+  ! k-space Ewald and the slab correction of charged systems (md_main.f90:128-132)
+  subroutine coulomb_k()
+#if SYSTEM == 2 || SYSTEM == 3
+    call ewald_k(1)
+#  if SYMMETRY == 0
+    call dipolar_correction()
+#  endif
+#endif
+  end subroutine coulomb_k
+
   subroutine put_state(unit, nreb)
     integer, intent(in) :: unit, nreb
     write(unit) n_part, nreb
-    write(unit) v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid
+    write(unit) v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, v_coul
     write(unit) r0(:, 1:n_part), v(:, 1:n_part), a(:, 1:n_part), force(:, 1:n_part)
     write(unit) mic_count(:, 1:n_part)
   end subroutine put_state

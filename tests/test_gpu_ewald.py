@@ -1,0 +1,47 @@
+"""Ewald sum on the GPU (real space in the pair pass, k space and the slab
+dipole correction as FP64 kernels) against the oracle and the Madelung
+constant of rock salt."""
+import numpy as np
+import pytest
+
+from mfa_amd import lib
+from oracle import oracle as orc
+from tests.test_oracle import _rock_salt
+from tests.util import force_error, gpu_ctx, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", [lib.MFA_FP64, lib.MFA_MIXED])
+def test_madelung_rock_salt(precision):
+    x, r, t = _rock_salt(8, 1.5, 1.96, 20, 3.0)
+    p = lib.derive(x)
+    print("rock salt", len(t), "ions, precision", precision, flush=True)
+    ctx = gpu_ctx(p, r, np.zeros_like(r), np.zeros_like(r), t, precision=precision)
+    obs = ctx.energies(step=1)
+    ctx.close()
+    expect = -(len(t) / 2) * 1.747564594633 / 1.5
+    tol = 1e-6 if precision == lib.MFA_FP64 else 1e-5
+    assert abs(obs["v_coul"] - expect) < tol * abs(expect), (obs["v_coul"], expect)
+
+
+@pytest.mark.parametrize("precision", [lib.MFA_FP64, lib.MFA_MIXED])
+def test_displaced_rock_salt_matches_oracle(precision):
+    x, r, t = _rock_salt(8, 1.5, 1.44, 20, 3.0)
+    rng = np.random.default_rng(5)
+    r = r + rng.uniform(-0.2, 0.2, r.shape)
+    s = orc.System(x)
+    s.set_state(r, np.zeros_like(r), np.zeros_like(r), t)
+    s.binning()
+    fref, eref = s.forces(step=1)
+    p = lib.derive(x)
+    ctx = gpu_ctx(p, r, np.zeros_like(r), np.zeros_like(r), t, precision=precision)
+    obs = ctx.energies(step=1)
+    f = ctx.get_state()["f"]
+    ctx.close()
+    per, rms = force_error(f, fref)
+    print("displaced rock salt precision", precision, "max", per, "rms", rms)
+    if precision == lib.MFA_FP64:
+        assert per < 1e-12 and rel(obs["v_coul"], eref["v_coul"]) < 1e-12
+    else:
+        assert rms < 1e-5 and rel(obs["v_coul"], eref["v_coul"]) < 1e-6

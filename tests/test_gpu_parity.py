@@ -13,7 +13,8 @@ from tests.util import force_error, gpu_ctx, make_case, oracle_sys, rel
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["brush_small", "melt_small", "melt_wca", "melt_switch_on"]
+CASES = ["brush_small", "melt_small", "melt_wca", "melt_switch_on", "pe_brush_compat",
+         "pe_brush_small"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -49,6 +50,8 @@ def test_forces_fp64(case):
     if p.symmetry == 0:
         assert rel(obs["v_fluid_wall"], eref["v_fluid_wall"]) < 1e-12
         assert rel(obs["wall_fz_bottom"], eref["wall_fz_bottom"]) < 1e-12
+    if p.coulomb:
+        assert rel(obs["v_coul"], eref["v_coul"]) < 1e-12, (obs["v_coul"], eref["v_coul"])
     ctx.close()
 
 
@@ -63,9 +66,14 @@ def test_subroutine_entry_points_fp64(case):
     ctx = gpu_ctx(p, r, v, a, t)
     ctx.binning()
     ctx.zero_force()
-    vff, _ = ctx.fluid_fluid(3)
+    vff, vc = ctx.fluid_fluid(3)
     vfw = ctx.fluid_wall(2)
     vin = ctx.intra_molec()
+    if p.coulomb:  # md_main.f90:128-132
+        vc += ctx.ewald_k(1)
+        if p.symmetry == 0:
+            vc += ctx.dipolar_correction()
+        assert rel(vc, eref["v_coul"]) < 1e-12, (vc, eref["v_coul"])
     f = ctx.get_state()["f"]
     per, _ = force_error(f, fref)
     assert per < 1e-12
@@ -138,6 +146,8 @@ def test_trajectory_fp64(case):
     assert obs["n_rebuilds"] == ref["n_rebuilds"]
     for k in ("v_fluid_fluid", "v_intra_molec", "t_kin"):
         assert rel(obs[k], ref[k]) < 1e-9, (k, obs[k], ref[k])
+    if p.coulomb:
+        assert rel(obs["v_coul"], ref["v_coul"]) < 1e-9, (obs["v_coul"], ref["v_coul"])
     ctx.close()
 
 

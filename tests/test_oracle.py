@@ -334,3 +334,88 @@ def test_generators_are_seeded_and_valid():
     chain_bonds = bonds[(np.arange(1, p.n_part) % p.n_mon_d) != 0]
     assert np.allclose(chain_bonds, 0.97, atol=1e-12)
     assert np.abs(v1.sum(0)).max() < 1e-9
+
+
+def _rock_salt(n_side, a0, alpha_c, n_kmax, rc):
+    """Rock-salt crystal of n_side^3 unit charges (+1 type 3, -1 type 4) at
+    nearest-neighbour distance a0 in a periodic cube, Ewald in the corrected
+    form (1/sqrt(pi), real space within rc), no LJ or DPD forces."""
+    from mfa_amd import lib
+    x = configs.base_inputs()
+    x.system, x.symmetry = 2, 1
+    x.dt = 0.005
+    L = n_side * a0
+    x.n_cell_w_x = x.n_cell_w_y = n_side
+    x.x_space = x.y_space = a0
+    x.z_space_wall = L
+    x.n_mon = x.n_chain = 0
+    x.n_mon_d, x.n_chain_d = 1, n_side ** 3
+    x.particle_4, x.inter4 = 1, 1
+    for t in range(lib.NTYPE):
+        x.friction[t] = 0.0
+        for u in range(lib.NTYPE):
+            x.epsil[t][u] = 0.0
+    x.charge[2], x.charge[3] = 1.0, -1.0
+    x.alpha_c, x.n_kmax, x.rc_coul, x.ewald_compat = alpha_c, n_kmax, rc, 0
+    idx = np.array([(i, j, k) for i in range(n_side) for j in range(n_side) for k in range(n_side)])
+    r = (idx + 0.5) * a0
+    t = np.where(idx.sum(axis=1) % 2 == 0, 3, 4).astype(np.int32)
+    return x, r, t
+
+
+def _coulomb_energy(x, r, t):
+    s = orc.System(x)
+    s.set_state(r, np.zeros_like(r), np.zeros_like(r), t)
+    s.binning()
+    f, e = s.forces(step=1)
+    return e["v_coul"], f
+
+
+def test_ewald_madelung_rock_salt():
+    """Corrected form: the Ewald sum of a rock-salt crystal reproduces the
+    Madelung constant 1.747564594633 (E = -N/2 M / a0), and the forces on
+    every ion vanish by symmetry."""
+    x, r, t = _rock_salt(8, 1.5, 1.96, 20, 3.0)
+    e, f = _coulomb_energy(x, r, t)
+    expect = -(len(t) / 2) * 1.747564594633 / 1.5
+    # residual 3e-7: erfc(4.2) real-space truncation and the float32 pi
+    assert abs(e - expect) < 1e-6 * abs(expect), (e, expect)
+    assert np.abs(f).max() < 1e-6
+
+
+def test_ewald_total_independent_of_alpha():
+    """The sum of real, self and k-space terms does not depend on the
+    splitting parameter (a displaced rock salt, two alpha values)."""
+    x, r, t = _rock_salt(8, 1.5, 1.44, 20, 3.0)
+    rng = np.random.default_rng(3)
+    r = r + rng.uniform(-0.2, 0.2, r.shape)
+    e1, f1 = _coulomb_energy(x, r, t)
+    x.alpha_c = 1.96
+    e2, f2 = _coulomb_energy(x, r, t)
+    assert abs(e1 - e2) < 2e-6 * abs(e1), (e1, e2)
+    assert np.abs(f1 - f2).max() < 1e-4 * np.abs(f1).max()
+
+
+def test_ewald_k_count_matches_brute_enumeration():
+    """The half-space k set of ewald_k.f90:104-124: every k in the sphere
+    appears exactly once with its mirror image -k absent."""
+    kind, inp = configs.small_cases()["pe_brush_compat"]
+    p, r, v, t = configs.build(inp, kind)
+    s = orc.System(inp)
+    s.set_state(r, v, np.zeros_like(r), t)
+    kv = s.k_vectors()
+    keys = {tuple(np.round(k, 9)) for k in kv}
+    assert len(keys) == len(kv)
+    assert not any(tuple(np.round(-k, 9)) in keys for k in kv)
+    # the full sphere (both halves, minus k = 0) has twice as many points
+    dk = np.array([kv[:, 0][kv[:, 0] > 0].min(), kv[:, 1][kv[:, 1] > 0].min(),
+                   kv[:, 2][kv[:, 2] > 0].min()])
+    kc = (dk[0] + dk[1]) * inp.n_kmax / 2
+    n = [int(kc / d) for d in dk]
+    full = 0
+    for a in range(-n[0], n[0] + 1):
+        for b in range(-n[1], n[1] + 1):
+            for c in range(-n[2], n[2] + 1):
+                k2 = (a * dk[0]) ** 2 + (b * dk[1]) ** 2 + (c * dk[2]) ** 2
+                full += 0 < k2 <= kc * kc
+    assert full == 2 * len(kv)

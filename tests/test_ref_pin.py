@@ -32,6 +32,9 @@ CASES = {
     "brush": ("brush_small", "ref_md_brush", 3, 20),
     "melt": ("melt_wca", "ref_md_melt", 3, 30),
     "melt_switch_on": ("melt_switch_on", "ref_md_melt_fso", 3, 30),
+    # charged brush: ewald_real inside the LJ range with the 1/pi factor,
+    # ewald_k over the slab and the dipolar correction (the reference's form)
+    "pe_brush": ("pe_brush_compat", "ref_md_brush_q", 3, 20),
 }
 
 
@@ -47,9 +50,9 @@ def _read(path, n, init):
 
     n_part, nreb = take(np.int32, 2)
     assert n_part == n
-    e = take(np.float64, 4)
+    e = take(np.float64, 5)
     out = {"n_rebuilds": int(nreb), "v_fluid_fluid": e[0], "v_fluid_wall": e[1],
-           "v_intra_molec": e[2], "t_kin": e[3]}
+           "v_intra_molec": e[2], "t_kin": e[3], "v_coul": e[4]}
     for k in ("r", "v", "a", "f"):
         out[k] = take(np.float64, 3 * n).reshape(n, 3).copy()
     out["mic"] = take(np.int32, 3 * n).reshape(n, 3).copy()
@@ -119,6 +122,8 @@ def test_force_pass_matches_reference(run):
         assert rel(e[k], ref[k]) < 1e-12, (k, e[k], ref[k])
     if run["p"].symmetry == 0:
         assert rel(e["v_fluid_wall"], ref["v_fluid_wall"]) < 1e-12
+    if run["p"].coulomb:
+        assert rel(e["v_coul"], ref["v_coul"]) < 1e-12, (e["v_coul"], ref["v_coul"])
 
 
 def test_trajectory_matches_reference(run):
@@ -133,3 +138,5 @@ def test_trajectory_matches_reference(run):
     assert obs["n_rebuilds"] == ref["n_rebuilds"]
     assert rel(obs["t_kin"], ref["t_kin"]) < 1e-10
     assert rel(obs["v_fluid_fluid"], ref["v_fluid_fluid"]) < 1e-10
+    if run["p"].coulomb:
+        assert rel(obs["v_coul"], ref["v_coul"]) < 1e-10
