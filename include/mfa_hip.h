@@ -218,6 +218,11 @@ int32_t mfa_fluid_fluid(mfa_ctx* ctx, int64_t step, double* v_ff, double* v_coul
 int32_t mfa_fluid_wall(mfa_ctx* ctx, int32_t inter_type, double* v_fw);
 /* intra_molec (intra_molec.f90:16-139): FENE, one thread per bond. */
 int32_t mfa_intra_molec(mfa_ctx* ctx, double* v_intra);
+/* bending(mode_bend) (bending.f90:2-127, md_main.f90:108-111): mode 2 adds
+ * the harmonic angle forces of the brush chains, k_bend/2 (alpha -
+ * alpha_eq)^2, and returns v_bend; modes 0 and 1 do nothing, as in the
+ * reference.  No-op unless the inputs enable bending. */
+int32_t mfa_bending(mfa_ctx* ctx, int32_t mode, double* v_bend);
 /* verlet_velocities (verlet_velocities.f90:18-199); *t_kin = 0.5 sum m v^2. */
 int32_t mfa_verlet_velocities(mfa_ctx* ctx, double* t_kin);
 

@@ -138,6 +138,15 @@ def pe_brush(n_chain, n_mon, n_solvent, lx, ly, lz, x_space=1.0, dt=0.005, gamma
     return x
 
 
+def stiff(x, k_bend, alpha_eq):
+    """Adds chain stiffness (BENDING): V = k_bend/2 (alpha - alpha_eq)^2 per
+    angle of the brush chains (bending.f90:33-127)."""
+    x.bending = 1
+    x.k_bend = k_bend
+    x.alpha_eq = alpha_eq
+    return x
+
+
 WORKLOADS = {
     # configs[0]: plain FP64 DPD polymer brush (CPU-runnable reference case)
     "brush9k": dict(kind=1, steps=1000, min_dist=0.7,
@@ -156,6 +165,8 @@ def small_cases():
         "melt_wca": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=13, n_cell=16)),
         "melt_switch_on": (2, melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=14, n_cell=16,
                                    switch_on=True)),
+        # stiff brush: harmonic bending of the grafted chains (BENDING)
+        "brush_bend": (1, stiff(brush(8, 10, 300, 6.0, 6.0, 8.0, x_space=0.5, seed=17), 5.0, 0.4)),
         # charged brush, the reference's own Ewald form (compat) and the
         # corrected one with a real-space cut-off of 3
         "pe_brush_compat": (3, pe_brush(8, 12, 60, 6.0, 6.0, 8.0, x_space=0.5, n_kmax=4,

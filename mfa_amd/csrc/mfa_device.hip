@@ -47,7 +47,8 @@ enum {
   T_VFF = 0, T_VFW, T_VINTRA, T_VBEND, T_VCOUL, T_TKIN, T_VIR, T_FZB, T_FZT, T_ZMOM, T_SPARE0, T_SPARE1
 };
 // force-pass term mask
-enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16 };
+enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16, TM_BEND = 32 };
+#define TM_ALL (TM_PAIR | TM_WALL | TM_BOND | TM_BEND)
 // device flag slots
 enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_NFLAGS = 8 };
 
@@ -78,6 +79,8 @@ struct KParams {
   // init_config.f90:230), inside rc_coul or (rc_coul_2 = 0) inside the LJ range
   double q_type[MFA_NTYPE], alpha_c, sqrt_alpha, inv_sqrt_pi, rc_coul_2;
   int coulomb;
+  int bending;                // BENDING: harmonic angles of the brush chains
+  double k_bend, alpha_eq;
   float pf_margin;               // bound on |r^2(FP32) - r^2(FP64)| in the list build
   float ff_margin;               // the same for the cell-relative FP32 view of the force pass
   int nb[3];
@@ -787,6 +790,61 @@ __device__ __forceinline__ void wall_93(const KParams& k, int ti, double zi, dou
   ft = -(9.0 * aabs) * sw9 * (iz8 * iz2) + (3.0 * aw) * sw3 * iz4;
 }
 
+// Minimum image in x, y and z as bending.f90:48-52 writes it (also with walls).
+__device__ __forceinline__ void min_image3(const KParams& k, double* d) {
+  for (int j = 0; j < 3; ++j) d[j] = pbc(d[j], k.L[j], k.invL[j]);
+}
+
+// The angle of a brush chain centred at bead `cen` (bending.f90:43-118):
+// fp = F_mod dir_prev (on cen-1), fn = F_mod dir_next (on cen+1); cen itself
+// takes -(fp + fn).  Returns 0.5 F_mod delta_alpha.
+__device__ double bend_angle(const KParams& k, const double* __restrict__ r, int cen, double* fp,
+                             double* fn) {
+  double rp[3], rn[3];
+  for (int j = 0; j < 3; ++j) {
+    rp[j] = r[3 * (size_t)cen + j] - r[3 * (size_t)(cen - 1) + j];
+    rn[j] = r[3 * (size_t)(cen + 1) + j] - r[3 * (size_t)cen + j];
+  }
+  min_image3(k, rp);
+  min_image3(k, rn);
+  double dot_pr = 0.0, rp2 = 0.0, rn2 = 0.0, dp2 = 0.0, dn2 = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    dot_pr = dot_pr + rn[j] * rp[j];
+    rp2 = rp2 + rp[j] * rp[j];
+    rn2 = rn2 + rn[j] * rn[j];
+  }
+  const double cos_alpha = dot_pr / sqrt(rp2 * rn2);
+  double dn[3], dp[3];
+  for (int j = 0; j < 3; ++j) {
+    dn[j] = rp[j] * rn2 - rn[j] * dot_pr;
+    dp[j] = -rn[j] * rp2 + rp[j] * dot_pr;
+  }
+  for (int j = 0; j < 3; ++j) {
+    dp2 = dp2 + dp[j] * dp[j];
+    dn2 = dn2 + dn[j] * dn[j];
+  }
+  const double eps = (double)0.0000001f;  // a default-real literal (bending.f90:72,81)
+  if (dn2 < eps) {
+    for (int j = 0; j < 3; ++j) dn[j] = dn[j] * 0.0;
+  } else {
+    const double nn = sqrt(dn2 * rn2);
+    for (int j = 0; j < 3; ++j) dn[j] = dn[j] / nn;
+  }
+  if (dp2 < eps) {
+    for (int j = 0; j < 3; ++j) dp[j] = dp[j] * 0.0;
+  } else {
+    const double np_ = sqrt(dp2 * rp2);
+    for (int j = 0; j < 3; ++j) dp[j] = dp[j] / np_;
+  }
+  const double delta_alpha = cos_alpha >= 1.0 ? 0.0 : acos(cos_alpha) - k.alpha_eq;
+  const double f_mod = k.k_bend * delta_alpha;
+  for (int j = 0; j < 3; ++j) {
+    fp[j] = f_mod * dp[j];
+    fn[j] = f_mod * dn[j];
+  }
+  return .5 * f_mod * delta_alpha;
+}
+
 template <int TPP>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* __restrict__ r,
                                                        const double* __restrict__ v,
@@ -813,6 +871,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
   const double vxi = v[3 * (size_t)ii + 0], vyi = v[3 * (size_t)ii + 1], vzi = v[3 * (size_t)ii + 2];
   const int ti = type[ii] - 1;
   double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0, e_coul = 0.0;
+  double e_bend = 0.0;
   double ax = 0.0, ay = 0.0, az = 0.0;
 
   if (active && (mask & TM_PAIR)) {
@@ -925,6 +984,28 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
         }
       }
     }
+    if ((mask & TM_BEND) && k.bending && i < k.n_brush) {
+      const int base = (i / k.n_mon) * k.n_mon, m = i - base;
+      for (int cen = m - 1; cen <= m + 1; ++cen) {
+        if (cen < 1 || cen > k.n_mon - 2) continue;
+        double fp[3], fn[3];
+        const double eb = bend_angle(k, r, base + cen, fp, fn);
+        if (cen == m) {
+          e_bend += eb;
+          ax += -fp[0] - fn[0];
+          ay += -fp[1] - fn[1];
+          az += -fp[2] - fn[2];
+        } else if (cen == m + 1) {
+          ax += fp[0];
+          ay += fp[1];
+          az += fp[2];
+        } else {
+          ax += fn[0];
+          ay += fn[1];
+          az += fn[2];
+        }
+      }
+    }
     if ((mask & TM_WALL) && k.symmetry == 0) {
       wall_93(k, ti, zi, fzb, fzt, e_fw);
       az += fzb;
@@ -954,6 +1035,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
       vals[T_VFW] = e_fw;
       vals[T_VINTRA] = e_bond * k.k_chain;
       vals[T_VCOUL] = e_coul;
+      vals[T_VBEND] = e_bend;
       vals[T_FZB] = fzb;
       vals[T_FZT] = fzt;
       if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
@@ -1003,7 +1085,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const float4 vi4 = vel4[ii];
   const int ti = __float_as_int(pi4.w);
   double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0, zi = 0.0;
-  double e_coul = 0.0;
+  double e_coul = 0.0, e_bend = 0.0;
   const float rc2f = (float)k.rc_coul_2, sa = (float)k.sqrt_alpha, alf = (float)k.alpha_c;
   const float pref = (float)(k.sqrt_alpha * k.inv_sqrt_pi * 2.0);
   long long ax = 0, ay = 0, az = 0;
@@ -1081,6 +1163,33 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 
   // Bonds and walls: lane 0 of the group, from the FP64 positions.
   if (active && sub == 0 && (ENERGY || (mask & (TM_BOND | TM_WALL)))) zi = r[3 * (size_t)i + 2];
+  if (active && sub == 0 && (mask & TM_BEND) && k.bending && i < k.n_brush) {
+    // each term through float and fixed point; the centre takes minus the
+    // fixed-point sum of its neighbours' terms, so every angle sums to zero
+    const int base = (i / k.n_mon) * k.n_mon, m = i - base;
+    for (int cen = m - 1; cen <= m + 1; ++cen) {
+      if (cen < 1 || cen > k.n_mon - 2) continue;
+      double fp[3], fn[3];
+      const double eb = bend_angle(k, r, base + cen, fp, fn);
+      long long px = 0, py = 0, pz = 0, nx = 0, ny = 0, nz = 0;
+      fix_add3(px, py, pz, (float)fp[0], (float)fp[1], (float)fp[2], bad);
+      fix_add3(nx, ny, nz, (float)fn[0], (float)fn[1], (float)fn[2], bad);
+      if (cen == m) {
+        if (ENERGY) e_bend += eb;
+        ax -= px + nx;
+        ay -= py + ny;
+        az -= pz + nz;
+      } else if (cen == m + 1) {
+        ax += px;
+        ay += py;
+        az += pz;
+      } else {
+        ax += nx;
+        ay += ny;
+        az += nz;
+      }
+    }
+  }
   if (active && sub == 0 && (mask & (TM_BOND | TM_WALL))) {
     if (mask & TM_BOND) {
       int base, len;
@@ -1136,6 +1245,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       vals[T_VFW] = e_fw;
       vals[T_VINTRA] = e_bond * k.k_chain;
       vals[T_VCOUL] = e_coul;
+      vals[T_VBEND] = e_bend;
       vals[T_FZB] = fzb;
       vals[T_FZT] = fzt;
       if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
@@ -1387,6 +1497,9 @@ static KParams make_kparams(const mfa_params& p) {
   k.fso_r2_init = p.fso_r2_init;
   k.fso_time = p.fso_time;
   k.coulomb = p.coulomb;
+  k.bending = p.bending;
+  k.k_bend = p.k_bend;
+  k.alpha_eq = p.alpha_eq;
   for (int t = 0; t < MFA_NTYPE; ++t) k.q_type[t] = p.charge[t];
   k.alpha_c = p.alpha_c;
   k.sqrt_alpha = p.sqrt_alpha;
@@ -1608,7 +1721,7 @@ static void drop_graph(mfa_ctx* c) {
 static int enqueue_middle_step(mfa_ctx* c) {
   int rc = launch_rebuild(c);
   if (rc) return rc;
-  rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
+  rc = launch_force(c, TM_ALL);
   if (rc) return rc;
   rc = launch_ewald(c, c->p.symmetry == 0);  // md_main.f90:128-132
   if (rc) return rc;
@@ -2145,6 +2258,18 @@ extern "C" int32_t mfa_intra_molec(mfa_ctx* c, double* v_intra) {
   return MFA_OK;
 }
 
+extern "C" int32_t mfa_bending(mfa_ctx* c, int32_t mode, double* v_bend) {
+  if (!c) return MFA_ERR_ARG;
+  if (v_bend) *v_bend = 0.0;
+  if (mode == 0 || mode == 1 || !c->p.bending) return MFA_OK;  // bending.f90:12-25
+  if (mode != 2) return set_err(c, MFA_ERR_ARG, "bending: mode must be 0, 1 or 2");
+  double t[MFA_NTERMS];
+  int rc = force_terms(c, 0, TM_BEND, v_bend ? t : nullptr);
+  if (rc) return rc;
+  if (v_bend) *v_bend = t[T_VBEND];
+  return MFA_OK;
+}
+
 extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
   if (!c) return MFA_ERR_ARG;
   const int g = grid_of(c->n);
@@ -2230,7 +2355,7 @@ extern "C" int32_t mfa_energies(mfa_ctx* c, int64_t step, mfa_obs* obs) {
   rc = set_step(c, step);
   if (rc) return rc;
   // recompute forces into F (the pass overwrites F), then reduce
-  rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND | TM_ENERGY);
+  rc = launch_force(c, TM_ALL | TM_ENERGY);
   if (rc) return rc;
   rc = launch_ewald(c, c->p.symmetry == 0);
   if (rc) return rc;
@@ -2284,7 +2409,7 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   // last step: force pass with the energies, verlet_velocities alone
   rc = launch_rebuild(c);
   if (rc) return rc;
-  rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND | (obs ? TM_ENERGY : 0));
+  rc = launch_force(c, TM_ALL | (obs ? TM_ENERGY : 0));
   if (rc) return rc;
   rc = launch_ewald(c, c->p.symmetry == 0);
   if (rc) return rc;
@@ -2317,7 +2442,7 @@ extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
+    for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_ALL);
     c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -2339,7 +2464,7 @@ extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
     if (ge) {
       HIPCHK(c, hipGraphLaunch(ge, c->stream));
     } else {
-      for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_PAIR | TM_WALL | TM_BOND);
+      for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_ALL);
     }
   }
   c->timing = timing;

@@ -17,7 +17,7 @@ module md_hip_state
   real(c_double), allocatable :: r0(:,:), v(:,:), a(:,:), force(:,:)
   integer(c_int32_t), allocatable :: a_type(:), mic_count(:,:)
   integer(c_int32_t) :: f_skin
-  real(c_double) :: v_fluid_fluid, v_coul, v_fluid_wall, v_intra_molec, t_fluid
+  real(c_double) :: v_fluid_fluid, v_coul, v_fluid_wall, v_intra_molec, t_fluid, v_bend
 
 contains
 
@@ -45,6 +45,10 @@ contains
   end subroutine
   subroutine intra_molec()
     call mfa_check(mfa_intra_molec(ctx, v_intra_molec), "intra_molec", ctx)
+  end subroutine
+  subroutine bending(mode)
+    integer, intent(in) :: mode
+    call mfa_check(mfa_bending(ctx, int(mode, c_int32_t), v_bend), "bending", ctx)
   end subroutine
   subroutine ewald_k(mode)
     integer, intent(in) :: mode
@@ -155,6 +159,7 @@ program md_pb_hip
       call fluid_fluid(step)
       if (inp%symmetry == 0) call fluid_wall(2)
       call intra_molec()
+      if (inp%bending /= 0) call bending(2)            ! md_main.f90:108-111
       if (inp%system == 2 .or. inp%system == 3) then   ! md_main.f90:128-132
         call ewald_k(1)
         if (inp%symmetry == 0) call dipolar_correction()

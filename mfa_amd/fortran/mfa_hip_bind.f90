@@ -134,6 +134,12 @@ module mfa_hip_bind
       integer(c_int32_t), value :: mode
       real(c_double) :: v_coul
     end function
+    integer(c_int32_t) function mfa_bending(ctx, mode, v_bend) bind(C, name="mfa_bending")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+      real(c_double) :: v_bend
+    end function
     integer(c_int32_t) function mfa_dipolar_correction(ctx, v_coul) bind(C, name="mfa_dipolar_correction")
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

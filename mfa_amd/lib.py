@@ -117,6 +117,7 @@ SIGNATURES = {
     "mfa_intra_molec": (C.c_int32, [_P, _dp]),
     "mfa_ewald_k": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_dipolar_correction": (C.c_int32, [_P, _dp]),
+    "mfa_bending": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
@@ -315,6 +316,11 @@ class Context:
     def ewald_k(self, mode=1):
         e = C.c_double()
         self._chk(self.lib.mfa_ewald_k(self.h, mode, C.byref(e)), "mfa_ewald_k")
+        return e.value
+
+    def bending(self, mode=2):
+        e = C.c_double()
+        self._chk(self.lib.mfa_bending(self.h, mode, C.byref(e)), "mfa_bending")
         return e.value
 
     def dipolar_correction(self):

@@ -249,6 +249,9 @@ int orc_derive(const mfa_inputs* in, orc_params* p) {
   p->inv_sqrt_pi = in->ewald_compat ? 1. / PI_F : 1. / sqrt(3.14159265358979323846);
   p->n_kmax = in->n_kmax;
   p->rc_coul_2 = (!in->ewald_compat && in->rc_coul > 0.0) ? in->rc_coul * in->rc_coul : 0.0;
+  p->bending = in->bending;
+  p->k_bend = in->k_bend;
+  p->alpha_eq = in->alpha_eq;
   /* init_config.f90:264-281 and init_force_switch_on.f90:15-26 */
   if (in->force_switch_on && in->s_time == 1) {
     double maxsig = 0.;
@@ -726,6 +729,67 @@ double orc_dipolar(orc_sys* s) {
   return 2. * PI_F * m_z * m_z * s->inv_V;
 }
 
+/* bending(2) (bending.f90:33-127): harmonic angle between consecutive bonds
+ * of the brush chains, V = k_bend/2 (alpha - alpha_eq)^2, with the
+ * reference's minimum image in x, y and z and its cut-off of the force
+ * direction below |d|^2 < 1e-7.  Pinned by the compiled reference (BENDING
+ * build), finite differences of v_bend and zero net force per angle. */
+double orc_bending(orc_sys* s) {
+  const orc_params* p = &s->p;
+  if (!p->bending) return 0.;
+  double v_bend = 0.;
+  for (int l = 0; l < p->n_chain; ++l) {
+    const int b = l * p->n_mon; /* 0-based index of bead 1 */
+    double r_next[3], r_prev[3];
+    for (int j = 0; j < 3; ++j) r_next[j] = R(b + 1, j) - R(b, j);
+    for (int j = 0; j < 3; ++j) r_next[j] = pbc(r_next[j], p->boundary[j], p->inv_boundary[j]);
+    for (int i = 2; i <= p->n_mon - 1; ++i) { /* 1-based bead i, centre of the angle */
+      double dot_pr = 0, r_prev_2 = 0, r_next_2 = 0, dir_prev_2 = 0, dir_next_2 = 0;
+      for (int j = 0; j < 3; ++j) r_prev[j] = r_next[j];
+      for (int j = 0; j < 3; ++j) r_next[j] = R(b + i, j) - R(b + i - 1, j);
+      for (int j = 0; j < 3; ++j) r_next[j] = pbc(r_next[j], p->boundary[j], p->inv_boundary[j]);
+      for (int j = 0; j < 3; ++j) {
+        dot_pr = dot_pr + r_next[j] * r_prev[j];
+        r_prev_2 = r_prev_2 + r_prev[j] * r_prev[j];
+        r_next_2 = r_next_2 + r_next[j] * r_next[j];
+      }
+      const double cos_alpha = dot_pr / sqrt(r_prev_2 * r_next_2);
+      double dir_next[3], dir_prev[3];
+      for (int j = 0; j < 3; ++j) {
+        dir_next[j] = r_prev[j] * r_next_2 - r_next[j] * dot_pr;
+        dir_prev[j] = -r_next[j] * r_prev_2 + r_prev[j] * dot_pr;
+      }
+      for (int j = 0; j < 3; ++j) {
+        dir_prev_2 = dir_prev_2 + dir_prev[j] * dir_prev[j];
+        dir_next_2 = dir_next_2 + dir_next[j] * dir_next[j];
+      }
+      const double eps = (double)0.0000001f;
+      if (dir_next_2 < eps) {
+        for (int j = 0; j < 3; ++j) dir_next[j] = dir_next[j] * 0;
+      } else {
+        const double nn = sqrt(dir_next_2 * r_next_2);
+        for (int j = 0; j < 3; ++j) dir_next[j] = dir_next[j] / nn;
+      }
+      if (dir_prev_2 < eps) {
+        for (int j = 0; j < 3; ++j) dir_prev[j] = dir_prev[j] * 0;
+      } else {
+        const double np_ = sqrt(dir_prev_2 * r_prev_2);
+        for (int j = 0; j < 3; ++j) dir_prev[j] = dir_prev[j] / np_;
+      }
+      const double delta_alpha = cos_alpha >= 1.0 ? 0. : acos(cos_alpha) - p->alpha_eq;
+      const double f_mod = p->k_bend * delta_alpha;
+      v_bend = v_bend + .5 * f_mod * delta_alpha;
+      for (int j = 0; j < 3; ++j) {
+        const double fp = f_mod * dir_prev[j], fn = f_mod * dir_next[j];
+        F(b + i - 2, j) = F(b + i - 2, j) + fp;
+        F(b + i - 1, j) = F(b + i - 1, j) - fp - fn;
+        F(b + i, j) = F(b + i, j) + fn;
+      }
+    }
+  }
+  return v_bend;
+}
+
 static double ipow(double x, int n) { /* square-and-multiply, lowest bit first */
   double res = 1.0, sq = x;
   int first = 1;
@@ -861,6 +925,7 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
   }
   const int reb0 = s->n_rebuilds; /* count the rebuilds inside the loop only */
   double e_ff[3] = {0, 0, 0}, e_fw[3] = {0, 0, 0}, e_in[2] = {0, 0}, tk = 0., v_coul = 0.;
+  double v_bend = 0.;
   for (int it = 0; it < n; ++it) {
     const int64_t step = first_step + it;
     orc_verlet_positions(s);
@@ -872,6 +937,7 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
     orc_fluid_fluid(s, step, e_ff);
     if (s->p.symmetry == 0) orc_fluid_wall(s, e_fw);
     orc_intra_molec(s, e_in);
+    v_bend = orc_bending(s); /* md_main.f90:110-111 */
     v_coul = e_ff[2];
     if (s->p.coulomb) { /* md_main.f90:128-132 */
       v_coul = v_coul + orc_ewald_k(s);
@@ -888,6 +954,7 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
     obs[5] = (double)(s->n_rebuilds - reb0);
     obs[6] = e_fw[1];
     obs[7] = v_coul;
+    obs[8] = v_bend;
   }
   return 0;
 }

@@ -46,6 +46,9 @@ typedef struct orc_params {
   /* Coulomb (SYSTEM 2/3): init_system.f90:83-98, init_config.f90:221-239 */
   int coulomb, ewald_compat, n_kmax;
   double charge[ORC_NT], alpha_c, sqrt_alpha, inv_sqrt_pi, rc_coul_2;
+  /* BENDING (init_params.f90:211-213) */
+  int bending;
+  double k_bend, alpha_eq;
 } orc_params;
 
 typedef struct orc_sys orc_sys;
@@ -77,13 +80,16 @@ double orc_ewald_k(orc_sys* s);
 int orc_ewald_nk(orc_sys* s, double* kvecs);
 /* dipolar_correction: adds the slab forces; returns its energy. */
 double orc_dipolar(orc_sys* s);
+/* bending(2): adds the angle forces of the brush chains; returns v_bend. */
+double orc_bending(orc_sys* s);
 /* out[0]=v_fluid_wall, out[1]=sum Fz bottom, out[2]=sum Fz top */
 void orc_fluid_wall(orc_sys* s, double* out);
 /* out[0]=v_intra_molec, out[1]=virial contribution */
 void orc_intra_molec(orc_sys* s, double* out);
 double orc_verlet_velocities(orc_sys* s);
-/* Loop body of md_main.f90:69-155 for n steps.  obs[0..7]: v_ff, v_fw,
- * v_intra, t_kin, virial, n_rebuilds, wall Fz bottom, v_coul of the last step. */
+/* Loop body of md_main.f90:69-155 for n steps.  obs[0..8]: v_ff, v_fw,
+ * v_intra, t_kin, virial, n_rebuilds, wall Fz bottom, v_coul, v_bend of the
+ * last step. */
 int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs);
 /* Brute-force O(N^2) list criterion (for pinning the cell scan). */
 int64_t orc_brute_pairs(orc_sys* s, int32_t* pairs, int64_t cap);

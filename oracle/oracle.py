@@ -43,6 +43,7 @@ class OrcParams(C.Structure):
         ("coulomb", C.c_int), ("ewald_compat", C.c_int), ("n_kmax", C.c_int),
         ("charge", _d4), ("alpha_c", C.c_double), ("sqrt_alpha", C.c_double),
         ("inv_sqrt_pi", C.c_double), ("rc_coul_2", C.c_double),
+        ("bending", C.c_int), ("k_bend", C.c_double), ("alpha_eq", C.c_double),
     ]
 
 
@@ -99,6 +100,8 @@ def load():
     lib.orc_ewald_nk.restype = C.c_int
     lib.orc_dipolar.argtypes = [P]
     lib.orc_dipolar.restype = C.c_double
+    lib.orc_bending.argtypes = [P]
+    lib.orc_bending.restype = C.c_double
     _lib = lib
     return lib
 
@@ -214,13 +217,16 @@ class System:
         return self.lib.orc_verlet_velocities(self.h)
 
     def run_steps(self, first_step, n):
-        obs = np.zeros(8)
+        obs = np.zeros(9)
         rc = self.lib.orc_run_steps(self.h, first_step, n, _dp(obs))
         if rc:
             raise RuntimeError("orc_run_steps error %d" % rc)
         return {"v_fluid_fluid": obs[0], "v_fluid_wall": obs[1], "v_intra_molec": obs[2],
                 "t_kin": obs[3], "virial": obs[4], "n_rebuilds": obs[5], "wall_fz_bottom": obs[6],
-                "v_coul": obs[7]}
+                "v_coul": obs[7], "v_bend": obs[8]}
+
+    def bending(self):
+        return self.lib.orc_bending(self.h)
 
     def ewald_k(self):
         return self.lib.orc_ewald_k(self.h)
@@ -242,6 +248,7 @@ class System:
         e_ff = self.fluid_fluid(step)
         e_fw = self.fluid_wall() if self.p.symmetry == 0 else np.zeros(3)
         e_in = self.intra_molec()
+        v_bend = self.bending()  # md_main.f90:110-111
         v_coul = e_ff[2]
         if self.p.coulomb:  # md_main.f90:128-132
             v_coul += self.ewald_k()
@@ -250,4 +257,4 @@ class System:
         return self.get_state()["f"], {"v_fluid_fluid": e_ff[0], "v_fluid_wall": e_fw[0],
                                        "v_intra_molec": e_in[0], "virial": e_ff[1] + e_in[1],
                                        "wall_fz_bottom": e_fw[1], "wall_fz_top": e_fw[2],
-                                       "v_coul": v_coul}
+                                       "v_coul": v_coul, "v_bend": v_bend}

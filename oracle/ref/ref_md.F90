@@ -14,7 +14,7 @@
 !   ref_final.bin  the state after n_steps of the main loop
 !                  (md_main.f90:68-147, observation and storage left out).
 ! Record layout (stream, native endianness): int32 n_part, int32 n_rebuilds,
-! float64 v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, v_coul, then
+! float64 v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, v_coul, v_bend, then
 ! r0, v, a, force as float64 (3, n_part) and mic_count as int32 (3, n_part);
 ! ref_init.bin adds the cells int32 (3, n_part), int32 n_pairs and the pairs
 ! int32 (2, n_pairs), each pair as (min, max).
@@ -55,6 +55,7 @@ program ref_md
   call fluid_wall(wall_flag)
 #endif
   call intra_molec()
+  call angles()
   call coulomb_k()
   open(newunit=u, file="ref_init.bin", access="stream", form="unformatted", status="replace")
   call put_state(u, 0)
@@ -83,6 +84,7 @@ program ref_md
     call fluid_wall(wall_flag)
 #endif
     call intra_molec
+    call angles()
     call coulomb_k()
     call verlet_velocities()
   end do
@@ -91,6 +93,13 @@ program ref_md
   close(u)
 
 contains
+
+  ! chain stiffness (md_main.f90:108-111)
+  subroutine angles()
+#ifdef BENDING
+    call bending(2)
+#endif
+  end subroutine angles
 
   ! k-space Ewald and the slab correction of charged systems (md_main.f90:128-132)
   subroutine coulomb_k()
@@ -104,8 +113,13 @@ contains
 
   subroutine put_state(unit, nreb)
     integer, intent(in) :: unit, nreb
+    real(kind=8) :: vb
+    vb = 0.d0
+#ifdef BENDING
+    vb = v_bend
+#endif
     write(unit) n_part, nreb
-    write(unit) v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, v_coul
+    write(unit) v_fluid_fluid, v_fluid_wall, v_intra_molec, t_fluid, v_coul, vb
     write(unit) r0(:, 1:n_part), v(:, 1:n_part), a(:, 1:n_part), force(:, 1:n_part)
     write(unit) mic_count(:, 1:n_part)
   end subroutine put_state

@@ -14,7 +14,7 @@ from tests.util import force_error, gpu_ctx, make_case, oracle_sys, rel
 pytestmark = pytest.mark.gpu
 
 CASES = ["brush_small", "melt_small", "melt_wca", "melt_switch_on", "pe_brush_compat",
-         "pe_brush_small"]
+         "pe_brush_small", "brush_bend"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -52,6 +52,8 @@ def test_forces_fp64(case):
         assert rel(obs["wall_fz_bottom"], eref["wall_fz_bottom"]) < 1e-12
     if p.coulomb:
         assert rel(obs["v_coul"], eref["v_coul"]) < 1e-12, (obs["v_coul"], eref["v_coul"])
+    if p.bending:
+        assert rel(obs["v_bend"], eref["v_bend"]) < 1e-12, (obs["v_bend"], eref["v_bend"])
     ctx.close()
 
 
@@ -69,6 +71,9 @@ def test_subroutine_entry_points_fp64(case):
     vff, vc = ctx.fluid_fluid(3)
     vfw = ctx.fluid_wall(2)
     vin = ctx.intra_molec()
+    vb = ctx.bending(2)  # md_main.f90:108-111
+    if p.bending:
+        assert rel(vb, eref["v_bend"]) < 1e-12
     if p.coulomb:  # md_main.f90:128-132
         vc += ctx.ewald_k(1)
         if p.symmetry == 0:
@@ -148,6 +153,8 @@ def test_trajectory_fp64(case):
         assert rel(obs[k], ref[k]) < 1e-9, (k, obs[k], ref[k])
     if p.coulomb:
         assert rel(obs["v_coul"], ref["v_coul"]) < 1e-9, (obs["v_coul"], ref["v_coul"])
+    if p.bending:
+        assert rel(obs["v_bend"], ref["v_bend"]) < 1e-9, (obs["v_bend"], ref["v_bend"])
     ctx.close()
 
 

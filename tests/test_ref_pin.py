@@ -35,6 +35,8 @@ CASES = {
     # charged brush: ewald_real inside the LJ range with the 1/pi factor,
     # ewald_k over the slab and the dipolar correction (the reference's form)
     "pe_brush": ("pe_brush_compat", "ref_md_brush_q", 3, 20),
+    # stiff brush: bending(2) of the grafted chains
+    "brush_bend": ("brush_bend", "ref_md_brush_bend", 3, 20),
 }
 
 
@@ -50,9 +52,9 @@ def _read(path, n, init):
 
     n_part, nreb = take(np.int32, 2)
     assert n_part == n
-    e = take(np.float64, 5)
+    e = take(np.float64, 6)
     out = {"n_rebuilds": int(nreb), "v_fluid_fluid": e[0], "v_fluid_wall": e[1],
-           "v_intra_molec": e[2], "t_kin": e[3], "v_coul": e[4]}
+           "v_intra_molec": e[2], "t_kin": e[3], "v_coul": e[4], "v_bend": e[5]}
     for k in ("r", "v", "a", "f"):
         out[k] = take(np.float64, 3 * n).reshape(n, 3).copy()
     out["mic"] = take(np.int32, 3 * n).reshape(n, 3).copy()
@@ -124,6 +126,8 @@ def test_force_pass_matches_reference(run):
         assert rel(e["v_fluid_wall"], ref["v_fluid_wall"]) < 1e-12
     if run["p"].coulomb:
         assert rel(e["v_coul"], ref["v_coul"]) < 1e-12, (e["v_coul"], ref["v_coul"])
+    if run["inp"].bending:
+        assert rel(e["v_bend"], ref["v_bend"]) < 1e-12, (e["v_bend"], ref["v_bend"])
 
 
 def test_trajectory_matches_reference(run):
@@ -140,3 +144,5 @@ def test_trajectory_matches_reference(run):
     assert rel(obs["v_fluid_fluid"], ref["v_fluid_fluid"]) < 1e-10
     if run["p"].coulomb:
         assert rel(obs["v_coul"], ref["v_coul"]) < 1e-10
+    if run["inp"].bending:
+        assert rel(obs["v_bend"], ref["v_bend"]) < 1e-10
