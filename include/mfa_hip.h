@@ -256,6 +256,28 @@ int32_t mfa_copy_probe(int64_t bytes, double* ms);
 int32_t mfa_timing(mfa_ctx* ctx, int32_t enable, double* force_ms_avg, int64_t* force_launches,
                    double* integrate_ms_avg, int64_t* integrate_launches);
 
+/* ------------------------------------------------------------------------
+ * Batches of independent replicas (no counterpart in the reference, which
+ * runs one system per process).  Replica q owns its own box, cell grid,
+ * chains and seed; all replicas share the compile-time switches (system,
+ * symmetry, thermostat, DPD cut-off) and carry no charges.  One
+ * launch of each kernel advances every replica, so small replicas share the
+ * GPU instead of queueing behind each other.  mfa_run_steps then advances all
+ * replicas in lockstep; its obs argument receives replica 0's observations
+ * and mfa_replica_obs the others'.  A single system is a batch of one. */
+int32_t mfa_create_batch(const mfa_params* params, int32_t n_rep, int32_t precision,
+                         int32_t list_capacity, mfa_ctx** ctx);
+int32_t mfa_set_state_replica(mfa_ctx* ctx, int32_t replica, const double* r0, const double* v,
+                              const double* a, const int32_t* a_type, const int32_t* mic);
+int32_t mfa_get_state_replica(mfa_ctx* ctx, int32_t replica, double* r0, double* v, double* a,
+                              double* force, int32_t* mic);
+int32_t mfa_replica_obs(mfa_ctx* ctx, int32_t replica, mfa_obs* obs);
+/* Autotuner: runs n_steps from first_step with each number of threads per
+ * particle in {1, 2, 4, 8, 16} (advancing the state, so call it during
+ * warm-up), keeps the fastest and returns it with its time per step. */
+int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
+                     double* ms_per_step);
+
 /* Seeded input generators (the reference's conf_default/gen_brush/gen_droplet
  * are replaced by these; conf_default.f90:144-217, gen_brush.f90:101-161).
  * kind 1: brush + ideal solvent in a channel; kind 2: melt in a periodic box;

@@ -147,6 +147,29 @@ def stiff(x, k_bend, alpha_eq):
     return x
 
 
+def ensemble_replica(q, base_seed=4004, lz=12.5, solvent_density=3.0):
+    """Replica q of the uneven brush ensemble: grafting density uniform in
+    [0.05, 0.5], chain length uniform in {20..100}, lateral edge uniform in
+    {8..40} sigma, channel height lz, total density 3 with the solvent
+    filling what the brush leaves (so sizes run from ~2.4k to ~60k beads);
+    chains are capped at a brush density of 0.5 so they fit the channel.
+    Seed base_seed*1000 + q.  Fresh start with the force switch-on ramp
+    (the generated beads keep 0.7 sigma apart)."""
+    rng = np.random.default_rng(base_seed * 1000 + q)
+    rho_g = rng.uniform(0.05, 0.5)
+    n_mon = int(rng.integers(20, 101))
+    L = int(rng.integers(8, 41))
+    area = float(L * L)
+    n_chain = max(1, min(int(round(rho_g * area)), int(0.5 * area * lz / n_mon)))
+    n_chain = max(2, n_chain - n_chain % 2)  # even (init_params.f90:881-887)
+    n_total = int(round(solvent_density * area * lz))
+    n_solv = max(0, n_total - n_chain * n_mon)
+    x = brush(n_chain, n_mon, n_solv, float(L), float(L), lz, seed=base_seed * 1000 + q)
+    x.s_time = 1
+    x.force_switch_on = 1
+    return x
+
+
 WORKLOADS = {
     # configs[0]: plain FP64 DPD polymer brush (CPU-runnable reference case)
     "brush9k": dict(kind=1, steps=1000, min_dist=0.7,

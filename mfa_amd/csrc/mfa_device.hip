@@ -85,6 +85,9 @@ struct KParams {
   float ff_margin;               // the same for the cell-relative FP32 view of the force pass
   int nb[3];
   int n_cells, cell_cap;
+  // replica layout: particles [p0, p0 + n) (p0 a multiple of 256), cells
+  // [c0, c0 + n_cells) of the concatenated arrays; flags at rep * F_NFLAGS
+  int p0, c0, rep;
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
   int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
   uint32_t seed;
@@ -100,8 +103,16 @@ struct EwaldK {
 };
 
 struct mfa_ctx {
-  mfa_params p;
-  KParams k;
+  mfa_params p;  // replica 0 (all replicas share the force-field switches)
+  // replicas: particles [p0, p0 + n) and cells [c0, c0 + n_cells) of the
+  // concatenated arrays; n and n_cells below are the padded totals
+  int n_rep;
+  std::vector<mfa_params> ps;
+  std::vector<KParams> ks;
+  KParams* d_kp;     // per replica
+  int* d_rmap;       // replica of each 256-particle block
+  int* d_cell_rep;   // replica of each cell
+  std::vector<mfa_obs> rep_obs;  // per-replica observations of the last mfa_run_steps
   int precision;
   int n;
   int cap;
@@ -229,12 +240,12 @@ __device__ void block_partials(double* vals, double* part_out) {
 // Final sums: one block per term; each thread adds a fixed strided subset
 // of the block partials, then a fixed tree combines the threads.
 __global__ __launch_bounds__(MFA_BLOCK) void k_reduce_partials(const double* __restrict__ part,
-                                                               int nblocks,
+                                                               int b0, int nblocks,
                                                                double* __restrict__ out) {
   __shared__ double sh[MFA_BLOCK];
   const int t = blockIdx.x;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += MFA_BLOCK) s += part[(size_t)b * MFA_NTERMS + t];
+  for (int b = threadIdx.x; b < nblocks; b += MFA_BLOCK) s += part[(size_t)(b0 + b) * MFA_NTERMS + t];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
@@ -242,6 +253,25 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_reduce_partials(const double* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) out[t] = sh[0];
+}
+
+// The same per replica: block (term, replica) sums the replica's partials,
+// `per` particles per block (256 / threads per particle, or 256).
+__global__ __launch_bounds__(MFA_BLOCK) void k_reduce_replicas(const double* __restrict__ part,
+                                                               const KParams* __restrict__ kb,
+                                                               int per, double* __restrict__ out) {
+  __shared__ double sh[MFA_BLOCK];
+  const int t = blockIdx.x, q = blockIdx.y;
+  const int b0 = kb[q].p0 / per, nb = (kb[q].n + per - 1) / per;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += MFA_BLOCK) s += part[(size_t)(b0 + b) * MFA_NTERMS + t];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(size_t)q * MFA_NTERMS + t] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -300,11 +330,17 @@ __device__ __forceinline__ void vv_one(const KParams& k, int i, int ti, double* 
   }
 }
 
-// Clears the cell counts for the rebuild that may follow (grid-stride).
-__device__ __forceinline__ void zero_counts(const KParams& k, int* __restrict__ cell_count) {
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k.n_cells; c += gridDim.x * blockDim.x)
+// Clears the cell counts of every replica for the rebuild that may follow
+// (grid-stride).
+__device__ __forceinline__ void zero_counts(int nc_tot, int* __restrict__ cell_count) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc_tot; c += gridDim.x * blockDim.x)
     cell_count[c] = 0;
 }
+
+// Parameters of the replica a block works on.  Replicas own 256-aligned
+// particle ranges of the concatenated arrays, so a block of particle-indexed
+// work never spans two; a single system is a batch of one.
+#define MFA_REPLICA(first_particle) const KParams& k = kb[rmap[(first_particle) >> 8]]
 
 // FP32 view of particle i: position relative to the origin of `cell` (its
 // cell at the last rebuild), wrapped back across periodic faces it crossed
@@ -323,16 +359,18 @@ __device__ __forceinline__ void write_view(const KParams& k, int i, int cell, in
   vel4[i] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
 }
 
-__global__ void k_verlet_positions(KParams k, double* __restrict__ r, double* __restrict__ v,
+__global__ void k_verlet_positions(const KParams* __restrict__ kb, const int* __restrict__ rmap,
+                                   double* __restrict__ r, double* __restrict__ v,
                                    const double* __restrict__ a, int* __restrict__ mic,
                                    const double* __restrict__ r_ref, int* __restrict__ flags,
-                                   int with_skin, int* __restrict__ cell_count,
+                                   int with_skin, int nc_tot, int* __restrict__ cell_count,
                                    const int* __restrict__ type, const int* __restrict__ cell_of,
                                    float4* __restrict__ pos4, float4* __restrict__ vel4) {
-  zero_counts(k, cell_count);
+  zero_counts(nc_tot, cell_count);
+  MFA_REPLICA(blockIdx.x * blockDim.x);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double over = 0.0;
-  if (i < k.n) {
+  if (i - k.p0 < k.n) {
     double ri[3], vi[3], ai[3];
     int mi[3];
     for (int d = 0; d < 3; ++d) {
@@ -358,16 +396,17 @@ __global__ void k_verlet_positions(KParams k, double* __restrict__ r, double* __
   }
   if (with_skin) {
     over = wave_max(over);
-    if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[F_REBUILD], 1);
+    if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[k.rep * F_NFLAGS + F_REBUILD], 1);
   }
 }
 
-__global__ void k_check_skin(KParams k, const double* __restrict__ r,
-                             const double* __restrict__ r_ref, const int* __restrict__ mic,
-                             int* __restrict__ flags) {
+__global__ void k_check_skin(const KParams* __restrict__ kb, const int* __restrict__ rmap,
+                             const double* __restrict__ r, const double* __restrict__ r_ref,
+                             const int* __restrict__ mic, int* __restrict__ flags) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double over = 0.0;
-  if (i < k.n) {
+  if (i - k.p0 < k.n) {
     double ri[3], rr[3];
     int mi[3];
     for (int d = 0; d < 3; ++d) {
@@ -378,32 +417,34 @@ __global__ void k_check_skin(KParams k, const double* __restrict__ r,
     over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
   }
   over = wave_max(over);
-  if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[F_REBUILD], 1);
+  if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[k.rep * F_NFLAGS + F_REBUILD], 1);
 }
 
 // verlet_velocities alone: writes v, a and zeroes head forces; kinetic energy
 // partials when `energy` (verlet_velocities.f90:155-177).
-__global__ void k_verlet_velocities(KParams k, double* __restrict__ f, double* __restrict__ v,
+__global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* __restrict__ rmap,
+                                    double* __restrict__ f, double* __restrict__ v,
                                     double* __restrict__ a, const int* __restrict__ type,
                                     double* __restrict__ part, int energy,
                                     float4* __restrict__ vel4) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double vals[MFA_NTERMS];
   for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
-  if (i < k.n) {
+  if (i - k.p0 < k.n) {
     const int ti = type[i] - 1;
     double fi[3], vi[3], ai[3];
     for (int d = 0; d < 3; ++d) {
       fi[d] = f[3 * (size_t)i + d];
       vi[d] = v[3 * (size_t)i + d];
     }
-    vv_one(k, i, ti, fi, vi, ai);
+    vv_one(k, i - k.p0, ti, fi, vi, ai);
     for (int d = 0; d < 3; ++d) {
       v[3 * (size_t)i + d] = vi[d];
       a[3 * (size_t)i + d] = ai[d];
     }
     vel4[i] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
-    if (is_head(k, i))
+    if (is_head(k, i - k.p0))
       for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] = 0.0;
     vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
   }
@@ -412,21 +453,23 @@ __global__ void k_verlet_velocities(KParams k, double* __restrict__ f, double* _
 
 // Fused: verlet_velocities of step s, then verlet_positions and check_skin of
 // step s+1.  Reads F, v, r, mic, r_ref; writes v, r (mic only on a wrap).
-__global__ void k_fused_integrate(KParams k, const double* __restrict__ f, double* __restrict__ v,
+__global__ void k_fused_integrate(const KParams* __restrict__ kb, const int* __restrict__ rmap,
+                                  const double* __restrict__ f, double* __restrict__ v,
                                   double* __restrict__ r, int* __restrict__ mic,
                                   const double* __restrict__ r_ref, const int* __restrict__ type,
                                   int* __restrict__ flags, double* __restrict__ part, int energy,
-                                  int* __restrict__ cell_count, long long* __restrict__ d_step,
-                                  const int* __restrict__ cell_of, float4* __restrict__ pos4,
-                                  float4* __restrict__ vel4) {
-  if (flags[F_ERROR] != 0) return;
-  zero_counts(k, cell_count);
+                                  int nc_tot, int* __restrict__ cell_count,
+                                  long long* __restrict__ d_step, const int* __restrict__ cell_of,
+                                  float4* __restrict__ pos4, float4* __restrict__ vel4) {
+  zero_counts(nc_tot, cell_count);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *d_step += 1;  // next step; read by the next force pass only
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  if (flags[k.rep * F_NFLAGS + F_ERROR] != 0) return;
   double over = 0.0;
   double vals[MFA_NTERMS];
   for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
-  if (i < k.n) {
+  if (i - k.p0 < k.n) {
     const int ti = type[i] - 1;
     double fi[3], vi[3], ai[3], ri[3], rr[3];
     int mi[3];
@@ -437,7 +480,7 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
       mi[d] = mic[3 * (size_t)i + d];
       rr[d] = r_ref[3 * (size_t)i + d];
     }
-    vv_one(k, i, ti, fi, vi, ai);
+    vv_one(k, i - k.p0, ti, fi, vi, ai);
     if (energy) vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
     bool wrapped = false;
     vp_one(k, ri, vi, ai, mi, wrapped);
@@ -451,7 +494,7 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
     over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
   }
   over = wave_max(over);
-  if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[F_REBUILD], 1);
+  if ((threadIdx.x & 63) == 0 && over > 0.0) atomicOr(&flags[k.rep * F_NFLAGS + F_REBUILD], 1);
   if (energy) block_partials(vals, part);
 }
 
@@ -460,20 +503,29 @@ __global__ void k_fused_integrate(KParams k, const double* __restrict__ f, doubl
 // returns at once unless the rebuild flag is set, so a step can enqueue the
 // rebuild unconditionally and let the skin test decide on the device.
 
-__global__ void k_zero_counts(KParams k, int* __restrict__ cell_count) { zero_counts(k, cell_count); }
+__global__ void k_zero_counts(int nc_tot, int* __restrict__ cell_count) {
+  zero_counts(nc_tot, cell_count);
+}
 
 __global__ void k_set_step(long long* __restrict__ d_step, long long step) { *d_step = step; }
 
-__global__ void k_set_flag(int* __restrict__ flags, int idx, int value) { flags[idx] = value; }
+// Sets flag `idx` of every replica.
+__global__ void k_set_flag(int* __restrict__ flags, int n_rep, int idx, int value) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_rep; q += gridDim.x * blockDim.x)
+    flags[q * F_NFLAGS + idx] = value;
+}
 
 // Cell of each particle, mod(int(r*inv_r_bin), n_bin) per axis, and a slot in
 // that cell (binning.f90:25-48).
-__global__ void k_cell_fill(KParams k, const double* __restrict__ r, int* __restrict__ cell_of,
+__global__ void k_cell_fill(const KParams* __restrict__ kb, const int* __restrict__ rmap,
+                            const double* __restrict__ r, int* __restrict__ cell_of,
                             int* __restrict__ cell_count, int* __restrict__ cell_slots,
-                            int* __restrict__ flags) {
+                            int* __restrict__ flags_all) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  int* flags = flags_all + k.rep * F_NFLAGS;
   if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.n) return;
+  if (i - k.p0 >= k.n) return;
   const int cx = cell_coord(r[3 * (size_t)i + 0], k.inv_r_bin[0], k.nb[0]);
   const int cy = cell_coord(r[3 * (size_t)i + 1], k.inv_r_bin[1], k.nb[1]);
   const int cz = cell_coord(r[3 * (size_t)i + 2], k.inv_r_bin[2], k.nb[2]);
@@ -482,11 +534,11 @@ __global__ void k_cell_fill(KParams k, const double* __restrict__ r, int* __rest
     cell_of[i] = 0;
     return;
   }
-  const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;
+  const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;  // within the replica's grid
   cell_of[i] = c;
-  const int slot = atomicAdd(&cell_count[c], 1);
+  const int slot = atomicAdd(&cell_count[k.c0 + c], 1);
   if (slot < k.cell_cap)
-    cell_slots[(size_t)c * k.cell_cap + slot] = i;
+    cell_slots[(size_t)(k.c0 + c) * k.cell_cap + slot] = i;
   else
     atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
 }
@@ -508,12 +560,15 @@ __global__ void k_cell_fill(KParams k, const double* __restrict__ r, int* __rest
 // cell's own particles relative to this cell.
 #define LTPP 8
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
-    KParams k, const double* __restrict__ r, const double* __restrict__ v,
+    const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
+    const double* __restrict__ r, const double* __restrict__ v,
     const int* __restrict__ type, const int* __restrict__ mic,
     const int* __restrict__ cell_count, const int* __restrict__ cell_slots,
     unsigned* __restrict__ nl, int* __restrict__ nn, int cap, double* __restrict__ r_ref,
-    int* __restrict__ flags, float4* __restrict__ pos4, float4* __restrict__ vel4) {
-  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the grid
+    int* __restrict__ flags_all, float4* __restrict__ pos4, float4* __restrict__ vel4) {
+  const KParams& k = kb[cell_rep[blockIdx.x]];
+  int* flags = flags_all + k.rep * F_NFLAGS;
+  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the replica
   extern __shared__ unsigned char sh_raw[];
   float4* cpos = (float4*)sh_raw;             // [9*cell_cap]
   int* cid = (int*)(cpos + 9 * k.cell_cap);   // [9*cell_cap]
@@ -524,7 +579,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   __shared__ int own[MFA_BLOCK / LTPP];
   __shared__ int m_own_s;
   __shared__ int blk_max;
-  const int c = blockIdx.x;
+  const int c = blockIdx.x - k.c0;  // cell within the replica's grid
+  const int cg = blockIdx.x;         // the same cell in the concatenated arrays
   const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
   const float Lf[3] = {(float)k.L[0], (float)k.L[1], (float)k.L[2]};
   const int lane = threadIdx.x & 63;
@@ -534,7 +590,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   // the own cell's particles, ascending id
   if (threadIdx.x == 0) {
     blk_max = 0;
-    m_own_s = min(cell_count[c], k.cell_cap);
+    m_own_s = min(cell_count[cg], k.cell_cap);
   }
   __syncthreads();
   const int m_own = m_own_s;
@@ -542,9 +598,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   // passes of MFA_BLOCK/LTPP particles (one lane group each)
   for (int p0 = 0; p0 < m_own; p0 += MFA_BLOCK / LTPP) {
     const int np = min(MFA_BLOCK / LTPP, m_own - p0);
-    if (threadIdx.x < m_own) tmp[threadIdx.x] = cell_slots[(size_t)c * k.cell_cap + threadIdx.x];
+    if (threadIdx.x < m_own) tmp[threadIdx.x] = cell_slots[(size_t)cg * k.cell_cap + threadIdx.x];
     for (int t = threadIdx.x + MFA_BLOCK; t < m_own; t += MFA_BLOCK)
-      tmp[t] = cell_slots[(size_t)c * k.cell_cap + t];
+      tmp[t] = cell_slots[(size_t)cg * k.cell_cap + t];
     __syncthreads();
     for (int t = threadIdx.x; t < m_own; t += MFA_BLOCK) {
       const int id = tmp[t];
@@ -573,7 +629,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
         const int kx = (jx + k.nb[0]) % k.nb[0];
         const int ky = (jy + k.nb[1]) % k.nb[1];
         const int kz = (jz + k.nb[2]) % k.nb[2];
-        const int cc = (kz * k.nb[1] + ky) * k.nb[0] + kx;
+        const int cc = k.c0 + (kz * k.nb[1] + ky) * k.nb[0] + kx;
         cell_base[q] = cc;
         off[q + 1] = (k.symmetry == 0 && (jz < 0 || jz >= k.nb[2])) ? 0
                                                                     : min(cell_count[cc], k.cell_cap);
@@ -846,7 +902,9 @@ __device__ double bend_angle(const KParams& k, const double* __restrict__ r, int
 }
 
 template <int TPP>
-__global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* __restrict__ r,
+__global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict__ kb,
+                                                       const int* __restrict__ rmap,
+                                                       const double* __restrict__ r,
                                                        const double* __restrict__ v,
                                                        const int* __restrict__ type,
                                                        const unsigned* __restrict__ nl,
@@ -854,19 +912,22 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
                                                        double* __restrict__ f,
                                                        const long long* __restrict__ d_step,
                                                        int mask, double* __restrict__ part,
-                                                       int* __restrict__ flags) {
+                                                       int* __restrict__ flags_all) {
+  MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
+  int* flags = flags_all + k.rep * F_NFLAGS;
   __shared__ int err_s;
   if (threadIdx.x == 0) err_s = flags[F_ERROR];
   __syncthreads();
   if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / TPP;
+  const int i = gtid / TPP;   // in the concatenated arrays
+  const int il = i - k.p0;    // within the replica (chains, heads, random keys)
   const int sub = gtid % TPP;
-  if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  if (il == 0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const double r2min = switch_on_r2min(k, step);
-  const bool active = i < k.n;
-  const int ii = active ? i : 0;
+  const bool active = il < k.n;
+  const int ii = active ? i : k.p0;
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
   const double vxi = v[3 * (size_t)ii + 0], vyi = v[3 * (size_t)ii + 1], vzi = v[3 * (size_t)ii + 2];
   const int ti = type[ii] - 1;
@@ -920,7 +981,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
         vir += 0.5 * (-(rd * r2raw));
       }
       if (dpd) {
-        const double xi_n = pair_normal_f64(k.seed, step, i, j);
+        const double xi_n = pair_normal_f64(k.seed, step, il, j - k.p0);
         const double wr = 1.0 - sqrt(w_arg);
         const double wd = wr * wr;
         const double ex = dx * inv_sqrt_r_2, ey = dy * inv_sqrt_r_2, ez = dz * inv_sqrt_r_2;
@@ -959,12 +1020,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
   if (active && sub == 0) {
     if (mask & TM_BOND) {
       int base, len;
-      chain_of(k, i, base, len);
+      chain_of(k, il, base, len);
       for (int side = 0; side < 2; ++side) {
         // bond (lo, lo+1): side 0 -> (i-1, i), side 1 -> (i, i+1)
-        const int lo = side == 0 ? i - 1 : i;
-        if (len < 2 || lo < base || lo + 1 >= base + len) continue;
-        const int hi = lo + 1;
+        const int lol = side == 0 ? il - 1 : il;
+        if (len < 2 || lol < base || lol + 1 >= base + len) continue;
+        const int lo = k.p0 + lol, hi = lo + 1;
         double dx = r[3 * (size_t)lo + 0] - r[3 * (size_t)hi + 0];
         double dy = r[3 * (size_t)lo + 1] - r[3 * (size_t)hi + 1];
         double dz = r[3 * (size_t)lo + 2] - r[3 * (size_t)hi + 2];
@@ -984,12 +1045,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
         }
       }
     }
-    if ((mask & TM_BEND) && k.bending && i < k.n_brush) {
-      const int base = (i / k.n_mon) * k.n_mon, m = i - base;
+    if ((mask & TM_BEND) && k.bending && il < k.n_brush) {
+      const int base = (il / k.n_mon) * k.n_mon, m = il - base;
       for (int cen = m - 1; cen <= m + 1; ++cen) {
         if (cen < 1 || cen > k.n_mon - 2) continue;
         double fp[3], fn[3];
-        const double eb = bend_angle(k, r, base + cen, fp, fn);
+        const double eb = bend_angle(k, r, k.p0 + base + cen, fp, fn);
         if (cen == m) {
           e_bend += eb;
           ax += -fp[0] - fn[0];
@@ -1046,11 +1107,14 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(KParams k, const double* 
 
 template <int TPP, int ENERGY>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
-    KParams k, const double* __restrict__ r, const float4* __restrict__ pos4,
-    const float4* __restrict__ vel4, const unsigned* __restrict__ nl, const int* __restrict__ nn,
-    int cap, double* __restrict__ f, const long long* __restrict__ d_step, int mask,
-    double* __restrict__ part, int* __restrict__ flags) {
+    const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
+    const float4* __restrict__ pos4, const float4* __restrict__ vel4,
+    const unsigned* __restrict__ nl, const int* __restrict__ nn, int cap, double* __restrict__ f,
+    const long long* __restrict__ d_step, int mask, double* __restrict__ part,
+    int* __restrict__ flags_all) {
 #pragma clang fp contract(fast)
+  MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
+  int* flags = flags_all + k.rep * F_NFLAGS;
   // cell offsets -(d)*r_bin of the 27 codes, and the FP32 pair tables
   __shared__ float4 s_off[27];
   __shared__ float4 s_tab[MFA_NTYPE * MFA_NTYPE];  // range_2, 4 eps, sigma_2, e_shift
@@ -1072,15 +1136,16 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   __syncthreads();
   if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / TPP;
+  const int i = gtid / TPP;   // in the concatenated arrays
+  const int il = i - k.p0;    // within the replica (chains, heads, random keys)
   const int sub = gtid % TPP;
-  if (gtid == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  if (il == 0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const float r2min = (float)switch_on_r2min(k, step);
   const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd2 = (float)(1.0 / k.r_cut_dpd_2);
   const float sigf = (float)k.sig, gam = (float)k.friction1, margin = k.ff_margin;
-  const bool active = i < k.n;
-  const int ii = active ? i : 0;
+  const bool active = il < k.n;
+  const int ii = active ? i : k.p0;
   const float4 pi4 = pos4[ii];
   const float4 vi4 = vel4[ii];
   const int ti = __float_as_int(pi4.w);
@@ -1143,7 +1208,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         }
       }
       if (dpd) {
-        const float xi_n = pair_normal_f32(k.seed, step, i, j);
+        const float xi_n = pair_normal_f32(k.seed, step, il, j - k.p0);
         const float w_arg = k.dpd_cut_off_on ? r2 * inv_rdpd2 : r2 * s_inv_range_2[ti * MFA_NTYPE + tj];
         const float wr = 1.0f - __builtin_amdgcn_sqrtf(w_arg);  // v_sqrt_f32, ~1 ulp
         const float4 vj = vel4[j];
@@ -1163,14 +1228,14 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 
   // Bonds and walls: lane 0 of the group, from the FP64 positions.
   if (active && sub == 0 && (ENERGY || (mask & (TM_BOND | TM_WALL)))) zi = r[3 * (size_t)i + 2];
-  if (active && sub == 0 && (mask & TM_BEND) && k.bending && i < k.n_brush) {
+  if (active && sub == 0 && (mask & TM_BEND) && k.bending && il < k.n_brush) {
     // each term through float and fixed point; the centre takes minus the
     // fixed-point sum of its neighbours' terms, so every angle sums to zero
-    const int base = (i / k.n_mon) * k.n_mon, m = i - base;
+    const int base = (il / k.n_mon) * k.n_mon, m = il - base;
     for (int cen = m - 1; cen <= m + 1; ++cen) {
       if (cen < 1 || cen > k.n_mon - 2) continue;
       double fp[3], fn[3];
-      const double eb = bend_angle(k, r, base + cen, fp, fn);
+      const double eb = bend_angle(k, r, k.p0 + base + cen, fp, fn);
       long long px = 0, py = 0, pz = 0, nx = 0, ny = 0, nz = 0;
       fix_add3(px, py, pz, (float)fp[0], (float)fp[1], (float)fp[2], bad);
       fix_add3(nx, ny, nz, (float)fn[0], (float)fn[1], (float)fn[2], bad);
@@ -1193,11 +1258,11 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   if (active && sub == 0 && (mask & (TM_BOND | TM_WALL))) {
     if (mask & TM_BOND) {
       int base, len;
-      chain_of(k, i, base, len);
+      chain_of(k, il, base, len);
       for (int side = 0; side < 2; ++side) {
-        const int lo = side == 0 ? i - 1 : i;
-        if (len < 2 || lo < base || lo + 1 >= base + len) continue;
-        const int hi = lo + 1;
+        const int lol = side == 0 ? il - 1 : il;
+        if (len < 2 || lol < base || lol + 1 >= base + len) continue;
+        const int lo = k.p0 + lol, hi = lo + 1;
         double dx = r[3 * (size_t)lo + 0] - r[3 * (size_t)hi + 0];
         double dy = r[3 * (size_t)lo + 1] - r[3 * (size_t)hi + 1];
         double dz = r[3 * (size_t)lo + 2] - r[3 * (size_t)hi + 2];
@@ -1564,11 +1629,11 @@ static size_t list_lds_bytes(int cell_cap) {
 
 static int launch_rebuild(mfa_ctx* c) {
   const int g = grid_of(c->n);
-  hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->cell_of,
-                     c->cell_count, c->cell_slots, c->flags);
+  hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
+                     c->cell_of, c->cell_count, c->cell_slots, c->flags);
   hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c->cell_cap),
-                     c->stream, c->k, c->r, c->v, c->type, c->mic, c->cell_count, c->cell_slots,
-                     c->nl, c->nn, c->cap, c->r_ref, c->flags, c->pos4, c->vel4);
+                     c->stream, c->d_kp, c->d_cell_rep, c->r, c->v, c->type, c->mic, c->cell_count,
+                     c->cell_slots, c->nl, c->nn, c->cap, c->r_ref, c->flags, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -1578,16 +1643,17 @@ static void launch_force_t(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED) {
     if (mask & TM_ENERGY)
-      hipLaunchKernelGGL((k_force_mixed<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                         c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step, mask,
-                         c->part, c->flags);
+      hipLaunchKernelGGL((k_force_mixed<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                         c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step,
+                         mask, c->part, c->flags);
     else
-      hipLaunchKernelGGL((k_force_mixed<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                         c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step, mask,
-                         c->part, c->flags);
+      hipLaunchKernelGGL((k_force_mixed<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                         c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step,
+                         mask, c->part, c->flags);
   } else {
-    hipLaunchKernelGGL((k_force64<TPP>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                       c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part, c->flags);
+    hipLaunchKernelGGL((k_force64<TPP>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                       c->r, c->v, c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part,
+                       c->flags);
   }
 }
 
@@ -1656,9 +1722,10 @@ static int launch_fused(mfa_ctx* c, int energy) {
     rc = timed_begin(c, c->ev_integ, c->ev_integ_used);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_fused_integrate, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                     c->f, c->v, c->r, c->mic, c->r_ref, c->type, c->flags, c->part, energy,
-                     c->cell_count, c->d_step, c->cell_of, c->pos4, c->vel4);
+  hipLaunchKernelGGL(k_fused_integrate, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream,
+                     c->d_kp, c->d_rmap, c->f, c->v, c->r, c->mic, c->r_ref, c->type, c->flags,
+                     c->part, energy, c->n_cells, c->cell_count, c->d_step, c->cell_of, c->pos4,
+                     c->vel4);
   HIPCHK(c, hipGetLastError());
   if (timed) return timed_end(c, c->ev_integ, c->ev_integ_used);
   return MFA_OK;
@@ -1670,9 +1737,10 @@ static int set_step(mfa_ctx* c, int64_t step) {
   return MFA_OK;
 }
 
-static int reduce_terms(mfa_ctx* c, int nblocks, double* out) {
+// Sums of the block partials [b0, b0 + nblocks) (one replica's blocks).
+static int reduce_terms_range(mfa_ctx* c, int b0, int nblocks, double* out) {
   hipLaunchKernelGGL(k_reduce_partials, dim3(MFA_NTERMS), dim3(MFA_BLOCK), 0, c->stream, c->part,
-                     nblocks, c->red);
+                     b0, nblocks, c->red);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->h_red, c->red, MFA_NTERMS * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
@@ -1681,19 +1749,50 @@ static int reduce_terms(mfa_ctx* c, int nblocks, double* out) {
   return MFA_OK;
 }
 
+// Blocks of the force pass (TPP lanes per particle) that cover replica q.
+static void force_block_range(const mfa_ctx* c, int q, int* b0, int* nb) {
+  const int per = MFA_BLOCK / c->tpp;  // particles per block; divides 256
+  *b0 = c->ks[(size_t)q].p0 / per;
+  *nb = (c->ks[(size_t)q].n + per - 1) / per;
+}
+static void particle_block_range(const mfa_ctx* c, int q, int* b0, int* nb) {
+  *b0 = c->ks[(size_t)q].p0 / MFA_BLOCK;
+  *nb = grid_of(c->ks[(size_t)q].n);
+}
+
+// Replica 0's sums of a force pass / an integrate pass.
+static int reduce_terms(mfa_ctx* c, int nblocks, double* out) {
+  (void)nblocks;
+  int b0, nb;
+  force_block_range(c, 0, &b0, &nb);
+  return reduce_terms_range(c, b0, nb, out);
+}
+static int reduce_terms_particles(mfa_ctx* c, double* out) {
+  int b0, nb;
+  particle_block_range(c, 0, &b0, &nb);
+  return reduce_terms_range(c, b0, nb, out);
+}
+
 static int check_flags(mfa_ctx* c) {
-  HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags, F_NFLAGS * sizeof(int32_t), hipMemcpyDeviceToHost,
-                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags, (size_t)c->n_rep * F_NFLAGS * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const int e = c->h_flags[F_ERROR];
-  if (e == MFA_ERR_LIST_OVERFLOW)
-    return set_err(c, e, "neighbour list overflow: %d neighbours > capacity %d",
-                   c->h_flags[F_MAXNEIGH], c->cap);
-  if (e == MFA_ERR_CELL_OVERFLOW)
-    return set_err(c, e, "binning cell overflow: more than %d particles in a cell", c->cell_cap);
-  if (e == MFA_ERR_CELL) return set_err(c, e, "particle outside the cell grid");
-  if (e == MFA_ERR_NAN) return set_err(c, e, "non-finite force");
-  if (e != 0) return set_err(c, e, "device error %d", e);
+  for (int q = 0; q < c->n_rep; ++q) {
+    const int* h = c->h_flags + (size_t)q * F_NFLAGS;
+    const int e = h[F_ERROR];
+    if (e == 0) continue;
+    char who[32] = "";
+    if (c->n_rep > 1) snprintf(who, sizeof who, "replica %d: ", q);
+    if (e == MFA_ERR_LIST_OVERFLOW)
+      return set_err(c, e, "%sneighbour list overflow: %d neighbours > capacity %d", who,
+                     h[F_MAXNEIGH], c->cap);
+    if (e == MFA_ERR_CELL_OVERFLOW)
+      return set_err(c, e, "%sbinning cell overflow: more than %d particles in a cell", who,
+                     c->cell_cap);
+    if (e == MFA_ERR_CELL) return set_err(c, e, "%sparticle outside the cell grid", who);
+    if (e == MFA_ERR_NAN) return set_err(c, e, "%snon-finite force", who);
+    return set_err(c, e, "%sdevice error %d", who, e);
+  }
   return MFA_OK;
 }
 
@@ -1701,12 +1800,14 @@ static int check_flags(mfa_ctx* c) {
 // state): clear the counts, raise the flag, rebuild, lower the flag.
 static int ensure_list(mfa_ctx* c) {
   if (!c->need_rebuild) return MFA_OK;
-  hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                     c->cell_count);
-  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->stream, c->flags, (int)F_REBUILD, 1);
+  hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream,
+                     c->n_cells, c->cell_count);
+  hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                     c->n_rep, (int)F_REBUILD, 1);
   int rc = launch_rebuild(c);
   if (rc) return rc;
-  HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                     c->n_rep, (int)F_REBUILD, 0);
   c->need_rebuild = false;
   return check_flags(c);
 }
@@ -1845,7 +1946,7 @@ static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
   std::vector<int> qi;
   std::vector<double> qv;
   double self = 0.;
-  for (int i = 0; i < c->n; ++i) {
+  for (int i = 0; i < c->p.n_part; ++i) {
     const double q = c->p.charge[a_type[i] - 1];
     self = self - q * q;
     if (q != 0.0) {
@@ -1906,12 +2007,6 @@ static int ewald_energies(mfa_ctx* c, double* uk, double* udip) {
   return MFA_OK;
 }
 
-static KParams make_kparams_ctx(mfa_ctx* c) {
-  KParams k = make_kparams(c->p);
-  k.cell_cap = c->cell_cap;
-  return k;
-}
-
 // ---------------------------------------------------------------------------
 // C ABI
 
@@ -1919,22 +2014,33 @@ extern "C" const char* mfa_last_error(mfa_ctx* c) {
   return c ? c->err.c_str() : g_noctx_err.c_str();
 }
 
-extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t list_capacity,
-                              mfa_ctx** out) {
-  if (!p || !out) return set_err(nullptr, MFA_ERR_ARG, "null argument");
+// One context for n_rep replicas (n_rep = 1 for a single system).  Replica q
+// owns particles [p0, p0 + n_part) with p0 a multiple of 256 and cells
+// [c0, c0 + n_cells); cell and list capacities are the largest any replica
+// needs.  Replicas share the compile-time switches of the reference (system,
+// symmetry, thermostat, DPD cut-off; bending may differ) and carry no charges.
+static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precision,
+                           int32_t list_capacity, mfa_ctx** out) {
+  if (!ps || !out || n_rep < 1) return set_err(nullptr, MFA_ERR_ARG, "null argument");
   if (precision != MFA_FP64 && precision != MFA_MIXED)
     return set_err(nullptr, MFA_ERR_ARG, "unknown precision %d", precision);
-  if (p->n_part <= 0) return set_err(nullptr, MFA_ERR_ARG, "n_part must be positive");
-  if ((unsigned)p->n_part > MFA_JMASK)
-    return set_err(nullptr, MFA_ERR_ARG, "at most %u particles per context", MFA_JMASK);
+  for (int q = 0; q < n_rep; ++q) {
+    const mfa_params& p = ps[q];
+    if (p.n_part <= 0) return set_err(nullptr, MFA_ERR_ARG, "replica %d: n_part must be positive", q);
+    if (n_rep > 1 && (p.system != ps[0].system || p.symmetry != ps[0].symmetry ||
+                      p.thermostat != ps[0].thermostat || p.dpd_cut_off_on != ps[0].dpd_cut_off_on))
+      return set_err(nullptr, MFA_ERR_ARG, "replica %d: switches differ from replica 0", q);
+    if (n_rep > 1 && p.coulomb)
+      return set_err(nullptr, MFA_ERR_ARG, "charged systems run one per context");
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, MFA_ERR_HIP, "no HIP device");
   mfa_ctx* c = new mfa_ctx();
-  c->p = *p;
+  c->p = ps[0];
+  c->n_rep = n_rep;
+  c->ps.assign(ps, ps + n_rep);
   c->precision = precision;
-  c->n = p->n_part;
-  c->n_cells = p->n_bin[0] * p->n_bin[1] * p->n_bin[2];
   c->tpp = 8;
   c->need_rebuild = true;
   c->timing = false;
@@ -1943,33 +2049,51 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   if (const char* e = getenv("MFA_GRAPH_STEPS")) c->graph_steps = atoi(e);  // 0: direct launches
   if (c->graph_steps < 0 || c->graph_steps > 1024) c->graph_steps = 16;
   c->gexec = nullptr;
-  const double vol = p->boundary[0] * p->boundary[1] * p->boundary[2];
-  const double dens = (double)p->n_part / vol;
-  if (list_capacity <= 0) {
-    // expected full-list count at the mean density, with room for fluctuations
-    double rl2 = 0.0;
-    for (int t = 0; t < MFA_NTYPE; ++t)
-      for (int u = 0; u < MFA_NTYPE; ++u) rl2 = p->list_2[t][u] > rl2 ? p->list_2[t][u] : rl2;
-    const double rl = sqrt(rl2);
-    const double expect = dens * 4.18879020478639 * rl * rl * rl;
-    int cap = (int)(2.0 * expect + 48.0);
-    list_capacity = (cap + 31) / 32 * 32;
-  }
-  c->cap = list_capacity;
-  {
+  // capacities and layout
+  int cap = list_capacity, cell_cap = 0;
+  int64_t ntot = 0, ctot = 0;
+  for (int q = 0; q < n_rep; ++q) {
+    const mfa_params& p = ps[q];
+    const double vol = p.boundary[0] * p.boundary[1] * p.boundary[2];
+    const double dens = (double)p.n_part / vol;
+    if (list_capacity <= 0) {
+      // expected full-list count at the mean density, with room for fluctuations
+      double rl2 = 0.0;
+      for (int t = 0; t < MFA_NTYPE; ++t)
+        for (int u = 0; u < MFA_NTYPE; ++u) rl2 = p.list_2[t][u] > rl2 ? p.list_2[t][u] : rl2;
+      const double rl = sqrt(rl2);
+      const double expect = dens * 4.18879020478639 * rl * rl * rl;
+      const int cq = (((int)(2.0 * expect + 48.0)) + 31) / 32 * 32;
+      cap = cq > cap ? cq : cap;
+    }
     // cell capacity: twice the mean occupancy plus 32 (the reference allows 300)
-    const double occ = dens * p->r_bin[0] * p->r_bin[1] * p->r_bin[2];
-    int cc = (int)(2.0 * occ + 32.0);
-    c->cell_cap = (cc + 31) / 32 * 32;
+    const double occ = dens * p.r_bin[0] * p.r_bin[1] * p.r_bin[2];
+    const int cc = (((int)(2.0 * occ + 32.0)) + 31) / 32 * 32;
+    cell_cap = cc > cell_cap ? cc : cell_cap;
+    KParams k = make_kparams(p);
+    k.p0 = (int)ntot;
+    k.c0 = (int)ctot;
+    k.rep = q;
+    c->ks.push_back(k);
+    ntot += ((int64_t)p.n_part + MFA_BLOCK - 1) / MFA_BLOCK * MFA_BLOCK;
+    ctot += (int64_t)p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
   }
-  c->k = make_kparams_ctx(c);
+  if (ntot > (int64_t)MFA_JMASK) {
+    delete c;
+    return set_err(nullptr, MFA_ERR_ARG, "at most %u particles per context", MFA_JMASK);
+  }
+  for (KParams& k : c->ks) k.cell_cap = cell_cap;
+  c->cap = cap;
+  c->cell_cap = cell_cap;
+  c->n = (int)ntot;
+  c->n_cells = (int)ctot;
   {
     // LDS of the list build: the 27-cell neighbourhood as (float4, id)
     const size_t lds = list_lds_bytes(c->cell_cap);
     if (lds > 150 * 1024) {
       delete c;
-      return set_err(nullptr, MFA_ERR_ARG,
-                     "binning cells too crowded for the list build (capacity %d)", c->cell_cap);
+      return set_err(nullptr, MFA_ERR_ARG, "binning cells too crowded for the list build (capacity %d)",
+                     cell_cap);
     }
     if (hipFuncSetAttribute((const void*)k_build_list, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess) {
@@ -2002,15 +2126,18 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   ALLOC(c->nn, n * sizeof(int32_t));
   ALLOC(c->pos4, n * sizeof(float4));
   ALLOC(c->vel4, n * sizeof(float4));
-  ALLOC(c->flags, F_NFLAGS * sizeof(int32_t));
+  ALLOC(c->flags, (size_t)n_rep * F_NFLAGS * sizeof(int32_t));
   ALLOC(c->d_step, sizeof(long long));
   ALLOC(c->part, (size_t)c->max_blocks * MFA_NTERMS * sizeof(double));
-  ALLOC(c->red, MFA_NTERMS * sizeof(double));
+  ALLOC(c->red, (size_t)n_rep * MFA_NTERMS * sizeof(double));
+  ALLOC(c->d_kp, (size_t)n_rep * sizeof(KParams));
+  ALLOC(c->d_rmap, (n / MFA_BLOCK + 1) * sizeof(int));
+  ALLOC(c->d_cell_rep, (size_t)c->n_cells * sizeof(int));
 #undef ALLOC
-  if (hipHostMalloc((void**)&c->h_flags, F_NFLAGS * sizeof(int32_t), hipHostMallocDefault) !=
-          hipSuccess ||
-      hipHostMalloc((void**)&c->h_red, MFA_NTERMS * sizeof(double), hipHostMallocDefault) !=
-          hipSuccess) {
+  if (hipHostMalloc((void**)&c->h_flags, (size_t)n_rep * F_NFLAGS * sizeof(int32_t),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_red, (size_t)n_rep * MFA_NTERMS * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess) {
     mfa_destroy(c);
     return set_err(nullptr, MFA_ERR_NOMEM, "hipHostMalloc failed");
   }
@@ -2018,16 +2145,41 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
     mfa_destroy(c);
     return set_err(nullptr, MFA_ERR_HIP, "hipStreamCreate failed");
   }
+  {
+    std::vector<int> rmap(n / MFA_BLOCK + 1, 0), crep((size_t)c->n_cells, 0);
+    for (int q = 0; q < n_rep; ++q) {
+      const KParams& k = c->ks[(size_t)q];
+      const int nb = grid_of(k.n);
+      for (int b = 0; b < nb; ++b) rmap[(size_t)(k.p0 / MFA_BLOCK + b)] = q;
+      for (int cc = 0; cc < k.n_cells; ++cc) crep[(size_t)(k.c0 + cc)] = q;
+    }
+    if (hipMemcpy(c->d_kp, c->ks.data(), (size_t)n_rep * sizeof(KParams), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->d_rmap, rmap.data(), rmap.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->d_cell_rep, crep.data(), crep.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      mfa_destroy(c);
+      return set_err(nullptr, MFA_ERR_HIP, "upload of the replica tables failed");
+    }
+  }
+  c->rep_obs.assign((size_t)n_rep, mfa_obs());
   c->ev_force_used = c->ev_integ_used = 0;
-  if (p->coulomb) {
+  if (c->p.coulomb) {
     const int rc = ewald_setup(c);
     if (rc) {
       mfa_destroy(c);
       return rc;
     }
   }
+  // padding particles stay type 1 at the origin and never interact
+  std::vector<int32_t> ones(n, 1);
+  hipMemcpyAsync(c->type, ones.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+  hipMemsetAsync(c->r, 0, n * 3 * sizeof(double), c->stream);
+  hipMemsetAsync(c->v, 0, n * 3 * sizeof(double), c->stream);
+  hipMemsetAsync(c->r_ref, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
-  hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->flags, 0, (size_t)n_rep * F_NFLAGS * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->d_step, 0, sizeof(long long), c->stream);
   hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->f, 0, n * 3 * sizeof(double), c->stream);
@@ -2042,13 +2194,24 @@ extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t li
   return MFA_OK;
 }
 
+extern "C" int32_t mfa_create(const mfa_params* p, int32_t precision, int32_t list_capacity,
+                              mfa_ctx** out) {
+  return create_impl(p, 1, precision, list_capacity, out);
+}
+
+extern "C" int32_t mfa_create_batch(const mfa_params* params, int32_t n_rep, int32_t precision,
+                                    int32_t list_capacity, mfa_ctx** out) {
+  return create_impl(params, n_rep, precision, list_capacity, out);
+}
+
 extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   if (!c) return MFA_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void* ptrs[] = {c->r,  c->v,          c->a,     c->f,     c->r_ref,  c->type,
-                  c->mic, c->cell_of,   c->cell_count, c->cell_slots, c->nl, c->nn,
-                  c->pos4, c->vel4,     c->flags, c->d_step, c->part,   c->red};
+  void* ptrs[] = {c->r,   c->v,       c->a,          c->f,          c->r_ref, c->type,
+                  c->mic, c->cell_of, c->cell_count, c->cell_slots, c->nl,    c->nn,
+                  c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
+                  c->d_kp, c->d_rmap, c->d_cell_rep};
   for (void* q : ptrs)
     if (q) hipFree(q);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
@@ -2087,22 +2250,26 @@ extern "C" int32_t mfa_set_graph_steps(mfa_ctx* c, int32_t steps) {
   return MFA_OK;
 }
 
-extern "C" int32_t mfa_set_state(mfa_ctx* c, const double* r0, const double* v, const double* a,
-                                 const int32_t* a_type, const int32_t* mic) {
+// Loads replica q's state (the reference's (3,N) arrays).
+static int32_t set_state_impl(mfa_ctx* c, int q, const double* r0, const double* v, const double* a,
+                              const int32_t* a_type, const int32_t* mic) {
   if (!c || !r0 || !v || !a || !a_type) return MFA_ERR_ARG;
-  const size_t n = (size_t)c->n;
+  if (q < 0 || q >= c->n_rep) return set_err(c, MFA_ERR_ARG, "replica %d out of range", q);
+  const KParams& k = c->ks[(size_t)q];
+  const size_t n = (size_t)k.n, o = (size_t)k.p0;
   for (size_t i = 0; i < n; ++i)
     if (a_type[i] < 1 || a_type[i] > MFA_NTYPE)
       return set_err(c, MFA_ERR_ARG, "a_type(%zu) = %d outside 1..%d", i + 1, a_type[i], MFA_NTYPE);
-  HIPCHK(c, hipMemcpyAsync(c->r, r0, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->v, v, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->a, a, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->type, a_type, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->r + 3 * o, r0, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->v + 3 * o, v, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->a + 3 * o, a, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->type + o, a_type, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
   if (mic)
-    HIPCHK(c, hipMemcpyAsync(c->mic, mic, n * 3 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->mic + 3 * o, mic, n * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
   else
-    HIPCHK(c, hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->flags, 0, F_NFLAGS * sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mic + 3 * o, 0, n * 3 * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->flags, 0, (size_t)c->n_rep * F_NFLAGS * sizeof(int32_t), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->need_rebuild = true;
   if (c->ewald) {
@@ -2112,26 +2279,60 @@ extern "C" int32_t mfa_set_state(mfa_ctx* c, const double* r0, const double* v, 
   return MFA_OK;
 }
 
+static int32_t get_state_impl(mfa_ctx* c, int q, double* r0, double* v, double* a, double* force,
+                              int32_t* mic) {
+  if (!c) return MFA_ERR_ARG;
+  if (q < 0 || q >= c->n_rep) return set_err(c, MFA_ERR_ARG, "replica %d out of range", q);
+  const KParams& k = c->ks[(size_t)q];
+  const size_t n = (size_t)k.n, o = (size_t)k.p0;
+  if (r0)
+    HIPCHK(c, hipMemcpyAsync(r0, c->r + 3 * o, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (v)
+    HIPCHK(c, hipMemcpyAsync(v, c->v + 3 * o, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (a)
+    HIPCHK(c, hipMemcpyAsync(a, c->a + 3 * o, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (force)
+    HIPCHK(c, hipMemcpyAsync(force, c->f + 3 * o, n * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+  if (mic)
+    HIPCHK(c, hipMemcpyAsync(mic, c->mic + 3 * o, n * 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_set_state(mfa_ctx* c, const double* r0, const double* v, const double* a,
+                                 const int32_t* a_type, const int32_t* mic) {
+  return set_state_impl(c, 0, r0, v, a, a_type, mic);
+}
+
 extern "C" int32_t mfa_get_state(mfa_ctx* c, double* r0, double* v, double* a, double* force,
                                  int32_t* mic) {
-  if (!c) return MFA_ERR_ARG;
-  const size_t n = (size_t)c->n;
-  if (r0) HIPCHK(c, hipMemcpyAsync(r0, c->r, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (v) HIPCHK(c, hipMemcpyAsync(v, c->v, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (a) HIPCHK(c, hipMemcpyAsync(a, c->a, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (force)
-    HIPCHK(c, hipMemcpyAsync(force, c->f, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (mic)
-    HIPCHK(c, hipMemcpyAsync(mic, c->mic, n * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return get_state_impl(c, 0, r0, v, a, force, mic);
+}
+
+extern "C" int32_t mfa_set_state_replica(mfa_ctx* c, int32_t replica, const double* r0,
+                                         const double* v, const double* a, const int32_t* a_type,
+                                         const int32_t* mic) {
+  return set_state_impl(c, replica, r0, v, a, a_type, mic);
+}
+
+extern "C" int32_t mfa_get_state_replica(mfa_ctx* c, int32_t replica, double* r0, double* v,
+                                         double* a, double* force, int32_t* mic) {
+  return get_state_impl(c, replica, r0, v, a, force, mic);
+}
+
+extern "C" int32_t mfa_replica_obs(mfa_ctx* c, int32_t replica, mfa_obs* obs) {
+  if (!c || !obs || replica < 0 || replica >= c->n_rep) return MFA_ERR_ARG;
+  *obs = c->rep_obs[(size_t)replica];
   return MFA_OK;
 }
 
 extern "C" int32_t mfa_verlet_positions(mfa_ctx* c) {
   if (!c) return MFA_ERR_ARG;
-  hipLaunchKernelGGL(k_verlet_positions, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k,
-                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0, c->cell_count, c->type,
-                     c->cell_of, c->pos4, c->vel4);
+  hipLaunchKernelGGL(k_verlet_positions, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream,
+                     c->d_kp, c->d_rmap, c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0, c->n_cells,
+                     c->cell_count, c->type, c->cell_of, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFA_OK;
@@ -2143,14 +2344,17 @@ extern "C" int32_t mfa_check_skin(mfa_ctx* c, int32_t* f_skin) {
     *f_skin = 1;
     return MFA_OK;
   }
-  HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
-  hipLaunchKernelGGL(k_check_skin, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r,
-                     c->r_ref, c->mic, c->flags);
+  hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                     c->n_rep, (int)F_REBUILD, 0);
+  hipLaunchKernelGGL(k_check_skin, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                     c->d_rmap, c->r, c->r_ref, c->mic, c->flags);
   HIPCHK(c, hipGetLastError());
   int rc = check_flags(c);
   if (rc) return rc;
-  *f_skin = c->h_flags[F_REBUILD] ? 1 : 0;
-  HIPCHK(c, hipMemsetAsync(c->flags + F_REBUILD, 0, sizeof(int), c->stream));
+  *f_skin = 0;
+  for (int q = 0; q < c->n_rep; ++q) *f_skin |= c->h_flags[(size_t)q * F_NFLAGS + F_REBUILD] ? 1 : 0;
+  hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                     c->n_rep, (int)F_REBUILD, 0);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFA_OK;
 }
@@ -2165,10 +2369,11 @@ extern "C" int32_t mfa_get_cells(mfa_ctx* c, int32_t* cell_xyz) {
   if (!c || !cell_xyz) return MFA_ERR_ARG;
   int rc = ensure_list(c);
   if (rc) return rc;
-  std::vector<int32_t> co((size_t)c->n);
-  HIPCHK(c, hipMemcpy(co.data(), c->cell_of, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int n0 = c->ks[0].n;  // replica 0
+  std::vector<int32_t> co((size_t)n0);
+  HIPCHK(c, hipMemcpy(co.data(), c->cell_of, (size_t)n0 * sizeof(int32_t), hipMemcpyDeviceToHost));
   const int nx = c->p.n_bin[0], ny = c->p.n_bin[1];
-  for (int i = 0; i < c->n; ++i) {
+  for (int i = 0; i < n0; ++i) {
     const int cc = co[(size_t)i];
     cell_xyz[3 * (size_t)i + 0] = cc % nx;
     cell_xyz[3 * (size_t)i + 1] = (cc / nx) % ny;
@@ -2181,12 +2386,13 @@ extern "C" int32_t mfa_get_pairs(mfa_ctx* c, int32_t* pairs, int64_t cap, int64_
   if (!c || !n_pairs) return MFA_ERR_ARG;
   int rc = ensure_list(c);
   if (rc) return rc;
-  std::vector<int32_t> nn((size_t)c->n);
-  std::vector<unsigned> nl((size_t)c->n * c->cap);
+  const int n0 = c->ks[0].n;  // replica 0
+  std::vector<int32_t> nn((size_t)n0);
+  std::vector<unsigned> nl((size_t)n0 * c->cap);
   HIPCHK(c, hipMemcpy(nn.data(), c->nn, nn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(nl.data(), c->nl, nl.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
   int64_t m = 0;
-  for (int i = 0; i < c->n; ++i)
+  for (int i = 0; i < n0; ++i)
     for (int q = 0; q < nn[(size_t)i]; ++q) {
       const int j = (int)(nl[(size_t)i * c->cap + q] & MFA_JMASK);
       if (i < j) {
@@ -2273,12 +2479,12 @@ extern "C" int32_t mfa_bending(mfa_ctx* c, int32_t mode, double* v_bend) {
 extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
   if (!c) return MFA_ERR_ARG;
   const int g = grid_of(c->n);
-  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,
-                     c->a, c->type, c->part, t_kin ? 1 : 0, c->vel4);
+  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->f, c->v, c->a, c->type, c->part, t_kin ? 1 : 0, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (t_kin) {
     double t[MFA_NTERMS];
-    int rc = reduce_terms(c, g, t);
+    int rc = reduce_terms_particles(c, t);
     if (rc) return rc;
     *t_kin = 0.5 * t[T_TKIN];
   }
@@ -2365,12 +2571,13 @@ extern "C" int32_t mfa_energies(mfa_ctx* c, int64_t step, mfa_obs* obs) {
   rc = add_coulomb_energy(c, t);
   if (rc) return rc;
   // kinetic energy of the current velocities
-  std::vector<double> vv((size_t)c->n * 3);
-  std::vector<int32_t> ty((size_t)c->n);
+  const int n0 = c->ks[0].n;  // replica 0
+  std::vector<double> vv((size_t)n0 * 3);
+  std::vector<int32_t> ty((size_t)n0);
   HIPCHK(c, hipMemcpy(vv.data(), c->v, vv.size() * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(ty.data(), c->type, ty.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   double tk = 0.0;
-  for (int i = 0; i < c->n; ++i) {
+  for (int i = 0; i < n0; ++i) {
     const double* q = &vv[3 * (size_t)i];
     tk += c->p.mass_type[ty[(size_t)i] - 1] * ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]);
   }
@@ -2383,14 +2590,15 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   if (n == 0) return MFA_OK;
   int rc = ensure_list(c);
   if (rc) return rc;
-  HIPCHK(c, hipMemsetAsync(c->flags + F_NREBUILD, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                     c->n_rep, (int)F_NREBUILD, 0);
   rc = set_step(c, first_step);
   if (rc) return rc;
   const int g = grid_of(c->n);
   // first half of the first step (verlet_positions + check_skin)
-  hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->r, c->v,
-                     c->a, c->mic, c->r_ref, c->flags, 1, c->cell_count, c->type, c->cell_of,
-                     c->pos4, c->vel4);
+  hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 1, c->n_cells, c->cell_count,
+                     c->type, c->cell_of, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   // steps 1 .. n-1: captured blocks of graph_steps steps, then single steps
   int middle = n - 1;
@@ -2413,23 +2621,74 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   if (rc) return rc;
   rc = launch_ewald(c, c->p.symmetry == 0);
   if (rc) return rc;
-  double t[MFA_NTERMS], tk[MFA_NTERMS];
+  const size_t nt = (size_t)c->n_rep * MFA_NTERMS;
+  std::vector<double> t(nt), tk(nt);
   if (obs) {
-    rc = reduce_terms(c, force_blocks(c), t);
-    if (rc) return rc;
-    rc = add_coulomb_energy(c, t);
+    hipLaunchKernelGGL(k_reduce_replicas, dim3(MFA_NTERMS, c->n_rep), dim3(MFA_BLOCK), 0, c->stream,
+                       c->part, c->d_kp, MFA_BLOCK / c->tpp, c->red);
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red, nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < nt; ++q) t[q] = c->h_red[q];
+    rc = add_coulomb_energy(c, t.data());  // replica 0 (charged systems run alone)
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->k, c->f, c->v,
-                     c->a, c->type, c->part, obs ? 1 : 0, c->vel4);
+  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->f, c->v, c->a, c->type, c->part, obs ? 1 : 0, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (obs) {
-    rc = reduce_terms(c, g, tk);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_replicas, dim3(MFA_NTERMS, c->n_rep), dim3(MFA_BLOCK), 0, c->stream,
+                       c->part, c->d_kp, MFA_BLOCK, c->red);
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red, nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < nt; ++q) tk[q] = c->h_red[q];
   }
   rc = check_flags(c);
   if (rc) return rc;
-  if (obs) fill_obs(t, tk[T_TKIN], c->h_flags[F_NREBUILD], obs);
+  if (obs) {
+    for (int q = 0; q < c->n_rep; ++q)
+      fill_obs(&t[(size_t)q * MFA_NTERMS], tk[(size_t)q * MFA_NTERMS + T_TKIN],
+               c->h_flags[(size_t)q * F_NFLAGS + F_NREBUILD], &c->rep_obs[(size_t)q]);
+    *obs = c->rep_obs[0];
+  }
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
+                                double* ms_per_step) {
+  if (!c || n_steps < 1) return MFA_ERR_ARG;
+  const int cand[5] = {1, 2, 4, 8, 16};
+  double best = 1e300;
+  int best_t = c->tpp;
+  int64_t step = first_step;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int rc = MFA_OK;
+  for (int q = 0; q < 5 && rc == MFA_OK; ++q) {
+    c->tpp = cand[q];
+    // one untimed call captures the step block for this choice
+    rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
+    step += c->graph_steps + 1;
+    if (rc) break;
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    rc = mfa_run_steps(c, step, n_steps, nullptr);
+    step += n_steps;
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_t = cand[q];
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->tpp = best_t;
+  drop_graph(c);
+  if (rc) return rc;
+  if (best_tpp) *best_tpp = best_t;
+  if (ms_per_step) *ms_per_step = best / (double)n_steps;
   return MFA_OK;
 }
 

@@ -118,6 +118,12 @@ SIGNATURES = {
     "mfa_ewald_k": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_dipolar_correction": (C.c_int32, [_P, _dp]),
     "mfa_bending": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_create_batch": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(_P)]),
+    "mfa_set_state_replica": (C.c_int32, [_P, C.c_int32, _dp, _dp, _dp, _ip, _ip]),
+    "mfa_get_state_replica": (C.c_int32, [_P, C.c_int32, _dp, _dp, _dp, _dp, _ip]),
+    "mfa_replica_obs": (C.c_int32, [_P, C.c_int32, C.POINTER(Obs)]),
+    "mfa_autotune": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _dp]),
     "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
@@ -200,6 +206,13 @@ def generate(params, kind, seed, head_z=1.0, bond=0.97, min_dist=0.8):
 
 class Context:
     """A device context: the state of one system in HBM."""
+
+    def autotune(self, first_step, n_steps):
+        t = C.c_int32()
+        ms = C.c_double()
+        self._chk(self.lib.mfa_autotune(self.h, first_step, n_steps, C.byref(t), C.byref(ms)),
+                  "mfa_autotune")
+        return t.value, ms.value
 
     def __init__(self, params, precision=MFA_FP64, list_capacity=0):
         self.lib = load()
@@ -357,3 +370,55 @@ class Context:
                                       C.byref(im), C.byref(inn)), "mfa_timing")
         return {"force_ms": fm.value, "force_n": fn.value, "integrate_ms": im.value,
                 "integrate_n": inn.value}
+
+
+class Batch(Context):
+    """Independent replicas (each its own box, chains and seed) advanced
+    together: one launch of each kernel per step covers every replica."""
+
+    def __init__(self, params_list, precision=MFA_FP64, list_capacity=0):
+        self.lib = load()
+        self.params_list = list(params_list)
+        self.params = self.params_list[0]
+        self.n = self.params.n_part
+        arr = (Params * len(self.params_list))(*self.params_list)
+        h = C.c_void_p()
+        rc = self.lib.mfa_create_batch(arr, len(self.params_list), precision, list_capacity,
+                                       C.byref(h))
+        if rc:
+            raise MfaError("mfa_create_batch: %s (%s)" % (
+                self.lib.mfa_last_error(None).decode(), ERRORS.get(rc, rc)))
+        self.h = h
+
+    def set_replica(self, q, r, v, a, a_type, mic=None):
+        n = self.params_list[q].n_part
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        t = np.ascontiguousarray(a_type, dtype=np.int32)
+        assert r.shape == (n, 3) and v.shape == (n, 3) and a.shape == (n, 3) and t.shape == (n,)
+        m = None if mic is None else np.ascontiguousarray(mic, dtype=np.int32)
+        self._chk(self.lib.mfa_set_state_replica(self.h, q, _ptr(r, C.c_double), _ptr(v, C.c_double),
+                                                 _ptr(a, C.c_double), _ptr(t, C.c_int32),
+                                                 _ptr(m, C.c_int32)), "mfa_set_state_replica")
+
+    def get_replica(self, q):
+        n = self.params_list[q].n_part
+        r, v, a, f = (np.zeros((n, 3)) for _ in range(4))
+        mic = np.zeros((n, 3), dtype=np.int32)
+        self._chk(self.lib.mfa_get_state_replica(self.h, q, _ptr(r, C.c_double), _ptr(v, C.c_double),
+                                                 _ptr(a, C.c_double), _ptr(f, C.c_double),
+                                                 _ptr(mic, C.c_int32)), "mfa_get_state_replica")
+        return {"r": r, "v": v, "a": a, "f": f, "mic": mic}
+
+    def replica_obs(self, q):
+        obs = Obs()
+        self._chk(self.lib.mfa_replica_obs(self.h, q, C.byref(obs)), "mfa_replica_obs")
+        return obs.as_dict()
+
+    def autotune(self, first_step, n_steps):
+        t = C.c_int32()
+        ms = C.c_double()
+        self._chk(self.lib.mfa_autotune(self.h, first_step, n_steps, C.byref(t), C.byref(ms)),
+                  "mfa_autotune")
+        return t.value, ms.value
