@@ -8,7 +8,12 @@ integrate.  `value` is the mixed FP32/FP64 form with the state resident in
 HBM; the plain FP64 form, the mixed-vs-FP64 force error and the oracle's
 single-thread CPU rate on a bounded sample are reported beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Other workloads (one JSON line each, same keys): brush9k (DPD brush, 9,000
+beads, dt 0.005), pe_brush27k (polyelectrolyte brush with Ewald, 27,000
+beads) and batch256 (256 uneven brush replicas, ~6M beads, one launch per
+kernel per step, against the same replicas run one after another).
 """
 import argparse
 import json
@@ -88,55 +93,13 @@ def cpu_baseline(inputs, st0, n_steps):
     return r.shape[0] * n_steps / dt, dt
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph-steps", type=int, default=16,
-                    help="steps per captured hipGraph block (0: direct launches)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(each GPU box has one GPU); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def cpu_rate(inputs, st0, n_steps):
+    rate, secs = cpu_baseline(inputs, st0, n_steps)
+    log("cpu baseline: %.3e particle-steps/s (%.1f s)" % (rate, secs))
+    return rate, secs
 
-    w = configs.WORKLOADS["melt100k"]
-    inputs = w["make"]()
-    t0 = time.perf_counter()
-    p, st0 = configs.equilibrate(inputs, w["kind"], min_dist=w["min_dist"])
-    t = st0["t"]
-    n = p.n_part
-    log("melt100k: n=%d box=%.4f cells=%s, prepared in %.1f s" % (
-        n, p.boundary[0], list(p.n_bin), time.perf_counter() - t0))
 
-    dt_m, obs_m, tm_m, st_m = run_form(p, st0, lib.MFA_MIXED, args.steps, args.warmup,
-                                       graph_steps=args.graph_steps)
-    log("mixed: %.4f s for %d steps, force pass %.4f ms avg, rebuilds %d, T=%.3f" % (
-        dt_m, args.steps, tm_m["force_ms"], obs_m["n_rebuilds"], 2.0 * obs_m["t_kin"] / (3.0 * n)))
-    dt_d, obs_d, tm_d, _ = run_form(p, st0, lib.MFA_FP64, args.steps, args.warmup,
-                                    graph_steps=args.graph_steps)
-    log("fp64 : %.4f s for %d steps, force %.4f ms avg" % (dt_d, args.steps, tm_d["force_ms"]))
-
-    st_m["t"] = t
-    err_max, err_rms = force_error_vs_fp64(p, st_m)
-    log("mixed vs fp64 force error: max %.3e rms %.3e" % (err_max, err_rms))
-
-    value = n * args.steps / dt_m
-    value_fp64 = n * args.steps / dt_d
-    force_s = tm_m["force_ms"] * 1e-3
-    achieved = FORCE_BYTES_PER_PARTICLE * n / force_s / 1e9 if force_s > 0 else None
-
-    cpu = None
-    if not args.no_cpu:
-        rate, secs = cpu_baseline(inputs, st0, args.cpu_steps)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": "oracle (plain-C FP64 restatement, 1 thread) on the same 100k-bead "
-                         "melt, %d steps after 2 warm-up steps (%.1f s)" % (args.cpu_steps, secs)}
-        log("cpu baseline: %.3e particle-steps/s (%.1f s)" % (rate, secs))
-
+def contract_line(args, workload, value, ms_per_step, dtype, data, config, extra):
     line = {
         "metric": "particle-steps/s on 1 MI355X, FP32/FP64 mixed vs plain FP64 at <=1e-5 force err",
         "value": value,
@@ -144,35 +107,240 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt_m / args.steps * 1e3,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair math, int64 fixed-point sums, fp64 state",
-        "data": "synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
-                "by the switch-on ramp before timing)",
-        "config": {"workload": "melt100k", "n_particles": n, "chains": 2000, "chain_length": 50,
-                   "density": 0.85, "lj_rc": 2.5, "dt": 0.01, "dpd_gamma": 1.0, "skin": 0.3,
-                   "parallelism": "single GPU"},
+        "dtype": dtype,
+        "data": data,
+        "config": dict({"workload": workload}, **config),
+    }
+    line.update(extra)
+    return line
+
+
+def roofline(n, force_ms, traffic=None, kernel="k_force_mixed<8,0,0>"):
+    force_s = force_ms * 1e-3
+    achieved = FORCE_BYTES_PER_PARTICLE * n / force_s / 1e9 if force_s > 0 else None
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": traffic, "force_ms_avg": force_ms}
+
+
+def bench_single(args, name):
+    """melt100k, brush9k, pe_brush27k: mixed and FP64 forms on one system."""
+    w = configs.WORKLOADS[name]
+    inputs = w["make"]()
+    t0 = time.perf_counter()
+    if w["kind"] == 2:
+        p, st0 = configs.equilibrate(inputs, w["kind"], min_dist=w["min_dist"])
+    else:
+        p, st0 = configs.prepare_brush(inputs, w["kind"], min_dist=w["min_dist"])
+    t = st0["t"]
+    n = p.n_part
+    log("%s: n=%d box=%s cells=%s, prepared in %.1f s" % (
+        name, n, [round(x, 4) for x in p.boundary], list(p.n_bin), time.perf_counter() - t0))
+    dt_m, obs_m, tm_m, st_m = run_form(p, st0, lib.MFA_MIXED, args.steps, args.warmup,
+                                       graph_steps=args.graph_steps)
+    log("mixed: %.4f s for %d steps, force pass %.4f ms avg, rebuilds %d, T=%.3f" % (
+        dt_m, args.steps, tm_m["force_ms"], obs_m["n_rebuilds"], 2.0 * obs_m["t_kin"] / (3.0 * n)))
+    dt_d, obs_d, tm_d, _ = run_form(p, st0, lib.MFA_FP64, args.steps, args.warmup,
+                                    graph_steps=args.graph_steps)
+    log("fp64 : %.4f s for %d steps, force %.4f ms avg" % (dt_d, args.steps, tm_d["force_ms"]))
+    st_m["t"] = t
+    err_max, err_rms = force_error_vs_fp64(p, st_m)
+    log("mixed vs fp64 force error: max %.3e rms %.3e" % (err_max, err_rms))
+    value = n * args.steps / dt_m
+    value_fp64 = n * args.steps / dt_d
+    cpu = None
+    if not args.no_cpu:
+        rate, secs = cpu_rate(inputs, st0, args.cpu_steps)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": "oracle (plain-C FP64 restatement, 1 thread) on the same %d-bead %s, "
+                         "%d steps after 2 warm-up steps (%.1f s)" % (n, name, args.cpu_steps, secs)}
+    if name == "melt100k":
+        config = {"n_particles": n, "chains": 2000, "chain_length": 50, "density": 0.85,
+                  "lj_rc": 2.5, "dt": 0.01, "dpd_gamma": 1.0, "skin": 0.3,
+                  "parallelism": "single GPU"}
+        data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
+                "by the switch-on ramp before timing)")
+        traffic = FORCE_TRAFFIC_BYTES
+    else:
+        config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
+                  "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
+                  "skin": inputs.skin, "parallelism": "single GPU"}
+        if name == "pe_brush27k":
+            config.update({"alpha_c": inputs.alpha_c, "n_kmax": inputs.n_kmax,
+                           "rc_coul": inputs.rc_coul})
+        data = ("synthetic (seeded grafted random walks and uniform solvent, Maxwell-Boltzmann "
+                "velocities, relaxed by the 40-tau switch-on ramp before timing)")
+        traffic = None
+    xt = 1 if (p.coulomb or p.bending) else 0
+    roof = roofline(n, tm_m["force_ms"], traffic, kernel="k_force_mixed<8,0,%d>" % xt)
+    roof["force_ms_avg_fp64"] = tm_d["force_ms"]
+    extra = {
         "fp64_value": value_fp64,
         "mixed_over_fp64": value / value_fp64,
         "force_err_max": err_max,
         "force_err_rms": err_rms,
         "rebuilds": obs_m["n_rebuilds"],
         "step_roofline_frac": value * STEP_BYTES_PER_PARTICLE / (HBM_PEAK_GBS * 1e9),
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_force_mixed<8,0>",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": FORCE_TRAFFIC_BYTES,
-            "force_ms_avg": tm_m["force_ms"],
-            "force_ms_avg_fp64": tm_d["force_ms"],
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    return contract_line(args, name, value, dt_m / args.steps * 1e3,
+                         "fp32 pair math, int64 fixed-point sums, fp64 state", data, config, extra)
+
+
+def bench_batch(args, n_rep=256):
+    """256 uneven brush replicas: one context, one launch of each kernel per
+    step for all of them (threads per particle from the autotuner), against
+    the same replicas advanced one context at a time."""
+    t0 = time.perf_counter()
+    reps = []
+    for q in range(n_rep):
+        x = configs.ensemble_replica(q)
+        p, r, v, t = configs.build(x, 1, min_dist=0.7)
+        reps.append((p, r, v, t))
+    ntot = sum(p.n_part for p, *_ in reps)
+    log("batch%d: %d beads (%d..%d per replica), generated in %.1f s" % (
+        n_rep, ntot, min(p.n_part for p, *_ in reps), max(p.n_part for p, *_ in reps),
+        time.perf_counter() - t0))
+    prec = lib.MFA_MIXED
+    with lib.Batch([s[0] for s in reps], prec) as b:
+        b.set_graph_steps(args.graph_steps)
+        for q, (p, r, v, t) in enumerate(reps):
+            b.set_replica(q, r, v, np.zeros_like(r), t)
+        tpp, ms_tune = b.autotune(1, 20)
+        log("autotune: %d threads per particle (%.3f ms/step)" % (tpp, ms_tune))
+        first = 1 + 5 * (args.graph_steps + 1 + 20)  # the steps the autotuner ran
+        b.run_steps(first, args.warmup, want_obs=False)
+        t1 = time.perf_counter()
+        b.run_steps(first + args.warmup, args.steps, want_obs=False)
+        dt_b = time.perf_counter() - t1
+        force_ms = b.time_force(50)
+    log("batched: %.3f s for %d steps" % (dt_b, args.steps))
+    dt_s = 0.0
+    for q, (p, r, v, t) in enumerate([] if args.no_seq else reps):
+        with lib.Context(p, prec) as c:
+            c.set_graph_steps(args.graph_steps)
+            c.set_tpp(tpp)
+            c.set_state(r, v, np.zeros_like(r), t)
+            c.run_steps(1, args.warmup, want_obs=False)
+            t1 = time.perf_counter()
+            c.run_steps(1 + args.warmup, args.steps, want_obs=False)
+            dt_s += time.perf_counter() - t1
+    log("sequential: %.3f s for %d steps of each replica" % (dt_s, args.steps))
+    value = ntot * args.steps / dt_b
+    dt_s = dt_s if dt_s > 0 else float("nan")
+    config = {"n_replicas": n_rep, "n_particles": ntot, "grafting_density": [0.05, 0.5],
+              "chain_length": [20, 100], "lateral_edge": [8, 40], "lz": 12.5, "dt": 0.005,
+              "solvent_density": 3.0, "threads_per_particle": tpp,
+              "parallelism": "single GPU, replicas batched"}
+    extra = {
+        "sequential_value": ntot * args.steps / dt_s,
+        "batched_over_sequential": dt_s / dt_b,
+        "roofline": roofline(ntot, force_ms, None, kernel="k_force_mixed<%d,0,0>" % tpp),
+        "cpu_baseline": None,
+    }
+    data = ("synthetic (seeded replicas, fresh starts timed inside the 40-tau switch-on ramp "
+            "of the reference's start-up; the clamp changes no work per step)")
+    return contract_line(args, "batch256", value, dt_b / args.steps * 1e3,
+                         "fp32 pair math, int64 fixed-point sums, fp64 state", data, config, extra)
+
+
+def bench_adjoint(args):
+    """Gradient of the window-averaged brush height and top-wall pressure with
+    respect to (epsilon_LJ, sigma, k_FENE, R0, gamma_DPD) over a window of
+    `steps` steps with checkpoints every 20 steps: the forward pass in the
+    mixed form (FP32 forces) and in the FP64 form, the adjoint in FP64; the
+    FP64 gradient against central finite differences (relative step 1e-4)
+    of the FP64 forward pass."""
+    w = configs.WORKLOADS["adjoint50k"]
+    inputs = w["make"]()
+    t0 = time.perf_counter()
+    p, st = configs.prepare_brush(inputs, w["kind"], min_dist=w["min_dist"])
+    n = p.n_part
+    log("adjoint50k: n=%d box=%s cells=%s, prepared in %.1f s" % (
+        n, list(p.boundary), list(p.n_bin), time.perf_counter() - t0))
+    theta = configs.theta_of(inputs)
+    first = 2 + args.warmup
+
+    def window(th, precision, want_grad):
+        with lib.Context(lib.derive(configs.with_theta(inputs, th)), precision) as c:
+            c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
+            c.run_steps(2, args.warmup, want_obs=False)
+            t1 = time.perf_counter()
+            J, G, nreb = c.adjoint_window(first, args.steps, 20, want_grad)
+            return J, G, time.perf_counter() - t1
+
+    J64, G64, t64 = window(theta, lib.MFA_FP64, True)
+    Jm, Gm, tm = window(theta, lib.MFA_MIXED, True)
+    _, _, tfwd = window(theta, lib.MFA_MIXED, False)
+    log("adjoint window: mixed forward %.3f s (forward only %.3f s), fp64 forward %.3f s" % (
+        tm, tfwd, t64))
+    h = 1e-4
+    Gfd = np.zeros((2, 5))
+    for q in range(5):
+        tp, tq = theta.copy(), theta.copy()
+        tp[q] *= 1.0 + h
+        tq[q] *= 1.0 - h
+        Gfd[:, q] = (window(tp, lib.MFA_FP64, False)[0] - window(tq, lib.MFA_FP64, False)[0]) / (
+            tp[q] - tq[q])
+    rel = np.abs(G64 - Gfd) / np.maximum(np.abs(Gfd), 1e-3 * np.abs(Gfd).max(axis=1, keepdims=True))
+    relm = np.abs(Gm - G64) / np.maximum(np.abs(G64), 1e-3 * np.abs(G64).max(axis=1, keepdims=True))
+    log("J %s\nadjoint (fp64)\n%s\nFD (fp64)\n%s\nmax rel err %.3e; mixed-forward vs fp64 %.3e" % (
+        J64, G64, Gfd, rel.max(), relm.max()))
+    value = n * args.steps / tm
+    config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
+              "box": list(p.boundary), "dt": inputs.dt, "window": args.steps,
+              "checkpoint_every": 20, "parameters": list(configs.THETA_NAMES),
+              "parallelism": "single GPU"}
+    extra = {
+        "observables": {"height": J64[0], "p_wall": J64[1]},
+        "grad_fp64": G64.tolist(), "grad_fd_fp64": Gfd.tolist(), "grad_mixed_forward": Gm.tolist(),
+        "grad_vs_fd_max_rel": float(rel.max()),
+        "grad_mixed_vs_fp64_max_rel": float(relm.max()),
+        "window_s_mixed": tm, "window_s_fp64": t64, "forward_only_s_mixed": tfwd,
+        "roofline": None, "cpu_baseline": None,
+    }
+    data = ("synthetic (seeded grafted random walks and uniform solvent, relaxed by the 40-tau "
+            "switch-on ramp)")
+    return contract_line(args, "adjoint50k", value, tm / args.steps * 1e3,
+                         "fp32 forward forces, fp64 adjoint", data, config, extra)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="melt100k",
+                    choices=["melt100k", "brush9k", "pe_brush27k", "batch256", "adjoint50k"])
+    ap.add_argument("--cpu-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-seq", action="store_true",
+                    help="batch256: skip the sequential comparison (profiling runs)")
+    ap.add_argument("--graph-steps", type=int, default=16,
+                    help="steps per captured hipGraph block (0: direct launches)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(each GPU box has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    defaults = {"melt100k": (5000, 500, 20), "brush9k": (1000, 200, 200),
+                "pe_brush27k": (2000, 200, 3), "batch256": (500, 100, 0),
+                "adjoint50k": (200, 20, 0)}
+    d_steps, d_warm, d_cpu = defaults[args.workload]
+    args.steps = d_steps if args.steps is None else args.steps
+    args.warmup = d_warm if args.warmup is None else args.warmup
+    args.cpu_steps = d_cpu if args.cpu_steps is None else args.cpu_steps
+    if args.workload == "batch256":
+        line = bench_batch(args)
+    elif args.workload == "adjoint50k":
+        line = bench_adjoint(args)
+    else:
+        line = bench_single(args, args.workload)
     print(json.dumps(line), flush=True)
 
 

@@ -278,6 +278,38 @@ int32_t mfa_replica_obs(mfa_ctx* ctx, int32_t replica, mfa_obs* obs);
 int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
                      double* ms_per_step);
 
+/*
+ * Adjoint of the force-and-integrate step (no counterpart in the reference,
+ * which has no derivatives).  Over a window of n_steps steps from the
+ * context's state (as mfa_run_steps(first_step, n_steps) would advance it)
+ * the call computes two time-averaged observables of a brush in a channel,
+ *   height  = (1/W) sum_n  mean z of the type-2 beads at r_n,
+ *   p_wall  = (1/W) sum_n  -(sum_i F_z^top(z_i)) / (Lx Ly), the normal
+ *             pressure the particles exert on the z = Lz wall,
+ * and, with want_grad, their derivatives with respect to the run-time
+ * parameters theta = (epsilon_LJ, sigma, k_FENE, R0, gamma_DPD) of
+ * the mfa_input file (init_params.f90:395-845): epsilon is the common value of every
+ * non-zero epsil(i,j), sigma the common sigma of every type (it moves the
+ * LJ ranges, the mixed sigma(i,j) and the wall sigma_wall(t)), k_FENE and R0
+ * the FENE constants, gamma_DPD = friction(1).  The DPD noise is the
+ * counter-based deviate of (seed, step, pair), fixed under theta.
+ * Reverse mode: the state is checkpointed every checkpoint_every steps, each
+ * segment is recomputed forward and the adjoint of the step (pair, bond and
+ * wall Jacobians applied to the adjoint vectors of both observables at once,
+ * FP64 sums) runs backward through it.  The forward pass computes in the
+ * context's precision form; the adjoint is FP64.  On return the context holds
+ * the state at the end of the window.  Systems with charges, bending or an
+ * active switch-on ramp are refused; a batch must hold one replica.
+ */
+typedef struct mfa_adjoint_out {
+  double height, p_wall;
+  double d_height[5];   /* d height / d(epsilon_LJ, sigma, k_FENE, R0, gamma_DPD) */
+  double d_p_wall[5];
+  int64_t n_rebuilds;   /* list rebuilds of the forward pass */
+} mfa_adjoint_out;
+int32_t mfa_adjoint_window(mfa_ctx* ctx, int64_t first_step, int32_t n_steps,
+                           int32_t checkpoint_every, int32_t want_grad, mfa_adjoint_out* out);
+
 /* Seeded input generators (the reference's conf_default/gen_brush/gen_droplet
  * are replaced by these; conf_default.f90:144-217, gen_brush.f90:101-161).
  * kind 1: brush + ideal solvent in a channel; kind 2: melt in a periodic box;

@@ -171,13 +171,40 @@ def ensemble_replica(q, base_seed=4004, lz=12.5, solvent_density=3.0):
 
 
 WORKLOADS = {
-    # plain FP64 DPD polymer brush (CPU-runnable reference case)
+    # plain FP64 DPD polymer brush (the CPU-runnable case)
     "brush9k": dict(kind=1, steps=1000, min_dist=0.7,
                     make=lambda: brush(100, 30, 6000, 10.0, 10.0, 20.0, seed=1001)),
-    # configs[1]: Kremer-Grest melt, the headline workload
+    # Kremer-Grest melt, the headline workload
     "melt100k": dict(kind=2, steps=5000, min_dist=0.8,
                      make=lambda: melt(2000, 50, 0.85, seed=2002)),
+    # polyelectrolyte brush with Ewald: 400 chains x 40 (charge -1 on every
+    # third bead), 5200 counterions, 5800 neutral solvent = 27,000 beads in
+    # 63 x 63 x 40; alpha_c = 0.81, kmax = 10 (3997 k vectors), rc_coul = 3
+    # adjoint window: DPD brush of 500 chains x 40 at grafting density 0.2
+    # plus 30,000 solvent = 50,000 beads in 50 x 50 x 30
+    "adjoint50k": dict(kind=1, steps=200, min_dist=0.7,
+                       make=lambda: brush(500, 40, 30000, 50.0, 50.0, 30.0, seed=5005)),
+    "pe_brush27k": dict(kind=3, steps=2000, min_dist=0.7,
+                        make=lambda: pe_brush(400, 40, 5800, 63.0, 63.0, 40.0, n_kmax=10,
+                                              alpha_c=0.81, rc_coul=3.0, seed=3003)),
 }
+
+
+def prepare_brush(inputs, kind, min_dist=0.7, precision=lib.MFA_MIXED):
+    """Seeded brush start relaxed by the reference's start-up procedure: the
+    FORCE_SWITCH_ON ramp of a fresh start (s_time = 1) at the production time
+    step, over the ramp's 40 tau (init_force_switch_on.f90:15-26,
+    fluid_fluid.f90:42-49).  Returns (production params, state dict)."""
+    p, r, v, t = build(inputs, kind, min_dist=min_dist)
+    w = lib.Inputs.from_buffer_copy(inputs)
+    w.s_time = 1
+    w.force_switch_on = 1
+    wp = lib.derive(w)
+    with lib.Context(wp, precision) as c:
+        c.set_state(r, v, np.zeros_like(r), t)
+        c.run_steps(1, ramp_steps(wp), want_obs=False)
+        st = c.get_state()
+    return p, {"r": st["r"], "v": st["v"], "a": st["a"], "mic": st["mic"], "t": t}
 
 
 def small_cases():
@@ -197,6 +224,35 @@ def small_cases():
         "pe_brush_small": (3, pe_brush(8, 12, 60, 12.0, 12.0, 14.0, x_space=0.5, n_kmax=4,
                                        rc_coul=3.0, seed=16)),
     }
+
+
+THETA_NAMES = ("epsilon_LJ", "sigma", "k_FENE", "R0", "gamma_DPD")
+
+
+def theta_of(inputs):
+    """The parameters the adjoint differentiates: the common non-zero
+    epsilon, the common sigma, FENE k and R0, and friction(1)."""
+    eps = sorted({inputs.epsil[t][u] for t in range(NT) for u in range(NT)} - {0.0})
+    sig = sorted({inputs.sigma_ii[t] for t in range(NT)})
+    if len(eps) != 1 or len(sig) != 1:
+        raise ValueError("epsilon and sigma must each take one value")
+    return np.array([eps[0], sig[0], inputs.k_chain, inputs.r_chain, inputs.friction[0]])
+
+
+def with_theta(inputs, theta):
+    """A copy of `inputs` with the parameters set to theta (epsilon replaces
+    every non-zero epsil entry, sigma every sigma_ii)."""
+    x = lib.Inputs.from_buffer_copy(inputs)
+    eps, sig, k, r0, gam = (float(v) for v in theta)
+    for t in range(NT):
+        x.sigma_ii[t] = sig
+        for u in range(NT):
+            if inputs.epsil[t][u] != 0.0:
+                x.epsil[t][u] = eps
+    x.k_chain = k
+    x.r_chain = r0
+    x.friction[0] = gam
+    return x
 
 
 def warmup_inputs(inputs, dt=0.002):

@@ -114,6 +114,7 @@ struct mfa_ctx {
   int* d_cell_rep;   // replica of each cell
   std::vector<mfa_obs> rep_obs;  // per-replica observations of the last mfa_run_steps
   int precision;
+  bool extra_terms;  // some replica has charges or bending: force kernels with XT = 1
   int n;
   int cap;
   int tpp;  // threads per particle in the force pass
@@ -901,7 +902,7 @@ __device__ double bend_angle(const KParams& k, const double* __restrict__ r, int
   return .5 * f_mod * delta_alpha;
 }
 
-template <int TPP>
+template <int TPP, int XT>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict__ kb,
                                                        const int* __restrict__ rmap,
                                                        const double* __restrict__ r,
@@ -963,7 +964,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
         }
       }
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
-      const bool coul = k.coulomb && (k.rc_coul_2 > 0.0 ? r2raw < k.rc_coul_2 : in_lj);
+      const bool coul = XT && k.coulomb && (k.rc_coul_2 > 0.0 ? r2raw < k.rc_coul_2 : in_lj);
       if (!in_lj && !dpd && !coul) continue;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       const double inv_r_2 = 1.0 / r2;
@@ -1045,7 +1046,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
         }
       }
     }
-    if ((mask & TM_BEND) && k.bending && il < k.n_brush) {
+    if (XT && (mask & TM_BEND) && k.bending && il < k.n_brush) {
       const int base = (il / k.n_mon) * k.n_mon, m = il - base;
       for (int cen = m - 1; cen <= m + 1; ++cen) {
         if (cen < 1 || cen > k.n_mon - 2) continue;
@@ -1105,7 +1106,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
 }
 
-template <int TPP, int ENERGY>
+template <int TPP, int ENERGY, int XT>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
     const float4* __restrict__ pos4, const float4* __restrict__ vel4,
@@ -1172,10 +1173,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       const float4 tb = s_tab[ti * MFA_NTYPE + tj];
       bool in_lj = r2raw < tb.x;
       bool in_dpd = k.dpd_cut_off_on ? r2raw < rdpd2 : in_lj;
-      bool in_coul = k.rc_coul_2 > 0.0 && r2raw < rc2f;
+      bool in_coul = XT && k.rc_coul_2 > 0.0 && r2raw < rc2f;
       bool near = fabsf(r2raw - tb.x) < margin;
       if (k.dpd_cut_off_on) near = near || fabsf(r2raw - rdpd2) < margin;
-      if (k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
+      if (XT && k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
       if (near) {
         // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
         double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
@@ -1185,12 +1186,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const double r2d = (ex * ex + ey * ey) + ez * ez;
         in_lj = r2d < k.range_2[ti][tj];
         in_dpd = k.dpd_cut_off_on ? r2d < k.r_cut_dpd_2 : in_lj;
-        in_coul = k.rc_coul_2 > 0.0 && r2d < k.rc_coul_2;
+        in_coul = XT && k.rc_coul_2 > 0.0 && r2d < k.rc_coul_2;
       }
       const bool dpd = k.dpd_on && in_dpd && !(k.dpd_head_excl && k.dpd_cut_off_on &&
                                                (ti == 0 || tj == 0));
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
-      const bool coul = k.coulomb && (k.rc_coul_2 > 0.0 ? in_coul : in_lj);
+      const bool coul = XT && k.coulomb && (k.rc_coul_2 > 0.0 ? in_coul : in_lj);
       if (!in_lj && !dpd && !coul) continue;
       const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
       const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
@@ -1228,7 +1229,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 
   // Bonds and walls: lane 0 of the group, from the FP64 positions.
   if (active && sub == 0 && (ENERGY || (mask & (TM_BOND | TM_WALL)))) zi = r[3 * (size_t)i + 2];
-  if (active && sub == 0 && (mask & TM_BEND) && k.bending && il < k.n_brush) {
+  if (XT && active && sub == 0 && (mask & TM_BEND) && k.bending && il < k.n_brush) {
     // each term through float and fixed point; the centre takes minus the
     // fixed-point sum of its neighbours' terms, so every angle sums to zero
     const int base = (il / k.n_mon) * k.n_mon, m = il - base;
@@ -1638,23 +1639,31 @@ static int launch_rebuild(mfa_ctx* c) {
   return MFA_OK;
 }
 
-template <int TPP>
-static void launch_force_t(mfa_ctx* c, int mask) {
+// XT: the Ewald real-space and bending terms are compiled in (charged or
+// stiff systems only; without them the pair loop needs far fewer registers).
+template <int TPP, int XT>
+static void launch_force_tx(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED) {
     if (mask & TM_ENERGY)
-      hipLaunchKernelGGL((k_force_mixed<TPP, 1>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
-                         c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step,
-                         mask, c->part, c->flags);
+      hipLaunchKernelGGL((k_force_mixed<TPP, 1, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f,
+                         c->d_step, mask, c->part, c->flags);
     else
-      hipLaunchKernelGGL((k_force_mixed<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
-                         c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f, c->d_step,
-                         mask, c->part, c->flags);
+      hipLaunchKernelGGL((k_force_mixed<TPP, 0, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f,
+                         c->d_step, mask, c->part, c->flags);
   } else {
-    hipLaunchKernelGGL((k_force64<TPP>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                       c->r, c->v, c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask, c->part,
-                       c->flags);
+    hipLaunchKernelGGL((k_force64<TPP, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                       c->d_rmap, c->r, c->v, c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask,
+                       c->part, c->flags);
   }
+}
+
+template <int TPP>
+static void launch_force_t(mfa_ctx* c, int mask) {
+  if (c->extra_terms) launch_force_tx<TPP, 1>(c, mask);
+  else launch_force_tx<TPP, 0>(c, mask);
 }
 
 static int force_blocks(mfa_ctx* c) { return grid_of(c->n * c->tpp); }
@@ -2039,6 +2048,8 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   mfa_ctx* c = new mfa_ctx();
   c->p = ps[0];
   c->n_rep = n_rep;
+  c->extra_terms = false;
+  for (int q = 0; q < n_rep; ++q) c->extra_terms |= ps[q].coulomb != 0 || ps[q].bending != 0;
   c->ps.assign(ps, ps + n_rep);
   c->precision = precision;
   c->tpp = 8;
@@ -2797,5 +2808,573 @@ extern "C" int32_t mfa_timing(mfa_ctx* c, int32_t enable, double* force_ms_avg,
   c->force_n = 0;
   c->integ_ms = 0.0;
   c->integ_n = 0;
+  return MFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the force-and-integrate step (no counterpart in the reference).
+//
+// Step n of the window maps (r, v, a)_{n-1} to (r, v, a)_n:
+//   r_n = r_{n-1} + dt v_{n-1} + dt^2/2 a_{n-1}     u_n = v_{n-1} + dt/2 a_{n-1}
+//   a_n = F(r_n, u_n; theta, xi_n) / m              v_n = u_n + dt/2 a_n
+// (pinned heads: v, a reset, so nothing flows through theirs).  Going
+// backward, with adjoint vectors (lr, lv, la) of (r, v, a)_n:
+//   lr += dJ_n/dr_n;  lu = lv;  lF = (la + dt/2 lv) / m;
+//   lr += (dF/dr)^T lF;  lu += (dF/du)^T lF;  g_theta += (dF/dtheta)^T lF;
+//   lv <- lu + dt lr;  la <- dt/2 lu + dt^2/2 lr.
+// A central pair term F_ij = s(r^2, w) d (d = r_i - r_j, w = (u_i - u_j).d)
+// contributes with L = (lF_i - lF_j).F_ij:
+//   dL/dr_i = s Lam + (Lam.d) (2 s_r2 d + s_w du),   dL/du_i = (Lam.d) s_w d,
+// and the negatives to j; the full list holds each pair in both directions,
+// so each particle gathers its own terms and the parameter sums take half of
+// every entry.  Both observables (brush height, top-wall pressure) are carried
+// at once: every adjoint array is double[N][2][3].
+
+#define ADJ_NO 2  // observables
+#define ADJ_NP 5  // epsilon_LJ, sigma, k_FENE, R0, gamma_DPD
+enum { AP_EPS = 0, AP_SIG, AP_K, AP_R0, AP_GAM };
+
+struct AdjK {
+  double inv_w;     // 1 / window length
+  double inv_n2;    // 1 / number of type-2 beads
+  double inv_area;  // 1 / (Lx Ly)
+  double eps_in;    // common epsilon (epsil / 4)
+  double sigma;     // common sigma
+  double k_in, r0;  // FENE k and R0
+  double gamma;     // friction(1)
+};
+
+// Top-wall force on a bead at height z (fluid_wall.f90:107-121) and its
+// derivatives with respect to z and to sigma_wall.
+__device__ __forceinline__ void wall_top_terms(double sw, double aw, double z, double zw,
+                                               double& f, double& df_dz, double& df_dsw) {
+  const double aabs = fabs(aw);
+  const double iz = 1.0 / (zw - z);
+  const double iz2 = iz * iz, iz4 = iz2 * iz2, iz5 = iz4 * iz, iz8 = iz4 * iz4;
+  const double iz10 = iz8 * iz2, iz11 = iz10 * iz;
+  const double sw2 = sw * sw, sw3 = sw2 * sw, sw8 = sw3 * sw3 * sw2, sw9 = sw8 * sw;
+  f = -(9.0 * aabs) * sw9 * iz10 + (3.0 * aw) * sw3 * iz4;
+  df_dz = -90.0 * aabs * sw9 * iz11 + 12.0 * aw * sw3 * iz5;
+  df_dsw = -(81.0 * aabs) * sw8 * iz10 + (9.0 * aw) * sw2 * iz4;
+}
+
+// Observables of the forward pass at r_n: partial sums of z over type-2
+// beads (slot 0) and of the top-wall force (slot 1).
+__global__ __launch_bounds__(MFA_BLOCK) void k_adj_obs(const KParams* __restrict__ kb,
+                                                       const int* __restrict__ rmap,
+                                                       const double* __restrict__ r,
+                                                       const int* __restrict__ type,
+                                                       double* __restrict__ part) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+  if (i - k.p0 < k.n) {
+    const int ti = type[i] - 1;
+    const double z = r[3 * (size_t)i + 2];
+    if (ti == 1) vals[0] = z;
+    double f, dz, ds;
+    wall_top_terms(k.sigma_wall[ti], k.a_wall[ti], z, k.z_space_wall, f, dz, ds);
+    vals[1] = f;
+  }
+  block_partials(vals, part);
+}
+
+// Seeds of step n and the force adjoint: lr += dJ_n/dr_n, lu = lv,
+// lF = (la + dt/2 lv)/m (0 for pinned heads).  Slot AP_SIG of observable 1
+// (MFA_NTERMS layout: o * ADJ_NP + p) gets the direct sigma term of the
+// pressure through sigma_wall = (sigma_w + sigma)/2.
+__global__ __launch_bounds__(MFA_BLOCK) void k_adj_pre(const KParams* __restrict__ kb,
+                                                       const int* __restrict__ rmap, AdjK ak,
+                                                       const double* __restrict__ r,
+                                                       const int* __restrict__ type,
+                                                       double* __restrict__ lr,
+                                                       const double* __restrict__ lv,
+                                                       const double* __restrict__ la,
+                                                       double* __restrict__ lf,
+                                                       double* __restrict__ part) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+  if (i - k.p0 < k.n) {
+    const int ti = type[i] - 1;
+    const double z = r[3 * (size_t)i + 2];
+    double f, dz, ds;
+    wall_top_terms(k.sigma_wall[ti], k.a_wall[ti], z, k.z_space_wall, f, dz, ds);
+    double* lri = lr + 6 * (size_t)i;
+    if (ti == 1) lri[2] += ak.inv_w * ak.inv_n2;
+    lri[5] += -ak.inv_w * ak.inv_area * dz;
+    vals[ADJ_NP + AP_SIG] = -ak.inv_w * ak.inv_area * ds * 0.5;
+    const bool head = is_head(k, i - k.p0);
+    const double im = k.inv_mass_type[ti];
+    for (int c = 0; c < 6; ++c) {
+      const double lac = la[6 * (size_t)i + c] + k.half_dt * lv[6 * (size_t)i + c];
+      lf[6 * (size_t)i + c] = head ? 0.0 : lac * im;
+    }
+  }
+  block_partials(vals, part);
+}
+
+// lv <- lu + dt lr, la <- dt/2 lu + dt^2/2 lr (the adjoint of
+// verlet_positions); nothing flows into a pinned head's v or a.
+__global__ __launch_bounds__(MFA_BLOCK) void k_adj_post(const KParams* __restrict__ kb,
+                                                        const int* __restrict__ rmap,
+                                                        const double* __restrict__ lr,
+                                                        double* __restrict__ lv,
+                                                        double* __restrict__ la) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i - k.p0 >= k.n) return;
+  const bool head = is_head(k, i - k.p0);
+  for (int c = 0; c < 6; ++c) {
+    const double lu = lv[6 * (size_t)i + c], lrc = lr[6 * (size_t)i + c];
+    lv[6 * (size_t)i + c] = head ? 0.0 : lu + k.dt * lrc;
+    la[6 * (size_t)i + c] = head ? 0.0 : k.half_dt * lu + k.half_dt_2 * lrc;
+  }
+}
+
+// The force adjoint: lr_i += (dF/dr_i)^T lF, lu_i += (dF/du_i)^T lF over the
+// pair, bond and wall terms of the force pass, FP64; parameter partials in
+// slots o * ADJ_NP + p.  TPP lanes share a particle's list as in k_force64.
+template <int TPP>
+__global__ __launch_bounds__(MFA_BLOCK) void k_adj_force(
+    const KParams* __restrict__ kb, const int* __restrict__ rmap, AdjK ak,
+    const double* __restrict__ r, const double* __restrict__ u, const int* __restrict__ type,
+    const unsigned* __restrict__ nl, const int* __restrict__ nn, int cap,
+    const double* __restrict__ lf, double* __restrict__ lr, double* __restrict__ lu,
+    const long long* __restrict__ d_step, double* __restrict__ part) {
+  MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gtid / TPP;
+  const int il = i - k.p0;
+  const int sub = gtid % TPP;
+  const int64_t step = *d_step;
+  const bool active = il < k.n;
+  const int ii = active ? i : k.p0;
+  const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
+  const double uxi = u[3 * (size_t)ii + 0], uyi = u[3 * (size_t)ii + 1], uzi = u[3 * (size_t)ii + 2];
+  const int ti = type[ii] - 1;
+  double lfi[6];
+  for (int c = 0; c < 6; ++c) lfi[c] = lf[6 * (size_t)ii + c];
+  double gr[6], gu[6], gp[ADJ_NO * ADJ_NP];
+  for (int c = 0; c < 6; ++c) gr[c] = gu[c] = 0.0;
+  for (int c = 0; c < ADJ_NO * ADJ_NP; ++c) gp[c] = 0.0;
+
+  if (active) {
+    const int cnt = nn[ii];
+    const unsigned* row = nl + (size_t)ii * cap;
+    for (int q = sub; q < cnt; q += TPP) {
+      const int j = (int)(row[q] & MFA_JMASK);
+      double d[3] = {xi - r[3 * (size_t)j + 0], yi - r[3 * (size_t)j + 1], zi - r[3 * (size_t)j + 2]};
+      min_image(k, d[0], d[1], d[2]);
+      const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+      const int tj = type[j] - 1;
+      const bool in_lj = r2 < k.range_2[ti][tj];
+      bool dpd = false;
+      double rc = 0.0;
+      if (k.dpd_on) {
+        if (!k.dpd_cut_off_on) {
+          dpd = in_lj;
+          rc = sqrt(k.range_2[ti][tj]);
+        } else {
+          const bool excl = k.dpd_head_excl && (ti == 0 || tj == 0);
+          dpd = !excl && r2 < k.r_cut_dpd_2;
+          rc = sqrt(k.r_cut_dpd_2);
+        }
+      }
+      if (!in_lj && !dpd) continue;
+      double s = 0.0, s_r2 = 0.0, s_w = 0.0, s_eps = 0.0, s_sig = 0.0, s_gam = 0.0;
+      const double du[3] = {uxi - u[3 * (size_t)j + 0], uyi - u[3 * (size_t)j + 1],
+                            uzi - u[3 * (size_t)j + 2]};
+      if (in_lj) {
+        const double eps = k.epsil[ti][tj];
+        const double inv_r2 = 1.0 / r2;
+        const double r61 = k.sigma_2[ti][tj] * inv_r2;
+        const double r6 = (r61 * r61) * r61, r12 = r6 * r6;
+        s += eps * (12.0 * r12 - 6.0 * r6) * inv_r2;
+        s_r2 += eps * (-84.0 * r12 + 24.0 * r6) * inv_r2 * inv_r2;
+        if (eps != 0.0) s_eps += 4.0 * (12.0 * r12 - 6.0 * r6) * inv_r2;
+        s_sig += eps * (144.0 * r12 - 36.0 * r6) * inv_r2 / ak.sigma;
+      }
+      if (dpd) {
+        const double rr = sqrt(r2);
+        const double wr = 1.0 - rr / rc;
+        const double xi_n = pair_normal_f64(k.seed, step, il, j - k.p0);
+        const double w = (du[0] * d[0] + du[1] * d[1]) + du[2] * d[2];
+        const double A = k.sig * wr * xi_n / rr;          // random term / r
+        const double B = -k.friction1 * wr * wr * w / r2;  // dissipative term / r
+        s += A + B;
+        s_r2 += -k.sig * xi_n / (2.0 * rr * r2) + k.friction1 * w * wr / (r2 * r2);
+        s_w += -k.friction1 * wr * wr / r2;
+        s_gam += A / (2.0 * k.friction1) + B / k.friction1;
+        if (!k.dpd_cut_off_on) {  // the DPD range is the LJ range: it moves with sigma
+          const double dwr = (rr / rc) / ak.sigma;
+          s_sig += k.sig * xi_n * dwr / rr - 2.0 * k.friction1 * wr * dwr * w / r2;
+        }
+      }
+      for (int o = 0; o < ADJ_NO; ++o) {
+        double lam[3];
+        for (int c = 0; c < 3; ++c) lam[c] = lfi[3 * o + c] - lf[6 * (size_t)j + 3 * o + c];
+        const double ld = (lam[0] * d[0] + lam[1] * d[1]) + lam[2] * d[2];
+        for (int c = 0; c < 3; ++c) {
+          gr[3 * o + c] += s * lam[c] + ld * (2.0 * s_r2 * d[c] + s_w * du[c]);
+          gu[3 * o + c] += ld * s_w * d[c];
+        }
+        gp[o * ADJ_NP + AP_EPS] += 0.5 * ld * s_eps;
+        gp[o * ADJ_NP + AP_SIG] += 0.5 * ld * s_sig;
+        gp[o * ADJ_NP + AP_GAM] += 0.5 * ld * s_gam;
+      }
+    }
+  }
+
+  if (active && sub == 0) {
+    // FENE bonds (intra_molec.f90:16-139): F_hi = rd d, F_lo = -rd d with
+    // d = r_lo - r_hi, rd = k / (1 - x), x = r^2/R0^2 clamped at 0.98.
+    int base, len;
+    chain_of(k, il, base, len);
+    for (int side = 0; side < 2; ++side) {
+      const int lol = side == 0 ? il - 1 : il;
+      if (len < 2 || lol < base || lol + 1 >= base + len) continue;
+      const int lo = k.p0 + lol, hi = lo + 1;
+      double d[3] = {r[3 * (size_t)lo + 0] - r[3 * (size_t)hi + 0],
+                     r[3 * (size_t)lo + 1] - r[3 * (size_t)hi + 1],
+                     r[3 * (size_t)lo + 2] - r[3 * (size_t)hi + 2]};
+      min_image(k, d[0], d[1], d[2]);
+      const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+      double x = r2 * k.inv_r_chain_2;
+      const double clamp = (double)0.98f;
+      const bool free = x < clamp;
+      x = free ? x : clamp;
+      const double rd = (k.k_chain / (1.0 - x)) * k.inv_r_chain_2;
+      const double rd_r2 = free ? rd / (1.0 - x) * k.inv_r_chain_2 : 0.0;
+      const double sg = side == 0 ? -1.0 : 1.0;  // i is hi (side 0) or lo (side 1)
+      for (int o = 0; o < ADJ_NO; ++o) {
+        double lam[3];
+        for (int c = 0; c < 3; ++c)
+          lam[c] = lf[6 * (size_t)hi + 3 * o + c] - lf[6 * (size_t)lo + 3 * o + c];
+        const double ld = (lam[0] * d[0] + lam[1] * d[1]) + lam[2] * d[2];
+        for (int c = 0; c < 3; ++c) gr[3 * o + c] += sg * (rd * lam[c] + 2.0 * ld * rd_r2 * d[c]);
+        if (side == 1) {  // each bond's parameter terms once, from its lo end
+          gp[o * ADJ_NP + AP_K] += ld * rd / ak.k_in;
+          if (free) gp[o * ADJ_NP + AP_R0] += ld * (-2.0 * x * rd / ((1.0 - x) * ak.r0));
+        }
+      }
+    }
+    if (k.symmetry == 0) {
+      // 9-3 walls (fluid_wall.f90:81-121): F_z(z) of both walls
+      const double sw = k.sigma_wall[ti], aw = k.a_wall[ti], aabs = fabs(aw);
+      const double iz = 1.0 / zi, iz2 = iz * iz, iz4 = iz2 * iz2, iz5 = iz4 * iz;
+      const double iz8 = iz4 * iz4, iz10 = iz8 * iz2, iz11 = iz10 * iz;
+      const double sw2 = sw * sw, sw3 = sw2 * sw, sw8 = sw3 * sw3 * sw2, sw9 = sw8 * sw;
+      const double db_dz = -90.0 * aabs * sw9 * iz11 + 12.0 * aw * sw3 * iz5;
+      const double db_dsw = (81.0 * aabs) * sw8 * iz10 - (9.0 * aw) * sw2 * iz4;
+      double ft, dt_dz, dt_dsw;
+      wall_top_terms(sw, aw, zi, k.z_space_wall, ft, dt_dz, dt_dsw);
+      for (int o = 0; o < ADJ_NO; ++o) {
+        gr[3 * o + 2] += lfi[3 * o + 2] * (db_dz + dt_dz);
+        gp[o * ADJ_NP + AP_SIG] += lfi[3 * o + 2] * (db_dsw + dt_dsw) * 0.5;
+      }
+    }
+  }
+
+  for (int c = 0; c < 6; ++c) {
+    gr[c] = group_sum<TPP>(gr[c]);
+    gu[c] = group_sum<TPP>(gu[c]);
+  }
+  if (active && sub == 0) {
+    for (int c = 0; c < 6; ++c) {
+      lr[6 * (size_t)i + c] += gr[c];
+      lu[6 * (size_t)i + c] += gu[c];
+    }
+  }
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = t < ADJ_NO * ADJ_NP ? gp[t] : 0.0;
+  block_partials(vals, part);
+}
+
+// Device buffers of one adjoint window.
+struct AdjBufs {
+  double *ck_r = nullptr, *ck_v = nullptr, *ck_a = nullptr;  // checkpoints [nseg][N][3]
+  int* ck_mic = nullptr;
+  // the neighbour list in effect at each checkpoint (rows, counts, skin
+  // reference, cells), so a recomputed segment repeats the forward bit for bit
+  unsigned* ck_nl = nullptr;
+  int *ck_nn = nullptr, *ck_cell = nullptr;
+  double* ck_rref = nullptr;
+  double *sg_r = nullptr, *sg_u = nullptr;  // recomputed segment [K][N][3]
+  int* sg_mic = nullptr;
+  double *end_r = nullptr, *end_v = nullptr, *end_a = nullptr, *end_f = nullptr;
+  int* end_mic = nullptr;
+  double *lr = nullptr, *lv = nullptr, *la = nullptr, *lf = nullptr;  // [N][2][3]
+  double* obs = nullptr;   // [W][MFA_NTERMS]
+  double* grad = nullptr;  // [W][2][MFA_NTERMS]
+  ~AdjBufs() {
+    void* ptrs[] = {ck_r, ck_v, ck_a, ck_mic, ck_nl, ck_nn, ck_cell, ck_rref, sg_r, sg_u, sg_mic,
+                    end_r, end_v, end_a, end_f, end_mic, lr, lv, la, lf, obs, grad};
+    for (void* q : ptrs)
+      if (q) hipFree(q);
+  }
+};
+
+template <int TPP>
+static void launch_adj_force_t(mfa_ctx* c, const AdjK& ak, const AdjBufs& b) {
+  hipLaunchKernelGGL((k_adj_force<TPP>), dim3(grid_of(c->n * TPP)), dim3(MFA_BLOCK), 0, c->stream,
+                     c->d_kp, c->d_rmap, ak, c->r, c->v, c->type, c->nl, c->nn, c->cap, b.lf, b.lr,
+                     b.lv, c->d_step, c->part);
+}
+
+static int launch_adj_force(mfa_ctx* c, const AdjK& ak, const AdjBufs& b) {
+  switch (c->tpp) {
+    case 1: launch_adj_force_t<1>(c, ak, b); break;
+    case 2: launch_adj_force_t<2>(c, ak, b); break;
+    case 4: launch_adj_force_t<4>(c, ak, b); break;
+    case 8: launch_adj_force_t<8>(c, ak, b); break;
+    case 16: launch_adj_force_t<16>(c, ak, b); break;
+    default: return set_err(c, MFA_ERR_ARG, "threads per particle %d unsupported", c->tpp);
+  }
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
+}
+
+static int d2d(mfa_ctx* c, void* dst, const void* src, size_t bytes) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return MFA_OK;
+}
+
+// One forward step of the window (verlet_positions + skin test, rebuild when
+// due, force pass, verlet_velocities) at step key `step`; with `obs_row` the
+// observables at r_n go to that row, with seg_* the force-pass inputs
+// (r_n, u_n, mic_n) are saved.
+static int adj_forward_step(mfa_ctx* c, int64_t step, double* obs_row, double* seg_r, double* seg_u,
+                            int* seg_mic) {
+  const size_t n3 = (size_t)c->n * 3;
+  int rc = set_step(c, step);
+  if (rc) return rc;
+  const int g = grid_of(c->n);
+  hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 1, c->n_cells, c->cell_count,
+                     c->type, c->cell_of, c->pos4, c->vel4);
+  HIPCHK(c, hipGetLastError());
+  rc = launch_rebuild(c);
+  if (rc) return rc;
+  if (seg_r) {
+    if ((rc = d2d(c, seg_r, c->r, n3 * sizeof(double))) || (rc = d2d(c, seg_u, c->v, n3 * sizeof(double))) ||
+        (rc = d2d(c, seg_mic, c->mic, n3 * sizeof(int))))
+      return rc;
+  }
+  rc = launch_force(c, TM_ALL);
+  if (rc) return rc;
+  if (obs_row) {
+    hipLaunchKernelGGL(k_adj_obs, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
+                       c->type, c->part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(MFA_NTERMS), dim3(MFA_BLOCK), 0, c->stream, c->part, 0,
+                       g, obs_row);
+    HIPCHK(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->f, c->v, c->a, c->type, c->part, 0, c->vel4);
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_steps,
+                                      int32_t checkpoint_every, int32_t want_grad,
+                                      mfa_adjoint_out* out) {
+  if (!c || !out || n_steps < 1 || checkpoint_every < 1)
+    return set_err(c, MFA_ERR_ARG, "adjoint: bad arguments");
+  const mfa_params& p = c->p;
+  if (c->n_rep != 1) return set_err(c, MFA_ERR_ARG, "adjoint: one replica per context");
+  if (p.coulomb || p.bending) return set_err(c, MFA_ERR_ARG, "adjoint: charges and bending are not covered");
+  if (p.fso_time > 0.0) return set_err(c, MFA_ERR_ARG, "adjoint: the switch-on ramp is not covered");
+  if (p.symmetry != 0) return set_err(c, MFA_ERR_ARG, "adjoint: the observables need the walls (SYMMETRY 0)");
+  // theta: the common epsilon and sigma of the tables
+  AdjK ak;
+  memset(&ak, 0, sizeof ak);
+  ak.eps_in = 0.0;
+  ak.sigma = 0.0;
+  for (int t = 0; t < MFA_NTYPE; ++t)
+    for (int q = 0; q < MFA_NTYPE; ++q) {
+      const double e = p.epsil[t][q] / 4.0, sg = p.sigma[t][q];
+      if (e != 0.0) {
+        if (ak.eps_in != 0.0 && e != ak.eps_in) return set_err(c, MFA_ERR_ARG, "adjoint: epsilon differs between types");
+        ak.eps_in = e;
+      }
+      if (sg != 0.0) {
+        if (ak.sigma != 0.0 && sg != ak.sigma) return set_err(c, MFA_ERR_ARG, "adjoint: sigma differs between types");
+        ak.sigma = sg;
+      }
+    }
+  if (ak.sigma == 0.0) return set_err(c, MFA_ERR_ARG, "adjoint: no sigma");
+  ak.r0 = sqrt(p.r_chain_2);
+  ak.k_in = p.k_chain * p.inv_r_chain_2;
+  ak.gamma = p.friction1;
+  const int W = n_steps, K = checkpoint_every < n_steps ? checkpoint_every : n_steps;
+  const int nseg = (W + K - 1) / K;
+  const size_t n = (size_t)c->n, n3 = n * 3;
+  AdjBufs b;
+#define AALLOC(ptr, bytes)                                                                 \
+  do {                                                                                     \
+    if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)                                  \
+      return set_err(c, MFA_ERR_NOMEM, "adjoint: hipMalloc of %zu bytes failed", (size_t)(bytes)); \
+  } while (0)
+  AALLOC(b.obs, (size_t)W * MFA_NTERMS * sizeof(double));
+  AALLOC(b.ck_r, (size_t)nseg * n3 * sizeof(double));
+  AALLOC(b.ck_v, (size_t)nseg * n3 * sizeof(double));
+  AALLOC(b.ck_a, (size_t)nseg * n3 * sizeof(double));
+  AALLOC(b.ck_mic, (size_t)nseg * n3 * sizeof(int));
+  if (want_grad) {
+    AALLOC(b.ck_nl, (size_t)nseg * n * c->cap * sizeof(unsigned));
+    AALLOC(b.ck_nn, (size_t)nseg * n * sizeof(int));
+    AALLOC(b.ck_cell, (size_t)nseg * n * sizeof(int));
+    AALLOC(b.ck_rref, (size_t)nseg * n3 * sizeof(double));
+    AALLOC(b.sg_r, (size_t)K * n3 * sizeof(double));
+    AALLOC(b.sg_u, (size_t)K * n3 * sizeof(double));
+    AALLOC(b.sg_mic, (size_t)K * n3 * sizeof(int));
+    AALLOC(b.end_r, n3 * sizeof(double));
+    AALLOC(b.end_v, n3 * sizeof(double));
+    AALLOC(b.end_a, n3 * sizeof(double));
+    AALLOC(b.end_f, n3 * sizeof(double));
+    AALLOC(b.end_mic, n3 * sizeof(int));
+    AALLOC(b.lr, n * 6 * sizeof(double));
+    AALLOC(b.lv, n * 6 * sizeof(double));
+    AALLOC(b.la, n * 6 * sizeof(double));
+    AALLOC(b.lf, n * 6 * sizeof(double));
+    AALLOC(b.grad, (size_t)W * 2 * MFA_NTERMS * sizeof(double));
+  }
+#undef AALLOC
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->flags, 1,
+                     (int)F_NREBUILD, 0);
+  // forward pass with checkpoints
+  for (int s = 1; s <= W; ++s) {
+    if ((s - 1) % K == 0) {
+      const size_t off = (size_t)((s - 1) / K) * n3;
+      if ((rc = d2d(c, b.ck_r + off, c->r, n3 * sizeof(double))) ||
+          (rc = d2d(c, b.ck_v + off, c->v, n3 * sizeof(double))) ||
+          (rc = d2d(c, b.ck_a + off, c->a, n3 * sizeof(double))) ||
+          (rc = d2d(c, b.ck_mic + off, c->mic, n3 * sizeof(int))))
+        return rc;
+      if (want_grad) {
+        const size_t sg = (size_t)((s - 1) / K);
+        if ((rc = d2d(c, b.ck_nl + sg * n * c->cap, c->nl, n * c->cap * sizeof(unsigned))) ||
+            (rc = d2d(c, b.ck_nn + sg * n, c->nn, n * sizeof(int))) ||
+            (rc = d2d(c, b.ck_cell + sg * n, c->cell_of, n * sizeof(int))) ||
+            (rc = d2d(c, b.ck_rref + off, c->r_ref, n3 * sizeof(double))))
+          return rc;
+      }
+    }
+    rc = adj_forward_step(c, first_step + s - 1, b.obs + (size_t)(s - 1) * MFA_NTERMS, nullptr,
+                          nullptr, nullptr);
+    if (rc) return rc;
+  }
+  rc = check_flags(c);
+  if (rc) return rc;
+  std::vector<double> h_obs((size_t)W * MFA_NTERMS);
+  HIPCHK(c, hipMemcpyAsync(h_obs.data(), b.obs, h_obs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memset(out, 0, sizeof *out);
+  out->n_rebuilds = c->h_flags[F_NREBUILD];
+  int n2 = 0;
+  {
+    std::vector<int32_t> ht(n);
+    HIPCHK(c, hipMemcpy(ht.data(), c->type, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < p.n_part; ++i) n2 += ht[(size_t)i] == 2;
+  }
+  if (n2 == 0) return set_err(c, MFA_ERR_ARG, "adjoint: no type-2 beads for the brush height");
+  ak.inv_w = 1.0 / (double)W;
+  ak.inv_n2 = 1.0 / (double)n2;
+  ak.inv_area = 1.0 / (p.boundary[0] * p.boundary[1]);
+  {
+    double zs = 0.0, fs = 0.0;
+    for (int s = 0; s < W; ++s) {
+      zs += h_obs[(size_t)s * MFA_NTERMS + 0] * ak.inv_n2;
+      fs += -h_obs[(size_t)s * MFA_NTERMS + 1] * ak.inv_area;
+    }
+    out->height = zs * ak.inv_w;
+    out->p_wall = fs * ak.inv_w;
+  }
+  if (!want_grad) return MFA_OK;
+
+  // keep the end state, then run the segments backward
+  if ((rc = d2d(c, b.end_r, c->r, n3 * sizeof(double))) || (rc = d2d(c, b.end_v, c->v, n3 * sizeof(double))) ||
+      (rc = d2d(c, b.end_a, c->a, n3 * sizeof(double))) || (rc = d2d(c, b.end_f, c->f, n3 * sizeof(double))) ||
+      (rc = d2d(c, b.end_mic, c->mic, n3 * sizeof(int))))
+    return rc;
+  HIPCHK(c, hipMemsetAsync(b.lr, 0, n * 6 * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(b.lv, 0, n * 6 * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(b.la, 0, n * 6 * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(b.lf, 0, n * 6 * sizeof(double), c->stream));
+  const int g = grid_of(c->n);
+  const int gf = grid_of(c->n * c->tpp);
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int s0 = seg * K, s1 = s0 + K < W ? s0 + K : W;
+    const size_t off = (size_t)seg * n3;
+    if ((rc = d2d(c, c->r, b.ck_r + off, n3 * sizeof(double))) ||
+        (rc = d2d(c, c->v, b.ck_v + off, n3 * sizeof(double))) ||
+        (rc = d2d(c, c->a, b.ck_a + off, n3 * sizeof(double))) ||
+        (rc = d2d(c, c->mic, b.ck_mic + off, n3 * sizeof(int))) ||
+        (rc = d2d(c, c->nl, b.ck_nl + (size_t)seg * n * c->cap, n * c->cap * sizeof(unsigned))) ||
+        (rc = d2d(c, c->nn, b.ck_nn + (size_t)seg * n, n * sizeof(int))) ||
+        (rc = d2d(c, c->cell_of, b.ck_cell + (size_t)seg * n, n * sizeof(int))) ||
+        (rc = d2d(c, c->r_ref, b.ck_rref + off, n3 * sizeof(double))))
+      return rc;
+    for (int s = s0 + 1; s <= s1; ++s) {
+      const size_t so = (size_t)(s - s0 - 1) * n3;
+      rc = adj_forward_step(c, first_step + s - 1, nullptr, b.sg_r + so, b.sg_u + so, b.sg_mic + so);
+      if (rc) return rc;
+    }
+    for (int s = s1; s > s0; --s) {
+      const size_t so = (size_t)(s - s0 - 1) * n3;
+      if ((rc = d2d(c, c->r, b.sg_r + so, n3 * sizeof(double))) ||
+          (rc = d2d(c, c->v, b.sg_u + so, n3 * sizeof(double))) ||
+          (rc = d2d(c, c->mic, b.sg_mic + so, n3 * sizeof(int))))
+        return rc;
+      // the list must hold the pairs at r_n: rebuild when r_n left the skin
+      hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream,
+                         c->n_cells, c->cell_count);
+      hipLaunchKernelGGL(k_check_skin, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                         c->r, c->r_ref, c->mic, c->flags);
+      rc = launch_rebuild(c);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->flags, 1,
+                         (int)F_REBUILD, 0);
+      rc = set_step(c, first_step + s - 1);
+      if (rc) return rc;
+      double* grow = b.grad + (size_t)(s - 1) * 2 * MFA_NTERMS;
+      hipLaunchKernelGGL(k_adj_pre, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, ak,
+                         c->r, c->type, b.lr, b.lv, b.la, b.lf, c->part);
+      hipLaunchKernelGGL(k_reduce_partials, dim3(MFA_NTERMS), dim3(MFA_BLOCK), 0, c->stream, c->part,
+                         0, g, grow);
+      rc = launch_adj_force(c, ak, b);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_reduce_partials, dim3(MFA_NTERMS), dim3(MFA_BLOCK), 0, c->stream, c->part,
+                         0, gf, grow + MFA_NTERMS);
+      hipLaunchKernelGGL(k_adj_post, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, b.lr,
+                         b.lv, b.la);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
+  rc = check_flags(c);
+  if (rc) return rc;
+  std::vector<double> h_grad((size_t)W * 2 * MFA_NTERMS);
+  HIPCHK(c, hipMemcpyAsync(h_grad.data(), b.grad, h_grad.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  // restore the end state of the window
+  if ((rc = d2d(c, c->r, b.end_r, n3 * sizeof(double))) || (rc = d2d(c, c->v, b.end_v, n3 * sizeof(double))) ||
+      (rc = d2d(c, c->a, b.end_a, n3 * sizeof(double))) || (rc = d2d(c, c->f, b.end_f, n3 * sizeof(double))) ||
+      (rc = d2d(c, c->mic, b.end_mic, n3 * sizeof(int))))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->need_rebuild = true;
+  for (int s = 0; s < W; ++s)
+    for (int row = 0; row < 2; ++row)
+      for (int q = 0; q < ADJ_NP; ++q) {
+        const double* gr = &h_grad[((size_t)s * 2 + row) * MFA_NTERMS];
+        out->d_height[q] += gr[q];
+        out->d_p_wall[q] += gr[ADJ_NP + q];
+      }
   return MFA_OK;
 }

@@ -521,7 +521,9 @@ extern "C" int32_t mfa_generate(const mfa_params* p, int32_t kind, uint64_t seed
         for (int tries = 0; tries < 4000 && !ok; ++tries) {
           double e[3], r[3];
           s.unit_vector(e);
-          if (tries < 2000 && e[2] < 0.0) e[2] = -e[2];
+          // upward steps while the chain is in the lower half of the channel, so
+          // long chains do not pile up against the top wall
+          if (tries < 2000 && e[2] < 0.0 && prev[2] < 0.5 * L[2]) e[2] = -e[2];
           for (int d = 0; d < 3; ++d) r[d] = prev[d] + bond * e[d];
           r[0] = wrap(r[0], L[0]);
           r[1] = wrap(r[1], L[1]);

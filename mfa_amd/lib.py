@@ -91,6 +91,14 @@ class Obs(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class AdjointOut(C.Structure):
+    _fields_ = [
+        ("height", C.c_double), ("p_wall", C.c_double),
+        ("d_height", C.c_double * 5), ("d_p_wall", C.c_double * 5),
+        ("n_rebuilds", C.c_int64),
+    ]
+
+
 # Every function include/mfa_hip.h declares, with its ctypes signature.
 _P = C.c_void_p
 _dp = C.POINTER(C.c_double)
@@ -134,6 +142,8 @@ SIGNATURES = {
     "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
     "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, _ip]),
+    "mfa_adjoint_window": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(AdjointOut)]),
 }
 
 _lib = None
@@ -351,6 +361,19 @@ class Context:
         self._chk(self.lib.mfa_run_steps(self.h, first_step, n, C.byref(obs) if want_obs else None),
                   "mfa_run_steps")
         return obs.as_dict() if want_obs else None
+
+    def adjoint_window(self, first_step, n_steps, checkpoint_every=20, want_grad=True):
+        """Brush height and top-wall pressure averaged over the window and
+        (want_grad) their gradients with respect to (epsilon_LJ, sigma,
+        k_FENE, R0, gamma_DPD).  Returns J (2,) and G (2, 5) as numpy arrays
+        plus the forward pass's rebuild count."""
+        out = AdjointOut()
+        self._chk(self.lib.mfa_adjoint_window(self.h, first_step, n_steps, checkpoint_every,
+                                              1 if want_grad else 0, C.byref(out)),
+                  "mfa_adjoint_window")
+        J = np.array([out.height, out.p_wall])
+        G = np.array([list(out.d_height), list(out.d_p_wall)])
+        return J, G, out.n_rebuilds
 
     def energies(self, step):
         obs = Obs()

@@ -6,7 +6,7 @@
 ! to pin the C oracle against the real reference; nothing ships it.
 !
 ! Run in a directory holding system_input, mfa_input and conf_old:
-!   ref_md <force_step> <n_steps>
+!   ref_md <force_step> <n_steps> [trace]
 ! writes
 !   ref_init.bin   the state, the cell of every particle, the pair list of
 !                  the first binning, and one force pass at r_time =
@@ -18,18 +18,27 @@
 ! r0, v, a, force as float64 (3, n_part) and mic_count as int32 (3, n_part);
 ! ref_init.bin adds the cells int32 (3, n_part), int32 n_pairs and the pairs
 ! int32 (2, n_pairs), each pair as (min, max).
+! With `trace`, ref_trace.bin gets per step (after verlet_velocities) two
+! float64: the sum of z over type-2 beads and the sum over all particles of
+! the top wall's z force (the second half of fluid_wall.f90:107-121).
+! Test infrastructure, used by the adjoint's finite-difference check.
 program ref_md
 #include "control_simulation.h"
   use commons
   implicit none
   character(len=32) :: arg
-  integer :: force_step, n_steps, n_rebuilds, n_pairs, k, j, u
+  integer :: force_step, n_steps, n_rebuilds, n_pairs, k, j, u, ut
   integer, allocatable :: cells(:,:)
+  logical :: trace
+  real(kind=8) :: zsum, ftop, izt
 
   call get_command_argument(1, arg)
   read(arg, *) force_step
   call get_command_argument(2, arg)
   read(arg, *) n_steps
+  trace = command_argument_count() >= 3
+  if (trace) open(newunit=ut, file="ref_trace.bin", access="stream", form="unformatted", &
+                  status="replace")
 
   call init_system()
   call init_params()
@@ -87,7 +96,21 @@ program ref_md
     call angles()
     call coulomb_k()
     call verlet_velocities()
+    if (trace) then
+      zsum = 0.d0
+      ftop = 0.d0
+      do k = 1, n_part
+        if (a_type(k) == 2) zsum = zsum + r0(3, k)
+#if SYMMETRY == 0
+        izt = 1.d0 / (z_space_wall - r0(3, k))
+        ftop = ftop - 9.d0 * abs(a_wall(a_type(k))) * sigma_wall(a_type(k))**9 * izt**10 &
+                    + 3.d0 * a_wall(a_type(k)) * sigma_wall(a_type(k))**3 * izt**4
+#endif
+      end do
+      write(ut) zsum, ftop
+    end if
   end do
+  if (trace) close(ut)
   open(newunit=u, file="ref_final.bin", access="stream", form="unformatted", status="replace")
   call put_state(u, n_rebuilds)
   close(u)

@@ -1,0 +1,170 @@
+"""The adjoint window (mfa_adjoint_window) on the GPU: gradients of the
+time-averaged brush height and top-wall pressure with respect to
+(epsilon_LJ, sigma, k_FENE, R0, gamma_DPD) against central finite
+differences (relative step 1e-4) of the same forward pass, of the oracle's
+trajectories and of the reference's own Fortran build; through the
+torch.autograd wrapper; and invariance under the checkpoint interval.
+
+Bar: every gradient component within 1e-4 relative of the finite
+difference (components below 1e-3 of their row's largest count against that
+largest value)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from mfa_amd import configs, inputs as io, lib
+from oracle import oracle as orc
+from tests.adjoint_torch import WindowObservables, central_fd, make_runner
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+W, K = 40, 10
+
+
+def _grad_close(G, Gfd, tol=1e-4):
+    for o in range(2):
+        scale = np.abs(Gfd[o]).max()
+        for q in range(G.shape[1]):
+            ref = Gfd[o, q]
+            den = abs(ref) if abs(ref) > 1e-3 * scale else scale
+            assert abs(G[o, q] - ref) <= tol * den, (o, q, G[o, q], ref)
+
+
+@pytest.fixture(scope="module")
+def relaxed():
+    """The small brush after the switch-on ramp (a start without overlaps)."""
+    kind, x = configs.small_cases()["brush_small"]
+    p, st = configs.prepare_brush(x, kind, precision=lib.MFA_FP64)
+    print("relaxed brush_small: n=%d" % p.n_part, flush=True)
+    return x, st
+
+
+def test_adjoint_matches_finite_differences(relaxed):
+    x, st = relaxed
+    run = make_runner(x, st, 1, W, K)
+    th = configs.theta_of(x)
+    J, G = run(th)
+    Gfd = central_fd(make_runner(x, st, 1, W, K, want_grad=False), th)
+    print("J", J, "\nadjoint\n", G, "\nFD\n", Gfd, flush=True)
+    _grad_close(G, Gfd)
+
+
+def test_checkpoint_interval_and_forward_state(relaxed):
+    x, st = relaxed
+    th = configs.theta_of(x)
+    p = lib.derive(x)
+    outs = []
+    for k in (W, 7, 1):
+        with lib.Context(p, lib.MFA_FP64) as c:
+            c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
+            J, G, _ = c.adjoint_window(1, W, k, True)
+            outs.append((J, G, c.get_state()))
+    with lib.Context(p, lib.MFA_FP64) as c:
+        c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
+        c.run_steps(1, W, want_obs=False)
+        ref = c.get_state()
+    # the forward pass and the recomputed segments repeat run_steps bit for
+    # bit; the adjoint's FP64 sums follow the order of the list rebuilt in the
+    # backward sweep, so the gradients agree to rounding
+    for J, G, s in outs:
+        assert np.array_equal(J, outs[0][0])
+        assert np.allclose(G, outs[0][1], rtol=1e-11, atol=1e-14 * np.abs(G).max())
+        for key in ("r", "v", "a", "mic"):
+            assert np.array_equal(s[key], ref[key]), key
+
+
+def test_torch_autograd_wrapper(relaxed):
+    x, st = relaxed
+    run = make_runner(x, st, 1, W, K)
+    th0 = configs.theta_of(x)
+    theta = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    J = WindowObservables.apply(theta, run)
+    loss = 2.0 * J[0] - 0.5 * J[1]
+    (g,) = torch.autograd.grad(loss, theta)
+    _, G = run(th0)
+    assert np.allclose(g.numpy(), 2.0 * G[0] - 0.5 * G[1], rtol=1e-12, atol=0.0)
+
+
+def test_mixed_forward_gradient_close_to_fp64(relaxed):
+    x, st = relaxed
+    th = configs.theta_of(x)
+    _, G64 = make_runner(x, st, 1, W, K)(th)
+    Jm, Gm = make_runner(x, st, 1, W, K, precision=lib.MFA_MIXED)(th)
+    print("mixed-forward gradient\n", Gm, flush=True)
+    _grad_close(Gm, G64, tol=1e-3)
+
+
+def _oracle_window(x, st, theta, first, n):
+    """The observables from the oracle's trajectory: mean z of the type-2
+    beads and -sum F_z(top wall)/(Lx Ly) after each step, averaged."""
+    xi = configs.with_theta(x, theta)
+    s = orc.System(xi)
+    s.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
+    p = lib.derive(xi)
+    t2 = st["t"] == 2
+    hs, ps = 0.0, 0.0
+    for k in range(n):
+        s.run_steps(first + k, 1)
+        r = s.get_state()["r"]
+        hs += r[t2, 2].mean()
+        s.zero_force()
+        ps += -s.fluid_wall()[2] / (p.boundary[0] * p.boundary[1])
+    return np.array([hs / n, ps / n]), None
+
+
+def test_adjoint_matches_oracle_finite_differences(relaxed):
+    x, st = relaxed
+    th = configs.theta_of(x)
+    _, G = make_runner(x, st, 1, W, K)(th)
+    J_orc, _ = _oracle_window(x, st, th, 1, W)
+    J_gpu, _ = make_runner(x, st, 1, W, K, want_grad=False)(th)
+    assert np.allclose(J_orc, J_gpu, rtol=1e-10, atol=0.0), (J_orc, J_gpu)
+    Gfd = central_fd(lambda t: _oracle_window(x, st, t, 1, W), th)
+    print("oracle FD\n", Gfd, flush=True)
+    _grad_close(G, Gfd)
+
+
+def test_adjoint_matches_reference_finite_differences(relaxed, tmp_path):
+    """FD of the reference's own build (ref_md_brush) at temperature 0 (no
+    random force, so its ziggurat stream drops out) and its list skin 0.4."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_md_brush")
+    if not os.path.exists(exe):
+        pytest.skip("reference build absent (oracle/ref/build_ref.sh needs the reference sources)")
+    x0, st0 = relaxed
+    x = lib.Inputs.from_buffer_copy(x0)
+    x.temp = 0.0
+    x.skin = 0.0
+    n = 20
+    th = configs.theta_of(x)
+    # both sides start from the conf_old text (7 decimals, init_config.f90:561-568)
+    d0 = str(tmp_path / "conf")
+    io.write_run_dir(d0, x, st0["t"], st0["r"], st0["v"], st0["a"])
+    t2, r2, v2, a2 = io.read_conf(os.path.join(d0, "conf_old"), len(st0["t"]))
+    st = {"r": r2, "v": v2, "a": a2, "t": t2, "mic": None}
+    _, G = make_runner(x, st, x.s_time, n, 5)(th)
+
+    def ref_run(theta):
+        xi = configs.with_theta(x, theta)
+        d = str(tmp_path / ("run_%d" % ref_run.calls))
+        ref_run.calls += 1
+        os.makedirs(d)
+        io.write_run_dir(d, xi, st["t"], st["r"], st["v"], st["a"])
+        res = subprocess.run([exe, "0", str(n), "trace"], cwd=d, capture_output=True, text=True,
+                             timeout=300)
+        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+        tr = np.fromfile(os.path.join(d, "ref_trace.bin"), dtype=np.float64).reshape(n, 2)
+        p = lib.derive(xi)
+        n2 = int((st["t"] == 2).sum())
+        return np.array([tr[:, 0].mean() / n2,
+                         -tr[:, 1].mean() / (p.boundary[0] * p.boundary[1])]), None
+    ref_run.calls = 0
+    J_ref, _ = ref_run(th)
+    J_gpu, _ = make_runner(x, st, x.s_time, n, 5, want_grad=False)(th)
+    print("J ref", J_ref, "gpu", J_gpu, flush=True)
+    assert np.allclose(J_ref, J_gpu, rtol=1e-8, atol=0.0)
+    Gfd = central_fd(ref_run, th)
+    print("adjoint\n", G, "\nreference FD\n", Gfd, flush=True)
+    _grad_close(G, Gfd)
