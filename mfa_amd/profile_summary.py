@@ -7,6 +7,9 @@
     force pass, scaled by the reading of the calibration copy of known size.
 --counters run_counter_collection.csv : median of every counter per dispatch
     of each kernel.
+--totals run_kernel_trace.csv : every kernel of the whole run by total time
+    (launches, average and total duration), for runs without a timed window
+    of the melt bench (the replica batch, the charged brush).
 """
 import argparse
 import csv
@@ -14,8 +17,16 @@ import statistics
 from collections import defaultdict
 
 COPY_BYTES = 256 * 1024 * 1024
-MIXED_FORCE = "void k_force_mixed<8, 0>"
-FP64_FORCE = "void k_force64<8>"
+# kernel-name prefixes of the force pass without energies (any extra-terms flag)
+MIXED_FORCE = "void k_force_mixed<8, 0"
+FP64_FORCE = "void k_force64<8"
+
+
+def _find(kern, prefix):
+    for k, v in kern.items():
+        if k.startswith(prefix):
+            return v
+    return None
 
 
 def trace(path, n_timed):
@@ -54,7 +65,7 @@ def pmc(path):
     out = []
     for counter, kern in by.items():
         copy = [v for k, v in kern.items() if "copybuffer" in k.lower()]
-        force = kern.get(MIXED_FORCE) or kern.get(FP64_FORCE) or []
+        force = _find(kern, MIXED_FORCE) or _find(kern, FP64_FORCE) or []
         if not copy or not force:
             out.append("%s: calibration copy or force pass not found (%s)" % (counter, list(kern)))
             continue
@@ -81,12 +92,27 @@ def counters(path):
     return "\n".join(out)
 
 
+def totals(path, top=14):
+    rows = list(csv.DictReader(open(path)))
+    d = defaultdict(list)
+    for r in rows:
+        d[r["Kernel_Name"].split("(")[0]].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    tot = sum(sum(v) for v in d.values())
+    out = []
+    for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        out.append("%-34s launches %7d  avg %9.2f us  total %9.2f ms (%4.1f%%)"
+                   % (k[:34], len(v), sum(v) / len(v), sum(v) / 1e3, 100.0 * sum(v) / tot))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace")
     ap.add_argument("--timed", type=int, default=2400)
     ap.add_argument("--pmc")
     ap.add_argument("--counters")
+    ap.add_argument("--totals")
     a = ap.parse_args()
     if a.trace:
         print(trace(a.trace, a.timed))
@@ -94,6 +120,8 @@ def main():
         print(pmc(a.pmc))
     if a.counters:
         print(counters(a.counters))
+    if a.totals:
+        print(totals(a.totals))
 
 
 if __name__ == "__main__":
