@@ -83,6 +83,7 @@ struct KParams {
   double k_bend, alpha_eq;
   float pf_margin;               // bound on |r^2(FP32) - r^2(FP64)| in the list build
   float ff_margin;               // the same for the cell-relative FP32 view of the force pass
+  float lf_margin;               // ... and of the list build, against the list radius
   int nb[3];
   int n_cells, cell_cap;
   // replica layout: particles [p0, p0 + n) (p0 a multiple of 256), cells
@@ -123,7 +124,8 @@ struct mfa_ctx {
   hipStream_t stream;
   double *r, *v, *a, *f, *r_ref;
   int32_t *type, *mic;
-  int32_t *cell_of, *cell_count, *cell_slots;
+  int32_t *cell_of, *cell_count, *cell_slots, *cell_start;
+  int32_t *perm, *sidx;  // sorted position -> particle, particle -> sorted position
   int cell_cap;
   unsigned* nl;
   int32_t* nn;
@@ -346,7 +348,7 @@ __device__ __forceinline__ void zero_counts(int nc_tot, int* __restrict__ cell_c
 // FP32 view of particle i: position relative to the origin of `cell` (its
 // cell at the last rebuild), wrapped back across periodic faces it crossed
 // since, and velocity.
-__device__ __forceinline__ void write_view(const KParams& k, int i, int cell, int ti,
+__device__ __forceinline__ void write_view(const KParams& k, int i, int cell, int ti, int il,
                                            const double* ri, const double* vi,
                                            float4* __restrict__ pos4, float4* __restrict__ vel4) {
   const int cx = cell % k.nb[0], cy = (cell / k.nb[0]) % k.nb[1], cz = cell / (k.nb[0] * k.nb[1]);
@@ -356,7 +358,7 @@ __device__ __forceinline__ void write_view(const KParams& k, int i, int cell, in
   lx -= k.L[0] * rint(lx * k.invL[0]);
   ly -= k.L[1] * rint(ly * k.invL[1]);
   if (k.symmetry == 1) lz -= k.L[2] * rint(lz * k.invL[2]);
-  pos4[i] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float(ti));
+  pos4[i] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float((il << 2) | ti));
   vel4[i] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
 }
 
@@ -366,7 +368,8 @@ __global__ void k_verlet_positions(const KParams* __restrict__ kb, const int* __
                                    const double* __restrict__ r_ref, int* __restrict__ flags,
                                    int with_skin, int nc_tot, int* __restrict__ cell_count,
                                    const int* __restrict__ type, const int* __restrict__ cell_of,
-                                   float4* __restrict__ pos4, float4* __restrict__ vel4) {
+                                   const int* __restrict__ sidx, float4* __restrict__ pos4,
+                                   float4* __restrict__ vel4) {
   zero_counts(nc_tot, cell_count);
   MFA_REPLICA(blockIdx.x * blockDim.x);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -388,7 +391,7 @@ __global__ void k_verlet_positions(const KParams* __restrict__ kb, const int* __
     }
     if (wrapped)
       for (int d = 0; d < 3; ++d) mic[3 * (size_t)i + d] = mi[d];
-    write_view(k, i, cell_of[i], type[i] - 1, ri, vi, pos4, vel4);
+    write_view(k, sidx[i], cell_of[i], type[i] - 1, i - k.p0, ri, vi, pos4, vel4);
     if (with_skin) {
       double rr[3];
       for (int d = 0; d < 3; ++d) rr[d] = r_ref[3 * (size_t)i + d];
@@ -427,7 +430,7 @@ __global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* _
                                     double* __restrict__ f, double* __restrict__ v,
                                     double* __restrict__ a, const int* __restrict__ type,
                                     double* __restrict__ part, int energy,
-                                    float4* __restrict__ vel4) {
+                                    const int* __restrict__ sidx, float4* __restrict__ vel4) {
   MFA_REPLICA(blockIdx.x * blockDim.x);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double vals[MFA_NTERMS];
@@ -444,7 +447,7 @@ __global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* _
       v[3 * (size_t)i + d] = vi[d];
       a[3 * (size_t)i + d] = ai[d];
     }
-    vel4[i] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
+    vel4[sidx[i]] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
     if (is_head(k, i - k.p0))
       for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] = 0.0;
     vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
@@ -461,7 +464,8 @@ __global__ void k_fused_integrate(const KParams* __restrict__ kb, const int* __r
                                   int* __restrict__ flags, double* __restrict__ part, int energy,
                                   int nc_tot, int* __restrict__ cell_count,
                                   long long* __restrict__ d_step, const int* __restrict__ cell_of,
-                                  float4* __restrict__ pos4, float4* __restrict__ vel4) {
+                                  const int* __restrict__ sidx, float4* __restrict__ pos4,
+                                  float4* __restrict__ vel4) {
   zero_counts(nc_tot, cell_count);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *d_step += 1;  // next step; read by the next force pass only
@@ -491,7 +495,7 @@ __global__ void k_fused_integrate(const KParams* __restrict__ kb, const int* __r
     }
     if (wrapped)
       for (int d = 0; d < 3; ++d) mic[3 * (size_t)i + d] = mi[d];
-    write_view(k, i, cell_of[i], ti, ri, vi, pos4, vel4);
+    write_view(k, sidx[i], cell_of[i], ti, i - k.p0, ri, vi, pos4, vel4);
     over = skin_one(k, ri, rr, mi) ? 1.0 : 0.0;
   }
   over = wave_max(over);
@@ -544,187 +548,224 @@ __global__ void k_cell_fill(const KParams* __restrict__ kb, const int* __restric
     atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
 }
 
+// Spatial order.  At each rebuild the particles of a replica are ordered by
+// cell (cells x fastest, each cell by ascending id): perm[s] is the particle
+// at sorted position s, sidx[i] the sorted position of particle i, and
+// [cell_start[c], cell_start[c] + cell_count[c]) the positions of cell c.
+// The FP32 view, the list rows and the list entries use sorted positions, so
+// a particle's neighbours lie close to it in memory; the FP64 state keeps the
+// reference's particle order.
+//
+// Exclusive scan of the cell counts of each replica (one workgroup per
+// replica): cell_start[c] = p0 + particles in the replica's cells before c.
+#define SCAN_BLOCK 1024
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_scan(const KParams* __restrict__ kb,
+                                                          const int* __restrict__ cell_count,
+                                                          int* __restrict__ cell_start,
+                                                          const int* __restrict__ flags_all) {
+  const KParams& k = kb[blockIdx.x];
+  const int* flags = flags_all + k.rep * F_NFLAGS;
+  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;
+  __shared__ int wsum[SCAN_BLOCK / 64];
+  __shared__ int carry_s;
+  if (threadIdx.x == 0) carry_s = k.p0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < k.n_cells; c0 += SCAN_BLOCK) {
+    const int c = c0 + threadIdx.x;
+    const int v = c < k.n_cells ? min(cell_count[k.c0 + c], k.cell_cap) : 0;
+    int x = v;  // inclusive scan within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int before = carry_s;
+    for (int w = 0; w < wid; ++w) before += wsum[w];
+    if (c < k.n_cells) cell_start[k.c0 + c] = before + x - v;
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) carry_s = before + x;
+    __syncthreads();
+  }
+}
+
+// One wave per cell, four cells per workgroup: orders each cell's particles
+// by id, records the permutation, writes the FP32 view of each (relative to
+// its cell's origin) at its sorted position and resets the skin reference
+// (binning.f90:125-127).
+__global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
+    const KParams* __restrict__ kb, const int* __restrict__ cell_rep, int n_cells,
+    const double* __restrict__ r, const double* __restrict__ v, const int* __restrict__ type,
+    const int* __restrict__ mic, const int* __restrict__ cell_count,
+    const int* __restrict__ cell_slots, const int* __restrict__ cell_start,
+    int* __restrict__ perm, int* __restrict__ sidx, double* __restrict__ r_ref,
+    const int* __restrict__ flags_all, float4* __restrict__ pos4, float4* __restrict__ vel4) {
+  extern __shared__ int sort_ids[];  // [4][cell_cap]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cg = blockIdx.x * 4 + w;
+  const KParams& k = kb[cell_rep[cg < n_cells ? cg : 0]];
+  const bool on = cg < n_cells && flags_all[k.rep * F_NFLAGS + F_REBUILD] != 0 &&
+                  flags_all[k.rep * F_NFLAGS + F_ERROR] == 0;
+  int* ids = sort_ids + w * k.cell_cap;
+  const int m = on ? min(cell_count[cg], k.cell_cap) : 0;
+  for (int t = lane; t < m; t += 64) ids[t] = cell_slots[(size_t)cg * k.cell_cap + t];
+  __syncthreads();
+  if (!on) return;
+  const int c = cg - k.c0;
+  const int s0 = cell_start[cg];
+  for (int t = lane; t < m; t += 64) {
+    const int id = ids[t];
+    int rank = 0;
+    for (int u = 0; u < m; ++u) rank += ids[u] < id ? 1 : 0;
+    const int s = s0 + rank;
+    perm[s] = id;
+    sidx[id] = s;
+    double ri[3], vi[3];
+    for (int d = 0; d < 3; ++d) {
+      ri[d] = r[3 * (size_t)id + d];
+      vi[d] = v[3 * (size_t)id + d];
+      r_ref[3 * (size_t)id + d] = ri[d] + (double)mic[3 * (size_t)id + d] * k.L[d];
+    }
+    write_view(k, s, c, type[id] - 1, id - k.p0, ri, vi, pos4, vel4);
+  }
+}
+
 // Full Verlet list: every j != i in the 27 cells around i with
 // r^2 < list_2(ti,tj) (binning.f90:64-113, both directions stored).
-// One workgroup per cell.  The neighbourhood is staged in LDS one x-plane of
-// 9 cells at a time (cells x outer, y, z inner, each in ascending id) as
-// (id, float x, y, z shifted by the cell's periodic image, type).  A group of
-// LTPP lanes handles one particle and tests LTPP consecutive candidates at a
-// time: an FP32 distance decides when it is farther than its rounding bound
-// from the radius, the FP64 criterion with the reference's image convention
-// decides the rest, and an ordered ballot appends the hits, so each list
-// holds its neighbours in the order of a sequential scan.  Each entry carries
-// the offset code of the neighbour's cell.  With walls (no periodicity in z)
-// the cells beyond the z faces hold no neighbours (their particles are more
-// than a cell width away without the image) and are not staged.  Also resets
-// the skin reference (binning.f90:125-127) and writes the FP32 view of the
-// cell's own particles relative to this cell.
-#define LTPP 8
+// One wave per cell, four cells per workgroup.  The lanes take the
+// candidates of the 27 cells 64 at a time, in offset-code order
+// 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in ascending id (the order of a
+// sequential scan); for each particle of the cell in turn the wave tests the
+// 64 candidates at once and a ballot appends the hits to that particle's row
+// in lane order (the row lengths live in the lanes: particle h's in lane
+// h % 64).  Separations come from the FP32 view (both positions relative to
+// their cell origins, plus the offset of the candidate's cell): no periodic
+// arithmetic.  A separation whose r^2 lies within the view's rounding bound
+// of the radius is decided again in FP64 with the reference's image
+// convention (binning.f90:84-105).  Each entry is the candidate's sorted
+// position with its cell's offset code above bit 27.  With walls (no
+// periodicity in z) cells beyond the z faces hold no neighbours.
+#define LB_SLOTS 4  // particles per cell up to 64 * LB_SLOTS
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
-    const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
-    const double* __restrict__ r, const double* __restrict__ v,
-    const int* __restrict__ type, const int* __restrict__ mic,
-    const int* __restrict__ cell_count, const int* __restrict__ cell_slots,
-    unsigned* __restrict__ nl, int* __restrict__ nn, int cap, double* __restrict__ r_ref,
-    int* __restrict__ flags_all, float4* __restrict__ pos4, float4* __restrict__ vel4) {
-  const KParams& k = kb[cell_rep[blockIdx.x]];
+    const KParams* __restrict__ kb, const int* __restrict__ cell_rep, int n_cells,
+    const double* __restrict__ r, const int* __restrict__ cell_count,
+    const int* __restrict__ cell_start, const int* __restrict__ perm,
+    const float4* __restrict__ pos4, unsigned* __restrict__ nl, int* __restrict__ nn, int cap,
+    int* __restrict__ flags_all) {
+  extern __shared__ float4 home_pos[];  // [4][cell_cap]
+  __shared__ int q_first_s[4][28], q_start_s[4][27];
+  __shared__ float4 q_off_s[4][27];
+  __shared__ float l2tab_s[4][MFA_NTYPE * MFA_NTYPE];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cg = blockIdx.x * 4 + w;
+  const KParams& k = kb[cell_rep[cg < n_cells ? cg : 0]];
   int* flags = flags_all + k.rep * F_NFLAGS;
-  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the replica
-  extern __shared__ unsigned char sh_raw[];
-  float4* cpos = (float4*)sh_raw;             // [9*cell_cap]
-  int* cid = (int*)(cpos + 9 * k.cell_cap);   // [9*cell_cap]
-  int* tmp = cid + 9 * k.cell_cap;            // [9*cell_cap]
-  __shared__ int off[10];
-  __shared__ int cell_base[9];
-  __shared__ float shift[9][3];
-  __shared__ int own[MFA_BLOCK / LTPP];
-  __shared__ int m_own_s;
-  __shared__ int blk_max;
-  const int c = blockIdx.x - k.c0;  // cell within the replica's grid
-  const int cg = blockIdx.x;         // the same cell in the concatenated arrays
+  float* l2tab = l2tab_s[w];
+  const bool on = cg < n_cells && flags[F_REBUILD] != 0 && flags[F_ERROR] == 0;
+  int* q_first = q_first_s[w];
+  int* q_start = q_start_s[w];
+  float4* q_off = q_off_s[w];
+  float4* hpos = home_pos + w * k.cell_cap;
+  const int c = cg - k.c0;
   const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
-  const float Lf[3] = {(float)k.L[0], (float)k.L[1], (float)k.L[2]};
-  const int lane = threadIdx.x & 63;
-  const int grp = threadIdx.x / LTPP, sub = threadIdx.x % LTPP;
-  const int gbase = lane & ~(LTPP - 1);
-  const unsigned long long below = (1ull << sub) - 1ull;
-  // the own cell's particles, ascending id
-  if (threadIdx.x == 0) {
-    blk_max = 0;
-    m_own_s = min(cell_count[cg], k.cell_cap);
-  }
-  __syncthreads();
-  const int m_own = m_own_s;
-  int my_max = 0;
-  // passes of MFA_BLOCK/LTPP particles (one lane group each)
-  for (int p0 = 0; p0 < m_own; p0 += MFA_BLOCK / LTPP) {
-    const int np = min(MFA_BLOCK / LTPP, m_own - p0);
-    if (threadIdx.x < m_own) tmp[threadIdx.x] = cell_slots[(size_t)cg * k.cell_cap + threadIdx.x];
-    for (int t = threadIdx.x + MFA_BLOCK; t < m_own; t += MFA_BLOCK)
-      tmp[t] = cell_slots[(size_t)cg * k.cell_cap + t];
-    __syncthreads();
-    for (int t = threadIdx.x; t < m_own; t += MFA_BLOCK) {
-      const int id = tmp[t];
-      int rank = 0;
-      for (int u = 0; u < m_own; ++u) rank += tmp[u] < id ? 1 : 0;
-      if (rank >= p0 && rank < p0 + np) own[rank - p0] = id;
+  const int m = on ? min(cell_count[cg], k.cell_cap) : 0;
+  const int hs0 = on ? cell_start[cg] : 0;
+  if (on && lane < 27) {
+    const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+    const int jx = cx + dx, jy = cy + dy, jz = cz + dz;
+    const int cc = k.c0 + (((jz + k.nb[2]) % k.nb[2]) * k.nb[1] + (jy + k.nb[1]) % k.nb[1]) * k.nb[0] +
+                   (jx + k.nb[0]) % k.nb[0];
+    const bool gone = k.symmetry == 0 && (jz < 0 || jz >= k.nb[2]);
+    const int cnt = gone ? 0 : min(cell_count[cc], k.cell_cap);
+    // inclusive prefix over the 27 codes, lanes 0..26 of this wave
+    int x = cnt;
+    for (int o = 1; o < 32; o <<= 1) {
+      const int y = __shfl_up(x, o, 32);
+      if ((lane & 31) >= o) x += y;
     }
-    __syncthreads();
-    // particle state of the groups of this pass
-    const bool active = grp < np;
-    int i = active ? own[grp] : 0;
-    const double xi = r[3 * (size_t)i + 0], yi = r[3 * (size_t)i + 1], zi = r[3 * (size_t)i + 2];
-    const float xf = (float)(xi >= k.L[0] ? xi - k.L[0] : xi);
-    const float yf = (float)(yi >= k.L[1] ? yi - k.L[1] : yi);
-    const float zf = (float)(k.symmetry == 1 && zi >= k.L[2] ? zi - k.L[2] : zi);
-    const int ti = type[i] - 1;
-    float l2f[MFA_NTYPE];
-    for (int t = 0; t < MFA_NTYPE; ++t) l2f[t] = (float)k.list_2[ti][t];
-    unsigned* row = nl + (size_t)i * cap;
-    int cnt = 0;
-    for (int sx = 0; sx < 3; ++sx) {
-      __syncthreads();  // the previous plane is no longer read
-      if (threadIdx.x < 9) {
-        const int q = threadIdx.x;
-        const int jx = cx + sx - 1, jy = cy + q / 3 - 1, jz = cz + q % 3 - 1;
-        const int kx = (jx + k.nb[0]) % k.nb[0];
-        const int ky = (jy + k.nb[1]) % k.nb[1];
-        const int kz = (jz + k.nb[2]) % k.nb[2];
-        const int cc = k.c0 + (kz * k.nb[1] + ky) * k.nb[0] + kx;
-        cell_base[q] = cc;
-        off[q + 1] = (k.symmetry == 0 && (jz < 0 || jz >= k.nb[2])) ? 0
-                                                                    : min(cell_count[cc], k.cell_cap);
-        // periodic image of the neighbour cell as seen from cell c
-        shift[q][0] = jx < 0 ? -Lf[0] : (jx >= k.nb[0] ? Lf[0] : 0.0f);
-        shift[q][1] = jy < 0 ? -Lf[1] : (jy >= k.nb[1] ? Lf[1] : 0.0f);
-        shift[q][2] = k.symmetry == 1 ? (jz < 0 ? -Lf[2] : (jz >= k.nb[2] ? Lf[2] : 0.0f)) : 0.0f;
+    q_first[lane + 1] = x;
+    if (lane == 0) q_first[0] = 0;
+    q_start[lane] = gone ? 0 : cell_start[cc];
+    q_off[lane] = make_float4(-(float)dx * (float)k.r_bin[0], -(float)dy * (float)k.r_bin[1],
+                              -(float)dz * (float)k.r_bin[2], 0.0f);
+  }
+  if (lane >= 32 && lane < 32 + MFA_NTYPE * MFA_NTYPE) {
+    const int q = lane - 32;
+    l2tab[q] = (float)k.list_2[q / MFA_NTYPE][q % MFA_NTYPE];
+  }
+  for (int t = lane; t < m; t += 64) hpos[t] = pos4[hs0 + t];
+  __syncthreads();
+  if (on && c == 0 && lane == 0) atomicAdd(&flags[F_NREBUILD], 1);
+  if (!on || m == 0) return;
+  const int total = q_first[27];
+  const float margin = k.lf_margin;
+  int hc[LB_SLOTS];  // row length of particle 64 s + lane
+  for (int sl = 0; sl < LB_SLOTS; ++sl) hc[sl] = 0;
+  for (int e0 = 0; e0 < total; e0 += 64) {
+    // this lane's candidate: cell code q, sorted position js (a lane past the
+    // end takes the cell's first particle and never hits)
+    const int e = e0 + lane;
+    const bool live = e < total;
+    int q = 0;
+    if (live)
+      while (q_first[q + 1] <= e) ++q;
+    const int js = live ? q_start[q] + (e - q_first[q]) : hs0;
+    const float4 pj = pos4[js];
+    const float4 o = q_off[q];
+    // the candidate relative to this cell's origin: d_ij = p_i - (p_j - o)
+    const float pjx = pj.x - o.x, pjy = pj.y - o.y, pjz = pj.z - o.z;
+    const int tj = __float_as_int(pj.w) & 3;
+    const unsigned entry = (unsigned)js | ((unsigned)q << MFA_JBITS);
+    for (int h = 0; h < m; ++h) {
+      const float4 pi = hpos[h];
+      const int ti = __float_as_int(pi.w) & 3;
+      const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      const float l2 = l2tab[ti * MFA_NTYPE + tj];
+      bool hit = false;
+      if (live && js != hs0 + h) {
+        if (d2 < l2 - margin) {
+          hit = true;
+        } else if (d2 < l2 + margin) {
+          const int i = perm[hs0 + h], j = perm[js];
+          double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+          double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+          double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+          min_image(k, ex, ey, ez);
+          const double r2 = (ex * ex + ey * ey) + ez * ez;
+          hit = r2 < k.list_2[ti][tj];
+        }
       }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        off[0] = 0;
-        for (int q = 0; q < 9; ++q) off[q + 1] += off[q];
-      }
-      __syncthreads();
-      const int total = off[9];
-      for (int e = threadIdx.x; e < total; e += MFA_BLOCK) {
-        int q = 0;
-        while (off[q + 1] <= e) ++q;
-        tmp[e] = cell_slots[(size_t)cell_base[q] * k.cell_cap + (e - off[q])];
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < total; e += MFA_BLOCK) {
-        int q = 0;
-        while (off[q + 1] <= e) ++q;
-        const int id = tmp[e];
-        int rank = 0;
-        for (int u = off[q]; u < off[q + 1]; ++u) rank += tmp[u] < id ? 1 : 0;
-        // a coordinate equal to L belongs to cell 0 (mod), i.e. to the image at 0
-        double px = r[3 * (size_t)id + 0], py = r[3 * (size_t)id + 1], pz = r[3 * (size_t)id + 2];
-        if (px >= k.L[0]) px -= k.L[0];
-        if (py >= k.L[1]) py -= k.L[1];
-        if (k.symmetry == 1 && pz >= k.L[2]) pz -= k.L[2];
-        cpos[off[q] + rank] = make_float4((float)px + shift[q][0], (float)py + shift[q][1],
-                                          (float)pz + shift[q][2], (float)type[id]);
-        cid[off[q] + rank] = id | (q << MFA_JBITS);
-      }
-      __syncthreads();
-      if (active) {
-        for (int q0 = 0; q0 < total; q0 += LTPP) {
-          const int q = q0 + sub;
-          bool hit = false;
-          int j = -1;
-          unsigned entry = 0u;
-          if (q < total) {
-            const unsigned tagged = (unsigned)cid[q];
-            j = (int)(tagged & MFA_JMASK);
-            entry = (unsigned)j | ((unsigned)(9 * sx + (int)(tagged >> MFA_JBITS)) << MFA_JBITS);
-            const float4 pj = cpos[q];
-            const float dx = xf - pj.x, dy = yf - pj.y, dz = zf - pj.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const int tj = (int)pj.w - 1;
-            const float l2 = tj == 0 ? l2f[0] : (tj == 1 ? l2f[1] : (tj == 2 ? l2f[2] : l2f[3]));
-            if (j != i) {
-              if (d2 < l2 - k.pf_margin) {
-                hit = true;  // inside by more than the FP32 error: the FP64 test agrees
-              } else if (d2 < l2 + k.pf_margin) {
-                // within the FP32 error of the radius: the criterion in FP64
-                // with the reference's image convention (binning.f90:84-105)
-                double ex = xi - r[3 * (size_t)j + 0];
-                double ey = yi - r[3 * (size_t)j + 1];
-                double ez = zi - r[3 * (size_t)j + 2];
-                min_image(k, ex, ey, ez);
-                const double r2 = (ex * ex + ey * ey) + ez * ez;
-                hit = r2 < k.list_2[ti][tj];
-              }
-            }
-          }
-          const unsigned long long mask = (__ballot(hit) >> gbase) & ((1ull << LTPP) - 1ull);
-          if (hit) {
-            const int pos = cnt + __popcll(mask & below);
-            if (pos < cap) row[pos] = entry;
-          }
-          cnt += __popcll(mask);
+      const unsigned long long mask = __ballot(hit);
+      if (mask) {
+        const int sl = h >> 6, ln = h & 63;
+        int mine = hc[0];
+        for (int u = 1; u < LB_SLOTS; ++u) mine = sl == u ? hc[u] : mine;
+        const int base = __builtin_amdgcn_readlane(mine, ln);
+        if (hit) {
+          const int pos = base + (int)__popcll(mask & ((1ull << lane) - 1ull));
+          if (pos < cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
+        }
+        if (lane == ln) {
+          const int add = (int)__popcll(mask);
+          for (int u = 0; u < LB_SLOTS; ++u) hc[u] += sl == u ? add : 0;
         }
       }
     }
-    if (active && sub == 0) {
-      nn[i] = cnt < cap ? cnt : cap;
+  }
+  int my_max = 0;
+  for (int sl = 0; sl < LB_SLOTS; ++sl) {
+    const int t = sl * 64 + lane;
+    if (t < m) {
+      const int cnt = hc[sl];
+      nn[hs0 + t] = cnt < cap ? cnt : cap;
       if (cnt > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
-      const double ri[3] = {xi, yi, zi};
-      const double vi[3] = {v[3 * (size_t)i + 0], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]};
-      for (int d = 0; d < 3; ++d)
-        r_ref[3 * (size_t)i + d] = ri[d] + (double)mic[3 * (size_t)i + d] * k.L[d];
-      write_view(k, i, c, ti, ri, vi, pos4, vel4);
+      my_max = max(my_max, cnt);
     }
-    if (active) my_max = max(my_max, cnt);
-    __syncthreads();  // own[] and tmp[] are reused by the next pass
   }
-  atomicMax(&blk_max, my_max);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (blk_max > 0) atomicMax(&flags[F_MAXNEIGH], blk_max);
-    if (c == 0) atomicAdd(&flags[F_NREBUILD], 1);
-  }
+  for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, __shfl_xor(my_max, o, 64));
+  if (lane == 0 && my_max > 0) atomicMax(&flags[F_MAXNEIGH], my_max);
 }
 
 // ---------------------------------------------------------------------------
@@ -908,6 +949,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
                                                        const double* __restrict__ r,
                                                        const double* __restrict__ v,
                                                        const int* __restrict__ type,
+                                                       const int* __restrict__ perm,
                                                        const unsigned* __restrict__ nl,
                                                        const int* __restrict__ nn, int cap,
                                                        double* __restrict__ f,
@@ -921,14 +963,16 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   __syncthreads();
   if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / TPP;   // in the concatenated arrays
-  const int il = i - k.p0;    // within the replica (chains, heads, random keys)
+  const int sp = gtid / TPP;  // sorted position in the concatenated arrays
   const int sub = gtid % TPP;
-  if (il == 0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  const bool active = sp - k.p0 < k.n;
+  const int ss = active ? sp : k.p0;
+  const int i = perm[ss];     // the particle (reference order)
+  const int il = i - k.p0;    // within the replica (chains, heads, random keys)
+  if (sp == k.p0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const double r2min = switch_on_r2min(k, step);
-  const bool active = il < k.n;
-  const int ii = active ? i : k.p0;
+  const int ii = i;
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
   const double vxi = v[3 * (size_t)ii + 0], vyi = v[3 * (size_t)ii + 1], vzi = v[3 * (size_t)ii + 2];
   const int ti = type[ii] - 1;
@@ -937,10 +981,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   double ax = 0.0, ay = 0.0, az = 0.0;
 
   if (active && (mask & TM_PAIR)) {
-    const int cnt = nn[ii];
-    const unsigned* row = nl + (size_t)ii * cap;
+    const int cnt = nn[ss];
+    const unsigned* row = nl + (size_t)ss * cap;
     for (int q = sub; q < cnt; q += TPP) {
-      const int j = (int)(row[q] & MFA_JMASK);
+      const int j = perm[row[q] & MFA_JMASK];
       double dx = xi - r[3 * (size_t)j + 0];
       double dy = yi - r[3 * (size_t)j + 1];
       double dz = zi - r[3 * (size_t)j + 2];
@@ -1109,7 +1153,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
 template <int TPP, int ENERGY, int XT>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
-    const float4* __restrict__ pos4, const float4* __restrict__ vel4,
+    const float4* __restrict__ pos4, const float4* __restrict__ vel4, const int* __restrict__ perm,
     const unsigned* __restrict__ nl, const int* __restrict__ nn, int cap, double* __restrict__ f,
     const long long* __restrict__ d_step, int mask, double* __restrict__ part,
     int* __restrict__ flags_all) {
@@ -1137,19 +1181,20 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   __syncthreads();
   if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / TPP;   // in the concatenated arrays
-  const int il = i - k.p0;    // within the replica (chains, heads, random keys)
+  const int sp = gtid / TPP;  // sorted position in the concatenated arrays
   const int sub = gtid % TPP;
-  if (il == 0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
+  const bool active = sp - k.p0 < k.n;
+  const int ss = active ? sp : k.p0;
+  const int i = perm[ss];     // the particle (reference order)
+  const int il = i - k.p0;    // within the replica (chains, heads, random keys)
+  if (sp == k.p0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const float r2min = (float)switch_on_r2min(k, step);
   const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd2 = (float)(1.0 / k.r_cut_dpd_2);
   const float sigf = (float)k.sig, gam = (float)k.friction1, margin = k.ff_margin;
-  const bool active = il < k.n;
-  const int ii = active ? i : k.p0;
-  const float4 pi4 = pos4[ii];
-  const float4 vi4 = vel4[ii];
-  const int ti = __float_as_int(pi4.w);
+  const float4 pi4 = pos4[ss];
+  const float4 vi4 = vel4[ss];
+  const int ti = __float_as_int(pi4.w) & 3;
   double e_ff = 0.0, vir = 0.0, e_fw = 0.0, e_bond = 0.0, fzb = 0.0, fzt = 0.0, zi = 0.0;
   double e_coul = 0.0, e_bend = 0.0;
   const float rc2f = (float)k.rc_coul_2, sa = (float)k.sqrt_alpha, alf = (float)k.alpha_c;
@@ -1158,18 +1203,19 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   bool bad = false;
 
   if (active && (mask & TM_PAIR)) {
-    const int cnt = nn[ii];
-    const unsigned* row = nl + (size_t)ii * cap;
+    const int cnt = nn[ss];
+    const unsigned* row = nl + (size_t)ss * cap;
     for (int q = sub; q < cnt; q += TPP) {
       const unsigned e = row[q];
-      const int j = (int)(e & MFA_JMASK);
+      const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
       const float4 o = s_off[e >> MFA_JBITS];
-      const float4 pj = pos4[j];
+      const float4 pj = pos4[js];
       const float dx = (pi4.x - pj.x) + o.x;
       const float dy = (pi4.y - pj.y) + o.y;
       const float dz = (pi4.z - pj.z) + o.z;
       const float r2raw = dx * dx + dy * dy + dz * dz;
-      const int tj = __float_as_int(pj.w);
+      const int wj = __float_as_int(pj.w);
+      const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
       const float4 tb = s_tab[ti * MFA_NTYPE + tj];
       bool in_lj = r2raw < tb.x;
       bool in_dpd = k.dpd_cut_off_on ? r2raw < rdpd2 : in_lj;
@@ -1179,6 +1225,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       if (XT && k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
       if (near) {
         // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
+        const int j = k.p0 + jl;
         double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
         double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
         double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
@@ -1209,10 +1256,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         }
       }
       if (dpd) {
-        const float xi_n = pair_normal_f32(k.seed, step, il, j - k.p0);
+        const float xi_n = pair_normal_f32(k.seed, step, il, jl);
         const float w_arg = k.dpd_cut_off_on ? r2 * inv_rdpd2 : r2 * s_inv_range_2[ti * MFA_NTYPE + tj];
         const float wr = 1.0f - __builtin_amdgcn_sqrtf(w_arg);  // v_sqrt_f32, ~1 ulp
-        const float4 vj = vel4[j];
+        const float4 vj = vel4[js];
         const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
         s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
       }
@@ -1388,7 +1435,9 @@ __global__ __launch_bounds__(64) void k_ewald_rho(EwaldK e, const double* __rest
 
 // rho(k) = sum of the wave partials (fixed order), stored weighted by
 // g_k = exp(-k^2/(4 alpha_c))/k^2; per-block sums of g_k |rho(k)|^2 for the
-// energy; block 0 also sums the dipole moment.
+// energy; block 0 also sums the dipole moment.  A block of 4 waves covers 64 k
+// vectors: wave w adds the partials pw = w, w+4, ..., then wave 0 combines
+// the 4 in a fixed order.
 __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_sum(EwaldK e,
                                                          const double2* __restrict__ rho_part,
                                                          const double4* __restrict__ kvec,
@@ -1396,76 +1445,89 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_sum(EwaldK e,
                                                          double* __restrict__ uk_part,
                                                          const double* __restrict__ mz_part,
                                                          double* __restrict__ mz) {
-  __shared__ double sh[MFA_BLOCK];
-  const int kk = blockIdx.x * MFA_BLOCK + threadIdx.x;
-  double u = 0.0;
-  if (kk < e.nk) {
-    double re = 0.0, im = 0.0;
-    for (int pw = 0; pw < e.n_pw; ++pw) {
+  __shared__ double2 sh[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int kk = blockIdx.x * 64 + lane;
+  double re = 0.0, im = 0.0;
+  if (kk < e.nk)
+    for (int pw = w; pw < e.n_pw; pw += 4) {
       const double2 t = rho_part[(size_t)pw * e.nk + kk];
       re += t.x;
       im += t.y;
     }
-    const double g = kvec[kk].w;
-    grho[kk] = make_double2(g * re, g * im);
-    u = g * (re * re + im * im);
-  }
-  sh[threadIdx.x] = u;
+  sh[w][lane] = make_double2(re, im);
   __syncthreads();
-  for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    uk_part[blockIdx.x] = sh[0];
-    if (blockIdx.x == 0) {
-      double m = 0.0;
-      for (int pw = 0; pw < e.n_pw; ++pw) m += mz_part[pw];
-      *mz = m;
+  if (w == 0) {
+    double u = 0.0;
+    if (kk < e.nk) {
+      re = (sh[0][lane].x + sh[1][lane].x) + (sh[2][lane].x + sh[3][lane].x);
+      im = (sh[0][lane].y + sh[1][lane].y) + (sh[2][lane].y + sh[3][lane].y);
+      const double g = kvec[kk].w;
+      grho[kk] = make_double2(g * re, g * im);
+      u = g * (re * re + im * im);
+    }
+    for (int o = 32; o > 0; o >>= 1) u += __shfl_down(u, o, 64);
+    if (lane == 0) {
+      uk_part[blockIdx.x] = u;
+      if (blockIdx.x == 0) {
+        double m = 0.0;
+        for (int pw = 0; pw < e.n_pw; ++pw) m += mz_part[pw];
+        *mz = m;
+      }
     }
   }
 }
 
-// Sum over one group of runs of k g_k (cos(k.r) Im rho - sin(k.r) Re rho) for
-// each charged particle (ewald_k.f90:177-192 without the prefactor).
-__global__ __launch_bounds__(64) void k_ewald_force(EwaldK e, const double* __restrict__ r,
-                                                    const int* __restrict__ qidx,
-                                                    const int4* __restrict__ runs,
-                                                    const int* __restrict__ run_k0,
-                                                    const double2* __restrict__ grho,
-                                                    double* __restrict__ fk_part) {
-  const int idx = blockIdx.x * 64 + threadIdx.x, kg = blockIdx.y;
-  if (idx >= e.nq) return;
-  const int i = qidx[idx];
-  const double x = r[3 * (size_t)i + 0], y = r[3 * (size_t)i + 1], z = r[3 * (size_t)i + 2];
-  double sx, cx;
-  sincos(e.dk[0] * x, &sx, &cx);
+// Sum over groups of runs of k g_k (cos(k.r) Im rho - sin(k.r) Re rho) for
+// each charged particle (ewald_k.f90:177-192 without the prefactor).  A block
+// holds 64 particles and 4 waves; wave w takes run group 4 blockIdx.y + w, and
+// the 4 partial sums are combined in a fixed order into one partial per block.
+__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const double* __restrict__ r,
+                                                           const int* __restrict__ qidx,
+                                                           const int4* __restrict__ runs,
+                                                           const int* __restrict__ run_k0,
+                                                           const double2* __restrict__ grho,
+                                                           double* __restrict__ fk_part) {
+  __shared__ double sh[4][64][3];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 64 + lane, kg = blockIdx.y * 4 + w;
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  const int r0 = kg * e.runs_per_kgroup, r1 = min(e.nruns, r0 + e.runs_per_kgroup);
-  for (int run = r0; run < r1; ++run) {
-    const int4 rn = runs[run];
-    const double kx0 = e.dk[0] * (double)rn.z, ky = e.dk[1] * (double)rn.x,
-                 kz = e.dk[2] * (double)rn.y;
-    double s, c;
-    sincos(kx0 * x + ky * y + kz * z, &s, &c);
-    int kidx = run_k0[run];
-    double ty = 0.0;  // sum of t over the run (ky, kz are constant along it)
-    for (int nx = rn.z; nx <= rn.w; ++nx, ++kidx) {
-      const double2 g = grho[kidx];
-      const double t = c * g.y - s * g.x;
-      fx += (e.dk[0] * (double)nx) * t;
-      ty += t;
-      const double cn = c * cx - s * sx;
-      s = s * cx + c * sx;
-      c = cn;
+  if (idx < e.nq) {
+    const int i = qidx[idx];
+    const double x = r[3 * (size_t)i + 0], y = r[3 * (size_t)i + 1], z = r[3 * (size_t)i + 2];
+    double sx, cx;
+    sincos(e.dk[0] * x, &sx, &cx);
+    const int r0 = kg * e.runs_per_kgroup, r1 = min(e.nruns, r0 + e.runs_per_kgroup);
+    for (int run = r0; run < r1; ++run) {
+      const int4 rn = runs[run];
+      const double kx0 = e.dk[0] * (double)rn.z, ky = e.dk[1] * (double)rn.x,
+                   kz = e.dk[2] * (double)rn.y;
+      double s, c;
+      sincos(kx0 * x + ky * y + kz * z, &s, &c);
+      int kidx = run_k0[run];
+      double ty = 0.0;  // sum of t over the run (ky, kz are constant along it)
+      for (int nx = rn.z; nx <= rn.w; ++nx, ++kidx) {
+        const double2 g = grho[kidx];
+        const double t = c * g.y - s * g.x;
+        fx += (e.dk[0] * (double)nx) * t;
+        ty += t;
+        const double cn = c * cx - s * sx;
+        s = s * cx + c * sx;
+        c = cn;
+      }
+      fy += ky * ty;
+      fz += kz * ty;
     }
-    fy += ky * ty;
-    fz += kz * ty;
   }
-  double* o = fk_part + 3 * ((size_t)kg * e.nq + idx);
-  o[0] = fx;
-  o[1] = fy;
-  o[2] = fz;
+  sh[w][lane][0] = fx;
+  sh[w][lane][1] = fy;
+  sh[w][lane][2] = fz;
+  __syncthreads();
+  if (w == 0 && idx < e.nq) {
+    double* o = fk_part + 3 * ((size_t)blockIdx.y * e.nq + idx);
+    for (int d = 0; d < 3; ++d)
+      o[d] = (sh[0][lane][d] + sh[1][lane][d]) + (sh[2][lane][d] + sh[3][lane][d]);
+  }
 }
 
 // Dipole moment of the charged particles, one block (the per-subroutine
@@ -1600,6 +1662,8 @@ static KParams make_kparams(const mfa_params& p) {
       for (int q = 0; q < MFA_NTYPE; ++q) c2max = p.range_2[t][q] > c2max ? p.range_2[t][q] : c2max;
     k.ff_margin = (float)(4.0 * (2.0 * sqrt(3.0 * c2max) * dsep + 3.0 * dsep * dsep +
                                  6.0 * c2max * u));
+    k.lf_margin = (float)(4.0 * (2.0 * sqrt(3.0 * l2max) * dsep + 3.0 * dsep * dsep +
+                                 6.0 * l2max * u));
   }
   k.n_cells = p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
   k.n = p.n_part;
@@ -1624,39 +1688,45 @@ static KParams make_kparams(const mfa_params& p) {
 static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
 static int launch_ewald(mfa_ctx* c, int add_dip);
 
-static size_t list_lds_bytes(int cell_cap) {
-  return (size_t)9 * cell_cap * (sizeof(float4) + 2 * sizeof(int));
-}
+static size_t list_lds_bytes(int cell_cap) { return (size_t)4 * cell_cap * sizeof(float4); }
 
+// A rebuild: cells of every particle, their sorted ranges, the spatial order
+// and FP32 view, then the lists.  Each kernel returns at once unless the
+// replica's rebuild flag is set.
 static int launch_rebuild(mfa_ctx* c) {
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
                      c->cell_of, c->cell_count, c->cell_slots, c->flags);
-  hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c->cell_cap),
-                     c->stream, c->d_kp, c->d_cell_rep, c->r, c->v, c->type, c->mic, c->cell_count,
-                     c->cell_slots, c->nl, c->nn, c->cap, c->r_ref, c->flags, c->pos4, c->vel4);
+  hipLaunchKernelGGL(k_cell_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_kp,
+                     c->cell_count, c->cell_start, c->flags);
+  const int cb = (c->n_cells + 3) / 4;  // four cells (waves) per workgroup
+  hipLaunchKernelGGL(k_cell_sort, dim3(cb), dim3(MFA_BLOCK), (size_t)4 * c->cell_cap * sizeof(int),
+                     c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->v, c->type, c->mic,
+                     c->cell_count, c->cell_slots, c->cell_start, c->perm, c->sidx, c->r_ref,
+                     c->flags, c->pos4, c->vel4);
+  hipLaunchKernelGGL(k_build_list, dim3(cb), dim3(MFA_BLOCK), list_lds_bytes(c->cell_cap), c->stream,
+                     c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->cell_count, c->cell_start,
+                     c->perm, c->pos4, c->nl, c->nn, c->cap, c->flags);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
 
-// XT: the Ewald real-space and bending terms are compiled in (charged or
-// stiff systems only; without them the pair loop needs far fewer registers).
 template <int TPP, int XT>
 static void launch_force_tx(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED) {
     if (mask & TM_ENERGY)
       hipLaunchKernelGGL((k_force_mixed<TPP, 1, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
-                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f,
+                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags);
     else
       hipLaunchKernelGGL((k_force_mixed<TPP, 0, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
-                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->nl, c->nn, c->cap, c->f,
+                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags);
   } else {
     hipLaunchKernelGGL((k_force64<TPP, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
-                       c->d_rmap, c->r, c->v, c->type, c->nl, c->nn, c->cap, c->f, c->d_step, mask,
-                       c->part, c->flags);
+                       c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
+                       mask, c->part, c->flags);
   }
 }
 
@@ -1733,8 +1803,8 @@ static int launch_fused(mfa_ctx* c, int energy) {
   }
   hipLaunchKernelGGL(k_fused_integrate, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream,
                      c->d_kp, c->d_rmap, c->f, c->v, c->r, c->mic, c->r_ref, c->type, c->flags,
-                     c->part, energy, c->n_cells, c->cell_count, c->d_step, c->cell_of, c->pos4,
-                     c->vel4);
+                     c->part, energy, c->n_cells, c->cell_count, c->d_step, c->cell_of, c->sidx,
+                     c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (timed) return timed_end(c, c->ev_integ, c->ev_integ_used);
   return MFA_OK;
@@ -1922,7 +1992,7 @@ static int ewald_setup(mfa_ctx* c) {
   e.nk = (int)kv.size();
   e.nruns = (int)runs.size();
   c->ew_inv_V = p.inv_boundary[0] * p.inv_boundary[1] / (p.boundary[2] * (1. + z_slab));
-  c->ew_sum_blocks = (e.nk + MFA_BLOCK - 1) / MFA_BLOCK;
+  c->ew_sum_blocks = (e.nk + 63) / 64;
   const size_t n = (size_t)c->n;
   const size_t n_pw_max = (n + 64 * EW_QPT - 1) / (64 * EW_QPT);
   const size_t nk = (size_t)(e.nk > 0 ? e.nk : 1), nr = (size_t)(e.nruns > 0 ? e.nruns : 1);
@@ -1969,16 +2039,24 @@ static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
   if (e.nq == 0 || e.nruns == 0) return MFA_OK;
   HIPCHK(c, hipMemcpy(c->d_qidx, qi.data(), qi.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_qval, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice));
+  // Launch shapes: about 8192 waves for each of the structure factor and the
+  // forces (1024 SIMDs; the sincos and recurrence chains need several waves
+  // per SIMD to hide their latency).  The force pass's run groups come in
+  // fours (one block of 4 waves), padded with empty groups; its partial sums
+  // (one per block row) fit the buffer ewald_setup sized for 2048 rows of 64.
   e.n_pw = (e.nq + 64 * EW_QPT - 1) / (64 * EW_QPT);
-  int ng = 2048 / e.n_pw;
+  int ng = 8192 / e.n_pw;
   ng = ng < 1 ? 1 : (ng > e.nruns ? e.nruns : ng);
   e.runs_per_rgroup = (e.nruns + ng - 1) / ng;
   c->ew_n_rgroups = (e.nruns + e.runs_per_rgroup - 1) / e.runs_per_rgroup;
   const int qb = (e.nq + 63) / 64;
-  int kg = 2048 / qb;
-  kg = kg < 1 ? 1 : (kg > e.nruns ? e.nruns : kg);
+  int kg = 8192 / qb;
+  kg = kg < 4 ? 4 : (kg > e.nruns ? e.nruns : kg);
   e.runs_per_kgroup = (e.nruns + kg - 1) / kg;
-  c->ew_n_kgroups = (e.nruns + e.runs_per_kgroup - 1) / e.runs_per_kgroup;
+  kg = (e.nruns + e.runs_per_kgroup - 1) / e.runs_per_kgroup;
+  c->ew_n_kgroups = (kg + 3) / 4 * 4;
+  if ((size_t)(c->ew_n_kgroups / 4) * (size_t)e.nq > (size_t)2048 * 64 + (size_t)c->n)
+    return set_err(c, MFA_ERR_ARG, "Ewald: too many force partials");
   return MFA_OK;
 }
 
@@ -1992,10 +2070,10 @@ static int launch_ewald(mfa_ctx* c, int add_dip) {
                      c->d_qidx, c->d_qval, c->d_runs, c->d_run_k0, c->d_rho_part, c->d_mz_part);
   hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(MFA_BLOCK), 0, c->stream, e,
                      c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
-  hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups), dim3(64), 0, c->stream,
-                     e, c->r, c->d_qidx, c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
+  hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups / 4), dim3(MFA_BLOCK), 0,
+                     c->stream, e, c->r, c->d_qidx, c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
   hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups / 4, c->d_mz,
                      c->ew_inv_V, 1, add_dip, c->f);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
@@ -2099,17 +2177,12 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->n = (int)ntot;
   c->n_cells = (int)ctot;
   {
-    // LDS of the list build: the 27-cell neighbourhood as (float4, id)
+    // LDS of the list build: the cell's own particles as (float4, row length)
     const size_t lds = list_lds_bytes(c->cell_cap);
-    if (lds > 150 * 1024) {
+    if (lds > 64 * 1024 || c->cell_cap > 64 * LB_SLOTS) {
       delete c;
       return set_err(nullptr, MFA_ERR_ARG, "binning cells too crowded for the list build (capacity %d)",
                      cell_cap);
-    }
-    if (hipFuncSetAttribute((const void*)k_build_list, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess) {
-      delete c;
-      return set_err(nullptr, MFA_ERR_HIP, "cannot reserve %zu bytes of LDS", lds);
     }
   }
   const size_t n = (size_t)c->n;
@@ -2133,6 +2206,9 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   ALLOC(c->cell_of, n * sizeof(int32_t));
   ALLOC(c->cell_count, (size_t)c->n_cells * sizeof(int32_t));
   ALLOC(c->cell_slots, ncs * sizeof(int32_t));
+  ALLOC(c->cell_start, ((size_t)c->n_cells + 1) * sizeof(int32_t));
+  ALLOC(c->perm, n * sizeof(int32_t));
+  ALLOC(c->sidx, n * sizeof(int32_t));
   ALLOC(c->nl, n * (size_t)c->cap * sizeof(unsigned));
   ALLOC(c->nn, n * sizeof(int32_t));
   ALLOC(c->pos4, n * sizeof(float4));
@@ -2197,6 +2273,14 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   hipMemsetAsync(c->a, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->nn, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->cell_of, 0, n * sizeof(int32_t), c->stream);
+  {
+    // identity order until the first rebuild (padding keeps it for good)
+    std::vector<int32_t> id(n);
+    for (size_t q = 0; q < n; ++q) id[q] = (int32_t)q;
+    hipMemcpyAsync(c->perm, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(c->sidx, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+  }
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     mfa_destroy(c);
     return set_err(nullptr, MFA_ERR_HIP, "initial clears failed");
@@ -2222,7 +2306,7 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   void* ptrs[] = {c->r,   c->v,       c->a,          c->f,          c->r_ref, c->type,
                   c->mic, c->cell_of, c->cell_count, c->cell_slots, c->nl,    c->nn,
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
-                  c->d_kp, c->d_rmap, c->d_cell_rep};
+                  c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx};
   for (void* q : ptrs)
     if (q) hipFree(q);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
@@ -2343,7 +2427,7 @@ extern "C" int32_t mfa_verlet_positions(mfa_ctx* c) {
   if (!c) return MFA_ERR_ARG;
   hipLaunchKernelGGL(k_verlet_positions, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream,
                      c->d_kp, c->d_rmap, c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 0, c->n_cells,
-                     c->cell_count, c->type, c->cell_of, c->pos4, c->vel4);
+                     c->cell_count, c->type, c->cell_of, c->sidx, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFA_OK;
@@ -2397,15 +2481,17 @@ extern "C" int32_t mfa_get_pairs(mfa_ctx* c, int32_t* pairs, int64_t cap, int64_
   if (!c || !n_pairs) return MFA_ERR_ARG;
   int rc = ensure_list(c);
   if (rc) return rc;
-  const int n0 = c->ks[0].n;  // replica 0
-  std::vector<int32_t> nn((size_t)n0);
+  const int n0 = c->ks[0].n;  // replica 0 (sorted positions [0, n0))
+  std::vector<int32_t> nn((size_t)n0), perm((size_t)n0);
   std::vector<unsigned> nl((size_t)n0 * c->cap);
   HIPCHK(c, hipMemcpy(nn.data(), c->nn, nn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(nl.data(), c->nl, nl.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(perm.data(), c->perm, perm.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   int64_t m = 0;
-  for (int i = 0; i < n0; ++i)
-    for (int q = 0; q < nn[(size_t)i]; ++q) {
-      const int j = (int)(nl[(size_t)i * c->cap + q] & MFA_JMASK);
+  for (int sp = 0; sp < n0; ++sp)
+    for (int q = 0; q < nn[(size_t)sp]; ++q) {
+      const int i = perm[(size_t)sp];
+      const int j = perm[(size_t)(nl[(size_t)sp * c->cap + q] & MFA_JMASK)];
       if (i < j) {
         if (pairs && m < cap) {
           pairs[2 * m + 0] = i + 1;
@@ -2491,7 +2577,7 @@ extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
   if (!c) return MFA_ERR_ARG;
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, t_kin ? 1 : 0, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, t_kin ? 1 : 0, c->sidx, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (t_kin) {
     double t[MFA_NTERMS];
@@ -2555,7 +2641,7 @@ extern "C" int32_t mfa_dipolar_correction(mfa_ctx* c, double* v_coul) {
   hipLaunchKernelGGL(k_ewald_mz, dim3(1), dim3(MFA_BLOCK), 0, c->stream, e, c->r, c->d_qidx,
                      c->d_qval, c->d_mz);
   hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups / 4, c->d_mz,
                      c->ew_inv_V, 0, 1, c->f);
   HIPCHK(c, hipGetLastError());
   double uk, udip;
@@ -2609,7 +2695,7 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   // first half of the first step (verlet_positions + check_skin)
   hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
                      c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 1, c->n_cells, c->cell_count,
-                     c->type, c->cell_of, c->pos4, c->vel4);
+                     c->type, c->cell_of, c->sidx, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   // steps 1 .. n-1: captured blocks of graph_steps steps, then single steps
   int middle = n - 1;
@@ -2644,7 +2730,7 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, obs ? 1 : 0, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, obs ? 1 : 0, c->sidx, c->vel4);
   HIPCHK(c, hipGetLastError());
   if (obs) {
     hipLaunchKernelGGL(k_reduce_replicas, dim3(MFA_NTERMS, c->n_rep), dim3(MFA_BLOCK), 0, c->stream,
@@ -2941,17 +3027,19 @@ template <int TPP>
 __global__ __launch_bounds__(MFA_BLOCK) void k_adj_force(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, AdjK ak,
     const double* __restrict__ r, const double* __restrict__ u, const int* __restrict__ type,
-    const unsigned* __restrict__ nl, const int* __restrict__ nn, int cap,
-    const double* __restrict__ lf, double* __restrict__ lr, double* __restrict__ lu,
+    const int* __restrict__ perm, const unsigned* __restrict__ nl, const int* __restrict__ nn,
+    int cap, const double* __restrict__ lf, double* __restrict__ lr, double* __restrict__ lu,
     const long long* __restrict__ d_step, double* __restrict__ part) {
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / TPP;
-  const int il = i - k.p0;
+  const int sp = gtid / TPP;  // sorted position
   const int sub = gtid % TPP;
   const int64_t step = *d_step;
-  const bool active = il < k.n;
-  const int ii = active ? i : k.p0;
+  const bool active = sp - k.p0 < k.n;
+  const int ss = active ? sp : k.p0;
+  const int i = perm[ss];
+  const int il = i - k.p0;
+  const int ii = i;
   const double xi = r[3 * (size_t)ii + 0], yi = r[3 * (size_t)ii + 1], zi = r[3 * (size_t)ii + 2];
   const double uxi = u[3 * (size_t)ii + 0], uyi = u[3 * (size_t)ii + 1], uzi = u[3 * (size_t)ii + 2];
   const int ti = type[ii] - 1;
@@ -2962,10 +3050,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_adj_force(
   for (int c = 0; c < ADJ_NO * ADJ_NP; ++c) gp[c] = 0.0;
 
   if (active) {
-    const int cnt = nn[ii];
-    const unsigned* row = nl + (size_t)ii * cap;
+    const int cnt = nn[ss];
+    const unsigned* row = nl + (size_t)ss * cap;
     for (int q = sub; q < cnt; q += TPP) {
-      const int j = (int)(row[q] & MFA_JMASK);
+      const int j = perm[row[q] & MFA_JMASK];
       double d[3] = {xi - r[3 * (size_t)j + 0], yi - r[3 * (size_t)j + 1], zi - r[3 * (size_t)j + 2]};
       min_image(k, d[0], d[1], d[2]);
       const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
@@ -3100,7 +3188,7 @@ struct AdjBufs {
   // the neighbour list in effect at each checkpoint (rows, counts, skin
   // reference, cells), so a recomputed segment repeats the forward bit for bit
   unsigned* ck_nl = nullptr;
-  int *ck_nn = nullptr, *ck_cell = nullptr;
+  int *ck_nn = nullptr, *ck_cell = nullptr, *ck_perm = nullptr, *ck_sidx = nullptr;
   double* ck_rref = nullptr;
   double *sg_r = nullptr, *sg_u = nullptr;  // recomputed segment [K][N][3]
   int* sg_mic = nullptr;
@@ -3110,7 +3198,7 @@ struct AdjBufs {
   double* obs = nullptr;   // [W][MFA_NTERMS]
   double* grad = nullptr;  // [W][2][MFA_NTERMS]
   ~AdjBufs() {
-    void* ptrs[] = {ck_r, ck_v, ck_a, ck_mic, ck_nl, ck_nn, ck_cell, ck_rref, sg_r, sg_u, sg_mic,
+    void* ptrs[] = {ck_r, ck_v, ck_a, ck_mic, ck_nl, ck_nn, ck_cell, ck_perm, ck_sidx, ck_rref, sg_r, sg_u, sg_mic,
                     end_r, end_v, end_a, end_f, end_mic, lr, lv, la, lf, obs, grad};
     for (void* q : ptrs)
       if (q) hipFree(q);
@@ -3120,8 +3208,8 @@ struct AdjBufs {
 template <int TPP>
 static void launch_adj_force_t(mfa_ctx* c, const AdjK& ak, const AdjBufs& b) {
   hipLaunchKernelGGL((k_adj_force<TPP>), dim3(grid_of(c->n * TPP)), dim3(MFA_BLOCK), 0, c->stream,
-                     c->d_kp, c->d_rmap, ak, c->r, c->v, c->type, c->nl, c->nn, c->cap, b.lf, b.lr,
-                     b.lv, c->d_step, c->part);
+                     c->d_kp, c->d_rmap, ak, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap,
+                     b.lf, b.lr, b.lv, c->d_step, c->part);
 }
 
 static int launch_adj_force(mfa_ctx* c, const AdjK& ak, const AdjBufs& b) {
@@ -3154,7 +3242,7 @@ static int adj_forward_step(mfa_ctx* c, int64_t step, double* obs_row, double* s
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
                      c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 1, c->n_cells, c->cell_count,
-                     c->type, c->cell_of, c->pos4, c->vel4);
+                     c->type, c->cell_of, c->sidx, c->pos4, c->vel4);
   HIPCHK(c, hipGetLastError());
   rc = launch_rebuild(c);
   if (rc) return rc;
@@ -3173,7 +3261,7 @@ static int adj_forward_step(mfa_ctx* c, int64_t step, double* obs_row, double* s
     HIPCHK(c, hipGetLastError());
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, 0, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, 0, c->sidx, c->vel4);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -3227,6 +3315,8 @@ extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_
     AALLOC(b.ck_nl, (size_t)nseg * n * c->cap * sizeof(unsigned));
     AALLOC(b.ck_nn, (size_t)nseg * n * sizeof(int));
     AALLOC(b.ck_cell, (size_t)nseg * n * sizeof(int));
+    AALLOC(b.ck_perm, (size_t)nseg * n * sizeof(int));
+    AALLOC(b.ck_sidx, (size_t)nseg * n * sizeof(int));
     AALLOC(b.ck_rref, (size_t)nseg * n3 * sizeof(double));
     AALLOC(b.sg_r, (size_t)K * n3 * sizeof(double));
     AALLOC(b.sg_u, (size_t)K * n3 * sizeof(double));
@@ -3261,6 +3351,8 @@ extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_
         if ((rc = d2d(c, b.ck_nl + sg * n * c->cap, c->nl, n * c->cap * sizeof(unsigned))) ||
             (rc = d2d(c, b.ck_nn + sg * n, c->nn, n * sizeof(int))) ||
             (rc = d2d(c, b.ck_cell + sg * n, c->cell_of, n * sizeof(int))) ||
+            (rc = d2d(c, b.ck_perm + sg * n, c->perm, n * sizeof(int))) ||
+            (rc = d2d(c, b.ck_sidx + sg * n, c->sidx, n * sizeof(int))) ||
             (rc = d2d(c, b.ck_rref + off, c->r_ref, n3 * sizeof(double))))
           return rc;
       }
@@ -3319,6 +3411,8 @@ extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_
         (rc = d2d(c, c->nl, b.ck_nl + (size_t)seg * n * c->cap, n * c->cap * sizeof(unsigned))) ||
         (rc = d2d(c, c->nn, b.ck_nn + (size_t)seg * n, n * sizeof(int))) ||
         (rc = d2d(c, c->cell_of, b.ck_cell + (size_t)seg * n, n * sizeof(int))) ||
+        (rc = d2d(c, c->perm, b.ck_perm + (size_t)seg * n, n * sizeof(int))) ||
+        (rc = d2d(c, c->sidx, b.ck_sidx + (size_t)seg * n, n * sizeof(int))) ||
         (rc = d2d(c, c->r_ref, b.ck_rref + off, n3 * sizeof(double))))
       return rc;
     for (int s = s0 + 1; s <= s1; ++s) {
