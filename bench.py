@@ -40,7 +40,7 @@ STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
 # HBM bytes per force pass of the mixed form on this workload, from the
 # FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
 # each scaled by the same counter's reading for a copy of known size.
-FORCE_TRAFFIC_BYTES = 79.433e6 + 2.443e6
+FORCE_TRAFFIC_BYTES = 77.574e6 + 3.416e6
 
 
 def log(*a):
@@ -55,6 +55,9 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     ctx = lib.Context(p, precision)
     ctx.set_graph_steps(graph_steps)
     ctx.set_state(st0["r"], st0["v"], st0["a"], st0["t"], st0["mic"])
+    # untimed: the autotuner picks the threads per particle of the force pass
+    tpp, _ = ctx.autotune(first_step, 20)
+    first_step += 5 * (max(graph_steps, 0) + 1 + 20)
     ctx.run_steps(first_step, warmup, want_obs=False)
     t0 = time.perf_counter()
     obs = ctx.run_steps(first_step + warmup, steps, want_obs=True)
@@ -62,7 +65,7 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     force_ms = ctx.time_force(200)
     st = ctx.get_state()
     ctx.close()
-    return dt, obs, {"force_ms": force_ms}, st
+    return dt, obs, {"force_ms": force_ms, "tpp": tpp}, st
 
 
 def force_error_vs_fp64(p, st):
@@ -142,11 +145,13 @@ def bench_single(args, name):
         name, n, [round(x, 4) for x in p.boundary], list(p.n_bin), time.perf_counter() - t0))
     dt_m, obs_m, tm_m, st_m = run_form(p, st0, lib.MFA_MIXED, args.steps, args.warmup,
                                        graph_steps=args.graph_steps)
-    log("mixed: %.4f s for %d steps, force pass %.4f ms avg, rebuilds %d, T=%.3f" % (
-        dt_m, args.steps, tm_m["force_ms"], obs_m["n_rebuilds"], 2.0 * obs_m["t_kin"] / (3.0 * n)))
+    log("mixed: %.4f s for %d steps, force pass %.4f ms avg (%d threads per particle), "
+        "rebuilds %d, T=%.3f" % (dt_m, args.steps, tm_m["force_ms"], tm_m["tpp"],
+                                 obs_m["n_rebuilds"], 2.0 * obs_m["t_kin"] / (3.0 * n)))
     dt_d, obs_d, tm_d, _ = run_form(p, st0, lib.MFA_FP64, args.steps, args.warmup,
                                     graph_steps=args.graph_steps)
-    log("fp64 : %.4f s for %d steps, force %.4f ms avg" % (dt_d, args.steps, tm_d["force_ms"]))
+    log("fp64 : %.4f s for %d steps, force %.4f ms avg (%d threads per particle)" % (
+        dt_d, args.steps, tm_d["force_ms"], tm_d["tpp"]))
     st_m["t"] = t
     err_max, err_rms = force_error_vs_fp64(p, st_m)
     log("mixed vs fp64 force error: max %.3e rms %.3e" % (err_max, err_rms))
@@ -176,8 +181,10 @@ def bench_single(args, name):
                 "velocities, relaxed by the 40-tau switch-on ramp before timing)")
         traffic = None
     xt = 1 if (p.coulomb or p.bending) else 0
-    roof = roofline(n, tm_m["force_ms"], traffic, kernel="k_force_mixed<8,0,%d>" % xt)
+    roof = roofline(n, tm_m["force_ms"], traffic,
+                    kernel="k_force_mixed<%d,0,%d>" % (tm_m["tpp"], xt))
     roof["force_ms_avg_fp64"] = tm_d["force_ms"]
+    config["threads_per_particle"] = {"mixed": tm_m["tpp"], "fp64": tm_d["tpp"]}
     extra = {
         "fp64_value": value_fp64,
         "mixed_over_fp64": value / value_fp64,

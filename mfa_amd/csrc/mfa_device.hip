@@ -34,6 +34,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -50,7 +51,7 @@ enum {
 enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16, TM_BEND = 32 };
 #define TM_ALL (TM_PAIR | TM_WALL | TM_BOND | TM_BEND)
 // device flag slots
-enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_NFLAGS = 8 };
+enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_FILLDONE = 4, F_NFLAGS = 8 };
 
 // Fixed-point scale of the mixed form: 2^22 per unit force (resolution
 // 2.4e-7; a pair term converts through int32 up to |F| < 512, through int64
@@ -98,9 +99,12 @@ struct KParams {
 struct EwaldK {
   double dk[3];       // delta_k (ewald_k.f90:44-52)
   double lz_half;     // init_config.f90:246
-  int nk, nruns, nq;
-  int n_pw, runs_per_rgroup;   // structure factor: particle waves x run groups
-  int runs_per_kgroup;         // forces: run groups
+  int nk, nsegs, nq;
+  int nxm, nym, nzm;  // largest |nx|, |ny|, nz of the k grid
+  int nex, ney, nez;  // phase-table entries per particle and axis
+  int n_chunks;       // structure-factor chunks of EW_CH charged particles
+  int n_mz;           // dipole-moment partials (table blocks)
+  int segs_per_kgroup;  // forces: segments per block row
 };
 
 struct mfa_ctx {
@@ -136,8 +140,9 @@ struct mfa_ctx {
   double ew_inv_V;
   int ew_n_rgroups, ew_n_kgroups, ew_sum_blocks;
   double4* d_kvec;
-  int4* d_runs;
-  int* d_run_k0;
+  int4* d_runs;     // segments (ny, nz, nx_lo, length), longest first
+  int* d_run_k0;    // k index of each segment's first vector
+  double2 *d_tx, *d_ty, *d_tz;  // phase tables [entry][charged particle]
   int* d_qidx;
   double* d_qval;
   double2 *d_rho_part, *d_grho;
@@ -522,30 +527,70 @@ __global__ void k_set_flag(int* __restrict__ flags, int n_rep, int idx, int valu
 
 // Cell of each particle, mod(int(r*inv_r_bin), n_bin) per axis, and a slot in
 // that cell (binning.f90:25-48).
-__global__ void k_cell_fill(const KParams* __restrict__ kb, const int* __restrict__ rmap,
-                            const double* __restrict__ r, int* __restrict__ cell_of,
-                            int* __restrict__ cell_count, int* __restrict__ cell_slots,
-                            int* __restrict__ flags_all) {
+__global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
+    const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
+    int* __restrict__ cell_of, int* __restrict__ cell_count, int* __restrict__ cell_slots,
+    int* __restrict__ cell_start, int* __restrict__ flags_all) {
   MFA_REPLICA(blockIdx.x * blockDim.x);
   int* flags = flags_all + k.rep * F_NFLAGS;
-  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;
+  if (flags[F_REBUILD] == 0) return;  // uniform over the replica and the launch
+  const bool ok = flags[F_ERROR] == 0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i - k.p0 >= k.n) return;
-  const int cx = cell_coord(r[3 * (size_t)i + 0], k.inv_r_bin[0], k.nb[0]);
-  const int cy = cell_coord(r[3 * (size_t)i + 1], k.inv_r_bin[1], k.nb[1]);
-  const int cz = cell_coord(r[3 * (size_t)i + 2], k.inv_r_bin[2], k.nb[2]);
-  if (cx < 0 || cy < 0 || cz < 0) {  // the reference would index out of bounds
-    atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL);
-    cell_of[i] = 0;
-    return;
+  if (ok && i - k.p0 < k.n) {
+    const int cx = cell_coord(r[3 * (size_t)i + 0], k.inv_r_bin[0], k.nb[0]);
+    const int cy = cell_coord(r[3 * (size_t)i + 1], k.inv_r_bin[1], k.nb[1]);
+    const int cz = cell_coord(r[3 * (size_t)i + 2], k.inv_r_bin[2], k.nb[2]);
+    if (cx < 0 || cy < 0 || cz < 0) {  // the reference would index out of bounds
+      atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL);
+      cell_of[i] = 0;
+    } else {
+      const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;  // within the replica's grid
+      cell_of[i] = c;
+      const int slot = atomicAdd(&cell_count[k.c0 + c], 1);
+      if (slot < k.cell_cap)
+        cell_slots[(size_t)(k.c0 + c) * k.cell_cap + slot] = i;
+      else
+        atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
+    }
   }
-  const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;  // within the replica's grid
-  cell_of[i] = c;
-  const int slot = atomicAdd(&cell_count[k.c0 + c], 1);
-  if (slot < k.cell_cap)
-    cell_slots[(size_t)(k.c0 + c) * k.cell_cap + slot] = i;
-  else
-    atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
+  // The replica's last block to finish scans its cell counts:
+  // cell_start[c] = p0 + particles in the replica's cells before c.  No block
+  // waits for another; the counter is reset by the block that finds itself
+  // last.
+  __shared__ int last_s, wsum[MFA_BLOCK / 64], carry_s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int nb = (k.n + MFA_BLOCK - 1) / MFA_BLOCK;
+    last_s = atomicAdd(&flags[F_FILLDONE], 1) == nb - 1;
+    carry_s = k.p0;
+  }
+  __syncthreads();
+  if (!last_s) return;
+  if (threadIdx.x == 0) atomicExch(&flags[F_FILLDONE], 0);
+  // acquire: the other blocks' count increments (atomics at L2) are visible
+  // to plain loads after this fence
+  __threadfence();
+  // one pass: thread t sums the counts of cells [t per, (t+1) per), a block
+  // scan of the 256 sums gives each thread its offset, then its cells' starts
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int per = (k.n_cells + MFA_BLOCK - 1) / MFA_BLOCK;
+  const int cb = threadIdx.x * per, ce = min(k.n_cells, cb + per);
+  int tot = 0;
+  for (int c = cb; c < ce; ++c) tot += min(cell_count[k.c0 + c], k.cell_cap);
+  int x = tot;  // inclusive scan of the thread sums within the wave
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int run = carry_s + x - tot;
+  for (int w = 0; w < wid; ++w) run += wsum[w];
+  for (int c = cb; c < ce; ++c) {
+    cell_start[k.c0 + c] = run;
+    run += min(cell_count[k.c0 + c], k.cell_cap);
+  }
 }
 
 // Spatial order.  At each rebuild the particles of a replica are ordered by
@@ -556,40 +601,6 @@ __global__ void k_cell_fill(const KParams* __restrict__ kb, const int* __restric
 // a particle's neighbours lie close to it in memory; the FP64 state keeps the
 // reference's particle order.
 //
-// Exclusive scan of the cell counts of each replica (one workgroup per
-// replica): cell_start[c] = p0 + particles in the replica's cells before c.
-#define SCAN_BLOCK 1024
-__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_scan(const KParams* __restrict__ kb,
-                                                          const int* __restrict__ cell_count,
-                                                          int* __restrict__ cell_start,
-                                                          const int* __restrict__ flags_all) {
-  const KParams& k = kb[blockIdx.x];
-  const int* flags = flags_all + k.rep * F_NFLAGS;
-  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;
-  __shared__ int wsum[SCAN_BLOCK / 64];
-  __shared__ int carry_s;
-  if (threadIdx.x == 0) carry_s = k.p0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < k.n_cells; c0 += SCAN_BLOCK) {
-    const int c = c0 + threadIdx.x;
-    const int v = c < k.n_cells ? min(cell_count[k.c0 + c], k.cell_cap) : 0;
-    int x = v;  // inclusive scan within the wave
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    int before = carry_s;
-    for (int w = 0; w < wid; ++w) before += wsum[w];
-    if (c < k.n_cells) cell_start[k.c0 + c] = before + x - v;
-    __syncthreads();
-    if (threadIdx.x == SCAN_BLOCK - 1) carry_s = before + x;
-    __syncthreads();
-  }
-}
-
 // One wave per cell, four cells per workgroup: orders each cell's particles
 // by id, records the permutation, writes the FP32 view of each (relative to
 // its cell's origin) at its sorted position and resets the skin reference
@@ -717,39 +728,52 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const float pjx = pj.x - o.x, pjy = pj.y - o.y, pjz = pj.z - o.z;
     const int tj = __float_as_int(pj.w) & 3;
     const unsigned entry = (unsigned)js | ((unsigned)q << MFA_JBITS);
-    for (int h = 0; h < m; ++h) {
-      const float4 pi = hpos[h];
-      const int ti = __float_as_int(pi.w) & 3;
-      const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
-      const float d2 = dx * dx + dy * dy + dz * dz;
-      const float l2 = l2tab[ti * MFA_NTYPE + tj];
-      bool hit = false;
-      if (live && js != hs0 + h) {
-        if (d2 < l2 - margin) {
-          hit = true;
-        } else if (d2 < l2 + margin) {
-          const int i = perm[hs0 + h], j = perm[js];
-          double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
-          double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
-          double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
-          min_image(k, ex, ey, ez);
-          const double r2 = (ex * ex + ey * ey) + ez * ez;
-          hit = r2 < k.list_2[ti][tj];
+    // four particles of the cell per round: their distance tests are
+    // independent, then the hits are appended particle by particle
+    for (int h0 = 0; h0 < m; h0 += 4) {
+      bool hit[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int h = h0 + u;
+        hit[u] = false;
+        if (h < m) {
+          const float4 pi = hpos[h];
+          const int ti = __float_as_int(pi.w) & 3;
+          const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          const float l2 = l2tab[ti * MFA_NTYPE + tj];
+          if (live && js != hs0 + h) {
+            if (d2 < l2 - margin) {
+              hit[u] = true;
+            } else if (d2 < l2 + margin) {
+              const int i = perm[hs0 + h], j = perm[js];
+              double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+              double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+              double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+              min_image(k, ex, ey, ez);
+              const double r2 = (ex * ex + ey * ey) + ez * ez;
+              hit[u] = r2 < k.list_2[ti][tj];
+            }
+          }
         }
       }
-      const unsigned long long mask = __ballot(hit);
-      if (mask) {
-        const int sl = h >> 6, ln = h & 63;
-        int mine = hc[0];
-        for (int u = 1; u < LB_SLOTS; ++u) mine = sl == u ? hc[u] : mine;
-        const int base = __builtin_amdgcn_readlane(mine, ln);
-        if (hit) {
-          const int pos = base + (int)__popcll(mask & ((1ull << lane) - 1ull));
-          if (pos < cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
-        }
-        if (lane == ln) {
-          const int add = (int)__popcll(mask);
-          for (int u = 0; u < LB_SLOTS; ++u) hc[u] += sl == u ? add : 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const unsigned long long mask = __ballot(hit[u]);
+        if (mask) {
+          const int h = h0 + u;
+          const int sl = h >> 6, ln = h & 63;
+          int mine = hc[0];
+          for (int q2 = 1; q2 < LB_SLOTS; ++q2) mine = sl == q2 ? hc[q2] : mine;
+          const int base = __builtin_amdgcn_readlane(mine, ln);
+          if (hit[u]) {
+            const int pos = base + (int)__popcll(mask & ((1ull << lane) - 1ull));
+            if (pos < cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
+          }
+          if (lane == ln) {
+            const int add = (int)__popcll(mask);
+            for (int q2 = 0; q2 < LB_SLOTS; ++q2) hc[q2] += sl == q2 ? add : 0;
+          }
         }
       }
     }
@@ -1370,88 +1394,134 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 // ---------------------------------------------------------------------------
 // Ewald sum in k space (ewald_k.f90:150-199) and the slab dipole correction
 // (dipolar_correction.f90:20-33), FP64.  The k vectors of the half space
-// (ewald_k.f90:104-124) come in runs of consecutive nx at fixed (ny, nz); a
-// thread evaluates exp(i k.r) once per run with sincos and steps along the
-// run by the factor exp(i dkx x), so a run of m vectors costs one sincos and
-// m complex products.  Charged particles only (q = 0 contributes nothing).
-#define EW_QPT 4  // charged particles per thread in the structure factor
+// (ewald_k.f90:104-124) are grouped into segments of at most EW_SEG
+// consecutive kx at fixed (ky, kz).  Per charged particle j and step the
+// phase factors factorise, exp(i k.r_j) = X_j(nx) Y_j(ny) Z_j(nz), and are
+// tabulated once per axis (one sincos per entry); every (k, j) term is then a
+// product of table entries, with no per-k reduction across lanes and no
+// serial recurrence.  Charged particles only (q = 0 contributes nothing).
+#define EW_SEG 8   // k vectors per segment
+#define EW_CH 32   // charged particles per structure-factor block
 
-// Partial structure factor of one wave of 64*EW_QPT charged particles over
-// one group of runs, reduced across the wave in a fixed tree; and (run group
-// 0) the partial dipole moment sum q (z - lz_half).
-__global__ __launch_bounds__(64) void k_ewald_rho(EwaldK e, const double* __restrict__ r,
-                                                  const int* __restrict__ qidx,
-                                                  const double* __restrict__ qval,
-                                                  const int4* __restrict__ runs,
-                                                  const int* __restrict__ run_k0,
-                                                  double2* __restrict__ rho_part,
-                                                  double* __restrict__ mz_part) {
-  const int pw = blockIdx.x, rg = blockIdx.y, lane = threadIdx.x;
-  double x[EW_QPT], y[EW_QPT], z[EW_QPT], q[EW_QPT], cx[EW_QPT], sx[EW_QPT];
+// Phase tables X[n][j] (n = nx + nxm), Y[n][j] (n = ny + nym), Z[n][j]
+// (n = nz) of the charged particles: one thread per table entry (one sincos),
+// entries of consecutive particles in consecutive threads; blocks of the
+// first entry row also write the partial dipole moments sum q (z - lz_half).
+__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_tables(EwaldK e, const double* __restrict__ r,
+                                                            const int* __restrict__ qidx,
+                                                            const double* __restrict__ qval,
+                                                            double2* __restrict__ tx,
+                                                            double2* __restrict__ ty,
+                                                            double2* __restrict__ tz,
+                                                            double* __restrict__ mz_part) {
+  __shared__ double sh[MFA_BLOCK / 64];
+  const int idx = blockIdx.x * MFA_BLOCK + threadIdx.x;  // charged particle
+  const int n = blockIdx.y;                               // table entry (all three axes)
   double mz = 0.0;
-  for (int u = 0; u < EW_QPT; ++u) {
-    const int idx = (pw * EW_QPT + u) * 64 + lane;
-    if (idx < e.nq) {
-      const int i = qidx[idx];
-      x[u] = r[3 * (size_t)i + 0];
-      y[u] = r[3 * (size_t)i + 1];
-      z[u] = r[3 * (size_t)i + 2];
-      q[u] = qval[idx];
+  if (idx < e.nq) {
+    const int i = qidx[idx];
+    double s, c;
+    if (n < e.nex) {
+      sincos(e.dk[0] * (double)(n - e.nxm) * r[3 * (size_t)i + 0], &s, &c);
+      tx[(size_t)n * e.nq + idx] = make_double2(c, s);
+    } else if (n < e.nex + e.ney) {
+      const int m = n - e.nex;
+      sincos(e.dk[1] * (double)(m - e.nym) * r[3 * (size_t)i + 1], &s, &c);
+      ty[(size_t)m * e.nq + idx] = make_double2(c, s);
     } else {
-      x[u] = y[u] = z[u] = q[u] = 0.0;
+      const int m = n - e.nex - e.ney;
+      sincos(e.dk[2] * (double)m * r[3 * (size_t)i + 2], &s, &c);
+      tz[(size_t)m * e.nq + idx] = make_double2(c, s);
     }
-    sincos(e.dk[0] * x[u], &sx[u], &cx[u]);
-    mz += q[u] * (z[u] - e.lz_half);
+    if (n == 0) mz = qval[idx] * (r[3 * (size_t)i + 2] - e.lz_half);
   }
-  if (rg == 0) {
-    for (int o = 32; o > 0; o >>= 1) mz += __shfl_down(mz, o, 64);
-    if (lane == 0) mz_part[pw] = mz;
-  }
-  const int r0 = rg * e.runs_per_rgroup, r1 = min(e.nruns, r0 + e.runs_per_rgroup);
-  for (int run = r0; run < r1; ++run) {
-    const int4 rn = runs[run];  // (ny, nz, nx_lo, nx_hi)
-    const double kx0 = e.dk[0] * (double)rn.z, ky = e.dk[1] * (double)rn.x,
-                 kz = e.dk[2] * (double)rn.y;
-    double c[EW_QPT], s[EW_QPT];
-    for (int u = 0; u < EW_QPT; ++u) sincos(kx0 * x[u] + ky * y[u] + kz * z[u], &s[u], &c[u]);
-    int kidx = run_k0[run];
-    for (int nx = rn.z; nx <= rn.w; ++nx, ++kidx) {
-      double re = 0.0, im = 0.0;
-      for (int u = 0; u < EW_QPT; ++u) {
-        re += q[u] * c[u];
-        im += q[u] * s[u];
-        const double cn = c[u] * cx[u] - s[u] * sx[u];
-        s[u] = s[u] * cx[u] + c[u] * sx[u];
-        c[u] = cn;
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        re += __shfl_down(re, o, 64);
-        im += __shfl_down(im, o, 64);
-      }
-      if (lane == 0) rho_part[(size_t)pw * e.nk + kidx] = make_double2(re, im);
-    }
+  if (n != 0) return;
+  for (int o = 32; o > 0; o >>= 1) mz += __shfl_down(mz, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mz;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < MFA_BLOCK / 64; ++w) m += sh[w];
+    mz_part[blockIdx.x] = m;
   }
 }
 
-// rho(k) = sum of the wave partials (fixed order), stored weighted by
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Partial structure factor of one chunk of EW_CH charged particles: thread t
+// of block (chunk, g) owns segment 256 g + t and accumulates
+// sum_j q_j X_j(nx) Y_j(ny) Z_j(nz) over the chunk for its k vectors; the
+// chunk's table rows are staged in LDS particle-major.
+__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_rho(EwaldK e, const double* __restrict__ qval,
+                                                         const double2* __restrict__ tx,
+                                                         const double2* __restrict__ ty,
+                                                         const double2* __restrict__ tz,
+                                                         const int4* __restrict__ segs,
+                                                         const int* __restrict__ seg_k0,
+                                                         double2* __restrict__ rho_part) {
+  extern __shared__ double2 ew_lds[];  // [EW_CH][nex + ney + nez]
+  const int row = e.nex + e.ney + e.nez;
+  const int j0 = blockIdx.x * EW_CH;
+  const int nj = min(EW_CH, e.nq - j0);
+  for (int t = threadIdx.x; t < nj * row; t += MFA_BLOCK) {
+    const int j = t % nj, n = t / nj;  // coalesced over j
+    double2 v;
+    if (n < e.nex) {
+      v = tx[(size_t)n * e.nq + j0 + j];
+    } else if (n < e.nex + e.ney) {
+      v = ty[(size_t)(n - e.nex) * e.nq + j0 + j];
+    } else {
+      const double q = qval[j0 + j];
+      const double2 zz = tz[(size_t)(n - e.nex - e.ney) * e.nq + j0 + j];
+      v = make_double2(q * zz.x, q * zz.y);
+    }
+    ew_lds[j * row + n] = v;
+  }
+  __syncthreads();
+  const int sg = blockIdx.y * MFA_BLOCK + threadIdx.x;
+  if (sg >= e.nsegs) return;
+  const int4 s = segs[sg];  // (ny, nz, nx_lo, length)
+  const int yo = e.nex + s.x + e.nym, zo = e.nex + e.ney + s.y, xo = s.z + e.nxm;
+  double2 acc[EW_SEG];
+  for (int u = 0; u < EW_SEG; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int j = 0; j < nj; ++j) {
+    const double2* tj = ew_lds + j * row;
+    const double2 a = cmul(tj[yo], tj[zo]);
+    for (int u = 0; u < EW_SEG; ++u) {
+      if (u < s.w) {
+        const double2 xv = tj[xo + u];
+        acc[u].x += xv.x * a.x - xv.y * a.y;
+        acc[u].y += xv.x * a.y + xv.y * a.x;
+      }
+    }
+  }
+  const int k0 = seg_k0[sg];
+  for (int u = 0; u < EW_SEG; ++u)
+    if (u < s.w) rho_part[(size_t)blockIdx.x * e.nk + k0 + u] = acc[u];
+}
+
+// rho(k) = sum of the chunk partials (fixed order), stored weighted by
 // g_k = exp(-k^2/(4 alpha_c))/k^2; per-block sums of g_k |rho(k)|^2 for the
-// energy; block 0 also sums the dipole moment.  A block of 4 waves covers 64 k
-// vectors: wave w adds the partials pw = w, w+4, ..., then wave 0 combines
-// the 4 in a fixed order.
-__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_sum(EwaldK e,
-                                                         const double2* __restrict__ rho_part,
-                                                         const double4* __restrict__ kvec,
-                                                         double2* __restrict__ grho,
-                                                         double* __restrict__ uk_part,
-                                                         const double* __restrict__ mz_part,
-                                                         double* __restrict__ mz) {
-  __shared__ double2 sh[4][64];
+// energy; block 0 also sums the dipole moment.  A block of EW_SUMW waves
+// covers 64 k vectors: wave w adds the partials c = w, w + EW_SUMW, ..., then
+// wave 0 combines the wave sums in a fixed order.
+#define EW_SUMW 16
+__global__ __launch_bounds__(64 * EW_SUMW) void k_ewald_sum(EwaldK e,
+                                                            const double2* __restrict__ rho_part,
+                                                            const double4* __restrict__ kvec,
+                                                            double2* __restrict__ grho,
+                                                            double* __restrict__ uk_part,
+                                                            const double* __restrict__ mz_part,
+                                                            double* __restrict__ mz) {
+  __shared__ double2 sh[EW_SUMW][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int kk = blockIdx.x * 64 + lane;
   double re = 0.0, im = 0.0;
   if (kk < e.nk)
-    for (int pw = w; pw < e.n_pw; pw += 4) {
-      const double2 t = rho_part[(size_t)pw * e.nk + kk];
+    for (int cb = w; cb < e.n_chunks; cb += EW_SUMW) {
+      const double2 t = rho_part[(size_t)cb * e.nk + kk];
       re += t.x;
       im += t.y;
     }
@@ -1460,8 +1530,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_sum(EwaldK e,
   if (w == 0) {
     double u = 0.0;
     if (kk < e.nk) {
-      re = (sh[0][lane].x + sh[1][lane].x) + (sh[2][lane].x + sh[3][lane].x);
-      im = (sh[0][lane].y + sh[1][lane].y) + (sh[2][lane].y + sh[3][lane].y);
+      re = 0.0;
+      im = 0.0;
+      for (int q = 0; q < EW_SUMW; ++q) {
+        re += sh[q][lane].x;
+        im += sh[q][lane].y;
+      }
       const double g = kvec[kk].w;
       grho[kk] = make_double2(g * re, g * im);
       u = g * (re * re + im * im);
@@ -1471,59 +1545,83 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_sum(EwaldK e,
       uk_part[blockIdx.x] = u;
       if (blockIdx.x == 0) {
         double m = 0.0;
-        for (int pw = 0; pw < e.n_pw; ++pw) m += mz_part[pw];
+        for (int b = 0; b < e.n_mz; ++b) m += mz_part[b];
         *mz = m;
       }
     }
   }
 }
 
-// Sum over groups of runs of k g_k (cos(k.r) Im rho - sin(k.r) Re rho) for
-// each charged particle (ewald_k.f90:177-192 without the prefactor).  A block
-// holds 64 particles and 4 waves; wave w takes run group 4 blockIdx.y + w, and
-// the 4 partial sums are combined in a fixed order into one partial per block.
-__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const double* __restrict__ r,
-                                                           const int* __restrict__ qidx,
-                                                           const int4* __restrict__ runs,
-                                                           const int* __restrict__ run_k0,
+// Sum over k of k g_k (cos(k.r) Im rho - sin(k.r) Re rho) for each charged
+// particle (ewald_k.f90:177-192 without the prefactor).  A block holds 64
+// particles (the lanes) and 4 waves; their X table columns and g_k rho(k) of
+// the block's segments (group blockIdx.y) are staged in LDS.  The segments
+// are dealt to the waves in turn, all lanes of a wave on the same segment;
+// each wave loads the Y and Z entries of its next segment while it works on
+// the current one.  The 4 wave sums are combined in a fixed order into one
+// partial per block.
+__global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const double2* __restrict__ tx,
+                                                           const double2* __restrict__ ty,
+                                                           const double2* __restrict__ tz,
+                                                           const int4* __restrict__ segs,
+                                                           const int* __restrict__ seg_k0,
                                                            const double2* __restrict__ grho,
                                                            double* __restrict__ fk_part) {
+  extern __shared__ double2 ew_xs[];  // [nex][64], then [segs_per_kgroup][EW_SEG]
   __shared__ double sh[4][64][3];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int idx = blockIdx.x * 64 + lane, kg = blockIdx.y * 4 + w;
+  const int idx = blockIdx.x * 64 + lane;
+  const bool live = idx < e.nq;
+  const int jq = live ? idx : 0;
+  double2* gs = ew_xs + e.nex * 64;
+  const int g0 = blockIdx.y * e.segs_per_kgroup, g1 = min(e.nsegs, g0 + e.segs_per_kgroup);
+  for (int n = w; n < e.nex; n += 4) ew_xs[n * 64 + lane] = tx[(size_t)n * e.nq + jq];
+  for (int t = threadIdx.x; t < (g1 - g0) * EW_SEG; t += MFA_BLOCK) {
+    const int sg = g0 + t / EW_SEG, u = t % EW_SEG;
+    const int4 s = segs[sg];
+    gs[t] = u < s.w ? grho[seg_k0[sg] + u] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  if (idx < e.nq) {
-    const int i = qidx[idx];
-    const double x = r[3 * (size_t)i + 0], y = r[3 * (size_t)i + 1], z = r[3 * (size_t)i + 2];
-    double sx, cx;
-    sincos(e.dk[0] * x, &sx, &cx);
-    const int r0 = kg * e.runs_per_kgroup, r1 = min(e.nruns, r0 + e.runs_per_kgroup);
-    for (int run = r0; run < r1; ++run) {
-      const int4 rn = runs[run];
-      const double kx0 = e.dk[0] * (double)rn.z, ky = e.dk[1] * (double)rn.x,
-                   kz = e.dk[2] * (double)rn.y;
-      double s, c;
-      sincos(kx0 * x + ky * y + kz * z, &s, &c);
-      int kidx = run_k0[run];
-      double ty = 0.0;  // sum of t over the run (ky, kz are constant along it)
-      for (int nx = rn.z; nx <= rn.w; ++nx, ++kidx) {
-        const double2 g = grho[kidx];
-        const double t = c * g.y - s * g.x;
-        fx += (e.dk[0] * (double)nx) * t;
-        ty += t;
-        const double cn = c * cx - s * sx;
-        s = s * cx + c * sx;
-        c = cn;
-      }
-      fy += ky * ty;
-      fz += kz * ty;
+  int sg = g0 + w;
+  double2 yv = make_double2(0.0, 0.0), zv = yv;
+  int4 s = make_int4(0, 0, 0, 0);
+  if (sg < g1) {
+    s = segs[sg];
+    yv = ty[(size_t)(s.x + e.nym) * e.nq + jq];
+    zv = tz[(size_t)s.y * e.nq + jq];
+  }
+  while (sg < g1) {
+    const int nsg = sg + 4;
+    int4 ns = make_int4(0, 0, 0, 0);
+    double2 nyv = make_double2(0.0, 0.0), nzv = nyv;
+    if (nsg < g1) {  // the next segment's entries, loaded ahead
+      ns = segs[nsg];
+      nyv = ty[(size_t)(ns.x + e.nym) * e.nq + jq];
+      nzv = tz[(size_t)ns.y * e.nq + jq];
     }
+    const double2 b = cmul(yv, zv);
+    const double2* xs = ew_xs + (s.z + e.nxm) * 64 + lane;
+    const double2* g = gs + (sg - g0) * EW_SEG;
+    double tsum = 0.0, kx = e.dk[0] * (double)s.z;
+    for (int u = 0; u < s.w; ++u, kx += e.dk[0]) {  // s.w is uniform across the wave
+      const double2 xe = cmul(xs[u * 64], b);
+      const double t = xe.x * g[u].y - xe.y * g[u].x;
+      fx += kx * t;
+      tsum += t;
+    }
+    fy += (e.dk[1] * (double)s.x) * tsum;
+    fz += (e.dk[2] * (double)s.y) * tsum;
+    sg = nsg;
+    s = ns;
+    yv = nyv;
+    zv = nzv;
   }
   sh[w][lane][0] = fx;
   sh[w][lane][1] = fy;
   sh[w][lane][2] = fz;
   __syncthreads();
-  if (w == 0 && idx < e.nq) {
+  if (w == 0 && live) {
     double* o = fk_part + 3 * ((size_t)blockIdx.y * e.nq + idx);
     for (int d = 0; d < 3; ++d)
       o[d] = (sh[0][lane][d] + sh[1][lane][d]) + (sh[2][lane][d] + sh[3][lane][d]);
@@ -1696,9 +1794,7 @@ static size_t list_lds_bytes(int cell_cap) { return (size_t)4 * cell_cap * sizeo
 static int launch_rebuild(mfa_ctx* c) {
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
-                     c->cell_of, c->cell_count, c->cell_slots, c->flags);
-  hipLaunchKernelGGL(k_cell_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_kp,
-                     c->cell_count, c->cell_start, c->flags);
+                     c->cell_of, c->cell_count, c->cell_slots, c->cell_start, c->flags);
   const int cb = (c->n_cells + 3) / 4;  // four cells (waves) per workgroup
   hipLaunchKernelGGL(k_cell_sort, dim3(cb), dim3(MFA_BLOCK), (size_t)4 * c->cell_cap * sizeof(int),
                      c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->v, c->type, c->mic,
@@ -1957,7 +2053,7 @@ static int ewald_setup(mfa_ctx* c) {
   const int nxm = (int)(k_c / dk[0]), nym = (int)(k_c / dk[1]), nzm = (int)(k_c / dk[2]);
   const double fac_alpha = 1. / (4. * p.alpha_c);
   std::vector<double4> kv;
-  std::vector<int4> runs;
+  std::vector<int4> runs;   // maximal runs of consecutive kx: (ny, nz, nx_lo, nx_hi)
   std::vector<int> run_k0;
   for (int nkz = 0; nkz <= nzm; ++nkz)
     for (int nky = -nym; nky <= nym; ++nky) {
@@ -1985,33 +2081,68 @@ static int ewald_setup(mfa_ctx* c) {
         }
       }
     }
+  // segments of at most EW_SEG vectors, longest first (stable), so the
+  // lanes of a wave loop over equal lengths
+  std::vector<int4> segs;
+  std::vector<int> seg_k0;
+  for (size_t q = 0; q < runs.size(); ++q)
+    for (int lo = runs[q].z; lo <= runs[q].w; lo += EW_SEG) {
+      const int len = runs[q].w - lo + 1 < EW_SEG ? runs[q].w - lo + 1 : EW_SEG;
+      segs.push_back(make_int4(runs[q].x, runs[q].y, lo, len));
+      seg_k0.push_back(run_k0[q] + (lo - runs[q].z));
+    }
+  {
+    std::vector<int> order(segs.size());
+    for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return segs[a].w > segs[b].w; });
+    std::vector<int4> s2(segs.size());
+    std::vector<int> k2(segs.size());
+    for (size_t q = 0; q < order.size(); ++q) {
+      s2[q] = segs[(size_t)order[q]];
+      k2[q] = seg_k0[(size_t)order[q]];
+    }
+    segs.swap(s2);
+    seg_k0.swap(k2);
+  }
   EwaldK& e = c->ek;
   memset(&e, 0, sizeof e);
   for (int d = 0; d < 3; ++d) e.dk[d] = dk[d];
   e.lz_half = p.boundary[2] / 2.;
   e.nk = (int)kv.size();
-  e.nruns = (int)runs.size();
+  e.nsegs = (int)segs.size();
+  e.nxm = nxm;
+  e.nym = nym;
+  e.nzm = nzm;
+  e.nex = 2 * nxm + 1;
+  e.ney = 2 * nym + 1;
+  e.nez = nzm + 1;
+  if ((size_t)EW_CH * (e.nex + e.ney + e.nez) * sizeof(double2) > 64 * 1024)
+    return set_err(nullptr, MFA_ERR_ARG, "Ewald: k grid too large for the structure-factor tiles");
   c->ew_inv_V = p.inv_boundary[0] * p.inv_boundary[1] / (p.boundary[2] * (1. + z_slab));
   c->ew_sum_blocks = (e.nk + 63) / 64;
   const size_t n = (size_t)c->n;
-  const size_t n_pw_max = (n + 64 * EW_QPT - 1) / (64 * EW_QPT);
-  const size_t nk = (size_t)(e.nk > 0 ? e.nk : 1), nr = (size_t)(e.nruns > 0 ? e.nruns : 1);
+  const size_t n_chunk_max = (n + EW_CH - 1) / EW_CH;
+  const size_t nk = (size_t)(e.nk > 0 ? e.nk : 1), nr = (size_t)(e.nsegs > 0 ? e.nsegs : 1);
+  const size_t ntab = (size_t)(e.nex + e.ney + e.nez);
   void** bufs[] = {(void**)&c->d_kvec, (void**)&c->d_runs, (void**)&c->d_run_k0,
                    (void**)&c->d_qidx, (void**)&c->d_qval, (void**)&c->d_rho_part,
                    (void**)&c->d_grho, (void**)&c->d_uk_part, (void**)&c->d_mz_part,
-                   (void**)&c->d_mz, (void**)&c->d_fk_part, (void**)&c->d_ek};
+                   (void**)&c->d_mz, (void**)&c->d_fk_part, (void**)&c->d_ek, (void**)&c->d_tx};
   const size_t bytes[] = {nk * sizeof(double4), nr * sizeof(int4), nr * sizeof(int),
-                          n * sizeof(int), n * sizeof(double), n_pw_max * nk * sizeof(double2),
+                          n * sizeof(int), n * sizeof(double), n_chunk_max * nk * sizeof(double2),
                           nk * sizeof(double2), (size_t)(c->ew_sum_blocks + 1) * sizeof(double),
-                          n_pw_max * sizeof(double), sizeof(double),
-                          (2048 * 64 + n) * 3 * sizeof(double), 2 * sizeof(double)};
-  for (int b = 0; b < 12; ++b)
+                          (n / MFA_BLOCK + 1) * sizeof(double), sizeof(double),
+                          (2048 * 64 + n) * 3 * sizeof(double), 2 * sizeof(double),
+                          ntab * n * sizeof(double2)};
+  for (int b = 0; b < 13; ++b)
     if (hipMalloc(bufs[b], bytes[b]) != hipSuccess)
       return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (Ewald) failed", bytes[b]);
+  c->d_ty = c->d_tx + (size_t)e.nex * n;
+  c->d_tz = c->d_ty + (size_t)e.ney * n;
   if (e.nk > 0) {
     HIPCHK(nullptr, hipMemcpy(c->d_kvec, kv.data(), kv.size() * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(nullptr, hipMemcpy(c->d_runs, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIPCHK(nullptr, hipMemcpy(c->d_run_k0, run_k0.data(), run_k0.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_runs, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_run_k0, seg_k0.data(), seg_k0.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   HIPCHK(nullptr, hipMemset(c->d_mz, 0, sizeof(double)));
   c->ewald = true;
@@ -2036,44 +2167,47 @@ static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
   c->v_coul_self = self * c->p.sqrt_alpha / sqrt(pi_f);
   EwaldK& e = c->ek;
   e.nq = (int)qi.size();
-  if (e.nq == 0 || e.nruns == 0) return MFA_OK;
+  if (e.nq == 0 || e.nsegs == 0) return MFA_OK;
   HIPCHK(c, hipMemcpy(c->d_qidx, qi.data(), qi.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_qval, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice));
-  // Launch shapes: about 8192 waves for each of the structure factor and the
-  // forces (1024 SIMDs; the sincos and recurrence chains need several waves
-  // per SIMD to hide their latency).  The force pass's run groups come in
-  // fours (one block of 4 waves), padded with empty groups; its partial sums
-  // (one per block row) fit the buffer ewald_setup sized for 2048 rows of 64.
-  e.n_pw = (e.nq + 64 * EW_QPT - 1) / (64 * EW_QPT);
-  int ng = 8192 / e.n_pw;
-  ng = ng < 1 ? 1 : (ng > e.nruns ? e.nruns : ng);
-  e.runs_per_rgroup = (e.nruns + ng - 1) / ng;
-  c->ew_n_rgroups = (e.nruns + e.runs_per_rgroup - 1) / e.runs_per_rgroup;
+  e.n_chunks = (e.nq + EW_CH - 1) / EW_CH;
+  e.n_mz = (e.nq + MFA_BLOCK - 1) / MFA_BLOCK;
+  // forces: about 8192 waves (4 per block), block rows of segments; the
+  // partials (one per block row and particle) fit the buffer ewald_setup
+  // sized for 2048 rows of 64
   const int qb = (e.nq + 63) / 64;
-  int kg = 8192 / qb;
-  kg = kg < 4 ? 4 : (kg > e.nruns ? e.nruns : kg);
-  e.runs_per_kgroup = (e.nruns + kg - 1) / kg;
-  kg = (e.nruns + e.runs_per_kgroup - 1) / e.runs_per_kgroup;
-  c->ew_n_kgroups = (kg + 3) / 4 * 4;
-  if ((size_t)(c->ew_n_kgroups / 4) * (size_t)e.nq > (size_t)2048 * 64 + (size_t)c->n)
+  int kg = 2048 / qb;
+  kg = kg < 1 ? 1 : (kg > e.nsegs ? e.nsegs : kg);
+  e.segs_per_kgroup = (e.nsegs + kg - 1) / kg;
+  c->ew_n_kgroups = (e.nsegs + e.segs_per_kgroup - 1) / e.segs_per_kgroup;
+  if ((size_t)c->ew_n_kgroups * (size_t)e.nq > (size_t)2048 * 64 + (size_t)c->n)
     return set_err(c, MFA_ERR_ARG, "Ewald: too many force partials");
+  if (((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2) > 64 * 1024)
+    return set_err(c, MFA_ERR_ARG, "Ewald: k grid too large for the force tiles");
   return MFA_OK;
 }
 
-static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ek.nq > 0 && c->ek.nruns > 0; }
+static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ek.nq > 0 && c->ek.nsegs > 0; }
 
 // k-space forces (and, with add_dip, the slab dipole forces) added to F.
 static int launch_ewald(mfa_ctx* c, int add_dip) {
   if (!ewald_active(c)) return MFA_OK;
   const EwaldK& e = c->ek;
-  hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_pw, c->ew_n_rgroups), dim3(64), 0, c->stream, e, c->r,
-                     c->d_qidx, c->d_qval, c->d_runs, c->d_run_k0, c->d_rho_part, c->d_mz_part);
-  hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(MFA_BLOCK), 0, c->stream, e,
+  hipLaunchKernelGGL(k_ewald_tables, dim3(e.n_mz, e.nex + e.ney + e.nez), dim3(MFA_BLOCK), 0,
+                     c->stream, e, c->r, c->d_qidx, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
+                     c->d_mz_part);
+  hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_chunks, (e.nsegs + MFA_BLOCK - 1) / MFA_BLOCK),
+                     dim3(MFA_BLOCK), (size_t)EW_CH * (e.nex + e.ney + e.nez) * sizeof(double2),
+                     c->stream, e, c->d_qval, c->d_tx, c->d_ty, c->d_tz, c->d_runs, c->d_run_k0,
+                     c->d_rho_part);
+  hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
                      c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
-  hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups / 4), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->r, c->d_qidx, c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
+  hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups), dim3(MFA_BLOCK),
+                     ((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2),
+                     c->stream, e, c->d_tx, c->d_ty, c->d_tz,
+                     c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
   hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups / 4, c->d_mz,
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
                      c->ew_inv_V, 1, add_dip, c->f);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
@@ -2310,7 +2444,8 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   for (void* q : ptrs)
     if (q) hipFree(q);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
-                   c->d_grho,    c->d_uk_part, c->d_mz_part, c->d_mz,  c->d_fk_part, c->d_ek};
+                   c->d_grho,    c->d_uk_part, c->d_mz_part, c->d_mz,  c->d_fk_part, c->d_ek,
+                   c->d_tx};
   for (void* q : eptrs)
     if (q) hipFree(q);
   if (c->h_flags) hipHostFree(c->h_flags);
@@ -2641,7 +2776,7 @@ extern "C" int32_t mfa_dipolar_correction(mfa_ctx* c, double* v_coul) {
   hipLaunchKernelGGL(k_ewald_mz, dim3(1), dim3(MFA_BLOCK), 0, c->stream, e, c->r, c->d_qidx,
                      c->d_qval, c->d_mz);
   hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups / 4, c->d_mz,
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
                      c->ew_inv_V, 0, 1, c->f);
   HIPCHK(c, hipGetLastError());
   double uk, udip;

@@ -80,7 +80,7 @@ program md_pb_hip
   integer :: i_time, tot_time, i, u60, nchunk
   integer(c_int64_t) :: step
   integer(c_int32_t) :: rc
-  logical :: have_conf
+  logical :: have_conf, exact_restart
   real(8) :: inv_N, v_total, e_total, vcm(3), mtot
   integer(8) :: c0, c1, crate
   type(mfa_obs_t) :: obs
@@ -100,8 +100,11 @@ program md_pb_hip
   allocate(a_type(n_part), mic_count(3, n_part))
   mic_count = 0
   inquire(file="conf_old", exist=have_conf)
+  exact_restart = .false.
   if (have_conf) then
     call read_conf_old(n_part, a_type, r0, v, a)
+    call read_state_sidecar(n_part, r0, v, a, mic_count, exact_restart)
+    if (exact_restart) print '(a)', " * conf_old.bin: full-precision state and wrap counters restored"
   else
     print '(/a/)', " *   'conf_old' file not found. Using the seeded generator ..."
     rc = mfa_generate(par, merge(1_c_int32_t, 2_c_int32_t, inp%symmetry == 0), inp%iseed, &
@@ -109,8 +112,9 @@ program md_pb_hip
     call mfa_check(rc, "generate", c_null_ptr)
     a = 0d0
   end if
-  ! SYSTEM 2/3 in bulk: remove the centre-of-mass velocity (init_config.f90:336-356)
-  if (inp%symmetry == 1) then
+  ! SYSTEM 2/3 in bulk: remove the centre-of-mass velocity (init_config.f90:336-356);
+  ! an exact restart keeps the stored velocities
+  if (inp%symmetry == 1 .and. .not. exact_restart) then
     vcm = 0d0
     mtot = 0d0
     do i = 1, n_part
@@ -147,7 +151,7 @@ program md_pb_hip
       i_time = i_time + nchunk
       if (nchunk == 10) call write_fort60(i_time, obs%v_fluid_wall, obs%v_fluid_fluid, &
                                           obs%v_intra_molec, obs%t_kin, obs%v_coul)
-      if (mod(i_time, inp%n_safe) == 0) call store()
+      if (mod(i_time, inp%n_safe) == 0) call store(.true.)
     end do
   else
     do i_time = 1, tot_time
@@ -167,10 +171,10 @@ program md_pb_hip
       call verlet_velocities()
       if (mod(i_time, 10) == 0) call write_fort60(i_time, v_fluid_wall, v_fluid_fluid, &
                                                    v_intra_molec, t_fluid, v_coul)
-      if (mod(i_time, inp%n_safe) == 0) call store()
+      if (mod(i_time, inp%n_safe) == 0) call store(.true.)
     end do
   end if
-  call store()
+  call store(.false.)
   close(u60)
   rc = mfa_destroy(ctx)
   call system_clock(c1)
@@ -188,9 +192,16 @@ contains
     write(u60, '(i7,4g17.5)') it, v_total * inv_N, e_total * inv_N, tkin * inv_N, vc * inv_N
   end subroutine
 
-  subroutine store()
+  ! store_config (md_main.f90:168): conf_new and its full-precision sidecar.
+  ! Inside the loop the list is rebuilt right after, as a run resumed from
+  ! these files rebuilds it at its start: the mixed form's FP32 view is taken
+  ! relative to the cells of the last rebuild, so both runs then repeat the
+  ! same arithmetic and the resumed run continues bit for bit.
+  subroutine store(rebin)
+    logical, intent(in) :: rebin
     call fetch_state()
     call write_conf_new(inp, n_part, a_type, r0, v, a, mic_count)
+    if (rebin) call binning()
   end subroutine
 
 end program md_pb_hip

@@ -189,6 +189,31 @@ contains
     close(u)
   end subroutine
 
+  ! The full-precision sidecar conf_old.bin (a conf_new.bin renamed), if
+  ! present and of n particles: r0, v, a and the wrap counters as stored, so a
+  ! run resumed from it continues bit for bit.
+  subroutine read_state_sidecar(n, r0, v, a, mic, found)
+    integer, intent(in) :: n
+    real(c_double), intent(inout) :: r0(3, n), v(3, n), a(3, n)
+    integer(c_int32_t), intent(inout) :: mic(3, n)
+    logical, intent(out) :: found
+    integer :: u
+    integer(c_int32_t) :: m
+    inquire(file="conf_old.bin", exist=found)
+    if (.not. found) return
+    open(newunit=u, file="conf_old.bin", status="old", access="stream", form="unformatted", &
+         action="read")
+    read(u) m
+    if (m /= n) then
+      close(u)
+      print *, " * conf_old.bin holds ", m, " particles, not ", n, ": ignored"
+      found = .false.
+      return
+    end if
+    read(u) r0, v, a, mic
+    close(u)
+  end subroutine
+
   ! conf_new (store_config.f90:19-61 with WALL 2), plus a full-precision
   ! sidecar conf_new.bin (stream: n, r0, v, a, mic) for exact restarts.
   subroutine write_conf_new(inp, n, a_type, r0, v, a, mic)

@@ -6,7 +6,7 @@
 ! to pin the C oracle against the real reference; nothing ships it.
 !
 ! Run in a directory holding system_input, mfa_input and conf_old:
-!   ref_md <force_step> <n_steps> [trace]
+!   ref_md <force_step> <n_steps> [trace | snap <every>]
 ! writes
 !   ref_init.bin   the state, the cell of every particle, the pair list of
 !                  the first binning, and one force pass at r_time =
@@ -22,6 +22,8 @@
 ! float64: the sum of z over type-2 beads and the sum over all particles of
 ! the top wall's z force (the second half of fluid_wall.f90:107-121).
 ! Test infrastructure, used by the adjoint's finite-difference check.
+! With `snap <every>`, ref_snap.bin gets r0 and v as float64 (3, n_part)
+! after every <every>-th step (the statistics fixtures of the DPD runs).
 program ref_md
 #include "control_simulation.h"
   use commons
@@ -29,16 +31,29 @@ program ref_md
   character(len=32) :: arg
   integer :: force_step, n_steps, n_rebuilds, n_pairs, k, j, u, ut
   integer, allocatable :: cells(:,:)
-  logical :: trace
+  logical :: trace, snap
+  integer :: snap_every, us
   real(kind=8) :: zsum, ftop, izt
 
   call get_command_argument(1, arg)
   read(arg, *) force_step
   call get_command_argument(2, arg)
   read(arg, *) n_steps
-  trace = command_argument_count() >= 3
-  if (trace) open(newunit=ut, file="ref_trace.bin", access="stream", form="unformatted", &
-                  status="replace")
+  trace = .false.
+  snap = .false.
+  snap_every = 0
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    if (trim(arg) == "snap") then
+      snap = .true.
+      call get_command_argument(4, arg)
+      read(arg, *) snap_every
+      open(newunit=us, file="ref_snap.bin", access="stream", form="unformatted", status="replace")
+    else
+      trace = .true.
+      open(newunit=ut, file="ref_trace.bin", access="stream", form="unformatted", status="replace")
+    end if
+  end if
 
   call init_system()
   call init_params()
@@ -109,8 +124,12 @@ program ref_md
       end do
       write(ut) zsum, ftop
     end if
+    if (snap) then
+      if (mod(i_time, snap_every) == 0) write(us) r0(:, 1:n_part), v(:, 1:n_part)
+    end if
   end do
   if (trace) close(ut)
+  if (snap) close(us)
   open(newunit=u, file="ref_final.bin", access="stream", form="unformatted", status="replace")
   call put_state(u, n_rebuilds)
   close(u)

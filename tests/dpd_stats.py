@@ -1,0 +1,99 @@
+"""Statistics of DPD brush runs (test helper): the same code reduces the
+reference's snapshots (tests/golden/make_dpd_stats.py) and those of the
+oracle and the HIP path.
+
+The case: 16 chains of 12 beads grafted at z = 1 in an 8 x 8 x 6 channel
+with 960 ideal-DPD solvent beads (1152 in all), gamma 4.5, kT 1, dt 0.005.
+Run r starts from its own seeded configuration and noise (seed SEED + r),
+equilibrates N_EQ steps, then takes a snapshot every EVERY steps for N_RUN
+steps.  Observables, each a time average over one run:
+  temperature   sum m v^2 / (3 N_mobile), pinned heads excluded
+  rho_mon(z)    density profile of the brush monomers (type 2), bins of DZ
+  rho_sol(z)    density profile of the solvent (type 3)
+  p_mon(r)      distribution of monomer-monomer distances (minimum image in
+                x, y), bins of DR up to RMAX, normalised per pair
+Independent runs give honest error bars: the brush relaxes on the Rouse time
+of its chains (tens of tau), so snapshots within one run are correlated.
+"""
+import numpy as np
+
+from mfa_amd import configs, lib
+
+DZ = 0.25
+DR = 0.05
+RMAX = 3.0
+SEED = 77
+N_EQ = 2000
+N_RUN = 10000
+EVERY = 50
+OBSERVABLES = ("temperature", "rho_mon", "rho_sol", "p_mon")
+
+
+def case(run):
+    seed = SEED + run
+    x = configs.brush(16, 12, 960, 8.0, 8.0, 6.0, x_space=0.5, seed=seed)
+    x.skin = 0.0  # the reference's 0.4 (init_params.f90:800)
+    p = lib.derive(x)
+    r, v, t = lib.generate(p, 1, seed, head_z=1.0, bond=0.97, min_dist=0.85)
+    return x, p, r, v, t
+
+
+def checksum(r, v):
+    return float(np.sum(r * np.arange(1, r.size + 1).reshape(r.shape)) + np.sum(v))
+
+
+def reduce_run(snaps, t, box, n_mon, n_chain):
+    """Time averages of one run; snaps: list of (r, v) (N,3) arrays."""
+    lx, ly, lz = box
+    heads = np.zeros(len(t), dtype=bool)
+    heads[np.arange(0, n_mon * n_chain, n_mon)] = True
+    mob = ~heads
+    mon = (t == 2)
+    sol = (t == 3)
+    zbins = np.arange(0.0, lz + 1e-9, DZ)
+    rbins = np.arange(0.0, RMAX + 1e-9, DR)
+    iu = np.triu_indices(int(mon.sum()), 1)
+    temp, rm, rs, pr = [], [], [], []
+    for r, v in snaps:
+        temp.append(np.sum(v[mob] ** 2) / (3.0 * mob.sum()))
+        rm.append(np.histogram(r[mon, 2], bins=zbins)[0] / (lx * ly * DZ))
+        rs.append(np.histogram(r[sol, 2], bins=zbins)[0] / (lx * ly * DZ))
+        rmon = r[mon]
+        d = rmon[:, None, :] - rmon[None, :, :]
+        d[..., 0] -= lx * np.rint(d[..., 0] / lx)
+        d[..., 1] -= ly * np.rint(d[..., 1] / ly)
+        dist = np.sqrt((d ** 2).sum(axis=-1))[iu]
+        pr.append(np.histogram(dist, bins=rbins)[0] / len(dist))
+    return {"temperature": [float(np.mean(temp))], "rho_mon": np.mean(rm, axis=0).tolist(),
+            "rho_sol": np.mean(rs, axis=0).tolist(), "p_mon": np.mean(pr, axis=0).tolist()}
+
+
+def compare(runs, ref_runs, name):
+    """Mean over `runs` against the mean over `ref_runs` for one observable,
+    bin by bin, in units of the standard error of the difference with the
+    spread between the reference's runs taken for both sides.  Returns the
+    z scores of the bins where the reference's runs differ."""
+    a = np.array([r[name] for r in runs], dtype=np.float64)
+    b = np.array([r[name] for r in ref_runs], dtype=np.float64)
+    s = b.std(axis=0, ddof=1)
+    se = s * np.sqrt(1.0 / len(a) + 1.0 / len(b))
+    use = se > 0
+    return np.abs(a.mean(axis=0)[use] - b.mean(axis=0)[use]) / se[use]
+
+
+def check(runs, ref_runs, label, max_frac=0.15, max_chi2=2.0):
+    """The bars of the statistical tests: temperature within 2 sigma; for each
+    profile at most max_frac of the bins beyond 2 sigma and a mean chi^2 per
+    bin below max_chi2."""
+    z = compare(runs, ref_runs, "temperature")
+    t_a = np.mean([r["temperature"][0] for r in runs])
+    t_b = np.mean([r["temperature"][0] for r in ref_runs])
+    print("%s: T %.4f against reference %.4f (%.2f sigma)" % (label, t_a, t_b, z[0]), flush=True)
+    ok = bool(z[0] < 2.0)
+    for name in ("rho_mon", "rho_sol", "p_mon"):
+        z = compare(runs, ref_runs, name)
+        frac, chi2 = float(np.mean(z > 2.0)), float(np.mean(z ** 2))
+        print("  %-8s bins %3d beyond 2 sigma %.3f chi2/bin %.2f" % (name, len(z), frac, chi2),
+              flush=True)
+        ok = ok and frac <= max_frac and chi2 < max_chi2
+    return ok

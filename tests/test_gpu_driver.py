@@ -50,3 +50,34 @@ def test_fortran_driver_matches_oracle(tmp_path, case, mode, nsteps):
     # conf_new text is the reference format and round-trips
     t3, r3, _, _ = io.read_conf(os.path.join(d, "conf_new"), p.n_part)
     assert np.array_equal(t3, t2) and np.abs(r3 - out["r"]).max() <= 5.1e-8
+
+
+@pytest.mark.parametrize("mode", ["compat", "fused"])
+def test_restart_from_sidecar_is_bit_exact(tmp_path, mode):
+    """40 steps in one run against 20 steps, a restart from conf_new /
+    conf_new.bin (renamed conf_old / conf_old.bin, s_time moved on by 20) and
+    20 more: mixed form at kT = 1, so the DPD noise, keyed by the step, must
+    resume too.  The final states agree bit for bit."""
+    assert os.path.exists(PROG), "Fortran driver not built (python -m mfa_amd.build)"
+    kind, inp = configs.small_cases()["brush_small"]
+    p, r, v, t = configs.build(inp, kind)
+    inp.n_relax, inp.n_safe = 0, 20
+
+    def run(d, nsteps, s_time, conf=None):
+        inp.n_obser = nsteps
+        inp.s_time = s_time
+        io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="mixed", mode=mode)
+        if conf is not None:
+            for src, dst in (("conf_new", "conf_old"), ("conf_new.bin", "conf_old.bin")):
+                with open(os.path.join(conf, src), "rb") as f, open(os.path.join(d, dst), "wb") as g:
+                    g.write(f.read())
+        res = subprocess.run([PROG], cwd=d, capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stdout + res.stderr
+        return io.read_state_sidecar(os.path.join(d, "conf_new.bin"))
+
+    print("restart", mode, flush=True)
+    whole = run(str(tmp_path / "whole"), 40, 2)
+    run(str(tmp_path / "first"), 20, 2)
+    second = run(str(tmp_path / "second"), 20, 22, conf=str(tmp_path / "first"))
+    for k in ("r", "v", "a", "mic"):
+        assert np.array_equal(whole[k], second[k]), k
