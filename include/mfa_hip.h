@@ -93,6 +93,8 @@ typedef struct mfa_inputs {
   double rc_coul;       /* >0: real-space Coulomb cutoff; 0 = LJ range (P5)  */
   int32_t ewald_compat; /* 1: reference's 1/pi real-space factor (bug)      */
   int32_t force_switch_on; /* FORCE_SWITCH_ON defined: ramp when s_time == 1 */
+  int32_t dpd_vv;       /* DPD_VV defined: new_dpd_fd after verlet_velocities */
+  int32_t pad2;
 } mfa_inputs;
 
 /*
@@ -136,6 +138,7 @@ typedef struct mfa_params {
   /* FORCE_SWITCH_ON ramp (init_force_switch_on.f90:15-26): r_2 >= r_2_min with
    * r_2_min = (1 - t/fso_time) * fso_r2_init while t = step*dt < fso_time. */
   double fso_r2_init, fso_time;
+  int32_t dpd_vv, pad1; /* DPD_VV (md_main.f90:149-155) */
 } mfa_params;
 
 /* Observation sums returned by mfa_run_steps / mfa_energies. */
@@ -225,6 +228,10 @@ int32_t mfa_intra_molec(mfa_ctx* ctx, double* v_intra);
 int32_t mfa_bending(mfa_ctx* ctx, int32_t mode, double* v_bend);
 /* verlet_velocities (verlet_velocities.f90:18-199); *t_kin = 0.5 sum m v^2. */
 int32_t mfa_verlet_velocities(mfa_ctx* ctx, double* t_kin);
+/* md_main.f90:149-155 with DPD_VV: new_dpd_fd.f90:18-124, the dissipative
+ * force recomputed with the new velocities, swapped into F (F - F_D,old +
+ * F_D,new), a further half kick and a = F/m; t_kin as new_dpd_fd computes it. */
+int32_t mfa_new_dpd_fd(mfa_ctx* ctx, double* t_kin);
 
 /* Whole loop body (md_main.f90:69-155) for n steps, steps first_step ..
  * first_step+n-1, device resident: fused verlet_velocities + next

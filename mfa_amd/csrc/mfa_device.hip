@@ -92,6 +92,7 @@ struct KParams {
   int p0, c0, rep;
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
   int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
+  int dpd_vv;  // DPD_VV: heads move, new_dpd_fd after verlet_velocities
   uint32_t seed;
 };
 
@@ -134,6 +135,8 @@ struct mfa_ctx {
   unsigned* nl;
   int32_t* nn;
   float4 *pos4, *vel4;  // FP32 view of the mixed form
+  double *fd, *fd_new;  // DPD_VV: dissipative force of the last new_dpd_fd, and the new one
+  bool dpd_vv;
   // Ewald k space (charged systems)
   bool ewald;
   EwaldK ek;
@@ -325,7 +328,9 @@ __device__ __forceinline__ void vv_one(const KParams& k, int i, int ti, double* 
     a[d] = f[d] * im;
     v[d] = v[d] + k.half_dt * a[d];
   }
-  if (is_head(k, i)) {
+  // with DPD_VV the head block of verlet_velocities is compiled out
+  // (verlet_velocities.f90:79-230 is #ifndef DPD_VV)
+  if (is_head(k, i) && !k.dpd_vv) {
     const int c = i / k.n_mon;
     const double sgn = c < k.n_chain / 2 ? 1.0 : -1.0;
     v[0] = sgn * k.va[0];
@@ -453,7 +458,7 @@ __global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* _
       a[3 * (size_t)i + d] = ai[d];
     }
     vel4[sidx[i]] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
-    if (is_head(k, i - k.p0))
+    if (is_head(k, i - k.p0) && !k.dpd_vv)
       for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] = 0.0;
     vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
   }
@@ -577,6 +582,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   const int per = (k.n_cells + MFA_BLOCK - 1) / MFA_BLOCK;
   const int cb = threadIdx.x * per, ce = min(k.n_cells, cb + per);
   int tot = 0;
+#pragma unroll 8
   for (int c = cb; c < ce; ++c) tot += min(cell_count[k.c0 + c], k.cell_cap);
   int x = tot;  // inclusive scan of the thread sums within the wave
   for (int o = 1; o < 64; o <<= 1) {
@@ -587,6 +593,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   __syncthreads();
   int run = carry_s + x - tot;
   for (int w = 0; w < wid; ++w) run += wsum[w];
+#pragma unroll 8
   for (int c = cb; c < ce; ++c) {
     cell_start[k.c0 + c] = run;
     run += min(cell_count[k.c0 + c], k.cell_cap);
@@ -1681,6 +1688,105 @@ __global__ void k_ewald_energy(const double* __restrict__ uk_part, int nparts,
 }
 
 // ---------------------------------------------------------------------------
+// DPD_VV (md_main.f90:149-155, new_dpd_fd.f90:18-177, DPD_WEIGHT 0): after
+// verlet_velocities the dissipative force is computed again with the new
+// velocities, for list pairs inside range_2 with neither bead a head (type
+// 1), images in x and y only (new_dpd_fd.f90:38-44), w_d =
+// (1 - sqrt(r_2/range_2))^2; F_D,ij = -gamma w_d (dv.e) e.  Each particle
+// gathers its own terms from its full list row (sorted order, TPP lanes as
+// in the force pass); F_D,ji = -F_D,ij bit for bit.  FP64 in both forms.
+template <int TPP>
+__global__ __launch_bounds__(MFA_BLOCK) void k_dpd_fd(const KParams* __restrict__ kb,
+                                                      const int* __restrict__ rmap,
+                                                      const double* __restrict__ r,
+                                                      const double* __restrict__ v,
+                                                      const int* __restrict__ type,
+                                                      const int* __restrict__ perm,
+                                                      const unsigned* __restrict__ nl,
+                                                      const int* __restrict__ nn, int cap,
+                                                      double* __restrict__ fd_new) {
+  MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sp = gtid / TPP, sub = gtid % TPP;
+  const bool active = sp - k.p0 < k.n;
+  const int ss = active ? sp : k.p0;
+  const int i = perm[ss];
+  const int ti = type[i] - 1;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (active && ti != 0) {
+    const double xi = r[3 * (size_t)i + 0], yi = r[3 * (size_t)i + 1], zi = r[3 * (size_t)i + 2];
+    const double vxi = v[3 * (size_t)i + 0], vyi = v[3 * (size_t)i + 1], vzi = v[3 * (size_t)i + 2];
+    const int cnt = nn[ss];
+    const unsigned* row = nl + (size_t)ss * cap;
+    for (int q = sub; q < cnt; q += TPP) {
+      const int j = perm[row[q] & MFA_JMASK];
+      const int tj = type[j] - 1;
+      if (tj == 0) continue;
+      double dx = pbc(xi - r[3 * (size_t)j + 0], k.L[0], k.invL[0]);
+      double dy = pbc(yi - r[3 * (size_t)j + 1], k.L[1], k.invL[1]);
+      const double dz = zi - r[3 * (size_t)j + 2];
+      const double r2 = (dx * dx + dy * dy) + dz * dz;
+      if (!(r2 < k.range_2[ti][tj])) continue;
+      const double wr = 1.0 - sqrt(r2 / k.range_2[ti][tj]);
+      const double wd = wr * wr;
+      const double rn = sqrt(r2);
+      const double ex = dx / rn, ey = dy / rn, ez = dz / rn;
+      const double rd = ((vxi - v[3 * (size_t)j + 0]) * ex + (vyi - v[3 * (size_t)j + 1]) * ey) +
+                        (vzi - v[3 * (size_t)j + 2]) * ez;
+      const double c = (-1.0 * k.friction1) * wd * rd;
+      ax += c * ex;
+      ay += c * ey;
+      az += c * ez;
+    }
+  }
+  ax = group_sum<TPP>(ax);
+  ay = group_sum<TPP>(ay);
+  az = group_sum<TPP>(az);
+  if (active && sub == 0) {
+    fd_new[3 * (size_t)i + 0] = ax;
+    fd_new[3 * (size_t)i + 1] = ay;
+    fd_new[3 * (size_t)i + 2] = az;
+  }
+}
+
+// The rest of new_dpd_fd (:18-22,103-124,148-176): F = (F - F_D,old) + F_D,new,
+// F_D,old <- F_D,new, v += (dt/2) F/m, a = F/m; the view's velocity; the
+// kinetic energy partials; with inc_step the step counter moves on (this
+// kernel then ends the step, as the fused integrate kernel does otherwise).
+__global__ __launch_bounds__(MFA_BLOCK) void k_dpd_vv_apply(
+    const KParams* __restrict__ kb, const int* __restrict__ rmap, double* __restrict__ f,
+    double* __restrict__ fd, const double* __restrict__ fd_new, double* __restrict__ v,
+    double* __restrict__ a, const int* __restrict__ type, const int* __restrict__ sidx,
+    float4* __restrict__ vel4, double* __restrict__ part, int energy,
+    long long* __restrict__ d_step, int inc_step) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inc_step && i == 0) *d_step += 1;
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+  if (i - k.p0 < k.n) {
+    const int ti = type[i] - 1;
+    const double im = k.inv_mass_type[ti];
+    double vi[3];
+    double tk = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const size_t q = 3 * (size_t)i + d;
+      const double fn = fd_new[q];
+      const double fi = (f[q] - fd[q]) + fn;
+      fd[q] = fn;
+      f[q] = fi;
+      vi[d] = v[q] + (k.half_dt * fi) * im;
+      v[q] = vi[d];
+      a[q] = fi * im;
+      tk += k.mass_type[ti] * (vi[d] * vi[d]);
+    }
+    vel4[sidx[i]] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
+    vals[T_TKIN] = tk;
+  }
+  if (energy) block_partials(vals, part);
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 
 static KParams make_kparams(const mfa_params& p) {
@@ -1780,6 +1886,7 @@ static KParams make_kparams(const mfa_params& p) {
   // DPD_CUT_OFF excludes heads for brush systems in a channel (fluid_fluid.f90:179-185).
   k.dpd_head_excl = (p.system == 3 && p.symmetry == 0) ? 1 : 0;
   k.seed = (uint32_t)p.seed;
+  k.dpd_vv = p.dpd_vv;
   return k;
 }
 
@@ -1994,6 +2101,32 @@ static void drop_graph(mfa_ctx* c) {
 
 // One step after its verlet_positions: rebuild when due, force pass, and
 // verlet_velocities fused with the next step's verlet_positions.
+// new_dpd_fd (DPD_VV): the recomputed dissipative force and its application.
+template <int TPP>
+static void launch_dpd_fd_t(mfa_ctx* c) {
+  hipLaunchKernelGGL((k_dpd_fd<TPP>), dim3(grid_of(c->n * TPP)), dim3(MFA_BLOCK), 0, c->stream,
+                     c->d_kp, c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap,
+                     c->fd_new);
+}
+static int launch_dpd_vv(mfa_ctx* c, int energy, int inc_step) {
+  switch (c->tpp) {
+    case 1: launch_dpd_fd_t<1>(c); break;
+    case 2: launch_dpd_fd_t<2>(c); break;
+    case 4: launch_dpd_fd_t<4>(c); break;
+    case 8: launch_dpd_fd_t<8>(c); break;
+    case 16: launch_dpd_fd_t<16>(c); break;
+    default: return set_err(c, MFA_ERR_ARG, "threads per particle %d unsupported", c->tpp);
+  }
+  hipLaunchKernelGGL(k_dpd_vv_apply, dim3(grid_of(c->n)), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                     c->d_rmap, c->f, c->fd, c->fd_new, c->v, c->a, c->type, c->sidx, c->vel4,
+                     c->part, energy, c->d_step, inc_step);
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
+}
+
+// One step after its verlet_positions: rebuild when due, force pass, and
+// verlet_velocities fused with the next step's verlet_positions (with
+// DPD_VV: verlet_velocities, new_dpd_fd, then the next verlet_positions).
 static int enqueue_middle_step(mfa_ctx* c) {
   int rc = launch_rebuild(c);
   if (rc) return rc;
@@ -2001,7 +2134,17 @@ static int enqueue_middle_step(mfa_ctx* c) {
   if (rc) return rc;
   rc = launch_ewald(c, c->p.symmetry == 0);  // md_main.f90:128-132
   if (rc) return rc;
-  return launch_fused(c, 0);
+  if (!c->dpd_vv) return launch_fused(c, 0);
+  const int g = grid_of(c->n);
+  hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->f, c->v, c->a, c->type, c->part, 0, c->sidx, c->vel4);
+  rc = launch_dpd_vv(c, 0, 1);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
+                     c->r, c->v, c->a, c->mic, c->r_ref, c->flags, 1, c->n_cells, c->cell_count,
+                     c->type, c->cell_of, c->sidx, c->pos4, c->vel4);
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
 }
 
 // Captures graph_steps middle steps once per (precision, threads per particle).
@@ -2249,7 +2392,8 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
     const mfa_params& p = ps[q];
     if (p.n_part <= 0) return set_err(nullptr, MFA_ERR_ARG, "replica %d: n_part must be positive", q);
     if (n_rep > 1 && (p.system != ps[0].system || p.symmetry != ps[0].symmetry ||
-                      p.thermostat != ps[0].thermostat || p.dpd_cut_off_on != ps[0].dpd_cut_off_on))
+                      p.thermostat != ps[0].thermostat || p.dpd_cut_off_on != ps[0].dpd_cut_off_on ||
+                      p.dpd_vv != ps[0].dpd_vv))
       return set_err(nullptr, MFA_ERR_ARG, "replica %d: switches differ from replica 0", q);
     if (n_rep > 1 && p.coulomb)
       return set_err(nullptr, MFA_ERR_ARG, "charged systems run one per context");
@@ -2261,6 +2405,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->p = ps[0];
   c->n_rep = n_rep;
   c->extra_terms = false;
+  c->dpd_vv = ps[0].dpd_vv != 0;
   for (int q = 0; q < n_rep; ++q) c->extra_terms |= ps[q].coulomb != 0 || ps[q].bending != 0;
   c->ps.assign(ps, ps + n_rep);
   c->precision = precision;
@@ -2342,6 +2487,10 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   ALLOC(c->cell_slots, ncs * sizeof(int32_t));
   ALLOC(c->cell_start, ((size_t)c->n_cells + 1) * sizeof(int32_t));
   ALLOC(c->perm, n * sizeof(int32_t));
+  if (c->dpd_vv) {
+    ALLOC(c->fd, n * 3 * sizeof(double));
+    ALLOC(c->fd_new, n * 3 * sizeof(double));
+  }
   ALLOC(c->sidx, n * sizeof(int32_t));
   ALLOC(c->nl, n * (size_t)c->cap * sizeof(unsigned));
   ALLOC(c->nn, n * sizeof(int32_t));
@@ -2405,6 +2554,10 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->f, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->a, 0, n * 3 * sizeof(double), c->stream);
+  if (c->dpd_vv) {
+    hipMemsetAsync(c->fd, 0, n * 3 * sizeof(double), c->stream);
+    hipMemsetAsync(c->fd_new, 0, n * 3 * sizeof(double), c->stream);
+  }
   hipMemsetAsync(c->nn, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->cell_of, 0, n * sizeof(int32_t), c->stream);
   {
@@ -2440,7 +2593,8 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   void* ptrs[] = {c->r,   c->v,       c->a,          c->f,          c->r_ref, c->type,
                   c->mic, c->cell_of, c->cell_count, c->cell_slots, c->nl,    c->nn,
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
-                  c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx};
+                  c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx,
+                  c->fd,   c->fd_new};
   for (void* q : ptrs)
     if (q) hipFree(q);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
@@ -2500,6 +2654,8 @@ static int32_t set_state_impl(mfa_ctx* c, int q, const double* r0, const double*
   else
     HIPCHK(c, hipMemsetAsync(c->mic + 3 * o, 0, n * 3 * sizeof(int32_t), c->stream));
   HIPCHK(c, hipMemsetAsync(c->flags, 0, (size_t)c->n_rep * F_NFLAGS * sizeof(int32_t), c->stream));
+  if (c->dpd_vv)  // force_d starts at zero (init_params.f90:839)
+    HIPCHK(c, hipMemsetAsync(c->fd + 3 * o, 0, n * 3 * sizeof(double), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->need_rebuild = true;
   if (c->ewald) {
@@ -2724,6 +2880,23 @@ extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
   return MFA_OK;
 }
 
+extern "C" int32_t mfa_new_dpd_fd(mfa_ctx* c, double* t_kin) {
+  if (!c) return MFA_ERR_ARG;
+  if (!c->dpd_vv) return set_err(c, MFA_ERR_ARG, "new_dpd_fd: DPD_VV is not set");
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  rc = launch_dpd_vv(c, t_kin ? 1 : 0, 0);
+  if (rc) return rc;
+  if (t_kin) {
+    double t[MFA_NTERMS];
+    rc = reduce_terms_particles(c, t);
+    if (rc) return rc;
+    *t_kin = 0.5 * t[T_TKIN];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
 static void fill_obs(const double* t, double tkin_sum, int64_t nreb, mfa_obs* obs) {
   memset(obs, 0, sizeof *obs);
   obs->v_fluid_fluid = t[T_VFF];
@@ -2865,8 +3038,13 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, obs ? 1 : 0, c->sidx, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, (obs && !c->dpd_vv) ? 1 : 0, c->sidx,
+                     c->vel4);
   HIPCHK(c, hipGetLastError());
+  if (c->dpd_vv) {  // md_main.f90:149-155; the kinetic energy comes from new_dpd_fd
+    rc = launch_dpd_vv(c, obs ? 1 : 0, 0);
+    if (rc) return rc;
+  }
   if (obs) {
     hipLaunchKernelGGL(k_reduce_replicas, dim3(MFA_NTERMS, c->n_rep), dim3(MFA_BLOCK), 0, c->stream,
                        c->part, c->d_kp, MFA_BLOCK, c->red);
@@ -3410,6 +3588,7 @@ extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_
   if (c->n_rep != 1) return set_err(c, MFA_ERR_ARG, "adjoint: one replica per context");
   if (p.coulomb || p.bending) return set_err(c, MFA_ERR_ARG, "adjoint: charges and bending are not covered");
   if (p.fso_time > 0.0) return set_err(c, MFA_ERR_ARG, "adjoint: the switch-on ramp is not covered");
+  if (c->dpd_vv) return set_err(c, MFA_ERR_ARG, "adjoint: DPD_VV is not covered");
   if (p.symmetry != 0) return set_err(c, MFA_ERR_ARG, "adjoint: the observables need the walls (SYMMETRY 0)");
   // theta: the common epsilon and sigma of the tables
   AdjK ak;

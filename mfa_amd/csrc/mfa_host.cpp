@@ -52,6 +52,8 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   if (in->system == 3 && (in->n_chain % 2) != 0)
     return derive_fail("n_chain is not divisible by 2");
 
+  if (in->dpd_vv && in->thermostat != 0) return derive_fail("DPD_VV needs the DPD thermostat");
+  p->dpd_vv = in->dpd_vv ? 1 : 0;
   p->system = in->system;
   p->symmetry = in->symmetry;
   p->thermostat = in->thermostat;

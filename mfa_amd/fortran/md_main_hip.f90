@@ -64,6 +64,9 @@ contains
   subroutine verlet_velocities()
     call mfa_check(mfa_verlet_velocities(ctx, t_fluid), "verlet_velocities", ctx)
   end subroutine
+  subroutine new_dpd_fd()
+    call mfa_check(mfa_new_dpd_fd(ctx, t_fluid), "new_dpd_fd", ctx)
+  end subroutine
 
   subroutine fetch_state()
     call mfa_check(mfa_get_state(ctx, r0, v, a, force, mic_count), "get_state", ctx)
@@ -169,6 +172,7 @@ program md_pb_hip
         if (inp%symmetry == 0) call dipolar_correction()
       end if
       call verlet_velocities()
+      if (inp%dpd_vv /= 0) call new_dpd_fd()           ! md_main.f90:149-155
       if (mod(i_time, 10) == 0) call write_fort60(i_time, v_fluid_wall, v_fluid_fluid, &
                                                    v_intra_molec, t_fluid, v_coul)
       if (mod(i_time, inp%n_safe) == 0) call store(.true.)

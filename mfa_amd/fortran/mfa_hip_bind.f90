@@ -32,6 +32,7 @@ module mfa_hip_bind
     integer(c_int32_t) :: particle_4 = 0, bending = 0
     real(c_double) :: dpd_cut_off = 0, skin = 0, lj_rc = 0, rc_coul = 0
     integer(c_int32_t) :: ewald_compat = 0, force_switch_on = 0
+    integer(c_int32_t) :: dpd_vv = 0, pad2 = 0
   end type
 
   type, bind(C) :: mfa_params_t
@@ -60,6 +61,7 @@ module mfa_hip_bind
     integer(c_int32_t) :: n_kmax, pad0
     integer(c_int64_t) :: seed
     real(c_double) :: fso_r2_init, fso_time
+    integer(c_int32_t) :: dpd_vv, pad1
   end type
 
   type, bind(C) :: mfa_obs_t
@@ -157,6 +159,11 @@ module mfa_hip_bind
       real(c_double) :: v_intra
     end function
     integer(c_int32_t) function mfa_verlet_velocities(ctx, t_kin) bind(C, name="mfa_verlet_velocities")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: t_kin
+    end function
+    integer(c_int32_t) function mfa_new_dpd_fd(ctx, t_kin) bind(C, name="mfa_new_dpd_fd")
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double) :: t_kin

@@ -51,6 +51,7 @@ contains
       case ("rc_coul");         read(val, *) inp%rc_coul
       case ("ewald_compat");    read(val, *) inp%ewald_compat
       case ("force_switch_on"); read(val, *) inp%force_switch_on
+      case ("dpd_vv");          read(val, *) inp%dpd_vv
       case ("list_capacity");   read(val, *) list_capacity
       case ("mode");            run_mode = trim(val)
       case ("precision")

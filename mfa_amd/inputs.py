@@ -73,7 +73,7 @@ def write_mfa_input(inp, path):
 
 def write_hip_options(inp, path, precision="fp64", mode="compat"):
     keys = ["system", "symmetry", "solvent", "thermostat", "particle_4", "bending",
-            "dpd_cut_off", "skin", "lj_rc", "rc_coul", "ewald_compat", "force_switch_on"]
+            "dpd_cut_off", "skin", "lj_rc", "rc_coul", "ewald_compat", "force_switch_on", "dpd_vv"]
     L = ["# control_simulation.h switches as run-time options"]
     for k in keys:
         val = getattr(inp, k)

@@ -47,6 +47,7 @@ class Inputs(C.Structure):
         ("thermostat", C.c_int32), ("particle_4", C.c_int32), ("bending", C.c_int32),
         ("dpd_cut_off", C.c_double), ("skin", C.c_double), ("lj_rc", C.c_double),
         ("rc_coul", C.c_double), ("ewald_compat", C.c_int32), ("force_switch_on", C.c_int32),
+        ("dpd_vv", C.c_int32), ("pad2", C.c_int32),
     ]
 
 
@@ -76,6 +77,7 @@ class Params(C.Structure):
         ("inv_sqrt_pi_real", C.c_double), ("rc_coul_2", C.c_double),
         ("v_coul_self", C.c_double), ("n_kmax", C.c_int32), ("pad0", C.c_int32),
         ("seed", C.c_uint64), ("fso_r2_init", C.c_double), ("fso_time", C.c_double),
+        ("dpd_vv", C.c_int32), ("pad1", C.c_int32),
     ]
 
 
@@ -133,6 +135,7 @@ SIGNATURES = {
     "mfa_replica_obs": (C.c_int32, [_P, C.c_int32, C.POINTER(Obs)]),
     "mfa_autotune": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _dp]),
     "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
+    "mfa_new_dpd_fd": (C.c_int32, [_P, _dp]),
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
     "mfa_timing": (C.c_int32, [_P, C.c_int32, _dp, _lp, _dp, _lp]),
@@ -354,6 +357,11 @@ class Context:
     def verlet_velocities(self):
         t = C.c_double()
         self._chk(self.lib.mfa_verlet_velocities(self.h, C.byref(t)), "mfa_verlet_velocities")
+        return t.value
+
+    def new_dpd_fd(self):
+        t = C.c_double()
+        self._chk(self.lib.mfa_new_dpd_fd(self.h, C.byref(t)), "mfa_new_dpd_fd")
         return t.value
 
     def run_steps(self, first_step, n, want_obs=True):

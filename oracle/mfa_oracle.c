@@ -21,6 +21,7 @@ struct orc_sys {
   int n_neigh;   /* n_neigh_fl (mfa_common.f90:78) */
   int n_bin_fl;  /* mfa_common.f90:76 */
   double *r0, *v, *a, *force, *r0_old;
+  double* force_d; /* DPD_VV: the dissipative force of the last new_dpd_fd (mfa_common.f90:197) */
   int32_t *a_type, *mic, *mic_old;
   int32_t* ff_list;    /* [N][0..n_neigh]: row 0 is the count, rows hold 1-based ids */
   int32_t* bin_fluid;  /* [cells][0..n_bin_fl] */
@@ -250,6 +251,7 @@ int orc_derive(const mfa_inputs* in, orc_params* p) {
   p->n_kmax = in->n_kmax;
   p->rc_coul_2 = (!in->ewald_compat && in->rc_coul > 0.0) ? in->rc_coul * in->rc_coul : 0.0;
   p->bending = in->bending;
+  p->dpd_vv = in->dpd_vv ? 1 : 0;
   p->k_bend = in->k_bend;
   p->alpha_eq = in->alpha_eq;
   /* init_config.f90:264-281 and init_force_switch_on.f90:15-26 */
@@ -339,6 +341,7 @@ orc_sys* orc_create(const orc_params* p, int n_neigh) {
   s->v = (double*)calloc(3 * n, sizeof(double));
   s->a = (double*)calloc(3 * n, sizeof(double));
   s->force = (double*)calloc(3 * n, sizeof(double));
+  s->force_d = (double*)calloc(3 * n, sizeof(double)); /* init_params.f90:839 */
   s->r0_old = (double*)calloc(3 * n, sizeof(double));
   s->a_type = (int32_t*)calloc(n, sizeof(int32_t));
   s->mic = (int32_t*)calloc(3 * n, sizeof(int32_t));
@@ -354,7 +357,7 @@ orc_sys* orc_create(const orc_params* p, int n_neigh) {
 
 void orc_destroy(orc_sys* s) {
   if (!s) return;
-  free(s->r0); free(s->v); free(s->a); free(s->force); free(s->r0_old);
+  free(s->r0); free(s->v); free(s->a); free(s->force); free(s->force_d); free(s->r0_old);
   free(s->a_type); free(s->mic); free(s->mic_old); free(s->ff_list); free(s->bin_fluid);
   free(s->cell_of);
   free(s->q);
@@ -892,6 +895,9 @@ double orc_verlet_velocities(orc_sys* s) {
     for (int d = 0; d < 3; ++d) A(i, d) = F(i, d) * inv_mass;
     for (int d = 0; d < 3; ++d) V(i, d) = V(i, d) + 0.5 * p->dt * A(i, d);
   }
+  /* with DPD_VV the head block and the energies are left to new_dpd_fd
+   * (verlet_velocities.f90:79-230 is #ifndef DPD_VV) */
+  if (p->dpd_vv) return 0.;
   if (p->symmetry == 0 && (p->system == 0 || p->system == 3)) {
     for (int c = 0; c < p->n_chain / 2; ++c) {
       const int h = p->n_mon * c;
@@ -914,6 +920,65 @@ double orc_verlet_velocities(orc_sys* s) {
     t_inst = t_inst + m * (V(i, 0) * V(i, 0) + V(i, 1) * V(i, 1) + V(i, 2) * V(i, 2));
   }
   return 0.5 * t_inst;
+}
+
+/* new_dpd_fd.f90:18-177 (DPD_VV, DPD_WEIGHT 0, WALL 2): the dissipative
+ * force recomputed with the velocities after verlet_velocities over the
+ * half list, heads (type 1) excluded, inside range_2 with w_d =
+ * (1 - sqrt(r_2/range_2))^2 and images in x and y only (:38-44); swapped
+ * into the force (force - force_d_old + force_d_new), then a further half
+ * kick v += 0.5 dt F/m and a = F/m (:106-124).  Returns t_fluid = sum m v^2 / 2,
+ * summed dimension by dimension (:148-176).  Pinned against the reference's
+ * own DPD_VV build (tests/test_ref_pin.py). */
+double orc_new_dpd_fd(orc_sys* s) {
+  const orc_params* p = &s->p;
+  const int ls = s->n_neigh + 1;
+  const size_t n3 = 3 * (size_t)s->n;
+  for (size_t q = 0; q < n3; ++q) s->force[q] = s->force[q] - s->force_d[q];
+  for (size_t q = 0; q < n3; ++q) s->force_d[q] = 0.;
+  for (int ip = 0; ip < s->n; ++ip) {
+    const int ti = s->a_type[ip] - 1;
+    if (ti == 0) continue;
+    for (int k = 1; k <= s->ff_list[(size_t)ip * ls]; ++k) {
+      const int jp = s->ff_list[(size_t)ip * ls + k] - 1;
+      const int tj = s->a_type[jp] - 1;
+      if (tj == 0) continue;
+      double dr[3];
+      for (int d = 0; d < 3; ++d) dr[d] = R(ip, d) - R(jp, d);
+      dr[0] = pbc(dr[0], p->boundary[0], p->inv_boundary[0]);
+      dr[1] = pbc(dr[1], p->boundary[1], p->inv_boundary[1]);
+      const double r_2 = (dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2];
+      if (!(r_2 < p->range_2[ti][tj])) continue;
+      const double wr = 1. - sqrt(r_2 / p->range_2[ti][tj]);
+      const double w_d = wr * wr;
+      const double rn = sqrt(r_2);
+      double rv[3];
+      for (int d = 0; d < 3; ++d) rv[d] = dr[d] / rn;
+      const double r_dummy = ((V(ip, 0) - V(jp, 0)) * rv[0] + (V(ip, 1) - V(jp, 1)) * rv[1]) +
+                             (V(ip, 2) - V(jp, 2)) * rv[2];
+      for (int d = 0; d < 3; ++d) {
+        const double vec = ((-1. * p->friction1) * w_d * r_dummy) * rv[d];
+        s->force_d[3 * (size_t)ip + d] = s->force_d[3 * (size_t)ip + d] + vec;
+        s->force_d[3 * (size_t)jp + d] = s->force_d[3 * (size_t)jp + d] - vec;
+      }
+    }
+  }
+  for (size_t q = 0; q < n3; ++q) s->force[q] = s->force[q] + s->force_d[q];
+  for (int i = 0; i < s->n; ++i) {
+    const double inv_mass = 1. / p->mass_type[s->a_type[i] - 1];
+    for (int d = 0; d < 3; ++d) V(i, d) = V(i, d) + ((0.5 * p->dt) * F(i, d)) * inv_mass;
+  }
+  for (int i = 0; i < s->n; ++i) {
+    const double inv_mass = 1. / p->mass_type[s->a_type[i] - 1];
+    for (int d = 0; d < 3; ++d) A(i, d) = F(i, d) * inv_mass;
+  }
+  double t_fluid = 0.;
+  for (int d = 0; d < 3; ++d)
+    for (int i = 0; i < s->n; ++i) {
+      const double m = p->mass_type[s->a_type[i] - 1];
+      t_fluid = t_fluid + m * (V(i, d) * V(i, d));
+    }
+  return t_fluid / 2.;
 }
 
 /* md_main.f90:69-155 for SYSTEM 2/3, WALL 2, THERMOSTAT 0 or none, BIN_TYPE 0. */
@@ -944,6 +1009,7 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
       if (s->p.symmetry == 0) v_coul = v_coul + orc_dipolar(s);
     }
     tk = orc_verlet_velocities(s);
+    if (s->p.dpd_vv) tk = orc_new_dpd_fd(s); /* md_main.f90:149-155 */
   }
   if (obs) {
     obs[0] = e_ff[0];

@@ -49,6 +49,8 @@ typedef struct orc_params {
   /* BENDING (init_params.f90:211-213) */
   int bending;
   double k_bend, alpha_eq;
+  /* DPD_VV (md_main.f90:149-155) */
+  int dpd_vv;
 } orc_params;
 
 typedef struct orc_sys orc_sys;
@@ -87,6 +89,7 @@ void orc_fluid_wall(orc_sys* s, double* out);
 /* out[0]=v_intra_molec, out[1]=virial contribution */
 void orc_intra_molec(orc_sys* s, double* out);
 double orc_verlet_velocities(orc_sys* s);
+double orc_new_dpd_fd(orc_sys* s);
 /* Loop body of md_main.f90:69-155 for n steps.  obs[0..8]: v_ff, v_fw,
  * v_intra, t_kin, virial, n_rebuilds, wall Fz bottom, v_coul, v_bend of the
  * last step. */

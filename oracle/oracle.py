@@ -44,6 +44,7 @@ class OrcParams(C.Structure):
         ("charge", _d4), ("alpha_c", C.c_double), ("sqrt_alpha", C.c_double),
         ("inv_sqrt_pi", C.c_double), ("rc_coul_2", C.c_double),
         ("bending", C.c_int), ("k_bend", C.c_double), ("alpha_eq", C.c_double),
+        ("dpd_vv", C.c_int),
     ]
 
 
@@ -92,6 +93,8 @@ def load():
     lib.orc_intra_molec.argtypes = [P, dp]
     lib.orc_verlet_velocities.argtypes = [P]
     lib.orc_verlet_velocities.restype = C.c_double
+    lib.orc_new_dpd_fd.argtypes = [P]
+    lib.orc_new_dpd_fd.restype = C.c_double
     lib.orc_run_steps.argtypes = [P, C.c_int64, C.c_int, dp]
     lib.orc_run_steps.restype = C.c_int
     lib.orc_ewald_k.argtypes = [P]
@@ -215,6 +218,9 @@ class System:
 
     def verlet_velocities(self):
         return self.lib.orc_verlet_velocities(self.h)
+
+    def new_dpd_fd(self):
+        return self.lib.orc_new_dpd_fd(self.h)
 
     def run_steps(self, first_step, n):
         obs = np.zeros(9)

@@ -25,9 +25,9 @@ SOURCES="mfa_common ziggurat R250 INR250 SUN util functions messages \
   dpd_forces lgv_forces ewald_real ewald_k dipolar_correction fluid_fluid \
   fluid_wall intra_molec bending init_force_switch_on init_system init_params \
   gen_brush gen_chain gen_wall gen_droplet conf_default init_config init_obser \
-  obser_out observation store_config fix_CM constant_force geom_constraint"
+  obser_out observation store_config fix_CM constant_force geom_constraint new_dpd_fd"
 
-for cfg in brush brush_q brush_bend melt melt_fso; do
+for cfg in brush brush_q brush_bend brush_vv melt melt_fso; do
   TMP=$(mktemp -d)
   cp "$HERE/control_$cfg.h" "$TMP/control_simulation.h"
   objs=""

@@ -111,6 +111,9 @@ program ref_md
     call angles()
     call coulomb_k()
     call verlet_velocities()
+#ifdef DPD_VV
+    call new_dpd_fd()
+#endif
     if (trace) then
       zsum = 0.d0
       ftop = 0.d0

@@ -8,7 +8,7 @@ result; trajectories of the FP64 form within 1e-9 after 40 steps.
 import numpy as np
 import pytest
 
-from mfa_amd import lib
+from mfa_amd import configs, lib
 from tests.util import force_error, gpu_ctx, make_case, oracle_sys, rel
 
 pytestmark = pytest.mark.gpu
@@ -240,3 +240,86 @@ def test_pairs_bit_exact_melt100k():
     assert np.array_equal(cells, s.get_cells())
     op = s.get_pairs()
     assert gp.shape == op.shape and np.array_equal(gp, op)
+
+
+def _vv_case(name):
+    """A small case with DPD_VV switched on (new_dpd_fd after each
+    verlet_velocities, md_main.f90:149-155)."""
+    kind, inputs = configs.small_cases()[name]
+    inputs.dpd_vv = 1
+    p, r, v, t = configs.build(inputs, kind)
+    return inputs, p, r, v, t, np.zeros_like(r)
+
+
+@pytest.mark.parametrize("case", ["brush_small", "brush_bend"])
+def test_dpd_vv_trajectory_fp64(case):
+    inputs, p, r, v, t, a = _vv_case(case)
+    s = oracle_sys(inputs, r, v, a, t)
+    ref = s.run_steps(first_step=1, n=40)
+    st_ref = s.get_state()
+    ctx = gpu_ctx(p, r, v, a, t)
+    obs = ctx.run_steps(first_step=1, n=40)
+    st = ctx.get_state()
+    dr = np.abs(st["r"] - st_ref["r"]).max()
+    dv = np.abs(st["v"] - st_ref["v"]).max()
+    print(case, "dr", dr, "dv", dv, "rebuilds", obs["n_rebuilds"], ref["n_rebuilds"])
+    assert np.array_equal(st["mic"], st_ref["mic"])
+    assert dr < 1e-9 and dv < 1e-8
+    assert np.abs(st["a"] - st_ref["a"]).max() < 1e-8
+    assert obs["n_rebuilds"] == ref["n_rebuilds"]
+    for k in ("v_fluid_fluid", "v_intra_molec", "t_kin"):
+        assert rel(obs[k], ref[k]) < 1e-9, (k, obs[k], ref[k])
+    ctx.close()
+
+
+def test_dpd_vv_entry_point_loop_matches_oracle():
+    """The driver's call sequence with DPD_VV (md_main.f90:73-155): the
+    per-subroutine entry points and new_dpd_fd, 10 steps."""
+    inputs, p, r, v, t, a = _vv_case("brush_small")
+    s = oracle_sys(inputs, r, v, a, t)
+    ref = s.run_steps(first_step=1, n=10)
+    st_ref = s.get_state()
+    ctx = gpu_ctx(p, r, v, a, t)
+    ctx.binning()
+    for it in range(10):
+        ctx.verlet_positions()
+        if ctx.check_skin():
+            ctx.binning()
+        ctx.zero_force()
+        ctx.fluid_fluid(1 + it)
+        ctx.fluid_wall(2)
+        ctx.intra_molec()
+        ctx.verlet_velocities()
+        tk = ctx.new_dpd_fd()
+    st = ctx.get_state()
+    assert np.abs(st["r"] - st_ref["r"]).max() < 1e-10
+    assert np.abs(st["v"] - st_ref["v"]).max() < 1e-9
+    assert rel(tk, ref["t_kin"]) < 1e-10
+    ctx.close()
+
+
+def test_new_dpd_fd_refused_without_dpd_vv():
+    inputs, p, r, v, t, a = make_case("brush_small")
+    ctx = gpu_ctx(p, r, v, a, t)
+    with pytest.raises(lib.MfaError):
+        ctx.new_dpd_fd()
+    ctx.close()
+
+
+def test_dpd_vv_graph_replays_match_direct_launches():
+    inputs, p, r, v, t, a = _vv_case("brush_small")
+    eager = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    eager.set_graph_steps(0)
+    graph = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    graph.set_graph_steps(4)
+    step = 1
+    for chunk in range(4):
+        oe = eager.run_steps(step, 11)
+        og = graph.run_steps(step, 11)
+        step += 11
+        se, sg = eager.get_state(), graph.get_state()
+        for k in ("r", "v", "f", "a", "mic"):
+            assert np.array_equal(se[k], sg[k]), (chunk, k)
+        assert oe["t_kin"] == og["t_kin"]
+    eager.close()
+    graph.close()

@@ -37,7 +37,10 @@ CASES = {
     "pe_brush": ("pe_brush_compat", "ref_md_brush_q", 3, 20),
     # stiff brush: bending(2) of the grafted chains
     "brush_bend": ("brush_bend", "ref_md_brush_bend", 3, 20),
+    # DPD_VV: new_dpd_fd after each verlet_velocities
+    "brush_vv": ("brush_small", "ref_md_brush_vv", 3, 20),
 }
+DPD_VV = {"brush_vv"}
 
 
 def _read(path, n, init):
@@ -75,6 +78,7 @@ def run(request, tmp_path_factory):
     kind, inp = configs.small_cases()[case]
     inp.skin = 0.0      # the reference's hard-coded 0.4 (init_params.f90:800)
     inp.temp = 0.0
+    inp.dpd_vv = 1 if request.param in DPD_VV else 0
     p, r, v, t = configs.build(inp, kind)
     d = str(tmp_path_factory.mktemp(request.param))
     io.write_run_dir(d, inp, t, r, v, np.zeros_like(r))
