@@ -90,6 +90,7 @@ struct KParams {
   // replica layout: particles [p0, p0 + n) (p0 a multiple of 256), cells
   // [c0, c0 + n_cells) of the concatenated arrays; flags at rep * F_NFLAGS
   int p0, c0, rep;
+  int fg0;  // first of the replica's completion counters of the cell fill
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
   int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
   int dpd_vv;  // DPD_VV: heads move, new_dpd_fd after verlet_velocities
@@ -131,6 +132,7 @@ struct mfa_ctx {
   int32_t *type, *mic;
   int32_t *cell_of, *cell_count, *cell_slots, *cell_start;
   int32_t *perm, *sidx;  // sorted position -> particle, particle -> sorted position
+  int32_t* fill_cnt;     // completion counters of the cell fill (zero between launches)
   int cell_cap;
   unsigned* nl;
   int32_t* nn;
@@ -531,17 +533,24 @@ __global__ void k_set_flag(int* __restrict__ flags, int n_rep, int idx, int valu
 }
 
 // Cell of each particle, mod(int(r*inv_r_bin), n_bin) per axis, and a slot in
-// that cell (binning.f90:25-48).
+// that cell (binning.f90:25-48).  Lanes take the particles in the spatial
+// order of the last rebuild (s -> perm[s]), so a wave's particles come in
+// runs of equal cells: one atomic per run reserves its slots.
+#define FILL_GROUP 16  // blocks per first-level completion counter
 __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
-    int* __restrict__ cell_of, int* __restrict__ cell_count, int* __restrict__ cell_slots,
-    int* __restrict__ cell_start, int* __restrict__ flags_all) {
+    const int* __restrict__ perm, int* __restrict__ cell_of, int* __restrict__ cell_count,
+    int* __restrict__ cell_slots, int* __restrict__ cell_start, int* __restrict__ flags_all,
+    int* __restrict__ fill_cnt) {
   MFA_REPLICA(blockIdx.x * blockDim.x);
   int* flags = flags_all + k.rep * F_NFLAGS;
   if (flags[F_REBUILD] == 0) return;  // uniform over the replica and the launch
   const bool ok = flags[F_ERROR] == 0;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ok && i - k.p0 < k.n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int cell = -1, i = 0;
+  if (ok && s - k.p0 < k.n) {
+    i = perm[s];
     const int cx = cell_coord(r[3 * (size_t)i + 0], k.inv_r_bin[0], k.nb[0]);
     const int cy = cell_coord(r[3 * (size_t)i + 1], k.inv_r_bin[1], k.nb[1]);
     const int cz = cell_coord(r[3 * (size_t)i + 2], k.inv_r_bin[2], k.nb[2]);
@@ -549,25 +558,48 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
       atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL);
       cell_of[i] = 0;
     } else {
-      const int c = (cz * k.nb[1] + cy) * k.nb[0] + cx;  // within the replica's grid
-      cell_of[i] = c;
-      const int slot = atomicAdd(&cell_count[k.c0 + c], 1);
-      if (slot < k.cell_cap)
-        cell_slots[(size_t)(k.c0 + c) * k.cell_cap + slot] = i;
-      else
-        atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
+      cell = (cz * k.nb[1] + cy) * k.nb[0] + cx;  // within the replica's grid
+      cell_of[i] = cell;
     }
+  }
+  // slots: each run of lanes with the same cell (the spatial order makes
+  // runs long) reserves its slots with one atomic, all runs of the wave at once
+  const int prev = __shfl_up(cell, 1, 64);
+  const unsigned long long starts = __ballot(lane == 0 || cell != prev);
+  const unsigned long long upto = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const int r0 = 63 - __builtin_clzll(upto);  // first lane of this lane's run
+  const unsigned long long above = starts & ~(lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const int r1 = above ? __builtin_ctzll(above) : 64;  // first lane of the next run
+  int base = 0;
+  if (cell >= 0 && lane == r0) base = atomicAdd(&cell_count[k.c0 + cell], r1 - r0);
+  base = __shfl(base, r0, 64);
+  if (cell >= 0) {
+    const int slot = base + (lane - r0);
+    if (slot < k.cell_cap)
+      cell_slots[(size_t)(k.c0 + cell) * k.cell_cap + slot] = i;
+    else
+      atomicCAS(&flags[F_ERROR], 0, MFA_ERR_CELL_OVERFLOW);
   }
   // The replica's last block to finish scans its cell counts:
   // cell_start[c] = p0 + particles in the replica's cells before c.  No block
-  // waits for another; the counter is reset by the block that finds itself
-  // last.
+  // waits for another.  Completion is counted in two levels (blocks of a
+  // group of FILL_GROUP, then groups), so no counter takes more than a few
+  // dozen atomics; each counter is reset by the block that finds itself last.
   __shared__ int last_s, wsum[MFA_BLOCK / 64], carry_s;
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     const int nb = (k.n + MFA_BLOCK - 1) / MFA_BLOCK;
-    last_s = atomicAdd(&flags[F_FILLDONE], 1) == nb - 1;
+    const int b = (int)blockIdx.x - k.p0 / MFA_BLOCK;  // block within the replica
+    const int g = b / FILL_GROUP, ng = (nb + FILL_GROUP - 1) / FILL_GROUP;
+    const int gsz = min(FILL_GROUP, nb - g * FILL_GROUP);
+    int last = 0;
+    if (atomicAdd(&fill_cnt[k.fg0 + g], 1) == gsz - 1) {
+      atomicExch(&fill_cnt[k.fg0 + g], 0);
+      __threadfence();
+      last = atomicAdd(&flags[F_FILLDONE], 1) == ng - 1;
+    }
+    last_s = last;
     carry_s = k.p0;
   }
   __syncthreads();
@@ -578,7 +610,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   __threadfence();
   // one pass: thread t sums the counts of cells [t per, (t+1) per), a block
   // scan of the 256 sums gives each thread its offset, then its cells' starts
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int per = (k.n_cells + MFA_BLOCK - 1) / MFA_BLOCK;
   const int cb = threadIdx.x * per, ce = min(k.n_cells, cb + per);
   int tot = 0;
@@ -651,109 +682,122 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 
 // Full Verlet list: every j != i in the 27 cells around i with
 // r^2 < list_2(ti,tj) (binning.f90:64-113, both directions stored).
-// One wave per cell, four cells per workgroup.  The lanes take the
-// candidates of the 27 cells 64 at a time, in offset-code order
-// 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in ascending id (the order of a
-// sequential scan); for each particle of the cell in turn the wave tests the
-// 64 candidates at once and a ballot appends the hits to that particle's row
-// in lane order (the row lengths live in the lanes: particle h's in lane
-// h % 64).  Separations come from the FP32 view (both positions relative to
-// their cell origins, plus the offset of the candidate's cell): no periodic
-// arithmetic.  A separation whose r^2 lies within the view's rounding bound
-// of the radius is decided again in FP64 with the reference's image
-// convention (binning.f90:84-105).  Each entry is the candidate's sorted
-// position with its cell's offset code above bit 27.  With walls (no
-// periodicity in z) cells beyond the z faces hold no neighbours.
-#define LB_SLOTS 4  // particles per cell up to 64 * LB_SLOTS
+// One workgroup per cell; its four waves split the cell's particles (particle
+// h of the cell belongs to wave h % 4, whose lane h / 4 holds its row
+// length).  Each wave takes the candidates of the 27 cells 64 at a time, in
+// offset-code order 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in ascending
+// id (the order of a sequential scan); for each of its particles in turn it
+// tests the 64 candidates at once and a ballot appends the hits to that
+// particle's row in lane order.  The next 64 candidates are fetched while the
+// current ones are tested.  Separations come from the FP32 view (both
+// positions relative to their cell origins, plus the offset of the
+// candidate's cell): no periodic arithmetic.  A separation whose r^2 lies
+// within the view's rounding bound of the radius is decided again in FP64
+// with the reference's image convention (binning.f90:84-105).  Each entry is
+// the candidate's sorted position with its cell's offset code above bit 27.
+// With walls (no periodicity in z) cells beyond the z faces hold no
+// neighbours.
+#define LB_WAVES 4                   // waves per cell (MFA_BLOCK / 64)
+#define LB_MAX_CELL (64 * LB_WAVES)  // particles per cell the build takes
+#define LB_UNROLL 2                  // particles of a wave tested per round
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
-    const KParams* __restrict__ kb, const int* __restrict__ cell_rep, int n_cells,
+    const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
     const int* __restrict__ cell_start, const int* __restrict__ perm,
     const float4* __restrict__ pos4, unsigned* __restrict__ nl, int* __restrict__ nn, int cap,
     int* __restrict__ flags_all) {
-  extern __shared__ float4 home_pos[];  // [4][cell_cap]
-  __shared__ int q_first_s[4][28], q_start_s[4][27];
-  __shared__ float4 q_off_s[4][27];
-  __shared__ float l2tab_s[4][MFA_NTYPE * MFA_NTYPE];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cg = blockIdx.x * 4 + w;
-  const KParams& k = kb[cell_rep[cg < n_cells ? cg : 0]];
+  extern __shared__ float4 hpos[];  // [cell_cap]: the cell's own particles
+  __shared__ int q_first[28], q_start[27];
+  __shared__ float4 q_off[27];
+  __shared__ float l2tab[MFA_NTYPE * MFA_NTYPE];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cg = blockIdx.x;
+  const KParams& k = kb[cell_rep[cg]];
   int* flags = flags_all + k.rep * F_NFLAGS;
-  float* l2tab = l2tab_s[w];
-  const bool on = cg < n_cells && flags[F_REBUILD] != 0 && flags[F_ERROR] == 0;
-  int* q_first = q_first_s[w];
-  int* q_start = q_start_s[w];
-  float4* q_off = q_off_s[w];
-  float4* hpos = home_pos + w * k.cell_cap;
+  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the workgroup
   const int c = cg - k.c0;
   const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
-  const int m = on ? min(cell_count[cg], k.cell_cap) : 0;
-  const int hs0 = on ? cell_start[cg] : 0;
-  if (on && lane < 27) {
-    const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+  const int m = min(cell_count[cg], k.cell_cap);
+  const int hs0 = cell_start[cg];
+  if (threadIdx.x < 27) {
+    const int t = threadIdx.x;
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
     const int jx = cx + dx, jy = cy + dy, jz = cz + dz;
     const int cc = k.c0 + (((jz + k.nb[2]) % k.nb[2]) * k.nb[1] + (jy + k.nb[1]) % k.nb[1]) * k.nb[0] +
                    (jx + k.nb[0]) % k.nb[0];
     const bool gone = k.symmetry == 0 && (jz < 0 || jz >= k.nb[2]);
     const int cnt = gone ? 0 : min(cell_count[cc], k.cell_cap);
-    // inclusive prefix over the 27 codes, lanes 0..26 of this wave
+    // inclusive prefix over the 27 codes (lanes 0..26 of wave 0)
     int x = cnt;
     for (int o = 1; o < 32; o <<= 1) {
       const int y = __shfl_up(x, o, 32);
-      if ((lane & 31) >= o) x += y;
+      if (t >= o) x += y;
     }
-    q_first[lane + 1] = x;
-    if (lane == 0) q_first[0] = 0;
-    q_start[lane] = gone ? 0 : cell_start[cc];
-    q_off[lane] = make_float4(-(float)dx * (float)k.r_bin[0], -(float)dy * (float)k.r_bin[1],
-                              -(float)dz * (float)k.r_bin[2], 0.0f);
+    q_first[t + 1] = x;
+    if (t == 0) q_first[0] = 0;
+    q_start[t] = gone ? 0 : cell_start[cc];
+    q_off[t] = make_float4(-(float)dx * (float)k.r_bin[0], -(float)dy * (float)k.r_bin[1],
+                           -(float)dz * (float)k.r_bin[2], 0.0f);
   }
-  if (lane >= 32 && lane < 32 + MFA_NTYPE * MFA_NTYPE) {
-    const int q = lane - 32;
+  if (threadIdx.x >= 32 && threadIdx.x < 32 + MFA_NTYPE * MFA_NTYPE) {
+    const int q = threadIdx.x - 32;
     l2tab[q] = (float)k.list_2[q / MFA_NTYPE][q % MFA_NTYPE];
   }
-  for (int t = lane; t < m; t += 64) hpos[t] = pos4[hs0 + t];
+  for (int t = threadIdx.x; t < m; t += MFA_BLOCK) hpos[t] = pos4[hs0 + t];
   __syncthreads();
-  if (on && c == 0 && lane == 0) atomicAdd(&flags[F_NREBUILD], 1);
-  if (!on || m == 0) return;
+  if (c == 0 && threadIdx.x == 0) atomicAdd(&flags[F_NREBUILD], 1);
+  const int mt = (m - w + LB_WAVES - 1) / LB_WAVES;  // this wave's particles: h = w + 4 t
+  if (mt <= 0) return;
   const int total = q_first[27];
+  const int self0 = q_first[13];  // the cell's own particles: candidates self0 .. self0 + m - 1
   const float margin = k.lf_margin;
-  int hc[LB_SLOTS];  // row length of particle 64 s + lane
-  for (int sl = 0; sl < LB_SLOTS; ++sl) hc[sl] = 0;
+  int hc = 0;  // row length of this wave's particle t = lane
+  // candidate e = e0 + lane: cell code q (never decreases along e), sorted
+  // position js; a lane past the end takes the cell's first particle, placed
+  // far away so that it never hits
+  int q = 0;
+  bool live = lane < total;
+  if (live)
+    while (q_first[q + 1] <= lane) ++q;
+  int js = live ? q_start[q] + (lane - q_first[q]) : hs0;
+  float4 pj = pos4[js];
   for (int e0 = 0; e0 < total; e0 += 64) {
-    // this lane's candidate: cell code q, sorted position js (a lane past the
-    // end takes the cell's first particle and never hits)
-    const int e = e0 + lane;
-    const bool live = e < total;
-    int q = 0;
-    if (live)
-      while (q_first[q + 1] <= e) ++q;
-    const int js = live ? q_start[q] + (e - q_first[q]) : hs0;
-    const float4 pj = pos4[js];
-    const float4 o = q_off[q];
+    const bool live_c = live;
+    const int js_c = js, q_c = q;
+    const float4 pj_c = pj;
+    {  // fetch the next 64 candidates
+      const int e = e0 + 64 + lane;
+      live = e < total;
+      if (live)
+        while (q_first[q + 1] <= e) ++q;
+      js = live ? q_start[q] + (e - q_first[q]) : hs0;
+      if (e0 + 64 < total) pj = pos4[js];
+    }
+    const float4 o = q_off[q_c];
     // the candidate relative to this cell's origin: d_ij = p_i - (p_j - o)
-    const float pjx = pj.x - o.x, pjy = pj.y - o.y, pjz = pj.z - o.z;
-    const int tj = __float_as_int(pj.w) & 3;
-    const unsigned entry = (unsigned)js | ((unsigned)q << MFA_JBITS);
-    // four particles of the cell per round: their distance tests are
-    // independent, then the hits are appended particle by particle
-    for (int h0 = 0; h0 < m; h0 += 4) {
-      bool hit[4];
+    const float far = live_c ? 0.0f : 1.0e30f;
+    const float pjx = pj_c.x - o.x + far, pjy = pj_c.y - o.y, pjz = pj_c.z - o.z;
+    const int tj = __float_as_int(pj_c.w) & 3;
+    const unsigned entry = (unsigned)js_c | ((unsigned)q_c << MFA_JBITS);
+    const int erel = e0 + lane - self0;  // = h for the candidate that is particle h itself
+    for (int t0 = 0; t0 < mt; t0 += LB_UNROLL) {
+      bool hit[LB_UNROLL];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int h = h0 + u;
+      for (int u = 0; u < LB_UNROLL; ++u) {
+        const int t = t0 + u, h = w + LB_WAVES * t;
         hit[u] = false;
-        if (h < m) {
+        if (t < mt) {
           const float4 pi = hpos[h];
           const int ti = __float_as_int(pi.w) & 3;
           const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
           const float d2 = dx * dx + dy * dy + dz * dz;
           const float l2 = l2tab[ti * MFA_NTYPE + tj];
-          if (live && js != hs0 + h) {
+          if (erel != h) {
             if (d2 < l2 - margin) {
               hit[u] = true;
-            } else if (d2 < l2 + margin) {
-              const int i = perm[hs0 + h], j = perm[js];
+            } else if (d2 < l2 + margin) {  // rare: decide in FP64
+              const int i = perm[hs0 + h], j = perm[js_c];
               double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
               double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
               double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
@@ -765,38 +809,34 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const unsigned long long mask = __ballot(hit[u]);
+      for (int u = 0; u < LB_UNROLL; ++u) {
+        const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit[u]);
         if (mask) {
-          const int h = h0 + u;
-          const int sl = h >> 6, ln = h & 63;
-          int mine = hc[0];
-          for (int q2 = 1; q2 < LB_SLOTS; ++q2) mine = sl == q2 ? hc[q2] : mine;
-          const int base = __builtin_amdgcn_readlane(mine, ln);
+          const int t = t0 + u, h = w + LB_WAVES * t;
+          const int base = __builtin_amdgcn_readlane(hc, t);
           if (hit[u]) {
-            const int pos = base + (int)__popcll(mask & ((1ull << lane) - 1ull));
-            if (pos < cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
+            const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
+            if (pos < (unsigned)cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
           }
-          if (lane == ln) {
-            const int add = (int)__popcll(mask);
-            for (int q2 = 0; q2 < LB_SLOTS; ++q2) hc[q2] += sl == q2 ? add : 0;
-          }
+          hc = lane == t ? base + (int)__popcll(mask) : hc;
         }
       }
     }
   }
   int my_max = 0;
-  for (int sl = 0; sl < LB_SLOTS; ++sl) {
-    const int t = sl * 64 + lane;
-    if (t < m) {
-      const int cnt = hc[sl];
-      nn[hs0 + t] = cnt < cap ? cnt : cap;
-      if (cnt > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
-      my_max = max(my_max, cnt);
-    }
+  if (lane < mt) {
+    const int h = w + LB_WAVES * lane;
+    nn[hs0 + h] = hc < cap ? hc : cap;
+    if (hc > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
+    my_max = hc;
   }
   for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, __shfl_xor(my_max, o, 64));
-  if (lane == 0 && my_max > 0) atomicMax(&flags[F_MAXNEIGH], my_max);
+  // the maximum only grows: most builds find it already at least this large
+  // and take no atomic on the one shared address
+  if (lane == 0 && my_max > __hip_atomic_load(&flags[F_MAXNEIGH], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&flags[F_MAXNEIGH], my_max);
 }
 
 // ---------------------------------------------------------------------------
@@ -1893,7 +1933,7 @@ static KParams make_kparams(const mfa_params& p) {
 static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
 static int launch_ewald(mfa_ctx* c, int add_dip);
 
-static size_t list_lds_bytes(int cell_cap) { return (size_t)4 * cell_cap * sizeof(float4); }
+
 
 // A rebuild: cells of every particle, their sorted ranges, the spatial order
 // and FP32 view, then the lists.  Each kernel returns at once unless the
@@ -1901,15 +1941,17 @@ static size_t list_lds_bytes(int cell_cap) { return (size_t)4 * cell_cap * sizeo
 static int launch_rebuild(mfa_ctx* c) {
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
-                     c->cell_of, c->cell_count, c->cell_slots, c->cell_start, c->flags);
+                     c->perm, c->cell_of, c->cell_count, c->cell_slots, c->cell_start, c->flags,
+                     c->fill_cnt);
   const int cb = (c->n_cells + 3) / 4;  // four cells (waves) per workgroup
   hipLaunchKernelGGL(k_cell_sort, dim3(cb), dim3(MFA_BLOCK), (size_t)4 * c->cell_cap * sizeof(int),
                      c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->v, c->type, c->mic,
                      c->cell_count, c->cell_slots, c->cell_start, c->perm, c->sidx, c->r_ref,
                      c->flags, c->pos4, c->vel4);
-  hipLaunchKernelGGL(k_build_list, dim3(cb), dim3(MFA_BLOCK), list_lds_bytes(c->cell_cap), c->stream,
-                     c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->cell_count, c->cell_start,
-                     c->perm, c->pos4, c->nl, c->nn, c->cap, c->flags);
+  hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK),
+                     (size_t)c->cell_cap * sizeof(float4), c->stream, c->d_kp,
+                     c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm, c->pos4, c->nl,
+                     c->nn, c->cap, c->flags);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -2419,7 +2461,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->gexec = nullptr;
   // capacities and layout
   int cap = list_capacity, cell_cap = 0;
-  int64_t ntot = 0, ctot = 0;
+  int64_t ntot = 0, ctot = 0, gtot = 0;
   for (int q = 0; q < n_rep; ++q) {
     const mfa_params& p = ps[q];
     const double vol = p.boundary[0] * p.boundary[1] * p.boundary[2];
@@ -2441,9 +2483,12 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
     KParams k = make_kparams(p);
     k.p0 = (int)ntot;
     k.c0 = (int)ctot;
+    k.fg0 = (int)gtot;
     k.rep = q;
     c->ks.push_back(k);
-    ntot += ((int64_t)p.n_part + MFA_BLOCK - 1) / MFA_BLOCK * MFA_BLOCK;
+    const int64_t nbq = ((int64_t)p.n_part + MFA_BLOCK - 1) / MFA_BLOCK;
+    ntot += nbq * MFA_BLOCK;
+    gtot += (nbq + FILL_GROUP - 1) / FILL_GROUP;
     ctot += (int64_t)p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
   }
   if (ntot > (int64_t)MFA_JMASK) {
@@ -2456,9 +2501,8 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->n = (int)ntot;
   c->n_cells = (int)ctot;
   {
-    // LDS of the list build: the cell's own particles as (float4, row length)
-    const size_t lds = list_lds_bytes(c->cell_cap);
-    if (lds > 64 * 1024 || c->cell_cap > 64 * LB_SLOTS) {
+    // the list build gives each particle of a cell a lane of one of its waves
+    if (c->cell_cap > LB_MAX_CELL) {
       delete c;
       return set_err(nullptr, MFA_ERR_ARG, "binning cells too crowded for the list build (capacity %d)",
                      cell_cap);
@@ -2487,6 +2531,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   ALLOC(c->cell_slots, ncs * sizeof(int32_t));
   ALLOC(c->cell_start, ((size_t)c->n_cells + 1) * sizeof(int32_t));
   ALLOC(c->perm, n * sizeof(int32_t));
+  ALLOC(c->fill_cnt, (size_t)gtot * sizeof(int32_t));
   if (c->dpd_vv) {
     ALLOC(c->fd, n * 3 * sizeof(double));
     ALLOC(c->fd_new, n * 3 * sizeof(double));
@@ -2551,6 +2596,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->flags, 0, (size_t)n_rep * F_NFLAGS * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->d_step, 0, sizeof(long long), c->stream);
+  hipMemsetAsync(c->fill_cnt, 0, (size_t)gtot * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->mic, 0, n * 3 * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->f, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->a, 0, n * 3 * sizeof(double), c->stream);
@@ -2594,7 +2640,7 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
                   c->mic, c->cell_of, c->cell_count, c->cell_slots, c->nl,    c->nn,
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
                   c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx,
-                  c->fd,   c->fd_new};
+                  c->fd,   c->fd_new, c->fill_cnt};
   for (void* q : ptrs)
     if (q) hipFree(q);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
