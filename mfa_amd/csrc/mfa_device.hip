@@ -585,10 +585,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   // waits for another.  Completion is counted in two levels (blocks of a
   // group of FILL_GROUP, then groups), so no counter takes more than a few
   // dozen atomics; each counter is reset by the block that finds itself last.
+  // The counts are only ever changed by device-scope atomics, each of which
+  // has returned (its slot is used above) before the barrier; the last block
+  // reads them with device-scope atomic loads.  No cache-wide fence is needed.
   __shared__ int last_s, wsum[MFA_BLOCK / 64], carry_s;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
     const int nb = (k.n + MFA_BLOCK - 1) / MFA_BLOCK;
     const int b = (int)blockIdx.x - k.p0 / MFA_BLOCK;  // block within the replica
     const int g = b / FILL_GROUP, ng = (nb + FILL_GROUP - 1) / FILL_GROUP;
@@ -596,7 +598,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
     int last = 0;
     if (atomicAdd(&fill_cnt[k.fg0 + g], 1) == gsz - 1) {
       atomicExch(&fill_cnt[k.fg0 + g], 0);
-      __threadfence();
       last = atomicAdd(&flags[F_FILLDONE], 1) == ng - 1;
     }
     last_s = last;
@@ -605,16 +606,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   __syncthreads();
   if (!last_s) return;
   if (threadIdx.x == 0) atomicExch(&flags[F_FILLDONE], 0);
-  // acquire: the other blocks' count increments (atomics at L2) are visible
-  // to plain loads after this fence
-  __threadfence();
   // one pass: thread t sums the counts of cells [t per, (t+1) per), a block
   // scan of the 256 sums gives each thread its offset, then its cells' starts
   const int per = (k.n_cells + MFA_BLOCK - 1) / MFA_BLOCK;
   const int cb = threadIdx.x * per, ce = min(k.n_cells, cb + per);
   int tot = 0;
 #pragma unroll 8
-  for (int c = cb; c < ce; ++c) tot += min(cell_count[k.c0 + c], k.cell_cap);
+  for (int c = cb; c < ce; ++c)
+    tot += min(__hip_atomic_load(&cell_count[k.c0 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+               k.cell_cap);
   int x = tot;  // inclusive scan of the thread sums within the wave
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(x, o, 64);
@@ -627,7 +627,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
 #pragma unroll 8
   for (int c = cb; c < ce; ++c) {
     cell_start[k.c0 + c] = run;
-    run += min(cell_count[k.c0 + c], k.cell_cap);
+    run += min(__hip_atomic_load(&cell_count[k.c0 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+               k.cell_cap);
   }
 }
 
