@@ -219,7 +219,7 @@ def bench_batch(args, n_rep=256):
         for q, (p, r, v, t) in enumerate(reps):
             b.set_replica(q, r, v, np.zeros_like(r), t)
         tpp, ms_tune = b.autotune(1, 20)
-        log("autotune: %d threads per particle (%.3f ms/step)" % (tpp, ms_tune))
+        log("autotune: %d threads per particle (force pass %.3f ms)" % (tpp, ms_tune))
         first = 1 + 5 * (args.graph_steps + 1 + 20)  # the steps the autotuner ran
         b.run_steps(first, args.warmup, want_obs=False)
         t1 = time.perf_counter()

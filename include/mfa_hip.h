@@ -279,11 +279,12 @@ int32_t mfa_set_state_replica(mfa_ctx* ctx, int32_t replica, const double* r0, c
 int32_t mfa_get_state_replica(mfa_ctx* ctx, int32_t replica, double* r0, double* v, double* a,
                               double* force, int32_t* mic);
 int32_t mfa_replica_obs(mfa_ctx* ctx, int32_t replica, mfa_obs* obs);
-/* Autotuner: runs n_steps from first_step with each number of threads per
- * particle in {1, 2, 4, 8, 16} (advancing the state, so call it during
- * warm-up), keeps the fastest and returns it with its time per step. */
+/* Autotuner: for each number of threads per particle in {1, 2, 4, 8, 16}
+ * runs n_steps from first_step (advancing the state, so call it during
+ * warm-up), then times 20 force passes on the state they left; keeps the
+ * fastest and returns it with its time per force pass (ms). */
 int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
-                     double* ms_per_step);
+                     double* ms_per_pass);
 
 /*
  * Adjoint of the force-and-integrate step (no counterpart in the reference,

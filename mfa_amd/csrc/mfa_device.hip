@@ -855,30 +855,23 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
 // a separation whose r^2 lies within the view's error bound of a cut-off is
 // decided again in FP64 with the reference's image convention, so the
 // interacting set is the FP64 one.  Pair math in FP32; each pair term enters
-// an int64 fixed-point sum.
+// an exact fixed-point sum.
 
-__device__ __forceinline__ long long to_fix(float v) {
-  return (long long)__float2ll_rz(v * MFA_FIX_SCALE);
+// Adds a pair/bond/wall term to the fixed-point sums.  Each component is
+// truncated toward zero to a multiple of 2^-22 and added as an
+// integer-valued double: such sums are exact below 2^53, so they do not
+// depend on the order of the terms (they equal the int64 sums, with one
+// FP64 add per component).  A lane sum at or beyond 2^52 (or not finite) is
+// reported by the caller as a bad force.
+__device__ __forceinline__ void fix_add3(double& ax, double& ay, double& az, float fx, float fy,
+                                         float fz) {
+  ax += (double)truncf(fx * MFA_FIX_SCALE);
+  ay += (double)truncf(fy * MFA_FIX_SCALE);
+  az += (double)truncf(fz * MFA_FIX_SCALE);
 }
-
-// Adds a pair/bond/wall term to the fixed-point sums.  Terms below 512 in
-// every component take the int32 conversion; both routes truncate, so the
-// sum does not depend on which one ran.
-__device__ __forceinline__ void fix_add3(long long& ax, long long& ay, long long& az, float fx,
-                                         float fy, float fz, bool& bad) {
-  const float sx = fx * MFA_FIX_SCALE, sy = fy * MFA_FIX_SCALE, sz = fz * MFA_FIX_SCALE;
-  const float lim = 2147483520.0f;  // largest float below 2^31
-  if (fabsf(sx) < lim && fabsf(sy) < lim && fabsf(sz) < lim) {
-    ax += (long long)(int)sx;
-    ay += (long long)(int)sy;
-    az += (long long)(int)sz;
-  } else {
-    // large or non-finite term (a NaN fails every comparison above)
-    bad = bad || !(fabsf(sx) <= 3.0e38f && fabsf(sy) <= 3.0e38f && fabsf(sz) <= 3.0e38f);
-    ax += (long long)__float2ll_rz(sx);
-    ay += (long long)__float2ll_rz(sy);
-    az += (long long)__float2ll_rz(sz);
-  }
+__device__ __forceinline__ bool fix_bad(double ax, double ay, double az) {
+  const double lim = 4503599627370496.0;  // 2^52
+  return !(fabs(ax) < lim && fabs(ay) < lim && fabs(az) < lim);
 }
 
 template <int TPP>
@@ -1235,7 +1228,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   // cell offsets -(d)*r_bin of the 27 codes, and the FP32 pair tables
   __shared__ float4 s_off[27];
   __shared__ float4 s_tab[MFA_NTYPE * MFA_NTYPE];  // range_2, 4 eps, sigma_2, e_shift
-  __shared__ float s_inv_range_2[MFA_NTYPE * MFA_NTYPE];
+  __shared__ float s_inv_rc[MFA_NTYPE * MFA_NTYPE];  // 1 / sqrt(range_2)
   __shared__ int err_s;
   if (threadIdx.x < 27) {
     const int c = threadIdx.x;
@@ -1247,7 +1240,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const int q = threadIdx.x - 32, a = q / MFA_NTYPE, b = q % MFA_NTYPE;
     s_tab[q] = make_float4((float)k.range_2[a][b], (float)k.epsil[a][b], (float)k.sigma_2[a][b],
                            (float)k.e_shift[a][b]);
-    s_inv_range_2[q] = (float)k.inv_range_2[a][b];
+    s_inv_rc[q] = (float)sqrt(k.inv_range_2[a][b]);
   }
   if (threadIdx.x == 64) err_s = flags[F_ERROR];
   __syncthreads();
@@ -1262,7 +1255,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   if (sp == k.p0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const float r2min = (float)switch_on_r2min(k, step);
-  const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd2 = (float)(1.0 / k.r_cut_dpd_2);
+  const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd = (float)(1.0 / sqrt(k.r_cut_dpd_2));
   const float sigf = (float)k.sig, gam = (float)k.friction1, margin = k.ff_margin;
   const float4 pi4 = pos4[ss];
   const float4 vi4 = vel4[ss];
@@ -1271,8 +1264,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   double e_coul = 0.0, e_bend = 0.0;
   const float rc2f = (float)k.rc_coul_2, sa = (float)k.sqrt_alpha, alf = (float)k.alpha_c;
   const float pref = (float)(k.sqrt_alpha * k.inv_sqrt_pi * 2.0);
-  long long ax = 0, ay = 0, az = 0;
-  bool bad = false;
+  double ax = 0.0, ay = 0.0, az = 0.0;  // fixed-point sums (integer-valued)
 
   if (active && (mask & TM_PAIR)) {
     const int cnt = nn[ss];
@@ -1329,8 +1321,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       }
       if (dpd) {
         const float xi_n = pair_normal_f32(k.seed, step, il, jl);
-        const float w_arg = k.dpd_cut_off_on ? r2 * inv_rdpd2 : r2 * s_inv_range_2[ti * MFA_NTYPE + tj];
-        const float wr = 1.0f - __builtin_amdgcn_sqrtf(w_arg);  // v_sqrt_f32, ~1 ulp
+        // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
+        const float inv_rc = k.dpd_cut_off_on ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj];
+        const float wr = 1.0f - (r2 * inv_r) * inv_rc;
         const float4 vj = vel4[js];
         const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
         s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
@@ -1342,7 +1335,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
         if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
       }
-      fix_add3(ax, ay, az, s * dx, s * dy, s * dz, bad);
+      fix_add3(ax, ay, az, s * dx, s * dy, s * dz);
     }
   }
 
@@ -1356,9 +1349,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       if (cen < 1 || cen > k.n_mon - 2) continue;
       double fp[3], fn[3];
       const double eb = bend_angle(k, r, k.p0 + base + cen, fp, fn);
-      long long px = 0, py = 0, pz = 0, nx = 0, ny = 0, nz = 0;
-      fix_add3(px, py, pz, (float)fp[0], (float)fp[1], (float)fp[2], bad);
-      fix_add3(nx, ny, nz, (float)fn[0], (float)fn[1], (float)fn[2], bad);
+      double px = 0.0, py = 0.0, pz = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
+      fix_add3(px, py, pz, (float)fp[0], (float)fp[1], (float)fp[2]);
+      fix_add3(nx, ny, nz, (float)fn[0], (float)fn[1], (float)fn[2]);
       if (cen == m) {
         if (ENERGY) e_bend += eb;
         ax -= px + nx;
@@ -1392,7 +1385,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         x = x < 0.98f ? x : 0.98f;
         const float rd = (float)k.k_chain / (1.0f - x) * (float)k.inv_r_chain_2;
         const float sg = side == 0 ? rd : -rd;
-        fix_add3(ax, ay, az, sg * (float)dx, sg * (float)dy, sg * (float)dz, bad);
+        fix_add3(ax, ay, az, sg * (float)dx, sg * (float)dy, sg * (float)dz);
         if (ENERGY && side == 1) {
           e_bond += -0.5 * log(1.0 - (double)x);
           vir += (double)(-rd * (float)r2);
@@ -1401,15 +1394,16 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     }
     if ((mask & TM_WALL) && k.symmetry == 0) {
       wall_93(k, ti, zi, fzb, fzt, e_fw);
-      fix_add3(ax, ay, az, 0.0f, 0.0f, (float)fzb, bad);
-      fix_add3(ax, ay, az, 0.0f, 0.0f, (float)fzt, bad);
+      fix_add3(ax, ay, az, 0.0f, 0.0f, (float)fzb);
+      fix_add3(ax, ay, az, 0.0f, 0.0f, (float)fzt);
     }
   }
 
   // Combine the lane group and write F_i.
-  double fx = (double)group_sum<TPP>(ax) * MFA_FIX_INV;
-  double fy = (double)group_sum<TPP>(ay) * MFA_FIX_INV;
-  double fz = (double)group_sum<TPP>(az) * MFA_FIX_INV;
+  const bool bad = fix_bad(ax, ay, az);
+  double fx = group_sum<TPP>(ax) * MFA_FIX_INV;
+  double fy = group_sum<TPP>(ay) * MFA_FIX_INV;
+  double fz = group_sum<TPP>(az) * MFA_FIX_INV;
   if (active && sub == 0) {
     if (mask & TM_ACCUM) {
       fx = f[3 * (size_t)i + 0] + fx;
@@ -3111,25 +3105,34 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
 }
 
 extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
-                                double* ms_per_step) {
+                                double* ms_per_pass) {
   if (!c || n_steps < 1) return MFA_ERR_ARG;
   const int cand[5] = {1, 2, 4, 8, 16};
+  const int reps = 20;
   double best = 1e300;
   int best_t = c->tpp;
   int64_t step = first_step;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
+  const bool timing = c->timing;
+  c->timing = false;
   int rc = MFA_OK;
   for (int q = 0; q < 5 && rc == MFA_OK; ++q) {
     c->tpp = cand[q];
-    // one untimed call captures the step block for this choice
+    // the candidate's steps (untimed; the first call captures its step block)
     rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
     step += c->graph_steps + 1;
     if (rc) break;
-    HIPCHK(c, hipEventRecord(e0, c->stream));
     rc = mfa_run_steps(c, step, n_steps, nullptr);
     step += n_steps;
+    if (rc) break;
+    // then its force pass, repeated on the state the steps left (the same
+    // list and step counter, so every repetition writes the same forces):
+    // one comparison on identical work, free of the rebuilds the steps hit
+    for (int w = 0; w < 3 && rc == MFA_OK; ++w) rc = launch_force(c, TM_ALL);
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int w = 0; w < reps && rc == MFA_OK; ++w) rc = launch_force(c, TM_ALL);
     HIPCHK(c, hipEventRecord(e1, c->stream));
     HIPCHK(c, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -3139,14 +3142,15 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
       best_t = cand[q];
     }
   }
+  c->timing = timing;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   c->tpp = best_t;
   drop_graph(c);
   if (rc) return rc;
   if (best_tpp) *best_tpp = best_t;
-  if (ms_per_step) *ms_per_step = best / (double)n_steps;
-  return MFA_OK;
+  if (ms_per_pass) *ms_per_pass = best / (double)reps;
+  return check_flags(c);
 }
 
 extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
