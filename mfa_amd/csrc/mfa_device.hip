@@ -133,6 +133,7 @@ struct mfa_ctx {
   int32_t *cell_of, *cell_count, *cell_slots, *cell_start;
   int32_t *perm, *sidx;  // sorted position -> particle, particle -> sorted position
   int32_t* fill_cnt;     // completion counters of the cell fill (zero between launches)
+  bool lb_octants;       // list build with octant groups (dense cells)
   int cell_cap;
   unsigned* nl;
   int32_t* nn;
@@ -683,34 +684,45 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 
 // Full Verlet list: every j != i in the 27 cells around i with
 // r^2 < list_2(ti,tj) (binning.f90:64-113, both directions stored).
-// One workgroup per cell; its four waves split the cell's particles (particle
-// h of the cell belongs to wave h % 4, whose lane h / 4 holds its row
-// length).  Each wave takes the candidates of the 27 cells 64 at a time, in
-// offset-code order 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in ascending
-// id (the order of a sequential scan); for each of its particles in turn it
-// tests the 64 candidates at once and a ballot appends the hits to that
-// particle's row in lane order.  The next 64 candidates are fetched while the
-// current ones are tested.  Separations come from the FP32 view (both
+// One workgroup per cell, four waves; wave w takes the cell's particles in
+// octants 2w and 2w+1 of the cell (two groups; a particle's row length lives
+// in the lane of its place in the wave's list).  Each wave walks the
+// candidates of the 27 cells 64 at a time, in offset-code order
+// 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in ascending id (the order of a
+// sequential scan), fetching the next 64 while it works on the current ones.
+// A candidate farther than the list radius from the bounding box of a
+// group's particles cannot be a neighbour of any of them; the others are
+// queued, in order, for that group.  Each time 64 are queued, the group's
+// particles are tested in turn against all 64 at once and a ballot appends
+// the hits to the particle's row in lane order, so every row holds its
+// neighbours in scan order.  Separations come from the FP32 view (both
 // positions relative to their cell origins, plus the offset of the
 // candidate's cell): no periodic arithmetic.  A separation whose r^2 lies
 // within the view's rounding bound of the radius is decided again in FP64
 // with the reference's image convention (binning.f90:84-105).  Each entry is
 // the candidate's sorted position with its cell's offset code above bit 27.
 // With walls (no periodicity in z) cells beyond the z faces hold no
-// neighbours.
+// neighbours.  A cell with fewer than LB_MIN_OCTANT particles, or with more
+// than 32 in one octant, takes the same path with one group per wave
+// (particles w + 4t) and no box test.
 #define LB_WAVES 4                   // waves per cell (MFA_BLOCK / 64)
 #define LB_MAX_CELL (64 * LB_WAVES)  // particles per cell the build takes
-#define LB_UNROLL 2                  // particles of a wave tested per round
+#define LB_RING 128                  // queued candidates per group (a block of 64 adds at most 64)
+#define LB_MIN_OCTANT 16             // particles in a cell for the octant groups
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
     const int* __restrict__ cell_start, const int* __restrict__ perm,
     const float4* __restrict__ pos4, unsigned* __restrict__ nl, int* __restrict__ nn, int cap,
-    int* __restrict__ flags_all) {
-  extern __shared__ float4 hpos[];  // [cell_cap]: the cell's own particles
+    int* __restrict__ flags_all, int octants) {
+  // dynamic LDS: the cell's own particles [cell_cap]; with octants, then per
+  // wave and group the queued candidates (view, .w type bits) and entries
+  extern __shared__ float4 lds_lb[];
   __shared__ int q_first[28], q_start[27];
   __shared__ float4 q_off[27];
   __shared__ float l2tab[MFA_NTYPE * MFA_NTYPE];
+  __shared__ int hlist[LB_WAVES][64];  // the wave's particles, group 0 then group 1
+  __shared__ int crowded_s;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int cg = blockIdx.x;
@@ -721,6 +733,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
   const int m = min(cell_count[cg], k.cell_cap);
   const int hs0 = cell_start[cg];
+  float4* hpos = lds_lb;
+  float4* qpos_w = lds_lb + k.cell_cap + w * 2 * LB_RING;  // [2][LB_RING] of this wave
+  unsigned* qent_w = (unsigned*)(lds_lb + k.cell_cap + LB_WAVES * 2 * LB_RING) + w * 2 * LB_RING;
   if (threadIdx.x < 27) {
     const int t = threadIdx.x;
     const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
@@ -745,18 +760,134 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const int q = threadIdx.x - 32;
     l2tab[q] = (float)k.list_2[q / MFA_NTYPE][q % MFA_NTYPE];
   }
+  if (threadIdx.x == 64) crowded_s = 0;
   for (int t = threadIdx.x; t < m; t += MFA_BLOCK) hpos[t] = pos4[hs0 + t];
   __syncthreads();
   if (c == 0 && threadIdx.x == 0) atomicAdd(&flags[F_NREBUILD], 1);
-  const int mt = (m - w + LB_WAVES - 1) / LB_WAVES;  // this wave's particles: h = w + 4 t
-  if (mt <= 0) return;
+  // the wave's two octants: particle lists and their counts
+  const float hx = 0.5f * (float)k.r_bin[0], hy = 0.5f * (float)k.r_bin[1],
+              hz = 0.5f * (float)k.r_bin[2];
+  int n0 = 0, n1 = 0;  // wave-uniform
+  // sparse cells (a few particles per wave) gain nothing from the box test
+  const bool sparse = !octants || m < LB_MIN_OCTANT;
+  for (int h0 = 0; h0 < (sparse ? 0 : m); h0 += 64) {
+    const int h = h0 + lane;
+    int oc = -1;
+    if (h < m) {
+      const float4 p = hpos[h];
+      oc = (p.x >= hx ? 1 : 0) | (p.y >= hy ? 2 : 0) | (p.z >= hz ? 4 : 0);
+    }
+    const unsigned long long m0 = __builtin_amdgcn_ballot_w64(oc == 2 * w);
+    const unsigned long long m1 = __builtin_amdgcn_ballot_w64(oc == 2 * w + 1);
+    if (oc == 2 * w || oc == 2 * w + 1) {
+      const unsigned long long mm = oc == 2 * w ? m0 : m1;
+      const int at = (oc == 2 * w ? n0 : 32 + n1) +
+                     (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mm, 0u));
+      if (at < 64 && (oc == 2 * w ? at < 32 : true)) hlist[w][at] = h;
+    }
+    n0 += (int)__popcll(m0);
+    n1 += (int)__popcll(m1);
+  }
+  if (lane == 0 && (n0 > 32 || n1 > 32)) atomicOr(&crowded_s, 1);
+  __syncthreads();
+  const bool crowded = sparse || crowded_s != 0;
+  // groups: [first particle in hlist, count]; a crowded cell takes particles
+  // w + 4t as one group with no box test
+  int g_n[2], g_at[2];
+  if (crowded) {
+    g_n[0] = (m - w + LB_WAVES - 1) / LB_WAVES;
+    g_n[1] = 0;
+    g_at[0] = 0;
+    g_at[1] = 0;
+  } else {
+    g_n[0] = n0;
+    g_n[1] = n1;
+    g_at[0] = 0;
+    g_at[1] = 32;
+  }
+  if (g_n[0] + g_n[1] == 0) return;
   const int total = q_first[27];
-  const int self0 = q_first[13];  // the cell's own particles: candidates self0 .. self0 + m - 1
   const float margin = k.lf_margin;
-  int hc = 0;  // row length of this wave's particle t = lane
+  float l2max = 0.0f;
+  for (int q = 0; q < MFA_NTYPE * MFA_NTYPE; ++q) l2max = fmaxf(l2max, l2tab[q]);
+  const float reach2 = l2max + 2.0f * margin;
+  // each group's box: the bounding box of its particles' view positions
+  float bmin[2][3], bmax[2][3];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    if (crowded) break;
+    const bool mine = lane < g_n[g];
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (mine) p = hpos[hlist[w][g_at[g] + lane]];
+    float lo[3] = {mine ? p.x : 3.0e38f, mine ? p.y : 3.0e38f, mine ? p.z : 3.0e38f};
+    float hi[3] = {mine ? p.x : -3.0e38f, mine ? p.y : -3.0e38f, mine ? p.z : -3.0e38f};
+    for (int o = 32; o > 0; o >>= 1)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = fminf(lo[d], __shfl_xor(lo[d], o, 64));
+        hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o, 64));
+      }
+    for (int d = 0; d < 3; ++d) {
+      bmin[g][d] = lo[d];
+      bmax[g][d] = hi[d];
+    }
+  }
+  int hc = 0;  // row length of the wave's particle in this lane's place
+  int head[2] = {0, 0}, tail[2] = {0, 0};  // wave-uniform queue positions
+
+  // Tests 64 candidates (one per lane; view position relative to this cell,
+  // entry) against each particle of group g and appends the hits.
+  auto test_block = [&](int g, const float4 cp, const unsigned entry, const bool live) {
+    const int js_c = (int)(entry & MFA_JMASK);
+    const float far = live ? 0.0f : 1.0e30f;
+    const float pjx = cp.x + far, pjy = cp.y, pjz = cp.z;
+    const int tj = __float_as_int(cp.w) & 3;
+    for (int t = 0; t < g_n[g]; ++t) {
+      const int h = crowded ? w + LB_WAVES * t : hlist[w][g_at[g] + t];
+      const int slot = (g == 0 ? 0 : g_n[0]) + t;
+      const float4 pi = hpos[h];
+      const int ti = __float_as_int(pi.w) & 3;
+      const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      const float l2 = l2tab[ti * MFA_NTYPE + tj];
+      bool hit = false;
+      if (live && js_c != hs0 + h) {
+        if (d2 < l2 - margin) {
+          hit = true;
+        } else if (d2 < l2 + margin) {  // rare: decide in FP64
+          const int i = perm[hs0 + h], j = perm[js_c];
+          double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+          double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+          double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+          min_image(k, ex, ey, ez);
+          const double r2 = (ex * ex + ey * ey) + ez * ez;
+          hit = r2 < k.list_2[ti][tj];
+        }
+      }
+      const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
+      if (mask) {
+        const int base = __builtin_amdgcn_readlane(hc, slot);
+        if (hit) {
+          const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+              (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
+          if (pos < (unsigned)cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
+        }
+        hc = lane == slot ? base + (int)__popcll(mask) : hc;
+      }
+    }
+  };
+  // The queued candidates [head, head + cnt) of group g (cnt <= 64).
+  auto drain = [&](int g, int hd, int cnt) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int e = (hd + lane) & (LB_RING - 1);
+    test_block(g, qpos_w[g * LB_RING + e], qent_w[g * LB_RING + e], lane < cnt);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+
   // candidate e = e0 + lane: cell code q (never decreases along e), sorted
-  // position js; a lane past the end takes the cell's first particle, placed
-  // far away so that it never hits
+  // position js; a lane past the end takes the cell's first particle and is
+  // never queued
   int q = 0;
   bool live = lane < total;
   if (live)
@@ -776,58 +907,43 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
       if (e0 + 64 < total) pj = pos4[js];
     }
     const float4 o = q_off[q_c];
-    // the candidate relative to this cell's origin: d_ij = p_i - (p_j - o)
-    const float far = live_c ? 0.0f : 1.0e30f;
-    const float pjx = pj_c.x - o.x + far, pjy = pj_c.y - o.y, pjz = pj_c.z - o.z;
-    const int tj = __float_as_int(pj_c.w) & 3;
+    // the candidate relative to this cell's origin
+    const float4 cp = make_float4(pj_c.x - o.x, pj_c.y - o.y, pj_c.z - o.z, pj_c.w);
     const unsigned entry = (unsigned)js_c | ((unsigned)q_c << MFA_JBITS);
-    const int erel = e0 + lane - self0;  // = h for the candidate that is particle h itself
-    for (int t0 = 0; t0 < mt; t0 += LB_UNROLL) {
-      bool hit[LB_UNROLL];
+    if (crowded) {  // one group, every candidate
+      test_block(0, cp, entry, live_c);
+      continue;
+    }
 #pragma unroll
-      for (int u = 0; u < LB_UNROLL; ++u) {
-        const int t = t0 + u, h = w + LB_WAVES * t;
-        hit[u] = false;
-        if (t < mt) {
-          const float4 pi = hpos[h];
-          const int ti = __float_as_int(pi.w) & 3;
-          const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          const float l2 = l2tab[ti * MFA_NTYPE + tj];
-          if (erel != h) {
-            if (d2 < l2 - margin) {
-              hit[u] = true;
-            } else if (d2 < l2 + margin) {  // rare: decide in FP64
-              const int i = perm[hs0 + h], j = perm[js_c];
-              double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
-              double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
-              double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
-              min_image(k, ex, ey, ez);
-              const double r2 = (ex * ex + ey * ey) + ez * ez;
-              hit[u] = r2 < k.list_2[ti][tj];
-            }
-          }
-        }
+    for (int g = 0; g < 2; ++g) {
+      if (g_n[g] == 0) continue;
+      const float bx = fmaxf(fmaxf(bmin[g][0] - cp.x, cp.x - bmax[g][0]), 0.0f);
+      const float by = fmaxf(fmaxf(bmin[g][1] - cp.y, cp.y - bmax[g][1]), 0.0f);
+      const float bz = fmaxf(fmaxf(bmin[g][2] - cp.z, cp.z - bmax[g][2]), 0.0f);
+      const bool pass = live_c && bx * bx + by * by + bz * bz < reach2;
+      const unsigned long long pm = __builtin_amdgcn_ballot_w64(pass);
+      if (pass) {
+        const int at = (tail[g] + (int)__builtin_amdgcn_mbcnt_hi(
+                                      (unsigned)(pm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u))) &
+                       (LB_RING - 1);
+        qpos_w[g * LB_RING + at] = cp;
+        qent_w[g * LB_RING + at] = entry;
       }
-#pragma unroll
-      for (int u = 0; u < LB_UNROLL; ++u) {
-        const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit[u]);
-        if (mask) {
-          const int t = t0 + u, h = w + LB_WAVES * t;
-          const int base = __builtin_amdgcn_readlane(hc, t);
-          if (hit[u]) {
-            const unsigned pos = __builtin_amdgcn_mbcnt_hi(
-                (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
-            if (pos < (unsigned)cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
-          }
-          hc = lane == t ? base + (int)__popcll(mask) : hc;
-        }
+      tail[g] += (int)__popcll(pm);
+      if (tail[g] - head[g] >= 64) {
+        drain(g, head[g], 64);
+        head[g] += 64;
       }
     }
   }
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+    if (tail[g] > head[g]) drain(g, head[g], tail[g] - head[g]);
+  const int nw = g_n[0] + g_n[1];
   int my_max = 0;
-  if (lane < mt) {
-    const int h = w + LB_WAVES * lane;
+  if (lane < nw) {
+    const int g = lane < g_n[0] ? 0 : 1, t = lane - (g == 0 ? 0 : g_n[0]);
+    const int h = crowded ? w + LB_WAVES * t : hlist[w][g_at[g] + t];
     nn[hs0 + h] = hc < cap ? hc : cap;
     if (hc > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
     my_max = hc;
@@ -1930,6 +2046,14 @@ static int launch_ewald(mfa_ctx* c, int add_dip);
 
 
 
+// Dynamic LDS of the list build: the cell's particles, and with the octant
+// groups the candidate queues (per wave and group, LB_RING float4 + entry).
+static size_t list_lds_bytes(const mfa_ctx* c) {
+  size_t b = (size_t)c->cell_cap * sizeof(float4);
+  if (c->lb_octants) b += (size_t)LB_WAVES * 2 * LB_RING * (sizeof(float4) + sizeof(unsigned));
+  return b;
+}
+
 // A rebuild: cells of every particle, their sorted ranges, the spatial order
 // and FP32 view, then the lists.  Each kernel returns at once unless the
 // replica's rebuild flag is set.
@@ -1943,10 +2067,9 @@ static int launch_rebuild(mfa_ctx* c) {
                      c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->v, c->type, c->mic,
                      c->cell_count, c->cell_slots, c->cell_start, c->perm, c->sidx, c->r_ref,
                      c->flags, c->pos4, c->vel4);
-  hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK),
-                     (size_t)c->cell_cap * sizeof(float4), c->stream, c->d_kp,
-                     c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm, c->pos4, c->nl,
-                     c->nn, c->cap, c->flags);
+  hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
+                     c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm, c->pos4,
+                     c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -2491,6 +2614,15 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
     return set_err(nullptr, MFA_ERR_ARG, "at most %u particles per context", MFA_JMASK);
   }
   for (KParams& k : c->ks) k.cell_cap = cell_cap;
+  {
+    // octant groups pay off where cells hold many particles
+    int64_t np = 0, nc = 0;
+    for (int q = 0; q < n_rep; ++q) {
+      np += ps[q].n_part;
+      nc += (int64_t)ps[q].n_bin[0] * ps[q].n_bin[1] * ps[q].n_bin[2];
+    }
+    c->lb_octants = nc > 0 && np >= (int64_t)LB_MIN_OCTANT * nc;
+  }
   c->cap = cap;
   c->cell_cap = cell_cap;
   c->n = (int)ntot;
