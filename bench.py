@@ -40,7 +40,7 @@ STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
 # HBM bytes per force pass of the mixed form on this workload, from the
 # FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
 # each scaled by the same counter's reading for a copy of known size.
-FORCE_TRAFFIC_BYTES = 77.574e6 + 3.416e6
+FORCE_TRAFFIC_BYTES = 77.339e6 + 3.411e6
 
 
 def log(*a):
@@ -196,7 +196,7 @@ def bench_single(args, name):
         "cpu_baseline": cpu,
     }
     return contract_line(args, name, value, dt_m / args.steps * 1e3,
-                         "fp32 pair math, int64 fixed-point sums, fp64 state", data, config, extra)
+                         "fp32 pair math, exact fixed-point sums, fp64 state", data, config, extra)
 
 
 def bench_batch(args, n_rep=256):
@@ -253,7 +253,7 @@ def bench_batch(args, n_rep=256):
     data = ("synthetic (seeded replicas, fresh starts timed inside the 40-tau switch-on ramp "
             "of the reference's start-up; the clamp changes no work per step)")
     return contract_line(args, "batch256", value, dt_b / args.steps * 1e3,
-                         "fp32 pair math, int64 fixed-point sums, fp64 state", data, config, extra)
+                         "fp32 pair math, exact fixed-point sums, fp64 state", data, config, extra)
 
 
 def bench_adjoint(args):
