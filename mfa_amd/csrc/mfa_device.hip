@@ -94,6 +94,7 @@ struct KParams {
   int n, n_mon, n_chain, n_brush, part_init_d, n_mon_d, part_init_e, n_mon_e;
   int symmetry, pinned_heads, dpd_on, dpd_cut_off_on, dpd_head_excl;
   int dpd_vv;  // DPD_VV: heads move, new_dpd_fd after verlet_velocities
+  int lgv;     // THERMOSTAT 1: Langevin force on every particle instead of DPD
   uint32_t seed;
 };
 
@@ -1019,6 +1020,24 @@ __device__ __forceinline__ float pair_normal_f32(uint32_t seed, int64_t step, in
   return __builtin_amdgcn_sqrtf(-1.3862943611198906f * l2) * __builtin_amdgcn_cosf(u2);
 }
 
+// Langevin force on particle il (THERMOSTAT 1; lgv_forces.f90:25-43, called
+// for every particle at the end of its fluid_fluid row, fluid_fluid.f90:219-221):
+// per component (sig * xi_d) * sqrt(m) - (gamma_1 m) v_d, xi_d the particle's
+// counter deviate d (the reference draws rnor() three times).
+__device__ __forceinline__ void langevin_force(const KParams& k, int64_t step, int il, int ti,
+                                               double vx, double vy, double vz, double* fl) {
+  const double m = k.mass_type[ti], sm = sqrt(m);
+  const double vv[3] = {vx, vy, vz};
+  for (int d = 0; d < 3; ++d) {
+    uint32_t w0, w1;
+    mfa_particle_words(k.seed, step, il, d, &w0, &w1);
+    const double u1 = (double)(w0 >> 8) * 0x1p-24 + 0x1p-25;
+    const double u2 = (double)(w1 >> 8) * 0x1p-24;
+    const double xi = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    fl[d] = (k.sig * xi) * sm - (k.friction1 * m) * vv[d];
+  }
+}
+
 // Switch-on clamp of r^2 (fluid_fluid.f90:42-49) with r_time = step*dt
 // (md_main.f90:69, step = s_time + i_time - 1); 0 once the ramp is over.
 __device__ __forceinline__ double switch_on_r2min(const KParams& k, int64_t step) {
@@ -1242,7 +1261,14 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
     }
   }
 
-  // Bonds and walls: lane 0 of the group (at most two bonds per particle).
+  // Langevin, bonds and walls: lane 0 of the group (at most two bonds per particle).
+  if (active && sub == 0 && k.lgv && (mask & TM_PAIR)) {
+    double fl[3];
+    langevin_force(k, step, il, ti, vxi, vyi, vzi, fl);
+    ax += fl[0];
+    ay += fl[1];
+    az += fl[2];
+  }
   if (active && sub == 0) {
     if (mask & TM_BOND) {
       int base, len;
@@ -1455,7 +1481,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     }
   }
 
-  // Bonds and walls: lane 0 of the group, from the FP64 positions.
+  // Langevin, bonds and walls: lane 0 of the group, from the FP64 positions.
+  if (active && sub == 0 && k.lgv && (mask & TM_PAIR)) {
+    double fl[3];
+    langevin_force(k, step, il, ti, (double)vi4.x, (double)vi4.y, (double)vi4.z, fl);
+    fix_add3(ax, ay, az, (float)fl[0], (float)fl[1], (float)fl[2]);
+  }
   if (active && sub == 0 && (ENERGY || (mask & (TM_BOND | TM_WALL)))) zi = r[3 * (size_t)i + 2];
   if (XT && active && sub == 0 && (mask & TM_BEND) && k.bending && il < k.n_brush) {
     // each term through float and fixed point; the centre takes minus the
@@ -2033,6 +2064,7 @@ static KParams make_kparams(const mfa_params& p) {
   // Heads pinned for brush systems in a channel (verlet_velocities.f90:113-134).
   k.pinned_heads = (p.system == 3 && p.symmetry == 0) ? 1 : 0;
   k.dpd_on = p.thermostat == 0 ? 1 : 0;
+  k.lgv = p.thermostat == 1 ? 1 : 0;
   k.dpd_cut_off_on = p.dpd_cut_off_on;
   // DPD_CUT_OFF excludes heads for brush systems in a channel (fluid_fluid.f90:179-185).
   k.dpd_head_excl = (p.system == 3 && p.symmetry == 0) ? 1 : 0;
@@ -3772,6 +3804,7 @@ extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_
   if (p.coulomb || p.bending) return set_err(c, MFA_ERR_ARG, "adjoint: charges and bending are not covered");
   if (p.fso_time > 0.0) return set_err(c, MFA_ERR_ARG, "adjoint: the switch-on ramp is not covered");
   if (c->dpd_vv) return set_err(c, MFA_ERR_ARG, "adjoint: DPD_VV is not covered");
+  if (p.thermostat == 1) return set_err(c, MFA_ERR_ARG, "adjoint: the Langevin thermostat is not covered");
   if (p.symmetry != 0) return set_err(c, MFA_ERR_ARG, "adjoint: the observables need the walls (SYMMETRY 0)");
   // theta: the common epsilon and sigma of the tables
   AdjK ak;

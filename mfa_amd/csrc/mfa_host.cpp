@@ -45,8 +45,8 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   if (in->system != 2 && in->system != 3) return derive_fail("SYSTEM must be 2 or 3");
   if (in->symmetry != 0 && in->symmetry != 1) return derive_fail("SYMMETRY must be 0 or 1");
   if (in->solvent != 0 && in->solvent != 1) return derive_fail("SOLVENT must be 0 or 1");
-  if (in->thermostat != 0 && in->thermostat != -1)
-    return derive_fail("THERMOSTAT must be 0 (DPD) or -1 (none)");
+  if (in->thermostat != 0 && in->thermostat != 1 && in->thermostat != -1)
+    return derive_fail("THERMOSTAT must be 0 (DPD), 1 (Langevin) or -1 (none)");
   if (in->dt <= 0.0 || in->r_chain <= 0.0) return derive_fail("dt and r_chain must be positive");
   // SYSTEM 3 needs an even number of brush chains (init_params.f90:878-884).
   if (in->system == 3 && (in->n_chain % 2) != 0)

@@ -62,3 +62,13 @@ MFA_HD void mfa_pair_words(uint32_t seed, int64_t step, int32_t i, int32_t j, ui
   const uint32_t hi = (uint32_t)(i < j ? j : i);
   mfa_threefry2x32(lo, hi, seed, (uint32_t)step, MFA_TF_ROUNDS, w0, w1);
 }
+
+// The two words of particle i's deviate d (0, 1, 2: the components of the
+// Langevin random force, lgv_forces.f90:25-31) at `step`.  The second counter
+// word has its top bit set, which no pair counter has (particle indices stay
+// below 2^27), so particle and pair streams never meet.
+MFA_HD void mfa_particle_words(uint32_t seed, int64_t step, int32_t i, int d, uint32_t* w0,
+                               uint32_t* w1) {
+  mfa_threefry2x32((uint32_t)i, 0x80000000u | (uint32_t)d, seed, (uint32_t)step, MFA_TF_ROUNDS, w0,
+                   w1);
+}

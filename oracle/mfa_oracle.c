@@ -77,6 +77,19 @@ double orc_pair_normal(uint32_t seed, int64_t step, int i0, int j0) {
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+/* Normal deviate d (0, 1, 2) of particle i0 (0-based id) at `step`, for the
+ * Langevin random force (lgv_forces.f90:25-31, one rnor() per component):
+ * the same Box-Muller transform on Threefry-2x32-20 with counter
+ * (i0, 2^31 + d), a counter no pair uses (pair counters stay below 2^27).
+ * Pinned by moment tests and by the FP64 GPU form on identical keys. */
+double orc_particle_normal(uint32_t seed, int64_t step, int i0, int d) {
+  uint32_t w0, w1;
+  orc_threefry2x32((uint32_t)i0, 0x80000000u | (uint32_t)d, seed, (uint32_t)step, 20, &w0, &w1);
+  const double u1 = (double)(w0 >> 8) * (1.0 / 16777216.0) + (1.0 / 33554432.0);
+  const double u2 = (double)(w1 >> 8) * (1.0 / 16777216.0);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Derived constants: init_params.f90:395-845 for SOLVENT 0/1, WALL 2,
  * SYSTEM 2/3 (with and without PARTICLE_4), make_binning_boxes.f90:14-64 and
@@ -682,6 +695,13 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
       }
     }
     for (int d = 0; d < 3; ++d) F(ip, d) = F(ip, d) + f_ipart[d];
+    if (p->thermostat == 1) { /* lgv_forces (fluid_fluid.f90:219-221, lgv_forces.f90:25-43) */
+      const double m = p->mass_type[ti];
+      double vec[3];
+      for (int d = 0; d < 3; ++d) vec[d] = p->sig * orc_particle_normal(p->seed, step, ip, d) * sqrt(m);
+      for (int d = 0; d < 3; ++d) vec[d] = vec[d] - p->friction1 * m * V(ip, d);
+      for (int d = 0; d < 3; ++d) F(ip, d) = F(ip, d) + vec[d];
+    }
   }
   out[0] = v_ff;
   out[1] = vir;

@@ -59,6 +59,7 @@ typedef struct orc_sys orc_sys;
 void orc_threefry2x32(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1, int rounds,
                       uint32_t* o0, uint32_t* o1);
 double orc_pair_normal(uint32_t seed, int64_t step, int i0, int j0);
+double orc_particle_normal(uint32_t seed, int64_t step, int i0, int d);
 
 int orc_derive(const mfa_inputs* in, orc_params* p);
 orc_sys* orc_create(const orc_params* p, int n_neigh);

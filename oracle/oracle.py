@@ -70,6 +70,8 @@ def load():
     lib.orc_threefry2x32.restype = None
     lib.orc_pair_normal.argtypes = [u32, C.c_int64, C.c_int, C.c_int]
     lib.orc_pair_normal.restype = C.c_double
+    lib.orc_particle_normal.argtypes = [u32, C.c_int64, C.c_int, C.c_int]
+    lib.orc_particle_normal.restype = C.c_double
     lib.orc_derive.argtypes = [C.c_void_p, C.POINTER(OrcParams)]
     lib.orc_derive.restype = C.c_int
     lib.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int]
@@ -118,6 +120,10 @@ def threefry2x32(c0, c1, k0, k1, rounds=20):
 
 def pair_normal(seed, step, i, j):
     return load().orc_pair_normal(seed, step, i, j)
+
+
+def particle_normal(seed, step, i, d):
+    return load().orc_particle_normal(seed, step, i, d)
 
 
 def derive(inputs):

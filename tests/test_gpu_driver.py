@@ -17,21 +17,26 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROG = os.path.join(ROOT, "mfa_amd", "mfa_hip_prog")
 
 
-@pytest.mark.parametrize("case,mode,nsteps,vv", [("brush_small", "compat", 30, 0),
-                                                  ("brush_small", "fused", 30, 0),
-                                                  ("melt_small", "fused", 20, 0),
-                                                  ("brush_small", "compat", 30, 1),
-                                                  ("brush_small", "fused", 30, 1)])
-def test_fortran_driver_matches_oracle(tmp_path, case, mode, nsteps, vv):
-    """vv: DPD_VV (new_dpd_fd after each verlet_velocities)."""
+@pytest.mark.parametrize("case,mode,nsteps,variant", [("brush_small", "compat", 30, ""),
+                                                       ("brush_small", "fused", 30, ""),
+                                                       ("melt_small", "fused", 20, ""),
+                                                       ("brush_small", "compat", 30, "vv"),
+                                                       ("brush_small", "fused", 30, "vv"),
+                                                       ("melt_small", "compat", 20, "lgv"),
+                                                       ("melt_small", "fused", 20, "lgv")])
+def test_fortran_driver_matches_oracle(tmp_path, case, mode, nsteps, variant):
+    """variant: vv = DPD_VV (new_dpd_fd after each verlet_velocities), lgv =
+    the Langevin thermostat (THERMOSTAT 1)."""
     assert os.path.exists(PROG), "Fortran driver not built (python -m mfa_amd.build)"
     kind, inp = configs.small_cases()[case]
-    inp.dpd_vv = vv
+    inp.dpd_vv = 1 if variant == "vv" else 0
+    if variant == "lgv":
+        inp.thermostat = 1
     p, r, v, t = configs.build(inp, kind)
     inp.n_relax, inp.n_obser, inp.n_safe = 0, nsteps, 1000
     d = str(tmp_path)
     io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="fp64", mode=mode)
-    print("driver", case, mode, "dpd_vv", vv, flush=True)
+    print("driver", case, mode, variant, flush=True)
     res = subprocess.run([PROG], cwd=d, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     out = io.read_state_sidecar(os.path.join(d, "conf_new.bin"))

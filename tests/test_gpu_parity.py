@@ -323,3 +323,66 @@ def test_dpd_vv_graph_replays_match_direct_launches():
         assert oe["t_kin"] == og["t_kin"]
     eager.close()
     graph.close()
+
+
+def _lgv_case(name):
+    """A small case with the Langevin thermostat (THERMOSTAT 1)."""
+    kind, inputs = configs.small_cases()[name]
+    inputs.thermostat = 1
+    p, r, v, t = configs.build(inputs, kind)
+    return inputs, p, r, v, t, np.zeros_like(r)
+
+
+@pytest.mark.parametrize("case", ["brush_small", "melt_small"])
+def test_langevin_trajectory_fp64(case):
+    """Langevin force (lgv_forces.f90:25-43) with the same counter deviates as
+    the oracle: FP64 trajectories within 1e-9 after 40 steps."""
+    inputs, p, r, v, t, a = _lgv_case(case)
+    s = oracle_sys(inputs, r, v, a, t)
+    ref = s.run_steps(first_step=1, n=40)
+    st_ref = s.get_state()
+    ctx = gpu_ctx(p, r, v, a, t)
+    obs = ctx.run_steps(first_step=1, n=40)
+    st = ctx.get_state()
+    dr = np.abs(st["r"] - st_ref["r"]).max()
+    dv = np.abs(st["v"] - st_ref["v"]).max()
+    print(case, "dr", dr, "dv", dv)
+    assert np.array_equal(st["mic"], st_ref["mic"])
+    assert dr < 1e-9 and dv < 1e-8
+    assert obs["n_rebuilds"] == ref["n_rebuilds"]
+    for k in ("v_fluid_fluid", "v_intra_molec", "t_kin"):
+        assert rel(obs[k], ref[k]) < 1e-9, (k, obs[k], ref[k])
+    ctx.close()
+
+
+def test_langevin_forces_mixed():
+    inputs, p, r, v, t, a = _lgv_case("melt_small")
+    s = oracle_sys(inputs, r, v, a, t)
+    s.binning()
+    fref, _ = s.forces(step=5)
+    ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    ctx.energies(step=5)
+    f = ctx.get_state()["f"]
+    per, rms = force_error(f, fref)
+    print("langevin mixed: max", per, "rms", rms)
+    assert rms < 1e-5 and per < 1e-3
+    ctx.close()
+
+
+def test_langevin_holds_the_temperature():
+    """The thermostat's fluctuation-dissipation balance: a melt started at
+    twice the target temperature relaxes to it (mean kinetic temperature over
+    the last 1500 of 2500 steps within 3% of T)."""
+    inputs, p, r, v, t, a = _lgv_case("melt_small")
+    ctx = gpu_ctx(p, r, 2.0 ** 0.5 * v, a, t, precision=lib.MFA_MIXED)
+    step, temps = 1, []
+    dof = 3.0 * p.n_part
+    for chunk in range(50):
+        obs = ctx.run_steps(step, 50)
+        step += 50
+        if chunk >= 20:
+            temps.append(2.0 * obs["t_kin"] / dof)
+    tm = float(np.mean(temps))
+    print("Langevin: T target %.3f measured %.4f" % (inputs.temp, tm))
+    assert abs(tm / inputs.temp - 1.0) < 0.03
+    ctx.close()

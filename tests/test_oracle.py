@@ -55,6 +55,30 @@ def test_pair_normal_moments_and_symmetry():
     assert orc.pair_normal(5, 3, 10, 20) != orc.pair_normal(5, 4, 10, 20)
 
 
+def test_particle_normal_moments_and_independence():
+    """The Langevin deviates (one per particle and component): unit normal
+    moments, no correlation between components, and a stream apart from the
+    pair deviates (counter word 1 has its top bit set)."""
+    n = 70000
+    xs = np.array([[orc.particle_normal(1234, 7, i, d) for d in range(3)] for i in range(n)])
+    for d in range(3):
+        x = xs[:, d]
+        assert abs(x.mean()) < 0.015
+        assert abs(x.var() - 1.0) < 0.015
+        k = ((x - x.mean()) ** 4).mean() / x.var() ** 2
+        assert abs(k - 3.0) < 0.08
+    c = np.corrcoef(xs.T)
+    assert np.abs(c - np.eye(3)).max() < 0.015
+    # new numbers per step and per seed; apart from the pair stream
+    assert orc.particle_normal(5, 3, 10, 0) != orc.particle_normal(5, 4, 10, 0)
+    assert orc.particle_normal(5, 3, 10, 0) != orc.particle_normal(6, 3, 10, 0)
+    w = orc.threefry2x32(10, 0x80000000, 5, 3)
+    u1 = (w[0] >> 8) * 2.0 ** -24 + 2.0 ** -25
+    u2 = (w[1] >> 8) * 2.0 ** -24
+    assert abs(orc.particle_normal(5, 3, 10, 0) -
+               np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)) < 1e-14
+
+
 # ---------------------------------------------------------------- constants
 
 def test_derived_constants_float32_literals():

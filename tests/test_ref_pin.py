@@ -39,8 +39,12 @@ CASES = {
     "brush_bend": ("brush_bend", "ref_md_brush_bend", 3, 20),
     # DPD_VV: new_dpd_fd after each verlet_velocities
     "brush_vv": ("brush_small", "ref_md_brush_vv", 3, 20),
+    # THERMOSTAT 1: lgv_forces on every particle (its friction; the random
+    # part is sig * rnor() with sig = 0 at T = 0)
+    "brush_lgv": ("brush_small", "ref_md_brush_lgv", 3, 20),
 }
 DPD_VV = {"brush_vv"}
+LANGEVIN = {"brush_lgv"}
 
 
 def _read(path, n, init):
@@ -79,6 +83,8 @@ def run(request, tmp_path_factory):
     inp.skin = 0.0      # the reference's hard-coded 0.4 (init_params.f90:800)
     inp.temp = 0.0
     inp.dpd_vv = 1 if request.param in DPD_VV else 0
+    if request.param in LANGEVIN:
+        inp.thermostat = 1
     p, r, v, t = configs.build(inp, kind)
     d = str(tmp_path_factory.mktemp(request.param))
     io.write_run_dir(d, inp, t, r, v, np.zeros_like(r))
