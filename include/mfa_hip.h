@@ -156,6 +156,25 @@ typedef struct mfa_obs {
   int64_t n_rebuilds;     /* list rebuilds during the call */
 } mfa_obs;
 
+/* Running sums of the observation stage (observation.f90:17-307 as built
+ * with PROFILES, SYSTEM 2/3, WALL 2: no wall atoms, so the wall-wall sums
+ * of the reference stay zero and are left out). */
+typedef struct mfa_obs_sums {
+  int64_t count_obs;                        /* observation.f90:17 */
+  double v_total_1, v_total_2;              /* :146-152 */
+  double t_total_1, t_total_2;
+  double e_total_1, e_total_2;
+  double v_fluid_fluid_1, v_fluid_fluid_2;  /* :138-144, v_ff + v_intra + v_fw */
+  double t_fluid_1, t_fluid_2;
+  double e_fluid_fluid_1, e_fluid_fluid_2;
+  double v_tot_intra_1, v_tot_intra_2;      /* :158-159 */
+  double tot_p[3];                          /* :286-291, sum of m v per observation */
+  double r_g2_mean[3][3];                   /* :271-273, [brush, melt, (brush+melt)/2][x,y,z] */
+  double r_par_per2[2];                     /* :275-276 */
+  int64_t count_hist;                       /* functions.f90:144, histo mode 2 calls */
+  int32_t hist_dim, pad;                    /* functions.f90:115, dim_prof_dens */
+} mfa_obs_sums;
+
 typedef struct mfa_ctx mfa_ctx;
 
 /* ABI self-check: sizes of the structs as compiled into the library. */
@@ -279,6 +298,26 @@ int32_t mfa_set_state_replica(mfa_ctx* ctx, int32_t replica, const double* r0, c
 int32_t mfa_get_state_replica(mfa_ctx* ctx, int32_t replica, double* r0, double* v, double* a,
                               double* force, int32_t* mic);
 int32_t mfa_replica_obs(mfa_ctx* ctx, int32_t replica, mfa_obs* obs);
+/* Observation stage on the device.  Replaces observation.f90:17-307 (with
+ * PROFILES: the density histogram of functions.f90:94-215) and the head-force
+ * capture of verlet_velocities.f90:14-19, without copying the state to the
+ * host: the sums stay in device memory until mfa_obs_read.
+ *   mfa_obs_init: zeroes the sums and sets up a histogram of hist_dim slabs
+ *     (init_obser.f90:15-91, histo mode 0; the reference uses 200).
+ *   mfa_observe: one observation of the current state, after a step
+ *     (md_main.f90:159-161).  e holds that step's energies (from
+ *     mfa_run_steps / mfa_verlet_velocities); coul_in_total = 1 on the steps
+ *     where the reference's verlet_velocities has just added v_coul to
+ *     v_total (every tenth call, verlet_velocities.f90:199-222, SYSTEM 2/3).
+ *   mfa_obs_read: the sums; dens_z (may be NULL) receives the raw slab counts
+ *     [3][hist_dim] (brush, melt, particle 4; histo mode 2), f_on_heads (may
+ *     be NULL) the force on each brush head [n_chain][3] as the last
+ *     verlet_velocities found it.
+ * Single-system contexts only. */
+int32_t mfa_obs_init(mfa_ctx* ctx, int32_t hist_dim);
+int32_t mfa_observe(mfa_ctx* ctx, const mfa_obs* e, int32_t coul_in_total);
+int32_t mfa_obs_read(mfa_ctx* ctx, mfa_obs_sums* out, double* dens_z, double* f_on_heads);
+
 /* Autotuner: for each number of threads per particle in {1, 2, 4, 8, 16}
  * runs n_steps from first_step (advancing the state, so call it during
  * warm-up), then times 20 force passes on the state they left; keeps the

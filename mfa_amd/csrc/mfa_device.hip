@@ -135,6 +135,14 @@ struct mfa_ctx {
   int32_t *perm, *sidx;  // sorted position -> particle, particle -> sorted position
   int32_t* fill_cnt;     // completion counters of the cell fill (zero between launches)
   bool lb_octants;       // list build with octant groups (dense cells)
+  // observation stage (mfa_obs_init): head forces of the last
+  // verlet_velocities, slab counts, per-chain Rg^2, device sums
+  double* head_f;
+  unsigned* d_hist;
+  double *d_chain_rg, *d_obs;
+  int hist_dim;
+  bool obs_on;
+  mfa_obs_sums obs_host;  // the energy sums, kept on the host
   int cell_cap;
   unsigned* nl;
   int32_t* nn;
@@ -444,7 +452,8 @@ __global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* _
                                     double* __restrict__ f, double* __restrict__ v,
                                     double* __restrict__ a, const int* __restrict__ type,
                                     double* __restrict__ part, int energy,
-                                    const int* __restrict__ sidx, float4* __restrict__ vel4) {
+                                    const int* __restrict__ sidx, float4* __restrict__ vel4,
+                                    double* __restrict__ head_f) {
   MFA_REPLICA(blockIdx.x * blockDim.x);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double vals[MFA_NTERMS];
@@ -456,6 +465,10 @@ __global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* _
       fi[d] = f[3 * (size_t)i + d];
       vi[d] = v[3 * (size_t)i + d];
     }
+    // f_on_heads = force(:, heads) before the heads are pinned
+    // (verlet_velocities.f90:14-19); single-system contexts
+    if (head_f && is_head(k, i - k.p0))
+      for (int d = 0; d < 3; ++d) head_f[3 * (size_t)((i - k.p0) / k.n_mon) + d] = fi[d];
     vv_one(k, i - k.p0, ti, fi, vi, ai);
     for (int d = 0; d < 3; ++d) {
       v[3 * (size_t)i + d] = vi[d];
@@ -2329,7 +2342,7 @@ static int enqueue_middle_step(mfa_ctx* c) {
   if (!c->dpd_vv) return launch_fused(c, 0);
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, 0, c->sidx, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, 0, c->sidx, c->vel4, nullptr);
   rc = launch_dpd_vv(c, 0, 1);
   if (rc) return rc;
   hipLaunchKernelGGL(k_verlet_positions, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
@@ -2799,7 +2812,8 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
                   c->mic, c->cell_of, c->cell_count, c->cell_slots, c->nl,    c->nn,
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
                   c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx,
-                  c->fd,   c->fd_new, c->fill_cnt};
+                  c->fd,   c->fd_new, c->fill_cnt, c->head_f, c->d_hist, c->d_chain_rg,
+                  c->d_obs};
   for (void* q : ptrs)
     if (q) hipFree(q);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
@@ -3073,7 +3087,8 @@ extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
   if (!c) return MFA_ERR_ARG;
   const int g = grid_of(c->n);
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, t_kin ? 1 : 0, c->sidx, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, t_kin ? 1 : 0, c->sidx, c->vel4,
+                     c->head_f);
   HIPCHK(c, hipGetLastError());
   if (t_kin) {
     double t[MFA_NTERMS];
@@ -3244,7 +3259,7 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
                      c->f, c->v, c->a, c->type, c->part, (obs && !c->dpd_vv) ? 1 : 0, c->sidx,
-                     c->vel4);
+                     c->vel4, c->head_f);
   HIPCHK(c, hipGetLastError());
   if (c->dpd_vv) {  // md_main.f90:149-155; the kinetic energy comes from new_dpd_fd
     rc = launch_dpd_vv(c, obs ? 1 : 0, 0);
@@ -3265,6 +3280,229 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
                c->h_flags[(size_t)q * F_NFLAGS + F_NREBUILD], &c->rep_obs[(size_t)q]);
     *obs = c->rep_obs[0];
   }
+  return MFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Observation stage (observation.f90:17-307 with PROFILES; functions.f90
+// histo).  Per particle: the momentum partials m v and the slab counts of z
+// (brush, melt, particle 4).  Per chain: the chain unfolded along its bonds
+// (one image correction per monomer and axis, observation.f90:163-199) and
+// its Rg^2 about its centre (:205-233).  A one-block kernel then adds the
+// observation to the device sums in a fixed order.
+__global__ __launch_bounds__(MFA_BLOCK) void k_obs_particles(
+    const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
+    const double* __restrict__ v, const int* __restrict__ type, unsigned* __restrict__ hist,
+    int hist_dim, double inv_z_step, double* __restrict__ part) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double vals[MFA_NTERMS];
+  for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+  if (i - k.p0 < k.n) {
+    const double m = k.mass_type[type[i] - 1];
+    for (int d = 0; d < 3; ++d) vals[d] = m * v[3 * (size_t)i + d];
+    if (hist_dim > 0) {
+      const int il = i - k.p0;
+      const int g = il < k.n_brush ? 0 : (il < k.part_init_e ? 1 : 2);
+      int zi = (int)(r[3 * (size_t)i + 2] * inv_z_step) + 1;  // functions.f90:148-156
+      zi = zi < 1 ? 1 : (zi > hist_dim ? hist_dim : zi);
+      atomicAdd(&hist[(size_t)g * hist_dim + (zi - 1)], 1u);
+    }
+  }
+  block_partials(vals, part);
+}
+
+// One chain's unfolded monomer m: positions from the chain start, each
+// corrected by one box length when it sits more than half a box from the
+// (corrected) previous one along an axis.
+__global__ __launch_bounds__(MFA_BLOCK) void k_obs_chains(const KParams* __restrict__ kb,
+                                                          const double* __restrict__ r,
+                                                          double* __restrict__ chain_rg) {
+  const KParams& k = kb[0];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_chain_d = (k.part_init_e - k.n_brush) / (k.n_mon_d > 0 ? k.n_mon_d : 1);
+  if (c >= k.n_chain + n_chain_d) return;
+  const bool brush = c < k.n_chain;
+  const int len = brush ? k.n_mon : k.n_mon_d;
+  const int base = brush ? c * k.n_mon : k.n_brush + (c - k.n_chain) * k.n_mon_d;
+  const bool unfold = brush ? k.symmetry == 0 : true;
+  double half[3];
+  for (int d = 0; d < 3; ++d) half[d] = k.L[d] / 2.;  // init_config.f90:22
+  double cm[3] = {0., 0., 0.}, vd[3] = {0., 0., 0.};
+  for (int pass = 0; pass < 2; ++pass) {
+    double prev[3] = {0., 0., 0.};
+    for (int mm = 0; mm < len; ++mm) {
+      const int i = base + mm;
+      double x[3];
+      for (int d = 0; d < 3; ++d) {
+        x[d] = r[3 * (size_t)i + d];
+        // mod(i_part-1, n_mon) on the global index, as the reference does
+        const bool first = brush ? (i % k.n_mon) == 0 : (i % k.n_mon_d) == 0;
+        if (unfold && !first) {
+          if (x[d] - prev[d] > half[d]) x[d] = x[d] - k.L[d];
+          else if (prev[d] - x[d] > half[d]) x[d] = x[d] + k.L[d];
+        }
+        prev[d] = x[d];
+        if (pass == 0) {
+          cm[d] = cm[d] + x[d];
+        } else {
+          const double y = x[d] - cm[d];
+          vd[d] = vd[d] + y * y;
+        }
+      }
+    }
+    if (pass == 0)
+      for (int d = 0; d < 3; ++d) cm[d] = brush ? cm[d] * (1. / (double)k.n_mon) : cm[d] / (double)k.n_mon_d;
+  }
+  for (int d = 0; d < 3; ++d)
+    chain_rg[3 * (size_t)c + d] = brush ? vd[d] * (1. / (double)k.n_mon) : vd[d] / (double)k.n_mon_d;
+}
+
+// Adds one observation to the device sums: obs[0..2] tot_P, [3..11]
+// r_g2_mean, [12..13] r_par_per2 (observation.f90:232-291).  Fixed-order sums.
+__global__ __launch_bounds__(MFA_BLOCK) void k_obs_finish(const KParams* __restrict__ kb,
+                                                          const double* __restrict__ part, int nblocks,
+                                                          const double* __restrict__ chain_rg,
+                                                          int with_rg, double* __restrict__ obs) {
+  const KParams& k = kb[0];
+  __shared__ double sh[MFA_BLOCK][3];
+  const int t = threadIdx.x;
+  const int n_chain_d = (k.part_init_e - k.n_brush) / (k.n_mon_d > 0 ? k.n_mon_d : 1);
+  double res[3][3];  // momentum, brush Rg^2 sum, melt Rg^2 sum
+  for (int q = 0; q < 3; ++q) {
+    double a[3] = {0., 0., 0.};
+    if (q == 0) {
+      for (int b = t; b < nblocks; b += MFA_BLOCK)
+        for (int d = 0; d < 3; ++d) a[d] += part[(size_t)b * MFA_NTERMS + d];
+    } else if (with_rg) {
+      const int c0 = q == 1 ? 0 : k.n_chain, c1 = q == 1 ? k.n_chain : k.n_chain + n_chain_d;
+      for (int c = c0 + t; c < c1; c += MFA_BLOCK)
+        for (int d = 0; d < 3; ++d) a[d] += chain_rg[3 * (size_t)c + d];
+    }
+    for (int d = 0; d < 3; ++d) sh[t][d] = a[d];
+    __syncthreads();
+    for (int o = MFA_BLOCK / 2; o > 0; o >>= 1) {
+      if (t < o)
+        for (int d = 0; d < 3; ++d) sh[t][d] += sh[t + o][d];
+      __syncthreads();
+    }
+    for (int d = 0; d < 3; ++d) res[q][d] = sh[0][d];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  for (int d = 0; d < 3; ++d) obs[d] = obs[d] + res[0][d];
+  if (!with_rg) return;
+  double rg[2][3];
+  for (int d = 0; d < 3; ++d) {
+    rg[0][d] = res[1][d] * (1. / (double)k.n_chain);
+    rg[1][d] = res[2][d] / (double)n_chain_d;  // 0/0 without melt chains, as in the reference
+  }
+  for (int d = 0; d < 3; ++d) {
+    obs[3 + d] = obs[3 + d] + rg[0][d];
+    obs[6 + d] = obs[6 + d] + rg[1][d];
+    obs[9 + d] = obs[9 + d] + (rg[0][d] + rg[1][d]) / 2.0;
+  }
+  for (int g = 0; g < 2; ++g) {
+    const double sum = (rg[g][0] + rg[g][1]) + rg[g][2];
+    obs[12 + g] = obs[12 + g] + (rg[g][2] - 0.5 * (rg[g][0] + rg[g][1])) / sum;
+  }
+}
+
+extern "C" int32_t mfa_obs_init(mfa_ctx* c, int32_t hist_dim) {
+  if (!c) return MFA_ERR_ARG;
+  if (c->n_rep != 1) return set_err(c, MFA_ERR_ARG, "observation: single-system contexts only");
+  if (hist_dim < 0 || hist_dim > (1 << 20)) return set_err(c, MFA_ERR_ARG, "observation: bad hist_dim %d", hist_dim);
+  const mfa_params& p = c->p;
+  if (!c->head_f) HIPCHK(c, hipMalloc(&c->head_f, 3 * (size_t)(p.n_chain > 0 ? p.n_chain : 1) * sizeof(double)));
+  if (!c->d_obs) HIPCHK(c, hipMalloc(&c->d_obs, 16 * sizeof(double)));
+  if (!c->d_chain_rg)
+    HIPCHK(c, hipMalloc(&c->d_chain_rg, 3 * (size_t)(p.n_chain + p.n_chain_d + 1) * sizeof(double)));
+  if (c->d_hist && hist_dim != c->hist_dim) {
+    hipFree(c->d_hist);
+    c->d_hist = nullptr;
+  }
+  if (hist_dim > 0 && !c->d_hist) HIPCHK(c, hipMalloc(&c->d_hist, 3 * (size_t)hist_dim * sizeof(unsigned)));
+  c->hist_dim = hist_dim;
+  HIPCHK(c, hipMemsetAsync(c->head_f, 0, 3 * (size_t)(p.n_chain > 0 ? p.n_chain : 1) * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_obs, 0, 16 * sizeof(double), c->stream));
+  if (hist_dim > 0) HIPCHK(c, hipMemsetAsync(c->d_hist, 0, 3 * (size_t)hist_dim * sizeof(unsigned), c->stream));
+  memset(&c->obs_host, 0, sizeof c->obs_host);
+  c->obs_host.hist_dim = hist_dim;
+  c->obs_on = true;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_observe(mfa_ctx* c, const mfa_obs* e, int32_t coul_in_total) {
+  if (!c || !e) return MFA_ERR_ARG;
+  if (!c->obs_on) return set_err(c, MFA_ERR_ARG, "observation: call mfa_obs_init first");
+  const mfa_params& p = c->p;
+  mfa_obs_sums* o = &c->obs_host;
+  // the energy totals of verlet_velocities.f90:183-196 and the sums of
+  // observation.f90:129-159 (no wall atoms: their terms are zero)
+  const double v_ff = e->v_fluid_fluid, v_fw = e->v_fluid_wall, v_in = e->v_intra_molec;
+  const double t_fluid = e->t_kin;
+  o->count_obs += 1;
+  double v_total = ((0. + v_fw) + v_ff) + v_in;
+  const double t_total = 0. + t_fluid;
+  const double e_total = v_total + t_total;
+  if (coul_in_total) v_total = v_total + e->v_coul;
+  const double vf = (v_ff + v_in) + v_fw;
+  o->v_fluid_fluid_1 += vf;
+  o->t_fluid_1 += t_fluid;
+  o->e_fluid_fluid_1 += v_ff + t_fluid;
+  o->v_fluid_fluid_2 += vf * vf;
+  o->t_fluid_2 += t_fluid * t_fluid;
+  o->e_fluid_fluid_2 += (v_ff + t_fluid) * (v_ff + t_fluid);
+  o->v_total_1 += v_total;
+  o->t_total_1 += t_total;
+  o->e_total_1 += e_total;
+  o->v_total_2 += v_total * v_total;
+  o->t_total_2 += t_total * t_total;
+  o->e_total_2 += e_total * e_total;
+  o->v_tot_intra_1 += v_in;
+  o->v_tot_intra_2 += v_in * v_in;
+  const int g = grid_of(c->n);
+  const double inv_z_step = 1. / (p.boundary[2] / (double)(c->hist_dim > 0 ? c->hist_dim : 1));
+  hipLaunchKernelGGL(k_obs_particles, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
+                     c->v, c->type, c->d_hist, c->hist_dim, inv_z_step, c->part);
+  const int with_rg = p.system == 3 ? 1 : 0;  // observation.f90:201 (SYSTEM 0 or 3)
+  const int nch = p.n_chain + p.n_chain_d;
+  if (with_rg && nch > 0)
+    hipLaunchKernelGGL(k_obs_chains, dim3((nch + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
+                       c->stream, c->d_kp, c->r, c->d_chain_rg);
+  hipLaunchKernelGGL(k_obs_finish, dim3(1), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->part, g,
+                     c->d_chain_rg, with_rg && nch > 0 ? 1 : 0, c->d_obs);
+  HIPCHK(c, hipGetLastError());
+  if (c->hist_dim > 0) o->count_hist += 1;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_obs_read(mfa_ctx* c, mfa_obs_sums* out, double* dens_z, double* f_on_heads) {
+  if (!c) return MFA_ERR_ARG;
+  if (!c->obs_on) return set_err(c, MFA_ERR_ARG, "observation: call mfa_obs_init first");
+  const mfa_params& p = c->p;
+  double dv[16];
+  HIPCHK(c, hipMemcpyAsync(dv, c->d_obs, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned> h;
+  if (dens_z && c->hist_dim > 0) {
+    h.resize(3 * (size_t)c->hist_dim);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_hist, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost,
+                             c->stream));
+  }
+  if (f_on_heads && p.n_chain > 0)
+    HIPCHK(c, hipMemcpyAsync(f_on_heads, c->head_f, 3 * (size_t)p.n_chain * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    *out = c->obs_host;
+    for (int d = 0; d < 3; ++d) out->tot_p[d] = dv[d];
+    for (int q = 0; q < 3; ++q)
+      for (int d = 0; d < 3; ++d) out->r_g2_mean[q][d] = dv[3 + 3 * q + d];
+    out->r_par_per2[0] = dv[12];
+    out->r_par_per2[1] = dv[13];
+  }
+  for (size_t q = 0; q < h.size(); ++q) dens_z[q] = (double)h[q];
   return MFA_OK;
 }
 
@@ -3789,7 +4027,7 @@ static int adj_forward_step(mfa_ctx* c, int64_t step, double* obs_row, double* s
     HIPCHK(c, hipGetLastError());
   }
   hipLaunchKernelGGL(k_verlet_velocities, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap,
-                     c->f, c->v, c->a, c->type, c->part, 0, c->sidx, c->vel4);
+                     c->f, c->v, c->a, c->type, c->part, 0, c->sidx, c->vel4, nullptr);
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }

@@ -5,7 +5,10 @@
 ! Run in a directory holding system_input, mfa_input, hip_options and
 ! (optionally) conf_old.  Writes fort.60 every 10 steps
 ! (verlet_velocities.f90:206-227), conf_new (+ conf_new.bin) every n_safe
-! steps and at the end, and the wall time.
+! steps and at the end, and the wall time.  After n_relax steps the
+! observation stage runs on the device every step (md_main.f90:159-161):
+! fort.40-43 and fort.80 every 5 observed steps (observation.f90:295-330)
+! and dens_prof.mide at the end (functions.f90:131-160, histo mode 1).
 module md_hip_state
   use iso_c_binding
   use mfa_hip_bind
@@ -80,13 +83,15 @@ program md_pb_hip
   use mfa_inputs_mod
   use md_hip_state
   implicit none
-  integer :: i_time, tot_time, i, u60, nchunk
+  integer :: i_time, tot_time, i, u60, nchunk, u40, u41, u42, u43, u80
   integer(c_int64_t) :: step
   integer(c_int32_t) :: rc
   logical :: have_conf, exact_restart
   real(8) :: inv_N, v_total, e_total, vcm(3), mtot
   integer(8) :: c0, c1, crate
   type(mfa_obs_t) :: obs
+  type(mfa_obs_sums_t) :: osum
+  real(c_double), allocatable :: dens_z(:,:), f_on_heads(:,:)
 
   call system_clock(c0, crate)
   print '(/a/)', " *** MFA_PROG on MI355X (libmfa_hip) ***"
@@ -142,18 +147,32 @@ program md_pb_hip
     inv_N = 1d0 / dble(par%n_mon_tot)
   end if
   open(newunit=u60, file="fort.60", status="replace", action="write")
+  if (observe /= 0) then   ! init_obser.f90:15-91
+    call mfa_check(mfa_obs_init(ctx, int(dim_prof_dens, c_int32_t)), "init_obser", ctx)
+    allocate(dens_z(dim_prof_dens, 3), f_on_heads(3, max(par%n_chain, 1)))
+    open(newunit=u40, file="fort.40", status="replace", action="write")
+    open(newunit=u41, file="fort.41", status="replace", action="write")
+    open(newunit=u42, file="fort.42", status="replace", action="write")
+    open(newunit=u43, file="fort.43", status="replace", action="write")
+    open(newunit=u80, file="fort.80", status="replace", action="write")
+  end if
 
   ! ---- main time loop (md_main.f90:68-184)
   tot_time = inp%n_relax + inp%n_obser
   if (trim(run_mode) == "fused") then
     i_time = 0
     do while (i_time < tot_time)
-      nchunk = min(10, tot_time - i_time)
+      ! blocks of 10 steps; one step at a time while observing (every step
+      ! past n_relax), so that each observed step is a block's last
+      nchunk = min(10 - mod(i_time, 10), tot_time - i_time)
+      if (observe /= 0 .and. i_time >= inp%n_relax) nchunk = 1
+      if (observe /= 0 .and. i_time < inp%n_relax) nchunk = min(nchunk, inp%n_relax - i_time)
       step = int(inp%s_time + i_time, c_int64_t)
       call mfa_check(mfa_run_steps(ctx, step, int(nchunk, c_int32_t), obs), "run_steps", ctx)
       i_time = i_time + nchunk
-      if (nchunk == 10) call write_fort60(i_time, obs%v_fluid_wall, obs%v_fluid_fluid, &
-                                          obs%v_intra_molec, obs%t_kin, obs%v_coul)
+      if (mod(i_time, 10) == 0) call write_fort60(i_time, obs%v_fluid_wall, obs%v_fluid_fluid, &
+                                                  obs%v_intra_molec, obs%t_kin, obs%v_coul)
+      if (observe /= 0 .and. i_time > inp%n_relax) call observation(i_time, obs)
       if (mod(i_time, inp%n_safe) == 0) call store(.true.)
     end do
   else
@@ -175,16 +194,78 @@ program md_pb_hip
       if (inp%dpd_vv /= 0) call new_dpd_fd()           ! md_main.f90:149-155
       if (mod(i_time, 10) == 0) call write_fort60(i_time, v_fluid_wall, v_fluid_fluid, &
                                                    v_intra_molec, t_fluid, v_coul)
+      if (observe /= 0 .and. i_time > inp%n_relax) then
+        obs%v_fluid_fluid = v_fluid_fluid
+        obs%v_fluid_wall = v_fluid_wall
+        obs%v_intra_molec = v_intra_molec
+        obs%t_kin = t_fluid
+        obs%v_coul = v_coul
+        call observation(i_time, obs)
+      end if
       if (mod(i_time, inp%n_safe) == 0) call store(.true.)
     end do
   end if
   call store(.false.)
   close(u60)
+  if (observe /= 0) call obser_out()
   rc = mfa_destroy(ctx)
   call system_clock(c1)
   print *, ' WALL TIME (s)= ', dble(c1 - c0) / dble(crate)
 
 contains
+
+  ! observation.f90:17-330 for the step just made (energies in e): the sums
+  ! and profiles on the device; every 5 observed steps the lines of fort.40,
+  ! 41, 42, 43 and 80.  v_total carries v_coul on the steps where the
+  ! reference's verlet_velocities has just added it (every tenth call,
+  ! verlet_velocities.f90:199-222).
+  subroutine observation(it, e)
+    integer, intent(in) :: it
+    type(mfa_obs_t), intent(inout) :: e
+    real(8) :: inv_count_obs, r_time
+    integer :: k
+    call mfa_check(mfa_observe(ctx, e, merge(1_c_int32_t, 0_c_int32_t, &
+                   (inp%system == 2 .or. inp%system == 3) .and. mod(it, 10) == 0)), "observation", ctx)
+    if (mod(it - inp%n_relax, 5) /= 0) return
+    call mfa_check(mfa_obs_read(ctx, osum, dens_z, f_on_heads), "observation", ctx)
+    inv_count_obs = 1d0 / dble(osum%count_obs)
+    write(u40, '(i7,6f15.5)') it, osum%v_total_1 * inv_count_obs, osum%t_total_1 * inv_count_obs, &
+        osum%e_total_1 * inv_count_obs, osum%tot_p(:) * inv_count_obs
+    write(u41, '(i7,3f15.5)') it, e%v_fluid_fluid, e%v_intra_molec, e%v_fluid_wall
+    write(u42, '(i7,8f15.7)') it, osum%r_g2_mean(:, 1) * inv_count_obs, &
+        sum(osum%r_g2_mean(:, 1)) * inv_count_obs, osum%r_g2_mean(:, 2) * inv_count_obs, &
+        sum(osum%r_g2_mean(:, 2)) * inv_count_obs
+    write(u43, '(i7,2f15.7)') it, osum%r_par_per2(1:2) * inv_count_obs
+    ! -sum(f_on_heads(:, heads), dim=1): per head, the sum of its force
+    ! components, upper-wall chains first (observation.f90:326-328; WALL 2)
+    r_time = dble(inp%s_time + it - 1) * inp%dt
+    write(u80, '(7e18.7)') r_time, (-sum(f_on_heads(:, k)), k = 1, par%n_chain / 2), &
+        (-sum(f_on_heads(:, k)), k = par%n_chain / 2 + 1, par%n_chain)
+  end subroutine
+
+  ! obser_out's density profile (functions.f90:131-160, histo mode 1):
+  ! dens_prof.mide, slab centres and the counts per unit volume and observation.
+  subroutine obser_out()
+    real(8) :: surf, r_dummy, r_bin
+    integer :: u73, k
+    call mfa_check(mfa_obs_read(ctx, osum, dens_z, f_on_heads), "obser_out", ctx)
+    close(u40)
+    close(u41)
+    close(u42)
+    close(u43)
+    close(u80)
+    if (osum%count_hist == 0) return
+    surf = par%boundary(1) * par%boundary(2)
+    dens_z(:, :) = dens_z(:, :) * dble(dim_prof_dens) * par%inv_boundary(3) / (dble(osum%count_hist) * surf)
+    open(newunit=u73, file="dens_prof.mide", status="replace", action="write")
+    r_dummy = par%boundary(3) / dble(dim_prof_dens)
+    r_bin = r_dummy / 2.
+    do k = 1, dim_prof_dens
+      write(u73, '(4f16.7)') r_bin, dens_z(k, 1), dens_z(k, 2), dens_z(k, 3)
+      r_bin = r_bin + r_dummy
+    end do
+    close(u73)
+  end subroutine
 
   ! fort.60 line for SYSTEM 2/3 (verlet_velocities.f90:185-227).
   subroutine write_fort60(it, vfw, vff, vin, tkin, vc)

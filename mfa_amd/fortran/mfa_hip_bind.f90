@@ -71,6 +71,19 @@ module mfa_hip_bind
     integer(c_int64_t) :: n_rebuilds = 0
   end type
 
+  ! mfa_obs_sums: running sums of the observation stage; C r_g2_mean[3][3]
+  ! ([group][axis]) appears as r_g2_mean(3,3) with r_g2_mean(axis, group)
+  type, bind(C) :: mfa_obs_sums_t
+    integer(c_int64_t) :: count_obs = 0
+    real(c_double) :: v_total_1 = 0, v_total_2 = 0, t_total_1 = 0, t_total_2 = 0
+    real(c_double) :: e_total_1 = 0, e_total_2 = 0
+    real(c_double) :: v_fluid_fluid_1 = 0, v_fluid_fluid_2 = 0, t_fluid_1 = 0, t_fluid_2 = 0
+    real(c_double) :: e_fluid_fluid_1 = 0, e_fluid_fluid_2 = 0, v_tot_intra_1 = 0, v_tot_intra_2 = 0
+    real(c_double) :: tot_p(3) = 0, r_g2_mean(3, 3) = 0, r_par_per2(2) = 0
+    integer(c_int64_t) :: count_hist = 0
+    integer(c_int32_t) :: hist_dim = 0, pad = 0
+  end type
+
   interface
     integer(c_int32_t) function mfa_abi_sizes(si, sp, so) bind(C, name="mfa_abi_sizes")
       import :: c_int32_t
@@ -167,6 +180,23 @@ module mfa_hip_bind
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double) :: t_kin
+    end function
+    integer(c_int32_t) function mfa_obs_init(ctx, hist_dim) bind(C, name="mfa_obs_init")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: hist_dim
+    end function
+    integer(c_int32_t) function mfa_observe(ctx, e, coul_in_total) bind(C, name="mfa_observe")
+      import :: c_int32_t, c_ptr, mfa_obs_t
+      type(c_ptr), value :: ctx
+      type(mfa_obs_t) :: e
+      integer(c_int32_t), value :: coul_in_total
+    end function
+    integer(c_int32_t) function mfa_obs_read(ctx, sums, dens_z, f_on_heads) bind(C, name="mfa_obs_read")
+      import :: c_int32_t, c_ptr, c_double, mfa_obs_sums_t
+      type(c_ptr), value :: ctx
+      type(mfa_obs_sums_t) :: sums
+      real(c_double) :: dens_z(*), f_on_heads(*)
     end function
     integer(c_int32_t) function mfa_run_steps(ctx, first_step, n, obs) bind(C, name="mfa_run_steps")
       import :: c_int32_t, c_int64_t, c_ptr, mfa_obs_t

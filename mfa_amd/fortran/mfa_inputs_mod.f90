@@ -17,6 +17,8 @@ module mfa_inputs_mod
   character(len=16) :: run_mode = "compat"   ! compat: one call per subroutine; fused: mfa_run_steps
   integer(c_int32_t) :: precision = MFA_FP64
   integer(c_int32_t) :: list_capacity = 0
+  integer :: observe = 1                     ! observation stage after n_relax (md_main.f90:159-161)
+  integer :: dim_prof_dens = 200             ! its density slabs (init_obser.f90:120)
 
 contains
 
@@ -53,6 +55,8 @@ contains
       case ("force_switch_on"); read(val, *) inp%force_switch_on
       case ("dpd_vv");          read(val, *) inp%dpd_vv
       case ("list_capacity");   read(val, *) list_capacity
+      case ("observe");         read(val, *) observe
+      case ("dim_prof_dens");   read(val, *) dim_prof_dens
       case ("mode");            run_mode = trim(val)
       case ("precision")
         if (trim(val) == "mixed") then

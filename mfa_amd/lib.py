@@ -93,6 +93,30 @@ class Obs(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ObsSums(C.Structure):
+    """mfa_obs_sums: running sums of the observation stage."""
+    _fields_ = [
+        ("count_obs", C.c_int64),
+        ("v_total_1", C.c_double), ("v_total_2", C.c_double),
+        ("t_total_1", C.c_double), ("t_total_2", C.c_double),
+        ("e_total_1", C.c_double), ("e_total_2", C.c_double),
+        ("v_fluid_fluid_1", C.c_double), ("v_fluid_fluid_2", C.c_double),
+        ("t_fluid_1", C.c_double), ("t_fluid_2", C.c_double),
+        ("e_fluid_fluid_1", C.c_double), ("e_fluid_fluid_2", C.c_double),
+        ("v_tot_intra_1", C.c_double), ("v_tot_intra_2", C.c_double),
+        ("tot_p", C.c_double * 3), ("r_g2_mean", (C.c_double * 3) * 3),
+        ("r_par_per2", C.c_double * 2), ("count_hist", C.c_int64),
+        ("hist_dim", C.c_int32), ("pad", C.c_int32),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for f, _ in self._fields_:
+            x = getattr(self, f)
+            d[f] = np.ctypeslib.as_array(x).copy() if hasattr(x, "_length_") else x
+        return d
+
+
 class AdjointOut(C.Structure):
     _fields_ = [
         ("height", C.c_double), ("p_wall", C.c_double),
@@ -134,6 +158,9 @@ SIGNATURES = {
     "mfa_get_state_replica": (C.c_int32, [_P, C.c_int32, _dp, _dp, _dp, _dp, _ip]),
     "mfa_replica_obs": (C.c_int32, [_P, C.c_int32, C.POINTER(Obs)]),
     "mfa_autotune": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _dp]),
+    "mfa_obs_init": (C.c_int32, [_P, C.c_int32]),
+    "mfa_observe": (C.c_int32, [_P, C.POINTER(Obs), C.c_int32]),
+    "mfa_obs_read": (C.c_int32, [_P, C.POINTER(ObsSums), _dp, _dp]),
     "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
     "mfa_new_dpd_fd": (C.c_int32, [_P, _dp]),
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
@@ -358,6 +385,32 @@ class Context:
         t = C.c_double()
         self._chk(self.lib.mfa_verlet_velocities(self.h, C.byref(t)), "mfa_verlet_velocities")
         return t.value
+
+    def obs_init(self, hist_dim=200):
+        self._chk(self.lib.mfa_obs_init(self.h, hist_dim), "mfa_obs_init")
+        self.hist_dim = hist_dim
+
+    def observe(self, obs, coul_in_total=False):
+        """One observation of the current state; obs: the step's Obs (or its
+        dict from run_steps)."""
+        if isinstance(obs, dict):
+            o = Obs()
+            for f, _ in Obs._fields_:
+                setattr(o, f, obs[f])
+            obs = o
+        self._chk(self.lib.mfa_observe(self.h, C.byref(obs), 1 if coul_in_total else 0), "mfa_observe")
+
+    def obs_read(self):
+        s = ObsSums()
+        hd = max(getattr(self, "hist_dim", 0), 0)
+        dens = np.zeros((3, max(hd, 1)))
+        heads = np.zeros((max(self.params.n_chain, 1), 3))
+        self._chk(self.lib.mfa_obs_read(self.h, C.byref(s), _ptr(dens, C.c_double) if hd else None,
+                                        _ptr(heads, C.c_double)), "mfa_obs_read")
+        d = s.as_dict()
+        d["dens_z"] = dens[:, :hd]
+        d["f_on_heads"] = heads[:self.params.n_chain]
+        return d
 
     def new_dpd_fd(self):
         t = C.c_double()

@@ -35,6 +35,10 @@ struct orc_sys {
   int nk;
   double *k_vecs, *k_vecs_2, *inv_k_vecs_2, *re_rho_k, *im_rho_k;
   double inv_V, lz_half, fac_alpha;
+  /* observation stage (init_obser.f90, observation.f90, functions.f90 histo) */
+  mfa_obs_sums obs;
+  double* dens_z;     /* [3][hist_dim] slab counts */
+  double* f_on_heads; /* [n_chain][3] (verlet_velocities.f90:18) */
 };
 
 static const double PI_F = (double)3.1415926536f; /* mfa_common.f90:332, a default-real literal */
@@ -364,6 +368,7 @@ orc_sys* orc_create(const orc_params* p, int n_neigh) {
   s->bin_fluid = (int32_t*)calloc(nc * (size_t)(s->n_bin_fl + 1), sizeof(int32_t));
   s->cell_of = (int32_t*)calloc(3 * n, sizeof(int32_t));
   s->q = (double*)calloc(n, sizeof(double));
+  s->f_on_heads = (double*)calloc(3 * (size_t)(p->n_chain > 0 ? p->n_chain : 1), sizeof(double));
   if (p->coulomb) ewald_init(s);
   return s;
 }
@@ -374,6 +379,8 @@ void orc_destroy(orc_sys* s) {
   free(s->a_type); free(s->mic); free(s->mic_old); free(s->ff_list); free(s->bin_fluid);
   free(s->cell_of);
   free(s->q);
+  free(s->dens_z);
+  free(s->f_on_heads);
   free(s->k_vecs); free(s->k_vecs_2); free(s->inv_k_vecs_2); free(s->re_rho_k); free(s->im_rho_k);
   free(s);
 }
@@ -910,6 +917,10 @@ void orc_intra_molec(orc_sys* s, double* out) {
  * :155-177.  Pinned by closed forms (constant force, pinned heads). */
 double orc_verlet_velocities(orc_sys* s) {
   const orc_params* p = &s->p;
+  /* f_on_heads = force(:, 1:part_init_d:n_mon), verlet_velocities.f90:14-19 */
+  if (p->symmetry == 0 && (p->system == 0 || p->system == 3))
+    for (int c = 0; c < p->n_chain; ++c)
+      for (int d = 0; d < 3; ++d) s->f_on_heads[3 * (size_t)c + d] = F(c * p->n_mon, d);
   for (int i = 0; i < s->n; ++i) {
     const double inv_mass = 1. / p->mass_type[s->a_type[i] - 1];
     for (int d = 0; d < 3; ++d) A(i, d) = F(i, d) * inv_mass;
@@ -1043,4 +1054,143 @@ int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs) {
     obs[8] = v_bend;
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Observation stage: init_obser.f90:15-91 (the sums), histo mode 0
+ * (functions.f90:106-129) with dim_prof_dens = hist_dim. */
+void orc_obs_init(orc_sys* s, int hist_dim) {
+  memset(&s->obs, 0, sizeof s->obs);
+  s->obs.hist_dim = hist_dim;
+  free(s->dens_z);
+  s->dens_z = hist_dim > 0 ? (double*)calloc(3 * (size_t)hist_dim, sizeof(double)) : NULL;
+}
+
+/* observation.f90:17-307 with PROFILES for SYSTEM 2/3, WALL 2 (no wall
+ * atoms), after a step whose verlet_velocities gave e = (v_fluid_fluid,
+ * v_fluid_wall, v_intra_molec, t_fluid, v_coul):
+ *  - the energy totals of verlet_velocities.f90:183-196 (v_total gains
+ *    v_coul only on its tenth calls, :199-222: coul_in_total) and the sums
+ *    of observation.f90:129-159;
+ *  - brushes (SYSTEM 3): the chains unfolded along their bonds, one image
+ *    correction per monomer and axis (:163-199), Rg^2 per chain about its
+ *    centre with inv_n_mon = 1/n_mon (init_obser.f90:121) for the brush and
+ *    1/n_mon_d for the melt chains, the means r_g2 and their sums
+ *    (:205-276);
+ *  - the total momentum sum m v (:286-291);
+ *  - histo mode 2 (functions.f90:143-206): slab counts of z for the brush,
+ *    melt and particle-4 monomers, index int(z / z_step) clamped to the
+ *    slabs.
+ * Pinned against the reference's own observation stage (tests/test_ref_pin.py,
+ * the files fort.40-43 and dens_prof.mide of the PROFILES build). */
+void orc_observe(orc_sys* s, const double* e, int coul_in_total) {
+  const orc_params* p = &s->p;
+  mfa_obs_sums* o = &s->obs;
+  const double v_ff = e[0], v_fw = e[1], v_in = e[2], t_fluid = e[3], v_coul = e[4];
+  o->count_obs = o->count_obs + 1;
+  double v_total = ((0. + v_fw) + v_ff) + v_in;
+  const double t_total = 0. + t_fluid;
+  const double e_total = v_total + t_total;
+  if (coul_in_total) v_total = v_total + v_coul;
+  const double vf = (v_ff + v_in) + v_fw;
+  o->v_fluid_fluid_1 = o->v_fluid_fluid_1 + vf;
+  o->t_fluid_1 = o->t_fluid_1 + t_fluid;
+  o->e_fluid_fluid_1 = o->e_fluid_fluid_1 + (v_ff + t_fluid);
+  o->v_fluid_fluid_2 = o->v_fluid_fluid_2 + vf * vf;
+  o->t_fluid_2 = o->t_fluid_2 + t_fluid * t_fluid;
+  o->e_fluid_fluid_2 = o->e_fluid_fluid_2 + (v_ff + t_fluid) * (v_ff + t_fluid);
+  o->v_total_1 = o->v_total_1 + v_total;
+  o->t_total_1 = o->t_total_1 + t_total;
+  o->e_total_1 = o->e_total_1 + e_total;
+  o->v_total_2 = o->v_total_2 + v_total * v_total;
+  o->t_total_2 = o->t_total_2 + t_total * t_total;
+  o->e_total_2 = o->e_total_2 + e_total * e_total;
+  o->v_tot_intra_1 = o->v_tot_intra_1 + v_in;
+  o->v_tot_intra_2 = o->v_tot_intra_2 + v_in * v_in;
+  if (p->system == 3) {
+    const size_t n = (size_t)s->n;
+    double* sp = (double*)malloc(3 * n * sizeof(double));
+    memcpy(sp, s->r0, 3 * n * sizeof(double));
+    double half[3];
+    for (int d = 0; d < 3; ++d) half[d] = p->boundary[d] / 2.; /* init_config.f90:22 */
+#define SP(i, d) sp[3 * (size_t)(i) + (d)]
+    if (p->symmetry == 0)
+      for (int i = 0; i < p->part_init_d; ++i)
+        for (int d = 0; d < 3; ++d) {
+          if (i % p->n_mon == 0) continue; /* mod(i_part-1, n_mon) */
+          if (SP(i, d) - SP(i - 1, d) > half[d]) SP(i, d) = SP(i, d) - p->boundary[d];
+          else if (SP(i - 1, d) - SP(i, d) > half[d]) SP(i, d) = SP(i, d) + p->boundary[d];
+        }
+    for (int i = p->part_init_d; i < p->part_init_e; ++i)
+      for (int d = 0; d < 3; ++d) {
+        if (i % p->n_mon_d == 0) continue; /* mod(i_part-1, n_mon_d), the global index */
+        if (SP(i, d) - SP(i - 1, d) > half[d]) SP(i, d) = SP(i, d) - p->boundary[d];
+        else if (SP(i - 1, d) - SP(i, d) > half[d]) SP(i, d) = SP(i, d) + p->boundary[d];
+      }
+    double r_g2[2][3] = {{0., 0., 0.}, {0., 0., 0.}};
+    const double inv_n_mon = 1. / (double)p->n_mon, inv_n_chain = 1. / (double)p->n_chain;
+    for (int c = 0; c < p->n_chain; ++c) {
+      double cm[3] = {0., 0., 0.}, vd[3] = {0., 0., 0.};
+      for (int m = 0; m < p->n_mon; ++m)
+        for (int d = 0; d < 3; ++d) cm[d] = cm[d] + SP(c * p->n_mon + m, d);
+      for (int d = 0; d < 3; ++d) cm[d] = cm[d] * inv_n_mon;
+      for (int m = 0; m < p->n_mon; ++m)
+        for (int d = 0; d < 3; ++d) {
+          const double x = SP(c * p->n_mon + m, d) - cm[d];
+          vd[d] = vd[d] + x * x;
+        }
+      for (int d = 0; d < 3; ++d) r_g2[0][d] = r_g2[0][d] + vd[d] * inv_n_mon;
+    }
+    for (int d = 0; d < 3; ++d) r_g2[0][d] = r_g2[0][d] * inv_n_chain;
+    for (int c = 0; c < p->n_chain_d; ++c) {
+      const int b = p->part_init_d + c * p->n_mon_d;
+      double cm[3] = {0., 0., 0.}, vd[3] = {0., 0., 0.};
+      for (int m = 0; m < p->n_mon_d; ++m)
+        for (int d = 0; d < 3; ++d) cm[d] = cm[d] + SP(b + m, d);
+      for (int d = 0; d < 3; ++d) cm[d] = cm[d] / (double)p->n_mon_d;
+      for (int m = 0; m < p->n_mon_d; ++m)
+        for (int d = 0; d < 3; ++d) {
+          const double x = SP(b + m, d) - cm[d];
+          vd[d] = vd[d] + x * x;
+        }
+      for (int d = 0; d < 3; ++d) r_g2[1][d] = r_g2[1][d] + vd[d] / (double)p->n_mon_d;
+    }
+    for (int d = 0; d < 3; ++d) r_g2[1][d] = r_g2[1][d] / (double)p->n_chain_d; /* 0/0 without melt chains, as in the reference */
+#undef SP
+    free(sp);
+    for (int d = 0; d < 3; ++d) {
+      o->r_g2_mean[0][d] = o->r_g2_mean[0][d] + r_g2[0][d];
+      o->r_g2_mean[1][d] = o->r_g2_mean[1][d] + r_g2[1][d];
+      o->r_g2_mean[2][d] = o->r_g2_mean[2][d] + (r_g2[0][d] + r_g2[1][d]) / 2.0;
+    }
+    for (int g = 0; g < 2; ++g) {
+      const double sum = (r_g2[g][0] + r_g2[g][1]) + r_g2[g][2];
+      o->r_par_per2[g] = o->r_par_per2[g] + (r_g2[g][2] - 0.5 * (r_g2[g][0] + r_g2[g][1])) / sum;
+    }
+  }
+  double pv[3] = {0., 0., 0.};
+  const int n_mon_tot = p->part_init_e + p->n_mon_e * p->n_chain_e;
+  for (int i = 0; i < n_mon_tot; ++i) {
+    const double m = p->mass_type[s->a_type[i] - 1];
+    for (int d = 0; d < 3; ++d) pv[d] = pv[d] + m * V(i, d);
+  }
+  for (int d = 0; d < 3; ++d) o->tot_p[d] = o->tot_p[d] + pv[d];
+  if (o->hist_dim > 0) {
+    const int hd = o->hist_dim;
+    o->count_hist = o->count_hist + 1;
+    const double z_step = p->boundary[2] / (double)hd, inv_z_step = 1. / z_step;
+    for (int i = 0; i < n_mon_tot; ++i) {
+      const int g = i < p->part_init_d ? 0 : (i < p->part_init_e ? 1 : 2);
+      int zi = (int)(R(i, 2) * inv_z_step) + 1;
+      if (zi < 1) zi = 1;
+      if (zi > hd) zi = hd;
+      s->dens_z[(size_t)g * hd + (zi - 1)] = s->dens_z[(size_t)g * hd + (zi - 1)] + 1.;
+    }
+  }
+}
+
+void orc_obs_read(orc_sys* s, mfa_obs_sums* out, double* dens_z, double* f_on_heads) {
+  if (out) *out = s->obs;
+  if (dens_z && s->dens_z) memcpy(dens_z, s->dens_z, 3 * (size_t)s->obs.hist_dim * sizeof(double));
+  if (f_on_heads) memcpy(f_on_heads, s->f_on_heads, 3 * (size_t)s->p.n_chain * sizeof(double));
 }

@@ -95,6 +95,11 @@ double orc_new_dpd_fd(orc_sys* s);
  * v_intra, t_kin, virial, n_rebuilds, wall Fz bottom, v_coul, v_bend of the
  * last step. */
 int orc_run_steps(orc_sys* s, int64_t first_step, int n, double* obs);
+/* Observation stage (observation.f90:17-307 with PROFILES): e = (v_ff,
+ * v_fw, v_intra, t_fluid, v_coul) of the step just made. */
+void orc_obs_init(orc_sys* s, int hist_dim);
+void orc_observe(orc_sys* s, const double* e, int coul_in_total);
+void orc_obs_read(orc_sys* s, mfa_obs_sums* out, double* dens_z, double* f_on_heads);
 /* Brute-force O(N^2) list criterion (for pinning the cell scan). */
 int64_t orc_brute_pairs(orc_sys* s, int32_t* pairs, int64_t cap);
 

@@ -51,6 +51,23 @@ class OrcParams(C.Structure):
 _lib = None
 
 
+class ObsSums(C.Structure):
+    """The layout of mfa_obs_sums (include/mfa_hip.h)."""
+    _fields_ = [
+        ("count_obs", C.c_int64),
+        ("v_total_1", C.c_double), ("v_total_2", C.c_double),
+        ("t_total_1", C.c_double), ("t_total_2", C.c_double),
+        ("e_total_1", C.c_double), ("e_total_2", C.c_double),
+        ("v_fluid_fluid_1", C.c_double), ("v_fluid_fluid_2", C.c_double),
+        ("t_fluid_1", C.c_double), ("t_fluid_2", C.c_double),
+        ("e_fluid_fluid_1", C.c_double), ("e_fluid_fluid_2", C.c_double),
+        ("v_tot_intra_1", C.c_double), ("v_tot_intra_2", C.c_double),
+        ("tot_p", C.c_double * 3), ("r_g2_mean", (C.c_double * 3) * 3),
+        ("r_par_per2", C.c_double * 2), ("count_hist", C.c_int64),
+        ("hist_dim", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -99,6 +116,12 @@ def load():
     lib.orc_new_dpd_fd.restype = C.c_double
     lib.orc_run_steps.argtypes = [P, C.c_int64, C.c_int, dp]
     lib.orc_run_steps.restype = C.c_int
+    lib.orc_obs_init.argtypes = [P, C.c_int]
+    lib.orc_obs_init.restype = None
+    lib.orc_observe.argtypes = [P, dp, C.c_int]
+    lib.orc_observe.restype = None
+    lib.orc_obs_read.argtypes = [P, C.POINTER(ObsSums), dp, dp]
+    lib.orc_obs_read.restype = None
     lib.orc_ewald_k.argtypes = [P]
     lib.orc_ewald_k.restype = C.c_double
     lib.orc_ewald_nk.argtypes = [P, dp]
@@ -236,6 +259,30 @@ class System:
         return {"v_fluid_fluid": obs[0], "v_fluid_wall": obs[1], "v_intra_molec": obs[2],
                 "t_kin": obs[3], "virial": obs[4], "n_rebuilds": obs[5], "wall_fz_bottom": obs[6],
                 "v_coul": obs[7], "v_bend": obs[8]}
+
+    def obs_init(self, hist_dim=200):
+        self.lib.orc_obs_init(self.h, hist_dim)
+        self.hist_dim = hist_dim
+
+    def observe(self, obs, coul_in_total=False):
+        """One observation after a step; obs as returned by run_steps."""
+        e = np.array([obs["v_fluid_fluid"], obs["v_fluid_wall"], obs["v_intra_molec"], obs["t_kin"],
+                      obs["v_coul"]])
+        self.lib.orc_observe(self.h, _dp(e), 1 if coul_in_total else 0)
+
+    def obs_read(self):
+        s = ObsSums()
+        hd = max(getattr(self, "hist_dim", 0), 0)
+        dens = np.zeros((3, max(hd, 1)))
+        heads = np.zeros((max(self.p.n_chain, 1), 3))
+        self.lib.orc_obs_read(self.h, C.byref(s), _dp(dens), _dp(heads))
+        d = {}
+        for f, _ in s._fields_:
+            x = getattr(s, f)
+            d[f] = np.ctypeslib.as_array(x).copy() if hasattr(x, "_length_") else x
+        d["dens_z"] = dens[:, :hd]
+        d["f_on_heads"] = heads[:self.p.n_chain]
+        return d
 
     def bending(self):
         return self.lib.orc_bending(self.h)

@@ -27,7 +27,7 @@ SOURCES="mfa_common ziggurat R250 INR250 SUN util functions messages \
   gen_brush gen_chain gen_wall gen_droplet conf_default init_config init_obser \
   obser_out observation store_config fix_CM constant_force geom_constraint new_dpd_fd"
 
-for cfg in brush brush_q brush_bend brush_vv brush_lgv melt melt_fso; do
+for cfg in brush brush_q brush_bend brush_vv brush_lgv brush_obs melt melt_fso; do
   TMP=$(mktemp -d)
   cp "$HERE/control_$cfg.h" "$TMP/control_simulation.h"
   objs=""

@@ -24,6 +24,10 @@
 ! Test infrastructure, used by the adjoint's finite-difference check.
 ! With `snap <every>`, ref_snap.bin gets r0 and v as float64 (3, n_part)
 ! after every <every>-th step (the statistics fixtures of the DPD runs).
+! With `obs`, the reference's own observation stage runs as md_main.f90
+! does it: init_obser before the loop, observation after every step past
+! n_relax (md_main.f90:159-161), obser_out at the end; its files (fort.40-43,
+! fort.80 and, built with PROFILES, dens_prof.mide) are what the tests read.
 program ref_md
 #include "control_simulation.h"
   use commons
@@ -31,7 +35,7 @@ program ref_md
   character(len=32) :: arg
   integer :: force_step, n_steps, n_rebuilds, n_pairs, k, j, u, ut
   integer, allocatable :: cells(:,:)
-  logical :: trace, snap
+  logical :: trace, snap, obs
   integer :: snap_every, us
   real(kind=8) :: zsum, ftop, izt
 
@@ -41,10 +45,13 @@ program ref_md
   read(arg, *) n_steps
   trace = .false.
   snap = .false.
+  obs = .false.
   snap_every = 0
   if (command_argument_count() >= 3) then
     call get_command_argument(3, arg)
-    if (trim(arg) == "snap") then
+    if (trim(arg) == "obs") then
+      obs = .true.
+    else if (trim(arg) == "snap") then
       snap = .true.
       call get_command_argument(4, arg)
       read(arg, *) snap_every
@@ -92,7 +99,8 @@ program ref_md
   end do
   close(u)
 
-  ! main loop, md_main.f90:68-147
+  ! main loop, md_main.f90:68-161
+  if (obs) call init_obser()
   n_rebuilds = 0
   do i_time = 1, n_steps
     r_time = dble(s_time + i_time - 1) * dt
@@ -130,7 +138,9 @@ program ref_md
     if (snap) then
       if (mod(i_time, snap_every) == 0) write(us) r0(:, 1:n_part), v(:, 1:n_part)
     end if
+    if (obs .and. i_time > n_relax) call observation
   end do
+  if (obs) call obser_out()
   if (trace) close(ut)
   if (snap) close(us)
   open(newunit=u, file="ref_final.bin", access="stream", form="unformatted", status="replace")

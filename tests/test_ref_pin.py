@@ -156,3 +156,87 @@ def test_trajectory_matches_reference(run):
         assert rel(obs["v_coul"], ref["v_coul"]) < 1e-10
     if run["inp"].bending:
         assert rel(obs["v_bend"], ref["v_bend"]) < 1e-10
+
+
+# ------------------------------------------------------- observation stage
+
+def read_obs_files(d, n_chain):
+    """fort.40-43, fort.80 and dens_prof.mide of an observing run (the
+    reference's formats, observation.f90:295-330, functions.f90:152-160)."""
+    out = {f: np.loadtxt(os.path.join(d, f), ndmin=2) for f in ("fort.40", "fort.41", "fort.42",
+                                                                 "fort.43")}
+    vals = np.array(open(os.path.join(d, "fort.80")).read().split(), dtype=float)
+    out["fort.80"] = vals.reshape(-1, 1 + n_chain)
+    p = os.path.join(d, "dens_prof.mide")
+    out["dens_prof.mide"] = np.loadtxt(p) if os.path.exists(p) else None
+    return out
+
+
+def oracle_obs_lines(s, inp, p, n_steps, hist_dim=200):
+    """The same lines from the oracle's observation stage, one observation
+    after every step (n_relax = 0)."""
+    s.obs_init(hist_dim)
+    lines = {f: [] for f in ("fort.40", "fort.41", "fort.42", "fort.43", "fort.80")}
+    step = inp.s_time
+    for it in range(1, n_steps + 1):
+        o = s.run_steps(first_step=step, n=1)
+        step += 1
+        s.observe(o, coul_in_total=(it % 10 == 0 and inp.system in (2, 3)))
+        if it % 5:
+            continue
+        S = s.obs_read()
+        c = S["count_obs"]
+        lines["fort.40"].append([it, S["v_total_1"] / c, S["t_total_1"] / c, S["e_total_1"] / c,
+                                 *(S["tot_p"] / c)])
+        lines["fort.41"].append([it, o["v_fluid_fluid"], o["v_intra_molec"], o["v_fluid_wall"]])
+        rg = S["r_g2_mean"] / c
+        lines["fort.42"].append([it, *rg[0], rg[0].sum(), *rg[1], rg[1].sum()])
+        lines["fort.43"].append([it, *(S["r_par_per2"] / c)])
+        h = -S["f_on_heads"].sum(axis=1)
+        lines["fort.80"].append([(inp.s_time + it - 1) * inp.dt, *h])
+    S = s.obs_read()
+    dens = S["dens_z"] * hist_dim / p.boundary[2] / (S["count_hist"] * p.boundary[0] * p.boundary[1])
+    zc = (np.arange(hist_dim) + 0.5) * p.boundary[2] / hist_dim
+    out = {f: np.array(v) for f, v in lines.items()}
+    out["dens_prof.mide"] = np.column_stack([zc, dens.T])
+    return out
+
+
+def compare_obs(mine, ref):
+    """Equal to the printed digits: f15.5 / f15.7 / e18.7 / f16.7."""
+    tol = {"fort.40": 6e-5, "fort.41": 6e-5, "fort.42": 6e-7, "fort.43": 6e-7, "fort.80": 6e-7,
+           "dens_prof.mide": 6e-7}
+    for f, t in tol.items():
+        a, b = mine[f], ref[f]
+        assert a.shape == b.shape, (f, a.shape, b.shape)
+        assert np.array_equal(np.isnan(a), np.isnan(b)), f
+        ok = ~np.isnan(a)
+        scale = np.maximum(np.abs(b[ok]), 1.0) if f == "fort.80" else 1.0
+        err = np.max(np.abs(a[ok] - b[ok]) / scale) if ok.any() else 0.0
+        assert err < t, (f, err)
+
+
+def test_observation_matches_reference(tmp_path):
+    """The oracle's observation stage (observation.f90:17-330 with PROFILES,
+    histo) against the reference's own, run on the same input at T = 0."""
+    exe = os.path.join(REF, "ref_md_brush_obs")
+    if not os.path.exists(exe):
+        pytest.skip("reference build absent (oracle/ref/build_ref.sh needs the reference sources)")
+    kind, inp = configs.small_cases()["brush_small"]
+    inp.skin = 0.0
+    inp.temp = 0.0
+    inp.n_relax = 0
+    p, r, v, t = configs.build(inp, kind)
+    d = str(tmp_path)
+    io.write_run_dir(d, inp, t, r, v, np.zeros_like(r))
+    res = subprocess.run([exe, "3", "20", "obs"], cwd=d, capture_output=True, text=True, timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    ref = read_obs_files(d, p.n_chain)
+    t2, r2, v2, a2 = io.read_conf(os.path.join(d, "conf_old"), p.n_part)
+    s = orc.System(inp)
+    s.set_state(r2, v2, a2, t2)
+    s.binning()
+    mine = oracle_obs_lines(s, inp, p, 20)
+    compare_obs(mine, ref)
+    # the profiles see the whole brush and solvent, every step
+    assert ref["dens_prof.mide"][:, 1:].sum() > 0
