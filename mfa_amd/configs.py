@@ -194,11 +194,17 @@ def prepare_brush(inputs, kind, min_dist=0.7, precision=lib.MFA_MIXED):
     """Seeded brush start relaxed by the reference's start-up procedure: the
     FORCE_SWITCH_ON ramp of a fresh start (s_time = 1) at the production time
     step, over the ramp's 40 tau (init_force_switch_on.f90:15-26,
-    fluid_fluid.f90:42-49).  Returns (production params, state dict)."""
+    fluid_fluid.f90:42-49).  Charged systems run the ramp under the Langevin
+    thermostat (THERMOSTAT 1): while the clamp softens the LJ cores, opposite
+    charges fall together and the released Coulomb energy would heat the
+    dilute channel (the DPD thermostat barely acts there) until beads cross
+    the walls.  Returns (production params, state dict)."""
     p, r, v, t = build(inputs, kind, min_dist=min_dist)
     w = lib.Inputs.from_buffer_copy(inputs)
     w.s_time = 1
     w.force_switch_on = 1
+    if any(q != 0.0 for q in w.charge):
+        w.thermostat = 1
     wp = lib.derive(w)
     with lib.Context(wp, precision) as c:
         c.set_state(r, v, np.zeros_like(r), t)

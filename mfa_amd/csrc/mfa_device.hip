@@ -157,6 +157,10 @@ struct mfa_ctx {
   double4* d_kvec;
   int4* d_runs;     // segments (ny, nz, nx_lo, length), longest first
   int* d_run_k0;    // k index of each segment's first vector
+  int4* d_rows;     // rows: maximal runs (ny, nz, nx_lo, nx_hi) of consecutive kx
+  int* d_row_k0;    // k index of each row's first vector
+  int ew_nrows, ew_rows_per_group;
+  bool ew_force_rows;  // forces row by row (k_ewald_force_rows)
   double2 *d_tx, *d_ty, *d_tz;  // phase tables [entry][charged particle]
   int* d_qidx;
   double* d_qval;
@@ -1721,12 +1725,24 @@ __global__ __launch_bounds__(64 * EW_SUMW) void k_ewald_sum(EwaldK e,
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int kk = blockIdx.x * 64 + lane;
   double re = 0.0, im = 0.0;
-  if (kk < e.nk)
-    for (int cb = w; cb < e.n_chunks; cb += EW_SUMW) {
+  if (kk < e.nk) {
+    int cb = w;
+    for (; cb + 3 * EW_SUMW < e.n_chunks; cb += 4 * EW_SUMW) {  // four loads in flight
+      double2 t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = rho_part[(size_t)(cb + u * EW_SUMW) * e.nk + kk];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        re += t[u].x;
+        im += t[u].y;
+      }
+    }
+    for (; cb < e.n_chunks; cb += EW_SUMW) {
       const double2 t = rho_part[(size_t)cb * e.nk + kk];
       re += t.x;
       im += t.y;
     }
+  }
   sh[w][lane] = make_double2(re, im);
   __syncthreads();
   if (w == 0) {
@@ -1830,6 +1846,124 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const doubl
   }
 }
 
+// The same sums with the k vectors taken row by row (the maximal runs of
+// consecutive kx at fixed (ky, kz)): each lane owns EW_PPL charged particles
+// (lane + 256 u of its block's 256 EW_PPL), their X table rows (all
+// 2 NXM + 1 entries) held in registers; the block's rows of G = g_k rho(k)
+// are staged in LDS as full rows of 2 NXM + 1 entries (zero where a row has
+// no vector) and read by the whole wave at once; the unrolled sums skip, by
+// uniform branches, the entries outside the row.  Per k vector t = Im(conj(X Y Z) G) = cos(k.r) Im(g rho) -
+// sin(k.r) Re(g rho); per row the sums factorise as
+//   sum_n t_n       = Im(conj(Y Z) S),  S  = sum_n conj(X_n) G_n,
+//   sum_n kx_n t_n  = dk_x Im(conj(Y Z) S'),  S' = sum_n n conj(X_n) G_n,
+// eight FMA-class operations per k vector and particle, with no branches.
+// The launch is one-dimensional and XCD-aware: blocks b, b + 8, ... run on one
+// XCD (its own L2), so each XCD gets a contiguous range of (particle block,
+// row group) pairs, particle-block major, and its L2 holds the X, Y, Z table
+// entries of its own particles only.  Row group g covers rows
+// [g rows_per_group, (g + 1) rows_per_group).
+#define EW_PPL 1
+template <int NXM>
+__global__ __launch_bounds__(MFA_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_ewald_force_rows(
+    EwaldK e, const double2* __restrict__ tx, const double2* __restrict__ ty,
+    const double2* __restrict__ tz, const int4* __restrict__ rows, const int* __restrict__ row_k0,
+    int nrows, int rows_per_group, const double2* __restrict__ grho, double* __restrict__ fk_part) {
+  constexpr int NEX = 2 * NXM + 1;
+  constexpr int PB = MFA_BLOCK * EW_PPL;  // particles per block
+  extern __shared__ double2 ew_g[];       // [rows_per_group][NEX]
+  const int ngroups = (nrows + rows_per_group - 1) / rows_per_group;
+  const int L = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+  const int pblk = L / ngroups, grp = L - pblk * ngroups;
+  if (pblk * PB >= e.nq) return;  // padding of the grid to a multiple of 8
+  const int r0 = grp * rows_per_group, r1 = min(nrows, r0 + rows_per_group);
+  for (int t = threadIdx.x; t < (r1 - r0) * NEX; t += MFA_BLOCK) {
+    const int rl = t / NEX, nx = t % NEX - NXM;
+    const int4 rw = rows[r0 + rl];
+    ew_g[t] = (nx >= rw.z && nx <= rw.w) ? grho[row_k0[r0 + rl] + (nx - rw.z)] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  int idx[EW_PPL], jq[EW_PPL];
+  double xr[EW_PPL][NEX], xi[EW_PPL][NEX];
+  double fx[EW_PPL], fy[EW_PPL], fz[EW_PPL];
+#pragma unroll
+  for (int u = 0; u < EW_PPL; ++u) {
+    idx[u] = pblk * PB + u * MFA_BLOCK + threadIdx.x;
+    jq[u] = idx[u] < e.nq ? idx[u] : 0;
+    fx[u] = fy[u] = fz[u] = 0.0;
+#pragma unroll
+    for (int n = 0; n < NEX; ++n) {
+      const double2 v = tx[(size_t)n * e.nq + jq[u]];
+      xr[u][n] = v.x;
+      xi[u][n] = v.y;
+    }
+  }
+  // (ny, nz, nx_lo, nx_hi) of a row: the same for the whole block
+  auto row_of = [&](int r) {
+    const int4 q = rows[r < r1 ? r : r0];
+    return make_int4(__builtin_amdgcn_readfirstlane(q.x), __builtin_amdgcn_readfirstlane(q.y),
+                     __builtin_amdgcn_readfirstlane(q.z), __builtin_amdgcn_readfirstlane(q.w));
+  };
+  int4 nrw = row_of(r0);
+  double2 ny_[EW_PPL], nz_[EW_PPL];
+#pragma unroll
+  for (int u = 0; u < EW_PPL; ++u) {
+    ny_[u] = ty[(size_t)(nrw.x + e.nym) * e.nq + jq[u]];
+    nz_[u] = tz[(size_t)nrw.y * e.nq + jq[u]];
+  }
+  for (int r = r0; r < r1; ++r) {
+    const int4 rw = nrw;
+    double2 y[EW_PPL], z[EW_PPL];
+    nrw = row_of(r + 1);  // the next row's table entries, fetched during this one
+#pragma unroll
+    for (int u = 0; u < EW_PPL; ++u) {
+      y[u] = ny_[u];
+      z[u] = nz_[u];
+      ny_[u] = ty[(size_t)(nrw.x + e.nym) * e.nq + jq[u]];
+      nz_[u] = tz[(size_t)nrw.y * e.nq + jq[u]];
+    }
+    // the row's vectors only (nx_lo..nx_hi, the same for the whole wave)
+    const double2* g = ew_g + (r - r0) * NEX;
+    double sr[EW_PPL], si[EW_PPL], pr[EW_PPL], pi[EW_PPL];
+#pragma unroll
+    for (int u = 0; u < EW_PPL; ++u) sr[u] = si[u] = pr[u] = pi[u] = 0.0;
+#pragma unroll
+    for (int n = 0; n < NEX; ++n) {
+      if (n - NXM < rw.z || n - NXM > rw.w) continue;
+      const double2 gv = g[n];
+#pragma unroll
+      for (int u = 0; u < EW_PPL; ++u) {
+        const double a = xr[u][n] * gv.x + xi[u][n] * gv.y;
+        const double b = xr[u][n] * gv.y - xi[u][n] * gv.x;
+        sr[u] += a;
+        si[u] += b;
+        if (n != NXM) {
+          pr[u] += (double)(n - NXM) * a;
+          pi[u] += (double)(n - NXM) * b;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < EW_PPL; ++u) {
+      const double br = y[u].x * z[u].x - y[u].y * z[u].y, bi = y[u].x * z[u].y + y[u].y * z[u].x;
+      const double tsum = br * si[u] - bi * sr[u];
+      fx[u] += e.dk[0] * (br * pi[u] - bi * pr[u]);
+      fy[u] += (e.dk[1] * (double)rw.x) * tsum;
+      fz[u] += (e.dk[2] * (double)rw.y) * tsum;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < EW_PPL; ++u)
+    if (idx[u] < e.nq) {
+      double* o = fk_part + 3 * ((size_t)grp * e.nq + idx[u]);
+      o[0] = fx[u];
+      o[1] = fy[u];
+      o[2] = fz[u];
+    }
+}
+#define EW_ROWS_MAX_NXM 12  // register rows up to 25 X entries per particle
+// force partials (one per block row and charged particle) the buffer holds
+#define EW_FK_CAP(n) ((size_t)2048 * 64 + (size_t)64 * (size_t)(n))
+
 // Dipole moment of the charged particles, one block (the per-subroutine
 // entry point; the step loop takes it from k_ewald_rho/k_ewald_sum).
 __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_mz(EwaldK e, const double* __restrict__ r,
@@ -1851,22 +1985,36 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_mz(EwaldK e, const double* 
 
 // Adds -8 pi q f inv_V (ewald_k.f90:195-196) and the slab dipole force
 // -4 pi q M_z inv_V (dipolar_correction.f90:29) to F.
+// One thread per particle and component; the k-group partials are summed in
+// group order, eight loads in flight.
 __global__ void k_ewald_apply(EwaldK e, const int* __restrict__ qidx,
                               const double* __restrict__ qval, const double* __restrict__ fk_part,
                               int n_kgroups, const double* __restrict__ mz, double inv_V,
                               int add_k, int add_dip, double* __restrict__ f) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= e.nq) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * e.nq) return;
+  const int idx = t / 3, d = t - 3 * idx;
   const int i = qidx[idx];
   const double q = qval[idx];
   const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
+  double fd = f[3 * (size_t)i + d];
   if (add_k) {
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int kg = 0; kg < n_kgroups; ++kg)
-      for (int d = 0; d < 3; ++d) s[d] += fk_part[3 * ((size_t)kg * e.nq + idx) + d];
-    for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] += -8.0 * pi_f * q * s[d] * inv_V;
+    const size_t stride = 3 * (size_t)e.nq;
+    const double* p = fk_part + t;
+    double s = 0.0;
+    int kg = 0;
+    for (; kg + 8 <= n_kgroups; kg += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(kg + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; kg < n_kgroups; ++kg) s += p[(size_t)kg * stride];
+    fd += -8.0 * pi_f * q * s * inv_V;
   }
-  if (add_dip) f[3 * (size_t)i + 2] -= 4.0 * pi_f * q * (*mz) * inv_V;
+  if (add_dip && d == 2) fd -= 4.0 * pi_f * q * (*mz) * inv_V;
+  f[3 * (size_t)i + d] = fd;
 }
 
 // u_cou_k = 4 pi inv_V sum g_k |rho_k|^2 (ewald_k.f90:160-170) and the dipole
@@ -2480,7 +2628,7 @@ static int ewald_setup(mfa_ctx* c) {
                           n * sizeof(int), n * sizeof(double), n_chunk_max * nk * sizeof(double2),
                           nk * sizeof(double2), (size_t)(c->ew_sum_blocks + 1) * sizeof(double),
                           (n / MFA_BLOCK + 1) * sizeof(double), sizeof(double),
-                          (2048 * 64 + n) * 3 * sizeof(double), 2 * sizeof(double),
+                          EW_FK_CAP(n) * 3 * sizeof(double), 2 * sizeof(double),
                           ntab * n * sizeof(double2)};
   for (int b = 0; b < 13; ++b)
     if (hipMalloc(bufs[b], bytes[b]) != hipSuccess)
@@ -2493,6 +2641,15 @@ static int ewald_setup(mfa_ctx* c) {
     HIPCHK(nullptr, hipMemcpy(c->d_run_k0, seg_k0.data(), seg_k0.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   HIPCHK(nullptr, hipMemset(c->d_mz, 0, sizeof(double)));
+  c->ew_nrows = (int)runs.size();
+  c->ew_force_rows = nxm >= 1 && nxm <= EW_ROWS_MAX_NXM && !runs.empty();
+  if (c->ew_force_rows) {
+    if (hipMalloc(&c->d_rows, runs.size() * sizeof(int4)) != hipSuccess ||
+        hipMalloc(&c->d_row_k0, runs.size() * sizeof(int)) != hipSuccess)
+      return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc (Ewald rows) failed");
+    HIPCHK(nullptr, hipMemcpy(c->d_rows, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_row_k0, run_k0.data(), run_k0.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   c->ewald = true;
   return MFA_OK;
 }
@@ -2522,16 +2679,29 @@ static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
   e.n_mz = (e.nq + MFA_BLOCK - 1) / MFA_BLOCK;
   // forces: about 8192 waves (4 per block), block rows of segments; the
   // partials (one per block row and particle) fit the buffer ewald_setup
-  // sized for 2048 rows of 64
+  // sized for (EW_FK_CAP)
   const int qb = (e.nq + 63) / 64;
   int kg = 2048 / qb;
   kg = kg < 1 ? 1 : (kg > e.nsegs ? e.nsegs : kg);
   e.segs_per_kgroup = (e.nsegs + kg - 1) / kg;
   c->ew_n_kgroups = (e.nsegs + e.segs_per_kgroup - 1) / e.segs_per_kgroup;
-  if ((size_t)c->ew_n_kgroups * (size_t)e.nq > (size_t)2048 * 64 + (size_t)c->n)
+  if ((size_t)c->ew_n_kgroups * (size_t)e.nq > EW_FK_CAP((size_t)c->n))
     return set_err(c, MFA_ERR_ARG, "Ewald: too many force partials");
   if (((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2) > 64 * 1024)
     return set_err(c, MFA_ERR_ARG, "Ewald: k grid too large for the force tiles");
+  if (c->ew_force_rows) {
+    // row groups: about 1024 blocks of 256 EW_PPL particles (four resident
+    // waves per SIMD), within the partials buffer
+    const int pb = (e.nq + MFA_BLOCK * EW_PPL - 1) / (MFA_BLOCK * EW_PPL);
+    int ng = (1024 + pb - 1) / pb;
+    const int cap = (int)(EW_FK_CAP((size_t)c->n) / (size_t)e.nq);
+    ng = ng > cap ? cap : ng;
+    ng = ng > c->ew_nrows ? c->ew_nrows : (ng < 1 ? 1 : ng);
+    c->ew_rows_per_group = (c->ew_nrows + ng - 1) / ng;
+    c->ew_n_kgroups = (c->ew_nrows + c->ew_rows_per_group - 1) / c->ew_rows_per_group;
+    if ((size_t)c->ew_rows_per_group * (size_t)e.nex * sizeof(double2) > 64 * 1024)
+      return set_err(c, MFA_ERR_ARG, "Ewald: too many k rows per force block");
+  }
   return MFA_OK;
 }
 
@@ -2550,11 +2720,29 @@ static int launch_ewald(mfa_ctx* c, int add_dip) {
                      c->d_rho_part);
   hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
                      c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
-  hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups), dim3(MFA_BLOCK),
-                     ((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2),
-                     c->stream, e, c->d_tx, c->d_ty, c->d_tz,
-                     c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
-  hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
+  if (c->ew_force_rows) {
+    const int nb = (e.nq + MFA_BLOCK * EW_PPL - 1) / (MFA_BLOCK * EW_PPL) * c->ew_n_kgroups;
+    const dim3 grid((nb + 7) / 8 * 8);
+    const size_t lds = (size_t)c->ew_rows_per_group * e.nex * sizeof(double2);
+#define EW_ROWS_CASE(X)                                                                           \
+  case X:                                                                                         \
+    hipLaunchKernelGGL((k_ewald_force_rows<X>), grid, dim3(MFA_BLOCK), lds, c->stream, e, c->d_tx, \
+                       c->d_ty, c->d_tz, c->d_rows, c->d_row_k0, c->ew_nrows,                    \
+                       c->ew_rows_per_group, c->d_grho, c->d_fk_part);                            \
+    break;
+    switch (e.nxm) {
+      EW_ROWS_CASE(1) EW_ROWS_CASE(2) EW_ROWS_CASE(3) EW_ROWS_CASE(4) EW_ROWS_CASE(5) EW_ROWS_CASE(6)
+      EW_ROWS_CASE(7) EW_ROWS_CASE(8) EW_ROWS_CASE(9) EW_ROWS_CASE(10) EW_ROWS_CASE(11) EW_ROWS_CASE(12)
+      default: return set_err(c, MFA_ERR_ARG, "Ewald: no row kernel for nxm = %d", e.nxm);
+    }
+#undef EW_ROWS_CASE
+  } else {
+    hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups), dim3(MFA_BLOCK),
+                       ((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2),
+                       c->stream, e, c->d_tx, c->d_ty, c->d_tz,
+                       c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
+  }
+  hipLaunchKernelGGL(k_ewald_apply, dim3((3 * e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
                      c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
                      c->ew_inv_V, 1, add_dip, c->f);
   HIPCHK(c, hipGetLastError());
@@ -2816,6 +3004,8 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
                   c->d_obs};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  if (c->d_rows) hipFree(c->d_rows);
+  if (c->d_row_k0) hipFree(c->d_row_k0);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
                    c->d_grho,    c->d_uk_part, c->d_mz_part, c->d_mz,  c->d_fk_part, c->d_ek,
                    c->d_tx};
@@ -3168,7 +3358,7 @@ extern "C" int32_t mfa_dipolar_correction(mfa_ctx* c, double* v_coul) {
   const EwaldK& e = c->ek;
   hipLaunchKernelGGL(k_ewald_mz, dim3(1), dim3(MFA_BLOCK), 0, c->stream, e, c->r, c->d_qidx,
                      c->d_qval, c->d_mz);
-  hipLaunchKernelGGL(k_ewald_apply, dim3((e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
+  hipLaunchKernelGGL(k_ewald_apply, dim3((3 * e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
                      c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
                      c->ew_inv_V, 0, 1, c->f);
   HIPCHK(c, hipGetLastError());
