@@ -1659,7 +1659,10 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 // Partial structure factor of one chunk of EW_CH charged particles: thread t
 // of block (chunk, g) owns segment 256 g + t and accumulates
 // sum_j q_j X_j(nx) Y_j(ny) Z_j(nz) over the chunk for its k vectors; the
-// chunk's table rows are staged in LDS particle-major.
+// chunk's table rows are staged in LDS particle-major, each followed by
+// EW_SEG zeros so that a segment always reads EW_SEG X entries (straight-line
+// code: the ten reads of a particle are in flight together; sums past the
+// segment's length are dropped).
 __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_rho(EwaldK e, const double* __restrict__ qval,
                                                          const double2* __restrict__ tx,
                                                          const double2* __restrict__ ty,
@@ -1667,14 +1670,16 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_rho(EwaldK e, const double*
                                                          const int4* __restrict__ segs,
                                                          const int* __restrict__ seg_k0,
                                                          double2* __restrict__ rho_part) {
-  extern __shared__ double2 ew_lds[];  // [EW_CH][nex + ney + nez]
-  const int row = e.nex + e.ney + e.nez;
+  extern __shared__ double2 ew_lds[];  // [EW_CH][nex + ney + nez + EW_SEG]
+  const int row = e.nex + e.ney + e.nez + EW_SEG;
   const int j0 = blockIdx.x * EW_CH;
   const int nj = min(EW_CH, e.nq - j0);
   for (int t = threadIdx.x; t < nj * row; t += MFA_BLOCK) {
     const int j = t % nj, n = t / nj;  // coalesced over j
     double2 v;
-    if (n < e.nex) {
+    if (n >= e.nex + e.ney + e.nez) {
+      v = make_double2(0.0, 0.0);
+    } else if (n < e.nex) {
       v = tx[(size_t)n * e.nq + j0 + j];
     } else if (n < e.nex + e.ney) {
       v = ty[(size_t)(n - e.nex) * e.nq + j0 + j];
@@ -1694,13 +1699,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_rho(EwaldK e, const double*
   for (int u = 0; u < EW_SEG; ++u) acc[u] = make_double2(0.0, 0.0);
   for (int j = 0; j < nj; ++j) {
     const double2* tj = ew_lds + j * row;
-    const double2 a = cmul(tj[yo], tj[zo]);
+    const double2 yv = tj[yo], zv = tj[zo];
+    double2 xv[EW_SEG];
+#pragma unroll
+    for (int u = 0; u < EW_SEG; ++u) xv[u] = tj[xo + u];
+    const double2 a = cmul(yv, zv);
+#pragma unroll
     for (int u = 0; u < EW_SEG; ++u) {
-      if (u < s.w) {
-        const double2 xv = tj[xo + u];
-        acc[u].x += xv.x * a.x - xv.y * a.y;
-        acc[u].y += xv.x * a.y + xv.y * a.x;
-      }
+      acc[u].x += xv[u].x * a.x - xv[u].y * a.y;
+      acc[u].y += xv[u].x * a.y + xv[u].y * a.x;
     }
   }
   const int k0 = seg_k0[sg];
@@ -2612,7 +2619,7 @@ static int ewald_setup(mfa_ctx* c) {
   e.nex = 2 * nxm + 1;
   e.ney = 2 * nym + 1;
   e.nez = nzm + 1;
-  if ((size_t)EW_CH * (e.nex + e.ney + e.nez) * sizeof(double2) > 64 * 1024)
+  if ((size_t)EW_CH * (e.nex + e.ney + e.nez + EW_SEG) * sizeof(double2) > 64 * 1024)
     return set_err(nullptr, MFA_ERR_ARG, "Ewald: k grid too large for the structure-factor tiles");
   c->ew_inv_V = p.inv_boundary[0] * p.inv_boundary[1] / (p.boundary[2] * (1. + z_slab));
   c->ew_sum_blocks = (e.nk + 63) / 64;
@@ -2715,7 +2722,7 @@ static int launch_ewald(mfa_ctx* c, int add_dip) {
                      c->stream, e, c->r, c->d_qidx, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
                      c->d_mz_part);
   hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_chunks, (e.nsegs + MFA_BLOCK - 1) / MFA_BLOCK),
-                     dim3(MFA_BLOCK), (size_t)EW_CH * (e.nex + e.ney + e.nez) * sizeof(double2),
+                     dim3(MFA_BLOCK), (size_t)EW_CH * (e.nex + e.ney + e.nez + EW_SEG) * sizeof(double2),
                      c->stream, e, c->d_qval, c->d_tx, c->d_ty, c->d_tz, c->d_runs, c->d_run_k0,
                      c->d_rho_part);
   hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
