@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 from mfa_amd import configs, lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # Input preparation (untimed): the seeded melt (overlaps down to 0.8 sigma) is
 # relaxed on the GPU by the FORCE_SWITCH_ON ramp under WCA at dt = 0.002
 # (configs.equilibrate); the warm-up and timed steps then run the production
@@ -63,9 +64,44 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     obs = ctx.run_steps(first_step + warmup, steps, want_obs=True)
     dt = time.perf_counter() - t0
     force_ms = ctx.time_force(200)
+    ewald_ms = ctx.time_ewald(200) if p.coulomb else None
     st = ctx.get_state()
     ctx.close()
-    return dt, obs, {"force_ms": force_ms, "tpp": tpp}, st
+    return dt, obs, {"force_ms": force_ms, "tpp": tpp, "ewald_ms": ewald_ms}, st
+
+
+def ewald_k_count(p, inputs):
+    """Number of k vectors of the half space (ewald_k.f90:104-124): |k| <=
+    (dk_x + dk_y) kmax / 2, without k = 0, the lower triangle of the plane
+    kz = 0 and half of its diagonal; slab period 3 Lz with walls."""
+    pi_f = float(np.float32(3.1415926536))
+    lz = p.boundary[2] * (1.0 if p.symmetry == 1 else 3.0)
+    dk = (2 * pi_f / p.boundary[0], 2 * pi_f / p.boundary[1], 2 * pi_f / lz)
+    kc = (dk[0] + dk[1]) * inputs.n_kmax / 2.0
+    m = [int(kc / d) for d in dk]
+    nk = 0
+    for z in range(m[2] + 1):
+        for y in range(-m[1], m[1] + 1):
+            for x in range(-m[0], m[0] + 1):
+                if (x * dk[0]) ** 2 + (y * dk[1]) ** 2 + (z * dk[2]) ** 2 > kc * kc:
+                    continue
+                if z == 0 and (x == 0 and y == 0 or x + y < 0 or (-x == y and y > 0)):
+                    continue
+                nk += 1
+    return nk
+
+
+def ewald_roofline(p, inputs, types, ewald_ms):
+    """FP64 roofline of the k-space stage: 24 flops per (k vector, charge),
+    8 for its term of rho(k) and 16 for its two terms of the force sums."""
+    nq = int(sum(1 for a in types if inputs.charge[int(a) - 1] != 0.0))
+    nk = ewald_k_count(p, inputs)
+    flops = 24.0 * nq * nk
+    achieved = flops / (ewald_ms * 1e-3) / 1e12
+    return {"bound": "fp64_valu", "kernels": "k_ewald_tables, k_ewald_rho, k_ewald_sum, "
+            "k_ewald_force_rows, k_ewald_apply", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_step": flops,
+            "stage_ms_avg": ewald_ms, "charges": nq, "k_vectors": nk}
 
 
 def force_error_vs_fp64(p, st):
@@ -195,6 +231,10 @@ def bench_single(args, name):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if tm_m.get("ewald_ms"):
+        extra["ewald_roofline"] = ewald_roofline(p, inputs, t, tm_m["ewald_ms"])
+        log("k-space stage %.4f ms avg: %.2f TFLOP/s FP64" % (
+            tm_m["ewald_ms"], extra["ewald_roofline"]["achieved"]))
     return contract_line(args, name, value, dt_m / args.steps * 1e3,
                          "fp32 pair math, exact fixed-point sums, fp64 state", data, config, extra)
 

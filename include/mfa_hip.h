@@ -273,6 +273,12 @@ int32_t mfa_stats(mfa_ctx* ctx, int64_t* out /* [4] */);
  * the context's stream.  F is recomputed (same values). */
 int32_t mfa_time_force(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
+/* Average duration (ms) of the k-space Ewald stage of a step (ewald_k(1) and,
+ * with walls, the slab dipole correction: ewald_k.f90:150-199,
+ * dipolar_correction.f90:20-33) at the current state, timed like
+ * mfa_time_force; F is left as it was.  MFA_ERR_ARG without charges. */
+int32_t mfa_time_ewald(mfa_ctx* ctx, int32_t reps, double* ms_avg);
+
 /* Device-to-device copy of `bytes` (read `bytes`, write `bytes`) timed with
  * HIP events: the reference point for HBM counters and bandwidth. */
 int32_t mfa_copy_probe(int64_t bytes, double* ms);

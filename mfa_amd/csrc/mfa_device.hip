@@ -3752,16 +3752,17 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   return check_flags(c);
 }
 
-extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
-  if (!c || !ms_avg || reps < 1 || reps > 4096) return MFA_ERR_ARG;
-  int rc = ensure_list(c);
-  if (rc) return rc;
+// Average duration of `reps` launches of `launch` on the context's stream,
+// captured in one graph when graph steps are on, bracketed by two HIP events
+// (a first untimed round warms up).
+static int time_launches(mfa_ctx* c, int reps, int (*launch)(mfa_ctx*), double* ms_avg) {
+  int rc = MFA_OK;
   hipGraphExec_t ge = nullptr;
   if (c->graph_steps > 0) {  // back-to-back launches captured in one graph
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_ALL);
+    for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch(c);
     c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -3783,7 +3784,7 @@ extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
     if (ge) {
       HIPCHK(c, hipGraphLaunch(ge, c->stream));
     } else {
-      for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch_force(c, TM_ALL);
+      for (int q = 0; q < reps && rc == MFA_OK; ++q) rc = launch(c);
     }
   }
   c->timing = timing;
@@ -3796,7 +3797,40 @@ extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
   if (ge) hipGraphExecDestroy(ge);
   if (rc) return rc;
   *ms_avg = (double)ms / (double)reps;
-  return check_flags(c);
+  return MFA_OK;
+}
+
+static int launch_force_all(mfa_ctx* c) { return launch_force(c, TM_ALL); }
+
+extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
+  if (!c || !ms_avg || reps < 1 || reps > 4096) return MFA_ERR_ARG;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  rc = time_launches(c, reps, launch_force_all, ms_avg);
+  return rc ? rc : check_flags(c);
+}
+
+// The k-space stage as the step runs it: ewald_k(1) and, with walls
+// (SYMMETRY 0), the slab dipole correction.
+static int launch_ewald_stage(mfa_ctx* c) { return launch_ewald(c, c->p.symmetry == 0 ? 1 : 0); }
+
+extern "C" int32_t mfa_time_ewald(mfa_ctx* c, int32_t reps, double* ms_avg) {
+  if (!c || !ms_avg || reps < 1 || reps > 4096) return MFA_ERR_ARG;
+  if (!ewald_active(c)) return set_err(c, MFA_ERR_ARG, "no k-space Ewald stage (no charges)");
+  // the launches add to F: keep F and put it back afterwards
+  const size_t fb = 3 * (size_t)c->n * sizeof(double);
+  double* keep = nullptr;
+  if (hipMalloc(&keep, fb) != hipSuccess)
+    return set_err(c, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (F copy) failed", fb);
+  int rc = MFA_OK;
+  if (hipMemcpyAsync(keep, c->f, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    rc = set_err(c, MFA_ERR_HIP, "F copy failed");
+  if (!rc) rc = time_launches(c, reps, launch_ewald_stage, ms_avg);
+  if (hipMemcpyAsync(c->f, keep, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess && !rc)
+    rc = set_err(c, MFA_ERR_HIP, "F restore failed");
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, MFA_ERR_HIP, "sync failed");
+  hipFree(keep);
+  return rc ? rc : check_flags(c);
 }
 
 extern "C" int32_t mfa_copy_probe(int64_t bytes, double* ms) {

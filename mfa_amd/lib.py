@@ -169,6 +169,7 @@ SIGNATURES = {
     "mfa_stats": (C.c_int32, [_P, _lp]),
     "mfa_set_graph_steps": (C.c_int32, [_P, C.c_int32]),
     "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
     "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, _ip]),
@@ -299,6 +300,12 @@ class Context:
     def time_force(self, reps=100):
         ms = C.c_double()
         self._chk(self.lib.mfa_time_force(self.h, reps, C.byref(ms)), "mfa_time_force")
+        return ms.value
+
+    def time_ewald(self, reps=100):
+        """Average duration (ms) of the k-space Ewald stage; F is kept."""
+        ms = C.c_double()
+        self._chk(self.lib.mfa_time_ewald(self.h, reps, C.byref(ms)), "mfa_time_ewald")
         return ms.value
 
     def set_state(self, r, v, a, a_type, mic=None):

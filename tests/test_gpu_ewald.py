@@ -45,3 +45,28 @@ def test_displaced_rock_salt_matches_oracle(precision):
         assert per < 1e-12 and rel(obs["v_coul"], eref["v_coul"]) < 1e-12
     else:
         assert rms < 1e-5 and rel(obs["v_coul"], eref["v_coul"]) < 1e-6
+
+
+def test_time_ewald_keeps_forces():
+    """mfa_time_ewald times the k-space stage and leaves F as it was; a
+    system without charges is refused."""
+    x, r, t = _rock_salt(8, 1.5, 1.44, 20, 3.0)
+    r = r + np.random.default_rng(6).uniform(-0.2, 0.2, r.shape)
+    p = lib.derive(x)
+    print("time_ewald on displaced rock salt,", len(t), "ions", flush=True)
+    ctx = gpu_ctx(p, r, np.zeros_like(r), np.zeros_like(r), t)
+    ctx.energies(step=1)
+    f0 = ctx.get_state()["f"].copy()
+    for graph_steps in (0, 16):
+        ctx.set_graph_steps(graph_steps)
+        ms = ctx.time_ewald(20)
+        assert 0.0 < ms < 100.0, ms
+        assert np.array_equal(ctx.get_state()["f"], f0)
+    ctx.close()
+    y = lib.Inputs.from_buffer_copy(x)
+    for q in range(4):
+        y.charge[q] = 0.0
+    ctx = gpu_ctx(lib.derive(y), r, np.zeros_like(r), np.zeros_like(r), t)
+    with pytest.raises(lib.MfaError):
+        ctx.time_ewald(5)
+    ctx.close()
