@@ -42,6 +42,12 @@ STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
 # FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
 # each scaled by the same counter's reading for a copy of known size.
 FORCE_TRAFFIC_BYTES = 77.339e6 + 3.411e6
+# VALU wave-instructions per force pass of the mixed form on this workload
+# (SQ_INSTS_VALU, median of 8 launches, profiles/force_and_list_melt100k_sq_counters.md)
+# and the issue peak: 1024 SIMDs, one wave64 VALU instruction per 4 cycles
+# at 2.4 GHz.
+FORCE_VALU_INSTS = 25.441912e6
+VALU_PEAK_GINSTS = 1024 * 2.4 / 4
 
 
 def log(*a):
@@ -206,6 +212,7 @@ def bench_single(args, name):
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
         traffic = FORCE_TRAFFIC_BYTES
+        valu = FORCE_VALU_INSTS
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
@@ -216,10 +223,17 @@ def bench_single(args, name):
         data = ("synthetic (seeded grafted random walks and uniform solvent, Maxwell-Boltzmann "
                 "velocities, relaxed by the 40-tau switch-on ramp before timing)")
         traffic = None
+        valu = None
     xt = 1 if (p.coulomb or p.bending) else 0
     roof = roofline(n, tm_m["force_ms"], traffic,
                     kernel="k_force_mixed<%d,0,%d>" % (tm_m["tpp"], xt))
     roof["force_ms_avg_fp64"] = tm_d["force_ms"]
+    if valu:
+        # the kernel is VALU-issue bound: its instruction rate against the issue peak
+        g = valu / (tm_m["force_ms"] * 1e-3) / 1e9
+        roof["valu_issue"] = {"achieved": g, "peak": VALU_PEAK_GINSTS,
+                              "unit": "G wave-instructions/s", "frac": g / VALU_PEAK_GINSTS,
+                              "instructions_per_launch": valu}
     config["threads_per_particle"] = {"mixed": tm_m["tpp"], "fp64": tm_d["tpp"]}
     extra = {
         "fp64_value": value_fp64,

@@ -122,7 +122,8 @@ struct mfa_ctx {
   int* d_cell_rep;   // replica of each cell
   std::vector<mfa_obs> rep_obs;  // per-replica observations of the last mfa_run_steps
   int precision;
-  bool extra_terms;  // some replica has charges or bending: force kernels with XT = 1
+  bool extra_terms;  // some replica has charges or bending: force kernels with XT & 1
+  bool lgv_terms;    // some replica runs THERMOSTAT 1: force kernels with XT & 2
   int n;
   int cap;
   int tpp;  // threads per particle in the force pass
@@ -1225,7 +1226,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
         }
       }
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
-      const bool coul = XT && k.coulomb && (k.rc_coul_2 > 0.0 ? r2raw < k.rc_coul_2 : in_lj);
+      const bool coul = (XT & 1) && k.coulomb && (k.rc_coul_2 > 0.0 ? r2raw < k.rc_coul_2 : in_lj);
       if (!in_lj && !dpd && !coul) continue;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       const double inv_r_2 = 1.0 / r2;
@@ -1279,7 +1280,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
 
   // Langevin, bonds and walls: lane 0 of the group (at most two bonds per particle).
-  if (active && sub == 0 && k.lgv && (mask & TM_PAIR)) {
+  if ((XT & 2) && active && sub == 0 && k.lgv && (mask & TM_PAIR)) {
     double fl[3];
     langevin_force(k, step, il, ti, vxi, vyi, vzi, fl);
     ax += fl[0];
@@ -1314,7 +1315,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
         }
       }
     }
-    if (XT && (mask & TM_BEND) && k.bending && il < k.n_brush) {
+    if ((XT & 1) && (mask & TM_BEND) && k.bending && il < k.n_brush) {
       const int base = (il / k.n_mon) * k.n_mon, m = il - base;
       for (int cen = m - 1; cen <= m + 1; ++cen) {
         if (cen < 1 || cen > k.n_mon - 2) continue;
@@ -1442,10 +1443,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       const float4 tb = s_tab[ti * MFA_NTYPE + tj];
       bool in_lj = r2raw < tb.x;
       bool in_dpd = k.dpd_cut_off_on ? r2raw < rdpd2 : in_lj;
-      bool in_coul = XT && k.rc_coul_2 > 0.0 && r2raw < rc2f;
+      bool in_coul = (XT & 1) && k.rc_coul_2 > 0.0 && r2raw < rc2f;
       bool near = fabsf(r2raw - tb.x) < margin;
       if (k.dpd_cut_off_on) near = near || fabsf(r2raw - rdpd2) < margin;
-      if (XT && k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
+      if ((XT & 1) && k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
       if (near) {
         // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
         const int j = k.p0 + jl;
@@ -1456,12 +1457,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const double r2d = (ex * ex + ey * ey) + ez * ez;
         in_lj = r2d < k.range_2[ti][tj];
         in_dpd = k.dpd_cut_off_on ? r2d < k.r_cut_dpd_2 : in_lj;
-        in_coul = XT && k.rc_coul_2 > 0.0 && r2d < k.rc_coul_2;
+        in_coul = (XT & 1) && k.rc_coul_2 > 0.0 && r2d < k.rc_coul_2;
       }
       const bool dpd = k.dpd_on && in_dpd && !(k.dpd_head_excl && k.dpd_cut_off_on &&
                                                (ti == 0 || tj == 0));
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
-      const bool coul = XT && k.coulomb && (k.rc_coul_2 > 0.0 ? in_coul : in_lj);
+      const bool coul = (XT & 1) && k.coulomb && (k.rc_coul_2 > 0.0 ? in_coul : in_lj);
       if (!in_lj && !dpd && !coul) continue;
       const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
       const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
@@ -1499,13 +1500,13 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   }
 
   // Langevin, bonds and walls: lane 0 of the group, from the FP64 positions.
-  if (active && sub == 0 && k.lgv && (mask & TM_PAIR)) {
+  if ((XT & 2) && active && sub == 0 && k.lgv && (mask & TM_PAIR)) {
     double fl[3];
     langevin_force(k, step, il, ti, (double)vi4.x, (double)vi4.y, (double)vi4.z, fl);
     fix_add3(ax, ay, az, (float)fl[0], (float)fl[1], (float)fl[2]);
   }
   if (active && sub == 0 && (ENERGY || (mask & (TM_BOND | TM_WALL)))) zi = r[3 * (size_t)i + 2];
-  if (XT && active && sub == 0 && (mask & TM_BEND) && k.bending && il < k.n_brush) {
+  if ((XT & 1) && active && sub == 0 && (mask & TM_BEND) && k.bending && il < k.n_brush) {
     // each term through float and fixed point; the centre takes minus the
     // fixed-point sum of its neighbours' terms, so every angle sums to zero
     const int base = (il / k.n_mon) * k.n_mon, m = il - base;
@@ -2293,10 +2294,17 @@ static void launch_force_tx(mfa_ctx* c, int mask) {
   }
 }
 
+// XT bit 1: Coulomb real space and bending; bit 2: the Langevin force.  The
+// terms a system does not use are compiled out of its force kernels (they
+// would otherwise cost registers, hence occupancy, in every launch).
 template <int TPP>
 static void launch_force_t(mfa_ctx* c, int mask) {
-  if (c->extra_terms) launch_force_tx<TPP, 1>(c, mask);
-  else launch_force_tx<TPP, 0>(c, mask);
+  switch ((c->extra_terms ? 1 : 0) | (c->lgv_terms ? 2 : 0)) {
+    case 0: launch_force_tx<TPP, 0>(c, mask); break;
+    case 1: launch_force_tx<TPP, 1>(c, mask); break;
+    case 2: launch_force_tx<TPP, 2>(c, mask); break;
+    default: launch_force_tx<TPP, 3>(c, mask); break;
+  }
 }
 
 static int force_blocks(mfa_ctx* c) { return grid_of(c->n * c->tpp); }
@@ -2806,7 +2814,11 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->n_rep = n_rep;
   c->extra_terms = false;
   c->dpd_vv = ps[0].dpd_vv != 0;
-  for (int q = 0; q < n_rep; ++q) c->extra_terms |= ps[q].coulomb != 0 || ps[q].bending != 0;
+  c->lgv_terms = false;
+  for (int q = 0; q < n_rep; ++q) {
+    c->extra_terms |= ps[q].coulomb != 0 || ps[q].bending != 0;
+    c->lgv_terms |= ps[q].thermostat == 1;
+  }
   c->ps.assign(ps, ps + n_rep);
   c->precision = precision;
   c->tpp = 8;
