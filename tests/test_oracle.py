@@ -443,3 +443,21 @@ def test_ewald_k_count_matches_brute_enumeration():
                 k2 = (a * dk[0]) ** 2 + (b * dk[1]) ** 2 + (c * dk[2]) ** 2
                 full += 0 < k2 <= kc * kc
     assert full == 2 * len(kv)
+
+
+def test_bench_ewald_flop_count_uses_the_oracle_k_set():
+    """bench.py's k-vector count (the flop count of the k-space roofline) is
+    the oracle's k set, for the slab pe brushes and a periodic box."""
+    import bench
+    cases = configs.small_cases()
+    for name in ("pe_brush_compat", "pe_brush_small"):
+        kind, inp = cases[name]
+        p, r, v, t = configs.build(inp, kind)
+        s = orc.System(inp)
+        s.set_state(r, v, np.zeros_like(r), t)
+        assert bench.ewald_k_count(p, inp) == len(s.k_vectors()), name
+    x, r, t = _rock_salt(8, 1.5, 1.96, 20, 3.0)
+    p = lib.derive(x)
+    s = orc.System(x)
+    s.set_state(r, np.zeros_like(r), np.zeros_like(r), t)
+    assert p.symmetry == 1 and bench.ewald_k_count(p, x) == len(s.k_vectors())
