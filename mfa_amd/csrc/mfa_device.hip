@@ -1934,9 +1934,11 @@ __global__ __launch_bounds__(MFA_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
     double sr[EW_PPL], si[EW_PPL], pr[EW_PPL], pi[EW_PPL];
 #pragma unroll
     for (int u = 0; u < EW_PPL; ++u) sr[u] = si[u] = pr[u] = pi[u] = 0.0;
+    // the row's entries as one scalar bit mask: one bit test per entry
+    const unsigned in_row = ((2u << (rw.w - rw.z)) - 1u) << (rw.z + NXM);
 #pragma unroll
     for (int n = 0; n < NEX; ++n) {
-      if (n - NXM < rw.z || n - NXM > rw.w) continue;
+      if (!(in_row & (1u << n))) continue;
       const double2 gv = g[n];
 #pragma unroll
       for (int u = 0; u < EW_PPL; ++u) {
