@@ -1938,7 +1938,7 @@ __global__ __launch_bounds__(MFA_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
     const unsigned in_row = ((2u << (rw.w - rw.z)) - 1u) << (rw.z + NXM);
 #pragma unroll
     for (int n = 0; n < NEX; ++n) {
-      if (!(in_row & (1u << n))) continue;
+      if (__builtin_expect((in_row & (1u << n)) == 0u, 0)) continue;  // entries in the row fall through
       const double2 gv = g[n];
 #pragma unroll
       for (int u = 0; u < EW_PPL; ++u) {
