@@ -149,7 +149,8 @@ typedef struct mfa_obs {
   double v_bend;          /* bending energy (bending.f90) */
   double v_coul;          /* Ewald real + k + self + dipolar */
   double t_kin;           /* 0.5 sum m v^2 (verlet_velocities.f90:155-177) */
-  double virial;          /* sum over pairs r_ij . F_ij (conservative) */
+  double virial;          /* sum of r_ij . F_ij over LJ pairs and FENE bonds only
+                             (no DPD, Coulomb, wall or bending terms) */
   double wall_fz_bottom;  /* sum of wall z-forces from the z=0 wall */
   double wall_fz_top;     /* sum of wall z-forces from the z=Lz wall */
   double z_moment;        /* sum of z over brush monomers (type 2) */
@@ -214,6 +215,21 @@ int32_t mfa_get_state(mfa_ctx* ctx, double* r0, double* v, double* a, double* fo
  * grid); mode 1 adds the k-space forces of the charged particles to F and
  * returns u_cou_k in *v_coul.  No-op without charges. */
 int32_t mfa_ewald_k(mfa_ctx* ctx, int32_t mode, double* v_coul);
+/* The launch plan of ewald_k's k-space stage for a box and a number of
+ * charged particles, with every bound the kernels rely on checked (LDS per
+ * workgroup within 64 KB, rows and segments inside the k grid and the k index
+ * range, force partials inside their buffer, the XCD-aware row grid).  The
+ * k grid is the half-space enumeration of ewald_k.f90:104-124.  Host-only;
+ * mfa_create and mfa_set_state run the same checks (plus the device's LDS
+ * limit and a register-spill check of the row kernel) and refuse a system
+ * that fails them.  out[MFA_EWPLAN_N]: k vectors, segments, rows, nx_max,
+ * ny_max, nz_max, row kernel (0/1), rows per group, force groups, force
+ * blocks, force grid, LDS bytes of the structure factor, LDS bytes of the
+ * force kernel, force partials used, force partials available,
+ * structure-factor chunks.  Returns MFA_ERR_ARG (message in
+ * mfa_last_error(NULL)) when a bound fails; out is filled either way. */
+#define MFA_EWPLAN_N 16
+int32_t mfa_ewald_plan(const mfa_params* p, int32_t n_charged, int64_t* out, int32_t n_out);
 /* dipolar_correction() (dipolar_correction.f90:20-33, md_main.f90:131): adds
  * -4 pi q M_z / V to F_z of the charged particles (walls only) and returns
  * 2 pi M_z^2 / V in *v_coul. */

@@ -110,6 +110,23 @@ struct EwaldK {
   int segs_per_kgroup;  // forces: segments per block row
 };
 
+// Host plan of the k-space stage: the k grid of a box (ewald_grid) and the
+// launch shapes for a number of charged particles (ewald_shape), checked
+// against every bound the kernels rely on before anything is launched.
+struct EwPlan {
+  EwaldK e;
+  double inv_V;
+  int sum_blocks;                   // k_ewald_sum blocks (64 k vectors each)
+  std::vector<double4> kv;          // (kx, ky, kz, exp(-k^2/(4 alpha_c))/k^2)
+  std::vector<int4> segs, rows;     // segments (ny, nz, nx_lo, len), rows (ny, nz, nx_lo, nx_hi)
+  std::vector<int> seg_k0, row_k0;  // k index of each segment's / row's first vector
+  bool rows_kernel;                 // forces row by row (k_ewald_force_rows)
+  int rows_per_group, n_kgroups;    // force partial groups (row groups or segment groups)
+  int force_blocks, force_grid;     // row kernel: blocks, and the grid padded to a multiple of 8
+  size_t lds_rho, lds_force;        // dynamic LDS bytes of k_ewald_rho and the force kernel
+  size_t fk_cap;                    // force partials (per particle and group) the buffer holds
+};
+
 struct mfa_ctx {
   mfa_params p;  // replica 0 (all replicas share the force-field switches)
   // replicas: particles [p0, p0 + n) and cells [c0, c0 + n_cells) of the
@@ -150,18 +167,16 @@ struct mfa_ctx {
   float4 *pos4, *vel4;  // FP32 view of the mixed form
   double *fd, *fd_new;  // DPD_VV: dissipative force of the last new_dpd_fd, and the new one
   bool dpd_vv;
-  // Ewald k space (charged systems)
+  // Ewald k space (charged systems): the checked host plan and its tables
   bool ewald;
-  EwaldK ek;
-  double ew_inv_V;
-  int ew_n_rgroups, ew_n_kgroups, ew_sum_blocks;
+  EwPlan ew;
+  bool ew_rows_ok;  // the row kernel exists for the grid and does not spill
+  size_t lds_limit; // LDS bytes per workgroup the kernels may use
   double4* d_kvec;
   int4* d_runs;     // segments (ny, nz, nx_lo, length), longest first
   int* d_run_k0;    // k index of each segment's first vector
   int4* d_rows;     // rows: maximal runs (ny, nz, nx_lo, nx_hi) of consecutive kx
   int* d_row_k0;    // k index of each row's first vector
-  int ew_nrows, ew_rows_per_group;
-  bool ew_force_rows;  // forces row by row (k_ewald_force_rows)
   double2 *d_tx, *d_ty, *d_tz;  // phase tables [entry][charged particle]
   int* d_qidx;
   double* d_qval;
@@ -1859,12 +1874,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const doubl
 // (lane + 256 u of its block's 256 EW_PPL), their X table rows (all
 // 2 NXM + 1 entries) held in registers; the block's rows of G = g_k rho(k)
 // are staged in LDS as full rows of 2 NXM + 1 entries (zero where a row has
-// no vector) and read by the whole wave at once; the unrolled sums skip, by
-// uniform branches, the entries outside the row.  Per k vector t = Im(conj(X Y Z) G) = cos(k.r) Im(g rho) -
-// sin(k.r) Re(g rho); per row the sums factorise as
+// no vector) and read by the whole wave at once.  Per k vector
+// t = Im(conj(X Y Z) G) = cos(k.r) Im(g rho) - sin(k.r) Re(g rho); per row the
+// sums factorise as
 //   sum_n t_n       = Im(conj(Y Z) S),  S  = sum_n conj(X_n) G_n,
 //   sum_n kx_n t_n  = dk_x Im(conj(Y Z) S'),  S' = sum_n n conj(X_n) G_n,
-// eight FMA-class operations per k vector and particle, with no branches.
+// eight FMA-class operations per k vector and particle.  The row's entries
+// are one scalar bit mask; the unrolled sums test one bit per entry and skip
+// the entries outside the row by a wave-uniform branch, with the entries in
+// the row on the fall-through path.
 // The launch is one-dimensional and XCD-aware: blocks b, b + 8, ... run on one
 // XCD (its own L2), so each XCD gets a contiguous range of (particle block,
 // row group) pairs, particle-block major, and its L2 holds the X, Y, Z table
@@ -2547,10 +2565,10 @@ static int ensure_graph(mfa_ctx* c) {
 // ewald_k(0) (ewald_k.f90:35-146): the half-space k grid inside |k| <= k_c,
 // without k = 0, the lower triangle of the plane kz = 0 and half of its
 // diagonal, in the reference's order (kz, ky outer, kx inner), grouped into
-// runs of consecutive kx; slab period 3 Lz with walls.  Device arrays hold
-// (kx, ky, kz, exp(-k^2/(4 alpha_c))/k^2) per vector and the runs.
-static int ewald_setup(mfa_ctx* c) {
-  const mfa_params& p = c->p;
+// runs of consecutive kx (the rows of the force kernel) and segments of at
+// most EW_SEG vectors (the structure factor); slab period 3 Lz with walls.
+// Host only: the plan is built, checked and then uploaded.
+static void ewald_grid(const mfa_params& p, EwPlan* w) {
   const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
   double dk[3], z_slab;
   if (p.symmetry == 1) {
@@ -2565,12 +2583,12 @@ static int ewald_setup(mfa_ctx* c) {
   const double k_c = (dk[0] + dk[1]) * (double)p.n_kmax / 2.;
   const int nxm = (int)(k_c / dk[0]), nym = (int)(k_c / dk[1]), nzm = (int)(k_c / dk[2]);
   const double fac_alpha = 1. / (4. * p.alpha_c);
-  std::vector<double4> kv;
-  std::vector<int4> runs;   // maximal runs of consecutive kx: (ny, nz, nx_lo, nx_hi)
-  std::vector<int> run_k0;
+  w->kv.clear();
+  w->rows.clear();
+  w->row_k0.clear();
   for (int nkz = 0; nkz <= nzm; ++nkz)
     for (int nky = -nym; nky <= nym; ++nky) {
-      int open = -1;  // index of the open run
+      int open = -1;  // index of the open row
       for (int nkx = -nxm; nkx <= nxm; ++nkx) {
         const double ax = (double)nkx * dk[0], ay = (double)nky * dk[1], az = (double)nkz * dk[2];
         const double rk_2 = ax * ax + ay * ay + az * az;
@@ -2584,13 +2602,13 @@ static int ewald_setup(mfa_ctx* c) {
         }
         const double kx = dk[0] * (double)nkx, ky = dk[1] * (double)nky, kz = dk[2] * (double)nkz;
         const double k2 = kx * kx + ky * ky + kz * kz;
-        kv.push_back(make_double4(kx, ky, kz, (1. / k2) * exp(-k2 * fac_alpha)));
+        w->kv.push_back(make_double4(kx, ky, kz, (1. / k2) * exp(-k2 * fac_alpha)));
         if (open < 0) {
-          runs.push_back(make_int4(nky, nkz, nkx, nkx));
-          run_k0.push_back((int)kv.size() - 1);
-          open = (int)runs.size() - 1;
+          w->rows.push_back(make_int4(nky, nkz, nkx, nkx));
+          w->row_k0.push_back((int)w->kv.size() - 1);
+          open = (int)w->rows.size() - 1;
         } else {
-          runs[(size_t)open].w = nkx;
+          w->rows[(size_t)open].w = nkx;
         }
       }
     }
@@ -2598,75 +2616,179 @@ static int ewald_setup(mfa_ctx* c) {
   // lanes of a wave loop over equal lengths
   std::vector<int4> segs;
   std::vector<int> seg_k0;
-  for (size_t q = 0; q < runs.size(); ++q)
-    for (int lo = runs[q].z; lo <= runs[q].w; lo += EW_SEG) {
-      const int len = runs[q].w - lo + 1 < EW_SEG ? runs[q].w - lo + 1 : EW_SEG;
-      segs.push_back(make_int4(runs[q].x, runs[q].y, lo, len));
-      seg_k0.push_back(run_k0[q] + (lo - runs[q].z));
+  for (size_t q = 0; q < w->rows.size(); ++q)
+    for (int lo = w->rows[q].z; lo <= w->rows[q].w; lo += EW_SEG) {
+      const int len = w->rows[q].w - lo + 1 < EW_SEG ? w->rows[q].w - lo + 1 : EW_SEG;
+      segs.push_back(make_int4(w->rows[q].x, w->rows[q].y, lo, len));
+      seg_k0.push_back(w->row_k0[q] + (lo - w->rows[q].z));
     }
-  {
-    std::vector<int> order(segs.size());
-    for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return segs[a].w > segs[b].w; });
-    std::vector<int4> s2(segs.size());
-    std::vector<int> k2(segs.size());
-    for (size_t q = 0; q < order.size(); ++q) {
-      s2[q] = segs[(size_t)order[q]];
-      k2[q] = seg_k0[(size_t)order[q]];
-    }
-    segs.swap(s2);
-    seg_k0.swap(k2);
+  std::vector<int> order(segs.size());
+  for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return segs[a].w > segs[b].w; });
+  w->segs.resize(segs.size());
+  w->seg_k0.resize(segs.size());
+  for (size_t q = 0; q < order.size(); ++q) {
+    w->segs[q] = segs[(size_t)order[q]];
+    w->seg_k0[q] = seg_k0[(size_t)order[q]];
   }
-  EwaldK& e = c->ek;
+  EwaldK& e = w->e;
   memset(&e, 0, sizeof e);
   for (int d = 0; d < 3; ++d) e.dk[d] = dk[d];
   e.lz_half = p.boundary[2] / 2.;
-  e.nk = (int)kv.size();
-  e.nsegs = (int)segs.size();
+  e.nk = (int)w->kv.size();
+  e.nsegs = (int)w->segs.size();
   e.nxm = nxm;
   e.nym = nym;
   e.nzm = nzm;
   e.nex = 2 * nxm + 1;
   e.ney = 2 * nym + 1;
   e.nez = nzm + 1;
-  if ((size_t)EW_CH * (e.nex + e.ney + e.nez + EW_SEG) * sizeof(double2) > 64 * 1024)
-    return set_err(nullptr, MFA_ERR_ARG, "Ewald: k grid too large for the structure-factor tiles");
-  c->ew_inv_V = p.inv_boundary[0] * p.inv_boundary[1] / (p.boundary[2] * (1. + z_slab));
-  c->ew_sum_blocks = (e.nk + 63) / 64;
+  w->inv_V = p.inv_boundary[0] * p.inv_boundary[1] / (p.boundary[2] * (1. + z_slab));
+  w->sum_blocks = (e.nk + 63) / 64;
+  w->rows_kernel = false;
+  w->rows_per_group = w->n_kgroups = w->force_blocks = w->force_grid = 0;
+}
+
+// Launch shapes for nq charged particles out of n_alloc (the padded particle
+// count the buffers are sized for), and every bound the kernels rely on:
+// LDS within lds_limit, table and row indices inside their tables, the force
+// partials inside their buffer, the XCD-aware grid a multiple of 8 whose
+// decoding covers exactly the (particle block, row group) pairs.  rows_ok:
+// the row kernel exists for this grid and runs without register spills.
+// Returns 0, or MFA_ERR_ARG with the reason in *why.
+static int ewald_shape(EwPlan* w, int nq, size_t n_alloc, size_t lds_limit, bool rows_ok,
+                       std::string* why) {
+  EwaldK& e = w->e;
+  char buf[256];
+  auto fail = [&](const char* fmt, long long a, long long b) {
+    snprintf(buf, sizeof buf, fmt, a, b);
+    *why = buf;
+    return MFA_ERR_ARG;
+  };
+  e.nq = nq;
+  w->fk_cap = EW_FK_CAP(n_alloc);
+  w->lds_rho = (size_t)EW_CH * (e.nex + e.ney + e.nez + EW_SEG) * sizeof(double2);
+  if (w->lds_rho > lds_limit)
+    return fail("Ewald: structure-factor tile of %lld bytes exceeds the LDS limit %lld",
+                (long long)w->lds_rho, (long long)lds_limit);
+  if ((size_t)nq > n_alloc) return fail("Ewald: %lld charges > %lld particles", nq, (long long)n_alloc);
+  // every row inside the grid and the k index range; rows are the maximal
+  // runs, so they cover the k vectors in order
+  long long next_k = 0;
+  for (size_t q = 0; q < w->rows.size(); ++q) {
+    const int4 r = w->rows[q];
+    if (r.z < -e.nxm || r.w > e.nxm || r.w < r.z || r.x < -e.nym || r.x > e.nym || r.y < 0 ||
+        r.y > e.nzm)
+      return fail("Ewald: row %lld outside the k grid (nx_hi %lld)", (long long)q, r.w);
+    if (w->row_k0[q] != next_k) return fail("Ewald: row %lld starts at k %lld", (long long)q, w->row_k0[q]);
+    next_k += r.w - r.z + 1;
+  }
+  if (next_k != e.nk) return fail("Ewald: rows hold %lld of %lld k vectors", next_k, e.nk);
+  for (size_t q = 0; q < w->segs.size(); ++q)
+    if (w->seg_k0[q] < 0 || w->seg_k0[q] + w->segs[q].w > e.nk || w->segs[q].w < 1 ||
+        w->segs[q].w > EW_SEG)
+      return fail("Ewald: segment %lld outside the k range (k0 %lld)", (long long)q, w->seg_k0[q]);
+  w->rows_kernel = rows_ok && !w->rows.empty() && e.nxm >= 1 && e.nxm <= EW_ROWS_MAX_NXM;
+  if (nq == 0 || e.nsegs == 0) return MFA_OK;
+  e.n_chunks = (nq + EW_CH - 1) / EW_CH;
+  e.n_mz = (nq + MFA_BLOCK - 1) / MFA_BLOCK;
+  if (w->rows_kernel) {
+    // row groups: about 1024 blocks of 256 EW_PPL particles (four resident
+    // waves per SIMD), each group's G rows within the LDS limit, the
+    // partials within their buffer
+    const int nrows = (int)w->rows.size();
+    const int pb = (nq + MFA_BLOCK * EW_PPL - 1) / (MFA_BLOCK * EW_PPL);
+    int ng = (1024 + pb - 1) / pb;
+    const int cap = (int)(w->fk_cap / (size_t)nq);
+    ng = ng > cap ? cap : ng;
+    ng = ng > nrows ? nrows : (ng < 1 ? 1 : ng);
+    int rpg = (nrows + ng - 1) / ng;
+    const int rpg_lds = (int)(lds_limit / ((size_t)e.nex * sizeof(double2)));
+    if (rpg > rpg_lds) rpg = rpg_lds;
+    if (rpg < 1) return fail("Ewald: one k row of %lld entries exceeds the LDS limit %lld", e.nex,
+                             (long long)lds_limit);
+    w->rows_per_group = rpg;
+    w->n_kgroups = (nrows + rpg - 1) / rpg;
+    w->lds_force = (size_t)rpg * e.nex * sizeof(double2);
+    w->force_blocks = pb * w->n_kgroups;
+    w->force_grid = (w->force_blocks + 7) / 8 * 8;
+    // the kernel decodes block b as L = (b % 8) (grid / 8) + b / 8, particle
+    // block L / groups, group L % groups, and skips L >= force_blocks
+    if (w->force_grid % 8 != 0 || (long long)(pb - 1) * MFA_BLOCK * EW_PPL >= nq ||
+        (w->n_kgroups - 1) * rpg >= nrows)
+      return fail("Ewald: inconsistent row launch (%lld blocks, %lld groups)", w->force_blocks,
+                  w->n_kgroups);
+  } else {
+    // segment kernel: about 8192 waves (4 per block), block rows of segments
+    const int qb = (nq + 63) / 64;
+    int kg = 2048 / qb;
+    kg = kg < 1 ? 1 : (kg > e.nsegs ? e.nsegs : kg);
+    e.segs_per_kgroup = (e.nsegs + kg - 1) / kg;
+    w->n_kgroups = (e.nsegs + e.segs_per_kgroup - 1) / e.segs_per_kgroup;
+    w->lds_force = ((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2);
+    if (w->lds_force > lds_limit)
+      return fail("Ewald: force tile of %lld bytes exceeds the LDS limit %lld", (long long)w->lds_force,
+                  (long long)lds_limit);
+  }
+  if ((size_t)w->n_kgroups * (size_t)nq > w->fk_cap)
+    return fail("Ewald: %lld force partials exceed the buffer of %lld",
+                (long long)w->n_kgroups * nq, (long long)w->fk_cap);
+  return MFA_OK;
+}
+
+// Whether k_ewald_force_rows<nxm> runs without scratch (register spills).
+static bool rows_kernel_fits(int nxm) {
+  hipFuncAttributes a;
+  hipError_t err = hipErrorInvalidValue;
+#define EW_ROWS_ATTR(X) \
+  case X: err = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_ewald_force_rows<X>)); break;
+  switch (nxm) {
+    EW_ROWS_ATTR(1) EW_ROWS_ATTR(2) EW_ROWS_ATTR(3) EW_ROWS_ATTR(4) EW_ROWS_ATTR(5) EW_ROWS_ATTR(6)
+    EW_ROWS_ATTR(7) EW_ROWS_ATTR(8) EW_ROWS_ATTR(9) EW_ROWS_ATTR(10) EW_ROWS_ATTR(11) EW_ROWS_ATTR(12)
+    default: return false;
+  }
+#undef EW_ROWS_ATTR
+  return err == hipSuccess && a.localSizeBytes == 0;
+}
+
+// Device side of ewald_k(0): the plan of the box, its buffers and tables.
+static int ewald_setup(mfa_ctx* c) {
+  EwPlan& w = c->ew;
+  ewald_grid(c->p, &w);
+  const EwaldK& e = w.e;
+  c->ew_rows_ok = e.nxm >= 1 && e.nxm <= EW_ROWS_MAX_NXM && rows_kernel_fits(e.nxm);
+  std::string why;
+  if (ewald_shape(&w, 0, (size_t)c->n, c->lds_limit, c->ew_rows_ok, &why))
+    return set_err(nullptr, MFA_ERR_ARG, "%s", why.c_str());
   const size_t n = (size_t)c->n;
   const size_t n_chunk_max = (n + EW_CH - 1) / EW_CH;
   const size_t nk = (size_t)(e.nk > 0 ? e.nk : 1), nr = (size_t)(e.nsegs > 0 ? e.nsegs : 1);
+  const size_t nrw = w.rows.empty() ? 1 : w.rows.size();
   const size_t ntab = (size_t)(e.nex + e.ney + e.nez);
   void** bufs[] = {(void**)&c->d_kvec, (void**)&c->d_runs, (void**)&c->d_run_k0,
                    (void**)&c->d_qidx, (void**)&c->d_qval, (void**)&c->d_rho_part,
                    (void**)&c->d_grho, (void**)&c->d_uk_part, (void**)&c->d_mz_part,
-                   (void**)&c->d_mz, (void**)&c->d_fk_part, (void**)&c->d_ek, (void**)&c->d_tx};
+                   (void**)&c->d_mz, (void**)&c->d_fk_part, (void**)&c->d_ek, (void**)&c->d_tx,
+                   (void**)&c->d_rows, (void**)&c->d_row_k0};
   const size_t bytes[] = {nk * sizeof(double4), nr * sizeof(int4), nr * sizeof(int),
                           n * sizeof(int), n * sizeof(double), n_chunk_max * nk * sizeof(double2),
-                          nk * sizeof(double2), (size_t)(c->ew_sum_blocks + 1) * sizeof(double),
+                          nk * sizeof(double2), (size_t)(w.sum_blocks + 1) * sizeof(double),
                           (n / MFA_BLOCK + 1) * sizeof(double), sizeof(double),
-                          EW_FK_CAP(n) * 3 * sizeof(double), 2 * sizeof(double),
-                          ntab * n * sizeof(double2)};
-  for (int b = 0; b < 13; ++b)
+                          w.fk_cap * 3 * sizeof(double), 2 * sizeof(double),
+                          ntab * n * sizeof(double2), nrw * sizeof(int4), nrw * sizeof(int)};
+  for (int b = 0; b < 15; ++b)
     if (hipMalloc(bufs[b], bytes[b]) != hipSuccess)
       return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (Ewald) failed", bytes[b]);
   c->d_ty = c->d_tx + (size_t)e.nex * n;
   c->d_tz = c->d_ty + (size_t)e.ney * n;
   if (e.nk > 0) {
-    HIPCHK(nullptr, hipMemcpy(c->d_kvec, kv.data(), kv.size() * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(nullptr, hipMemcpy(c->d_runs, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIPCHK(nullptr, hipMemcpy(c->d_run_k0, seg_k0.data(), seg_k0.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_kvec, w.kv.data(), w.kv.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_runs, w.segs.data(), w.segs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_run_k0, w.seg_k0.data(), w.seg_k0.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_rows, w.rows.data(), w.rows.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->d_row_k0, w.row_k0.data(), w.row_k0.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   HIPCHK(nullptr, hipMemset(c->d_mz, 0, sizeof(double)));
-  c->ew_nrows = (int)runs.size();
-  c->ew_force_rows = nxm >= 1 && nxm <= EW_ROWS_MAX_NXM && !runs.empty();
-  if (c->ew_force_rows) {
-    if (hipMalloc(&c->d_rows, runs.size() * sizeof(int4)) != hipSuccess ||
-        hipMalloc(&c->d_row_k0, runs.size() * sizeof(int)) != hipSuccess)
-      return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc (Ewald rows) failed");
-    HIPCHK(nullptr, hipMemcpy(c->d_rows, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIPCHK(nullptr, hipMemcpy(c->d_row_k0, run_k0.data(), run_k0.size() * sizeof(int), hipMemcpyHostToDevice));
-  }
   c->ewald = true;
   return MFA_OK;
 }
@@ -2687,65 +2809,39 @@ static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
     }
   }
   c->v_coul_self = self * c->p.sqrt_alpha / sqrt(pi_f);
-  EwaldK& e = c->ek;
-  e.nq = (int)qi.size();
-  if (e.nq == 0 || e.nsegs == 0) return MFA_OK;
+  std::string why;
+  if (ewald_shape(&c->ew, (int)qi.size(), (size_t)c->n, c->lds_limit, c->ew_rows_ok, &why)) {
+    c->ew.e.nq = 0;  // no launch of an unchecked shape
+    return set_err(c, MFA_ERR_ARG, "%s", why.c_str());
+  }
+  if (qi.empty() || c->ew.e.nsegs == 0) return MFA_OK;
   HIPCHK(c, hipMemcpy(c->d_qidx, qi.data(), qi.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_qval, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice));
-  e.n_chunks = (e.nq + EW_CH - 1) / EW_CH;
-  e.n_mz = (e.nq + MFA_BLOCK - 1) / MFA_BLOCK;
-  // forces: about 8192 waves (4 per block), block rows of segments; the
-  // partials (one per block row and particle) fit the buffer ewald_setup
-  // sized for (EW_FK_CAP)
-  const int qb = (e.nq + 63) / 64;
-  int kg = 2048 / qb;
-  kg = kg < 1 ? 1 : (kg > e.nsegs ? e.nsegs : kg);
-  e.segs_per_kgroup = (e.nsegs + kg - 1) / kg;
-  c->ew_n_kgroups = (e.nsegs + e.segs_per_kgroup - 1) / e.segs_per_kgroup;
-  if ((size_t)c->ew_n_kgroups * (size_t)e.nq > EW_FK_CAP((size_t)c->n))
-    return set_err(c, MFA_ERR_ARG, "Ewald: too many force partials");
-  if (((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2) > 64 * 1024)
-    return set_err(c, MFA_ERR_ARG, "Ewald: k grid too large for the force tiles");
-  if (c->ew_force_rows) {
-    // row groups: about 1024 blocks of 256 EW_PPL particles (four resident
-    // waves per SIMD), within the partials buffer
-    const int pb = (e.nq + MFA_BLOCK * EW_PPL - 1) / (MFA_BLOCK * EW_PPL);
-    int ng = (1024 + pb - 1) / pb;
-    const int cap = (int)(EW_FK_CAP((size_t)c->n) / (size_t)e.nq);
-    ng = ng > cap ? cap : ng;
-    ng = ng > c->ew_nrows ? c->ew_nrows : (ng < 1 ? 1 : ng);
-    c->ew_rows_per_group = (c->ew_nrows + ng - 1) / ng;
-    c->ew_n_kgroups = (c->ew_nrows + c->ew_rows_per_group - 1) / c->ew_rows_per_group;
-    if ((size_t)c->ew_rows_per_group * (size_t)e.nex * sizeof(double2) > 64 * 1024)
-      return set_err(c, MFA_ERR_ARG, "Ewald: too many k rows per force block");
-  }
   return MFA_OK;
 }
 
-static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ek.nq > 0 && c->ek.nsegs > 0; }
+static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ew.e.nq > 0 && c->ew.e.nsegs > 0; }
 
 // k-space forces (and, with add_dip, the slab dipole forces) added to F.
+// Shapes come from the checked plan (ewald_shape).
 static int launch_ewald(mfa_ctx* c, int add_dip) {
   if (!ewald_active(c)) return MFA_OK;
-  const EwaldK& e = c->ek;
+  const EwPlan& w = c->ew;
+  const EwaldK& e = w.e;
   hipLaunchKernelGGL(k_ewald_tables, dim3(e.n_mz, e.nex + e.ney + e.nez), dim3(MFA_BLOCK), 0,
                      c->stream, e, c->r, c->d_qidx, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
                      c->d_mz_part);
   hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_chunks, (e.nsegs + MFA_BLOCK - 1) / MFA_BLOCK),
-                     dim3(MFA_BLOCK), (size_t)EW_CH * (e.nex + e.ney + e.nez + EW_SEG) * sizeof(double2),
-                     c->stream, e, c->d_qval, c->d_tx, c->d_ty, c->d_tz, c->d_runs, c->d_run_k0,
-                     c->d_rho_part);
-  hipLaunchKernelGGL(k_ewald_sum, dim3(c->ew_sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
+                     dim3(MFA_BLOCK), w.lds_rho, c->stream, e, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
+                     c->d_runs, c->d_run_k0, c->d_rho_part);
+  hipLaunchKernelGGL(k_ewald_sum, dim3(w.sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
                      c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
-  if (c->ew_force_rows) {
-    const int nb = (e.nq + MFA_BLOCK * EW_PPL - 1) / (MFA_BLOCK * EW_PPL) * c->ew_n_kgroups;
-    const dim3 grid((nb + 7) / 8 * 8);
-    const size_t lds = (size_t)c->ew_rows_per_group * e.nex * sizeof(double2);
-#define EW_ROWS_CASE(X)                                                                           \
-  case X:                                                                                         \
-    hipLaunchKernelGGL((k_ewald_force_rows<X>), grid, dim3(MFA_BLOCK), lds, c->stream, e, c->d_tx, \
-                       c->d_ty, c->d_tz, c->d_rows, c->d_row_k0, c->ew_nrows,                    \
-                       c->ew_rows_per_group, c->d_grho, c->d_fk_part);                            \
+  if (w.rows_kernel) {
+#define EW_ROWS_CASE(X)                                                                              \
+  case X:                                                                                            \
+    hipLaunchKernelGGL((k_ewald_force_rows<X>), dim3(w.force_grid), dim3(MFA_BLOCK), w.lds_force,    \
+                       c->stream, e, c->d_tx, c->d_ty, c->d_tz, c->d_rows, c->d_row_k0,             \
+                       (int)w.rows.size(), w.rows_per_group, c->d_grho, c->d_fk_part);              \
     break;
     switch (e.nxm) {
       EW_ROWS_CASE(1) EW_ROWS_CASE(2) EW_ROWS_CASE(3) EW_ROWS_CASE(4) EW_ROWS_CASE(5) EW_ROWS_CASE(6)
@@ -2754,15 +2850,37 @@ static int launch_ewald(mfa_ctx* c, int add_dip) {
     }
 #undef EW_ROWS_CASE
   } else {
-    hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, c->ew_n_kgroups), dim3(MFA_BLOCK),
-                       ((size_t)e.nex * 64 + (size_t)e.segs_per_kgroup * EW_SEG) * sizeof(double2),
-                       c->stream, e, c->d_tx, c->d_ty, c->d_tz,
-                       c->d_runs, c->d_run_k0, c->d_grho, c->d_fk_part);
+    hipLaunchKernelGGL(k_ewald_force, dim3((e.nq + 63) / 64, w.n_kgroups), dim3(MFA_BLOCK),
+                       w.lds_force, c->stream, e, c->d_tx, c->d_ty, c->d_tz, c->d_runs, c->d_run_k0,
+                       c->d_grho, c->d_fk_part);
   }
   hipLaunchKernelGGL(k_ewald_apply, dim3((3 * e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
-                     c->ew_inv_V, 1, add_dip, c->f);
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, w.n_kgroups, c->d_mz,
+                     w.inv_V, 1, add_dip, c->f);
   HIPCHK(c, hipGetLastError());
+  return MFA_OK;
+}
+
+// Host-only: the k-space plan of a box for n_charged charges (see the
+// header for the fields).  The row kernel's register check needs the device
+// and runs at mfa_create; here the row kernel is assumed to fit.
+extern "C" int32_t mfa_ewald_plan(const mfa_params* p, int32_t n_charged, int64_t* out,
+                                  int32_t n_out) {
+  if (!p || !out || n_out < MFA_EWPLAN_N || n_charged < 0 || n_charged > p->n_part)
+    return set_err(nullptr, MFA_ERR_ARG, "mfa_ewald_plan: bad argument");
+  EwPlan w;
+  ewald_grid(*p, &w);
+  const size_t n_alloc = ((size_t)p->n_part + MFA_BLOCK - 1) / MFA_BLOCK * MFA_BLOCK;
+  std::string why;
+  const bool rows_ok = w.e.nxm >= 1 && w.e.nxm <= EW_ROWS_MAX_NXM;
+  const int rc = ewald_shape(&w, n_charged, n_alloc, 65536, rows_ok, &why);
+  const int64_t v[MFA_EWPLAN_N] = {w.e.nk, w.e.nsegs, (int64_t)w.rows.size(), w.e.nxm, w.e.nym,
+                                   w.e.nzm, w.rows_kernel ? 1 : 0, w.rows_per_group, w.n_kgroups,
+                                   w.force_blocks, w.force_grid, (int64_t)w.lds_rho,
+                                   (int64_t)w.lds_force, (int64_t)w.n_kgroups * n_charged,
+                                   (int64_t)w.fk_cap, w.e.n_chunks};
+  for (int q = 0; q < MFA_EWPLAN_N; ++q) out[q] = v[q];
+  if (rc) return set_err(nullptr, rc, "%s", why.c_str());
   return MFA_OK;
 }
 
@@ -2770,8 +2888,8 @@ static int launch_ewald(mfa_ctx* c, int add_dip) {
 static int ewald_energies(mfa_ctx* c, double* uk, double* udip) {
   *uk = *udip = 0.0;
   if (!ewald_active(c)) return MFA_OK;
-  hipLaunchKernelGGL(k_ewald_energy, dim3(1), dim3(64), 0, c->stream, c->d_uk_part, c->ew_sum_blocks,
-                     c->d_mz, c->ew_inv_V, c->d_ek);
+  hipLaunchKernelGGL(k_ewald_energy, dim3(1), dim3(64), 0, c->stream, c->d_uk_part, c->ew.sum_blocks,
+                     c->d_mz, c->ew.inv_V, c->d_ek);
   HIPCHK(c, hipGetLastError());
   double h[2];
   HIPCHK(c, hipMemcpyAsync(h, c->d_ek, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2812,6 +2930,18 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, MFA_ERR_HIP, "no HIP device");
   mfa_ctx* c = new mfa_ctx();
+  {
+    // LDS per workgroup: the device's limit, at most the 64 KB the kernels'
+    // tiles are sized for
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        lds <= 0) {
+      delete c;
+      return set_err(nullptr, MFA_ERR_HIP, "cannot read the device's LDS limit");
+    }
+    c->lds_limit = (size_t)(lds < 65536 ? lds : 65536);
+  }
   c->p = ps[0];
   c->n_rep = n_rep;
   c->extra_terms = false;
@@ -3376,12 +3506,12 @@ extern "C" int32_t mfa_dipolar_correction(mfa_ctx* c, double* v_coul) {
   if (!c) return MFA_ERR_ARG;
   if (v_coul) *v_coul = 0.0;
   if (!ewald_active(c) || c->p.symmetry != 0) return MFA_OK;
-  const EwaldK& e = c->ek;
+  const EwaldK& e = c->ew.e;
   hipLaunchKernelGGL(k_ewald_mz, dim3(1), dim3(MFA_BLOCK), 0, c->stream, e, c->r, c->d_qidx,
                      c->d_qval, c->d_mz);
   hipLaunchKernelGGL(k_ewald_apply, dim3((3 * e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew_n_kgroups, c->d_mz,
-                     c->ew_inv_V, 0, 1, c->f);
+                     c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, c->ew.n_kgroups, c->d_mz,
+                     c->ew.inv_V, 0, 1, c->f);
   HIPCHK(c, hipGetLastError());
   double uk, udip;
   int rc = ewald_energies(c, &uk, &udip);

@@ -150,6 +150,7 @@ SIGNATURES = {
     "mfa_fluid_wall": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_intra_molec": (C.c_int32, [_P, _dp]),
     "mfa_ewald_k": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_ewald_plan": (C.c_int32, [C.POINTER(Params), C.c_int32, _lp, C.c_int32]),
     "mfa_dipolar_correction": (C.c_int32, [_P, _dp]),
     "mfa_bending": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_create_batch": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32,
@@ -206,7 +207,11 @@ def abi_check():
 
 
 class MfaError(RuntimeError):
-    pass
+    """A nonzero status of the C ABI; `code` is the MFA_ERR_* value."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def copy_probe(nbytes):
@@ -230,6 +235,21 @@ def derive(inputs):
     if rc:
         raise MfaError("mfa_derive: %s (%s)" % (lib.mfa_last_error(None).decode(), ERRORS[rc]))
     return p
+
+
+EWPLAN_FIELDS = ("k_vectors", "segments", "rows", "nx_max", "ny_max", "nz_max", "row_kernel",
+                 "rows_per_group", "force_groups", "force_blocks", "force_grid", "lds_rho",
+                 "lds_force", "force_partials", "force_partials_cap", "chunks")
+
+
+def ewald_plan(params, n_charged):
+    """The checked launch plan of the k-space stage (host only).  Returns
+    (status, dict of EWPLAN_FIELDS, message)."""
+    lib = load()
+    out = np.zeros(len(EWPLAN_FIELDS), dtype=np.int64)
+    rc = lib.mfa_ewald_plan(C.byref(params), n_charged, out.ctypes.data_as(_lp), len(out))
+    msg = lib.mfa_last_error(None).decode() if rc else ""
+    return rc, dict(zip(EWPLAN_FIELDS, (int(x) for x in out))), msg
 
 
 def generate(params, kind, seed, head_z=1.0, bond=0.97, min_dist=0.8):
@@ -286,7 +306,7 @@ class Context:
     def _chk(self, rc, what):
         if rc:
             raise MfaError("%s: %s (%s)" % (what, self.lib.mfa_last_error(self.h).decode(),
-                                            ERRORS.get(rc, rc)))
+                                            ERRORS.get(rc, rc)), rc)
 
     def set_precision(self, precision):
         self._chk(self.lib.mfa_set_precision(self.h, precision), "mfa_set_precision")

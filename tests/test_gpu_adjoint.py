@@ -2,20 +2,19 @@
 time-averaged brush height and top-wall pressure with respect to
 (epsilon_LJ, sigma, k_FENE, R0, gamma_DPD) against central finite
 differences (relative step 1e-4) of the same forward pass, of the oracle's
-trajectories and of the reference's own Fortran build; through the
+trajectories and of the reference's own Fortran build (a committed fixture); through the
 torch.autograd wrapper; and invariance under the checkpoint interval.
 
 Bar: every gradient component within 1e-4 relative of the finite
 difference (components below 1e-3 of their row's largest count against that
 largest value)."""
 import os
-import subprocess
 
 import numpy as np
 import pytest
 import torch
 
-from mfa_amd import configs, inputs as io, lib
+from mfa_amd import configs, lib
 from oracle import oracle as orc
 from tests.adjoint_torch import WindowObservables, central_fd, make_runner
 
@@ -127,44 +126,22 @@ def test_adjoint_matches_oracle_finite_differences(relaxed):
     _grad_close(G, Gfd)
 
 
-def test_adjoint_matches_reference_finite_differences(relaxed, tmp_path):
+def test_adjoint_matches_reference_finite_differences():
     """FD of the reference's own build (ref_md_brush) at temperature 0 (no
-    random force, so its ziggurat stream drops out) and its list skin 0.4."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_md_brush")
-    if not os.path.exists(exe):
-        pytest.skip("reference build absent (oracle/ref/build_ref.sh needs the reference sources)")
-    x0, st0 = relaxed
-    x = lib.Inputs.from_buffer_copy(x0)
+    random force, so its ziggurat stream drops out) and its list skin 0.4,
+    recorded by tests/golden/make_ref_fixtures.py from the same start state
+    (the relaxed small brush through the conf_old text)."""
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "adjoint_ref_fd.npz"))
+    kind, x = configs.small_cases()["brush_small"]
     x.temp = 0.0
     x.skin = 0.0
-    n = 20
     th = configs.theta_of(x)
-    # both sides start from the conf_old text (7 decimals, init_config.f90:561-568)
-    d0 = str(tmp_path / "conf")
-    io.write_run_dir(d0, x, st0["t"], st0["r"], st0["v"], st0["a"])
-    t2, r2, v2, a2 = io.read_conf(os.path.join(d0, "conf_old"), len(st0["t"]))
-    st = {"r": r2, "v": v2, "a": a2, "t": t2, "mic": None}
-    _, G = make_runner(x, st, x.s_time, n, 5)(th)
-
-    def ref_run(theta):
-        xi = configs.with_theta(x, theta)
-        d = str(tmp_path / ("run_%d" % ref_run.calls))
-        ref_run.calls += 1
-        os.makedirs(d)
-        io.write_run_dir(d, xi, st["t"], st["r"], st["v"], st["a"])
-        res = subprocess.run([exe, "0", str(n), "trace"], cwd=d, capture_output=True, text=True,
-                             timeout=300)
-        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
-        tr = np.fromfile(os.path.join(d, "ref_trace.bin"), dtype=np.float64).reshape(n, 2)
-        p = lib.derive(xi)
-        n2 = int((st["t"] == 2).sum())
-        return np.array([tr[:, 0].mean() / n2,
-                         -tr[:, 1].mean() / (p.boundary[0] * p.boundary[1])]), None
-    ref_run.calls = 0
-    J_ref, _ = ref_run(th)
-    J_gpu, _ = make_runner(x, st, x.s_time, n, 5, want_grad=False)(th)
-    print("J ref", J_ref, "gpu", J_gpu, flush=True)
-    assert np.allclose(J_ref, J_gpu, rtol=1e-8, atol=0.0)
-    Gfd = central_fd(ref_run, th)
-    print("adjoint\n", G, "\nreference FD\n", Gfd, flush=True)
-    _grad_close(G, Gfd)
+    assert np.array_equal(th, fx["theta"])
+    n, s_time = int(fx["n_steps"]), int(fx["s_time"])
+    st = {"r": fx["r"], "v": fx["v"], "a": fx["a"], "t": fx["t"], "mic": None}
+    print("reference FD fixture: n=%d, %d steps" % (len(st["t"]), n), flush=True)
+    J_gpu, G = make_runner(x, st, s_time, n, 5)(th)
+    print("J ref", fx["J"], "gpu", J_gpu, flush=True)
+    assert np.allclose(fx["J"], J_gpu, rtol=1e-8, atol=0.0)
+    print("adjoint\n", G, "\nreference FD\n", fx["G_fd"], flush=True)
+    _grad_close(G, fx["G_fd"])

@@ -9,7 +9,7 @@ import pytest
 
 from mfa_amd import configs, inputs as io, lib
 from oracle import oracle as orc
-from tests.test_ref_pin import REF, compare_obs, oracle_obs_lines, read_obs_files
+from tests.test_ref_pin import compare_obs, read_obs_files
 from tests.util import gpu_ctx, make_case, oracle_sys, rel
 
 pytestmark = pytest.mark.gpu
@@ -75,22 +75,21 @@ def test_observe_refused_for_batches():
 @pytest.mark.parametrize("mode", ["compat", "fused"])
 def test_driver_observation_files_match_reference(tmp_path, mode):
     """The Fortran driver's fort.40-43, fort.80 and dens_prof.mide against
-    the reference's own observation stage on the same input (T = 0)."""
-    exe = os.path.join(REF, "ref_md_brush_obs")
-    if not os.path.exists(exe):
-        pytest.skip("reference build absent (oracle/ref/build_ref.sh needs the reference sources)")
+    the reference's own observation stage on the same input (T = 0), as
+    recorded by tests/golden/make_ref_fixtures.py."""
+    import hashlib
+    gold = os.path.join(ROOT, "tests", "golden", "obs_brush_small")
     assert os.path.exists(PROG), "Fortran driver not built (python -m mfa_amd.build)"
     kind, inp = configs.small_cases()["brush_small"]
     inp.skin = 0.0
     inp.temp = 0.0
     inp.n_relax, inp.n_obser, inp.n_safe = 0, 20, 1000
     p, r, v, t = configs.build(inp, kind)
-    dref, dme = str(tmp_path / "ref"), str(tmp_path / "hip")
-    for d in (dref, dme):
-        io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="fp64", mode=mode)
+    dme = str(tmp_path / "hip")
+    io.write_run_dir(dme, inp, t, r, v, np.zeros_like(r), precision="fp64", mode=mode)
+    digest = hashlib.sha256(open(os.path.join(dme, "conf_old"), "rb").read()).hexdigest()
+    assert digest == open(os.path.join(gold, "conf_old.sha256")).read().strip()
     print("driver observation", mode, flush=True)
-    res = subprocess.run([exe, "3", "20", "obs"], cwd=dref, capture_output=True, text=True, timeout=300)
-    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
     res = subprocess.run([PROG], cwd=dme, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
-    compare_obs(read_obs_files(dme, p.n_chain), read_obs_files(dref, p.n_chain))
+    compare_obs(read_obs_files(dme, p.n_chain), read_obs_files(gold, p.n_chain))

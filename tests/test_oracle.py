@@ -212,6 +212,78 @@ def test_fene_closed_form():
     assert abs(e["v_intra_molec"] + 0.5 * k * R0 ** 2 * np.log(1 - (rr / R0) ** 2)) < 1e-11
 
 
+# ---------------------------------------------------------------- virial
+# The virial W = sum over LJ pairs and FENE bonds of r_ij . F_ij (F_ij the
+# force on i from j, r_ij = r_i - r_j) = -sum r dU/dr: conservative pair and
+# bond terms only (no DPD, Coulomb, wall or bending terms).  The reference
+# computes no virial; these pins fix its definition: a two-body closed form
+# for each term (sign and factor) and W = -dU(lambda)/dlambda at lambda = 1
+# under uniform scaling of positions and box (every term present once).
+
+@pytest.mark.parametrize("rr", [0.95, 1.0, 1.12, 1.5, 2.2, 2.49])
+def test_virial_two_body_lj(rr):
+    s = _two_body(rr)
+    s.binning()
+    f, e = s.forces(step=0)
+    # r_01 = -rr x, F_0 = -F_1; W = r_01 . F_0 = rr * F_1x = -rr dU/dr
+    w = rr * 24.0 * (2.0 / rr ** 13 - 1.0 / rr ** 7)
+    assert abs(e["virial"] - w) < 1e-12 * max(1.0, abs(w)), (e["virial"], w)
+    assert (e["virial"] > 0) == (rr < C6)  # positive while repulsive
+
+
+def test_virial_two_body_fene():
+    inp = configs.melt(1, 2, 0.001, lj_rc=2.5, thermostat=-1, n_cell=16)
+    inp.x_space = inp.y_space = 12.0 / 16
+    inp.z_space_wall = 12.0
+    for t_eps in range(4):
+        for u in range(4):
+            inp.epsil[t_eps][u] = 0.0
+    k, R0 = 30.0, 1.5
+    for rr in (0.8, 0.97, 1.1, 1.3):
+        s = orc.System(inp)
+        r = np.array([[5.0, 5.0, 5.0], [5.0, 5.0 + rr, 5.0]])
+        z = np.zeros((2, 3))
+        s.set_state(r, z, z, np.array([3, 3], dtype=np.int32))
+        s.binning()
+        _, e = s.forces(step=0)
+        w = -k * rr ** 2 / (1 - (rr / R0) ** 2)  # attractive bond: W < 0
+        assert abs(e["virial"] - w) < 1e-11 * abs(w), (rr, e["virial"], w)
+
+
+def test_virial_is_minus_scaling_derivative_of_the_energy():
+    """W = -dU(lambda r, lambda L)/dlambda at lambda = 1 (central
+    difference) on the periodic melt with LJ rc = 2.5 and FENE bonds."""
+    kind, inp = configs.small_cases()["melt_small"]
+    inp.thermostat = -1
+    p, r, v, t = configs.build(inp, kind)
+    s = orc.System(inp)
+    s.set_state(r, v, np.zeros_like(r), t)
+    s.binning()
+    _, e0 = s.forces(step=0)
+
+    def energy(lam):
+        x = lib.Inputs.from_buffer_copy(inp)
+        x.x_space *= lam
+        x.y_space *= lam
+        x.z_space_wall *= lam
+        q = lib.derive(x)
+        ratio = np.array([q.boundary[d] / p.boundary[d] for d in range(3)])
+        assert np.allclose(ratio, lam, rtol=1e-12, atol=0.0)
+        s2 = orc.System(x)
+        s2.set_state(r * ratio, v, np.zeros_like(r), t)
+        s2.binning()
+        _, e = s2.forces(step=0)
+        return e["v_fluid_fluid"] + e["v_intra_molec"]
+    h = 1e-5
+    dudl = (energy(1.0 + h) - energy(1.0 - h)) / (2 * h)
+    w = e0["virial"]
+    print("virial", w, "-dU/dlambda", -dudl)
+    assert abs(w + dudl) < 1e-6 * abs(w), (w, -dudl)
+    # both terms matter here: the bonds alone carry a sizeable share
+    s.zero_force()
+    assert abs(s.intra_molec()[1]) > 0.05 * abs(w)
+
+
 def test_wall_closed_form_and_zero_force_point():
     inp = configs.brush(2, 1, 2, 6.0, 6.0, 8.0, x_space=0.5, thermostat=-1)
     s = orc.System(inp)
@@ -461,3 +533,70 @@ def test_bench_ewald_flop_count_uses_the_oracle_k_set():
     s = orc.System(x)
     s.set_state(r, np.zeros_like(r), np.zeros_like(r), t)
     assert p.symmetry == 1 and bench.ewald_k_count(p, x) == len(s.k_vectors())
+
+
+# ---------------------------------------------------------------- Ewald plan
+# The k-space stage's launch plan (mfa_ewald_plan, host only) and its
+# checks: the k set equals the oracle's enumeration, the row table covers
+# it exactly, and every launch shape stays inside its buffers.
+
+def _plan_invariants(plan, n_charged):
+    assert plan["lds_rho"] <= 65536 and plan["lds_force"] <= 65536
+    assert plan["force_partials"] <= plan["force_partials_cap"]
+    if plan["row_kernel"]:
+        assert plan["force_grid"] % 8 == 0 and plan["force_grid"] >= plan["force_blocks"]
+        pb = -(-n_charged // 256)
+        assert plan["force_blocks"] == pb * plan["force_groups"]
+        assert (plan["force_groups"] - 1) * plan["rows_per_group"] < plan["rows"]
+        assert plan["force_groups"] * plan["rows_per_group"] >= plan["rows"]
+        assert plan["lds_force"] == plan["rows_per_group"] * (2 * plan["nx_max"] + 1) * 16
+
+
+def test_ewald_plan_pe_brush27k_grid():
+    w = configs.WORKLOADS["pe_brush27k"]
+    x = w["make"]()
+    p = lib.derive(x)
+    rc, plan, msg = lib.ewald_plan(p, 10400)
+    assert rc == 0, msg
+    assert plan["k_vectors"] == len(orc.System(x).k_vectors()) == 3997
+    assert plan["row_kernel"] == 1 and plan["nx_max"] == 10
+    _plan_invariants(plan, 10400)
+
+
+@pytest.mark.parametrize("case", ["pe_brush_compat", "pe_brush_small"])
+def test_ewald_plan_test_grids(case):
+    kind, x = configs.small_cases()[case]
+    p = lib.derive(x)
+    nq = p.n_part // 3
+    rc, plan, msg = lib.ewald_plan(p, nq)
+    assert rc == 0, msg
+    assert plan["k_vectors"] == len(orc.System(x).k_vectors())
+    _plan_invariants(plan, nq)
+
+
+def test_ewald_plan_wide_grid_takes_the_segment_kernel():
+    """kmax = 16 in a cube (nx_max 16 > 12): no row kernel; the segment
+    kernel's tiles still fit."""
+    x = configs.melt(40, 10, 0.85, lj_rc=0.0, dt=0.005, seed=13, n_cell=16)
+    x.charge[2] = 1.0
+    x.alpha_c = 0.81
+    x.n_kmax = 16
+    p = lib.derive(x)
+    rc, plan, msg = lib.ewald_plan(p, p.n_part)
+    assert rc == 0, msg
+    assert plan["row_kernel"] == 0 and plan["nx_max"] == 16
+    _plan_invariants(plan, p.n_part)
+
+
+@pytest.mark.parametrize("n_charged", [65600, 300000])
+def test_ewald_plan_many_charges_on_the_kmax10_grid(n_charged):
+    """Beyond ~65.5k charges the segment kernel's force tile would exceed
+    64 KB; the row kernel only needs its rows of G, so such systems are
+    accepted (rows per group bounded by the LDS limit)."""
+    x = configs.pe_brush(400, 40, 5800, 63.0, 63.0, 40.0, n_kmax=10, alpha_c=0.81, rc_coul=3.0)
+    p = lib.derive(x)
+    p.n_part = max(p.n_part, n_charged)
+    rc, plan, msg = lib.ewald_plan(p, n_charged)
+    assert rc == 0, msg
+    assert plan["row_kernel"] == 1
+    _plan_invariants(plan, n_charged)

@@ -240,3 +240,66 @@ def test_observation_matches_reference(tmp_path):
     compare_obs(mine, ref)
     # the profiles see the whole brush and solvent, every step
     assert ref["dens_prof.mide"][:, 1:].sum() > 0
+
+
+# ------------------------------------------- committed reference fixtures
+# The same pins from outputs of the reference's own runs recorded by
+# tests/golden/make_ref_fixtures.py, so they hold without the reference built.
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def test_oracle_matches_reference_observation_fixture():
+    import hashlib
+    gold = os.path.join(GOLD, "obs_brush_small")
+    kind, inp = configs.small_cases()["brush_small"]
+    inp.skin = 0.0
+    inp.temp = 0.0
+    inp.n_relax, inp.n_obser, inp.n_safe = 0, 20, 1000
+    p, r, v, t = configs.build(inp, kind)
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="fp64", mode="compat")
+        conf = os.path.join(d, "conf_old")
+        assert hashlib.sha256(open(conf, "rb").read()).hexdigest() == \
+            open(os.path.join(gold, "conf_old.sha256")).read().strip()
+        t2, r2, v2, a2 = io.read_conf(conf, p.n_part)
+    s = orc.System(inp)
+    s.set_state(r2, v2, a2, t2)
+    s.binning()
+    compare_obs(oracle_obs_lines(s, inp, p, 20), read_obs_files(gold, p.n_chain))
+
+
+def test_oracle_matches_reference_window_fixture():
+    """The window observables of the adjoint tests (mean z of the type-2
+    beads, top-wall pressure, averaged over the steps) from the oracle's
+    trajectory against the reference's, at the fixture's theta and at its
+    perturbed thetas (the reference's finite differences)."""
+    fx = np.load(os.path.join(GOLD, "adjoint_ref_fd.npz"))
+    kind, x = configs.small_cases()["brush_small"]
+    x.temp = 0.0
+    x.skin = 0.0
+    n, first = int(fx["n_steps"]), int(fx["s_time"])
+    th = fx["theta"]
+
+    def window(theta):
+        xi = configs.with_theta(x, theta)
+        s = orc.System(xi)
+        s.set_state(fx["r"], fx["v"], fx["a"], fx["t"])
+        q = orc.derive(xi)
+        t2 = fx["t"] == 2
+        hs = ps = 0.0
+        for k in range(n):
+            s.run_steps(first + k, 1)
+            hs += s.get_state()["r"][t2, 2].mean()
+            s.zero_force()
+            ps += -s.fluid_wall()[2] / (q.boundary[0] * q.boundary[1])
+        return np.array([hs / n, ps / n])
+    assert np.allclose(window(th), fx["J"], rtol=1e-9, atol=0.0)
+    h = float(fx["h"])
+    for q in (1, 3):  # sigma and R0: the largest reference FD entries
+        tp, tm = th.copy(), th.copy()
+        tp[q] *= 1.0 + h
+        tm[q] *= 1.0 - h
+        g = (window(tp) - window(tm)) / (tp[q] - tm[q])
+        assert np.allclose(g, fx["G_fd"][:, q], rtol=1e-4, atol=0.0), (q, g, fx["G_fd"][:, q])
