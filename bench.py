@@ -9,6 +9,8 @@ HBM; the plain FP64 form, the mixed-vs-FP64 force error and the oracle's
 single-thread CPU rate on a bounded sample are reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py --workload NAME --save-state S.npz     (then, under rocprofv3:)
+    python bench.py --workload NAME --counters --state S.npz [--reps R]
 
 Other workloads (one JSON line each, same keys): brush9k (DPD brush, 9,000
 beads, dt 0.005), pe_brush27k (polyelectrolyte brush with Ewald, 27,000
@@ -38,15 +40,14 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # v (24) and the type (4), write F (24).
 FORCE_BYTES_PER_PARTICLE = 76
 STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
-# HBM bytes per force pass of the mixed form on this workload, from the
-# FETCH_SIZE and WRITE_SIZE counter runs (profiles/force_pass_melt100k_hbm_traffic.md),
-# each scaled by the same counter's reading for a copy of known size.
-FORCE_TRAFFIC_BYTES = 77.339e6 + 3.411e6
-# VALU wave-instructions per force pass of the mixed form on this workload
-# (SQ_INSTS_VALU, median of 8 launches, profiles/force_and_list_melt100k_sq_counters.md)
-# and the issue peak: 1024 SIMDs, one wave64 VALU instruction per 4 cycles
-# at 2.4 GHz.
-FORCE_VALU_INSTS = 25.441912e6
+# Counter figures of the mixed force pass on this workload, per threads per
+# particle (the autotuner's choice), measured by `bench.py --counters` on the
+# state the timed steps leave (profiles/force_pass_melt100k_hbm_traffic.md,
+# profiles/force_and_list_melt100k_sq_counters.md): HBM bytes per pass
+# (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a copy of known
+# size) and VALU wave-instructions per pass (SQ_INSTS_VALU).  The issue peak:
+# 1024 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz.
+FORCE_COUNTERS = {}
 VALU_PEAK_GINSTS = 1024 * 2.4 / 4
 
 
@@ -126,6 +127,19 @@ def force_error_vs_fp64(p, st):
     return float((d / np.maximum(nref, 1e-3 * frms)).max()), float(np.sqrt(np.mean(d ** 2)) / frms)
 
 
+def host_cpu():
+    """Model name and logical CPU count of the host running the bench."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(inputs, st0, n_steps):
     from oracle import oracle as orc
     s = orc.System(inputs)
@@ -195,6 +209,10 @@ def bench_single(args, name):
     log("fp64 : %.4f s for %d steps, force %.4f ms avg (%d threads per particle)" % (
         dt_d, args.steps, tm_d["force_ms"], tm_d["tpp"]))
     st_m["t"] = t
+    if args.save_state:  # the state the timed steps left, for the counter runs
+        np.savez(args.save_state, r=st_m["r"], v=st_m["v"], a=st_m["a"], mic=st_m["mic"], t=t,
+                 tpp=tm_m["tpp"])
+        log("saved the timed state to %s" % args.save_state)
     err_max, err_rms = force_error_vs_fp64(p, st_m)
     log("mixed vs fp64 force error: max %.3e rms %.3e" % (err_max, err_rms))
     value = n * args.steps / dt_m
@@ -202,17 +220,19 @@ def bench_single(args, name):
     cpu = None
     if not args.no_cpu:
         rate, secs = cpu_rate(inputs, st0, args.cpu_steps)
+        model, ncpu = host_cpu()
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": "oracle (plain-C FP64 restatement, 1 thread) on the same %d-bead %s, "
-                         "%d steps after 2 warm-up steps (%.1f s)" % (n, name, args.cpu_steps, secs)}
+               "cpu_model": model, "host_logical_cpus": ncpu,
+               "sample": "oracle (plain-C FP64 restatement of the reference's loop, gcc -O3 "
+                         "-march=x86-64-v3, 1 thread) on the same %d-bead %s start, %d steps "
+                         "after 2 warm-up steps (%.1f s)" % (n, name, args.cpu_steps, secs)}
     if name == "melt100k":
         config = {"n_particles": n, "chains": 2000, "chain_length": 50, "density": 0.85,
                   "lj_rc": 2.5, "dt": 0.01, "dpd_gamma": 1.0, "skin": 0.3,
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
-        traffic = FORCE_TRAFFIC_BYTES
-        valu = FORCE_VALU_INSTS
+        traffic, valu = FORCE_COUNTERS.get(tm_m["tpp"], (None, None))
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
@@ -378,6 +398,39 @@ def bench_adjoint(args):
                          "fp32 forward forces, fp64 adjoint", data, config, extra)
 
 
+COPY_BYTES = 256 * 1024 * 1024  # bytes read and bytes written by the calibration copy
+
+
+def counter_workload(args):
+    """The workload of the counter runs, meant to run under rocprofv3 with
+    one counter set per run (few dispatches, so each pass takes seconds):
+    loads the state a `--save-state` run left (the state the timed steps
+    reached, and the threads per particle the autotuner chose there), bins
+    it `--rebuilds` times, runs the force pass `--reps` times with direct
+    launches (melt100k, brush9k) or the k-space Ewald stage `--reps` times
+    (pe_brush27k), then a device copy of known size: the calibration of
+    FETCH_SIZE and WRITE_SIZE."""
+    w = configs.WORKLOADS[args.workload]
+    p = lib.derive(w["make"]())
+    st = np.load(args.state)
+    tpp = int(st["tpp"])
+    with lib.Context(p, lib.MFA_MIXED) as c:
+        c.set_graph_steps(0)
+        c.set_tpp(tpp)
+        c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
+        for _ in range(args.rebuilds):
+            c.binning()
+        if p.coulomb:
+            for _ in range(args.reps):
+                c.ewald_k(1)
+            ms = float("nan")
+        else:
+            ms = c.time_force(args.reps)
+    cms = lib.copy_probe(COPY_BYTES)
+    log("%s: %d threads per particle; force pass %.4f ms; calibration copy of %d bytes in %.4f ms"
+        % (args.workload, tpp, ms, COPY_BYTES, cms))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,12 +444,26 @@ def main():
                     help="batch256: skip the sequential comparison (profiling runs)")
     ap.add_argument("--graph-steps", type=int, default=16,
                     help="steps per captured hipGraph block (0: direct launches)")
+    ap.add_argument("--save-state", default=None,
+                    help="write the state the timed mixed steps left (and the chosen threads "
+                         "per particle) to this .npz, for --counters")
+    ap.add_argument("--counters", action="store_true",
+                    help="counter-run workload: load --state, bin, run --reps force passes "
+                         "(or k-space stages), then a calibration copy; prints no JSON line")
+    ap.add_argument("--state", default=None)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--rebuilds", type=int, default=1)
     args = ap.parse_args()
+    if args.counters:
+        if not args.state:
+            ap.error("--counters needs --state (written by a --save-state run)")
+        counter_workload(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(each GPU box has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    defaults = {"melt100k": (5000, 500, 20), "brush9k": (1000, 200, 200),
+    defaults = {"melt100k": (5000, 500, 40), "brush9k": (1000, 200, 1000),
                 "pe_brush27k": (2000, 200, 3), "batch256": (500, 100, 0),
                 "adjoint50k": (200, 20, 0)}
     d_steps, d_warm, d_cpu = defaults[args.workload]

@@ -20,6 +20,7 @@ MFA_MIXED = 1
 ERRORS = {
     0: "OK", 1: "MFA_ERR_ARG", 2: "MFA_ERR_HIP", 3: "MFA_ERR_LIST_OVERFLOW",
     4: "MFA_ERR_CELL", 5: "MFA_ERR_NOMEM", 6: "MFA_ERR_NAN", 7: "MFA_ERR_IO",
+    8: "MFA_ERR_CELL_OVERFLOW",
 }
 
 _d4 = C.c_double * NTYPE

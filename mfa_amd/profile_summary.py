@@ -13,18 +13,19 @@
 """
 import argparse
 import csv
+import re
 import statistics
 from collections import defaultdict
 
 COPY_BYTES = 256 * 1024 * 1024
-# kernel-name prefixes of the force pass without energies (any extra-terms flag)
-MIXED_FORCE = "void k_force_mixed<8, 0"
-FP64_FORCE = "void k_force64<8"
+# kernel names of the force pass without energies (any width, any extra-terms flag)
+MIXED_FORCE = re.compile(r"void k_force_mixed<\d+, 0")
+FP64_FORCE = re.compile(r"void k_force64<\d+")
 
 
-def _find(kern, prefix):
+def _find(kern, pattern):
     for k, v in kern.items():
-        if k.startswith(prefix):
+        if pattern.match(k):
             return v
     return None
 
@@ -34,10 +35,10 @@ def trace(path, n_timed):
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     names = [r["Kernel_Name"] for r in rows]
     try:
-        end = next(i for i, n in enumerate(names) if n.startswith(FP64_FORCE))
+        end = next(i for i, n in enumerate(names) if FP64_FORCE.match(n))
     except StopIteration:
         end = len(rows)
-    idx = [i for i in range(end) if names[i].startswith(MIXED_FORCE)]
+    idx = [i for i in range(end) if MIXED_FORCE.match(names[i])]
     seg = rows[idx[-n_timed]:end]
     d = defaultdict(list)
     for r in seg:

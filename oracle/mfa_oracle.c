@@ -456,6 +456,20 @@ int orc_check_skin(orc_sys* s) {
 
 static double pbc(double d, double L, double invL) { return d - L * (double)(int)(2 * d * invL); }
 
+/* Compensated (Neumaier) sum for the virial, a quantity of this build (the
+ * reference has none, so no summation order to follow): the result is the
+ * sum of the terms to about one rounding, whatever the number of terms. */
+typedef struct { double s, c; } ksum_t;
+static void ksum_add(ksum_t* k, double x) {
+  const double t = k->s + x;
+  if (fabs(k->s) >= fabs(x))
+    k->c += (k->s - t) + x;
+  else
+    k->c += (x - t) + k->s;
+  k->s = t;
+}
+static double ksum_value(const ksum_t* k) { return k->s + k->c; }
+
 static double list_crit(const orc_params* p, int ti, int tj) {
   double l2 = p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : p->range_2[ti][tj] + p->skin_2;
   if (p->coulomb && p->rc_coul_2 + p->skin_2 > l2) l2 = p->rc_coul_2 + p->skin_2;
@@ -622,7 +636,8 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
   const orc_params* p = &s->p;
   const int ls = s->n_neigh + 1;
   const int head_excl = p->symmetry == 0 && (p->system == 0 || p->system == 1 || p->system == 3);
-  double v_ff = 0., vir = 0., v_coul = 0.;
+  double v_ff = 0., v_coul = 0.;
+  ksum_t vir = {0., 0.};
   /* force switch-on (fluid_fluid.f90:42-49), r_time = step*dt (md_main.f90:69) */
   double r_2_min = 0.;
   const double r_time = (double)step * p->dt;
@@ -656,7 +671,7 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
         const double pot_loc = (r_12 - r_6) - p->e_shift[ti][tj];
         v_ff = v_ff + l_eps * pot_loc;
         const double r_dummy = l_eps * (-12 * r_12 + 6 * r_6) * inv_r_2;
-        vir = vir + (-r_dummy * r_2_raw);
+        ksum_add(&vir, -r_dummy * r_2_raw);
         for (int d = 0; d < 3; ++d) {
           const double fl = r_dummy * dr[d];
           f_ipart[d] = f_ipart[d] - fl;
@@ -711,7 +726,7 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
     }
   }
   out[0] = v_ff;
-  out[1] = vir;
+  out[1] = ksum_value(&vir);
   out[2] = p->coulomb ? v_coul + s->v_coul_self : 0.; /* fluid_fluid.f90:238-240 */
 }
 
@@ -864,7 +879,7 @@ void orc_fluid_wall(orc_sys* s, double* out) {
   out[2] = fzt;
 }
 
-static void fene_bond(orc_sys* s, int ip, int jp, double* pot, double* vir) {
+static void fene_bond(orc_sys* s, int ip, int jp, double* pot, ksum_t* vir) {
   const orc_params* p = &s->p;
   double dr[3];
   for (int d = 0; d < 3; ++d) dr[d] = R(ip, d) - R(jp, d);
@@ -877,7 +892,7 @@ static void fene_bond(orc_sys* s, int ip, int jp, double* pot, double* vir) {
   if (r_dummy > (double)0.98f) r_dummy = (double)0.98f;
   *pot = *pot + (-0.5 * log(1. - r_dummy));
   r_dummy = p->k_chain / (1. - r_dummy) * p->inv_r_chain_2;
-  *vir = *vir + (-r_dummy * r_2);
+  ksum_add(vir, -r_dummy * r_2);
   for (int d = 0; d < 3; ++d) {
     const double fl = r_dummy * dr[d];
     F(ip, d) = F(ip, d) - fl;
@@ -890,7 +905,8 @@ static void fene_bond(orc_sys* s, int ip, int jp, double* pot, double* vir) {
  * differences of the energy. */
 void orc_intra_molec(orc_sys* s, double* out) {
   const orc_params* p = &s->p;
-  double vb = 0., vd = 0., ve = 0., vir = 0.;
+  double vb = 0., vd = 0., ve = 0.;
+  ksum_t vir = {0., 0.};
   for (int c = 0; c < p->n_chain; ++c)
     for (int m = 0; m < p->n_mon - 1; ++m) {
       const int ip = c * p->n_mon + m;
@@ -910,7 +926,7 @@ void orc_intra_molec(orc_sys* s, double* out) {
     }
   ve = p->k_chain * ve;
   out[0] = vb + vd + ve;
-  out[1] = vir;
+  out[1] = ksum_value(&vir);
 }
 
 /* verlet_velocities.f90:25-36, head pinning :113-134, kinetic energy

@@ -600,3 +600,10 @@ def test_ewald_plan_many_charges_on_the_kmax10_grid(n_charged):
     assert rc == 0, msg
     assert plan["row_kernel"] == 1
     _plan_invariants(plan, n_charged)
+
+
+def test_error_codes_match_the_header():
+    """Every MFA_ERR_* of include/mfa_hip.h has its name in lib.ERRORS."""
+    text = open(os.path.join(ROOT, "include", "mfa_hip.h")).read()
+    codes = {int(v): k for k, v in re.findall(r"(MFA_ERR_\w+)\s*=\s*(\d+)", text)}
+    assert codes and all(lib.ERRORS.get(v) == k for v, k in codes.items()), codes
