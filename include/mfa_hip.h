@@ -263,6 +263,20 @@ int32_t mfa_intra_molec(mfa_ctx* ctx, double* v_intra);
 int32_t mfa_bending(mfa_ctx* ctx, int32_t mode, double* v_bend);
 /* verlet_velocities (verlet_velocities.f90:18-199); *t_kin = 0.5 sum m v^2. */
 int32_t mfa_verlet_velocities(mfa_ctx* ctx, double* t_kin);
+/* verlet_velocities on arrays the caller owns, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same arithmetic as
+ * mfa_verlet_velocities (a = F/m, v += dt/2 a, pinned heads) on device
+ * arrays force, v, a (double[N][3]) and a_type (int32[N], 1-based), N =
+ * p->n_part <= MFA_NOALLOC_MAX_N, on `stream` (a hipStream_t; NULL: the null
+ * stream).  t_kin (host pointer, may be NULL) receives 0.5 sum m v^2 and then
+ * the call synchronises the stream; without it the call is asynchronous.
+ * Scratch: one file-scope device array per GPU (about 1.5 MB), shared by
+ * every call on that GPU: calls that overlap on one GPU race. */
+#define MFA_NOALLOC_MAX_N (1 << 22)
+int32_t mfa_verlet_velocities_noalloc(const mfa_params* p, double* force, double* v, double* a,
+                                      const int32_t* a_type, void* stream, double* t_kin);
 /* md_main.f90:149-155 with DPD_VV: new_dpd_fd.f90:18-124, the dissipative
  * force recomputed with the new velocities, swapped into F (F - F_D,old +
  * F_D,new), a further half kick and a = F/m; t_kin as new_dpd_fd computes it. */

@@ -35,6 +35,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -467,15 +468,14 @@ __global__ void k_check_skin(const KParams* __restrict__ kb, const int* __restri
 }
 
 // verlet_velocities alone: writes v, a and zeroes head forces; kinetic energy
-// partials when `energy` (verlet_velocities.f90:155-177).
-__global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* __restrict__ rmap,
-                                    double* __restrict__ f, double* __restrict__ v,
-                                    double* __restrict__ a, const int* __restrict__ type,
-                                    double* __restrict__ part, int energy,
-                                    const int* __restrict__ sidx, float4* __restrict__ vel4,
-                                    double* __restrict__ head_f) {
-  MFA_REPLICA(blockIdx.x * blockDim.x);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// partials when `energy` (verlet_velocities.f90:155-177).  Particle i of the
+// block's replica k; the FP32 view (vel4 at sidx) and the head forces are
+// written when their arrays are given.
+__device__ __forceinline__ void vv_body(const KParams& k, int i, double* __restrict__ f,
+                                        double* __restrict__ v, double* __restrict__ a,
+                                        const int* __restrict__ type, double* __restrict__ part,
+                                        int energy, const int* __restrict__ sidx,
+                                        float4* __restrict__ vel4, double* __restrict__ head_f) {
   double vals[MFA_NTERMS];
   for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
   if (i - k.p0 < k.n) {
@@ -494,12 +494,31 @@ __global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* _
       v[3 * (size_t)i + d] = vi[d];
       a[3 * (size_t)i + d] = ai[d];
     }
-    vel4[sidx[i]] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
+    if (vel4) vel4[sidx[i]] = make_float4((float)vi[0], (float)vi[1], (float)vi[2], 0.0f);
     if (is_head(k, i - k.p0) && !k.dpd_vv)
       for (int d = 0; d < 3; ++d) f[3 * (size_t)i + d] = 0.0;
     vals[T_TKIN] = k.mass_type[ti] * ((vi[0] * vi[0] + vi[1] * vi[1]) + vi[2] * vi[2]);
   }
   if (energy) block_partials(vals, part);
+}
+
+__global__ void k_verlet_velocities(const KParams* __restrict__ kb, const int* __restrict__ rmap,
+                                    double* __restrict__ f, double* __restrict__ v,
+                                    double* __restrict__ a, const int* __restrict__ type,
+                                    double* __restrict__ part, int energy,
+                                    const int* __restrict__ sidx, float4* __restrict__ vel4,
+                                    double* __restrict__ head_f) {
+  MFA_REPLICA(blockIdx.x * blockDim.x);
+  vv_body(k, blockIdx.x * blockDim.x + threadIdx.x, f, v, a, type, part, energy, sidx, vel4, head_f);
+}
+
+// The same for one system on arrays the caller owns, its constants passed by
+// value (no context, no replica table).
+__global__ void k_verlet_velocities_1(KParams k, double* __restrict__ f, double* __restrict__ v,
+                                      double* __restrict__ a, const int* __restrict__ type,
+                                      double* __restrict__ part, int energy) {
+  vv_body(k, blockIdx.x * blockDim.x + threadIdx.x, f, v, a, type, part, energy, nullptr, nullptr,
+          nullptr);
 }
 
 // Fused: verlet_velocities of step s, then verlet_positions and check_skin of
@@ -3438,6 +3457,72 @@ extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
     *t_kin = 0.5 * t[T_TKIN];
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// verlet_velocities without any allocation, on arrays the caller owns.  Its
+// scratch (the kinetic-energy block partials and their sum) is one
+// file-scope device array; each GPU holds its own copy, whose address is
+// looked up once per device (with that device current) and kept.  Calls on
+// one GPU share it: calls that overlap on one GPU race.
+#define MFA_NOALLOC_BLOCKS (MFA_NOALLOC_MAX_N / MFA_BLOCK)
+static __device__ __align__(256) double scratch[(MFA_NOALLOC_BLOCKS + 1) * MFA_NTERMS];
+#define MFA_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MFA_MAX_DEVICES];
+
+static int scratch_of_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err(nullptr, MFA_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= MFA_MAX_DEVICES)
+    return set_err(nullptr, MFA_ERR_HIP, "device %d beyond the %d the library tracks", dev,
+                   MFA_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(nullptr, MFA_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_verlet_velocities_noalloc(const mfa_params* p, double* force, double* v,
+                                                 double* a, const int32_t* a_type, void* stream,
+                                                 double* t_kin) {
+  if (!p || !force || !v || !a || !a_type)
+    return set_err(nullptr, MFA_ERR_ARG, "mfa_verlet_velocities_noalloc: null argument");
+  if (p->n_part < 1 || p->n_part > MFA_NOALLOC_MAX_N)
+    return set_err(nullptr, MFA_ERR_ARG,
+                   "mfa_verlet_velocities_noalloc: n_part = %d outside 1..%d (the scratch limit)",
+                   p->n_part, MFA_NOALLOC_MAX_N);
+  double* scr = nullptr;
+  int rc = scratch_of_current_device(&scr);
+  if (rc) return rc;
+  KParams k = make_kparams(*p);  // p0 = 0, replica 0
+  hipStream_t s = (hipStream_t)stream;
+  const int g = grid_of(p->n_part);
+  // every partial the sum reads ([0, g) blocks) is written by this launch,
+  // and the sum overwrites its slots: nothing needs clearing
+  double* part = scr;
+  double* red = scr + (size_t)MFA_NOALLOC_BLOCKS * MFA_NTERMS;
+  hipLaunchKernelGGL(k_verlet_velocities_1, dim3(g), dim3(MFA_BLOCK), 0, s, k, force, v, a, a_type,
+                     part, t_kin ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(nullptr, MFA_ERR_HIP, "launch: %s", hipGetErrorString(e));
+  if (t_kin) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(MFA_NTERMS), dim3(MFA_BLOCK), 0, s, part, 0, g, red);
+    double h = 0.0;
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, red + T_TKIN, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_err(nullptr, MFA_ERR_HIP, "kinetic energy: %s", hipGetErrorString(e));
+    *t_kin = 0.5 * h;
+  }
   return MFA_OK;
 }
 

@@ -165,6 +165,8 @@ SIGNATURES = {
     "mfa_obs_read": (C.c_int32, [_P, C.POINTER(ObsSums), _dp, _dp]),
     "mfa_verlet_velocities": (C.c_int32, [_P, _dp]),
     "mfa_new_dpd_fd": (C.c_int32, [_P, _dp]),
+    "mfa_verlet_velocities_noalloc": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     "mfa_run_steps": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(Obs)]),
     "mfa_energies": (C.c_int32, [_P, C.c_int64, C.POINTER(Obs)]),
     "mfa_timing": (C.c_int32, [_P, C.c_int32, _dp, _lp, _dp, _lp]),
