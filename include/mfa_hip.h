@@ -272,9 +272,9 @@ int32_t mfa_verlet_velocities(mfa_ctx* ctx, double* t_kin);
  * p->n_part <= MFA_NOALLOC_MAX_N, on `stream` (a hipStream_t; NULL: the null
  * stream).  t_kin (host pointer, may be NULL) receives 0.5 sum m v^2 and then
  * the call synchronises the stream; without it the call is asynchronous.
- * Scratch: one file-scope device array per GPU (about 1.5 MB), shared by
+ * Scratch: one file-scope device array per GPU (about 400 KB), shared by
  * every call on that GPU: calls that overlap on one GPU race. */
-#define MFA_NOALLOC_MAX_N (1 << 22)
+#define MFA_NOALLOC_MAX_N (1 << 20)
 int32_t mfa_verlet_velocities_noalloc(const mfa_params* p, double* force, double* v, double* a,
                                       const int32_t* a_type, void* stream, double* t_kin);
 /* md_main.f90:149-155 with DPD_VV: new_dpd_fd.f90:18-124, the dissipative

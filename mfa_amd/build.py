@@ -39,9 +39,13 @@ def build_library(verbose=True):
     dev_o = os.path.join(objdir, "mfa_device.o")
     _run(["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-c",
           os.path.join(CSRC, "mfa_host.cpp"), "-o", host_o])
+    noalloc_o = os.path.join(objdir, "mfa_noalloc.o")
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mfa_device.hip"), "-o", dev_o])
+    # the allocation-free entry point and its file-scope scratch: a code
+    # object of its own, loaded only when that entry point runs
+    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mfa_noalloc.hip"), "-o", noalloc_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", dev_o, host_o, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", dev_o, noalloc_o, host_o, "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

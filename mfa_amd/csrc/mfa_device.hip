@@ -35,7 +35,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1409,6 +1408,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
 }
 
+// XT bits: 1 Coulomb real space and bending, 2 the Langevin force, 4 DPD_CUT_OFF
+// (a DPD range of its own, heads excluded from DPD).
 template <int TPP, int ENERGY, int XT>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
@@ -1417,6 +1418,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const long long* __restrict__ d_step, int mask, double* __restrict__ part,
     int* __restrict__ flags_all) {
 #pragma clang fp contract(fast)
+  constexpr bool DC = (XT & 4) != 0;
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   int* flags = flags_all + k.rep * F_NFLAGS;
   // cell offsets -(d)*r_bin of the 27 codes, and the FP32 pair tables
@@ -1460,6 +1462,12 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const float pref = (float)(k.sqrt_alpha * k.inv_sqrt_pi * 2.0);
   double ax = 0.0, ay = 0.0, az = 0.0;  // fixed-point sums (integer-valued)
 
+  // run-time switches uniform over the launch, read once
+  const uint32_t seed = k.seed;
+  const bool dpd_on = k.dpd_on != 0;
+  const bool head_excl = DC && k.dpd_head_excl != 0;
+  const bool coul_on = (XT & 1) && k.coulomb != 0;
+  const bool coul_rc = (XT & 1) && k.rc_coul_2 > 0.0;  // own real-space cut-off
   if (active && (mask & TM_PAIR)) {
     const int cnt = nn[ss];
     const unsigned* row = nl + (size_t)ss * cap;
@@ -1476,12 +1484,13 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
       const float4 tb = s_tab[ti * MFA_NTYPE + tj];
       bool in_lj = r2raw < tb.x;
-      bool in_dpd = k.dpd_cut_off_on ? r2raw < rdpd2 : in_lj;
-      bool in_coul = (XT & 1) && k.rc_coul_2 > 0.0 && r2raw < rc2f;
-      bool near = fabsf(r2raw - tb.x) < margin;
-      if (k.dpd_cut_off_on) near = near || fabsf(r2raw - rdpd2) < margin;
-      if ((XT & 1) && k.rc_coul_2 > 0.0) near = near || fabsf(r2raw - rc2f) < margin;
-      if (near) {
+      bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
+      bool in_coul = coul_rc && r2raw < rc2f;
+      // distance of r^2 to the nearest cut-off that decides something
+      float dm = fabsf(r2raw - tb.x);
+      if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
+      if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
+      if (__builtin_expect(dm < margin, 0)) {
         // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
         const int j = k.p0 + jl;
         double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
@@ -1490,13 +1499,13 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         min_image(k, ex, ey, ez);
         const double r2d = (ex * ex + ey * ey) + ez * ez;
         in_lj = r2d < k.range_2[ti][tj];
-        in_dpd = k.dpd_cut_off_on ? r2d < k.r_cut_dpd_2 : in_lj;
-        in_coul = (XT & 1) && k.rc_coul_2 > 0.0 && r2d < k.rc_coul_2;
+        in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
+        in_coul = coul_rc && r2d < k.rc_coul_2;
       }
-      const bool dpd = k.dpd_on && in_dpd && !(k.dpd_head_excl && k.dpd_cut_off_on &&
-                                               (ti == 0 || tj == 0));
+      // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
+      const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
-      const bool coul = (XT & 1) && k.coulomb && (k.rc_coul_2 > 0.0 ? in_coul : in_lj);
+      const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
       if (!in_lj && !dpd && !coul) continue;
       const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
       const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
@@ -1514,22 +1523,27 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         }
       }
       if (dpd) {
-        const float xi_n = pair_normal_f32(k.seed, step, il, jl);
+        const float xi_n = pair_normal_f32(seed, step, il, jl);
         // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
-        const float inv_rc = k.dpd_cut_off_on ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj];
+        const float inv_rc = DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj];
         const float wr = 1.0f - (r2 * inv_r) * inv_rc;
         const float4 vj = vel4[js];
         const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
         s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
       }
-      if (coul) {  // ewald_real.f90:15-32 with the clamped r_2
+      if ((XT & 1) && coul) {  // ewald_real.f90:15-32 with the clamped r_2
         const float qq = (float)k.q_type[ti] * (float)k.q_type[tj];
         const float rr = r2 * inv_r;
         const float ec = erfcf(sa * rr);
         s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
         if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
       }
-      fix_add3(ax, ay, az, s * dx, s * dy, s * dz);
+      // the pair's three components in fixed point: (s 2^22) d equals
+      // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
+      const float sf = s * MFA_FIX_SCALE;
+      ax += (double)truncf(sf * dx);
+      ay += (double)truncf(sf * dy);
+      az += (double)truncf(sf * dz);
     }
   }
 
@@ -2327,22 +2341,27 @@ static void launch_force_tx(mfa_ctx* c, int mask) {
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags);
   } else {
-    hipLaunchKernelGGL((k_force64<TPP, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+    hipLaunchKernelGGL((k_force64<TPP, XT & 3>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
                        c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
                        mask, c->part, c->flags);
   }
 }
 
-// XT bit 1: Coulomb real space and bending; bit 2: the Langevin force.  The
-// terms a system does not use are compiled out of its force kernels (they
-// would otherwise cost registers, hence occupancy, in every launch).
+// XT bit 1: Coulomb real space and bending; bit 2: the Langevin force; bit 4
+// (mixed form): DPD_CUT_OFF.  The terms a system does not use are compiled
+// out of its force kernels (they would otherwise cost registers, hence
+// occupancy, and decision logic in every launch).
 template <int TPP>
 static void launch_force_t(mfa_ctx* c, int mask) {
-  switch ((c->extra_terms ? 1 : 0) | (c->lgv_terms ? 2 : 0)) {
+  switch ((c->extra_terms ? 1 : 0) | (c->lgv_terms ? 2 : 0) | (c->p.dpd_cut_off_on ? 4 : 0)) {
     case 0: launch_force_tx<TPP, 0>(c, mask); break;
     case 1: launch_force_tx<TPP, 1>(c, mask); break;
     case 2: launch_force_tx<TPP, 2>(c, mask); break;
-    default: launch_force_tx<TPP, 3>(c, mask); break;
+    case 3: launch_force_tx<TPP, 3>(c, mask); break;
+    case 4: launch_force_tx<TPP, 4>(c, mask); break;
+    case 5: launch_force_tx<TPP, 5>(c, mask); break;
+    case 6: launch_force_tx<TPP, 6>(c, mask); break;
+    default: launch_force_tx<TPP, 7>(c, mask); break;
   }
 }
 
@@ -3461,55 +3480,18 @@ extern "C" int32_t mfa_verlet_velocities(mfa_ctx* c, double* t_kin) {
 }
 
 // ---------------------------------------------------------------------------
-// verlet_velocities without any allocation, on arrays the caller owns.  Its
-// scratch (the kinetic-energy block partials and their sum) is one
-// file-scope device array; each GPU holds its own copy, whose address is
-// looked up once per device (with that device current) and kept.  Calls on
-// one GPU share it: calls that overlap on one GPU race.
-#define MFA_NOALLOC_BLOCKS (MFA_NOALLOC_MAX_N / MFA_BLOCK)
-static __device__ __align__(256) double scratch[(MFA_NOALLOC_BLOCKS + 1) * MFA_NTERMS];
-#define MFA_MAX_DEVICES 64
-static std::mutex g_scratch_mu;
-static double* g_scratch_ptr[MFA_MAX_DEVICES];
-
-static int scratch_of_current_device(double** out) {
-  int dev = -1;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return set_err(nullptr, MFA_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
-  if (dev < 0 || dev >= MFA_MAX_DEVICES)
-    return set_err(nullptr, MFA_ERR_HIP, "device %d beyond the %d the library tracks", dev,
-                   MFA_MAX_DEVICES);
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  if (!g_scratch_ptr[dev]) {
-    void* ptr = nullptr;
-    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
-    if (e != hipSuccess || !ptr)
-      return set_err(nullptr, MFA_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
-    g_scratch_ptr[dev] = (double*)ptr;
-  }
-  *out = g_scratch_ptr[dev];
-  return MFA_OK;
-}
-
-extern "C" int32_t mfa_verlet_velocities_noalloc(const mfa_params* p, double* force, double* v,
-                                                 double* a, const int32_t* a_type, void* stream,
-                                                 double* t_kin) {
-  if (!p || !force || !v || !a || !a_type)
-    return set_err(nullptr, MFA_ERR_ARG, "mfa_verlet_velocities_noalloc: null argument");
-  if (p->n_part < 1 || p->n_part > MFA_NOALLOC_MAX_N)
-    return set_err(nullptr, MFA_ERR_ARG,
-                   "mfa_verlet_velocities_noalloc: n_part = %d outside 1..%d (the scratch limit)",
-                   p->n_part, MFA_NOALLOC_MAX_N);
-  double* scr = nullptr;
-  int rc = scratch_of_current_device(&scr);
-  if (rc) return rc;
+// verlet_velocities on arrays the caller owns, with a scratch buffer the
+// caller provides (part: one row of MFA_NTERMS block partials per 256
+// particles; red: MFA_NTERMS sums).  Allocates nothing; the allocation-free
+// entry point (mfa_noalloc.hip) passes its file-scope device array here.
+int mfa_internal_vv_scratch(const mfa_params* p, double* force, double* v, double* a,
+                            const int32_t* a_type, void* stream, double* t_kin, double* part,
+                            double* red) {
   KParams k = make_kparams(*p);  // p0 = 0, replica 0
   hipStream_t s = (hipStream_t)stream;
   const int g = grid_of(p->n_part);
   // every partial the sum reads ([0, g) blocks) is written by this launch,
   // and the sum overwrites its slots: nothing needs clearing
-  double* part = scr;
-  double* red = scr + (size_t)MFA_NOALLOC_BLOCKS * MFA_NTERMS;
   hipLaunchKernelGGL(k_verlet_velocities_1, dim3(g), dim3(MFA_BLOCK), 0, s, k, force, v, a, a_type,
                      part, t_kin ? 1 : 0);
   hipError_t e = hipGetLastError();
@@ -3525,6 +3507,8 @@ extern "C" int32_t mfa_verlet_velocities_noalloc(const mfa_params* p, double* fo
   }
   return MFA_OK;
 }
+
+int mfa_internal_set_error(int code, const char* msg) { return set_err(nullptr, code, "%s", msg); }
 
 extern "C" int32_t mfa_new_dpd_fd(mfa_ctx* c, double* t_kin) {
   if (!c) return MFA_ERR_ARG;

@@ -332,17 +332,19 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
 }
 
 // ---------------------------------------------------------------------------
-// Seeded generators.  Uniform doubles come from Threefry-2x32 on a running
-// counter (53 bits from two words), so the configurations are the same on
-// every machine.
+// Seeded generators.  Uniform doubles come from Threefry-2x32 (20 rounds,
+// independent of the rounds of the force pass's deviates, so the seeded
+// inputs do not change with them) on a running counter (53 bits from two
+// words), so the configurations are the same on every machine.
 namespace {
+#define GEN_TF_ROUNDS 20
 
 struct Stream {
   uint32_t k0, k1;
   uint64_t ctr;
   double uniform() {
     uint32_t a, b;
-    mfa_threefry2x32((uint32_t)ctr, (uint32_t)(ctr >> 32), k0, k1, MFA_TF_ROUNDS, &a, &b);
+    mfa_threefry2x32((uint32_t)ctr, (uint32_t)(ctr >> 32), k0, k1, GEN_TF_ROUNDS, &a, &b);
     ++ctr;
     const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);  // 53 bits
     return (double)m * (1.0 / 9007199254740992.0);

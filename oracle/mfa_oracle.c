@@ -69,13 +69,13 @@ void orc_threefry2x32(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1, int ro
 }
 
 /* Normal deviate of the pair (i0, j0) (0-based ids) at `step`: Box-Muller
- * on the two 24-bit uniforms of Threefry-2x32-20 with counter (min, max) and
+ * on the two 24-bit uniforms of Threefry-2x32-13 with counter (min, max) and
  * key (seed, step).  Pinned by moment tests (mean, variance, kurtosis) and by
  * the FP64 GPU form on identical keys. */
 double orc_pair_normal(uint32_t seed, int64_t step, int i0, int j0) {
   uint32_t w0, w1;
   const uint32_t lo = (uint32_t)(i0 < j0 ? i0 : j0), hi = (uint32_t)(i0 < j0 ? j0 : i0);
-  orc_threefry2x32(lo, hi, seed, (uint32_t)step, 20, &w0, &w1);
+  orc_threefry2x32(lo, hi, seed, (uint32_t)step, ORC_TF_ROUNDS, &w0, &w1);
   const double u1 = (double)(w0 >> 8) * (1.0 / 16777216.0) + (1.0 / 33554432.0);
   const double u2 = (double)(w1 >> 8) * (1.0 / 16777216.0);
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
@@ -83,12 +83,13 @@ double orc_pair_normal(uint32_t seed, int64_t step, int i0, int j0) {
 
 /* Normal deviate d (0, 1, 2) of particle i0 (0-based id) at `step`, for the
  * Langevin random force (lgv_forces.f90:25-31, one rnor() per component):
- * the same Box-Muller transform on Threefry-2x32-20 with counter
+ * the same Box-Muller transform on Threefry-2x32-13 with counter
  * (i0, 2^31 + d), a counter no pair uses (pair counters stay below 2^27).
  * Pinned by moment tests and by the FP64 GPU form on identical keys. */
 double orc_particle_normal(uint32_t seed, int64_t step, int i0, int d) {
   uint32_t w0, w1;
-  orc_threefry2x32((uint32_t)i0, 0x80000000u | (uint32_t)d, seed, (uint32_t)step, 20, &w0, &w1);
+  orc_threefry2x32((uint32_t)i0, 0x80000000u | (uint32_t)d, seed, (uint32_t)step, ORC_TF_ROUNDS, &w0,
+                   &w1);
   const double u1 = (double)(w0 >> 8) * (1.0 / 16777216.0) + (1.0 / 33554432.0);
   const double u2 = (double)(w1 >> 8) * (1.0 / 16777216.0);
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);

@@ -10,7 +10,7 @@
  * The only departure from the reference is the DPD noise: the reference draws
  * from one sequential ziggurat stream (dpd_forces.f90:273), which no parallel
  * code can reproduce; the oracle uses the counter-based normal that the GPU
- * path uses (Threefry-2x32-20 keyed by seed, step and the pair, Box-Muller),
+ * path uses (Threefry-2x32-13 keyed by seed, step and the pair, Box-Muller),
  * implemented here independently.
  */
 #ifndef MFA_ORACLE_H
@@ -56,6 +56,8 @@ typedef struct orc_params {
 typedef struct orc_sys orc_sys;
 
 /* counter-based RNG (independent implementation, pinned by known-answer tests) */
+/* rounds of the pair and particle deviates (the HIP path's MFA_TF_ROUNDS) */
+#define ORC_TF_ROUNDS 13
 void orc_threefry2x32(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1, int rounds,
                       uint32_t* o0, uint32_t* o1);
 double orc_pair_normal(uint32_t seed, int64_t step, int i0, int j0);

@@ -134,7 +134,7 @@ def load():
     return lib
 
 
-def threefry2x32(c0, c1, k0, k1, rounds=20):
+def threefry2x32(c0, c1, k0, k1, rounds=13):
     lib = load()
     a, b = C.c_uint32(), C.c_uint32()
     lib.orc_threefry2x32(c0, c1, k0, k1, rounds, C.byref(a), C.byref(b))

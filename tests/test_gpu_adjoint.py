@@ -7,7 +7,9 @@ torch.autograd wrapper; and invariance under the checkpoint interval.
 
 Bar: every gradient component within 1e-4 relative of the finite
 difference (components below 1e-3 of their row's largest count against that
-largest value)."""
+largest value); the GPU and oracle differences use h = 1e-6 (see
+test_adjoint_matches_finite_differences), the reference's fixture h = 1e-4
+over a 20-step window."""
 import os
 
 import numpy as np
@@ -42,13 +44,21 @@ def relaxed():
 
 
 def test_adjoint_matches_finite_differences(relaxed):
+    """Central differences at h = 1e-6 within 1e-4, and at h = 1e-4 within
+    1e-3: J(theta) is only once differentiable where a pair crosses a
+    cut-off inside the step (the WCA force and the DPD weight vanish there
+    with a kink), so the h = 1e-4 difference carries an O(h) error that can
+    reach a few 1e-4 on the sigma column."""
     x, st = relaxed
     run = make_runner(x, st, 1, W, K)
     th = configs.theta_of(x)
     J, G = run(th)
-    Gfd = central_fd(make_runner(x, st, 1, W, K, want_grad=False), th)
-    print("J", J, "\nadjoint\n", G, "\nFD\n", Gfd, flush=True)
+    fwd = make_runner(x, st, 1, W, K, want_grad=False)
+    Gfd = central_fd(fwd, th, h=1e-6)
+    Gfd4 = central_fd(fwd, th, h=1e-4)
+    print("J", J, "\nadjoint\n", G, "\nFD h=1e-6\n", Gfd, "\nFD h=1e-4\n", Gfd4, flush=True)
     _grad_close(G, Gfd)
+    _grad_close(G, Gfd4, tol=1e-3)
 
 
 def test_checkpoint_interval_and_forward_state(relaxed):
@@ -121,8 +131,8 @@ def test_adjoint_matches_oracle_finite_differences(relaxed):
     J_orc, _ = _oracle_window(x, st, th, 1, W)
     J_gpu, _ = make_runner(x, st, 1, W, K, want_grad=False)(th)
     assert np.allclose(J_orc, J_gpu, rtol=1e-10, atol=0.0), (J_orc, J_gpu)
-    Gfd = central_fd(lambda t: _oracle_window(x, st, t, 1, W), th)
-    print("oracle FD\n", Gfd, flush=True)
+    Gfd = central_fd(lambda t: _oracle_window(x, st, t, 1, W), th, h=1e-6)
+    print("oracle FD (h = 1e-6)\n", Gfd, flush=True)
     _grad_close(G, Gfd)
 
 

@@ -72,7 +72,7 @@ def test_particle_normal_moments_and_independence():
     # new numbers per step and per seed; apart from the pair stream
     assert orc.particle_normal(5, 3, 10, 0) != orc.particle_normal(5, 4, 10, 0)
     assert orc.particle_normal(5, 3, 10, 0) != orc.particle_normal(6, 3, 10, 0)
-    w = orc.threefry2x32(10, 0x80000000, 5, 3)
+    w = orc.threefry2x32(10, 0x80000000, 5, 3, rounds=13)
     u1 = (w[0] >> 8) * 2.0 ** -24 + 2.0 ** -25
     u2 = (w[1] >> 8) * 2.0 ** -24
     assert abs(orc.particle_normal(5, 3, 10, 0) -
@@ -607,3 +607,19 @@ def test_error_codes_match_the_header():
     text = open(os.path.join(ROOT, "include", "mfa_hip.h")).read()
     codes = {int(v): k for k, v in re.findall(r"(MFA_ERR_\w+)\s*=\s*(\d+)", text)}
     assert codes and all(lib.ERRORS.get(v) == k for v, k in codes.items()), codes
+
+
+def test_pair_deviate_is_box_muller_of_threefry13():
+    """The pair deviate is Box-Muller of Threefry-2x32-13 on counter (min,
+    max) and key (seed, step), in the oracle and (by its round constant) in
+    the HIP path."""
+    w = orc.threefry2x32(10, 20, 5, 3, rounds=13)
+    u1 = (w[0] >> 8) * 2.0 ** -24 + 2.0 ** -25
+    u2 = (w[1] >> 8) * 2.0 ** -24
+    assert abs(orc.pair_normal(5, 3, 20, 10) -
+               np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)) < 1e-14
+    dev = open(os.path.join(ROOT, "mfa_amd", "csrc", "mfa_rng.h")).read()
+    orc_h = open(os.path.join(ROOT, "oracle", "mfa_oracle.h")).read()
+    r_dev = int(re.search(r"#define MFA_TF_ROUNDS (\d+)", dev).group(1))
+    r_orc = int(re.search(r"#define ORC_TF_ROUNDS (\d+)", orc_h).group(1))
+    assert r_dev == r_orc == 13

@@ -56,7 +56,7 @@ def test_noalloc_matches_verlet_velocities_twice():
 def test_noalloc_refuses_sizes_beyond_the_scratch():
     inputs, p, r, v, t, a = make_case("melt_small")
     q = lib.Params.from_buffer_copy(p)
-    q.n_part = (1 << 22) + 1
+    q.n_part = (1 << 20) + 1
     L = lib.load()
     tk = C.c_double()
     dummy = C.c_void_p(1)  # never dereferenced: the size check comes first
