@@ -81,10 +81,14 @@ def compare(runs, ref_runs, name):
     return np.abs(a.mean(axis=0)[use] - b.mean(axis=0)[use]) / se[use]
 
 
-def check(runs, ref_runs, label, max_frac=0.15, max_chi2=2.0):
+def check(runs, ref_runs, label, max_frac=0.25, max_chi2=2.0):
     """The bars of the statistical tests: temperature within 2 sigma; for each
     profile at most max_frac of the bins beyond 2 sigma and a mean chi^2 per
-    bin below max_chi2."""
+    bin below max_chi2.  With identical statistics a bin lies beyond 2 sigma
+    with probability 0.046, so among the 18 bins of a density profile 3 do
+    so about one time in twenty (a bar of 15% would fail that often), 5 (28%)
+    about one time in 500; the chi^2 bar (mean of 18 squared z scores above
+    2) has a false-alarm rate below 1%."""
     z = compare(runs, ref_runs, "temperature")
     t_a = np.mean([r["temperature"][0] for r in runs])
     t_b = np.mean([r["temperature"][0] for r in ref_runs])
