@@ -36,10 +36,15 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 # relaxed on the GPU by the FORCE_SWITCH_ON ramp under WCA at dt = 0.002
 # (configs.equilibrate); the warm-up and timed steps then run the production
 # potential (rc = 2.5, dt = 0.01).
-# Algorithmic HBM bytes per particle of the force pass: read r (24) and
+# Algorithmic HBM bytes per particle of a force pass alone: read r (24) and
 # v (24) and the type (4), write F (24).
 FORCE_BYTES_PER_PARTICLE = 76
-STEP_BYTES_PER_PARTICLE = 196  # force pass 76 + fused integrate pass 120
+# ... and of a whole step, which in the mixed form is one force pass that also
+# integrates (verlet_velocities of the step, verlet_positions of the next):
+# read r, v (48) and the type (4), write r and v (48).  The acceleration is
+# formed and used in registers.  Everything else it moves (the FP32 view, the
+# list, the skin reference, the rebuild) is this implementation's own.
+STEP_BYTES_PER_PARTICLE = 100
 # Counter figures of the mixed force pass on this workload, per threads per
 # particle (the autotuner's choice), measured by `bench.py --counters` on the
 # state the timed steps leave (profiles/force_pass_melt100k_hbm_traffic.md,
@@ -58,8 +63,10 @@ def log(*a):
 def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     """Warm-up, then the timed region: `steps` steps in one mfa_run_steps call
     (blocks of 16 steps captured in a hipGraph), state resident in HBM.  Then
-    the force pass's average kernel duration at the final state, from a graph
-    of 200 back-to-back launches between two HIP events on the same stream."""
+    the average kernel duration of the force pass at the final state, alone
+    and as the step runs it (in the mixed form fused with the integration),
+    each from a graph of 200 back-to-back launches between two HIP events on
+    the same stream."""
     ctx = lib.Context(p, precision)
     ctx.set_graph_steps(graph_steps)
     ctx.set_state(st0["r"], st0["v"], st0["a"], st0["t"], st0["mic"])
@@ -71,10 +78,13 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     obs = ctx.run_steps(first_step + warmup, steps, want_obs=True)
     dt = time.perf_counter() - t0
     force_ms = ctx.time_force(200)
+    step_force_ms = ctx.time_force_step(200)
     ewald_ms = ctx.time_ewald(200) if p.coulomb else None
     st = ctx.get_state()
+    rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
-    return dt, obs, {"force_ms": force_ms, "tpp": tpp, "ewald_ms": ewald_ms}, st
+    return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp,
+                     "ewald_ms": ewald_ms, "rebuild_ms": rebuild}, st
 
 
 def ewald_k_count(p, inputs):
@@ -178,12 +188,14 @@ def contract_line(args, workload, value, ms_per_step, dtype, data, config, extra
     return line
 
 
-def roofline(n, force_ms, traffic=None, kernel="k_force_mixed<8,0,0>"):
-    force_s = force_ms * 1e-3
-    achieved = FORCE_BYTES_PER_PARTICLE * n / force_s / 1e9 if force_s > 0 else None
+def roofline(n, kernel_ms, traffic=None, kernel="k_force_mixed<8,0,0>",
+             bytes_per_particle=FORCE_BYTES_PER_PARTICLE):
+    s = kernel_ms * 1e-3
+    achieved = bytes_per_particle * n / s / 1e9 if s > 0 else None
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": traffic, "force_ms_avg": force_ms}
+            "traffic": traffic, "kernel_ms_avg": kernel_ms,
+            "algorithmic_bytes_per_particle": bytes_per_particle}
 
 
 def bench_single(args, name):
@@ -204,6 +216,7 @@ def bench_single(args, name):
     log("mixed: %.4f s for %d steps, force pass %.4f ms avg (%d threads per particle), "
         "rebuilds %d, T=%.3f" % (dt_m, args.steps, tm_m["force_ms"], tm_m["tpp"],
                                  obs_m["n_rebuilds"], 2.0 * obs_m["t_kin"] / (3.0 * n)))
+    log("rebuild kernels (ms): %s" % tm_m["rebuild_ms"])
     dt_d, obs_d, tm_d, _ = run_form(p, st0, lib.MFA_FP64, args.steps, args.warmup,
                                     graph_steps=args.graph_steps)
     log("fp64 : %.4f s for %d steps, force %.4f ms avg (%d threads per particle)" % (
@@ -245,10 +258,20 @@ def bench_single(args, name):
         traffic = None
         valu = None
     xt = 1 if (p.coulomb or p.bending) else 0
-    roof = roofline(n, tm_m["force_ms"], traffic,
-                    kernel="k_force_mixed<%d,0,%d>" % (tm_m["tpp"], xt))
+    # the dominant kernel of the mixed step: the force pass fused with the
+    # integration where the step runs it so (no k-space stage), else the
+    # force pass alone
+    fused = not p.coulomb
+    if fused:
+        roof = roofline(n, tm_m["step_force_ms"], None,
+                        kernel="k_force_mixed<%d,2,%d>" % (tm_m["tpp"], xt),
+                        bytes_per_particle=STEP_BYTES_PER_PARTICLE)
+    else:
+        roof = roofline(n, tm_m["force_ms"], traffic,
+                        kernel="k_force_mixed<%d,0,%d>" % (tm_m["tpp"], xt))
+    roof["force_alone_ms_avg"] = tm_m["force_ms"]
     roof["force_ms_avg_fp64"] = tm_d["force_ms"]
-    if valu:
+    if valu and not fused:
         # the kernel is VALU-issue bound: its instruction rate against the issue peak
         g = valu / (tm_m["force_ms"] * 1e-3) / 1e9
         roof["valu_issue"] = {"achieved": g, "peak": VALU_PEAK_GINSTS,
@@ -261,6 +284,12 @@ def bench_single(args, name):
         "force_err_max": err_max,
         "force_err_rms": err_rms,
         "rebuilds": obs_m["n_rebuilds"],
+        # kernel durations from HIP events on the context's stream (the
+        # force pass: 200 launches; a rebuild's three kernels: 20 rebuilds)
+        "kernel_ms": {"force": tm_m["force_ms"], "force_in_step": tm_m["step_force_ms"],
+                      **tm_m["rebuild_ms"],
+                      "rebuild_per_step": sum(tm_m["rebuild_ms"].values()) *
+                      obs_m["n_rebuilds"] / args.steps},
         "step_roofline_frac": value * STEP_BYTES_PER_PARTICLE / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -299,7 +328,7 @@ def bench_batch(args, n_rep=256):
         t1 = time.perf_counter()
         b.run_steps(first + args.warmup, args.steps, want_obs=False)
         dt_b = time.perf_counter() - t1
-        force_ms = b.time_force(50)
+        force_ms = b.time_force_step(50)
     log("batched: %.3f s for %d steps" % (dt_b, args.steps))
     dt_s = 0.0
     for q, (p, r, v, t) in enumerate([] if args.no_seq else reps):
@@ -321,7 +350,8 @@ def bench_batch(args, n_rep=256):
     extra = {
         "sequential_value": ntot * args.steps / dt_s,
         "batched_over_sequential": dt_s / dt_b,
-        "roofline": roofline(ntot, force_ms, None, kernel="k_force_mixed<%d,0,0>" % tpp),
+        "roofline": roofline(ntot, force_ms, None, kernel="k_force_mixed<%d,2,0>" % tpp,
+                             bytes_per_particle=STEP_BYTES_PER_PARTICLE),
         "cpu_baseline": None,
     }
     data = ("synthetic (seeded replicas, fresh starts timed inside the 40-tau switch-on ramp "

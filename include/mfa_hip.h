@@ -121,7 +121,7 @@ typedef struct mfa_params {
   double list_2[MFA_NTYPE][MFA_NTYPE];  /* Verlet-list criterion (binning.f90:101-105) */
   double r_cut_max, r_cut_max_2;
   double r_cut_dpd, r_cut_dpd_2;
-  double skin, skin_2, skin_half_2;     /* skin_half_2 = (skin/2)^2, check_skin.f90:149 */
+  double skin, skin_2, skin_half_2;     /* skin_half_2 = (skin/2)^2, check_skin.f90:18 */
   double friction1, sig;                /* sig = sqrt(2 T gamma / dt), init_params.f90:845 */
   double k_chain, r_chain_2, inv_r_chain_2;
   double a_w, sigma_wall[MFA_NTYPE], a_wall[MFA_NTYPE], z_space_wall;
@@ -235,9 +235,9 @@ int32_t mfa_ewald_plan(const mfa_params* p, int32_t n_charged, int64_t* out, int
  * 2 pi M_z^2 / V in *v_coul. */
 int32_t mfa_dipolar_correction(mfa_ctx* ctx, double* v_coul);
 
-/* verlet_positions (verlet_positions.f90:301-406). */
+/* verlet_positions (verlet_positions.f90:144-250). */
 int32_t mfa_verlet_positions(mfa_ctx* ctx);
-/* check_skin (check_skin.f90:135-155): *f_skin = 1 if a rebuild is due. */
+/* check_skin (check_skin.f90:4-24): *f_skin = 1 if a rebuild is due. */
 int32_t mfa_check_skin(mfa_ctx* ctx, int32_t* f_skin);
 /* binning (binning.f90:11-127): cell list + Verlet list, resets the skin
  * reference (binning.f90:125-127). */
@@ -248,7 +248,7 @@ int32_t mfa_get_cells(mfa_ctx* ctx, int32_t* cell_xyz /* [N][3] */);
 int32_t mfa_get_pairs(mfa_ctx* ctx, int32_t* pairs /* [cap][2] */, int64_t cap, int64_t* n_pairs);
 /* force(:,:) = 0 (md_main.f90:88). */
 int32_t mfa_zero_force(mfa_ctx* ctx);
-/* fluid_fluid (fluid_fluid.f90:52-240) with dpd_forces (dpd_forces.f90:256-352)
+/* fluid_fluid (fluid_fluid.f90:52-240) with dpd_forces (dpd_forces.f90:2-98)
  * and ewald_real (ewald_real.f90:15-32).  The DPD noise of pair (i,j) at
  * step `step` is a counter-based normal keyed by (seed, step, min, max). */
 int32_t mfa_fluid_fluid(mfa_ctx* ctx, int64_t step, double* v_ff, double* v_coul);
@@ -303,14 +303,31 @@ int32_t mfa_stats(mfa_ctx* ctx, int64_t* out /* [4] */);
  * the context's stream.  F is recomputed (same values). */
 int32_t mfa_time_force(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
+/* The force pass as mfa_run_steps runs it, timed like mfa_time_force.  In
+ * the mixed form without the k-space stage or DPD_VV (and with kernel timing
+ * off), the step's force kernel also integrates (verlet_velocities of this
+ * step, verlet_positions and check_skin of the next); it is timed with what
+ * it writes to the velocities, image counts, flags and cell counts sent to
+ * copies, so the state is left as it was.  Otherwise mfa_time_force. */
+int32_t mfa_time_force_step(mfa_ctx* ctx, int32_t reps, double* ms_avg);
+
 /* Average duration (ms) of the k-space Ewald stage of a step (ewald_k(1) and,
  * with walls, the slab dipole correction: ewald_k.f90:150-199,
  * dipolar_correction.f90:20-33) at the current state, timed like
  * mfa_time_force; F is left as it was.  MFA_ERR_ARG without charges. */
 int32_t mfa_time_ewald(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
-/* Device-to-device copy of `bytes` (read `bytes`, write `bytes`) timed with
- * HIP events: the reference point for HBM counters and bandwidth. */
+/* Average durations (ms) of the kernels of a rebuild: ms[0] the cell fill,
+ * ms[1] the cell sort, ms[2] the list build, each bracketed by HIP events on
+ * the context's stream, over `reps` rebuilds forced at the current state.
+ * Measurement helper with no reference counterpart. */
+int32_t mfa_time_rebuild(mfa_ctx* ctx, int32_t reps, double* ms /* [3] */);
+
+/* Measurement only, no reference counterpart and no part of the step: a
+ * device-to-device copy of `bytes` (read `bytes`, write `bytes`) timed with
+ * HIP events, the calibration point for the HBM byte counters
+ * (FETCH_SIZE/WRITE_SIZE) and the bandwidth the bench reports.  It allocates
+ * and frees its own two buffers. */
 int32_t mfa_copy_probe(int64_t bytes, double* ms);
 
 /* Kernel timing of the force pass: average duration (ms) over the launches

@@ -48,10 +48,11 @@ enum {
   T_VFF = 0, T_VFW, T_VINTRA, T_VBEND, T_VCOUL, T_TKIN, T_VIR, T_FZB, T_FZT, T_ZMOM, T_SPARE0, T_SPARE1
 };
 // force-pass term mask
-enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16, TM_BEND = 32 };
+enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16, TM_BEND = 32, TM_FUSE = 64 };
 #define TM_ALL (TM_PAIR | TM_WALL | TM_BOND | TM_BEND)
 // device flag slots
-enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_FILLDONE = 4, F_NFLAGS = 8 };
+enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_FILLDONE = 4, F_LIVE = 5,
+       F_NFLAGS = 8 };
 
 // Fixed-point scale of the mixed form: 2^22 per unit force (resolution
 // 2.4e-7; a pair term converts through int32 up to |F| < 512, through int64
@@ -165,6 +166,16 @@ struct mfa_ctx {
   unsigned* nl;
   int32_t* nn;
   float4 *pos4, *vel4;  // FP32 view of the mixed form
+  // the other buffers of positions and view: a mixed step whose force pass
+  // also integrates reads r, pos4, vel4 and writes these; the host then swaps
+  // the pointers (parity counts the swaps)
+  double* r_alt;
+  float4 *pos4_alt, *vel4_alt;
+  int parity;
+  // set only while mfa_time_force_step times that force pass: copies that
+  // take what it writes to v, mic, the flags and the cell counts
+  double* probe_v = nullptr;
+  int32_t *probe_mic = nullptr, *probe_flags = nullptr, *probe_count = nullptr;
   double *fd, *fd_new;  // DPD_VV: dissipative force of the last new_dpd_fd, and the new one
   bool dpd_vv;
   // Ewald k space (charged systems): the checked host plan and its tables
@@ -187,7 +198,8 @@ struct mfa_ctx {
   // captured step block: graph_steps steps of [rebuild, force, integrate]
   int graph_steps;
   hipGraphExec_t gexec;
-  int gexec_precision, gexec_tpp;
+  int gexec_precision, gexec_tpp, gexec_parity;
+  bool gexec_fused;
   bool capturing;
   int32_t* flags;       // device flags
   int32_t* h_flags;     // pinned mirror
@@ -320,7 +332,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_reduce_replicas(const double* __r
 // ---------------------------------------------------------------------------
 // Integrator kernels (FP64).
 
-// verlet_positions (verlet_positions.f90:301-326, wrap 392-406).
+// verlet_positions (verlet_positions.f90:144-170, wrap 233-250).
 __device__ __forceinline__ void vp_one(const KParams& k, double* r, double* v, const double* a,
                                        int* mic, bool& wrapped) {
   for (int d = 0; d < 3; ++d) {
@@ -341,7 +353,7 @@ __device__ __forceinline__ void vp_one(const KParams& k, double* r, double* v, c
   }
 }
 
-// check_skin for one particle (check_skin.f90:139-152).
+// check_skin for one particle (check_skin.f90:8-21).
 __device__ __forceinline__ bool skin_one(const KParams& k, const double* r, const double* rref,
                                          const int* mic) {
   double s = 0.0;
@@ -590,14 +602,25 @@ __global__ void k_set_flag(int* __restrict__ flags, int n_rep, int idx, int valu
 // order of the last rebuild (s -> perm[s]), so a wave's particles come in
 // runs of equal cells: one atomic per run reserves its slots.
 #define FILL_GROUP 16  // blocks per first-level completion counter
+#define FILL_KEEP 32   // cells per thread whose counts the scan keeps in registers
 __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
     const int* __restrict__ perm, int* __restrict__ cell_of, int* __restrict__ cell_count,
     int* __restrict__ cell_slots, int* __restrict__ cell_start, int* __restrict__ flags_all,
-    int* __restrict__ fill_cnt) {
+    int* __restrict__ fill_cnt, long long* __restrict__ d_step, int inc_step) {
+  // a step whose force pass integrates as well moves the step counter on
+  // here, ahead of that force pass (no kernel of the rebuild reads it)
+  if (inc_step && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
   MFA_REPLICA(blockIdx.x * blockDim.x);
   int* flags = flags_all + k.rep * F_NFLAGS;
-  if (flags[F_REBUILD] == 0) return;  // uniform over the replica and the launch
+  // F_REBUILD (raised by the skin checks) is handed on here: the replica's
+  // last block below moves it to F_LIVE, which the sort and list kernels
+  // read, and clears it; without a rebuild, the replica's first block clears
+  // F_LIVE.  So no kernel after the fill needs a completion count of its own.
+  if (flags[F_REBUILD] == 0) {  // uniform over the replica and the launch
+    if (threadIdx.x == 0 && (int)blockIdx.x == k.p0 / MFA_BLOCK) flags[F_LIVE] = 0;
+    return;
+  }
   const bool ok = flags[F_ERROR] == 0;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -658,16 +681,37 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   }
   __syncthreads();
   if (!last_s) return;
-  if (threadIdx.x == 0) atomicExch(&flags[F_FILLDONE], 0);
+  if (threadIdx.x == 0) {  // every block of the replica has read F_REBUILD by now
+    atomicExch(&flags[F_FILLDONE], 0);
+    atomicExch(&flags[F_LIVE], 1);
+    atomicExch(&flags[F_REBUILD], 0);
+  }
   // one pass: thread t sums the counts of cells [t per, (t+1) per), a block
-  // scan of the 256 sums gives each thread its offset, then its cells' starts
+  // scan of the 256 sums gives each thread its offset, then its cells' starts.
+  // Up to FILL_KEEP cells per thread the counts are read once, all loads in
+  // flight together, and kept in registers for the second pass.
   const int per = (k.n_cells + MFA_BLOCK - 1) / MFA_BLOCK;
   const int cb = threadIdx.x * per, ce = min(k.n_cells, cb + per);
   int tot = 0;
+  int kept[FILL_KEEP];
+  const bool keep = per <= FILL_KEEP;
+  if (keep) {
+#pragma unroll
+    for (int u = 0; u < FILL_KEEP; ++u)
+      kept[u] = cb + u < ce ? __hip_atomic_load(&cell_count[k.c0 + cb + u], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : 0;
+#pragma unroll
+    for (int u = 0; u < FILL_KEEP; ++u) {
+      kept[u] = min(kept[u], k.cell_cap);
+      tot += kept[u];
+    }
+  } else {
 #pragma unroll 8
-  for (int c = cb; c < ce; ++c)
-    tot += min(__hip_atomic_load(&cell_count[k.c0 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-               k.cell_cap);
+    for (int c = cb; c < ce; ++c)
+      tot += min(__hip_atomic_load(&cell_count[k.c0 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                 k.cell_cap);
+  }
   int x = tot;  // inclusive scan of the thread sums within the wave
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(x, o, 64);
@@ -677,6 +721,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
   __syncthreads();
   int run = carry_s + x - tot;
   for (int w = 0; w < wid; ++w) run += wsum[w];
+  if (keep) {
+#pragma unroll
+    for (int u = 0; u < FILL_KEEP; ++u)
+      if (cb + u < ce) {
+        cell_start[k.c0 + cb + u] = run;
+        run += kept[u];
+      }
+    return;
+  }
 #pragma unroll 8
   for (int c = cb; c < ce; ++c) {
     cell_start[k.c0 + c] = run;
@@ -708,7 +761,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int cg = blockIdx.x * 4 + w;
   const KParams& k = kb[cell_rep[cg < n_cells ? cg : 0]];
-  const bool on = cg < n_cells && flags_all[k.rep * F_NFLAGS + F_REBUILD] != 0 &&
+  const bool on = cg < n_cells && flags_all[k.rep * F_NFLAGS + F_LIVE] != 0 &&
                   flags_all[k.rep * F_NFLAGS + F_ERROR] == 0;
   int* ids = sort_ids + w * k.cell_cap;
   const int m = on ? min(cell_count[cg], k.cell_cap) : 0;
@@ -738,15 +791,17 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 // r^2 < list_2(ti,tj) (binning.f90:64-113, both directions stored).
 // One workgroup per cell, four waves; wave w takes the cell's particles in
 // octants 2w and 2w+1 of the cell (two groups; a particle's row length lives
-// in the lane of its place in the wave's list).  Each wave walks the
-// candidates of the 27 cells 64 at a time, in offset-code order
-// 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in ascending id (the order of a
-// sequential scan), fetching the next 64 while it works on the current ones.
-// A candidate farther than the list radius from the bounding box of a
-// group's particles cannot be a neighbour of any of them; the others are
-// queued, in order, for that group.  Each time 64 are queued, the group's
-// particles are tested in turn against all 64 at once and a ballot appends
-// the hits to the particle's row in lane order, so every row holds its
+// in the lane of its place in the wave's list).  The candidates of the 27
+// cells, in offset-code order 9(dx+1)+3(dy+1)+(dz+1) and, within a cell, in
+// ascending id (the order of a sequential scan), are staged once per
+// workgroup in LDS (position relative to this cell's origin, entry), in
+// pieces of at most LB_PIECE; every wave then walks them 64 at a time.  A
+// candidate farther than the list radius from the bounding box of a group's
+// particles cannot be a neighbour of any of them; the others are queued (as
+// indices into the staged piece), in order, for that group.  Each time 64
+// are queued, and at the end of each piece, the group's particles are tested
+// in turn against the queued candidates at once and a ballot appends the
+// hits to the particle's row in lane order, so every row holds its
 // neighbours in scan order.  Separations come from the FP32 view (both
 // positions relative to their cell origins, plus the offset of the
 // candidate's cell): no periodic arithmetic.  A separation whose r^2 lies
@@ -761,14 +816,16 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 #define LB_MAX_CELL (64 * LB_WAVES)  // particles per cell the build takes
 #define LB_RING 128                  // queued candidates per group (a block of 64 adds at most 64)
 #define LB_MIN_OCTANT 16             // particles in a cell for the octant groups
+#define LB_PIECE 1024                // candidates staged in LDS at a time
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
     const int* __restrict__ cell_start, const int* __restrict__ perm,
     const float4* __restrict__ pos4, unsigned* __restrict__ nl, int* __restrict__ nn, int cap,
-    int* __restrict__ flags_all, int octants) {
-  // dynamic LDS: the cell's own particles [cell_cap]; with octants, then per
-  // wave and group the queued candidates (view, .w type bits) and entries
+    int* __restrict__ flags_all, int octants, int piece) {
+  // dynamic LDS: the cell's own particles [cell_cap]; the staged candidates
+  // (view relative to this cell, .w type bits) [piece] and their entries
+  // [piece]; with octants, per wave and group the queued candidate indices
   extern __shared__ float4 lds_lb[];
   __shared__ int q_first[28], q_start[27];
   __shared__ float4 q_off[27];
@@ -780,14 +837,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   const int cg = blockIdx.x;
   const KParams& k = kb[cell_rep[cg]];
   int* flags = flags_all + k.rep * F_NFLAGS;
-  if (flags[F_REBUILD] == 0 || flags[F_ERROR] != 0) return;  // uniform over the workgroup
+  if (flags[F_LIVE] == 0 || flags[F_ERROR] != 0) return;  // uniform over the workgroup
   const int c = cg - k.c0;
   const int cx = c % k.nb[0], cy = (c / k.nb[0]) % k.nb[1], cz = c / (k.nb[0] * k.nb[1]);
   const int m = min(cell_count[cg], k.cell_cap);
   const int hs0 = cell_start[cg];
   float4* hpos = lds_lb;
-  float4* qpos_w = lds_lb + k.cell_cap + w * 2 * LB_RING;  // [2][LB_RING] of this wave
-  unsigned* qent_w = (unsigned*)(lds_lb + k.cell_cap + LB_WAVES * 2 * LB_RING) + w * 2 * LB_RING;
+  float4* cpos = lds_lb + k.cell_cap;
+  unsigned* cent = (unsigned*)(cpos + piece);
+  int* qidx_w = (int*)(cent + piece) + w * 2 * LB_RING;  // [2][LB_RING] of this wave
   if (threadIdx.x < 27) {
     const int t = threadIdx.x;
     const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
@@ -857,7 +915,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     g_at[0] = 0;
     g_at[1] = 32;
   }
-  if (g_n[0] + g_n[1] == 0) return;
+  // a wave without particles still stages candidates and meets the barriers
+  const bool busy = g_n[0] + g_n[1] > 0;
   const int total = q_first[27];
   const float margin = k.lf_margin;
   float l2max = 0.0f;
@@ -884,7 +943,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     }
   }
   int hc = 0;  // row length of the wave's particle in this lane's place
-  int head[2] = {0, 0}, tail[2] = {0, 0};  // wave-uniform queue positions
 
   // Tests 64 candidates (one per lane; view position relative to this cell,
   // entry) against each particle of group g and appends the hits.
@@ -894,18 +952,21 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const float pjx = cp.x + far, pjy = cp.y, pjz = cp.z;
     const int tj = __float_as_int(cp.w) & 3;
     for (int t = 0; t < g_n[g]; ++t) {
-      const int h = crowded ? w + LB_WAVES * t : hlist[w][g_at[g] + t];
+      const int h = __builtin_amdgcn_readfirstlane(crowded ? w + LB_WAVES * t : hlist[w][g_at[g] + t]);
       const int slot = (g == 0 ? 0 : g_n[0]) + t;
       const float4 pi = hpos[h];
       const int ti = __float_as_int(pi.w) & 3;
       const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
       const float d2 = dx * dx + dy * dy + dz * dz;
       const float l2 = l2tab[ti * MFA_NTYPE + tj];
-      bool hit = false;
-      if (live && js_c != hs0 + h) {
-        if (d2 < l2 - margin) {
-          hit = true;
-        } else if (d2 < l2 + margin) {  // rare: decide in FP64
+      const bool cand = live && js_c != hs0 + h;
+      bool hit = cand && d2 < l2 - margin;
+      const bool band = cand && !(d2 < l2 - margin) && d2 < l2 + margin;
+      // rare: a separation within the view's rounding bound of the radius is
+      // decided in FP64 (one wave-uniform test keeps the common path free of
+      // nested divergence)
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0ull, 0)) {
+        if (band) {
           const int i = perm[hs0 + h], j = perm[js_c];
           double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
           double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
@@ -927,70 +988,76 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
       }
     }
   };
-  // The queued candidates [head, head + cnt) of group g (cnt <= 64).
+  // The queued candidates [hd, hd + cnt) of group g (cnt <= 64): indices
+  // into the staged piece (a lane past cnt reads index 0 and is not live).
   auto drain = [&](int g, int hd, int cnt) {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    const int e = (hd + lane) & (LB_RING - 1);
-    test_block(g, qpos_w[g * LB_RING + e], qent_w[g * LB_RING + e], lane < cnt);
+    const int e = lane < cnt ? qidx_w[g * LB_RING + ((hd + lane) & (LB_RING - 1))] : 0;
+    test_block(g, cpos[e], cent[e], lane < cnt);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
   };
 
-  // candidate e = e0 + lane: cell code q (never decreases along e), sorted
-  // position js; a lane past the end takes the cell's first particle and is
-  // never queued
-  int q = 0;
-  bool live = lane < total;
-  if (live)
-    while (q_first[q + 1] <= lane) ++q;
-  int js = live ? q_start[q] + (lane - q_first[q]) : hs0;
-  float4 pj = pos4[js];
-  for (int e0 = 0; e0 < total; e0 += 64) {
-    const bool live_c = live;
-    const int js_c = js, q_c = q;
-    const float4 pj_c = pj;
-    {  // fetch the next 64 candidates
-      const int e = e0 + 64 + lane;
-      live = e < total;
-      if (live)
-        while (q_first[q + 1] <= e) ++q;
-      js = live ? q_start[q] + (e - q_first[q]) : hs0;
-      if (e0 + 64 < total) pj = pos4[js];
+  for (int p0 = 0; p0 < total; p0 += piece) {  // uniform over the workgroup
+    const int np = min(piece, total - p0);
+    // stage candidates [p0, p0 + np): code q of candidate e is the last q with
+    // q_first[q] <= e (binary search), its sorted position follows
+    for (int t = threadIdx.x; t < np; t += MFA_BLOCK) {
+      const int e = p0 + t;
+      int lo = 0, hi = 27;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (q_first[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      const int js = q_start[lo] + (e - q_first[lo]);
+      const float4 pj = pos4[js];
+      const float4 o = q_off[lo];
+      cpos[t] = make_float4(pj.x - o.x, pj.y - o.y, pj.z - o.z, pj.w);
+      cent[t] = (unsigned)js | ((unsigned)lo << MFA_JBITS);
     }
-    const float4 o = q_off[q_c];
-    // the candidate relative to this cell's origin
-    const float4 cp = make_float4(pj_c.x - o.x, pj_c.y - o.y, pj_c.z - o.z, pj_c.w);
-    const unsigned entry = (unsigned)js_c | ((unsigned)q_c << MFA_JBITS);
-    if (crowded) {  // one group, every candidate
-      test_block(0, cp, entry, live_c);
-      continue;
-    }
+    __syncthreads();
+    if (busy) {
+      int head[2] = {0, 0}, tail[2] = {0, 0};  // wave-uniform queue positions
+      for (int e0 = 0; e0 < np; e0 += 64) {
+        const int e = e0 + lane;
+        const bool live = e < np;
+        const int ec = live ? e : 0;
+        const float4 cp = cpos[ec];
+        if (crowded) {  // one group, every candidate
+          test_block(0, cp, cent[ec], live);
+          continue;
+        }
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      if (g_n[g] == 0) continue;
-      const float bx = fmaxf(fmaxf(bmin[g][0] - cp.x, cp.x - bmax[g][0]), 0.0f);
-      const float by = fmaxf(fmaxf(bmin[g][1] - cp.y, cp.y - bmax[g][1]), 0.0f);
-      const float bz = fmaxf(fmaxf(bmin[g][2] - cp.z, cp.z - bmax[g][2]), 0.0f);
-      const bool pass = live_c && bx * bx + by * by + bz * bz < reach2;
-      const unsigned long long pm = __builtin_amdgcn_ballot_w64(pass);
-      if (pass) {
-        const int at = (tail[g] + (int)__builtin_amdgcn_mbcnt_hi(
-                                      (unsigned)(pm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u))) &
-                       (LB_RING - 1);
-        qpos_w[g * LB_RING + at] = cp;
-        qent_w[g * LB_RING + at] = entry;
+        for (int g = 0; g < 2; ++g) {
+          if (g_n[g] == 0) continue;
+          const float bx = fmaxf(fmaxf(bmin[g][0] - cp.x, cp.x - bmax[g][0]), 0.0f);
+          const float by = fmaxf(fmaxf(bmin[g][1] - cp.y, cp.y - bmax[g][1]), 0.0f);
+          const float bz = fmaxf(fmaxf(bmin[g][2] - cp.z, cp.z - bmax[g][2]), 0.0f);
+          const bool pass = live && bx * bx + by * by + bz * bz < reach2;
+          const unsigned long long pm = __builtin_amdgcn_ballot_w64(pass);
+          if (pass) {
+            const int at = (tail[g] + (int)__builtin_amdgcn_mbcnt_hi(
+                                          (unsigned)(pm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u))) &
+                           (LB_RING - 1);
+            qidx_w[g * LB_RING + at] = e;
+          }
+          tail[g] += (int)__popcll(pm);
+          if (tail[g] - head[g] >= 64) {
+            drain(g, head[g], 64);
+            head[g] += 64;
+          }
+        }
       }
-      tail[g] += (int)__popcll(pm);
-      if (tail[g] - head[g] >= 64) {
-        drain(g, head[g], 64);
-        head[g] += 64;
-      }
+      // the queues refer to this piece: test what is left before it goes
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+        if (tail[g] > head[g]) drain(g, head[g], tail[g] - head[g]);
     }
+    __syncthreads();  // every wave is done with the piece
   }
-#pragma unroll
-  for (int g = 0; g < 2; ++g)
-    if (tail[g] > head[g]) drain(g, head[g], tail[g] - head[g]);
+  if (!busy) return;
   const int nw = g_n[0] + g_n[1];
   int my_max = 0;
   if (lane < nw) {
@@ -1015,7 +1082,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
 // here (pairs, walls, bonds), so no atomics are needed and F is written once.
 //
 // Plain FP64 (k_force64): the reference's arithmetic for LJ
-// (fluid_fluid.f90:94-126), DPD (dpd_forces.f90:273-348), 9-3 walls
+// (fluid_fluid.f90:94-126), DPD (dpd_forces.f90:19-94), 9-3 walls
 // (fluid_wall.f90:81-121) and FENE (intra_molec.f90:16-139), summed in FP64.
 //
 // Mixed (k_force_mixed): separations from the FP32 cell-relative view plus
@@ -1220,7 +1287,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   const int ss = active ? sp : k.p0;
   const int i = perm[ss];     // the particle (reference order)
   const int il = i - k.p0;    // within the replica (chains, heads, random keys)
-  if (sp == k.p0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const double r2min = switch_on_r2min(k, step);
   const int ii = i;
@@ -1408,16 +1474,41 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
 }
 
+// What the mixed force pass of a step that also integrates reads and writes
+// besides r, the view and the list: the next positions (the other position
+// buffer), velocities and wrap counts (in the step: in place, v_out = v_in,
+// mic_out = mic_in), the next FP32 view (the other view buffer), the skin
+// test's rebuild flags (one F_NFLAGS row per replica) and zeroed cell counts.
+struct FuseArgs {
+  double* r_out;
+  const double* v_in;
+  double* v_out;
+  const int* mic_in;
+  int* mic_out;
+  const double* r_ref;
+  const int* cell_of;
+  float4* pos4_out;
+  float4* vel4_out;
+  int* cell_count;
+  int* rebuild_flags;
+  int nc_tot;
+};
+
 // XT bits: 1 Coulomb real space and bending, 2 the Langevin force, 4 DPD_CUT_OFF
-// (a DPD range of its own, heads excluded from DPD).
-template <int TPP, int ENERGY, int XT>
+// (a DPD range of its own, heads excluded from DPD).  MODE: 0 forces, 1 forces
+// and energy partials, 2 forces, then verlet_velocities of this step and
+// verlet_positions and check_skin of the next (k_fused_integrate's work) in
+// the same pass, which leaves F unwritten.
+template <int TPP, int MODE, int XT>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
     const float4* __restrict__ pos4, const float4* __restrict__ vel4, const int* __restrict__ perm,
     const unsigned* __restrict__ nl, const int* __restrict__ nn, int cap, double* __restrict__ f,
     const long long* __restrict__ d_step, int mask, double* __restrict__ part,
-    int* __restrict__ flags_all) {
+    int* __restrict__ flags_all, FuseArgs fu) {
 #pragma clang fp contract(fast)
+  constexpr bool ENERGY = MODE == 1;
+  constexpr bool FUSE = MODE == 2;
   constexpr bool DC = (XT & 4) != 0;
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   int* flags = flags_all + k.rep * F_NFLAGS;
@@ -1448,7 +1539,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const int ss = active ? sp : k.p0;
   const int i = perm[ss];     // the particle (reference order)
   const int il = i - k.p0;    // within the replica (chains, heads, random keys)
-  if (sp == k.p0 && sub == 0) flags[F_REBUILD] = 0;  // every rebuild kernel of this step has run
   const int64_t step = *d_step;
   const float r2min = (float)switch_on_r2min(k, step);
   const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd = (float)(1.0 / sqrt(k.r_cut_dpd_2));
@@ -1617,7 +1707,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   double fx = group_sum<TPP>(ax) * MFA_FIX_INV;
   double fy = group_sum<TPP>(ay) * MFA_FIX_INV;
   double fz = group_sum<TPP>(az) * MFA_FIX_INV;
-  if (active && sub == 0) {
+  if (active && sub == 0 && !FUSE) {
     if (mask & TM_ACCUM) {
       fx = f[3 * (size_t)i + 0] + fx;
       fy = f[3 * (size_t)i + 1] + fy;
@@ -1643,6 +1733,38 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       if (sub == 0 && ti == 1) vals[T_ZMOM] = zi;
     }
     block_partials(vals, part);
+  }
+  if (FUSE) {
+    // verlet_velocities of this step with F_i (k_fused_integrate's order:
+    // a = F/m, v += dt/2 a, pinned heads), then verlet_positions and
+    // check_skin of the next.  Positions are read from r (this step's
+    // buffer, which the whole pass reads) and written to fu.r_out; only this
+    // lane reads or writes particle i's v and mic.
+    bool over = false;
+    if (active && sub == 0) {
+      double fi[3] = {fx, fy, fz}, vi[3], ai[3], ri[3], rr[3];
+      int mi[3];
+      for (int d = 0; d < 3; ++d) {
+        vi[d] = fu.v_in[3 * (size_t)i + d];
+        ri[d] = r[3 * (size_t)i + d];
+        mi[d] = fu.mic_in[3 * (size_t)i + d];
+        rr[d] = fu.r_ref[3 * (size_t)i + d];
+      }
+      vv_one(k, il, ti, fi, vi, ai);
+      bool wrapped = false;
+      vp_one(k, ri, vi, ai, mi, wrapped);
+      for (int d = 0; d < 3; ++d) {
+        fu.r_out[3 * (size_t)i + d] = ri[d];
+        fu.v_out[3 * (size_t)i + d] = vi[d];
+      }
+      if (wrapped)
+        for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
+      write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
+      over = skin_one(k, ri, rr, mi);
+    }
+    if (__builtin_amdgcn_ballot_w64(over) != 0ull && (threadIdx.x & 63) == 0)
+      atomicOr(&fu.rebuild_flags[k.rep * F_NFLAGS + F_REBUILD], 1);
+    zero_counts(fu.nc_tot, fu.cell_count);
   }
 }
 
@@ -2206,8 +2328,8 @@ static KParams make_kparams(const mfa_params& p) {
   }
   k.dt = p.dt;
   k.dt_2 = p.dt_2;
-  k.half_dt = 0.5 * p.dt;      // 0.5*dt   (verlet_positions.f90:324)
-  k.half_dt_2 = 0.5 * p.dt_2;  // 0.5*dt_2 (verlet_positions.f90:309)
+  k.half_dt = 0.5 * p.dt;      // 0.5*dt   (verlet_positions.f90:169)
+  k.half_dt_2 = 0.5 * p.dt_2;  // 0.5*dt_2 (verlet_positions.f90:154)
   for (int t = 0; t < MFA_NTYPE; ++t) {
     k.inv_mass_type[t] = p.inv_mass_type[t];
     k.mass_type[t] = p.mass_type[t];
@@ -2300,30 +2422,45 @@ static int launch_ewald(mfa_ctx* c, int add_dip);
 
 
 
-// Dynamic LDS of the list build: the cell's particles, and with the octant
-// groups the candidate queues (per wave and group, LB_RING float4 + entry).
+// Candidates the list build stages in LDS at a time: all 27 cells' worth of
+// a full cell capacity, at most LB_PIECE.
+static int list_piece(const mfa_ctx* c) {
+  const int all = 27 * c->cell_cap;
+  return all < LB_PIECE ? all : LB_PIECE;
+}
+
+// Dynamic LDS of the list build: the cell's particles, the staged candidates
+// (float4 + entry each) and the queues of candidate indices (per wave and
+// group, LB_RING).
 static size_t list_lds_bytes(const mfa_ctx* c) {
-  size_t b = (size_t)c->cell_cap * sizeof(float4);
-  if (c->lb_octants) b += (size_t)LB_WAVES * 2 * LB_RING * (sizeof(float4) + sizeof(unsigned));
-  return b;
+  return (size_t)c->cell_cap * sizeof(float4) +
+         (size_t)list_piece(c) * (sizeof(float4) + sizeof(unsigned)) +
+         (size_t)LB_WAVES * 2 * LB_RING * sizeof(int);
 }
 
 // A rebuild: cells of every particle, their sorted ranges, the spatial order
 // and FP32 view, then the lists.  Each kernel returns at once unless the
 // replica's rebuild flag is set.
-static int launch_rebuild(mfa_ctx* c) {
+// With ev (four events), the kernels are bracketed by them for timing.
+// inc_step: the step counter moves on at the start of the rebuild (the steps
+// whose force pass also integrates, which therefore cannot).
+static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0) {
   const int g = grid_of(c->n);
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   hipLaunchKernelGGL(k_cell_fill, dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp, c->d_rmap, c->r,
                      c->perm, c->cell_of, c->cell_count, c->cell_slots, c->cell_start, c->flags,
-                     c->fill_cnt);
+                     c->fill_cnt, c->d_step, inc_step);
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   const int cb = (c->n_cells + 3) / 4;  // four cells (waves) per workgroup
   hipLaunchKernelGGL(k_cell_sort, dim3(cb), dim3(MFA_BLOCK), (size_t)4 * c->cell_cap * sizeof(int),
                      c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->v, c->type, c->mic,
                      c->cell_count, c->cell_slots, c->cell_start, c->perm, c->sidx, c->r_ref,
                      c->flags, c->pos4, c->vel4);
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
                      c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm, c->pos4,
-                     c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0);
+                     c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
+  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -2332,14 +2469,34 @@ template <int TPP, int XT>
 static void launch_force_tx(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED) {
-    if (mask & TM_ENERGY)
+    FuseArgs fu;
+    memset(&fu, 0, sizeof fu);
+    if (mask & TM_FUSE) {
+      fu.r_out = c->r_alt;
+      fu.v_in = c->v;
+      fu.mic_in = c->mic;
+      // outputs as the step writes them, or the timing copies
+      fu.v_out = c->probe_v ? c->probe_v : c->v;
+      fu.mic_out = c->probe_mic ? c->probe_mic : c->mic;
+      fu.rebuild_flags = c->probe_flags ? c->probe_flags : c->flags;
+      fu.r_ref = c->r_ref;
+      fu.cell_of = c->cell_of;
+      fu.pos4_out = c->pos4_alt;
+      fu.vel4_out = c->vel4_alt;
+      fu.cell_count = c->probe_count ? c->probe_count : c->cell_count;
+      fu.nc_tot = c->n_cells;
+      hipLaunchKernelGGL((k_force_mixed<TPP, 2, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                         c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
+                         c->d_step, mask, c->part, c->flags, fu);
+    } else if (mask & TM_ENERGY) {
       hipLaunchKernelGGL((k_force_mixed<TPP, 1, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
-                         c->d_step, mask, c->part, c->flags);
-    else
+                         c->d_step, mask, c->part, c->flags, fu);
+    } else {
       hipLaunchKernelGGL((k_force_mixed<TPP, 0, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
-                         c->d_step, mask, c->part, c->flags);
+                         c->d_step, mask, c->part, c->flags, fu);
+    }
   } else {
     hipLaunchKernelGGL((k_force64<TPP, XT & 3>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
                        c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
@@ -2553,7 +2710,35 @@ static int launch_dpd_vv(mfa_ctx* c, int energy, int inc_step) {
 // One step after its verlet_positions: rebuild when due, force pass, and
 // verlet_velocities fused with the next step's verlet_positions (with
 // DPD_VV: verlet_velocities, new_dpd_fd, then the next verlet_positions).
+static bool ewald_active(const mfa_ctx* c);
+
+// Swaps the position and view buffers after a step whose force pass wrote
+// the other ones.
+static void swap_buffers(mfa_ctx* c) {
+  std::swap(c->r, c->r_alt);
+  std::swap(c->pos4, c->pos4_alt);
+  std::swap(c->vel4, c->vel4_alt);
+  c->parity ^= 1;
+}
+
+// Whether the middle steps of mfa_run_steps integrate inside the force pass:
+// the mixed form without the k-space Ewald stage (which adds to F after the
+// pair pass) or DPD_VV, and not while kernels are timed one by one.
+static bool fused_ok(const mfa_ctx* c) {
+  return c->precision == MFA_MIXED && !c->dpd_vv && !ewald_active(c) && !c->timing;
+}
+
 static int enqueue_middle_step(mfa_ctx* c) {
+  if (fused_ok(c)) {
+    // rebuild when due (the step counter moves on there), then the force
+    // pass with verlet_velocities and the next verlet_positions in it
+    int rc = launch_rebuild(c, nullptr, 1);
+    if (rc) return rc;
+    rc = launch_force(c, TM_ALL | TM_FUSE);
+    if (rc) return rc;
+    swap_buffers(c);
+    return MFA_OK;
+  }
   int rc = launch_rebuild(c);
   if (rc) return rc;
   rc = launch_force(c, TM_ALL);
@@ -2575,14 +2760,20 @@ static int enqueue_middle_step(mfa_ctx* c) {
 
 // Captures graph_steps middle steps once per (precision, threads per particle).
 static int ensure_graph(mfa_ctx* c) {
-  if (c->gexec && c->gexec_precision == c->precision && c->gexec_tpp == c->tpp) return MFA_OK;
+  const bool fused = fused_ok(c);
+  if (c->gexec && c->gexec_precision == c->precision && c->gexec_tpp == c->tpp &&
+      c->gexec_fused == fused && (!fused || c->gexec_parity == c->parity))
+    return MFA_OK;
   drop_graph(c);
   hipGraph_t g = nullptr;
+  const int parity0 = c->parity;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   c->capturing = true;
   int rc = MFA_OK;
   for (int s = 0; s < c->graph_steps && rc == MFA_OK; ++s) rc = enqueue_middle_step(c);
   c->capturing = false;
+  // capturing ran nothing: the buffers are still those of parity0
+  if (c->parity != parity0) swap_buffers(c);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (g) hipGraphDestroy(g);
@@ -2597,6 +2788,8 @@ static int ensure_graph(mfa_ctx* c) {
   }
   c->gexec_precision = c->precision;
   c->gexec_tpp = c->tpp;
+  c->gexec_fused = fused;
+  c->gexec_parity = parity0;
   return MFA_OK;
 }
 
@@ -3056,6 +3249,11 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
       return set_err(nullptr, MFA_ERR_ARG, "binning cells too crowded for the list build (capacity %d)",
                      cell_cap);
     }
+    if (list_lds_bytes(c) > c->lds_limit) {
+      const size_t b = list_lds_bytes(c);
+      delete c;
+      return set_err(nullptr, MFA_ERR_ARG, "list build needs %zu bytes of LDS", b);
+    }
   }
   const size_t n = (size_t)c->n;
   const size_t ncs = (size_t)c->n_cells * c->cell_cap;
@@ -3090,6 +3288,9 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   ALLOC(c->nn, n * sizeof(int32_t));
   ALLOC(c->pos4, n * sizeof(float4));
   ALLOC(c->vel4, n * sizeof(float4));
+  ALLOC(c->r_alt, n * 3 * sizeof(double));
+  ALLOC(c->pos4_alt, n * sizeof(float4));
+  ALLOC(c->vel4_alt, n * sizeof(float4));
   ALLOC(c->flags, (size_t)n_rep * F_NFLAGS * sizeof(int32_t));
   ALLOC(c->d_step, sizeof(long long));
   ALLOC(c->part, (size_t)c->max_blocks * MFA_NTERMS * sizeof(double));
@@ -3140,6 +3341,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   std::vector<int32_t> ones(n, 1);
   hipMemcpyAsync(c->type, ones.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
   hipMemsetAsync(c->r, 0, n * 3 * sizeof(double), c->stream);
+  hipMemsetAsync(c->r_alt, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->v, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->r_ref, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
@@ -3190,7 +3392,7 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
                   c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx,
                   c->fd,   c->fd_new, c->fill_cnt, c->head_f, c->d_hist, c->d_chain_rg,
-                  c->d_obs};
+                  c->d_obs, c->r_alt, c->pos4_alt, c->vel4_alt};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->d_rows) hipFree(c->d_rows);
@@ -3627,7 +3829,10 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
                      c->n_rep, (int)F_NREBUILD, 0);
-  rc = set_step(c, first_step);
+  // with the integration inside the force pass, each step's rebuild moves
+  // the step counter on before that step's force pass
+  const bool fused = fused_ok(c);
+  rc = set_step(c, fused ? first_step - 1 : first_step);
   if (rc) return rc;
   const int g = grid_of(c->n);
   // first half of the first step (verlet_positions + check_skin)
@@ -3637,7 +3842,10 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
   HIPCHK(c, hipGetLastError());
   // steps 1 .. n-1: captured blocks of graph_steps steps, then single steps
   int middle = n - 1;
-  if (c->graph_steps > 0 && !c->timing && middle >= c->graph_steps) {
+  // a captured block of fused steps must leave the buffers where it found
+  // them: an even number of steps
+  const bool graphs = c->graph_steps > 0 && !c->timing && !(fused && (c->graph_steps & 1));
+  if (graphs && middle >= c->graph_steps) {
     rc = ensure_graph(c);
     if (rc) return rc;
     while (middle >= c->graph_steps) {
@@ -3650,7 +3858,7 @@ extern "C" int32_t mfa_run_steps(mfa_ctx* c, int64_t first_step, int32_t n, mfa_
     if (rc) return rc;
   }
   // last step: force pass with the energies, verlet_velocities alone
-  rc = launch_rebuild(c);
+  rc = launch_rebuild(c, nullptr, fused ? 1 : 0);
   if (rc) return rc;
   rc = launch_force(c, TM_ALL | (obs ? TM_ENERGY : 0));
   if (rc) return rc;
@@ -4021,6 +4229,90 @@ extern "C" int32_t mfa_time_force(mfa_ctx* c, int32_t reps, double* ms_avg) {
   if (rc) return rc;
   rc = time_launches(c, reps, launch_force_all, ms_avg);
   return rc ? rc : check_flags(c);
+}
+
+// The force pass as the step runs it (see the header): in the fused form
+// its kernel also integrates, and is timed with its writes to v, mic, the
+// flags and the cell counts sent to copies, so the state stays as it was.
+static int launch_force_fused(mfa_ctx* c) { return launch_force(c, TM_ALL | TM_FUSE); }
+
+extern "C" int32_t mfa_time_force_step(mfa_ctx* c, int32_t reps, double* ms_avg) {
+  if (!c || !ms_avg || reps < 1 || reps > 4096) return MFA_ERR_ARG;
+  if (!fused_ok(c)) return mfa_time_force(c, reps, ms_avg);
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  const size_t n3 = 3 * (size_t)c->n, nf = (size_t)c->n_rep * F_NFLAGS;
+  double* pv = nullptr;
+  int32_t *pm = nullptr, *pf = nullptr, *pc = nullptr;
+  if (hipMalloc(&pv, n3 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&pm, n3 * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&pf, nf * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&pc, (size_t)c->n_cells * sizeof(int32_t)) != hipSuccess) {
+    hipFree(pv);
+    hipFree(pm);
+    hipFree(pf);
+    hipFree(pc);
+    return set_err(c, MFA_ERR_NOMEM, "hipMalloc of the timing copies failed");
+  }
+  if (hipMemcpyAsync(pf, c->flags, nf * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream) !=
+      hipSuccess)
+    rc = set_err(c, MFA_ERR_HIP, "flag copy failed");
+  if (!rc) {
+    c->probe_v = pv;
+    c->probe_mic = pm;
+    c->probe_flags = pf;
+    c->probe_count = pc;
+    rc = time_launches(c, reps, launch_force_fused, ms_avg);
+    c->probe_v = nullptr;
+    c->probe_mic = nullptr;
+    c->probe_flags = nullptr;
+    c->probe_count = nullptr;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, MFA_ERR_HIP, "sync failed");
+  hipFree(pv);
+  hipFree(pm);
+  hipFree(pf);
+  hipFree(pc);
+  return rc ? rc : check_flags(c);
+}
+
+// Average durations (ms) of the three kernels of a rebuild (cell fill, cell
+// sort, list build), each bracketed by HIP events on the context's stream,
+// over `reps` rebuilds forced at the current state (which they leave as it
+// is, the skin reference included).
+extern "C" int32_t mfa_time_rebuild(mfa_ctx* c, int32_t reps, double* ms) {
+  if (!c || !ms || reps < 1 || reps > 1024) return MFA_ERR_ARG;
+  int rc = ensure_list(c);
+  if (rc) return rc;
+  hipEvent_t ev[4];
+  for (int q = 0; q < 4; ++q) HIPCHK(c, hipEventCreate(&ev[q]));
+  double tot[3] = {0.0, 0.0, 0.0};
+  for (int it = 0; it < reps + 1 && rc == MFA_OK; ++it) {  // the first one warms up
+    hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream,
+                       c->n_cells, c->cell_count);
+    hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                       c->n_rep, (int)F_REBUILD, 1);
+    rc = launch_rebuild(c, ev);
+    hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                       c->n_rep, (int)F_REBUILD, 0);
+    if (rc) break;
+    if (hipEventSynchronize(ev[3]) != hipSuccess) {
+      rc = set_err(c, MFA_ERR_HIP, "event synchronize failed");
+      break;
+    }
+    for (int q = 0; q < 3 && it > 0; ++q) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) {
+        rc = set_err(c, MFA_ERR_HIP, "event elapsed time failed");
+        break;
+      }
+      tot[q] += t;
+    }
+  }
+  for (int q = 0; q < 4; ++q) hipEventDestroy(ev[q]);
+  if (rc) return rc;
+  for (int q = 0; q < 3; ++q) ms[q] = tot[q] / reps;
+  return check_flags(c);
 }
 
 // The k-space stage as the step runs it: ewald_k(1) and, with walls

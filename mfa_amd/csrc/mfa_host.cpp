@@ -248,7 +248,7 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < nt; ++j)
       p->inv_range_2[i][j] = range_2[i][j] != 0.0 ? 1.0 / range_2[i][j] : 0.0;
-  // Skin (init_params.f90:800-808) and check_skin's threshold (check_skin.f90:149).
+  // Skin (init_params.f90:800-808) and check_skin's threshold (check_skin.f90:18).
   p->skin = in->skin > 0.0 ? in->skin : (double)0.4f;
   p->skin_2 = (p->r_cut_max + p->skin) * (p->r_cut_max + p->skin) - p->r_cut_max * p->r_cut_max;
   p->skin_half_2 = (p->skin / 2.0) * (p->skin / 2.0);

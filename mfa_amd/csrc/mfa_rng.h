@@ -1,7 +1,7 @@
 // Counter-based random numbers for the DPD random force.
 //
 // The reference draws one Gaussian per interacting pair from a single
-// sequential ziggurat stream (dpd_forces.f90:273, ziggurat.f90:110-147), so the
+// sequential ziggurat stream (dpd_forces.f90:19, ziggurat.f90:110-147), so the
 // noise depends on the pair-loop order.  Here the noise of pair (i,j) at a
 // given step is a pure function of (seed, step, min(i,j), max(i,j)): any
 // thread visiting the pair from either side draws the same number, which keeps

@@ -173,7 +173,9 @@ SIGNATURES = {
     "mfa_stats": (C.c_int32, [_P, _lp]),
     "mfa_set_graph_steps": (C.c_int32, [_P, C.c_int32]),
     "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_time_force_step": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_time_rebuild": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
     "mfa_generate": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, _ip]),
@@ -324,6 +326,19 @@ class Context:
         ms = C.c_double()
         self._chk(self.lib.mfa_time_force(self.h, reps, C.byref(ms)), "mfa_time_force")
         return ms.value
+
+    def time_force_step(self, reps=100):
+        """Average ms of the force kernel the step runs (fused with the
+        integration in the mixed form), the state left as it was."""
+        ms = C.c_double()
+        self._chk(self.lib.mfa_time_force_step(self.h, reps, C.byref(ms)), "mfa_time_force_step")
+        return ms.value
+
+    def time_rebuild(self, reps=20):
+        """Average ms of the cell fill, cell sort and list build of a rebuild."""
+        out = np.zeros(3)
+        self._chk(self.lib.mfa_time_rebuild(self.h, reps, out.ctypes.data_as(_dp)), "mfa_time_rebuild")
+        return {"cell_fill": float(out[0]), "cell_sort": float(out[1]), "list_build": float(out[2])}
 
     def time_ewald(self, reps=100):
         """Average duration (ms) of the k-space Ewald stage; F is kept."""

@@ -420,7 +420,7 @@ void orc_get_state(orc_sys* s, double* r0, double* v, double* a, double* force, 
 #define F(i, d) s->force[3 * (size_t)(i) + (d)]
 #define MIC(i, d) s->mic[3 * (size_t)(i) + (d)]
 
-/* verlet_positions.f90:301-326 (update) and :392-406 (wrap).  Pinned by
+/* verlet_positions.f90:144-170 (update) and :233-250 (wrap).  Pinned by
  * closed forms: free flight and constant acceleration, tests/test_oracle.py. */
 void orc_verlet_positions(orc_sys* s) {
   const orc_params* p = &s->p;
@@ -441,7 +441,7 @@ void orc_verlet_positions(orc_sys* s) {
     }
 }
 
-/* check_skin.f90:139-152.  Pinned by a closed-form displacement test. */
+/* check_skin.f90:8-21.  Pinned by a closed-form displacement test. */
 int orc_check_skin(orc_sys* s) {
   const orc_params* p = &s->p;
   for (int i = 0; i < s->n; ++i) {
@@ -628,7 +628,7 @@ static void ewald_real(orc_sys* s, int ip, int jp, double r_2, const double* dr,
   }
 }
 
-/* fluid_fluid.f90:52-223 with dpd_forces.f90:256-352 (DPD_WEIGHT 0,
+/* fluid_fluid.f90:52-223 with dpd_forces.f90:2-98 (DPD_WEIGHT 0,
  * BIN_TYPE 0).  LJ is pinned by closed forms (minimum at 2^(1/6) sigma,
  * shifted energy zero at the cut-off) and energy/force finite differences;
  * the pair loop by antisymmetry (total force zero with DPD off); DPD by its
@@ -694,7 +694,7 @@ void orc_fluid_fluid(orc_sys* s, int64_t step, double* out) {
           w_arg = r_2 / p->r_cut_dpd_2;
         }
       }
-      if (do_dpd) { /* dpd_forces.f90:273-348 */
+      if (do_dpd) { /* dpd_forces.f90:19-94 */
         const double g_rand = orc_pair_normal(p->seed, step, ip, jp);
         double w_d = 1. - sqrt(w_arg);
         const double w_r = w_d;

@@ -8,7 +8,7 @@
  * operation order; every function cites the reference lines it restates.
  *
  * The only departure from the reference is the DPD noise: the reference draws
- * from one sequential ziggurat stream (dpd_forces.f90:273), which no parallel
+ * from one sequential ziggurat stream (dpd_forces.f90:19), which no parallel
  * code can reproduce; the oracle uses the counter-based normal that the GPU
  * path uses (Threefry-2x32-13 keyed by seed, step and the pair, Box-Muller),
  * implemented here independently.

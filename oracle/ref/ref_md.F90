@@ -28,6 +28,8 @@
 ! does it: init_obser before the loop, observation after every step past
 ! n_relax (md_main.f90:159-161), obser_out at the end; its files (fort.40-43,
 ! fort.80 and, built with PROFILES, dens_prof.mide) are what the tests read.
+! With `store`, the reference's restart writer runs after the loop
+! (store_config(2), md_main.f90:168-169) and leaves conf_new.
 program ref_md
 #include "control_simulation.h"
   use commons
@@ -35,7 +37,7 @@ program ref_md
   character(len=32) :: arg
   integer :: force_step, n_steps, n_rebuilds, n_pairs, k, j, u, ut
   integer, allocatable :: cells(:,:)
-  logical :: trace, snap, obs
+  logical :: trace, snap, obs, store
   integer :: snap_every, us
   real(kind=8) :: zsum, ftop, izt
 
@@ -46,11 +48,14 @@ program ref_md
   trace = .false.
   snap = .false.
   obs = .false.
+  store = .false.
   snap_every = 0
   if (command_argument_count() >= 3) then
     call get_command_argument(3, arg)
     if (trim(arg) == "obs") then
       obs = .true.
+    else if (trim(arg) == "store") then
+      store = .true.
     else if (trim(arg) == "snap") then
       snap = .true.
       call get_command_argument(4, arg)
@@ -141,6 +146,7 @@ program ref_md
     if (obs .and. i_time > n_relax) call observation
   end do
   if (obs) call obser_out()
+  if (store) call store_config(2)
   if (trace) close(ut)
   if (snap) close(us)
   open(newunit=u, file="ref_final.bin", access="stream", form="unformatted", status="replace")

@@ -8,13 +8,22 @@ no GPU test has to run the reference itself:
       central finite differences (relative step 1e-4), each from a run of
       oracle/_ref/ref_md_brush (the reference's Fortran, built from its
       sources by oracle/ref/build_ref.sh) at temperature 0 and list skin 0.4;
+  tests/golden/conf_new_brush_small/conf_new
+      the reference's restart file (store_config(2), oracle/_ref/ref_md_brush
+      with `store`) after 20 steps of the seeded small brush at temperature 0,
+      with the sha256 of the conf_old it started from;
+  tests/golden/conf_new_brush_small/conf_new_hip
+      the same restart file written by the Fortran driver on the GPU
+      (mfa_amd/mfa_hip_prog, the same run directory): made only with
+      --driver, on a machine with a GPU and the library built;
   tests/golden/obs_brush_small/
       the reference's observation files (fort.40-43, fort.80,
       dens_prof.mide) of 20 observed steps of the seeded small brush at
       temperature 0 (oracle/_ref/ref_md_brush_obs), plus the sha256 of the
       conf_old it read, which the test checks before it compares.
 
-    python3 tests/golden/make_ref_fixtures.py
+    python3 tests/golden/make_ref_fixtures.py            (reference fixtures)
+    python3 tests/golden/make_ref_fixtures.py --driver   (conf_new_hip, on a GPU)
 """
 import hashlib
 import os
@@ -132,13 +141,61 @@ def make_obs_fixture(tmp):
     print("wrote", out)
 
 
+def conf_new_inputs():
+    kind, inp = configs.small_cases()["brush_small"]
+    inp.skin = 0.0
+    inp.temp = 0.0
+    inp.n_relax, inp.n_obser, inp.n_safe = 0, 20, 20
+    return kind, inp
+
+
+def make_conf_new_fixture(tmp):
+    exe = os.path.join(REF, "ref_md_brush")
+    kind, inp = conf_new_inputs()
+    p, r, v, t = configs.build(inp, kind)
+    d = os.path.join(tmp, "store")
+    io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="fp64", mode="compat")
+    res = subprocess.run([exe, "3", "20", "store"], cwd=d, capture_output=True, text=True,
+                         timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
+    out = os.path.join(GOLD, "conf_new_brush_small")
+    os.makedirs(out, exist_ok=True)
+    shutil.copy(os.path.join(d, "conf_new"), os.path.join(out, "conf_new"))
+    with open(os.path.join(out, "conf_old.sha256"), "w") as fh:
+        fh.write(conf_digest(os.path.join(d, "conf_old")) + "\n")
+    print("wrote", out)
+
+
+def make_driver_conf_new(tmp):
+    exe = os.path.join(ROOT, "mfa_amd", "mfa_hip_prog")
+    kind, inp = conf_new_inputs()
+    p, r, v, t = configs.build(inp, kind)
+    d = os.path.join(tmp, "driver")
+    io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="fp64", mode="compat")
+    res = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
+    out = os.path.join(GOLD, "conf_new_brush_small")
+    want = open(os.path.join(out, "conf_old.sha256")).read().split()[0]
+    if conf_digest(os.path.join(d, "conf_old")) != want:
+        raise RuntimeError("the driver's conf_old differs from the reference run's")
+    shutil.copy(os.path.join(d, "conf_new"), os.path.join(out, "conf_new_hip"))
+    print("wrote", os.path.join(out, "conf_new_hip"))
+
+
 def main():
+    if "--driver" in sys.argv[1:]:
+        with tempfile.TemporaryDirectory() as tmp:
+            make_driver_conf_new(tmp)
+        return
     for prog in ("ref_md_brush", "ref_md_brush_obs"):
         if not os.path.exists(os.path.join(REF, prog)):
             sys.exit("%s missing: run oracle/ref/build_ref.sh first" % prog)
     with tempfile.TemporaryDirectory() as tmp:
         make_adjoint_fixture(tmp)
         make_obs_fixture(tmp)
+        make_conf_new_fixture(tmp)
 
 
 if __name__ == "__main__":

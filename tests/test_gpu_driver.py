@@ -90,3 +90,41 @@ def test_restart_from_sidecar_is_bit_exact(tmp_path, mode):
     second = run(str(tmp_path / "second"), 20, 22, conf=str(tmp_path / "first"))
     for k in ("r", "v", "a", "mic"):
         assert np.array_equal(whole[k], second[k]), k
+
+
+def _fields(line):
+    """A conf line as (widths, values): fixed-width fields of the reference's
+    formats (i1, 4i10, 2i10, i3, i3 3f13.7, 3f15.7, 3i13)."""
+    return len(line), [float(x) for x in line.split()]
+
+
+def test_driver_conf_new_matches_reference_text(tmp_path):
+    """The driver's conf_new after 20 steps of the seeded small brush at
+    T = 0 against the reference's own store_config(2) output on the same
+    input (tests/golden/conf_new_brush_small, store_config.f90:19-61,
+    277-285): same lines and field widths, the same integers, and every
+    number within one unit of its last printed digit (the two trajectories
+    agree to about 1e-10, so a value can round to the neighbouring digit)."""
+    import hashlib
+    gold = os.path.join(ROOT, "tests", "golden", "conf_new_brush_small")
+    assert os.path.exists(PROG), "Fortran driver not built (python -m mfa_amd.build)"
+    kind, inp = configs.small_cases()["brush_small"]
+    inp.skin = 0.0
+    inp.temp = 0.0
+    inp.n_relax, inp.n_obser, inp.n_safe = 0, 20, 20
+    p, r, v, t = configs.build(inp, kind)
+    d = str(tmp_path)
+    io.write_run_dir(d, inp, t, r, v, np.zeros_like(r), precision="fp64", mode="compat")
+    digest = hashlib.sha256(open(os.path.join(d, "conf_old"), "rb").read()).hexdigest()
+    assert digest == open(os.path.join(gold, "conf_old.sha256")).read().strip()
+    print("driver conf_new vs reference", flush=True)
+    res = subprocess.run([PROG], cwd=d, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    mine = open(os.path.join(d, "conf_new")).read().splitlines()
+    ref = open(os.path.join(gold, "conf_new")).read().splitlines()
+    assert len(mine) == len(ref) == 4 + 3 * p.n_part + 8
+    for k, (a, b) in enumerate(zip(mine, ref)):
+        wa, va = _fields(a)
+        wb, vb = _fields(b)
+        assert wa == wb and len(va) == len(vb), (k, a, b)
+        assert np.all(np.abs(np.array(va) - np.array(vb)) <= 1.01e-7), (k, a, b)

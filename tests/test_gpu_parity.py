@@ -386,3 +386,51 @@ def test_langevin_holds_the_temperature():
     print("Langevin: T target %.3f measured %.4f" % (inputs.temp, tm))
     assert abs(tm / inputs.temp - 1.0) < 0.03
     ctx.close()
+
+
+@pytest.mark.parametrize("case", ["melt_small", "brush_small"])
+def test_fused_step_matches_separate_kernels(case):
+    """The mixed step whose force pass also integrates (double-buffered
+    positions and view) against the same steps with the integration in its
+    own kernel (kernel timing on switches the fusion off): bit-identical
+    state and the same rebuilds, over 60 steps with several rebuilds."""
+    inputs, p, r, v, t, a = make_case(case)
+    fused = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    split = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    split.timing(True)
+    of = fused.run_steps(1, 60)
+    os_ = split.run_steps(1, 60)
+    sf, ss = fused.get_state(), split.get_state()
+    print("%s: %d rebuilds" % (case, of["n_rebuilds"]), flush=True)
+    assert of["n_rebuilds"] == os_["n_rebuilds"] and of["n_rebuilds"] >= 2
+    for k in ("r", "v", "f", "a", "mic"):
+        assert np.array_equal(sf[k], ss[k]), k
+    for k in ("v_fluid_fluid", "v_intra_molec", "t_kin"):
+        assert of[k] == os_[k], k
+    fused.close()
+    split.close()
+
+
+def test_time_force_step_leaves_the_state():
+    """mfa_time_force_step times the fused force kernel without changing the
+    state: the steps that follow equal those of an untimed context."""
+    inputs, p, r, v, t, a = make_case("melt_small")
+    timed = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    plain = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    timed.run_steps(1, 20)
+    plain.run_steps(1, 20)
+    before = timed.get_state()
+    ms = timed.time_force_step(10)
+    after = timed.get_state()
+    print("fused force pass %.4f ms" % ms, flush=True)
+    assert ms > 0.0
+    for k in ("r", "v", "f", "a", "mic"):
+        assert np.array_equal(before[k], after[k]), k
+    ot = timed.run_steps(21, 20)
+    op = plain.run_steps(21, 20)
+    st, sp = timed.get_state(), plain.get_state()
+    for k in ("r", "v", "mic"):
+        assert np.array_equal(st[k], sp[k]), k
+    assert ot["n_rebuilds"] == op["n_rebuilds"]
+    timed.close()
+    plain.close()

@@ -361,7 +361,7 @@ def test_verlet_free_flight_and_constant_acceleration():
     st = s.get_state()
     assert np.allclose(st["r"], r + dt * v + 0.5 * dt * dt * a, atol=1e-15)
     assert np.allclose(st["v"], v + 0.5 * dt * a, atol=1e-15)
-    # periodic wrap with the image counter (verlet_positions.f90:398-404)
+    # periodic wrap with the image counter (verlet_positions.f90:243-249)
     s.set_state(np.array([[11.999, 2.0, 0.001], [7.0, 8.0, 9.0]]),
                 np.array([[1.0, 0, -1.0], [0, 0, 0]]), np.zeros((2, 3)),
                 np.array([3, 3], dtype=np.int32))
@@ -372,7 +372,7 @@ def test_verlet_free_flight_and_constant_acceleration():
 
 
 def test_check_skin_threshold():
-    """check_skin.f90:139-152: rebuild once a displacement exceeds skin/2."""
+    """check_skin.f90:8-21: rebuild once a displacement exceeds skin/2."""
     inp = configs.melt(1, 2, 0.001, lj_rc=2.5, thermostat=-1, n_cell=16)
     inp.x_space = inp.y_space = 12.0 / 16
     inp.z_space_wall = 12.0

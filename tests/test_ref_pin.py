@@ -7,7 +7,7 @@ reference's sources at build time).
 Inputs keep the reference's own constants (list skin 0.4, the WCA cut-off
 its menu offers) and set the temperature in mfa_input to 0: the random DPD
 force is then sig * w * g with sig = sqrt(2 T gamma / dt) = 0
-(init_params.f90:845, dpd_forces.f90:58), so the reference's ziggurat stream
+(init_params.f90:845, dpd_forces.f90:63), so the reference's ziggurat stream
 drops out while its dissipative DPD force stays in.
 
 Bars: cells and the pair set bit-exact; forces and energies of one force
@@ -303,3 +303,37 @@ def test_oracle_matches_reference_window_fixture():
         tm[q] *= 1.0 - h
         g = (window(tp) - window(tm)) / (tp[q] - tm[q])
         assert np.allclose(g, fx["G_fd"][:, q], rtol=1e-4, atol=0.0), (q, g, fx["G_fd"][:, q])
+
+
+def test_reference_reads_the_driver_conf_new(tmp_path):
+    """The reference's own reader (init_config.f90:110-209) on a conf_new the
+    Fortran driver wrote (tests/golden/conf_new_brush_small/conf_new_hip,
+    written on the GPU by the driver after 20 steps of the small brush at
+    T = 0): it takes every position, velocity and acceleration as the text
+    gives them, and its first force pass equals the oracle's from the same
+    text."""
+    exe = os.path.join(REF, "ref_md_brush")
+    src = os.path.join(GOLD, "conf_new_brush_small", "conf_new_hip")
+    if not os.path.exists(exe):
+        pytest.skip("reference build absent (oracle/ref/build_ref.sh needs the reference sources)")
+    if not os.path.exists(src):
+        pytest.skip("driver conf_new fixture absent")
+    kind, inp = configs.small_cases()["brush_small"]
+    inp.skin = 0.0
+    inp.temp = 0.0
+    p, r, v, t = configs.build(inp, kind)
+    d = str(tmp_path)
+    io.write_run_dir(d, inp, t, r, v, np.zeros_like(r))
+    with open(src) as f, open(os.path.join(d, "conf_old"), "w") as g:
+        g.write(f.read())
+    res = subprocess.run([exe, "3", "0"], cwd=d, capture_output=True, text=True, timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    init = _read(os.path.join(d, "ref_init.bin"), p.n_part, True)
+    t2, r2, v2, a2 = io.read_conf(src, p.n_part)
+    assert np.array_equal(init["r"], r2) and np.array_equal(init["v"], v2)
+    assert np.array_equal(init["a"], a2)
+    s = orc.System(inp)
+    s.set_state(r2, v2, a2, t2)
+    s.binning()
+    f, e = s.forces(step=3)
+    assert force_error(init["f"], f)[0] < 1e-12
