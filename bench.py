@@ -80,11 +80,13 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     force_ms = ctx.time_force(200)
     step_force_ms = ctx.time_force_step(200)
     ewald_ms = ctx.time_ewald(200) if p.coulomb else None
+    ewald_parts = ctx.time_ewald_parts(50) if p.coulomb else None
     st = ctx.get_state()
     rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
     return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp,
-                     "ewald_ms": ewald_ms, "rebuild_ms": rebuild}, st
+                     "ewald_ms": ewald_ms, "ewald_parts_ms": ewald_parts,
+                     "rebuild_ms": rebuild}, st
 
 
 def ewald_k_count(p, inputs):
@@ -296,6 +298,7 @@ def bench_single(args, name):
     }
     if tm_m.get("ewald_ms"):
         extra["ewald_roofline"] = ewald_roofline(p, inputs, t, tm_m["ewald_ms"])
+        extra["ewald_roofline"]["kernel_ms"] = tm_m["ewald_parts_ms"]
         log("k-space stage %.4f ms avg: %.2f TFLOP/s FP64" % (
             tm_m["ewald_ms"], extra["ewald_roofline"]["achieved"]))
     return contract_line(args, name, value, dt_m / args.steps * 1e3,

@@ -317,6 +317,12 @@ int32_t mfa_time_force_step(mfa_ctx* ctx, int32_t reps, double* ms_avg);
  * mfa_time_force; F is left as it was.  MFA_ERR_ARG without charges. */
 int32_t mfa_time_ewald(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
+/* Average durations (ms) of the five kernels of that stage, each bracketed by
+ * HIP events: ms[0] the phase tables, ms[1] the structure factor, ms[2] its
+ * sum, ms[3] the forces, ms[4] adding them to F.  F is left as it was.
+ * Measurement helper with no reference counterpart. */
+int32_t mfa_time_ewald_parts(mfa_ctx* ctx, int32_t reps, double* ms /* [5] */);
+
 /* Average durations (ms) of the kernels of a rebuild: ms[0] the cell fill,
  * ms[1] the cell sort, ms[2] the list build, each bracketed by HIP events on
  * the context's stream, over `reps` rebuilds forced at the current state.

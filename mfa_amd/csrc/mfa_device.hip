@@ -1474,6 +1474,11 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
 }
 
+// Candidates per lane whose loads the mixed force pass issues together.
+#ifndef FM_BATCH
+#define FM_BATCH 4
+#endif
+
 // What the mixed force pass of a step that also integrates reads and writes
 // besides r, the view and the list: the next positions (the other position
 // buffer), velocities and wrap counts (in the step: in place, v_out = v_in,
@@ -1561,79 +1566,96 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   if (active && (mask & TM_PAIR)) {
     const int cnt = nn[ss];
     const unsigned* row = nl + (size_t)ss * cap;
-    for (int q = sub; q < cnt; q += TPP) {
-      const unsigned e = row[q];
-      const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
-      const float4 o = s_off[e >> MFA_JBITS];
-      const float4 pj = pos4[js];
-      const float dx = (pi4.x - pj.x) + o.x;
-      const float dy = (pi4.y - pj.y) + o.y;
-      const float dz = (pi4.z - pj.z) + o.z;
-      const float r2raw = dx * dx + dy * dy + dz * dz;
-      const int wj = __float_as_int(pj.w);
-      const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
-      const float4 tb = s_tab[ti * MFA_NTYPE + tj];
-      bool in_lj = r2raw < tb.x;
-      bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
-      bool in_coul = coul_rc && r2raw < rc2f;
-      // distance of r^2 to the nearest cut-off that decides something
-      float dm = fabsf(r2raw - tb.x);
-      if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
-      if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
-      if (__builtin_expect(dm < margin, 0)) {
-        // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
-        const int j = k.p0 + jl;
-        double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
-        double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
-        double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
-        min_image(k, ex, ey, ez);
-        const double r2d = (ex * ex + ey * ey) + ez * ez;
-        in_lj = r2d < k.range_2[ti][tj];
-        in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
-        in_coul = coul_rc && r2d < k.rc_coul_2;
+    // FM_BATCH candidates per lane and round: their list entries are loaded
+    // together, then their views together, so each round waits for two
+    // memory round trips instead of two per candidate.  Past the end of the
+    // row a slot reads the particle's own view and is skipped.
+    for (int q0 = sub; q0 < cnt; q0 += TPP * FM_BATCH) {
+      unsigned ev[FM_BATCH];
+      float4 pv[FM_BATCH];
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) {
+        const int q = q0 + u * TPP;
+        ev[u] = q < cnt ? row[q] : (unsigned)ss;
       }
-      // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
-      const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
-      // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
-      const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
-      if (!in_lj && !dpd && !coul) continue;
-      const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
-      const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
-      const float inv_r_2 = inv_r * inv_r;
-      float s = 0.0f;  // F_ij = s * (dx, dy, dz)
-      if (in_lj) {
-        const float r61 = tb.z * inv_r_2;
-        const float r6 = r61 * r61 * r61;
-        const float r12 = r6 * r6;
-        const float rd = tb.y * (6.0f * r6 - 12.0f * r12) * inv_r_2;
-        s = -rd;
-        if (ENERGY) {
-          e_ff += 0.5 * (double)(tb.y * ((r12 - r6) - tb.w));
-          vir += 0.5 * (double)(-rd * r2raw);
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) {
+        if (q0 + u * TPP >= cnt) continue;
+        const unsigned e = ev[u];
+        const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
+        const float4 o = s_off[e >> MFA_JBITS];
+        const float4 pj = pv[u];
+        const float dx = (pi4.x - pj.x) + o.x;
+        const float dy = (pi4.y - pj.y) + o.y;
+        const float dz = (pi4.z - pj.z) + o.z;
+        const float r2raw = dx * dx + dy * dy + dz * dz;
+        const int wj = __float_as_int(pj.w);
+        const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
+        const float4 tb = s_tab[ti * MFA_NTYPE + tj];
+        bool in_lj = r2raw < tb.x;
+        bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
+        bool in_coul = coul_rc && r2raw < rc2f;
+        // distance of r^2 to the nearest cut-off that decides something
+        float dm = fabsf(r2raw - tb.x);
+        if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
+        if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
+        if (__builtin_expect(dm < margin, 0)) {
+          // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
+          const int j = k.p0 + jl;
+          double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+          double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+          double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+          min_image(k, ex, ey, ez);
+          const double r2d = (ex * ex + ey * ey) + ez * ez;
+          in_lj = r2d < k.range_2[ti][tj];
+          in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
+          in_coul = coul_rc && r2d < k.rc_coul_2;
         }
+        // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
+        const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
+        // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
+        const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
+        if (!in_lj && !dpd && !coul) continue;
+        const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
+        const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
+        const float inv_r_2 = inv_r * inv_r;
+        float s = 0.0f;  // F_ij = s * (dx, dy, dz)
+        if (in_lj) {
+          const float r61 = tb.z * inv_r_2;
+          const float r6 = r61 * r61 * r61;
+          const float r12 = r6 * r6;
+          const float rd = tb.y * (6.0f * r6 - 12.0f * r12) * inv_r_2;
+          s = -rd;
+          if (ENERGY) {
+            e_ff += 0.5 * (double)(tb.y * ((r12 - r6) - tb.w));
+            vir += 0.5 * (double)(-rd * r2raw);
+          }
+        }
+        if (dpd) {
+          const float xi_n = pair_normal_f32(seed, step, il, jl);
+          // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
+          const float inv_rc = DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj];
+          const float wr = 1.0f - (r2 * inv_r) * inv_rc;
+          const float4 vj = vel4[js];
+          const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
+          s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
+        }
+        if ((XT & 1) && coul) {  // ewald_real.f90:15-32 with the clamped r_2
+          const float qq = (float)k.q_type[ti] * (float)k.q_type[tj];
+          const float rr = r2 * inv_r;
+          const float ec = erfcf(sa * rr);
+          s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
+          if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
+        }
+        // the pair's three components in fixed point: (s 2^22) d equals
+        // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
+        const float sf = s * MFA_FIX_SCALE;
+        ax += (double)truncf(sf * dx);
+        ay += (double)truncf(sf * dy);
+        az += (double)truncf(sf * dz);
       }
-      if (dpd) {
-        const float xi_n = pair_normal_f32(seed, step, il, jl);
-        // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
-        const float inv_rc = DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj];
-        const float wr = 1.0f - (r2 * inv_r) * inv_rc;
-        const float4 vj = vel4[js];
-        const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
-        s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
-      }
-      if ((XT & 1) && coul) {  // ewald_real.f90:15-32 with the clamped r_2
-        const float qq = (float)k.q_type[ti] * (float)k.q_type[tj];
-        const float rr = r2 * inv_r;
-        const float ec = erfcf(sa * rr);
-        s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
-        if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
-      }
-      // the pair's three components in fixed point: (s 2^22) d equals
-      // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
-      const float sf = s * MFA_FIX_SCALE;
-      ax += (double)truncf(sf * dx);
-      ay += (double)truncf(sf * dy);
-      az += (double)truncf(sf * dz);
     }
   }
 
@@ -2418,7 +2440,7 @@ static KParams make_kparams(const mfa_params& p) {
 }
 
 static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
-static int launch_ewald(mfa_ctx* c, int add_dip);
+static int launch_ewald(mfa_ctx* c, int add_dip, hipEvent_t* ev = nullptr);
 
 
 
@@ -3055,18 +3077,23 @@ static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ew.e.nq > 0 &
 
 // k-space forces (and, with add_dip, the slab dipole forces) added to F.
 // Shapes come from the checked plan (ewald_shape).
-static int launch_ewald(mfa_ctx* c, int add_dip) {
+// With ev (six events), the five kernels are bracketed by them for timing.
+static int launch_ewald(mfa_ctx* c, int add_dip, hipEvent_t* ev) {
   if (!ewald_active(c)) return MFA_OK;
   const EwPlan& w = c->ew;
   const EwaldK& e = w.e;
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   hipLaunchKernelGGL(k_ewald_tables, dim3(e.n_mz, e.nex + e.ney + e.nez), dim3(MFA_BLOCK), 0,
                      c->stream, e, c->r, c->d_qidx, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
                      c->d_mz_part);
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_chunks, (e.nsegs + MFA_BLOCK - 1) / MFA_BLOCK),
                      dim3(MFA_BLOCK), w.lds_rho, c->stream, e, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
                      c->d_runs, c->d_run_k0, c->d_rho_part);
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   hipLaunchKernelGGL(k_ewald_sum, dim3(w.sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
                      c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
+  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   if (w.rows_kernel) {
 #define EW_ROWS_CASE(X)                                                                              \
   case X:                                                                                            \
@@ -3085,9 +3112,11 @@ static int launch_ewald(mfa_ctx* c, int add_dip) {
                        w.lds_force, c->stream, e, c->d_tx, c->d_ty, c->d_tz, c->d_runs, c->d_run_k0,
                        c->d_grho, c->d_fk_part);
   }
+  if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
   hipLaunchKernelGGL(k_ewald_apply, dim3((3 * e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
                      c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, w.n_kgroups, c->d_mz,
                      w.inv_V, 1, add_dip, c->f);
+  if (ev) HIPCHK(c, hipEventRecord(ev[5], c->stream));
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -4336,6 +4365,54 @@ extern "C" int32_t mfa_time_ewald(mfa_ctx* c, int32_t reps, double* ms_avg) {
   if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, MFA_ERR_HIP, "sync failed");
   hipFree(keep);
   return rc ? rc : check_flags(c);
+}
+
+// Average durations (ms) of the five kernels of the k-space stage (tables,
+// structure factor, its sum, forces, apply), each bracketed by HIP events on
+// the context's stream, over `reps` launches at the current state; F is put
+// back afterwards.
+extern "C" int32_t mfa_time_ewald_parts(mfa_ctx* c, int32_t reps, double* ms) {
+  if (!c || !ms || reps < 1 || reps > 4096) return MFA_ERR_ARG;
+  if (!ewald_active(c)) return set_err(c, MFA_ERR_ARG, "no k-space Ewald stage (no charges)");
+  const size_t fb = 3 * (size_t)c->n * sizeof(double);
+  double* keep = nullptr;
+  if (hipMalloc(&keep, fb) != hipSuccess)
+    return set_err(c, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (F copy) failed", fb);
+  hipEvent_t ev[6];
+  int made = 0;
+  int rc = MFA_OK;
+  for (; made < 6; ++made)
+    if (hipEventCreate(&ev[made]) != hipSuccess) {
+      rc = set_err(c, MFA_ERR_HIP, "event create failed");
+      break;
+    }
+  if (!rc && hipMemcpyAsync(keep, c->f, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    rc = set_err(c, MFA_ERR_HIP, "F copy failed");
+  double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int it = 0; it < reps + 1 && !rc; ++it) {  // the first one warms up
+    rc = launch_ewald(c, c->p.symmetry == 0 ? 1 : 0, ev);
+    if (rc) break;
+    if (hipEventSynchronize(ev[5]) != hipSuccess) {
+      rc = set_err(c, MFA_ERR_HIP, "event synchronize failed");
+      break;
+    }
+    for (int q = 0; q < 5 && it > 0; ++q) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) {
+        rc = set_err(c, MFA_ERR_HIP, "event elapsed time failed");
+        break;
+      }
+      tot[q] += t;
+    }
+  }
+  if (hipMemcpyAsync(c->f, keep, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess && !rc)
+    rc = set_err(c, MFA_ERR_HIP, "F restore failed");
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, MFA_ERR_HIP, "sync failed");
+  for (int q = 0; q < made; ++q) hipEventDestroy(ev[q]);
+  hipFree(keep);
+  if (rc) return rc;
+  for (int q = 0; q < 5; ++q) ms[q] = tot[q] / reps;
+  return check_flags(c);
 }
 
 extern "C" int32_t mfa_copy_probe(int64_t bytes, double* ms) {

@@ -174,6 +174,7 @@ SIGNATURES = {
     "mfa_set_graph_steps": (C.c_int32, [_P, C.c_int32]),
     "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_force_step": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_time_ewald_parts": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_rebuild": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
@@ -333,6 +334,13 @@ class Context:
         ms = C.c_double()
         self._chk(self.lib.mfa_time_force_step(self.h, reps, C.byref(ms)), "mfa_time_force_step")
         return ms.value
+
+    def time_ewald_parts(self, reps=20):
+        """Average ms of the k-space stage's kernels."""
+        out = np.zeros(5)
+        self._chk(self.lib.mfa_time_ewald_parts(self.h, reps, out.ctypes.data_as(_dp)),
+                  "mfa_time_ewald_parts")
+        return dict(zip(("tables", "rho", "rho_sum", "force", "apply"), map(float, out)))
 
     def time_rebuild(self, reps=20):
         """Average ms of the cell fill, cell sort and list build of a rebuild."""

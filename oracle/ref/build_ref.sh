@@ -27,19 +27,29 @@ SOURCES="mfa_common ziggurat R250 INR250 SUN util functions messages \
   gen_brush gen_chain gen_wall gen_droplet conf_default init_config init_obser \
   obser_out observation store_config fix_CM constant_force geom_constraint new_dpd_fd"
 
-for cfg in brush brush_q brush_bend brush_vv brush_lgv brush_obs melt melt_fso; do
+# build <config> <output name> <flags>
+build() {
+  cfg=$1; name=$2; flags=$3
   TMP=$(mktemp -d)
   cp "$HERE/control_$cfg.h" "$TMP/control_simulation.h"
   objs=""
   for f in $SOURCES; do
     cpp -traditional-cpp -P -I"$TMP" -Dk_ring=k_chain < "$REF/$f.f90" > "$TMP/$f.f90"
-    (cd "$TMP" && "$FC" -c -O1 -ffp-contract=off "$f.f90" -o "$f.o" > "$TMP/$f.log" 2>&1) || {
+    (cd "$TMP" && "$FC" -c $flags "$f.f90" -o "$f.o" > "$TMP/$f.log" 2>&1) || {
       cat "$TMP/$f.log"; rm -rf "$TMP"; echo "build_ref: $f failed ($cfg)"; exit 1; }
     objs="$objs $TMP/$f.o"
   done
   cpp -traditional-cpp -P -I"$TMP" < "$HERE/ref_md.F90" > "$TMP/ref_md.f90"
-  (cd "$TMP" && "$FC" -O1 -ffp-contract=off ref_md.f90 $objs -o "$OUT/ref_md_$cfg" > "$TMP/link.log" 2>&1) || {
+  (cd "$TMP" && "$FC" $flags ref_md.f90 $objs -o "$OUT/$name" > "$TMP/link.log" 2>&1) || {
     cat "$TMP/link.log"; rm -rf "$TMP"; echo "build_ref: link failed ($cfg)"; exit 1; }
   rm -rf "$TMP"
-  echo "build_ref: $OUT/ref_md_$cfg"
+  echo "build_ref: $OUT/$name"
+}
+
+# the parity builds: no contraction, so the operation order is the source's
+for cfg in brush brush_q brush_bend brush_vv brush_lgv brush_obs melt melt_fso; do
+  build "$cfg" "ref_md_$cfg" "-O1 -ffp-contract=off"
 done
+# a timing build of the brush path (no parity use): -O3 for the host CPU, as
+# the reference's users would build it for speed (oracle/calibrate_cpu.py)
+build brush ref_md_brush_o3 "-O3 -march=native"
