@@ -60,8 +60,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
-    """Warm-up, then the timed region: `steps` steps in one mfa_run_steps call
+def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tune_skin=True):
+    """Warm-up (the autotuners' choices of the force pass's width and, where
+    the workload leaves it open, of the list skin, then `warmup` steps), then
+    the timed region: `steps` steps in one mfa_run_steps call
     (blocks of 16 steps captured in a hipGraph), state resident in HBM.  Then
     the average kernel duration of the force pass at the final state, alone
     and as the step runs it (in the mixed form fused with the integration),
@@ -73,6 +75,11 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     # untimed: the autotuner picks the threads per particle of the force pass
     tpp, _ = ctx.autotune(first_step, 20)
     first_step += 5 * (max(graph_steps, 0) + 1 + 20)
+    skin = p.skin
+    if tune_skin:
+        cand = [x for x in configs.SKIN_CANDIDATES if _fits(p, x)]
+        skin, _ = ctx.autotune_skin(first_step, 200, cand)
+        first_step += len(cand) * (max(graph_steps, 0) + 1 + 200)
     ctx.run_steps(first_step, warmup, want_obs=False)
     t0 = time.perf_counter()
     obs = ctx.run_steps(first_step + warmup, steps, want_obs=True)
@@ -84,9 +91,17 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16):
     st = ctx.get_state()
     rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
-    return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp,
+    return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp, "skin": skin,
                      "ewald_ms": ewald_ms, "ewald_parts_ms": ewald_parts,
                      "rebuild_ms": rebuild}, st
+
+
+def _fits(p, skin):
+    try:
+        lib.with_skin(p, skin)
+        return True
+    except lib.MfaError:
+        return False
 
 
 def ewald_k_count(p, inputs):
@@ -213,14 +228,15 @@ def bench_single(args, name):
     n = p.n_part
     log("%s: n=%d box=%s cells=%s, prepared in %.1f s" % (
         name, n, [round(x, 4) for x in p.boundary], list(p.n_bin), time.perf_counter() - t0))
+    tune = not w.get("skin_fixed", False)
     dt_m, obs_m, tm_m, st_m = run_form(p, st0, lib.MFA_MIXED, args.steps, args.warmup,
-                                       graph_steps=args.graph_steps)
+                                       graph_steps=args.graph_steps, tune_skin=tune)
     log("mixed: %.4f s for %d steps, force pass %.4f ms avg (%d threads per particle), "
         "rebuilds %d, T=%.3f" % (dt_m, args.steps, tm_m["force_ms"], tm_m["tpp"],
                                  obs_m["n_rebuilds"], 2.0 * obs_m["t_kin"] / (3.0 * n)))
     log("rebuild kernels (ms): %s" % tm_m["rebuild_ms"])
     dt_d, obs_d, tm_d, _ = run_form(p, st0, lib.MFA_FP64, args.steps, args.warmup,
-                                    graph_steps=args.graph_steps)
+                                    graph_steps=args.graph_steps, tune_skin=tune)
     log("fp64 : %.4f s for %d steps, force %.4f ms avg (%d threads per particle)" % (
         dt_d, args.steps, tm_d["force_ms"], tm_d["tpp"]))
     st_m["t"] = t
@@ -243,7 +259,7 @@ def bench_single(args, name):
                          "after 2 warm-up steps (%.1f s)" % (n, name, args.cpu_steps, secs)}
     if name == "melt100k":
         config = {"n_particles": n, "chains": 2000, "chain_length": 50, "density": 0.85,
-                  "lj_rc": 2.5, "dt": 0.01, "dpd_gamma": 1.0, "skin": 0.3,
+                  "lj_rc": 2.5, "dt": 0.01, "dpd_gamma": 1.0,
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
@@ -251,7 +267,7 @@ def bench_single(args, name):
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
-                  "skin": inputs.skin, "parallelism": "single GPU"}
+                  "parallelism": "single GPU"}
         if name == "pe_brush27k":
             config.update({"alpha_c": inputs.alpha_c, "n_kmax": inputs.n_kmax,
                            "rc_coul": inputs.rc_coul})
@@ -280,6 +296,8 @@ def bench_single(args, name):
                               "unit": "G wave-instructions/s", "frac": g / VALU_PEAK_GINSTS,
                               "instructions_per_launch": valu}
     config["threads_per_particle"] = {"mixed": tm_m["tpp"], "fp64": tm_d["tpp"]}
+    config["list_skin"] = {"mixed": tm_m["skin"], "fp64": tm_d["skin"],
+                           "chosen_by": "autotuner" if tune else "the workload's spec"}
     extra = {
         "fp64_value": value_fp64,
         "mixed_over_fp64": value / value_fp64,

@@ -186,6 +186,12 @@ int32_t mfa_abi_sizes(int32_t* sizeof_inputs, int32_t* sizeof_params, int32_t* s
  * and make_binning_boxes (make_binning_boxes.f90:14-64).  Host-only. */
 int32_t mfa_derive(const mfa_inputs* in, mfa_params* out);
 
+/* The skin-dependent constants of p (skin, skin_2, skin_half_2, list_2) for
+ * another list skin, as mfa_derive computes them (init_params.f90:800-808,
+ * binning.f90:101-105); the cells stay (make_binning_boxes.f90:42-47), so
+ * MFA_ERR_ARG if the list radius would exceed a cell's width.  Host-only. */
+int32_t mfa_params_set_skin(mfa_params* p, double skin);
+
 /* Create/destroy a device context for n_part particles.  Replaces the
  * allocations of init_system.f90:228-299 and make_binning_boxes.f90:70-78.
  * precision: MFA_FP64 or MFA_MIXED.  list_capacity: full neighbour-list slots
@@ -317,9 +323,10 @@ int32_t mfa_time_force_step(mfa_ctx* ctx, int32_t reps, double* ms_avg);
  * mfa_time_force; F is left as it was.  MFA_ERR_ARG without charges. */
 int32_t mfa_time_ewald(mfa_ctx* ctx, int32_t reps, double* ms_avg);
 
-/* Average durations (ms) of the five kernels of that stage, each bracketed by
- * HIP events: ms[0] the phase tables, ms[1] the structure factor, ms[2] its
- * sum, ms[3] the forces, ms[4] adding them to F.  F is left as it was.
+/* Average durations (ms) of the five kernels of that stage, each timed alone
+ * like mfa_time_force (back-to-back launches of that kernel between two HIP
+ * events): ms[0] the phase tables, ms[1] the structure factor, ms[2] its sum,
+ * ms[3] the forces, ms[4] adding them to F.  F is left as it was.
  * Measurement helper with no reference counterpart. */
 int32_t mfa_time_ewald_parts(mfa_ctx* ctx, int32_t reps, double* ms /* [5] */);
 
@@ -383,6 +390,21 @@ int32_t mfa_obs_read(mfa_ctx* ctx, mfa_obs_sums* out, double* dens_z, double* f_
  * fastest and returns it with its time per force pass (ms). */
 int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
                      double* ms_per_pass);
+
+/* Another list skin for every replica of a running context (no counterpart
+ * in the reference, whose skin is fixed at 0.4): the constants as
+ * mfa_params_set_skin gives them, a longer list buffer if the list grows,
+ * and a fresh list (with its skin reference) before the next force pass.
+ * Pairs within the cut-offs, and so the forces, do not depend on the skin;
+ * how often the list is rebuilt and how long it is do. */
+int32_t mfa_set_skin(mfa_ctx* ctx, double skin);
+
+/* Skin autotuner: for each candidate skin that fits the cells, runs
+ * n_steps whole steps (rebuilds included) from first_step, advancing the
+ * state, timed by HIP events; keeps the fastest and returns it with its time
+ * per step (ms).  Call it during warm-up. */
+int32_t mfa_autotune_skin(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t n_cand,
+                          const double* skins, double* best_skin, double* ms_per_step);
 
 /*
  * Adjoint of the force-and-integrate step (no counterpart in the reference,

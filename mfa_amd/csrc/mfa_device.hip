@@ -144,6 +144,7 @@ struct mfa_ctx {
   bool lgv_terms;    // some replica runs THERMOSTAT 1: force kernels with XT & 2
   int n;
   int cap;
+  bool cap_fixed;  // the capacity the caller chose at creation (no growth)
   int tpp;  // threads per particle in the force pass
   int n_cells;
   bool need_rebuild;
@@ -1802,10 +1803,18 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 #define EW_SEG 8   // k vectors per segment
 #define EW_CH 32   // charged particles per structure-factor block
 
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+#pragma clang fp contract(fast)
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 // Phase tables X[n][j] (n = nx + nxm), Y[n][j] (n = ny + nym), Z[n][j]
-// (n = nz) of the charged particles: one thread per table entry (one sincos),
-// entries of consecutive particles in consecutive threads; blocks of the
-// first entry row also write the partial dipole moments sum q (z - lz_half).
+// (n = nz) of the charged particles: one thread per particle and axis (block
+// row blockIdx.y = axis), one sincos of dk r, then the powers by complex
+// products, e^{i n t} = e^{i (n-1) t} e^{i t} (n ulp of drift at most, for
+// n <= nmax), and the negative n as conjugates; entries of consecutive
+// particles in consecutive threads.  The x blocks also write the partial
+// dipole moments sum q (z - lz_half).
 __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_tables(EwaldK e, const double* __restrict__ r,
                                                             const int* __restrict__ qidx,
                                                             const double* __restrict__ qval,
@@ -1813,28 +1822,38 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_tables(EwaldK e, const doub
                                                             double2* __restrict__ ty,
                                                             double2* __restrict__ tz,
                                                             double* __restrict__ mz_part) {
+#pragma clang fp contract(fast)  // FP64 sums within 1e-12: fused multiply-adds
   __shared__ double sh[MFA_BLOCK / 64];
   const int idx = blockIdx.x * MFA_BLOCK + threadIdx.x;  // charged particle
-  const int n = blockIdx.y;                               // table entry (all three axes)
+  const int ax = blockIdx.y;                              // 0 x, 1 y, 2 z
   double mz = 0.0;
   if (idx < e.nq) {
     const int i = qidx[idx];
-    double s, c;
-    if (n < e.nex) {
-      sincos(e.dk[0] * (double)(n - e.nxm) * r[3 * (size_t)i + 0], &s, &c);
-      tx[(size_t)n * e.nq + idx] = make_double2(c, s);
-    } else if (n < e.nex + e.ney) {
-      const int m = n - e.nex;
-      sincos(e.dk[1] * (double)(m - e.nym) * r[3 * (size_t)i + 1], &s, &c);
-      ty[(size_t)m * e.nq + idx] = make_double2(c, s);
+    const double x = r[3 * (size_t)i + ax];
+    double s1, c1;
+    sincos(e.dk[ax] * x, &s1, &c1);
+    const double2 t1 = make_double2(c1, s1);
+    double2 t = make_double2(1.0, 0.0);
+    const size_t nq = (size_t)e.nq;
+    if (ax == 2) {
+      tz[idx] = t;
+      for (int n = 1; n <= e.nzm; ++n) {
+        t = cmul(t, t1);
+        tz[(size_t)n * nq + idx] = t;
+      }
     } else {
-      const int m = n - e.nex - e.ney;
-      sincos(e.dk[2] * (double)m * r[3 * (size_t)i + 2], &s, &c);
-      tz[(size_t)m * e.nq + idx] = make_double2(c, s);
+      double2* tab = ax == 0 ? tx : ty;
+      const int m = ax == 0 ? e.nxm : e.nym;
+      tab[(size_t)m * nq + idx] = t;
+      for (int n = 1; n <= m; ++n) {
+        t = cmul(t, t1);
+        tab[(size_t)(m + n) * nq + idx] = t;
+        tab[(size_t)(m - n) * nq + idx] = make_double2(t.x, -t.y);
+      }
     }
-    if (n == 0) mz = qval[idx] * (r[3 * (size_t)i + 2] - e.lz_half);
+    if (ax == 0) mz = qval[idx] * (r[3 * (size_t)i + 2] - e.lz_half);
   }
-  if (n != 0) return;
+  if (ax != 0) return;
   for (int o = 32; o > 0; o >>= 1) mz += __shfl_down(mz, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mz;
   __syncthreads();
@@ -1843,10 +1862,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_tables(EwaldK e, const doub
     for (int w = 0; w < MFA_BLOCK / 64; ++w) m += sh[w];
     mz_part[blockIdx.x] = m;
   }
-}
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // Partial structure factor of one chunk of EW_CH charged particles: thread t
@@ -1863,6 +1878,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_rho(EwaldK e, const double*
                                                          const int4* __restrict__ segs,
                                                          const int* __restrict__ seg_k0,
                                                          double2* __restrict__ rho_part) {
+#pragma clang fp contract(fast)  // FP64 sums within 1e-12: fused multiply-adds
   extern __shared__ double2 ew_lds[];  // [EW_CH][nex + ney + nez + EW_SEG]
   const int row = e.nex + e.ney + e.nez + EW_SEG;
   const int j0 = blockIdx.x * EW_CH;
@@ -1899,8 +1915,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_rho(EwaldK e, const double*
     const double2 a = cmul(yv, zv);
 #pragma unroll
     for (int u = 0; u < EW_SEG; ++u) {
-      acc[u].x += xv[u].x * a.x - xv[u].y * a.y;
-      acc[u].y += xv[u].x * a.y + xv[u].y * a.x;
+      acc[u].x = fma(xv[u].x, a.x, acc[u].x);
+      acc[u].x = fma(-xv[u].y, a.y, acc[u].x);
+      acc[u].y = fma(xv[u].x, a.y, acc[u].y);
+      acc[u].y = fma(xv[u].y, a.x, acc[u].y);
     }
   }
   const int k0 = seg_k0[sg];
@@ -1921,6 +1939,7 @@ __global__ __launch_bounds__(64 * EW_SUMW) void k_ewald_sum(EwaldK e,
                                                             double* __restrict__ uk_part,
                                                             const double* __restrict__ mz_part,
                                                             double* __restrict__ mz) {
+#pragma clang fp contract(fast)  // FP64 sums within 1e-12: fused multiply-adds
   __shared__ double2 sh[EW_SUMW][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int kk = blockIdx.x * 64 + lane;
@@ -1985,6 +2004,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const doubl
                                                            const int* __restrict__ seg_k0,
                                                            const double2* __restrict__ grho,
                                                            double* __restrict__ fk_part) {
+#pragma clang fp contract(fast)  // FP64 sums within 1e-12: fused multiply-adds
   extern __shared__ double2 ew_xs[];  // [nex][64], then [segs_per_kgroup][EW_SEG]
   __shared__ double sh[4][64][3];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -2071,6 +2091,7 @@ __global__ __launch_bounds__(MFA_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
     EwaldK e, const double2* __restrict__ tx, const double2* __restrict__ ty,
     const double2* __restrict__ tz, const int4* __restrict__ rows, const int* __restrict__ row_k0,
     int nrows, int rows_per_group, const double2* __restrict__ grho, double* __restrict__ fk_part) {
+#pragma clang fp contract(fast)  // FP64 sums within 1e-12: fused multiply-adds
   constexpr int NEX = 2 * NXM + 1;
   constexpr int PB = MFA_BLOCK * EW_PPL;  // particles per block
   extern __shared__ double2 ew_g[];       // [rows_per_group][NEX]
@@ -2337,54 +2358,13 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_dpd_vv_apply(
 // ---------------------------------------------------------------------------
 // Host side
 
-static KParams make_kparams(const mfa_params& p) {
-  KParams k;
-  memset(&k, 0, sizeof k);
-  for (int d = 0; d < 3; ++d) {
-    k.L[d] = p.boundary[d];
-    k.invL[d] = p.inv_boundary[d];
-    k.inv_r_bin[d] = p.inv_r_bin[d];
-    k.r_bin[d] = p.r_bin[d];
-    k.nb[d] = p.n_bin[d];
-    k.va[d] = p.va_spring_twall[d];
-  }
-  k.dt = p.dt;
-  k.dt_2 = p.dt_2;
-  k.half_dt = 0.5 * p.dt;      // 0.5*dt   (verlet_positions.f90:169)
-  k.half_dt_2 = 0.5 * p.dt_2;  // 0.5*dt_2 (verlet_positions.f90:154)
-  for (int t = 0; t < MFA_NTYPE; ++t) {
-    k.inv_mass_type[t] = p.inv_mass_type[t];
-    k.mass_type[t] = p.mass_type[t];
-    k.sigma_wall[t] = p.sigma_wall[t];
-    k.a_wall[t] = p.a_wall[t];
-    for (int u = 0; u < MFA_NTYPE; ++u) {
-      k.epsil[t][u] = p.epsil[t][u];
-      k.sigma_2[t][u] = p.sigma_2[t][u];
-      k.range_2[t][u] = p.range_2[t][u];
-      k.inv_range_2[t][u] = p.inv_range_2[t][u];
-      k.e_shift[t][u] = p.e_shift[t][u];
-      k.list_2[t][u] = p.list_2[t][u];
-    }
-  }
-  k.r_cut_dpd_2 = p.r_cut_dpd_2;
-  k.friction1 = p.friction1;
-  k.sig = p.sig;
-  k.k_chain = p.k_chain;
-  k.inv_r_chain_2 = p.inv_r_chain_2;
-  k.a_w = p.a_w;
-  k.z_space_wall = p.z_space_wall;
+// The skin-dependent kernel constants of a replica: the list criterion, the
+// skin test and the FP32 decision margins (which bound coordinates by the
+// list radius and the skin).
+static void kparams_skin(KParams& k, const mfa_params& p) {
+  for (int t = 0; t < MFA_NTYPE; ++t)
+    for (int u = 0; u < MFA_NTYPE; ++u) k.list_2[t][u] = p.list_2[t][u];
   k.skin_half_2 = p.skin_half_2;
-  k.fso_r2_init = p.fso_r2_init;
-  k.fso_time = p.fso_time;
-  k.coulomb = p.coulomb;
-  k.bending = p.bending;
-  k.k_bend = p.k_bend;
-  k.alpha_eq = p.alpha_eq;
-  for (int t = 0; t < MFA_NTYPE; ++t) k.q_type[t] = p.charge[t];
-  k.alpha_c = p.alpha_c;
-  k.sqrt_alpha = p.sqrt_alpha;
-  k.inv_sqrt_pi = p.inv_sqrt_pi_real;
-  k.rc_coul_2 = p.coulomb ? p.rc_coul_2 : 0.0;
   {
     // FP32 separation from FP32 positions: the two coordinates, the raw
     // difference (|d| <= L before the image shift) and the FP32 box edge each
@@ -2417,6 +2397,67 @@ static KParams make_kparams(const mfa_params& p) {
     k.lf_margin = (float)(4.0 * (2.0 * sqrt(3.0 * l2max) * dsep + 3.0 * dsep * dsep +
                                  6.0 * l2max * u));
   }
+}
+
+// List capacity of a replica: the expected full-list count at its mean
+// density, with room for fluctuations.
+static int list_cap_of(const mfa_params& p) {
+  const double dens = (double)p.n_part / (p.boundary[0] * p.boundary[1] * p.boundary[2]);
+  double rl2 = 0.0;
+  for (int t = 0; t < MFA_NTYPE; ++t)
+    for (int u = 0; u < MFA_NTYPE; ++u) rl2 = p.list_2[t][u] > rl2 ? p.list_2[t][u] : rl2;
+  const double rl = sqrt(rl2);
+  const double expect = dens * 4.18879020478639 * rl * rl * rl;
+  return (((int)(2.0 * expect + 48.0)) + 31) / 32 * 32;
+}
+
+static KParams make_kparams(const mfa_params& p) {
+  KParams k;
+  memset(&k, 0, sizeof k);
+  for (int d = 0; d < 3; ++d) {
+    k.L[d] = p.boundary[d];
+    k.invL[d] = p.inv_boundary[d];
+    k.inv_r_bin[d] = p.inv_r_bin[d];
+    k.r_bin[d] = p.r_bin[d];
+    k.nb[d] = p.n_bin[d];
+    k.va[d] = p.va_spring_twall[d];
+  }
+  k.dt = p.dt;
+  k.dt_2 = p.dt_2;
+  k.half_dt = 0.5 * p.dt;      // 0.5*dt   (verlet_positions.f90:169)
+  k.half_dt_2 = 0.5 * p.dt_2;  // 0.5*dt_2 (verlet_positions.f90:154)
+  for (int t = 0; t < MFA_NTYPE; ++t) {
+    k.inv_mass_type[t] = p.inv_mass_type[t];
+    k.mass_type[t] = p.mass_type[t];
+    k.sigma_wall[t] = p.sigma_wall[t];
+    k.a_wall[t] = p.a_wall[t];
+    for (int u = 0; u < MFA_NTYPE; ++u) {
+      k.epsil[t][u] = p.epsil[t][u];
+      k.sigma_2[t][u] = p.sigma_2[t][u];
+      k.range_2[t][u] = p.range_2[t][u];
+      k.inv_range_2[t][u] = p.inv_range_2[t][u];
+      k.e_shift[t][u] = p.e_shift[t][u];
+    }
+  }
+  k.r_cut_dpd_2 = p.r_cut_dpd_2;
+  k.friction1 = p.friction1;
+  k.sig = p.sig;
+  k.k_chain = p.k_chain;
+  k.inv_r_chain_2 = p.inv_r_chain_2;
+  k.a_w = p.a_w;
+  k.z_space_wall = p.z_space_wall;
+  k.fso_r2_init = p.fso_r2_init;
+  k.fso_time = p.fso_time;
+  k.coulomb = p.coulomb;
+  k.bending = p.bending;
+  k.k_bend = p.k_bend;
+  k.alpha_eq = p.alpha_eq;
+  for (int t = 0; t < MFA_NTYPE; ++t) k.q_type[t] = p.charge[t];
+  k.alpha_c = p.alpha_c;
+  k.sqrt_alpha = p.sqrt_alpha;
+  k.inv_sqrt_pi = p.inv_sqrt_pi_real;
+  k.rc_coul_2 = p.coulomb ? p.rc_coul_2 : 0.0;
+  kparams_skin(k, p);
   k.n_cells = p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
   k.n = p.n_part;
   k.n_mon = p.n_mon > 0 ? p.n_mon : 1;
@@ -2440,7 +2481,7 @@ static KParams make_kparams(const mfa_params& p) {
 }
 
 static int grid_of(int n) { return (n + MFA_BLOCK - 1) / MFA_BLOCK; }
-static int launch_ewald(mfa_ctx* c, int add_dip, hipEvent_t* ev = nullptr);
+static int launch_ewald(mfa_ctx* c, int add_dip);
 
 
 
@@ -3077,23 +3118,32 @@ static bool ewald_active(const mfa_ctx* c) { return c->ewald && c->ew.e.nq > 0 &
 
 // k-space forces (and, with add_dip, the slab dipole forces) added to F.
 // Shapes come from the checked plan (ewald_shape).
-// With ev (six events), the five kernels are bracketed by them for timing.
-static int launch_ewald(mfa_ctx* c, int add_dip, hipEvent_t* ev) {
-  if (!ewald_active(c)) return MFA_OK;
+// The five kernels of the k-space stage, one function each (the timing
+// helper launches them one at a time).  Shapes come from the checked plan
+// (ewald_shape).
+static int ew_tables(mfa_ctx* c) {
+  const EwaldK& e = c->ew.e;
+  hipLaunchKernelGGL(k_ewald_tables, dim3(e.n_mz, 3), dim3(MFA_BLOCK), 0, c->stream, e, c->r,
+                     c->d_qidx, c->d_qval, c->d_tx, c->d_ty, c->d_tz, c->d_mz_part);
+  return MFA_OK;
+}
+static int ew_rho(mfa_ctx* c) {
   const EwPlan& w = c->ew;
   const EwaldK& e = w.e;
-  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-  hipLaunchKernelGGL(k_ewald_tables, dim3(e.n_mz, e.nex + e.ney + e.nez), dim3(MFA_BLOCK), 0,
-                     c->stream, e, c->r, c->d_qidx, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
-                     c->d_mz_part);
-  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   hipLaunchKernelGGL(k_ewald_rho, dim3(e.n_chunks, (e.nsegs + MFA_BLOCK - 1) / MFA_BLOCK),
                      dim3(MFA_BLOCK), w.lds_rho, c->stream, e, c->d_qval, c->d_tx, c->d_ty, c->d_tz,
                      c->d_runs, c->d_run_k0, c->d_rho_part);
-  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  hipLaunchKernelGGL(k_ewald_sum, dim3(w.sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, e,
+  return MFA_OK;
+}
+static int ew_sum(mfa_ctx* c) {
+  const EwPlan& w = c->ew;
+  hipLaunchKernelGGL(k_ewald_sum, dim3(w.sum_blocks), dim3(64 * EW_SUMW), 0, c->stream, w.e,
                      c->d_rho_part, c->d_kvec, c->d_grho, c->d_uk_part, c->d_mz_part, c->d_mz);
-  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+  return MFA_OK;
+}
+static int ew_force(mfa_ctx* c) {
+  const EwPlan& w = c->ew;
+  const EwaldK& e = w.e;
   if (w.rows_kernel) {
 #define EW_ROWS_CASE(X)                                                                              \
   case X:                                                                                            \
@@ -3112,11 +3162,27 @@ static int launch_ewald(mfa_ctx* c, int add_dip, hipEvent_t* ev) {
                        w.lds_force, c->stream, e, c->d_tx, c->d_ty, c->d_tz, c->d_runs, c->d_run_k0,
                        c->d_grho, c->d_fk_part);
   }
-  if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
+  return MFA_OK;
+}
+static int ew_apply(mfa_ctx* c, int add_dip) {
+  const EwPlan& w = c->ew;
+  const EwaldK& e = w.e;
   hipLaunchKernelGGL(k_ewald_apply, dim3((3 * e.nq + MFA_BLOCK - 1) / MFA_BLOCK), dim3(MFA_BLOCK), 0,
                      c->stream, e, c->d_qidx, c->d_qval, c->d_fk_part, w.n_kgroups, c->d_mz,
                      w.inv_V, 1, add_dip, c->f);
-  if (ev) HIPCHK(c, hipEventRecord(ev[5], c->stream));
+  return MFA_OK;
+}
+static int ew_apply_stage(mfa_ctx* c) { return ew_apply(c, c->p.symmetry == 0 ? 1 : 0); }
+
+// k-space forces (and, with add_dip, the slab dipole forces) added to F.
+static int launch_ewald(mfa_ctx* c, int add_dip) {
+  if (!ewald_active(c)) return MFA_OK;
+  int rc = ew_tables(c);
+  if (!rc) rc = ew_rho(c);
+  if (!rc) rc = ew_sum(c);
+  if (!rc) rc = ew_force(c);
+  if (!rc) rc = ew_apply(c, add_dip);
+  if (rc) return rc;
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
 }
@@ -3229,13 +3295,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
     const double vol = p.boundary[0] * p.boundary[1] * p.boundary[2];
     const double dens = (double)p.n_part / vol;
     if (list_capacity <= 0) {
-      // expected full-list count at the mean density, with room for fluctuations
-      double rl2 = 0.0;
-      for (int t = 0; t < MFA_NTYPE; ++t)
-        for (int u = 0; u < MFA_NTYPE; ++u) rl2 = p.list_2[t][u] > rl2 ? p.list_2[t][u] : rl2;
-      const double rl = sqrt(rl2);
-      const double expect = dens * 4.18879020478639 * rl * rl * rl;
-      const int cq = (((int)(2.0 * expect + 48.0)) + 31) / 32 * 32;
+      const int cq = list_cap_of(p);
       cap = cq > cap ? cq : cap;
     }
     // cell capacity: twice the mean occupancy plus 32 (the reference allows 300)
@@ -3268,6 +3328,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
     c->lb_octants = nc > 0 && np >= (int64_t)LB_MIN_OCTANT * nc;
   }
   c->cap = cap;
+  c->cap_fixed = list_capacity > 0;
   c->cell_cap = cell_cap;
   c->n = (int)ntot;
   c->n_cells = (int)ctot;
@@ -4202,6 +4263,78 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   return check_flags(c);
 }
 
+extern "C" int32_t mfa_set_skin(mfa_ctx* c, double skin) {
+  if (!c) return MFA_ERR_ARG;
+  std::vector<mfa_params> ps = c->ps;
+  int cap = c->cap_fixed ? c->cap : 0;
+  for (size_t q = 0; q < ps.size(); ++q) {
+    if (mfa_params_set_skin(&ps[q], skin) != MFA_OK)
+      return set_err(c, MFA_ERR_ARG, "replica %zu: %s", q, mfa_last_error(nullptr));
+    if (!c->cap_fixed) {
+      const int cq = list_cap_of(ps[q]);
+      cap = cq > cap ? cq : cap;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (cap > c->cap) {  // a longer list: grow its buffer (its contents are rebuilt)
+    unsigned* nl = nullptr;
+    const size_t bytes = (size_t)c->n * (size_t)cap * sizeof(unsigned);
+    if (hipMalloc((void**)&nl, bytes) != hipSuccess)
+      return set_err(c, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (list) failed", bytes);
+    hipFree(c->nl);
+    c->nl = nl;
+    c->cap = cap;
+  }
+  for (size_t q = 0; q < ps.size(); ++q) kparams_skin(c->ks[q], ps[q]);
+  HIPCHK(c, hipMemcpy(c->d_kp, c->ks.data(), c->ks.size() * sizeof(KParams), hipMemcpyHostToDevice));
+  c->ps = ps;
+  c->p = ps[0];
+  c->need_rebuild = true;  // a new list (and skin reference) before the next force pass
+  drop_graph(c);
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_autotune_skin(mfa_ctx* c, int64_t first_step, int32_t n_steps,
+                                     int32_t n_cand, const double* skins, double* best_skin,
+                                     double* ms_per_step) {
+  if (!c || n_steps < 1 || n_cand < 1 || n_cand > 64 || !skins) return MFA_ERR_ARG;
+  const double keep = c->p.skin;
+  double best = 1e300, best_s = keep;
+  int64_t step = first_step;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int rc = MFA_OK;
+  for (int q = 0; q < n_cand && rc == MFA_OK; ++q) {
+    if (mfa_set_skin(c, skins[q]) != MFA_OK) continue;  // too wide for the cells: skipped
+    // the candidate's first steps (untimed; they capture its step block)
+    rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
+    step += c->graph_steps + 1;
+    if (rc) break;
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    rc = mfa_run_steps(c, step, n_steps, nullptr);
+    step += n_steps;
+    if (rc) break;
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_s = skins[q];
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best > 1e299) return set_err(c, MFA_ERR_ARG, "no candidate skin fits the cells");
+  rc = mfa_set_skin(c, best_s);
+  if (rc) return rc;
+  if (best_skin) *best_skin = best_s;
+  if (ms_per_step) *ms_per_step = best / (double)n_steps;
+  return check_flags(c);
+}
+
 // Average duration of `reps` launches of `launch` on the context's stream,
 // captured in one graph when graph steps are on, bracketed by two HIP events
 // (a first untimed round warms up).
@@ -4368,9 +4501,10 @@ extern "C" int32_t mfa_time_ewald(mfa_ctx* c, int32_t reps, double* ms_avg) {
 }
 
 // Average durations (ms) of the five kernels of the k-space stage (tables,
-// structure factor, its sum, forces, apply), each bracketed by HIP events on
-// the context's stream, over `reps` launches at the current state; F is put
-// back afterwards.
+// structure factor, its sum, forces, apply), each timed like mfa_time_force:
+// `reps` back-to-back launches of that kernel alone (captured in one graph
+// when graph steps are on) between two HIP events, after one full stage has
+// filled its inputs; F is put back afterwards.
 extern "C" int32_t mfa_time_ewald_parts(mfa_ctx* c, int32_t reps, double* ms) {
   if (!c || !ms || reps < 1 || reps > 4096) return MFA_ERR_ARG;
   if (!ewald_active(c)) return set_err(c, MFA_ERR_ARG, "no k-space Ewald stage (no charges)");
@@ -4378,41 +4512,17 @@ extern "C" int32_t mfa_time_ewald_parts(mfa_ctx* c, int32_t reps, double* ms) {
   double* keep = nullptr;
   if (hipMalloc(&keep, fb) != hipSuccess)
     return set_err(c, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (F copy) failed", fb);
-  hipEvent_t ev[6];
-  int made = 0;
   int rc = MFA_OK;
-  for (; made < 6; ++made)
-    if (hipEventCreate(&ev[made]) != hipSuccess) {
-      rc = set_err(c, MFA_ERR_HIP, "event create failed");
-      break;
-    }
-  if (!rc && hipMemcpyAsync(keep, c->f, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+  if (hipMemcpyAsync(keep, c->f, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
     rc = set_err(c, MFA_ERR_HIP, "F copy failed");
-  double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int it = 0; it < reps + 1 && !rc; ++it) {  // the first one warms up
-    rc = launch_ewald(c, c->p.symmetry == 0 ? 1 : 0, ev);
-    if (rc) break;
-    if (hipEventSynchronize(ev[5]) != hipSuccess) {
-      rc = set_err(c, MFA_ERR_HIP, "event synchronize failed");
-      break;
-    }
-    for (int q = 0; q < 5 && it > 0; ++q) {
-      float t = 0.f;
-      if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) {
-        rc = set_err(c, MFA_ERR_HIP, "event elapsed time failed");
-        break;
-      }
-      tot[q] += t;
-    }
-  }
+  if (!rc) rc = launch_ewald_stage(c);
+  int (*parts[5])(mfa_ctx*) = {ew_tables, ew_rho, ew_sum, ew_force, ew_apply_stage};
+  for (int q = 0; q < 5 && !rc; ++q) rc = time_launches(c, reps, parts[q], &ms[q]);
   if (hipMemcpyAsync(c->f, keep, fb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess && !rc)
     rc = set_err(c, MFA_ERR_HIP, "F restore failed");
   if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, MFA_ERR_HIP, "sync failed");
-  for (int q = 0; q < made; ++q) hipEventDestroy(ev[q]);
   hipFree(keep);
-  if (rc) return rc;
-  for (int q = 0; q < 5; ++q) ms[q] = tot[q] / reps;
-  return check_flags(c);
+  return rc ? rc : check_flags(c);
 }
 
 extern "C" int32_t mfa_copy_probe(int64_t bytes, double* ms) {

@@ -38,6 +38,36 @@ extern "C" int32_t mfa_abi_sizes(int32_t* si, int32_t* sp, int32_t* so) {
   return MFA_OK;
 }
 
+// Skin (init_params.f90:800-808), check_skin's threshold (check_skin.f90:18)
+// and the Verlet-list criterion (binning.f90:101-105), from p's ranges and
+// r_cut_max; rc2 > 0: a real-space Coulomb cut-off of its own widens the list.
+static void set_skin_fields(mfa_params* p, double skin, double rc2) {
+  p->skin = skin;
+  p->skin_2 = (p->r_cut_max + p->skin) * (p->r_cut_max + p->skin) - p->r_cut_max * p->r_cut_max;
+  p->skin_half_2 = (p->skin / 2.0) * (p->skin / 2.0);
+  for (int i = 0; i < MFA_NTYPE; ++i)
+    for (int j = 0; j < MFA_NTYPE; ++j)
+      p->list_2[i][j] = p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : p->range_2[i][j] + p->skin_2;
+  if (rc2 > 0.0) {
+    const double lc = rc2 + p->skin_2;
+    for (int i = 0; i < MFA_NTYPE; ++i)
+      for (int j = 0; j < MFA_NTYPE; ++j) p->list_2[i][j] = p->list_2[i][j] > lc ? p->list_2[i][j] : lc;
+  }
+}
+
+extern "C" int32_t mfa_params_set_skin(mfa_params* p, double skin) {
+  if (!p || !(skin > 0.0)) return derive_fail("the list skin must be positive");
+  mfa_params q = *p;
+  set_skin_fields(&q, skin, q.coulomb && q.rc_coul_2 > 0.0 ? q.rc_coul_2 : 0.0);
+  for (int d = 0; d < 3; ++d)
+    for (int i = 0; i < MFA_NTYPE; ++i)
+      for (int j = 0; j < MFA_NTYPE; ++j)
+        if (q.list_2[i][j] > q.r_bin[d] * q.r_bin[d])
+          return derive_fail("binning cell narrower than the list radius");
+  *p = q;
+  return MFA_OK;
+}
+
 extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   if (!in || !p) return MFA_ERR_ARG;
   memset(p, 0, sizeof(*p));
@@ -248,19 +278,9 @@ extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < nt; ++j)
       p->inv_range_2[i][j] = range_2[i][j] != 0.0 ? 1.0 / range_2[i][j] : 0.0;
-  // Skin (init_params.f90:800-808) and check_skin's threshold (check_skin.f90:18).
-  p->skin = in->skin > 0.0 ? in->skin : (double)0.4f;
-  p->skin_2 = (p->r_cut_max + p->skin) * (p->r_cut_max + p->skin) - p->r_cut_max * p->r_cut_max;
-  p->skin_half_2 = (p->skin / 2.0) * (p->skin / 2.0);
-  // Verlet-list criterion (binning.f90:101-105).
-  for (int i = 0; i < nt; ++i)
-    for (int j = 0; j < nt; ++j)
-      p->list_2[i][j] = p->dpd_cut_off_on ? p->r_cut_max_2 + p->skin_2 : range_2[i][j] + p->skin_2;
-  if (charged_any && !in->ewald_compat && in->rc_coul > 0.0) {
-    const double lc = in->rc_coul * in->rc_coul + p->skin_2;
-    for (int i = 0; i < nt; ++i)
-      for (int j = 0; j < nt; ++j) p->list_2[i][j] = p->list_2[i][j] > lc ? p->list_2[i][j] : lc;
-  }
+  set_skin_fields(p, in->skin > 0.0 ? in->skin : (double)0.4f,
+                  charged_any && !in->ewald_compat && in->rc_coul > 0.0 ? in->rc_coul * in->rc_coul
+                                                                         : 0.0);
   // DPD (init_params.f90:845).
   p->friction1 = in->friction[0];
   p->sig = sqrt(2.0 * in->temp * in->friction[0] / in->dt);

@@ -175,6 +175,9 @@ SIGNATURES = {
     "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_force_step": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_ewald_parts": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_params_set_skin": (C.c_int32, [C.c_void_p, C.c_double]),
+    "mfa_set_skin": (C.c_int32, [_P, C.c_double]),
+    "mfa_autotune_skin": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_rebuild": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
@@ -243,6 +246,17 @@ def derive(inputs):
     return p
 
 
+def with_skin(params, skin):
+    """A copy of derived params with another list skin (mfa_params_set_skin)."""
+    lib = load()
+    q = Params.from_buffer_copy(params)
+    rc = lib.mfa_params_set_skin(C.byref(q), skin)
+    if rc:
+        raise MfaError("mfa_params_set_skin: %s (%s)" % (lib.mfa_last_error(None).decode(), ERRORS[rc]),
+                       rc)
+    return q
+
+
 EWPLAN_FIELDS = ("k_vectors", "segments", "rows", "nx_max", "ny_max", "nz_max", "row_kernel",
                  "rows_per_group", "force_groups", "force_blocks", "force_grid", "lds_rho",
                  "lds_force", "force_partials", "force_partials_cap", "chunks")
@@ -280,6 +294,19 @@ class Context:
         self._chk(self.lib.mfa_autotune(self.h, first_step, n_steps, C.byref(t), C.byref(ms)),
                   "mfa_autotune")
         return t.value, ms.value
+
+    def set_skin(self, skin):
+        self._chk(self.lib.mfa_set_skin(self.h, skin), "mfa_set_skin")
+
+    def autotune_skin(self, first_step, n_steps, skins):
+        """The fastest of `skins` over n_steps whole steps each (advances the
+        state); the context keeps it.  Returns (skin, ms per step)."""
+        cand = np.ascontiguousarray(skins, dtype=np.float64)
+        best, ms = C.c_double(), C.c_double()
+        self._chk(self.lib.mfa_autotune_skin(self.h, first_step, n_steps, len(cand),
+                                             cand.ctypes.data_as(_dp), C.byref(best), C.byref(ms)),
+                  "mfa_autotune_skin")
+        return best.value, ms.value
 
     def __init__(self, params, precision=MFA_FP64, list_capacity=0):
         self.lib = load()

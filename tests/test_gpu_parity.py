@@ -434,3 +434,53 @@ def test_time_force_step_leaves_the_state():
     assert ot["n_rebuilds"] == op["n_rebuilds"]
     timed.close()
     plain.close()
+
+
+@pytest.mark.parametrize("case", ["melt_small", "brush_small"])
+def test_set_skin_pairs_and_trajectory(case):
+    """mfa_set_skin on a running context: the pair set equals the oracle's
+    at the new skin, bit for bit, and an FP64 trajectory that changes skin
+    midway stays within 1e-9 of the oracle's at the old skin (the forces do
+    not depend on the skin; only the list's order and length do)."""
+    inputs, p, r, v, t, a = make_case(case)
+    skin2 = 0.45
+    y = lib.Inputs.from_buffer_copy(inputs)
+    y.skin = skin2
+    s2 = oracle_sys(y, r, v, a, t)
+    s2.binning()
+    ref_pairs = s2.get_pairs()
+    ctx = gpu_ctx(p, r, v, a, t)
+    ctx.set_skin(skin2)
+    ctx.binning()
+    gp = ctx.get_pairs()
+    gp = gp[np.lexsort((gp[:, 1], gp[:, 0]))]
+    assert gp.shape == ref_pairs.shape and np.array_equal(gp, ref_pairs)
+    ctx.close()
+    s = oracle_sys(inputs, r, v, a, t)
+    ref = s.run_steps(first_step=1, n=30)
+    st_ref = s.get_state()
+    ctx = gpu_ctx(p, r, v, a, t)
+    ctx.run_steps(1, 10)
+    ctx.set_skin(skin2)
+    obs = ctx.run_steps(11, 20)
+    st = ctx.get_state()
+    dr = np.abs(st["r"] - st_ref["r"]).max()
+    print("%s: skin %.2f -> %.2f, dr %.2e" % (case, p.skin, skin2, dr), flush=True)
+    assert dr < 1e-9 and np.array_equal(st["mic"], st_ref["mic"])
+    assert rel(obs["t_kin"], ref["t_kin"]) < 1e-9
+    ctx.close()
+
+
+def test_autotune_skin_keeps_a_candidate():
+    """The skin autotuner keeps one of the candidates that fit the cells
+    (a candidate wider than the cells is skipped) and the context runs on."""
+    inputs, p, r, v, t, a = make_case("melt_small")
+    ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    cand = [0.3, 0.4, 10.0]
+    skin, ms = ctx.autotune_skin(1, 40, cand)
+    print("autotune_skin: %.2f (%.4f ms per step)" % (skin, ms), flush=True)
+    assert skin in cand[:2] and ms > 0.0
+    first = 1 + 2 * (40 + 17)
+    obs = ctx.run_steps(first, 20)
+    assert np.isfinite(obs["t_kin"]) and obs["t_kin"] > 0.0
+    ctx.close()

@@ -552,6 +552,37 @@ def _plan_invariants(plan, n_charged):
         assert plan["lds_force"] == plan["rows_per_group"] * (2 * plan["nx_max"] + 1) * 16
 
 
+@pytest.mark.parametrize("name", ["brush9k", "melt100k", "pe_brush27k"])
+def test_params_set_skin_matches_derive(name):
+    """mfa_params_set_skin on derived params gives the params mfa_derive
+    gives with that skin, byte for byte (list criterion with and without
+    DPD_CUT_OFF, and with a real-space Coulomb cut-off of its own)."""
+    x = configs.WORKLOADS[name]["make"]()
+    base = lib.derive(x)
+    for skin in (0.2, 0.35, 0.5):
+        y = lib.Inputs.from_buffer_copy(x)
+        y.skin = skin
+        try:
+            want = lib.derive(y)
+        except lib.MfaError:
+            with pytest.raises(lib.MfaError):
+                lib.with_skin(base, skin)
+            continue
+        got = lib.with_skin(base, skin)
+        assert bytes(got) == bytes(want), (name, skin)
+
+
+def test_params_set_skin_refuses_lists_wider_than_the_cells():
+    x = configs.WORKLOADS["melt100k"]["make"]()
+    p = lib.derive(x)
+    reach = max(p.r_bin) - np.sqrt(max(max(row) for row in p.range_2))
+    with pytest.raises(lib.MfaError) as ei:
+        lib.with_skin(p, reach + 0.05)
+    assert "narrower than the list radius" in str(ei.value)
+    q = lib.with_skin(p, min(p.r_bin) - np.sqrt(max(max(row) for row in p.range_2)) - 0.01)
+    assert max(max(row) for row in q.list_2) <= min(p.r_bin) ** 2
+
+
 def test_ewald_plan_pe_brush27k_grid():
     w = configs.WORKLOADS["pe_brush27k"]
     x = w["make"]()
