@@ -1623,7 +1623,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
         const float inv_r_2 = inv_r * inv_r;
         float s = 0.0f;  // F_ij = s * (dx, dy, dz)
-        if (in_lj) {
+        // a pair with epsilon 0 (the ideal DPD solvent) has no LJ term: in
+        // FP64 it is exactly 0, in FP32 r^-12 r^-2 overflows once r < ~0.002
+        // (such beads overlap freely) and 0 * inf would be NaN
+        if (in_lj && tb.y != 0.0f) {
           const float r61 = tb.z * inv_r_2;
           const float r6 = r61 * r61 * r61;
           const float r12 = r6 * r6;

@@ -11,7 +11,9 @@ intra_molec.f90:36-44):
   - two particles at one point, or on a wall       -> MFA_ERR_NAN
   - a particle slightly below the bottom wall and a bond stretched beyond R0
     compute finite forces as the reference does (NO_WARNS: no push-back;
-    the FENE argument is clamped at 0.98), equal to the oracle's.
+    the FENE argument is clamped at 0.98), equal to the oracle's;
+  - two ideal-solvent beads (epsilon 0, DPD forces only) 1e-3 apart, which
+    such beads reach in long runs: finite forces equal to the oracle's.
 After an error, a new state makes the context usable again.
 """
 import numpy as np
@@ -105,3 +107,32 @@ def test_below_the_wall_and_stretched_bond_match_oracle():
     print("below-wall bead and stretched bond: max err %.2e" % per, flush=True)
     assert per < 1e-12
     assert abs(obs["v_intra_molec"] - eref["v_intra_molec"]) <= 1e-12 * abs(eref["v_intra_molec"])
+
+
+@pytest.mark.parametrize("precision", [lib.MFA_FP64, lib.MFA_MIXED])
+def test_overlapping_ideal_solvent_beads_have_finite_forces(precision):
+    """Ideal solvent beads interact only through DPD, so they may overlap;
+    the LJ term with epsilon 0 is exactly zero in the reference's FP64.  In
+    FP32, r^-12 r^-2 overflows below r ~ 0.002, so the mixed form must skip
+    it rather than multiply by 0."""
+    inputs, p, r, v, t, a = make_case("brush_small")
+    solv = np.flatnonzero(t == 3)
+    i, j = int(solv[0]), int(solv[1])
+    r = r.copy()
+    r[j] = r[i] + np.array([1e-3, 0.0, 0.0])
+    s = oracle_sys(inputs, r, v, a, t)
+    s.binning()
+    fref, eref = s.forces(step=7)
+    assert np.all(np.isfinite(fref))
+    ctx = gpu_ctx(p, r, v, a, t, precision=precision)
+    obs = ctx.energies(step=7)
+    f = ctx.get_state()["f"]
+    ctx.close()
+    per, rms = force_error(f, fref)
+    print("solvent pair at 1e-3: max %.2e rms %.2e" % (per, rms), flush=True)
+    assert np.all(np.isfinite(f))
+    if precision == lib.MFA_FP64:
+        assert per < 1e-12
+    else:
+        assert rms < 1e-5 and per < 1e-3
+    assert np.isfinite(obs["v_fluid_fluid"])
