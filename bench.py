@@ -88,12 +88,13 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     step_force_ms = ctx.time_force_step(200)
     ewald_ms = ctx.time_ewald(200) if p.coulomb else None
     ewald_parts = ctx.time_ewald_parts(50) if p.coulomb else None
+    rebuild_idle = ctx.time_rebuild(20, idle=True)
     st = ctx.get_state()
     rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
     return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp, "skin": skin,
                      "ewald_ms": ewald_ms, "ewald_parts_ms": ewald_parts,
-                     "rebuild_ms": rebuild}, st
+                     "rebuild_ms": rebuild, "rebuild_idle_ms": rebuild_idle}, st
 
 
 def _fits(p, skin):
@@ -308,6 +309,7 @@ def bench_single(args, name):
         # force pass: 200 launches; a rebuild's three kernels: 20 rebuilds)
         "kernel_ms": {"force": tm_m["force_ms"], "force_in_step": tm_m["step_force_ms"],
                       **tm_m["rebuild_ms"],
+                      "rebuild_idle": sum(tm_m["rebuild_idle_ms"].values()),
                       "rebuild_per_step": sum(tm_m["rebuild_ms"].values()) *
                       obs_m["n_rebuilds"] / args.steps},
         "step_roofline_frac": value * STEP_BYTES_PER_PARTICLE / (HBM_PEAK_GBS * 1e9),
@@ -364,7 +366,11 @@ def bench_batch(args, n_rep=256):
         b.run_steps(first + args.warmup, args.steps, want_obs=False)
         dt_b = time.perf_counter() - t1
         force_ms = b.time_force_step(50)
-    log("batched: %.3f s for %d steps" % (dt_b, args.steps))
+        rb_idle = b.time_rebuild(20, idle=True)
+        b.get_replica(0)
+        rb_full = b.time_rebuild(10)  # resets every skin reference: after the timed steps
+    log("batched: %.3f s for %d steps; force pass %.3f ms; rebuild kernels idle %s, all replicas %s"
+        % (dt_b, args.steps, force_ms, rb_idle, rb_full))
     dt_s = 0.0
     for q, st in enumerate(relaxed):
         p = reps[q][0]
@@ -388,6 +394,7 @@ def bench_batch(args, n_rep=256):
     extra = {
         "sequential_value": ntot * args.steps / dt_s,
         "batched_over_sequential": dt_s / dt_b,
+        "kernel_ms": {"force_in_step": force_ms, "rebuild_idle": rb_idle, "rebuild_all": rb_full},
         "roofline": roofline(ntot, force_ms, None, kernel="k_force_mixed<%d,2,0>" % tpp,
                              bytes_per_particle=STEP_BYTES_PER_PARTICLE),
         "cpu_baseline": None,
@@ -403,8 +410,11 @@ def bench_adjoint(args):
     respect to (epsilon_LJ, sigma, k_FENE, R0, gamma_DPD) over a window of
     `steps` steps with checkpoints every 20 steps: the forward pass in the
     mixed form (FP32 forces) and in the FP64 form, the adjoint in FP64; the
-    FP64 gradient against central finite differences (relative step 1e-4)
-    of the FP64 forward pass."""
+    FP64 gradient against central finite differences of the FP64 forward
+    pass at relative steps 1e-4 (the workload's spec) and 1e-6.  Over a
+    200-step window J(theta) is smooth only on scales below ~1e-4 (pairs
+    cross the cut-offs that sigma moves), so the 1e-6 differences are the
+    ones that resolve the derivative."""
     w = configs.WORKLOADS["adjoint50k"]
     inputs = w["make"]()
     t0 = time.perf_counter()
@@ -435,18 +445,24 @@ def bench_adjoint(args):
     _, _, tfwd = window(theta, lib.MFA_MIXED, False)
     log("adjoint window: mixed forward %.3f s (forward only %.3f s), fp64 forward %.3f s" % (
         tm, tfwd, t64))
-    h = 1e-4
-    Gfd = np.zeros((2, 5))
-    for q in range(5):
-        tp, tq = theta.copy(), theta.copy()
-        tp[q] *= 1.0 + h
-        tq[q] *= 1.0 - h
-        Gfd[:, q] = (window(tp, lib.MFA_FP64, False)[0] - window(tq, lib.MFA_FP64, False)[0]) / (
-            tp[q] - tq[q])
-    rel = np.abs(G64 - Gfd) / np.maximum(np.abs(Gfd), 1e-3 * np.abs(Gfd).max(axis=1, keepdims=True))
-    relm = np.abs(Gm - G64) / np.maximum(np.abs(G64), 1e-3 * np.abs(G64).max(axis=1, keepdims=True))
-    log("J %s\nadjoint (fp64)\n%s\nFD (fp64)\n%s\nmax rel err %.3e; mixed-forward vs fp64 %.3e" % (
-        J64, G64, Gfd, rel.max(), relm.max()))
+    def fd(h):
+        G = np.zeros((2, 5))
+        for q in range(5):
+            tp, tq = theta.copy(), theta.copy()
+            tp[q] *= 1.0 + h
+            tq[q] *= 1.0 - h
+            G[:, q] = (window(tp, lib.MFA_FP64, False)[0] - window(tq, lib.MFA_FP64, False)[0]) / (
+                tp[q] - tq[q])
+        return G
+
+    def relerr(G, ref):
+        return np.abs(G - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max(axis=1, keepdims=True))
+
+    Gfd, Gfd6 = fd(1e-4), fd(1e-6)
+    rel, rel6, relm = relerr(G64, Gfd), relerr(G64, Gfd6), relerr(Gm, G64)
+    log("J %s\nadjoint (fp64)\n%s\nFD h=1e-4 (fp64)\n%s\nFD h=1e-6 (fp64)\n%s\nmax rel err vs h=1e-4 "
+        "%.3e, vs h=1e-6 %.3e; mixed-forward vs fp64 %.3e" % (
+            J64, G64, Gfd, Gfd6, rel.max(), rel6.max(), relm.max()))
     value = n * args.steps / tm
     config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
               "box": list(p.boundary), "dt": inputs.dt, "window": args.steps,
@@ -456,6 +472,7 @@ def bench_adjoint(args):
         "observables": {"height": J64[0], "p_wall": J64[1]},
         "grad_fp64": G64.tolist(), "grad_fd_fp64": Gfd.tolist(), "grad_mixed_forward": Gm.tolist(),
         "grad_vs_fd_max_rel": float(rel.max()),
+        "grad_fd_h1e-6_fp64": Gfd6.tolist(), "grad_vs_fd_h1e-6_max_rel": float(rel6.max()),
         "grad_mixed_vs_fp64_max_rel": float(relm.max()),
         "window_s_mixed": tm, "window_s_fp64": t64, "forward_only_s_mixed": tfwd,
         "roofline": None, "cpu_baseline": None,

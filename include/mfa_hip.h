@@ -336,6 +336,11 @@ int32_t mfa_time_ewald_parts(mfa_ctx* ctx, int32_t reps, double* ms /* [5] */);
  * Measurement helper with no reference counterpart. */
 int32_t mfa_time_rebuild(mfa_ctx* ctx, int32_t reps, double* ms /* [3] */);
 
+/* The same three kernels with no replica due for a rebuild: what a step
+ * without a rebuild pays for them (each returns at once).  Measurement
+ * helper with no reference counterpart. */
+int32_t mfa_time_rebuild_idle(mfa_ctx* ctx, int32_t reps, double* ms /* [3] */);
+
 /* Measurement only, no reference counterpart and no part of the step: a
  * device-to-device copy of `bytes` (read `bytes`, write `bytes`) timed with
  * HIP events, the calibration point for the HBM byte counters

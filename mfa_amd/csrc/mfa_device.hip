@@ -4445,7 +4445,17 @@ extern "C" int32_t mfa_time_force_step(mfa_ctx* c, int32_t reps, double* ms_avg)
 // sort, list build), each bracketed by HIP events on the context's stream,
 // over `reps` rebuilds forced at the current state (which they leave as it
 // is, the skin reference included).
+static int time_rebuild_impl(mfa_ctx* c, int32_t reps, double* ms, int forced);
+
 extern "C" int32_t mfa_time_rebuild(mfa_ctx* c, int32_t reps, double* ms) {
+  return time_rebuild_impl(c, reps, ms, 1);
+}
+
+extern "C" int32_t mfa_time_rebuild_idle(mfa_ctx* c, int32_t reps, double* ms) {
+  return time_rebuild_impl(c, reps, ms, 0);
+}
+
+static int time_rebuild_impl(mfa_ctx* c, int32_t reps, double* ms, int forced) {
   if (!c || !ms || reps < 1 || reps > 1024) return MFA_ERR_ARG;
   int rc = ensure_list(c);
   if (rc) return rc;
@@ -4453,10 +4463,11 @@ extern "C" int32_t mfa_time_rebuild(mfa_ctx* c, int32_t reps, double* ms) {
   for (int q = 0; q < 4; ++q) HIPCHK(c, hipEventCreate(&ev[q]));
   double tot[3] = {0.0, 0.0, 0.0};
   for (int it = 0; it < reps + 1 && rc == MFA_OK; ++it) {  // the first one warms up
-    hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream,
-                       c->n_cells, c->cell_count);
+    if (forced)
+      hipLaunchKernelGGL(k_zero_counts, dim3(grid_of(c->n_cells)), dim3(MFA_BLOCK), 0, c->stream,
+                         c->n_cells, c->cell_count);
     hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
-                       c->n_rep, (int)F_REBUILD, 1);
+                       c->n_rep, (int)F_REBUILD, forced);
     rc = launch_rebuild(c, ev);
     hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
                        c->n_rep, (int)F_REBUILD, 0);

@@ -215,6 +215,20 @@ module mfa_hip_bind
       real(c_double) :: r0(*), v(*)
       integer(c_int32_t) :: a_type(*)
     end function
+    integer(c_int32_t) function mfa_set_skin(ctx, skin) bind(C, name="mfa_set_skin")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: skin
+    end function
+    integer(c_int32_t) function mfa_autotune_skin(ctx, first_step, n_steps, n_cand, skins, &
+        best_skin, ms_per_step) bind(C, name="mfa_autotune_skin")
+      import :: c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: first_step
+      integer(c_int32_t), value :: n_steps, n_cand
+      real(c_double) :: skins(*)
+      real(c_double) :: best_skin, ms_per_step
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

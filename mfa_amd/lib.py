@@ -175,6 +175,7 @@ SIGNATURES = {
     "mfa_time_force": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_force_step": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_ewald_parts": (C.c_int32, [_P, C.c_int32, _dp]),
+    "mfa_time_rebuild_idle": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_params_set_skin": (C.c_int32, [C.c_void_p, C.c_double]),
     "mfa_set_skin": (C.c_int32, [_P, C.c_double]),
     "mfa_autotune_skin": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, _dp, _dp, _dp]),
@@ -369,10 +370,12 @@ class Context:
                   "mfa_time_ewald_parts")
         return dict(zip(("tables", "rho", "rho_sum", "force", "apply"), map(float, out)))
 
-    def time_rebuild(self, reps=20):
-        """Average ms of the cell fill, cell sort and list build of a rebuild."""
+    def time_rebuild(self, reps=20, idle=False):
+        """Average ms of the cell fill, cell sort and list build of a rebuild
+        (idle: of the same kernels with no rebuild due)."""
         out = np.zeros(3)
-        self._chk(self.lib.mfa_time_rebuild(self.h, reps, out.ctypes.data_as(_dp)), "mfa_time_rebuild")
+        fn = self.lib.mfa_time_rebuild_idle if idle else self.lib.mfa_time_rebuild
+        self._chk(fn(self.h, reps, out.ctypes.data_as(_dp)), "mfa_time_rebuild")
         return {"cell_fill": float(out[0]), "cell_sort": float(out[1]), "list_build": float(out[2])}
 
     def time_ewald(self, reps=100):
