@@ -81,6 +81,7 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
         skin, _ = ctx.autotune_skin(first_step, 200, cand)
         first_step += len(cand) * (max(graph_steps, 0) + 1 + 200)
     ctx.run_steps(first_step, warmup, want_obs=False)
+    tuning = ctx.tuning()
     t0 = time.perf_counter()
     obs = ctx.run_steps(first_step + warmup, steps, want_obs=True)
     dt = time.perf_counter() - t0
@@ -93,6 +94,7 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
     return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp, "skin": skin,
+                     "pair_queue": tuning["pair_queue"],
                      "ewald_ms": ewald_ms, "ewald_parts_ms": ewald_parts,
                      "rebuild_ms": rebuild, "rebuild_idle_ms": rebuild_idle}, st
 
@@ -299,6 +301,7 @@ def bench_single(args, name):
     config["threads_per_particle"] = {"mixed": tm_m["tpp"], "fp64": tm_d["tpp"]}
     config["list_skin"] = {"mixed": tm_m["skin"], "fp64": tm_d["skin"],
                            "chosen_by": "autotuner" if tune else "the workload's spec"}
+    config["pair_queue"] = {"mixed": tm_m["pair_queue"], "chosen_by": "autotuner"}
     extra = {
         "fp64_value": value_fp64,
         "mixed_over_fp64": value / value_fp64,
@@ -354,6 +357,7 @@ def bench_batch(args, n_rep=256):
         skin, ms_skin = b.autotune_skin(first, 100, cand)
         first += len(cand) * (args.graph_steps + 1 + 100)
         log("autotune: skin %.2f (%.3f ms per step)" % (skin, ms_skin))
+        pair_queue = b.tuning()["pair_queue"]
         ramp = configs.ramp_steps(reps[0][0])  # every replica has dt = 0.005
         t2 = time.perf_counter()
         if first <= ramp:
@@ -378,6 +382,7 @@ def bench_batch(args, n_rep=256):
             c.set_graph_steps(args.graph_steps)
             c.set_tpp(tpp)
             c.set_skin(skin)
+            c.set_pair_queue(pair_queue)
             c.set_state(st["r"], st["v"], st["a"], reps[q][3], st["mic"])
             c.run_steps(first, args.warmup, want_obs=False)
             t1 = time.perf_counter()
@@ -390,6 +395,7 @@ def bench_batch(args, n_rep=256):
               "chain_length": [20, 100], "lateral_edge": [8, 40], "lz": 12.5, "dt": 0.005,
               "solvent_density": 3.0, "threads_per_particle": tpp,
               "list_skin": {"value": skin, "chosen_by": "autotuner"},
+              "pair_queue": {"value": pair_queue, "chosen_by": "autotuner"},
               "parallelism": "single GPU, replicas batched"}
     extra = {
         "sequential_value": ntot * args.steps / dt_s,

@@ -389,10 +389,22 @@ int32_t mfa_obs_init(mfa_ctx* ctx, int32_t hist_dim);
 int32_t mfa_observe(mfa_ctx* ctx, const mfa_obs* e, int32_t coul_in_total);
 int32_t mfa_obs_read(mfa_ctx* ctx, mfa_obs_sums* out, double* dens_z, double* f_on_heads);
 
+/* Pair queue of the mixed force pass (DPD_CUT_OFF systems; no counterpart
+ * in the reference): on, each wave pushes the list candidates that interact
+ * into a queue in LDS and evaluates them 64 at a time, so no lane idles
+ * where few candidates interact; the forces are bit-identical either way
+ * (exact fixed-point sums).  Off by default; the autotuner chooses. */
+int32_t mfa_set_pair_queue(mfa_ctx* ctx, int32_t on);
+
+/* The context's current choices: threads per particle, pair queue, list skin
+ * (replica 0).  Any out-pointer may be NULL. */
+int32_t mfa_get_tuning(mfa_ctx* ctx, int32_t* tpp, int32_t* pair_queue, double* skin);
+
 /* Autotuner: for each number of threads per particle in {1, 2, 4, 8, 16}
- * runs n_steps from first_step (advancing the state, so call it during
- * warm-up), then times 20 force passes on the state they left; keeps the
- * fastest and returns it with its time per force pass (ms). */
+ * (and, for mixed DPD_CUT_OFF systems, with the pair queue off and on) runs
+ * n_steps from first_step (advancing the state, so call it during warm-up),
+ * then times 20 force passes on the state they left; keeps the fastest and
+ * returns it with its time per force pass (ms). */
 int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
                      double* ms_per_pass);
 

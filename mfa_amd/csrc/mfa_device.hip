@@ -144,7 +144,8 @@ struct mfa_ctx {
   bool lgv_terms;    // some replica runs THERMOSTAT 1: force kernels with XT & 2
   int n;
   int cap;
-  bool cap_fixed;  // the capacity the caller chose at creation (no growth)
+  bool cap_fixed;
+  bool pair_queue;  // mixed pass with DPD_CUT_OFF: compact the interacting pairs per wave  // the capacity the caller chose at creation (no growth)
   int tpp;  // threads per particle in the force pass
   int n_cells;
   bool need_rebuild;
@@ -1475,10 +1476,62 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
 }
 
+// The scalar s of one interacting pair of the mixed form, F_ij = s (dx, dy,
+// dz): LJ with the switch-on clamp (skipped for epsilon 0: there it is
+// exactly 0 in FP64, while in FP32 r^-12 r^-2 overflows below r ~ 0.002,
+// where such beads may sit, and 0 * inf would be NaN), DPD and the
+// real-space Coulomb term as the flags say (fluid_fluid.f90:94-126,
+// dpd_forces.f90:19-94, ewald_real.f90:15-32); with ENERGY the pair's half
+// of the energies and virial.
+template <int XT, bool ENERGY>
+__device__ __forceinline__ float pair_scalar(const KParams& k, const float4 tb, float inv_rc,
+                                             bool in_lj, bool dpd, bool coul, float r2raw,
+                                             float r2min, float dx, float dy, float dz,
+                                             const float4 vi4, const float4* __restrict__ vel4,
+                                             int js, uint32_t seed, int64_t step, int il, int jl,
+                                             int ti, int tj, float sigf, float gam, float sa,
+                                             float alf, float pref, double& e_ff, double& vir,
+                                             double& e_coul) {
+#pragma clang fp contract(fast)
+  const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
+  const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
+  const float inv_r_2 = inv_r * inv_r;
+  float s = 0.0f;
+  if (in_lj && tb.y != 0.0f) {
+    const float r61 = tb.z * inv_r_2;
+    const float r6 = r61 * r61 * r61;
+    const float r12 = r6 * r6;
+    const float rd = tb.y * (6.0f * r6 - 12.0f * r12) * inv_r_2;
+    s = -rd;
+    if (ENERGY) {
+      e_ff += 0.5 * (double)(tb.y * ((r12 - r6) - tb.w));
+      vir += 0.5 * (double)(-rd * r2raw);
+    }
+  }
+  if (dpd) {
+    const float xi_n = pair_normal_f32(seed, step, il, jl);
+    // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
+    const float wr = 1.0f - (r2 * inv_r) * inv_rc;
+    const float4 vj = vel4[js];
+    const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
+    s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
+  }
+  if ((XT & 1) && coul) {  // ewald_real.f90:15-32 with the clamped r_2
+    const float qq = (float)k.q_type[ti] * (float)k.q_type[tj];
+    const float rr = r2 * inv_r;
+    const float ec = erfcf(sa * rr);
+    s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
+    if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
+  }
+  return s;
+}
+
 // Candidates per lane whose loads the mixed force pass issues together.
 #ifndef FM_BATCH
 #define FM_BATCH 4
 #endif
+// Pairs a wave's queue holds (a round adds at most 64 to fewer than 64).
+#define FMQ_CAP 128
 
 // What the mixed force pass of a step that also integrates reads and writes
 // besides r, the view and the list: the next positions (the other position
@@ -1505,7 +1558,7 @@ struct FuseArgs {
 // and energy partials, 2 forces, then verlet_velocities of this step and
 // verlet_positions and check_skin of the next (k_fused_integrate's work) in
 // the same pass, which leaves F unwritten.
-template <int TPP, int MODE, int XT>
+template <int TPP, int MODE, int XT, bool QUEUE>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
     const float4* __restrict__ pos4, const float4* __restrict__ vel4, const int* __restrict__ perm,
@@ -1564,28 +1617,80 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const bool head_excl = DC && k.dpd_head_excl != 0;
   const bool coul_on = (XT & 1) && k.coulomb != 0;
   const bool coul_rc = (XT & 1) && k.rc_coul_2 > 0.0;  // own real-space cut-off
-  if (active && (mask & TM_PAIR)) {
-    const int cnt = nn[ss];
+  bool qbad = false;  // a queued pair's fixed-point terms out of range
+  if constexpr (QUEUE) {
+    // The same pairs, with the wave's work compacted: each lane tests its
+    // candidates (as below) and pushes those with a term to evaluate into
+    // the wave's queue in LDS; whenever 64 are queued (and at the end) the
+    // whole wave evaluates them, one pair per lane, and adds each pair's
+    // three fixed-point terms to its particle's sums in LDS with 64-bit
+    // integer atomics.  The terms are the direct loop's, and integer sums do
+    // not depend on order, so the forces equal the direct loop's bit for
+    // bit; only the lanes are kept busy where few candidates interact (a
+    // DPD cut-off below the list radius).  Everything here is per wave, in
+    // wave-uniform control flow.
+    constexpr int NS = 64 / TPP;  // particles (lane groups) per wave
+    __shared__ int4 q_rec[MFA_BLOCK / 64][FMQ_CAP][2];
+    __shared__ float4 q_vi[MFA_BLOCK / 64][NS];
+    __shared__ int q_id[MFA_BLOCK / 64][NS];  // (il << 2) | ti
+    __shared__ unsigned long long q_acc[MFA_BLOCK / 64][NS][3];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, slot = lane / TPP;
+    if (sub == 0) {
+      q_vi[wv][slot] = vi4;
+      q_id[wv][slot] = (il << 2) | ti;
+      q_acc[wv][slot][0] = 0ull;
+      q_acc[wv][slot][1] = 0ull;
+      q_acc[wv][slot][2] = 0ull;
+    }
+    const int cnt = active && (mask & TM_PAIR) ? nn[ss] : 0;
+    int mcnt = cnt;
+    for (int o = 32; o > 0; o >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, o, 64));
+    mcnt = __builtin_amdgcn_readfirstlane(mcnt);
     const unsigned* row = nl + (size_t)ss * cap;
-    // FM_BATCH candidates per lane and round: their list entries are loaded
-    // together, then their views together, so each round waits for two
-    // memory round trips instead of two per candidate.  Past the end of the
-    // row a slot reads the particle's own view and is skipped.
-    for (int q0 = sub; q0 < cnt; q0 += TPP * FM_BATCH) {
+    int head = 0, tail = 0;  // queue positions, the same in every lane
+    // evaluates queued pairs head .. head + n - 1, one per lane
+    auto drain = [&](int n) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (lane < n) {
+        const int at = (head + lane) & (FMQ_CAP - 1);
+        const int4 a = q_rec[wv][at][0], b = q_rec[wv][at][1];
+        const float qdx = __int_as_float(a.x), qdy = __int_as_float(a.y), qdz = __int_as_float(a.z);
+        const int sl = b.z & 0xff, qtj = (b.z >> 8) & 3;
+        const int idq = q_id[wv][sl];
+        const int qti = idq & 3;
+        const float qs = pair_scalar<XT, ENERGY>(
+            k, s_tab[qti * MFA_NTYPE + qtj], DC ? inv_rdpd : s_inv_rc[qti * MFA_NTYPE + qtj],
+            (b.z >> 10) & 1, (b.z >> 11) & 1, (b.z >> 12) & 1, __int_as_float(a.w), r2min, qdx, qdy,
+            qdz, q_vi[wv][sl], vel4, b.x, seed, step, idq >> 2, b.y, qti, qtj, sigf, gam, sa, alf,
+            pref, e_ff, vir, e_coul);
+        const float sf = qs * MFA_FIX_SCALE;
+        const float cx = truncf(sf * qdx), cy = truncf(sf * qdy), cz = truncf(sf * qdz);
+        // |term| < 2^52, NaN excluded: then the int64 sums are exact
+        if (fabsf(cx) < 4.5e15f && fabsf(cy) < 4.5e15f && fabsf(cz) < 4.5e15f) {
+          atomicAdd(&q_acc[wv][sl][0], (unsigned long long)(long long)cx);
+          atomicAdd(&q_acc[wv][sl][1], (unsigned long long)(long long)cy);
+          atomicAdd(&q_acc[wv][sl][2], (unsigned long long)(long long)cz);
+        } else {
+          qbad = true;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    };
+    for (int q0 = 0; q0 < mcnt; q0 += TPP * FM_BATCH) {
       unsigned ev[FM_BATCH];
       float4 pv[FM_BATCH];
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) {
-        const int q = q0 + u * TPP;
+        const int q = q0 + sub + u * TPP;
         ev[u] = q < cnt ? row[q] : (unsigned)ss;
       }
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) {
-        if (q0 + u * TPP >= cnt) continue;
+        const bool valid = q0 + sub + u * TPP < cnt;
         const unsigned e = ev[u];
-        const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
+        const int js = (int)(e & MFA_JMASK);
         const float4 o = s_off[e >> MFA_JBITS];
         const float4 pj = pv[u];
         const float dx = (pi4.x - pj.x) + o.x;
@@ -1593,16 +1698,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const float dz = (pi4.z - pj.z) + o.z;
         const float r2raw = dx * dx + dy * dy + dz * dz;
         const int wj = __float_as_int(pj.w);
-        const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
+        const int tj = wj & 3, jl = wj >> 2;
         const float4 tb = s_tab[ti * MFA_NTYPE + tj];
         bool in_lj = r2raw < tb.x;
         bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
         bool in_coul = coul_rc && r2raw < rc2f;
-        // distance of r^2 to the nearest cut-off that decides something
         float dm = fabsf(r2raw - tb.x);
         if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
         if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
-        if (__builtin_expect(dm < margin, 0)) {
+        if (__builtin_expect(valid && dm < margin, 0)) {
           // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
           const int j = k.p0 + jl;
           double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
@@ -1614,51 +1718,105 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
           in_coul = coul_rc && r2d < k.rc_coul_2;
         }
-        // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
         const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
-        // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
         const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
-        if (!in_lj && !dpd && !coul) continue;
-        const float r2 = in_lj ? fmaxf(r2raw, r2min) : r2raw;
-        const float inv_r = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, ~1 ulp
-        const float inv_r_2 = inv_r * inv_r;
-        float s = 0.0f;  // F_ij = s * (dx, dy, dz)
-        // a pair with epsilon 0 (the ideal DPD solvent) has no LJ term: in
-        // FP64 it is exactly 0, in FP32 r^-12 r^-2 overflows once r < ~0.002
-        // (such beads overlap freely) and 0 * inf would be NaN
-        if (in_lj && tb.y != 0.0f) {
-          const float r61 = tb.z * inv_r_2;
-          const float r6 = r61 * r61 * r61;
-          const float r12 = r6 * r6;
-          const float rd = tb.y * (6.0f * r6 - 12.0f * r12) * inv_r_2;
-          s = -rd;
-          if (ENERGY) {
-            e_ff += 0.5 * (double)(tb.y * ((r12 - r6) - tb.w));
-            vir += 0.5 * (double)(-rd * r2raw);
+        const bool push = valid && ((in_lj && tb.y != 0.0f) || dpd || coul);
+        const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
+        if (push) {
+          const int at = (tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                     (unsigned)(pm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u))) &
+                         (FMQ_CAP - 1);
+          q_rec[wv][at][0] = make_int4(__float_as_int(dx), __float_as_int(dy), __float_as_int(dz),
+                                       __float_as_int(r2raw));
+          q_rec[wv][at][1] = make_int4(js, jl,
+                                       slot | (tj << 8) | ((in_lj ? 1 : 0) << 10) |
+                                           ((dpd ? 1 : 0) << 11) | ((coul ? 1 : 0) << 12),
+                                       0);
+        }
+        tail += (int)__popcll(pm);
+        if (tail - head >= 64) {
+          drain(64);
+          head += 64;
+        }
+      }
+    }
+    while (tail > head) {
+      const int n = tail - head < 64 ? tail - head : 64;
+      drain(n);
+      head += n;
+    }
+    if (sub == 0) {
+      ax += (double)(long long)q_acc[wv][slot][0];
+      ay += (double)(long long)q_acc[wv][slot][1];
+      az += (double)(long long)q_acc[wv][slot][2];
+    }
+  } else {
+    if (active && (mask & TM_PAIR)) {
+      const int cnt = nn[ss];
+      const unsigned* row = nl + (size_t)ss * cap;
+      // FM_BATCH candidates per lane and round: their list entries are loaded
+      // together, then their views together, so each round waits for two
+      // memory round trips instead of two per candidate.  Past the end of the
+      // row a slot reads the particle's own view and is skipped.
+      for (int q0 = sub; q0 < cnt; q0 += TPP * FM_BATCH) {
+        unsigned ev[FM_BATCH];
+        float4 pv[FM_BATCH];
+  #pragma unroll
+        for (int u = 0; u < FM_BATCH; ++u) {
+          const int q = q0 + u * TPP;
+          ev[u] = q < cnt ? row[q] : (unsigned)ss;
+        }
+  #pragma unroll
+        for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
+  #pragma unroll
+        for (int u = 0; u < FM_BATCH; ++u) {
+          if (q0 + u * TPP >= cnt) continue;
+          const unsigned e = ev[u];
+          const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
+          const float4 o = s_off[e >> MFA_JBITS];
+          const float4 pj = pv[u];
+          const float dx = (pi4.x - pj.x) + o.x;
+          const float dy = (pi4.y - pj.y) + o.y;
+          const float dz = (pi4.z - pj.z) + o.z;
+          const float r2raw = dx * dx + dy * dy + dz * dz;
+          const int wj = __float_as_int(pj.w);
+          const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
+          const float4 tb = s_tab[ti * MFA_NTYPE + tj];
+          bool in_lj = r2raw < tb.x;
+          bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
+          bool in_coul = coul_rc && r2raw < rc2f;
+          // distance of r^2 to the nearest cut-off that decides something
+          float dm = fabsf(r2raw - tb.x);
+          if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
+          if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
+          if (__builtin_expect(dm < margin, 0)) {
+            // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
+            const int j = k.p0 + jl;
+            double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+            double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+            double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+            min_image(k, ex, ey, ez);
+            const double r2d = (ex * ex + ey * ey) + ez * ez;
+            in_lj = r2d < k.range_2[ti][tj];
+            in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
+            in_coul = coul_rc && r2d < k.rc_coul_2;
           }
+          // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
+          const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
+          // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
+          const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
+          if (!in_lj && !dpd && !coul) continue;
+          const float s = pair_scalar<XT, ENERGY>(
+              k, tb, DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj], in_lj, dpd, coul, r2raw, r2min, dx,
+              dy, dz, vi4, vel4, js, seed, step, il, jl, ti, tj, sigf, gam, sa, alf, pref, e_ff, vir,
+              e_coul);
+          // the pair's three components in fixed point: (s 2^22) d equals
+          // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
+          const float sf = s * MFA_FIX_SCALE;
+          ax += (double)truncf(sf * dx);
+          ay += (double)truncf(sf * dy);
+          az += (double)truncf(sf * dz);
         }
-        if (dpd) {
-          const float xi_n = pair_normal_f32(seed, step, il, jl);
-          // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
-          const float inv_rc = DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj];
-          const float wr = 1.0f - (r2 * inv_r) * inv_rc;
-          const float4 vj = vel4[js];
-          const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
-          s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
-        }
-        if ((XT & 1) && coul) {  // ewald_real.f90:15-32 with the clamped r_2
-          const float qq = (float)k.q_type[ti] * (float)k.q_type[tj];
-          const float rr = r2 * inv_r;
-          const float ec = erfcf(sa * rr);
-          s += (qq * pref * __expf(-alf * r2) * inv_r + qq * ec * inv_r_2) * inv_r;
-          if (ENERGY) e_coul += 0.5 * (double)(qq * ec * inv_r);
-        }
-        // the pair's three components in fixed point: (s 2^22) d equals
-        // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
-        const float sf = s * MFA_FIX_SCALE;
-        ax += (double)truncf(sf * dx);
-        ay += (double)truncf(sf * dy);
-        az += (double)truncf(sf * dz);
       }
     }
   }
@@ -1729,7 +1887,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   }
 
   // Combine the lane group and write F_i.
-  const bool bad = fix_bad(ax, ay, az);
+  const bool bad = fix_bad(ax, ay, az) || qbad;
   double fx = group_sum<TPP>(ax) * MFA_FIX_INV;
   double fy = group_sum<TPP>(ay) * MFA_FIX_INV;
   double fz = group_sum<TPP>(az) * MFA_FIX_INV;
@@ -1747,12 +1905,14 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   if (ENERGY) {
     double vals[MFA_NTERMS];
     for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
+    // pair terms from every lane: with the pair queue a lane beyond the
+    // replica's particles may evaluate another lane's pairs (zero otherwise)
+    vals[T_VFF] = e_ff;
+    vals[T_VIR] = vir;
+    vals[T_VCOUL] = e_coul;
     if (active) {
-      vals[T_VFF] = e_ff;
-      vals[T_VIR] = vir;
       vals[T_VFW] = e_fw;
       vals[T_VINTRA] = e_bond * k.k_chain;
-      vals[T_VCOUL] = e_coul;
       vals[T_VBEND] = e_bend;
       vals[T_FZB] = fzb;
       vals[T_FZT] = fzt;
@@ -2531,8 +2691,8 @@ static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0
   return MFA_OK;
 }
 
-template <int TPP, int XT>
-static void launch_force_tx(mfa_ctx* c, int mask) {
+template <int TPP, int XT, bool Q>
+static void launch_force_txq(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED) {
     FuseArgs fu;
@@ -2551,15 +2711,15 @@ static void launch_force_tx(mfa_ctx* c, int mask) {
       fu.vel4_out = c->vel4_alt;
       fu.cell_count = c->probe_count ? c->probe_count : c->cell_count;
       fu.nc_tot = c->n_cells;
-      hipLaunchKernelGGL((k_force_mixed<TPP, 2, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+      hipLaunchKernelGGL((k_force_mixed<TPP, 2, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags, fu);
     } else if (mask & TM_ENERGY) {
-      hipLaunchKernelGGL((k_force_mixed<TPP, 1, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+      hipLaunchKernelGGL((k_force_mixed<TPP, 1, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags, fu);
     } else {
-      hipLaunchKernelGGL((k_force_mixed<TPP, 0, XT>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+      hipLaunchKernelGGL((k_force_mixed<TPP, 0, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags, fu);
     }
@@ -2568,6 +2728,19 @@ static void launch_force_tx(mfa_ctx* c, int mask) {
                        c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
                        mask, c->part, c->flags);
   }
+}
+
+// The mixed pass's pair queue (compiled for DPD_CUT_OFF systems only, where
+// few list candidates interact) when the context has it on.
+template <int TPP, int XT>
+static void launch_force_tx(mfa_ctx* c, int mask) {
+  if constexpr ((XT & 4) != 0) {
+    if (c->pair_queue && c->precision == MFA_MIXED) {
+      launch_force_txq<TPP, XT, true>(c, mask);
+      return;
+    }
+  }
+  launch_force_txq<TPP, XT, false>(c, mask);
 }
 
 // XT bit 1: Coulomb real space and bending; bit 2: the Langevin force; bit 4
@@ -3283,6 +3456,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->ps.assign(ps, ps + n_rep);
   c->precision = precision;
   c->tpp = 8;
+  c->pair_queue = false;
   c->need_rebuild = true;
   c->timing = false;
   c->capturing = false;
@@ -3517,6 +3691,21 @@ extern "C" int32_t mfa_set_threads_per_particle(mfa_ctx* c, int32_t tpp) {
   if (tpp != 1 && tpp != 2 && tpp != 4 && tpp != 8 && tpp != 16)
     return set_err(c, MFA_ERR_ARG, "threads per particle must be 1,2,4,8 or 16");
   c->tpp = tpp;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_get_tuning(mfa_ctx* c, int32_t* tpp, int32_t* pair_queue, double* skin) {
+  if (!c) return MFA_ERR_ARG;
+  if (tpp) *tpp = c->tpp;
+  if (pair_queue) *pair_queue = c->pair_queue ? 1 : 0;
+  if (skin) *skin = c->p.skin;
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_set_pair_queue(mfa_ctx* c, int32_t on) {
+  if (!c || (on != 0 && on != 1)) return MFA_ERR_ARG;
+  if ((on != 0) != c->pair_queue) drop_graph(c);
+  c->pair_queue = on != 0;
   return MFA_OK;
 }
 
@@ -4224,6 +4413,9 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   const int reps = 20;
   double best = 1e300;
   int best_t = c->tpp;
+  bool best_q = c->pair_queue;
+  // the pair queue is a choice where the kernels have it (mixed, DPD_CUT_OFF)
+  const int nq = c->precision == MFA_MIXED && c->p.dpd_cut_off_on ? 2 : 1;
   int64_t step = first_step;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
@@ -4231,8 +4423,13 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   const bool timing = c->timing;
   c->timing = false;
   int rc = MFA_OK;
-  for (int q = 0; q < 5 && rc == MFA_OK; ++q) {
+  for (int qq = 0; qq < 5 * nq && rc == MFA_OK; ++qq) {
+    const int q = qq % 5;
     c->tpp = cand[q];
+    if (nq == 2) {
+      c->pair_queue = qq >= 5;
+      drop_graph(c);
+    }
     // the candidate's steps (untimed; the first call captures its step block)
     rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
     step += c->graph_steps + 1;
@@ -4253,12 +4450,14 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
     if (ms < best) {
       best = ms;
       best_t = cand[q];
+      best_q = c->pair_queue;
     }
   }
   c->timing = timing;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   c->tpp = best_t;
+  c->pair_queue = best_q;
   drop_graph(c);
   if (rc) return rc;
   if (best_tpp) *best_tpp = best_t;

@@ -178,6 +178,8 @@ SIGNATURES = {
     "mfa_time_rebuild_idle": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_params_set_skin": (C.c_int32, [C.c_void_p, C.c_double]),
     "mfa_set_skin": (C.c_int32, [_P, C.c_double]),
+    "mfa_set_pair_queue": (C.c_int32, [_P, C.c_int32]),
+    "mfa_get_tuning": (C.c_int32, [_P, _ip, _ip, _dp]),
     "mfa_autotune_skin": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_rebuild": (C.c_int32, [_P, C.c_int32, _dp]),
@@ -298,6 +300,15 @@ class Context:
 
     def set_skin(self, skin):
         self._chk(self.lib.mfa_set_skin(self.h, skin), "mfa_set_skin")
+
+    def set_pair_queue(self, on):
+        self._chk(self.lib.mfa_set_pair_queue(self.h, 1 if on else 0), "mfa_set_pair_queue")
+
+    def tuning(self):
+        """The current choices: threads per particle, pair queue, list skin."""
+        t, q, sk = C.c_int32(), C.c_int32(), C.c_double()
+        self._chk(self.lib.mfa_get_tuning(self.h, C.byref(t), C.byref(q), C.byref(sk)), "mfa_get_tuning")
+        return {"threads_per_particle": t.value, "pair_queue": bool(q.value), "skin": sk.value}
 
     def autotune_skin(self, first_step, n_steps, skins):
         """The fastest of `skins` over n_steps whole steps each (advances the

@@ -484,3 +484,50 @@ def test_autotune_skin_keeps_a_candidate():
     obs = ctx.run_steps(first, 20)
     assert np.isfinite(obs["t_kin"]) and obs["t_kin"] > 0.0
     ctx.close()
+
+
+@pytest.mark.parametrize("tpp", [1, 4, 16])
+@pytest.mark.parametrize("case", ["brush_small", "pe_brush_small"])
+def test_pair_queue_forces_bit_identical(case, tpp):
+    """The mixed pass with the wave's interacting pairs queued in LDS gives
+    the direct loop's forces bit for bit (the same fixed-point terms, summed
+    exactly), and the energies to rounding; trajectories of 40 steps with
+    rebuilds stay bit-identical."""
+    inputs, p, r, v, t, a = make_case(case)
+    out = {}
+    for q in (False, True):
+        ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED, tpp=tpp)
+        ctx.set_pair_queue(q)
+        assert ctx.tuning()["pair_queue"] == q
+        e = ctx.energies(step=3)
+        f = ctx.get_state()["f"]
+        obs = ctx.run_steps(4, 40)
+        out[q] = (f, e, ctx.get_state(), obs)
+        ctx.close()
+    (f0, e0, s0, o0), (f1, e1, s1, o1) = out[False], out[True]
+    print("%s tpp %d: %d rebuilds" % (case, tpp, o0["n_rebuilds"]), flush=True)
+    assert np.array_equal(f0, f1)
+    for k in ("v_fluid_fluid", "virial", "v_coul"):
+        assert rel(e0[k], e1[k]) < 1e-12 or abs(e0[k] - e1[k]) < 1e-12, (k, e0[k], e1[k])
+    for k in ("r", "v", "f", "a", "mic"):
+        assert np.array_equal(s0[k], s1[k]), k
+    assert o0["n_rebuilds"] == o1["n_rebuilds"]
+
+
+def test_pair_queue_matches_oracle_brush9k():
+    """Full-size brush (the case the queue is for) against the oracle."""
+    w = configs.WORKLOADS["brush9k"]
+    inputs = w["make"]()
+    p, r, v, t = configs.build(inputs, w["kind"], min_dist=w["min_dist"])
+    a = np.zeros_like(r)
+    s = oracle_sys(inputs, r, v, a, t)
+    s.binning()
+    fref, _ = s.forces(step=5)
+    ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED, tpp=4)
+    ctx.set_pair_queue(True)
+    ctx.energies(step=5)
+    f = ctx.get_state()["f"]
+    ctx.close()
+    per, rms = force_error(f, fref)
+    print("brush9k queued: max %.2e rms %.2e" % (per, rms), flush=True)
+    assert rms < 1e-5
