@@ -45,14 +45,15 @@ FORCE_BYTES_PER_PARTICLE = 76
 # formed and used in registers.  Everything else it moves (the FP32 view, the
 # list, the skin reference, the rebuild) is this implementation's own.
 STEP_BYTES_PER_PARTICLE = 100
-# Counter figures of the mixed force pass on this workload, per threads per
-# particle (the autotuner's choice), measured by `bench.py --counters` on the
-# state the timed steps leave (profiles/force_pass_melt100k_hbm_traffic.md,
-# profiles/force_and_list_melt100k_sq_counters.md): HBM bytes per pass
+# Counter figures of the mixed step's force pass (fused with the integration)
+# on this workload, per (threads per particle, pair queue) as the autotuners
+# chose them, measured by `bench.py --counters` under rocprofv3 on the state
+# the timed steps leave (profiles/force_pass_melt100k_hbm_traffic.md,
+# profiles/force_and_list_melt100k_sq_counters.md): HBM bytes per launch
 # (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a copy of known
-# size) and VALU wave-instructions per pass (SQ_INSTS_VALU).  The issue peak:
-# 1024 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz.
-FORCE_COUNTERS = {}
+# size) and VALU wave-instructions per launch (SQ_INSTS_VALU).  The issue
+# peak: 1024 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz.
+FORCE_COUNTERS = {(8, False): (99.481e6 + 10.000e6, 25232977)}
 VALU_PEAK_GINSTS = 1024 * 2.4 / 4
 
 
@@ -245,7 +246,7 @@ def bench_single(args, name):
     st_m["t"] = t
     if args.save_state:  # the state the timed steps left, for the counter runs
         np.savez(args.save_state, r=st_m["r"], v=st_m["v"], a=st_m["a"], mic=st_m["mic"], t=t,
-                 tpp=tm_m["tpp"])
+                 tpp=tm_m["tpp"], skin=tm_m["skin"], pair_queue=int(tm_m["pair_queue"]))
         log("saved the timed state to %s" % args.save_state)
     err_max, err_rms = force_error_vs_fp64(p, st_m)
     log("mixed vs fp64 force error: max %.3e rms %.3e" % (err_max, err_rms))
@@ -266,7 +267,7 @@ def bench_single(args, name):
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
-        traffic, valu = FORCE_COUNTERS.get(tm_m["tpp"], (None, None))
+        traffic, valu = FORCE_COUNTERS.get((tm_m["tpp"], bool(tm_m["pair_queue"])), (None, None))
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
@@ -284,17 +285,17 @@ def bench_single(args, name):
     # force pass alone
     fused = not p.coulomb
     if fused:
-        roof = roofline(n, tm_m["step_force_ms"], None,
+        roof = roofline(n, tm_m["step_force_ms"], traffic,
                         kernel="k_force_mixed<%d,2,%d>" % (tm_m["tpp"], xt),
                         bytes_per_particle=STEP_BYTES_PER_PARTICLE)
     else:
-        roof = roofline(n, tm_m["force_ms"], traffic,
+        roof = roofline(n, tm_m["force_ms"], None,
                         kernel="k_force_mixed<%d,0,%d>" % (tm_m["tpp"], xt))
     roof["force_alone_ms_avg"] = tm_m["force_ms"]
     roof["force_ms_avg_fp64"] = tm_d["force_ms"]
-    if valu and not fused:
+    if valu:
         # the kernel is VALU-issue bound: its instruction rate against the issue peak
-        g = valu / (tm_m["force_ms"] * 1e-3) / 1e9
+        g = valu / ((tm_m["step_force_ms"] if fused else tm_m["force_ms"]) * 1e-3) / 1e9
         roof["valu_issue"] = {"achieved": g, "peak": VALU_PEAK_GINSTS,
                               "unit": "G wave-instructions/s", "frac": g / VALU_PEAK_GINSTS,
                               "instructions_per_launch": valu}
@@ -496,11 +497,13 @@ def counter_workload(args):
     """The workload of the counter runs, meant to run under rocprofv3 with
     one counter set per run (few dispatches, so each pass takes seconds):
     loads the state a `--save-state` run left (the state the timed steps
-    reached, and the threads per particle the autotuner chose there), bins
+    reached, and the width, list skin and pair-queue choice the autotuners
+    made there), bins
     it `--rebuilds` times, runs the force pass `--reps` times with direct
-    launches (melt100k, brush9k) or the k-space Ewald stage `--reps` times
-    (pe_brush27k), then a device copy of known size: the calibration of
-    FETCH_SIZE and WRITE_SIZE."""
+    launches and then `--reps` times as the step runs it (melt100k,
+    brush9k: fused with the integration), or the k-space Ewald stage
+    `--reps` times (pe_brush27k), then a device copy of known size: the
+    calibration of FETCH_SIZE and WRITE_SIZE."""
     w = configs.WORKLOADS[args.workload]
     p = lib.derive(w["make"]())
     st = np.load(args.state)
@@ -508,18 +511,24 @@ def counter_workload(args):
     with lib.Context(p, lib.MFA_MIXED) as c:
         c.set_graph_steps(0)
         c.set_tpp(tpp)
+        c.set_skin(float(st["skin"]))  # the autotuners' choices of the timed run
+        c.set_pair_queue(bool(st["pair_queue"]))
         c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
         for _ in range(args.rebuilds):
             c.binning()
+        ms = ms_step = float("nan")
         if p.coulomb:
             for _ in range(args.reps):
                 c.ewald_k(1)
-            ms = float("nan")
         else:
             ms = c.time_force(args.reps)
+            # the kernel the step runs (in the mixed form: the force pass that
+            # also integrates), its writes sent to copies
+            ms_step = c.time_force_step(args.reps)
     cms = lib.copy_probe(COPY_BYTES)
-    log("%s: %d threads per particle; force pass %.4f ms; calibration copy of %d bytes in %.4f ms"
-        % (args.workload, tpp, ms, COPY_BYTES, cms))
+    log("%s: %d threads per particle; force pass %.4f ms, as the step runs it %.4f ms; "
+        "calibration copy of %d bytes in %.4f ms"
+        % (args.workload, tpp, ms, ms_step, COPY_BYTES, cms))
 
 
 def main():

@@ -18,8 +18,11 @@ import statistics
 from collections import defaultdict
 
 COPY_BYTES = 256 * 1024 * 1024
-# kernel names of the force pass without energies (any width, any extra-terms flag)
-MIXED_FORCE = re.compile(r"void k_force_mixed<\d+, 0")
+# kernel names of the mixed force pass without energies (any width, extra-terms
+# flag and queue choice): MODE 0 (forces alone) and MODE 2 (the pass of a step,
+# which also integrates)
+MIXED_FORCE = re.compile(r"void k_force_mixed<\d+, [02]")
+MIXED_STEP = re.compile(r"void k_force_mixed<\d+, 2")
 FP64_FORCE = re.compile(r"void k_force64<\d+")
 
 
@@ -38,7 +41,8 @@ def trace(path, n_timed):
         end = next(i for i, n in enumerate(names) if FP64_FORCE.match(n))
     except StopIteration:
         end = len(rows)
-    idx = [i for i in range(end) if MIXED_FORCE.match(names[i])]
+    idx = [i for i in range(end) if MIXED_STEP.match(names[i])] or \
+        [i for i in range(end) if MIXED_FORCE.match(names[i])]
     seg = rows[idx[-n_timed]:end]
     d = defaultdict(list)
     for r in seg:
@@ -72,10 +76,13 @@ def pmc(path):
             continue
         cal = max(max(v) for v in copy)  # KiB read or written by the 256 MiB copy (2nd dispatch)
         scale = COPY_BYTES / (cal * 1024.0)
-        f = statistics.median(force)
-        out.append("%s: force pass %.1f KiB/dispatch (median of %d), copy %.1f KiB for %d B "
-                   "-> scale %.4f -> %.3f MB per force pass"
-                   % (counter, f, len(force), cal, COPY_BYTES, scale, f * 1024.0 * scale / 1e6))
+        for k, v in sorted(kern.items()):
+            if not (MIXED_FORCE.match(k) or FP64_FORCE.match(k)):
+                continue
+            f = statistics.median(v)
+            out.append("%s: %s %.1f KiB/dispatch (median of %d), copy %.1f KiB for %d B "
+                       "-> scale %.4f -> %.3f MB per dispatch"
+                       % (counter, k, f, len(v), cal, COPY_BYTES, scale, f * 1024.0 * scale / 1e6))
     return "\n".join(out)
 
 
