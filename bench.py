@@ -95,7 +95,7 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
     return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp, "skin": skin,
-                     "pair_queue": tuning["pair_queue"],
+                     "pair_queue": tuning["pair_queue"], "half_list": tuning["half_list"],
                      "ewald_ms": ewald_ms, "ewald_parts_ms": ewald_parts,
                      "rebuild_ms": rebuild, "rebuild_idle_ms": rebuild_idle}, st
 
@@ -246,7 +246,8 @@ def bench_single(args, name):
     st_m["t"] = t
     if args.save_state:  # the state the timed steps left, for the counter runs
         np.savez(args.save_state, r=st_m["r"], v=st_m["v"], a=st_m["a"], mic=st_m["mic"], t=t,
-                 tpp=tm_m["tpp"], skin=tm_m["skin"], pair_queue=int(tm_m["pair_queue"]))
+                 tpp=tm_m["tpp"], skin=tm_m["skin"], pair_queue=int(tm_m["pair_queue"]),
+                 half_list=int(tm_m["half_list"]))
         log("saved the timed state to %s" % args.save_state)
     err_max, err_rms = force_error_vs_fp64(p, st_m)
     log("mixed vs fp64 force error: max %.3e rms %.3e" % (err_max, err_rms))
@@ -267,7 +268,8 @@ def bench_single(args, name):
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
-        traffic, valu = FORCE_COUNTERS.get((tm_m["tpp"], bool(tm_m["pair_queue"])), (None, None))
+        traffic, valu = (FORCE_COUNTERS.get((tm_m["tpp"], bool(tm_m["pair_queue"])), (None, None))
+                         if not tm_m["half_list"] else (None, None))
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
@@ -303,6 +305,7 @@ def bench_single(args, name):
     config["list_skin"] = {"mixed": tm_m["skin"], "fp64": tm_d["skin"],
                            "chosen_by": "autotuner" if tune else "the workload's spec"}
     config["pair_queue"] = {"mixed": tm_m["pair_queue"], "chosen_by": "autotuner"}
+    config["half_list"] = {"mixed": tm_m["half_list"], "chosen_by": "autotuner"}
     extra = {
         "fp64_value": value_fp64,
         "mixed_over_fp64": value / value_fp64,
@@ -359,6 +362,7 @@ def bench_batch(args, n_rep=256):
         first += len(cand) * (args.graph_steps + 1 + 100)
         log("autotune: skin %.2f (%.3f ms per step)" % (skin, ms_skin))
         pair_queue = b.tuning()["pair_queue"]
+        half_list = b.tuning()["half_list"]
         ramp = configs.ramp_steps(reps[0][0])  # every replica has dt = 0.005
         t2 = time.perf_counter()
         if first <= ramp:
@@ -384,6 +388,7 @@ def bench_batch(args, n_rep=256):
             c.set_tpp(tpp)
             c.set_skin(skin)
             c.set_pair_queue(pair_queue)
+            c.set_half_list(half_list)
             c.set_state(st["r"], st["v"], st["a"], reps[q][3], st["mic"])
             c.run_steps(first, args.warmup, want_obs=False)
             t1 = time.perf_counter()
@@ -397,6 +402,7 @@ def bench_batch(args, n_rep=256):
               "solvent_density": 3.0, "threads_per_particle": tpp,
               "list_skin": {"value": skin, "chosen_by": "autotuner"},
               "pair_queue": {"value": pair_queue, "chosen_by": "autotuner"},
+              "half_list": {"value": half_list, "chosen_by": "autotuner"},
               "parallelism": "single GPU, replicas batched"}
     extra = {
         "sequential_value": ntot * args.steps / dt_s,
@@ -513,6 +519,7 @@ def counter_workload(args):
         c.set_tpp(tpp)
         c.set_skin(float(st["skin"]))  # the autotuners' choices of the timed run
         c.set_pair_queue(bool(st["pair_queue"]))
+        c.set_half_list(bool(st["half_list"]) if "half_list" in st else False)
         c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
         for _ in range(args.rebuilds):
             c.binning()

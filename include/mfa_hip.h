@@ -40,7 +40,8 @@ enum {
   MFA_ERR_NOMEM = 5,
   MFA_ERR_NAN = 6,          /* non-finite force or position */
   MFA_ERR_IO = 7,
-  MFA_ERR_CELL_OVERFLOW = 8 /* more particles in a cell than its capacity */
+  MFA_ERR_CELL_OVERFLOW = 8,/* more particles in a cell than its capacity */
+  MFA_ERR_INTERNAL = 9      /* a device-side consistency check failed (a bug) */
 };
 
 /* Precision forms of the force pass. */
@@ -396,12 +397,26 @@ int32_t mfa_obs_read(mfa_ctx* ctx, mfa_obs_sums* out, double* dens_z, double* f_
  * (exact fixed-point sums).  Off by default; the autotuner chooses. */
 int32_t mfa_set_pair_queue(mfa_ctx* ctx, int32_t on);
 
+/* Half-list form of the mixed force pass (no counterpart in the reference,
+ * whose binning.f90:81 list is a half list applied with +/- in
+ * fluid_fluid.f90:120-126): on, each pair is evaluated once, from the row
+ * of its lower sorted position, in a first pass that keeps the pair's scaled
+ * force scalar per list slot; a second pass gives the other particle its
+ * term from that scalar and its own (negated) separation, through the slot
+ * position found once per rebuild, so no atomics are needed and the forces
+ * are bit-identical to the full-list pass.  Applies at 4 or more threads per
+ * particle; allocates 6 bytes per list slot and 24 per particle (refused,
+ * MFA_ERR_ARG, beyond 2^31 slots).  Off by default; the autotuner chooses. */
+int32_t mfa_set_half_list(mfa_ctx* ctx, int32_t on);
+
 /* The context's current choices: threads per particle, pair queue, list skin
- * (replica 0).  Any out-pointer may be NULL. */
-int32_t mfa_get_tuning(mfa_ctx* ctx, int32_t* tpp, int32_t* pair_queue, double* skin);
+ * (replica 0), half-list form.  Any out-pointer may be NULL. */
+int32_t mfa_get_tuning(mfa_ctx* ctx, int32_t* tpp, int32_t* pair_queue, double* skin,
+                       int32_t* half_list);
 
 /* Autotuner: for each number of threads per particle in {1, 2, 4, 8, 16}
- * (and, for mixed DPD_CUT_OFF systems, with the pair queue off and on) runs
+ * (for mixed DPD_CUT_OFF systems with the pair queue off and on, for the
+ * mixed form at 4 or more threads with the half-list form off and on) runs
  * n_steps from first_step (advancing the state, so call it during warm-up),
  * then times 20 force passes on the state they left; keeps the fastest and
  * returns it with its time per force pass (ms). */

@@ -51,8 +51,10 @@ enum {
 enum { TM_PAIR = 1, TM_WALL = 2, TM_BOND = 4, TM_ACCUM = 8, TM_ENERGY = 16, TM_BEND = 32, TM_FUSE = 64 };
 #define TM_ALL (TM_PAIR | TM_WALL | TM_BOND | TM_BEND)
 // device flag slots
+// F_MIRROR: the list was rebuilt since the half-list pair pass last found
+// the mirror positions of its lower entries
 enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_FILLDONE = 4, F_LIVE = 5,
-       F_NFLAGS = 8 };
+       F_MIRROR = 6, F_NFLAGS = 8 };
 
 // Fixed-point scale of the mixed form: 2^22 per unit force (resolution
 // 2.4e-7; a pair term converts through int32 up to |F| < 512, through int64
@@ -61,6 +63,15 @@ enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_FILLDONE = 
 #define MFA_FIX_INV (1.0 / 4194304.0)
 #define MFA_JBITS 27
 #define MFA_JMASK ((1u << MFA_JBITS) - 1u)
+// List rows.  Row s (sorted position s) holds its "upper" entries (neighbours
+// at sorted positions above s) at [0, nup) in scan order and its "lower"
+// entries at [cap - nlo, cap), appended from the back (reverse scan order);
+// nn[s] = (nup << 16) | (nup + nlo).  A kernel that walks the whole row maps
+// its q-th entry to list_slot(q, nup, cnt, cap).
+#define MFA_CAP_MAX 32767
+__device__ __forceinline__ int list_slot(int q, int nup, int cnt, int cap) {
+  return q < nup ? q : q + (cap - cnt);
+}
 
 struct KParams {
   double L[3], invL[3];
@@ -146,6 +157,16 @@ struct mfa_ctx {
   int cap;
   bool cap_fixed;
   bool pair_queue;  // mixed pass with DPD_CUT_OFF: compact the interacting pairs per wave  // the capacity the caller chose at creation (no growth)
+  // half-list pair passes of the mixed form: each pair evaluated once, from
+  // the row of its lower sorted position; its scaled pair scalar kept per
+  // upper entry (pair_s), the position of each lower entry's pair in its
+  // neighbour's row (mir), and the first pass's fixed-point sums (fpart)
+  bool half_list;
+  bool list_split;  // rows built split into upper and lower entries (for the half-list form)
+  float* pair_s = nullptr;
+  uint16_t* mir = nullptr;
+  double* fpart = nullptr;
+  int half_cap = 0;  // the list capacity those buffers were sized for
   int tpp;  // threads per particle in the force pass
   int n_cells;
   bool need_rebuild;
@@ -803,8 +824,11 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 // indices into the staged piece), in order, for that group.  Each time 64
 // are queued, and at the end of each piece, the group's particles are tested
 // in turn against the queued candidates at once and a ballot appends the
-// hits to the particle's row in lane order, so every row holds its
-// neighbours in scan order.  Separations come from the FP32 view (both
+// hits to the particle's row in lane order: the upper ones (neighbours at
+// higher sorted positions) from the front, the lower ones from the back, so
+// the upper part of every row holds its neighbours in scan order, ascending
+// in entry value (code, then sorted position), which lets the half-list
+// pass find a pair's place in the other row by bisection.  Separations come from the FP32 view (both
 // positions relative to their cell origins, plus the offset of the
 // candidate's cell): no periodic arithmetic.  A separation whose r^2 lies
 // within the view's rounding bound of the radius is decided again in FP64
@@ -819,6 +843,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 #define LB_RING 128                  // queued candidates per group (a block of 64 adds at most 64)
 #define LB_MIN_OCTANT 16             // particles in a cell for the octant groups
 #define LB_PIECE 1024                // candidates staged in LDS at a time
+template <bool SPLIT>
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
@@ -875,7 +900,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   if (threadIdx.x == 64) crowded_s = 0;
   for (int t = threadIdx.x; t < m; t += MFA_BLOCK) hpos[t] = pos4[hs0 + t];
   __syncthreads();
-  if (c == 0 && threadIdx.x == 0) atomicAdd(&flags[F_NREBUILD], 1);
+  if (c == 0 && threadIdx.x == 0) {
+    atomicAdd(&flags[F_NREBUILD], 1);
+    if (SPLIT) atomicExch(&flags[F_MIRROR], 1);  // the half-list pass must find new mirror slots
+  }
   // the wave's two octants: particle lists and their counts
   const float hx = 0.5f * (float)k.r_bin[0], hy = 0.5f * (float)k.r_bin[1],
               hz = 0.5f * (float)k.r_bin[2];
@@ -945,6 +973,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     }
   }
   int hc = 0;  // row length of the wave's particle in this lane's place
+  int hu = 0;  // ... and how many of its entries are upper ones
 
   // Tests 64 candidates (one per lane; view position relative to this cell,
   // entry) against each particle of group g and appends the hits.
@@ -981,10 +1010,28 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
       const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
       if (mask) {
         const int base = __builtin_amdgcn_readlane(hc, slot);
-        if (hit) {
-          const unsigned pos = __builtin_amdgcn_mbcnt_hi(
-              (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
-          if (pos < (unsigned)cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
+        if constexpr (SPLIT) {
+          // split rows (the half-list form): upper entries (a neighbour at a
+          // higher sorted position) go to the front of the row, lower ones
+          // to its back; both in lane order
+          const bool up = js_c > hs0 + h;
+          const unsigned long long mu = __builtin_amdgcn_ballot_w64(hit && up);
+          const int bup = __builtin_amdgcn_readlane(hu, slot);
+          const int tot = base + (int)__popcll(mask);
+          if (hit && tot <= cap) {  // beyond the capacity: an overflow, reported below
+            const unsigned long long mm = up ? mu : mask & ~mu;
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+                (unsigned)(mm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mm, 0u));
+            const int pos = up ? bup + rank : cap - 1 - ((base - bup) + rank);
+            nl[(size_t)(hs0 + h) * cap + pos] = entry;
+          }
+          hu = lane == slot ? bup + (int)__popcll(mu) : hu;
+        } else {
+          if (hit) {
+            const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
+            if (pos < (unsigned)cap) nl[(size_t)(hs0 + h) * cap + pos] = entry;
+          }
         }
         hc = lane == slot ? base + (int)__popcll(mask) : hc;
       }
@@ -1065,7 +1112,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   if (lane < nw) {
     const int g = lane < g_n[0] ? 0 : 1, t = lane - (g == 0 ? 0 : g_n[0]);
     const int h = crowded ? w + LB_WAVES * t : hlist[w][g_at[g] + t];
-    nn[hs0 + h] = hc < cap ? hc : cap;
+    // unsplit rows: every entry counts as upper (list_slot(q) = q)
+    nn[hs0 + h] = hc <= cap ? (hc | ((SPLIT ? hu : hc) << 16)) : cap;
     if (hc > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
     my_max = hc;
   }
@@ -1300,10 +1348,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   double ax = 0.0, ay = 0.0, az = 0.0;
 
   if (active && (mask & TM_PAIR)) {
-    const int cnt = nn[ss];
+    const int wn = nn[ss], cnt = wn & 0xffff, nup = wn >> 16;
     const unsigned* row = nl + (size_t)ss * cap;
     for (int q = sub; q < cnt; q += TPP) {
-      const int j = perm[row[q] & MFA_JMASK];
+      const int j = perm[row[list_slot(q, nup, cnt, cap)] & MFA_JMASK];
       double dx = xi - r[3 * (size_t)j + 0];
       double dy = yi - r[3 * (size_t)j + 1];
       double dz = zi - r[3 * (size_t)j + 2];
@@ -1553,21 +1601,73 @@ struct FuseArgs {
   int nc_tot;
 };
 
+// The half-list form's buffers: per list slot, the scaled pair scalar of an
+// upper entry (written by the first pass, 0 where the pair does not
+// interact) and, for a lower entry, the slot of its pair in the neighbour's
+// row; per particle (sorted order), the first pass's fixed-point sums.
+struct HalfArgs {
+  float* pair_s;
+  uint16_t* mir;
+  double* fpart;
+};
+
+// The end of a force pass that also integrates: verlet_velocities of this
+// step with F_i (k_fused_integrate's order: a = F/m, v += dt/2 a, pinned
+// heads), then verlet_positions and check_skin of the next.  Positions are
+// read from r (this step's buffer, which the whole pass reads) and written
+// to fu.r_out; only the group's lane 0 reads or writes particle i's v and
+// mic.  Every lane of the launch takes part (ballot, count clearing).
+__device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, int sub, int i, int il,
+                                               int ti, int ss, double fx, double fy, double fz,
+                                               const double* __restrict__ r, const FuseArgs& fu) {
+  bool over = false;
+  if (active && sub == 0) {
+    double fi[3] = {fx, fy, fz}, vi[3], ai[3], ri[3], rr[3];
+    int mi[3];
+    for (int d = 0; d < 3; ++d) {
+      vi[d] = fu.v_in[3 * (size_t)i + d];
+      ri[d] = r[3 * (size_t)i + d];
+      mi[d] = fu.mic_in[3 * (size_t)i + d];
+      rr[d] = fu.r_ref[3 * (size_t)i + d];
+    }
+    vv_one(k, il, ti, fi, vi, ai);
+    bool wrapped = false;
+    vp_one(k, ri, vi, ai, mi, wrapped);
+    for (int d = 0; d < 3; ++d) {
+      fu.r_out[3 * (size_t)i + d] = ri[d];
+      fu.v_out[3 * (size_t)i + d] = vi[d];
+    }
+    if (wrapped)
+      for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
+    write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
+    over = skin_one(k, ri, rr, mi);
+  }
+  if (__builtin_amdgcn_ballot_w64(over) != 0ull && (threadIdx.x & 63) == 0)
+    atomicOr(&fu.rebuild_flags[k.rep * F_NFLAGS + F_REBUILD], 1);
+  zero_counts(fu.nc_tot, fu.cell_count);
+}
+
 // XT bits: 1 Coulomb real space and bending, 2 the Langevin force, 4 DPD_CUT_OFF
 // (a DPD range of its own, heads excluded from DPD).  MODE: 0 forces, 1 forces
 // and energy partials, 2 forces, then verlet_velocities of this step and
 // verlet_positions and check_skin of the next (k_fused_integrate's work) in
-// the same pass, which leaves F unwritten.
+// the same pass, which leaves F unwritten; 3 the first pass of the half-list
+// form: the upper entries of each row only, each interacting pair's scaled
+// scalar kept in hf.pair_s (0 for the others), the particle's terms
+// (pairs, bonds, walls, Langevin, bending) left in hf.fpart as fixed-point
+// sums for k_force_lower; after a rebuild it also finds, for each lower
+// entry, the slot of its pair in the neighbour's row (hf.mir).
 template <int TPP, int MODE, int XT, bool QUEUE>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
     const float4* __restrict__ pos4, const float4* __restrict__ vel4, const int* __restrict__ perm,
     const unsigned* __restrict__ nl, const int* __restrict__ nn, int cap, double* __restrict__ f,
     const long long* __restrict__ d_step, int mask, double* __restrict__ part,
-    int* __restrict__ flags_all, FuseArgs fu) {
+    int* __restrict__ flags_all, FuseArgs fu, HalfArgs hf) {
 #pragma clang fp contract(fast)
   constexpr bool ENERGY = MODE == 1;
   constexpr bool FUSE = MODE == 2;
+  constexpr bool HALF = MODE == 3;
   constexpr bool DC = (XT & 4) != 0;
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   int* flags = flags_all + k.rep * F_NFLAGS;
@@ -1575,7 +1675,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   __shared__ float4 s_off[27];
   __shared__ float4 s_tab[MFA_NTYPE * MFA_NTYPE];  // range_2, 4 eps, sigma_2, e_shift
   __shared__ float s_inv_rc[MFA_NTYPE * MFA_NTYPE];  // 1 / sqrt(range_2)
-  __shared__ int err_s;
+  __shared__ int err_s, mirror_s;
   if (threadIdx.x < 27) {
     const int c = threadIdx.x;
     const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
@@ -1589,6 +1689,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     s_inv_rc[q] = (float)sqrt(k.inv_range_2[a][b]);
   }
   if (threadIdx.x == 64) err_s = flags[F_ERROR];
+  if (threadIdx.x == 65) mirror_s = HALF ? flags[F_MIRROR] : 0;
   __syncthreads();
   if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1642,7 +1743,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       q_acc[wv][slot][1] = 0ull;
       q_acc[wv][slot][2] = 0ull;
     }
-    const int cnt = active && (mask & TM_PAIR) ? nn[ss] : 0;
+    const int wn = active && (mask & TM_PAIR) ? nn[ss] : 0;
+    const int tot = wn & 0xffff, nup = wn >> 16;
+    const int cnt = HALF ? nup : tot;  // the half-list pass: upper entries only
     int mcnt = cnt;
     for (int o = 32; o > 0; o >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, o, 64));
     mcnt = __builtin_amdgcn_readfirstlane(mcnt);
@@ -1664,6 +1767,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
             qdz, q_vi[wv][sl], vel4, b.x, seed, step, idq >> 2, b.y, qti, qtj, sigf, gam, sa, alf,
             pref, e_ff, vir, e_coul);
         const float sf = qs * MFA_FIX_SCALE;
+        if (HALF) hf.pair_s[(unsigned)b.w] = sf;  // the pair's slot (row * cap + entry)
         const float cx = truncf(sf * qdx), cy = truncf(sf * qdy), cz = truncf(sf * qdz);
         // |term| < 2^52, NaN excluded: then the int64 sums are exact
         if (fabsf(cx) < 4.5e15f && fabsf(cy) < 4.5e15f && fabsf(cz) < 4.5e15f) {
@@ -1682,7 +1786,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) {
         const int q = q0 + sub + u * TPP;
-        ev[u] = q < cnt ? row[q] : (unsigned)ss;
+        ev[u] = q < cnt ? row[HALF ? q : list_slot(q, nup, tot, cap)] : (unsigned)ss;
       }
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
@@ -1721,6 +1825,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
         const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
         const bool push = valid && ((in_lj && tb.y != 0.0f) || dpd || coul);
+        if (HALF && valid && !push) hf.pair_s[(size_t)ss * cap + (q0 + sub + u * TPP)] = 0.0f;
         const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
         if (push) {
           const int at = (tail + (int)__builtin_amdgcn_mbcnt_hi(
@@ -1731,7 +1836,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           q_rec[wv][at][1] = make_int4(js, jl,
                                        slot | (tj << 8) | ((in_lj ? 1 : 0) << 10) |
                                            ((dpd ? 1 : 0) << 11) | ((coul ? 1 : 0) << 12),
-                                       0);
+                                       HALF ? ss * cap + (q0 + sub + u * TPP) : 0);
         }
         tail += (int)__popcll(pm);
         if (tail - head >= 64) {
@@ -1752,8 +1857,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     }
   } else {
     if (active && (mask & TM_PAIR)) {
-      const int cnt = nn[ss];
+      const int wn = nn[ss], tot = wn & 0xffff, nup = wn >> 16;
+      const int cnt = HALF ? nup : tot;  // the half-list pass: upper entries only
       const unsigned* row = nl + (size_t)ss * cap;
+      float* srow = HALF ? hf.pair_s + (size_t)ss * cap : nullptr;
       // FM_BATCH candidates per lane and round: their list entries are loaded
       // together, then their views together, so each round waits for two
       // memory round trips instead of two per candidate.  Past the end of the
@@ -1764,7 +1871,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   #pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) {
           const int q = q0 + u * TPP;
-          ev[u] = q < cnt ? row[q] : (unsigned)ss;
+          ev[u] = q < cnt ? row[HALF ? q : list_slot(q, nup, tot, cap)] : (unsigned)ss;
         }
   #pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
@@ -1805,7 +1912,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
           // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
           const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
-          if (!in_lj && !dpd && !coul) continue;
+          if (!in_lj && !dpd && !coul) {
+            if (HALF) srow[q0 + u * TPP] = 0.0f;
+            continue;
+          }
           const float s = pair_scalar<XT, ENERGY>(
               k, tb, DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj], in_lj, dpd, coul, r2raw, r2min, dx,
               dy, dz, vi4, vel4, js, seed, step, il, jl, ti, tj, sigf, gam, sa, alf, pref, e_ff, vir,
@@ -1813,6 +1923,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           // the pair's three components in fixed point: (s 2^22) d equals
           // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
           const float sf = s * MFA_FIX_SCALE;
+          if (HALF) srow[q0 + u * TPP] = sf;
           ax += (double)truncf(sf * dx);
           ay += (double)truncf(sf * dy);
           az += (double)truncf(sf * dz);
@@ -1886,6 +1997,43 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     }
   }
 
+  if (HALF) {
+    // After a rebuild: each lower entry (i -> j, j below i in sorted order)
+    // finds its pair's slot in j's row, where i is an upper entry with the
+    // opposite cell offset code; the upper part of a row ascends in entry
+    // value, so a bisection over it finds the slot.
+    if (mirror_s != 0 && active) {
+      const int wn = nn[ss], tot = wn & 0xffff, nlo = tot - (wn >> 16);
+      const unsigned* row = nl + (size_t)ss * cap;
+      for (int q = sub; q < nlo; q += TPP) {
+        const int slot = cap - nlo + q;
+        const unsigned e = row[slot];
+        const int a = (int)(e & MFA_JMASK);
+        const unsigned key = (unsigned)ss | ((26u - (e >> MFA_JBITS)) << MFA_JBITS);
+        const unsigned* ra = nl + (size_t)a * cap;
+        const int na = nn[a] >> 16;
+        int lo = 0, hi = na;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (ra[mid] < key) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo >= na || ra[lo] != key) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_INTERNAL);
+        hf.mir[(size_t)ss * cap + slot] = (uint16_t)(lo < na ? lo : 0);
+      }
+    }
+    // the group's fixed-point sums, for the second pass
+    const bool bad = fix_bad(ax, ay, az) || qbad;
+    const double sx = group_sum<TPP>(ax), sy = group_sum<TPP>(ay), sz = group_sum<TPP>(az);
+    if (active && sub == 0) {
+      hf.fpart[3 * (size_t)ss + 0] = sx;
+      hf.fpart[3 * (size_t)ss + 1] = sy;
+      hf.fpart[3 * (size_t)ss + 2] = sz;
+    }
+    if (bad) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
+    return;
+  }
+
   // Combine the lane group and write F_i.
   const bool bad = fix_bad(ax, ay, az) || qbad;
   double fx = group_sum<TPP>(ax) * MFA_FIX_INV;
@@ -1920,38 +2068,104 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     }
     block_partials(vals, part);
   }
-  if (FUSE) {
-    // verlet_velocities of this step with F_i (k_fused_integrate's order:
-    // a = F/m, v += dt/2 a, pinned heads), then verlet_positions and
-    // check_skin of the next.  Positions are read from r (this step's
-    // buffer, which the whole pass reads) and written to fu.r_out; only this
-    // lane reads or writes particle i's v and mic.
-    bool over = false;
-    if (active && sub == 0) {
-      double fi[3] = {fx, fy, fz}, vi[3], ai[3], ri[3], rr[3];
-      int mi[3];
-      for (int d = 0; d < 3; ++d) {
-        vi[d] = fu.v_in[3 * (size_t)i + d];
-        ri[d] = r[3 * (size_t)i + d];
-        mi[d] = fu.mic_in[3 * (size_t)i + d];
-        rr[d] = fu.r_ref[3 * (size_t)i + d];
-      }
-      vv_one(k, il, ti, fi, vi, ai);
-      bool wrapped = false;
-      vp_one(k, ri, vi, ai, mi, wrapped);
-      for (int d = 0; d < 3; ++d) {
-        fu.r_out[3 * (size_t)i + d] = ri[d];
-        fu.v_out[3 * (size_t)i + d] = vi[d];
-      }
-      if (wrapped)
-        for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
-      write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
-      over = skin_one(k, ri, rr, mi);
-    }
-    if (__builtin_amdgcn_ballot_w64(over) != 0ull && (threadIdx.x & 63) == 0)
-      atomicOr(&fu.rebuild_flags[k.rep * F_NFLAGS + F_REBUILD], 1);
-    zero_counts(fu.nc_tot, fu.cell_count);
+  if (FUSE) fused_epilogue(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
+}
+
+// Second pass of the half-list form.  Each lower entry of a row (i -> j, j
+// below i in sorted order) takes its pair's scaled scalar from the slot of
+// j's row that the first pass wrote for the same pair (found after each
+// rebuild, hf.mir) and forms its fixed-point terms from its own separation,
+// which is the negated separation j's row used (the FP32 view differences
+// and cell offsets negate exactly); truncation toward zero is odd, so each
+// term is exactly minus the other row's and F_i equals the full-list pass's
+// bit for bit.  Then the first pass's sums of i are added and F_i written
+// (MODE 0, as k_force_mixed MODE 0 writes it) or the step integrated (MODE
+// 2, as k_force_mixed MODE 2).  The pass also clears the replica's F_MIRROR:
+// the first pass, which reads it, has finished.
+template <int TPP, int MODE>
+__global__ __launch_bounds__(MFA_BLOCK) void k_force_lower(
+    const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
+    const float4* __restrict__ pos4, const int* __restrict__ perm, const unsigned* __restrict__ nl,
+    const int* __restrict__ nn, int cap, double* __restrict__ f, int mask,
+    int* __restrict__ flags_all, FuseArgs fu, HalfArgs hf) {
+#pragma clang fp contract(fast)
+  constexpr bool FUSE = MODE == 2;
+  const int sp0 = blockIdx.x * (MFA_BLOCK / TPP);
+  MFA_REPLICA(sp0);
+  int* flags = flags_all + k.rep * F_NFLAGS;
+  __shared__ float4 s_off[27];
+  __shared__ int err_s;
+  if (threadIdx.x < 27) {
+    const int c = threadIdx.x;
+    const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
+    s_off[c] = make_float4(-(float)dx * (float)k.r_bin[0], -(float)dy * (float)k.r_bin[1],
+                           -(float)dz * (float)k.r_bin[2], 0.0f);
   }
+  if (threadIdx.x == 64) err_s = flags[F_ERROR];
+  __syncthreads();
+  if (sp0 == k.p0 && threadIdx.x == 0) flags[F_MIRROR] = 0;
+  if (err_s != 0) return;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sp = gtid / TPP, sub = gtid % TPP;
+  const bool active = sp - k.p0 < k.n;
+  const int ss = active ? sp : k.p0;
+  const int i = perm[ss];
+  const int il = i - k.p0;
+  const float4 pi4 = pos4[ss];
+  const int ti = __float_as_int(pi4.w) & 3;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (active && (mask & TM_PAIR)) {
+    const int wn = nn[ss], nlo = (wn & 0xffff) - (wn >> 16);
+    const size_t base = (size_t)ss * cap + (size_t)(cap - nlo);
+    for (int q0 = sub; q0 < nlo; q0 += TPP * FM_BATCH) {
+      unsigned ev[FM_BATCH];
+      int mv[FM_BATCH];
+      float sv[FM_BATCH];
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) {
+        const int q = q0 + u * TPP;
+        ev[u] = q < nlo ? nl[base + q] : (unsigned)ss;
+        mv[u] = q < nlo ? (int)hf.mir[base + q] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u)
+        sv[u] = mv[u] >= 0 ? hf.pair_s[(size_t)(ev[u] & MFA_JMASK) * cap + mv[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) {
+        if (sv[u] == 0.0f) continue;  // no interaction (or a zero term)
+        const unsigned e = ev[u];
+        const float4 pj = pos4[(int)(e & MFA_JMASK)];
+        const float4 o = s_off[e >> MFA_JBITS];
+        const float dx = (pi4.x - pj.x) + o.x;
+        const float dy = (pi4.y - pj.y) + o.y;
+        const float dz = (pi4.z - pj.z) + o.z;
+        ax += (double)truncf(sv[u] * dx);
+        ay += (double)truncf(sv[u] * dy);
+        az += (double)truncf(sv[u] * dz);
+      }
+    }
+  }
+  if (active && sub == 0) {
+    ax += hf.fpart[3 * (size_t)ss + 0];
+    ay += hf.fpart[3 * (size_t)ss + 1];
+    az += hf.fpart[3 * (size_t)ss + 2];
+  }
+  const bool bad = fix_bad(ax, ay, az);
+  double fx = group_sum<TPP>(ax) * MFA_FIX_INV;
+  double fy = group_sum<TPP>(ay) * MFA_FIX_INV;
+  double fz = group_sum<TPP>(az) * MFA_FIX_INV;
+  if (active && sub == 0 && !FUSE) {
+    if (mask & TM_ACCUM) {
+      fx = f[3 * (size_t)i + 0] + fx;
+      fy = f[3 * (size_t)i + 1] + fy;
+      fz = f[3 * (size_t)i + 2] + fz;
+    }
+    f[3 * (size_t)i + 0] = fx;
+    f[3 * (size_t)i + 1] = fy;
+    f[3 * (size_t)i + 2] = fz;
+  }
+  if (bad) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
+  if (FUSE) fused_epilogue(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // ---------------------------------------------------------------------------
@@ -2448,10 +2662,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_dpd_fd(const KParams* __restrict_
   if (active && ti != 0) {
     const double xi = r[3 * (size_t)i + 0], yi = r[3 * (size_t)i + 1], zi = r[3 * (size_t)i + 2];
     const double vxi = v[3 * (size_t)i + 0], vyi = v[3 * (size_t)i + 1], vzi = v[3 * (size_t)i + 2];
-    const int cnt = nn[ss];
+    const int wn = nn[ss], cnt = wn & 0xffff, nup = wn >> 16;
     const unsigned* row = nl + (size_t)ss * cap;
     for (int q = sub; q < cnt; q += TPP) {
-      const int j = perm[row[q] & MFA_JMASK];
+      const int j = perm[row[list_slot(q, nup, cnt, cap)] & MFA_JMASK];
       const int tj = type[j] - 1;
       if (tj == 0) continue;
       double dx = pbc(xi - r[3 * (size_t)j + 0], k.L[0], k.invL[0]);
@@ -2683,12 +2897,24 @@ static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0
                      c->cell_count, c->cell_slots, c->cell_start, c->perm, c->sidx, c->r_ref,
                      c->flags, c->pos4, c->vel4);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  hipLaunchKernelGGL(k_build_list, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
-                     c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm, c->pos4,
-                     c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
+  if (c->list_split)
+    hipLaunchKernelGGL(k_build_list<true>, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c),
+                       c->stream, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
+                       c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
+  else
+    hipLaunchKernelGGL(k_build_list<false>, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c),
+                       c->stream, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
+                       c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   HIPCHK(c, hipGetLastError());
   return MFA_OK;
+}
+
+// The half-list form applies to the mixed pair passes without energies, at
+// 4 or more threads per particle, once its buffers exist for this capacity.
+static bool half_ready(const mfa_ctx* c, int mask) {
+  return c->half_list && c->list_split && c->precision == MFA_MIXED && c->tpp >= 4 &&
+         (mask & TM_PAIR) && !(mask & TM_ENERGY) && c->pair_s && c->half_cap == c->cap;
 }
 
 template <int TPP, int XT, bool Q>
@@ -2697,6 +2923,38 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
   if (c->precision == MFA_MIXED) {
     FuseArgs fu;
     memset(&fu, 0, sizeof fu);
+    HalfArgs hf = {c->pair_s, c->mir, c->fpart};
+    if constexpr (TPP >= 4) {
+      if (half_ready(c, mask)) {
+        // first pass: upper entries, bonds, walls; second: lower entries,
+        // then F (or, in a step, the integration)
+        hipLaunchKernelGGL((k_force_mixed<TPP, 3, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                           c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap,
+                           c->f, c->d_step, mask, c->part, c->flags, fu, hf);
+        if (mask & TM_FUSE) {
+          fu.r_out = c->r_alt;
+          fu.v_in = c->v;
+          fu.mic_in = c->mic;
+          fu.v_out = c->probe_v ? c->probe_v : c->v;
+          fu.mic_out = c->probe_mic ? c->probe_mic : c->mic;
+          fu.rebuild_flags = c->probe_flags ? c->probe_flags : c->flags;
+          fu.r_ref = c->r_ref;
+          fu.cell_of = c->cell_of;
+          fu.pos4_out = c->pos4_alt;
+          fu.vel4_out = c->vel4_alt;
+          fu.cell_count = c->probe_count ? c->probe_count : c->cell_count;
+          fu.nc_tot = c->n_cells;
+          hipLaunchKernelGGL((k_force_lower<TPP, 2>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                             c->d_rmap, c->r, c->pos4, c->perm, c->nl, c->nn, c->cap, c->f, mask,
+                             c->flags, fu, hf);
+        } else {
+          hipLaunchKernelGGL((k_force_lower<TPP, 0>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                             c->d_rmap, c->r, c->pos4, c->perm, c->nl, c->nn, c->cap, c->f, mask,
+                             c->flags, fu, hf);
+        }
+        return;
+      }
+    }
     if (mask & TM_FUSE) {
       fu.r_out = c->r_alt;
       fu.v_in = c->v;
@@ -2713,15 +2971,15 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
       fu.nc_tot = c->n_cells;
       hipLaunchKernelGGL((k_force_mixed<TPP, 2, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
-                         c->d_step, mask, c->part, c->flags, fu);
+                         c->d_step, mask, c->part, c->flags, fu, hf);
     } else if (mask & TM_ENERGY) {
       hipLaunchKernelGGL((k_force_mixed<TPP, 1, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
-                         c->d_step, mask, c->part, c->flags, fu);
+                         c->d_step, mask, c->part, c->flags, fu, hf);
     } else {
       hipLaunchKernelGGL((k_force_mixed<TPP, 0, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
-                         c->d_step, mask, c->part, c->flags, fu);
+                         c->d_step, mask, c->part, c->flags, fu, hf);
     }
   } else {
     hipLaunchKernelGGL((k_force64<TPP, XT & 3>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
@@ -2734,7 +2992,7 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
 // few list candidates interact) when the context has it on.
 template <int TPP, int XT>
 static void launch_force_tx(mfa_ctx* c, int mask) {
-  if constexpr ((XT & 4) != 0) {
+  if constexpr ((XT & 4) != 0 || XT == 0) {
     if (c->pair_queue && c->precision == MFA_MIXED) {
       launch_force_txq<TPP, XT, true>(c, mask);
       return;
@@ -2919,6 +3177,41 @@ static int ensure_list(mfa_ctx* c) {
 static void drop_graph(mfa_ctx* c) {
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
+}
+
+static void free_half(mfa_ctx* c) {
+  if (c->pair_s) hipFree(c->pair_s);
+  if (c->mir) hipFree(c->mir);
+  if (c->fpart) hipFree(c->fpart);
+  c->pair_s = nullptr;
+  c->mir = nullptr;
+  c->fpart = nullptr;
+  c->half_cap = 0;
+}
+
+// Buffers of the half-list form for the current list capacity: allocated
+// when the form is switched on and again when the capacity grows; kept when
+// it is switched off (the autotuner switches it back and forth).
+static int ensure_half(mfa_ctx* c) {
+  if (!c->half_list || (c->pair_s && c->half_cap == c->cap)) return MFA_OK;
+  if ((size_t)c->n * (size_t)c->cap > 0x7fffffffu)
+    return set_err(c, MFA_ERR_ARG, "half-list form: %d particles x list capacity %d exceed 2^31 slots",
+                   c->n, c->cap);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_half(c);
+  const size_t slots = (size_t)c->n * (size_t)c->cap;
+  if (hipMalloc((void**)&c->pair_s, slots * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&c->mir, slots * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc((void**)&c->fpart, 3 * (size_t)c->n * sizeof(double)) != hipSuccess) {
+    free_half(c);
+    return set_err(c, MFA_ERR_NOMEM, "hipMalloc of the half-list buffers (%zu slots) failed", slots);
+  }
+  c->half_cap = c->cap;
+  // no mirror positions in the new buffers yet
+  hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                     c->n_rep, (int)F_MIRROR, 1);
+  HIPCHK(c, hipGetLastError());
+  return MFA_OK;
 }
 
 // One step after its verlet_positions: rebuild when due, force pass, and
@@ -3457,6 +3750,8 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->precision = precision;
   c->tpp = 8;
   c->pair_queue = false;
+  c->half_list = false;
+  c->list_split = false;
   c->need_rebuild = true;
   c->timing = false;
   c->capturing = false;
@@ -3503,6 +3798,10 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
       nc += (int64_t)ps[q].n_bin[0] * ps[q].n_bin[1] * ps[q].n_bin[2];
     }
     c->lb_octants = nc > 0 && np >= (int64_t)LB_MIN_OCTANT * nc;
+  }
+  if (cap > MFA_CAP_MAX) {  // row counts are packed in 16 bits
+    delete c;
+    return set_err(nullptr, MFA_ERR_ARG, "list capacity %d beyond %d", cap, MFA_CAP_MAX);
   }
   c->cap = cap;
   c->cap_fixed = list_capacity > 0;
@@ -3654,6 +3953,7 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   if (!c) return MFA_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
+  free_half(c);
   void* ptrs[] = {c->r,   c->v,       c->a,          c->f,          c->r_ref, c->type,
                   c->mic, c->cell_of, c->cell_count, c->cell_slots, c->nl,    c->nn,
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
@@ -3694,10 +3994,27 @@ extern "C" int32_t mfa_set_threads_per_particle(mfa_ctx* c, int32_t tpp) {
   return MFA_OK;
 }
 
-extern "C" int32_t mfa_get_tuning(mfa_ctx* c, int32_t* tpp, int32_t* pair_queue, double* skin) {
+extern "C" int32_t mfa_set_half_list(mfa_ctx* c, int32_t on) {
+  if (!c) return MFA_ERR_ARG;
+  if ((on != 0) != c->half_list) drop_graph(c);
+  c->half_list = on != 0;
+  const int rc = ensure_half(c);
+  if (rc) c->half_list = false;
+  if (c->half_list != c->list_split) {
+    // the rows take the layout the form needs at a fresh build before the
+    // next pass (the pair set is the same either way)
+    c->list_split = c->half_list;
+    c->need_rebuild = true;
+  }
+  return rc;
+}
+
+extern "C" int32_t mfa_get_tuning(mfa_ctx* c, int32_t* tpp, int32_t* pair_queue, double* skin,
+                                  int32_t* half_list) {
   if (!c) return MFA_ERR_ARG;
   if (tpp) *tpp = c->tpp;
   if (pair_queue) *pair_queue = c->pair_queue ? 1 : 0;
+  if (half_list) *half_list = c->half_list ? 1 : 0;
   if (skin) *skin = c->p.skin;
   return MFA_OK;
 }
@@ -3861,10 +4178,12 @@ extern "C" int32_t mfa_get_pairs(mfa_ctx* c, int32_t* pairs, int64_t cap, int64_
   HIPCHK(c, hipMemcpy(nl.data(), c->nl, nl.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(perm.data(), c->perm, perm.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   int64_t m = 0;
-  for (int sp = 0; sp < n0; ++sp)
-    for (int q = 0; q < nn[(size_t)sp]; ++q) {
+  for (int sp = 0; sp < n0; ++sp) {
+    const int cnt = nn[(size_t)sp] & 0xffff, nup = nn[(size_t)sp] >> 16;
+    for (int q = 0; q < cnt; ++q) {
+      const int slot = q < nup ? q : q + (c->cap - cnt);  // list_slot
       const int i = perm[(size_t)sp];
-      const int j = perm[(size_t)(nl[(size_t)sp * c->cap + q] & MFA_JMASK)];
+      const int j = perm[(size_t)(nl[(size_t)sp * c->cap + slot] & MFA_JMASK)];
       if (i < j) {
         if (pairs && m < cap) {
           pairs[2 * m + 0] = i + 1;
@@ -3873,6 +4192,7 @@ extern "C" int32_t mfa_get_pairs(mfa_ctx* c, int32_t* pairs, int64_t cap, int64_
         ++m;
       }
     }
+  }
   *n_pairs = m;
   return MFA_OK;
 }
@@ -4413,9 +4733,11 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   const int reps = 20;
   double best = 1e300;
   int best_t = c->tpp;
-  bool best_q = c->pair_queue;
-  // the pair queue is a choice where the kernels have it (mixed, DPD_CUT_OFF)
+  bool best_q = c->pair_queue, best_h = c->half_list;
+  // the pair queue is a choice where the kernels have it (mixed, DPD_CUT_OFF),
+  // the half-list form in the mixed form at 4 or more threads per particle
   const int nq = c->precision == MFA_MIXED && c->p.dpd_cut_off_on ? 2 : 1;
+  const int nh = c->precision == MFA_MIXED ? 2 : 1;
   int64_t step = first_step;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
@@ -4423,13 +4745,17 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   const bool timing = c->timing;
   c->timing = false;
   int rc = MFA_OK;
-  for (int qq = 0; qq < 5 * nq && rc == MFA_OK; ++qq) {
+  for (int qq = 0; qq < 5 * nq * nh && rc == MFA_OK; ++qq) {
     const int q = qq % 5;
+    const bool half = (qq / 5) % nh == 1;
+    if (half && cand[q] < 4) continue;  // the half-list passes need 4 or more lanes
     c->tpp = cand[q];
-    if (nq == 2) {
-      c->pair_queue = qq >= 5;
-      drop_graph(c);
+    c->pair_queue = nq == 2 && qq / (5 * nh) == 1;
+    if (nh == 2) {
+      rc = mfa_set_half_list(c, half ? 1 : 0);
+      if (rc) break;
     }
+    drop_graph(c);
     // the candidate's steps (untimed; the first call captures its step block)
     rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
     step += c->graph_steps + 1;
@@ -4451,6 +4777,7 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
       best = ms;
       best_t = cand[q];
       best_q = c->pair_queue;
+      best_h = c->half_list;
     }
   }
   c->timing = timing;
@@ -4458,6 +4785,10 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   hipEventDestroy(e1);
   c->tpp = best_t;
   c->pair_queue = best_q;
+  if (nh == 2) {
+    const int rh = mfa_set_half_list(c, best_h ? 1 : 0);
+    if (!rc) rc = rh;
+  }
   drop_graph(c);
   if (rc) return rc;
   if (best_tpp) *best_tpp = best_t;
@@ -4477,6 +4808,8 @@ extern "C" int32_t mfa_set_skin(mfa_ctx* c, double skin) {
       cap = cq > cap ? cq : cap;
     }
   }
+  if (cap > MFA_CAP_MAX)  // row counts are packed in 16 bits
+    return set_err(c, MFA_ERR_ARG, "list capacity %d beyond %d at skin %g", cap, MFA_CAP_MAX, skin);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (cap > c->cap) {  // a longer list: grow its buffer (its contents are rebuilt)
     unsigned* nl = nullptr;
@@ -4486,6 +4819,8 @@ extern "C" int32_t mfa_set_skin(mfa_ctx* c, double skin) {
     hipFree(c->nl);
     c->nl = nl;
     c->cap = cap;
+    const int rc = ensure_half(c);  // the half-list buffers follow the capacity
+    if (rc) return rc;
   }
   for (size_t q = 0; q < ps.size(); ++q) kparams_skin(c->ks[q], ps[q]);
   HIPCHK(c, hipMemcpy(c->d_kp, c->ks.data(), c->ks.size() * sizeof(KParams), hipMemcpyHostToDevice));
@@ -4952,10 +5287,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_adj_force(
   for (int c = 0; c < ADJ_NO * ADJ_NP; ++c) gp[c] = 0.0;
 
   if (active) {
-    const int cnt = nn[ss];
+    const int wn = nn[ss], cnt = wn & 0xffff, nup = wn >> 16;
     const unsigned* row = nl + (size_t)ss * cap;
     for (int q = sub; q < cnt; q += TPP) {
-      const int j = perm[row[q] & MFA_JMASK];
+      const int j = perm[row[list_slot(q, nup, cnt, cap)] & MFA_JMASK];
       double d[3] = {xi - r[3 * (size_t)j + 0], yi - r[3 * (size_t)j + 1], zi - r[3 * (size_t)j + 2]};
       min_image(k, d[0], d[1], d[2]);
       const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
@@ -5319,6 +5654,9 @@ extern "C" int32_t mfa_adjoint_window(mfa_ctx* c, int64_t first_step, int32_t n_
         (rc = d2d(c, c->sidx, b.ck_sidx + (size_t)seg * n, n * sizeof(int))) ||
         (rc = d2d(c, c->r_ref, b.ck_rref + off, n3 * sizeof(double))))
       return rc;
+    // a restored list: the half-list pass must find its mirror positions again
+    hipLaunchKernelGGL(k_set_flag, dim3(grid_of(c->n_rep)), dim3(MFA_BLOCK), 0, c->stream, c->flags,
+                       c->n_rep, (int)F_MIRROR, 1);
     for (int s = s0 + 1; s <= s1; ++s) {
       const size_t so = (size_t)(s - s0 - 1) * n3;
       rc = adj_forward_step(c, first_step + s - 1, nullptr, b.sg_r + so, b.sg_u + so, b.sg_mic + so);

@@ -20,7 +20,7 @@ MFA_MIXED = 1
 ERRORS = {
     0: "OK", 1: "MFA_ERR_ARG", 2: "MFA_ERR_HIP", 3: "MFA_ERR_LIST_OVERFLOW",
     4: "MFA_ERR_CELL", 5: "MFA_ERR_NOMEM", 6: "MFA_ERR_NAN", 7: "MFA_ERR_IO",
-    8: "MFA_ERR_CELL_OVERFLOW",
+    8: "MFA_ERR_CELL_OVERFLOW", 9: "MFA_ERR_INTERNAL",
 }
 
 _d4 = C.c_double * NTYPE
@@ -179,7 +179,8 @@ SIGNATURES = {
     "mfa_params_set_skin": (C.c_int32, [C.c_void_p, C.c_double]),
     "mfa_set_skin": (C.c_int32, [_P, C.c_double]),
     "mfa_set_pair_queue": (C.c_int32, [_P, C.c_int32]),
-    "mfa_get_tuning": (C.c_int32, [_P, _ip, _ip, _dp]),
+    "mfa_set_half_list": (C.c_int32, [_P, C.c_int32]),
+    "mfa_get_tuning": (C.c_int32, [_P, _ip, _ip, _dp, _ip]),
     "mfa_autotune_skin": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_rebuild": (C.c_int32, [_P, C.c_int32, _dp]),
@@ -304,11 +305,17 @@ class Context:
     def set_pair_queue(self, on):
         self._chk(self.lib.mfa_set_pair_queue(self.h, 1 if on else 0), "mfa_set_pair_queue")
 
+    def set_half_list(self, on):
+        self._chk(self.lib.mfa_set_half_list(self.h, 1 if on else 0), "mfa_set_half_list")
+
     def tuning(self):
-        """The current choices: threads per particle, pair queue, list skin."""
-        t, q, sk = C.c_int32(), C.c_int32(), C.c_double()
-        self._chk(self.lib.mfa_get_tuning(self.h, C.byref(t), C.byref(q), C.byref(sk)), "mfa_get_tuning")
-        return {"threads_per_particle": t.value, "pair_queue": bool(q.value), "skin": sk.value}
+        """The current choices: threads per particle, pair queue, list skin,
+        half-list form."""
+        t, q, sk, hl = C.c_int32(), C.c_int32(), C.c_double(), C.c_int32()
+        self._chk(self.lib.mfa_get_tuning(self.h, C.byref(t), C.byref(q), C.byref(sk), C.byref(hl)),
+                  "mfa_get_tuning")
+        return {"threads_per_particle": t.value, "pair_queue": bool(q.value), "skin": sk.value,
+                "half_list": bool(hl.value)}
 
     def autotune_skin(self, first_step, n_steps, skins):
         """The fastest of `skins` over n_steps whole steps each (advances the
