@@ -769,44 +769,108 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_fill(
 // a particle's neighbours lie close to it in memory; the FP64 state keeps the
 // reference's particle order.
 //
+// Work items of a grid-strided rebuild kernel: workgroup b takes items b,
+// b + gridDim.x, ... (at most 256 of them; the grid is sized so).  Which of
+// them lie in a replica whose rebuild is live is found with one flag read
+// per item, all in flight at once (thread t reads item b + t gridDim.x),
+// so a step without a rebuild costs each workgroup one round trip.  An item
+// is `cells` consecutive cells (it may straddle two replicas: both ends are
+// read).  Returns the items' bits in live[4] (LDS), uniform after the barrier.
+__device__ __forceinline__ void live_items(int n_items, int cells, int n_cells, int n_rep,
+                                           const int* __restrict__ cell_rep,
+                                           const int* __restrict__ flags_all,
+                                           unsigned long long* live) {
+  const int it = blockIdx.x + threadIdx.x * gridDim.x;
+  bool on = false;
+  if (it < n_items) {
+    const int c0 = it * cells, c1 = min(c0 + cells, n_cells) - 1;
+    const int ra = n_rep == 1 ? 0 : cell_rep[c0], rb = n_rep == 1 ? 0 : cell_rep[c1];
+    on = (flags_all[ra * F_NFLAGS + F_LIVE] != 0 && flags_all[ra * F_NFLAGS + F_ERROR] == 0) ||
+         (flags_all[rb * F_NFLAGS + F_LIVE] != 0 && flags_all[rb * F_NFLAGS + F_ERROR] == 0);
+  }
+  const unsigned long long bits = __builtin_amdgcn_ballot_w64(on);
+  if ((threadIdx.x & 63) == 0) live[threadIdx.x >> 6] = bits;
+  __syncthreads();
+}
+
+// Grid of a grid-strided rebuild kernel over n_items: at most `cap`
+// workgroups, and at least enough that none takes more than 256 items.
+static int strided_grid(int n_items, int cap) {
+  const int need = (n_items + MFA_BLOCK - 1) / MFA_BLOCK;
+  const int g = n_items < cap ? n_items : cap;
+  return g > need ? g : need;
+}
+
 // One wave per cell, four cells per workgroup: orders each cell's particles
 // by id, records the permutation, writes the FP32 view of each (relative to
 // its cell's origin) at its sorted position and resets the skin reference
-// (binning.f90:125-127).
+// (binning.f90:125-127).  A workgroup takes the groups of four cells
+// blockIdx.x, blockIdx.x + gridDim.x, ..., so that a step without a rebuild
+// costs a small launch even for a replica batch's many cells.
+__device__ __forceinline__ void sort_cell(
+    int cg, int n_cells, int lane, int* ids, const KParams* __restrict__ kb,
+    const int* __restrict__ cell_rep, const double* __restrict__ r, const double* __restrict__ v,
+    const int* __restrict__ type, const int* __restrict__ mic, const int* __restrict__ cell_count,
+    const int* __restrict__ cell_slots, const int* __restrict__ cell_start, int* __restrict__ perm,
+    int* __restrict__ sidx, double* __restrict__ r_ref, const int* __restrict__ flags_all,
+    float4* __restrict__ pos4, float4* __restrict__ vel4) {
+  const KParams& k = kb[cell_rep[cg < n_cells ? cg : 0]];
+  const bool on = cg < n_cells && flags_all[k.rep * F_NFLAGS + F_LIVE] != 0 &&
+                  flags_all[k.rep * F_NFLAGS + F_ERROR] == 0;
+  const int m = on ? min(cell_count[cg], k.cell_cap) : 0;
+  for (int t = lane; t < m; t += 64) ids[t] = cell_slots[(size_t)cg * k.cell_cap + t];
+  __syncthreads();
+  if (on) {
+    const int c = cg - k.c0;
+    const int s0 = cell_start[cg];
+    for (int t = lane; t < m; t += 64) {
+      const int id = ids[t];
+      int rank = 0;
+      for (int u = 0; u < m; ++u) rank += ids[u] < id ? 1 : 0;
+      const int s = s0 + rank;
+      perm[s] = id;
+      sidx[id] = s;
+      double ri[3], vi[3];
+      for (int d = 0; d < 3; ++d) {
+        ri[d] = r[3 * (size_t)id + d];
+        vi[d] = v[3 * (size_t)id + d];
+        r_ref[3 * (size_t)id + d] = ri[d] + (double)mic[3 * (size_t)id + d] * k.L[d];
+      }
+      write_view(k, s, c, type[id] - 1, id - k.p0, ri, vi, pos4, vel4);
+    }
+  }
+  __syncthreads();  // the next group of cells reuses the ids
+}
+
+#define SORT_GRID 8192
 __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
-    const KParams* __restrict__ kb, const int* __restrict__ cell_rep, int n_cells,
+    const KParams* __restrict__ kb, const int* __restrict__ cell_rep, int n_cells, int n_rep,
     const double* __restrict__ r, const double* __restrict__ v, const int* __restrict__ type,
     const int* __restrict__ mic, const int* __restrict__ cell_count,
     const int* __restrict__ cell_slots, const int* __restrict__ cell_start,
     int* __restrict__ perm, int* __restrict__ sidx, double* __restrict__ r_ref,
     const int* __restrict__ flags_all, float4* __restrict__ pos4, float4* __restrict__ vel4) {
   extern __shared__ int sort_ids[];  // [4][cell_cap]
+  __shared__ unsigned long long live[MFA_BLOCK / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cg = blockIdx.x * 4 + w;
-  const KParams& k = kb[cell_rep[cg < n_cells ? cg : 0]];
-  const bool on = cg < n_cells && flags_all[k.rep * F_NFLAGS + F_LIVE] != 0 &&
-                  flags_all[k.rep * F_NFLAGS + F_ERROR] == 0;
-  int* ids = sort_ids + w * k.cell_cap;
-  const int m = on ? min(cell_count[cg], k.cell_cap) : 0;
-  for (int t = lane; t < m; t += 64) ids[t] = cell_slots[(size_t)cg * k.cell_cap + t];
-  __syncthreads();
-  if (!on) return;
-  const int c = cg - k.c0;
-  const int s0 = cell_start[cg];
-  for (int t = lane; t < m; t += 64) {
-    const int id = ids[t];
-    int rank = 0;
-    for (int u = 0; u < m; ++u) rank += ids[u] < id ? 1 : 0;
-    const int s = s0 + rank;
-    perm[s] = id;
-    sidx[id] = s;
-    double ri[3], vi[3];
-    for (int d = 0; d < 3; ++d) {
-      ri[d] = r[3 * (size_t)id + d];
-      vi[d] = v[3 * (size_t)id + d];
-      r_ref[3 * (size_t)id + d] = ri[d] + (double)mic[3 * (size_t)id + d] * k.L[d];
+  const int cell_cap = kb[0].cell_cap;  // the same for every replica
+  const int n_grp = (n_cells + 3) / 4;
+  if ((int)gridDim.x >= n_grp) {  // one group per workgroup: it checks its own replica
+    sort_cell((int)blockIdx.x * 4 + w, n_cells, lane, sort_ids + w * cell_cap, kb, cell_rep, r, v,
+              type, mic, cell_count, cell_slots, cell_start, perm, sidx, r_ref, flags_all, pos4, vel4);
+    return;
+  }
+  live_items(n_grp, 4, n_cells, n_rep, cell_rep, flags_all, live);
+  for (int q = 0; q < MFA_BLOCK / 64; ++q) {
+    unsigned long long b = __builtin_amdgcn_readfirstlane(live[q] & 0xffffffffull) |
+                           ((unsigned long long)__builtin_amdgcn_readfirstlane(live[q] >> 32) << 32);
+    while (b) {
+      const int t = q * 64 + __builtin_ctzll(b);
+      b &= b - 1;
+      const int grp = blockIdx.x + t * gridDim.x;
+      sort_cell(grp * 4 + w, n_cells, lane, sort_ids + w * cell_cap, kb, cell_rep, r, v, type, mic,
+                cell_count, cell_slots, cell_start, perm, sidx, r_ref, flags_all, pos4, vel4);
     }
-    write_view(k, s, c, type[id] - 1, id - k.p0, ri, vi, pos4, vel4);
   }
 }
 
@@ -844,8 +908,8 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 #define LB_MIN_OCTANT 16             // particles in a cell for the octant groups
 #define LB_PIECE 1024                // candidates staged in LDS at a time
 template <bool SPLIT>
-__global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
-    const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
+__device__ __forceinline__ void build_cell(
+    int cg, const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
     const int* __restrict__ cell_start, const int* __restrict__ perm,
     const float4* __restrict__ pos4, unsigned* __restrict__ nl, int* __restrict__ nn, int cap,
@@ -861,7 +925,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   __shared__ int crowded_s;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int cg = blockIdx.x;
   const KParams& k = kb[cell_rep[cg]];
   int* flags = flags_all + k.rep * F_NFLAGS;
   if (flags[F_LIVE] == 0 || flags[F_ERROR] != 0) return;  // uniform over the workgroup
@@ -1123,6 +1186,39 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
   if (lane == 0 && my_max > __hip_atomic_load(&flags[F_MAXNEIGH], __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT))
     atomicMax(&flags[F_MAXNEIGH], my_max);
+}
+
+// The list build's launch: a grid of at most LB_GRID workgroups (more if a
+// workgroup would take over 256 cells), each taking the cells blockIdx.x,
+// blockIdx.x + gridDim.x, ... of replicas with a live rebuild in turn, so
+// that a step without a rebuild costs a small launch even for the hundreds
+// of thousands of cells of a replica batch.
+#define LB_GRID 8192
+template <bool SPLIT>
+__global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
+    int n_cells, int n_rep, const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
+    const double* __restrict__ r, const int* __restrict__ cell_count,
+    const int* __restrict__ cell_start, const int* __restrict__ perm,
+    const float4* __restrict__ pos4, unsigned* __restrict__ nl, int* __restrict__ nn, int cap,
+    int* __restrict__ flags_all, int octants, int piece) {
+  __shared__ unsigned long long live[MFA_BLOCK / 64];
+  if ((int)gridDim.x >= n_cells) {  // one cell per workgroup: it checks its own replica
+    build_cell<SPLIT>(blockIdx.x, kb, cell_rep, r, cell_count, cell_start, perm, pos4, nl, nn, cap,
+                      flags_all, octants, piece);
+    return;
+  }
+  live_items(n_cells, 1, n_cells, n_rep, cell_rep, flags_all, live);
+  for (int q = 0; q < MFA_BLOCK / 64; ++q) {
+    unsigned long long b = __builtin_amdgcn_readfirstlane(live[q] & 0xffffffffull) |
+                           ((unsigned long long)__builtin_amdgcn_readfirstlane(live[q] >> 32) << 32);
+    while (b) {
+      const int t = q * 64 + __builtin_ctzll(b);
+      b &= b - 1;
+      build_cell<SPLIT>(blockIdx.x + t * gridDim.x, kb, cell_rep, r, cell_count, cell_start, perm,
+                        pos4, nl, nn, cap, flags_all, octants, piece);
+      __syncthreads();  // the next cell reuses the LDS
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1786,7 +1882,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) {
         const int q = q0 + sub + u * TPP;
-        ev[u] = q < cnt ? row[HALF ? q : list_slot(q, nup, tot, cap)] : (unsigned)ss;
+        ev[u] = q < cnt ? row[ENERGY ? list_slot(q, nup, tot, cap) : q] : (unsigned)ss;
       }
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
@@ -1871,7 +1967,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   #pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) {
           const int q = q0 + u * TPP;
-          ev[u] = q < cnt ? row[HALF ? q : list_slot(q, nup, tot, cap)] : (unsigned)ss;
+          ev[u] = q < cnt ? row[ENERGY ? list_slot(q, nup, tot, cap) : q] : (unsigned)ss;
         }
   #pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
@@ -2891,19 +2987,20 @@ static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0
                      c->perm, c->cell_of, c->cell_count, c->cell_slots, c->cell_start, c->flags,
                      c->fill_cnt, c->d_step, inc_step);
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  const int cb = (c->n_cells + 3) / 4;  // four cells (waves) per workgroup
+  const int cb = strided_grid((c->n_cells + 3) / 4, SORT_GRID);  // groups of four cells (waves)
   hipLaunchKernelGGL(k_cell_sort, dim3(cb), dim3(MFA_BLOCK), (size_t)4 * c->cell_cap * sizeof(int),
-                     c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->r, c->v, c->type, c->mic,
+                     c->stream, c->d_kp, c->d_cell_rep, c->n_cells, c->n_rep, c->r, c->v, c->type, c->mic,
                      c->cell_count, c->cell_slots, c->cell_start, c->perm, c->sidx, c->r_ref,
                      c->flags, c->pos4, c->vel4);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  const int lg = strided_grid(c->n_cells, LB_GRID);
   if (c->list_split)
-    hipLaunchKernelGGL(k_build_list<true>, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c),
-                       c->stream, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
+    hipLaunchKernelGGL(k_build_list<true>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
+                       c->n_cells, c->n_rep, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
                        c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
   else
-    hipLaunchKernelGGL(k_build_list<false>, dim3(c->n_cells), dim3(MFA_BLOCK), list_lds_bytes(c),
-                       c->stream, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
+    hipLaunchKernelGGL(k_build_list<false>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
+                       c->n_cells, c->n_rep, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
                        c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   HIPCHK(c, hipGetLastError());
@@ -2913,8 +3010,8 @@ static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0
 // The half-list form applies to the mixed pair passes without energies, at
 // 4 or more threads per particle, once its buffers exist for this capacity.
 static bool half_ready(const mfa_ctx* c, int mask) {
-  return c->half_list && c->list_split && c->precision == MFA_MIXED && c->tpp >= 4 &&
-         (mask & TM_PAIR) && !(mask & TM_ENERGY) && c->pair_s && c->half_cap == c->cap;
+  return c->list_split && c->precision == MFA_MIXED && c->tpp >= 4 && (mask & TM_PAIR) &&
+         !(mask & TM_ENERGY) && c->pair_s && c->half_cap == c->cap;
 }
 
 template <int TPP, int XT, bool Q>
@@ -3978,11 +4075,27 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   return MFA_OK;
 }
 
+// Rows are built split exactly when the half-list passes will read them
+// (half-list form on, mixed form, 4 or more threads per particle), so the
+// full-list passes without energies take row entries in place; the energy
+// pass and the FP64, DPD_VV and adjoint passes map split rows with
+// list_slot.  A change of layout takes a fresh build before the next pass
+// (the pair set is the same either way).
+static void sync_split(mfa_ctx* c) {
+  const bool want = c->half_list && c->precision == MFA_MIXED && c->tpp >= 4;
+  if (want != c->list_split) {
+    c->list_split = want;
+    c->need_rebuild = true;
+    drop_graph(c);
+  }
+}
+
 extern "C" int32_t mfa_set_precision(mfa_ctx* c, int32_t precision) {
   if (!c) return MFA_ERR_ARG;
   if (precision != MFA_FP64 && precision != MFA_MIXED)
     return set_err(c, MFA_ERR_ARG, "unknown precision %d", precision);
   c->precision = precision;
+  sync_split(c);
   return MFA_OK;
 }
 
@@ -3991,6 +4104,7 @@ extern "C" int32_t mfa_set_threads_per_particle(mfa_ctx* c, int32_t tpp) {
   if (tpp != 1 && tpp != 2 && tpp != 4 && tpp != 8 && tpp != 16)
     return set_err(c, MFA_ERR_ARG, "threads per particle must be 1,2,4,8 or 16");
   c->tpp = tpp;
+  sync_split(c);
   return MFA_OK;
 }
 
@@ -4000,12 +4114,7 @@ extern "C" int32_t mfa_set_half_list(mfa_ctx* c, int32_t on) {
   c->half_list = on != 0;
   const int rc = ensure_half(c);
   if (rc) c->half_list = false;
-  if (c->half_list != c->list_split) {
-    // the rows take the layout the form needs at a fresh build before the
-    // next pass (the pair set is the same either way)
-    c->list_split = c->half_list;
-    c->need_rebuild = true;
-  }
+  sync_split(c);
   return rc;
 }
 
@@ -4750,6 +4859,7 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
     const bool half = (qq / 5) % nh == 1;
     if (half && cand[q] < 4) continue;  // the half-list passes need 4 or more lanes
     c->tpp = cand[q];
+    sync_split(c);
     c->pair_queue = nq == 2 && qq / (5 * nh) == 1;
     if (nh == 2) {
       rc = mfa_set_half_list(c, half ? 1 : 0);
@@ -4785,6 +4895,7 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   hipEventDestroy(e1);
   c->tpp = best_t;
   c->pair_queue = best_q;
+  sync_split(c);
   if (nh == 2) {
     const int rh = mfa_set_half_list(c, best_h ? 1 : 0);
     if (!rc) rc = rh;
