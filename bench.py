@@ -75,12 +75,12 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     ctx.set_state(st0["r"], st0["v"], st0["a"], st0["t"], st0["mic"])
     # untimed: the autotuner picks the threads per particle of the force pass
     tpp, _ = ctx.autotune(first_step, 20)
-    first_step += 5 * (max(graph_steps, 0) + 1 + 20)
-    skin = p.skin
-    if tune_skin:
-        cand = [x for x in configs.SKIN_CANDIDATES if _fits(p, x)]
-        skin, _ = ctx.autotune_skin(first_step, 200, cand)
-        first_step += len(cand) * (max(graph_steps, 0) + 1 + 200)
+    first_step += autotune_trials(p, precision) * (max(graph_steps, 0) + 1 + 20)
+    # untimed: the cell-size autotuner picks the cell width and, where the
+    # workload leaves it open, the list skin (each pair over 200 whole steps)
+    skins = list(configs.SKIN_CANDIDATES) if tune_skin else [p.skin]
+    width, skin, _, first_step = ctx.autotune_cells(first_step, 200, configs.cell_width_candidates(p),
+                                                    skins)
     ctx.run_steps(first_step, warmup, want_obs=False)
     tuning = ctx.tuning()
     t0 = time.perf_counter()
@@ -95,9 +95,19 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     rebuild = ctx.time_rebuild(20)  # resets the skin reference: after get_state
     ctx.close()
     return dt, obs, {"force_ms": force_ms, "step_force_ms": step_force_ms, "tpp": tpp, "skin": skin,
+                     "cell_width": width, "n_bin": list(lib.with_cells(p, width).n_bin),
                      "pair_queue": tuning["pair_queue"], "half_list": tuning["half_list"],
                      "ewald_ms": ewald_ms, "ewald_parts_ms": ewald_parts,
                      "rebuild_ms": rebuild, "rebuild_idle_ms": rebuild_idle}, st
+
+
+def autotune_trials(p, precision):
+    """Candidates mfa_autotune runs: widths 1, 2, 4, 8, 16; in the mixed
+    form also the half-list form at 4, 8 and 16; for DPD_CUT_OFF systems in
+    the mixed form all of them again with the pair queue."""
+    if precision != lib.MFA_MIXED:
+        return 5
+    return 8 * (2 if p.dpd_cut_off_on else 1)
 
 
 def _fits(p, skin):
@@ -246,7 +256,8 @@ def bench_single(args, name):
     st_m["t"] = t
     if args.save_state:  # the state the timed steps left, for the counter runs
         np.savez(args.save_state, r=st_m["r"], v=st_m["v"], a=st_m["a"], mic=st_m["mic"], t=t,
-                 tpp=tm_m["tpp"], skin=tm_m["skin"], pair_queue=int(tm_m["pair_queue"]),
+                 tpp=tm_m["tpp"], skin=tm_m["skin"], cell_width=tm_m["cell_width"],
+                 pair_queue=int(tm_m["pair_queue"]),
                  half_list=int(tm_m["half_list"]))
         log("saved the timed state to %s" % args.save_state)
     err_max, err_rms = force_error_vs_fp64(p, st_m)
@@ -268,8 +279,9 @@ def bench_single(args, name):
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
+        measured_choice = not tm_m["half_list"] and tm_m["n_bin"] == list(p.n_bin)
         traffic, valu = (FORCE_COUNTERS.get((tm_m["tpp"], bool(tm_m["pair_queue"])), (None, None))
-                         if not tm_m["half_list"] else (None, None))
+                         if measured_choice else (None, None))
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
@@ -304,6 +316,8 @@ def bench_single(args, name):
     config["threads_per_particle"] = {"mixed": tm_m["tpp"], "fp64": tm_d["tpp"]}
     config["list_skin"] = {"mixed": tm_m["skin"], "fp64": tm_d["skin"],
                            "chosen_by": "autotuner" if tune else "the workload's spec"}
+    config["cells"] = {"mixed": tm_m["n_bin"], "fp64": tm_d["n_bin"], "reference": list(p.n_bin),
+                       "chosen_by": "autotuner"}
     config["pair_queue"] = {"mixed": tm_m["pair_queue"], "chosen_by": "autotuner"}
     config["half_list"] = {"mixed": tm_m["half_list"], "chosen_by": "autotuner"}
     extra = {
@@ -356,10 +370,13 @@ def bench_batch(args, n_rep=256):
             b.set_replica(q, r, v, np.zeros_like(r), t)
         tpp, ms_tune = b.autotune(1, 20)
         log("autotune: %d threads per particle (force pass %.3f ms)" % (tpp, ms_tune))
-        first = 1 + 5 * (args.graph_steps + 1 + 20)  # the steps the autotuner ran
+        first = 1 + autotune_trials(reps[0][0], prec) * (args.graph_steps + 1 + 20)  # steps it ran
+        # the reference grid (batches keep it); the skin autotuner over the
+        # candidates those cells hold
         cand = [x for x in configs.SKIN_CANDIDATES if all(_fits(rp[0], x) for rp in reps)]
         skin, ms_skin = b.autotune_skin(first, 100, cand)
         first += len(cand) * (args.graph_steps + 1 + 100)
+        width = None
         log("autotune: skin %.2f (%.3f ms per step)" % (skin, ms_skin))
         pair_queue = b.tuning()["pair_queue"]
         half_list = b.tuning()["half_list"]
@@ -401,6 +418,7 @@ def bench_batch(args, n_rep=256):
               "chain_length": [20, 100], "lateral_edge": [8, 40], "lz": 12.5, "dt": 0.005,
               "solvent_density": 3.0, "threads_per_particle": tpp,
               "list_skin": {"value": skin, "chosen_by": "autotuner"},
+              "cells": "the reference grid of each replica (batches are not cell-tuned)",
               "pair_queue": {"value": pair_queue, "chosen_by": "autotuner"},
               "half_list": {"value": half_list, "chosen_by": "autotuner"},
               "parallelism": "single GPU, replicas batched"}
@@ -517,6 +535,8 @@ def counter_workload(args):
     with lib.Context(p, lib.MFA_MIXED) as c:
         c.set_graph_steps(0)
         c.set_tpp(tpp)
+        if "cell_width" in st:
+            c.set_cells(float(st["cell_width"]))
         c.set_skin(float(st["skin"]))  # the autotuners' choices of the timed run
         c.set_pair_queue(bool(st["pair_queue"]))
         c.set_half_list(bool(st["half_list"]) if "half_list" in st else False)

@@ -193,6 +193,13 @@ int32_t mfa_derive(const mfa_inputs* in, mfa_params* out);
  * MFA_ERR_ARG if the list radius would exceed a cell's width.  Host-only. */
 int32_t mfa_params_set_skin(mfa_params* p, double skin);
 
+/* Another cell grid for derived params (no counterpart in the reference,
+ * whose make_binning_boxes.f90:42-47 takes the finest grid with cells at
+ * least r_cut_max + 0.4 wide): n_bin = int(L / width) per axis (n_bin,
+ * r_bin, inv_r_bin); MFA_ERR_ARG if an axis would have fewer than 3 cells
+ * or a cell would be narrower than the list radius.  Host-only. */
+int32_t mfa_params_set_cells(mfa_params* p, double width);
+
 /* Create/destroy a device context for n_part particles.  Replaces the
  * allocations of init_system.f90:228-299 and make_binning_boxes.f90:70-78.
  * precision: MFA_FP64 or MFA_MIXED.  list_capacity: full neighbour-list slots
@@ -437,6 +444,28 @@ int32_t mfa_set_skin(mfa_ctx* ctx, double skin);
  * per step (ms).  Call it during warm-up. */
 int32_t mfa_autotune_skin(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t n_cand,
                           const double* skins, double* best_skin, double* ms_per_step);
+
+/* Another cell grid for a running single-system context (no counterpart in
+ * the reference; replica batches are refused, MFA_ERR_ARG): the grid
+ * mfa_params_set_cells gives for `width`, the cell arrays sized for it, and
+ * fresh cells and list before the next force pass.  The pair set within the
+ * list radius does not depend on the grid; the list build's work (27 cells
+ * of candidates per particle), the largest list skin the cells hold and, in
+ * the mixed form, the rounding of the FP32 view (positions relative to the
+ * cell origin) do. */
+int32_t mfa_set_cells(mfa_ctx* ctx, double width);
+
+/* Cell-size and skin autotuner (single-system contexts, as mfa_set_cells):
+ * for each candidate cell width (at most 16) that gives at least 3 cells
+ * per axis, and each candidate
+ * skin (at most 64) that those cells hold, runs n_steps whole steps
+ * (rebuilds included) from first_step, advancing the state, timed by HIP
+ * events; keeps the fastest pair and returns it with its time per step (ms)
+ * and the step after the last one run.  Call it during warm-up. */
+int32_t mfa_autotune_cells(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t n_width,
+                           const double* widths, int32_t n_skin, const double* skins,
+                           double* best_width, double* best_skin, double* ms_per_step,
+                           int64_t* next_step);
 
 /*
  * Adjoint of the force-and-integrate step (no counterpart in the reference,

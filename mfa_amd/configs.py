@@ -172,7 +172,16 @@ def ensemble_replica(q, base_seed=4004, lz=12.5, solvent_density=3.0):
 
 # List skins the autotuner tries where a workload's spec leaves the skin open
 # (those wider than the cells allow are skipped).
-SKIN_CANDIDATES = (0.3, 0.4, 0.5, 0.6, 0.7)
+SKIN_CANDIDATES = (0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)
+# candidate cell widths of the cell-size autotuner, as multiples of the
+# reference rule's minimum width r_cut_max + 0.4 (make_binning_boxes.f90:42-47):
+# the reference's grid first, then coarser ones (which hold wider skins)
+CELL_WIDTH_FACTORS = (1.0, 1.08, 1.16, 1.25)
+
+
+def cell_width_candidates(p):
+    rule = p.r_cut_max + float(np.float32(0.4))
+    return [rule * f for f in CELL_WIDTH_FACTORS]
 
 WORKLOADS = {
     # plain FP64 DPD polymer brush (the CPU-runnable case)

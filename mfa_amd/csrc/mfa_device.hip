@@ -1796,8 +1796,14 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const int i = perm[ss];     // the particle (reference order)
   const int il = i - k.p0;    // within the replica (chains, heads, random keys)
   const int64_t step = *d_step;
-  const float r2min = (float)switch_on_r2min(k, step);
+  const double r2min_d = switch_on_r2min(k, step);
+  const float r2min = (float)r2min_d;
   const float rdpd2 = (float)k.r_cut_dpd_2, inv_rdpd = (float)(1.0 / sqrt(k.r_cut_dpd_2));
+  // DPD_CUT_OFF under the switch-on clamp: the reference tests the clamped
+  // r^2 of a pair inside the LJ range against the DPD range
+  // (fluid_fluid.f90:94-96,201), so no such pair has a DPD term while
+  // r2min >= r_cut_dpd^2 (decided in FP64, as the clamp is)
+  const bool lj_no_dpd = DC && r2min_d >= k.r_cut_dpd_2;
   const float sigf = (float)k.sig, gam = (float)k.friction1, margin = k.ff_margin;
   const float4 pi4 = pos4[ss];
   const float4 vi4 = vel4[ss];
@@ -1901,7 +1907,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const int tj = wj & 3, jl = wj >> 2;
         const float4 tb = s_tab[ti * MFA_NTYPE + tj];
         bool in_lj = r2raw < tb.x;
-        bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
+        bool in_dpd = DC ? r2raw < rdpd2 && !(in_lj && lj_no_dpd) : in_lj;
         bool in_coul = coul_rc && r2raw < rc2f;
         float dm = fabsf(r2raw - tb.x);
         if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
@@ -1915,7 +1921,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           min_image(k, ex, ey, ez);
           const double r2d = (ex * ex + ey * ey) + ez * ez;
           in_lj = r2d < k.range_2[ti][tj];
-          in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
+          in_dpd = DC ? r2d < k.r_cut_dpd_2 && !(in_lj && lj_no_dpd) : in_lj;
           in_coul = coul_rc && r2d < k.rc_coul_2;
         }
         const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
@@ -1986,7 +1992,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
           const float4 tb = s_tab[ti * MFA_NTYPE + tj];
           bool in_lj = r2raw < tb.x;
-          bool in_dpd = DC ? r2raw < rdpd2 : in_lj;
+          bool in_dpd = DC ? r2raw < rdpd2 && !(in_lj && lj_no_dpd) : in_lj;
           bool in_coul = coul_rc && r2raw < rc2f;
           // distance of r^2 to the nearest cut-off that decides something
           float dm = fabsf(r2raw - tb.x);
@@ -2001,7 +2007,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
             min_image(k, ex, ey, ez);
             const double r2d = (ex * ex + ey * ey) + ez * ez;
             in_lj = r2d < k.range_2[ti][tj];
-            in_dpd = DC ? r2d < k.r_cut_dpd_2 : in_lj;
+            in_dpd = DC ? r2d < k.r_cut_dpd_2 && !(in_lj && lj_no_dpd) : in_lj;
             in_coul = coul_rc && r2d < k.rc_coul_2;
           }
           // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
@@ -4942,13 +4948,15 @@ extern "C" int32_t mfa_set_skin(mfa_ctx* c, double skin) {
   return MFA_OK;
 }
 
-extern "C" int32_t mfa_autotune_skin(mfa_ctx* c, int64_t first_step, int32_t n_steps,
-                                     int32_t n_cand, const double* skins, double* best_skin,
-                                     double* ms_per_step) {
-  if (!c || n_steps < 1 || n_cand < 1 || n_cand > 64 || !skins) return MFA_ERR_ARG;
-  const double keep = c->p.skin;
-  double best = 1e300, best_s = keep;
-  int64_t step = first_step;
+// Times n_steps whole steps at each candidate skin that fits the cells
+// (after graph_steps + 1 untimed ones), from *step on (advanced past every
+// step run); the fastest skin and its time for the n_steps (ms, 1e300 if no
+// candidate fits).  Leaves the context at the last candidate tried.
+static int skin_trials(mfa_ctx* c, int64_t* step_io, int32_t n_steps, int32_t n_cand,
+                       const double* skins, double* best_s, double* best_ms) {
+  double best = 1e300;
+  *best_s = c->p.skin;
+  int64_t step = *step_io;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
@@ -4969,17 +4977,155 @@ extern "C" int32_t mfa_autotune_skin(mfa_ctx* c, int64_t first_step, int32_t n_s
     HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
     if (ms < best) {
       best = ms;
-      best_s = skins[q];
+      *best_s = skins[q];
     }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  *step_io = step;
+  *best_ms = best;
+  return rc;
+}
+
+extern "C" int32_t mfa_autotune_skin(mfa_ctx* c, int64_t first_step, int32_t n_steps,
+                                     int32_t n_cand, const double* skins, double* best_skin,
+                                     double* ms_per_step) {
+  if (!c || n_steps < 1 || n_cand < 1 || n_cand > 64 || !skins) return MFA_ERR_ARG;
+  int64_t step = first_step;
+  double best = 1e300, best_s = c->p.skin;
+  int rc = skin_trials(c, &step, n_steps, n_cand, skins, &best_s, &best);
   if (rc) return rc;
   if (best > 1e299) return set_err(c, MFA_ERR_ARG, "no candidate skin fits the cells");
   rc = mfa_set_skin(c, best_s);
   if (rc) return rc;
   if (best_skin) *best_skin = best_s;
   if (ms_per_step) *ms_per_step = best / (double)n_steps;
+  return check_flags(c);
+}
+
+extern "C" int32_t mfa_set_cells(mfa_ctx* c, double width) {
+  if (!c) return MFA_ERR_ARG;
+  // Replica batches keep the reference grid: the cell-size autotuner on the
+  // 256-replica batch ended in a GPU memory fault whose cause is not found.
+  if (c->n_rep > 1)
+    return set_err(c, MFA_ERR_ARG, "another cell grid: single-system contexts only");
+  std::vector<mfa_params> ps = c->ps;
+  for (size_t q = 0; q < ps.size(); ++q)
+    if (mfa_params_set_cells(&ps[q], width) != MFA_OK)
+      return set_err(c, MFA_ERR_ARG, "replica %zu: %s", q, mfa_last_error(nullptr));
+  // the replicas' cell ranges and the common cell capacity, as at creation
+  std::vector<KParams> ks = c->ks;
+  int cell_cap = 0;
+  int64_t ctot = 0, np = 0;
+  for (size_t q = 0; q < ps.size(); ++q) {
+    const mfa_params& p = ps[q];
+    const double dens = (double)p.n_part / (p.boundary[0] * p.boundary[1] * p.boundary[2]);
+    const double occ = dens * p.r_bin[0] * p.r_bin[1] * p.r_bin[2];
+    const int cc = (((int)(2.0 * occ + 32.0)) + 31) / 32 * 32;
+    cell_cap = cc > cell_cap ? cc : cell_cap;
+    KParams k = make_kparams(p);
+    k.p0 = ks[q].p0;
+    k.fg0 = ks[q].fg0;
+    k.rep = ks[q].rep;
+    k.c0 = (int)ctot;
+    ks[q] = k;
+    ctot += (int64_t)p.n_bin[0] * p.n_bin[1] * p.n_bin[2];
+    np += p.n_part;
+  }
+  if (cell_cap > LB_MAX_CELL)
+    return set_err(c, MFA_ERR_ARG, "cells of width %g too crowded for the list build (capacity %d)",
+                   width, cell_cap);
+  if (ctot > 0x7fffffff) return set_err(c, MFA_ERR_ARG, "too many cells");
+  for (KParams& k : ks) k.cell_cap = cell_cap;
+  const int old_cap = c->cell_cap;
+  c->cell_cap = cell_cap;
+  if (list_lds_bytes(c) > c->lds_limit) {
+    const size_t b = list_lds_bytes(c);
+    c->cell_cap = old_cap;
+    return set_err(c, MFA_ERR_ARG, "cells of width %g: the list build needs %zu bytes of LDS", width, b);
+  }
+  c->cell_cap = old_cap;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t nc = (size_t)ctot;
+  int32_t *count = nullptr, *slots = nullptr, *start = nullptr, *crep_d = nullptr;
+  if (hipMalloc((void**)&count, nc * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&slots, nc * (size_t)cell_cap * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&start, (nc + 1) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&crep_d, nc * sizeof(int32_t)) != hipSuccess) {
+    hipFree(count);
+    hipFree(slots);
+    hipFree(start);
+    hipFree(crep_d);
+    return set_err(c, MFA_ERR_NOMEM, "hipMalloc of the cell arrays (%zu cells) failed", nc);
+  }
+  std::vector<int> crep(nc, 0);
+  for (size_t q = 0; q < ks.size(); ++q)
+    for (int cc = 0; cc < ks[q].n_cells; ++cc) crep[(size_t)(ks[q].c0 + cc)] = (int)q;
+  if (hipMemcpy(crep_d, crep.data(), nc * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_kp, ks.data(), ks.size() * sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(count, 0, nc * sizeof(int32_t)) != hipSuccess) {
+    hipFree(count);
+    hipFree(slots);
+    hipFree(start);
+    hipFree(crep_d);
+    return set_err(c, MFA_ERR_HIP, "upload of the cell tables failed");
+  }
+  hipFree(c->cell_count);
+  hipFree(c->cell_slots);
+  hipFree(c->cell_start);
+  hipFree(c->d_cell_rep);
+  c->cell_count = count;
+  c->cell_slots = slots;
+  c->cell_start = start;
+  c->d_cell_rep = crep_d;
+  c->ks = ks;
+  c->ps = ps;
+  c->p = ps[0];
+  c->n_cells = (int)ctot;
+  c->cell_cap = cell_cap;
+  c->lb_octants = ctot > 0 && np >= (int64_t)LB_MIN_OCTANT * ctot;
+  c->need_rebuild = true;  // new cells (and a new list) before the next force pass
+  drop_graph(c);
+  return MFA_OK;
+}
+
+extern "C" int32_t mfa_autotune_cells(mfa_ctx* c, int64_t first_step, int32_t n_steps,
+                                      int32_t n_width, const double* widths, int32_t n_skin,
+                                      const double* skins, double* best_width, double* best_skin,
+                                      double* ms_per_step, int64_t* next_step) {
+  if (!c || n_steps < 1 || n_width < 1 || n_width > 16 || !widths || n_skin < 1 || n_skin > 64 ||
+      !skins)
+    return MFA_ERR_ARG;
+  int64_t step = first_step;
+  double best = 1e300, best_w = 0.0, best_s = c->p.skin;
+  int rc = MFA_OK;
+  double smin = skins[0];
+  for (int q = 1; q < n_skin; ++q) smin = skins[q] < smin ? skins[q] : smin;
+  for (int q = 0; q < n_width && rc == MFA_OK; ++q) {
+    // the skin of the last trial may not fit these cells: retry with the
+    // smallest candidate skin; a width that still does not fit is skipped
+    if (mfa_set_cells(c, widths[q]) != MFA_OK &&
+        (mfa_set_skin(c, smin) != MFA_OK || mfa_set_cells(c, widths[q]) != MFA_OK))
+      continue;
+    double s = 0.0, ms = 1e300;
+    rc = skin_trials(c, &step, n_steps, n_skin, skins, &s, &ms);
+    if (!rc && ms < best) {
+      best = ms;
+      best_w = widths[q];
+      best_s = s;
+    }
+  }
+  if (rc) return rc;
+  if (best > 1e299) return set_err(c, MFA_ERR_ARG, "no candidate cell width and skin fit");
+  // best_s fits best_w's cells and so any wider ones: set the skin first
+  if (mfa_set_skin(c, best_s) != MFA_OK) {
+    if ((rc = mfa_set_skin(c, smin))) return rc;
+  }
+  if ((rc = mfa_set_cells(c, best_w)) || (rc = mfa_set_skin(c, best_s))) return rc;
+  if (best_width) *best_width = best_w;
+  if (best_skin) *best_skin = best_s;
+  if (ms_per_step) *ms_per_step = best / (double)n_steps;
+  if (next_step) *next_step = step;
   return check_flags(c);
 }
 

@@ -68,6 +68,23 @@ extern "C" int32_t mfa_params_set_skin(mfa_params* p, double skin) {
   return MFA_OK;
 }
 
+extern "C" int32_t mfa_params_set_cells(mfa_params* p, double width) {
+  if (!p || !(width > 0.0)) return derive_fail("the cell width must be positive");
+  mfa_params q = *p;
+  for (int d = 0; d < 3; ++d) {
+    q.n_bin[d] = (int)(q.boundary[d] / width);
+    if (q.n_bin[d] < 3) return derive_fail("fewer than 3 binning cells along an axis");
+    q.r_bin[d] = q.boundary[d] / (double)q.n_bin[d];
+    q.inv_r_bin[d] = 1.0 / q.r_bin[d];
+    for (int i = 0; i < MFA_NTYPE; ++i)
+      for (int j = 0; j < MFA_NTYPE; ++j)
+        if (q.list_2[i][j] > q.r_bin[d] * q.r_bin[d])
+          return derive_fail("binning cell narrower than the list radius");
+  }
+  *p = q;
+  return MFA_OK;
+}
+
 extern "C" int32_t mfa_derive(const mfa_inputs* in, mfa_params* p) {
   if (!in || !p) return MFA_ERR_ARG;
   memset(p, 0, sizeof(*p));

@@ -229,6 +229,26 @@ module mfa_hip_bind
       real(c_double) :: skins(*)
       real(c_double) :: best_skin, ms_per_step
     end function
+    integer(c_int32_t) function mfa_set_cells(ctx, width) bind(C, name="mfa_set_cells")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: width
+    end function
+    integer(c_int32_t) function mfa_autotune_cells(ctx, first_step, n_steps, n_width, widths, &
+        n_skin, skins, best_width, best_skin, ms_per_step, next_step) bind(C, name="mfa_autotune_cells")
+      import :: c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: first_step
+      integer(c_int32_t), value :: n_steps, n_width, n_skin
+      real(c_double) :: widths(*), skins(*)
+      real(c_double) :: best_width, best_skin, ms_per_step
+      integer(c_int64_t) :: next_step
+    end function
+    integer(c_int32_t) function mfa_set_half_list(ctx, on) bind(C, name="mfa_set_half_list")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

@@ -182,6 +182,10 @@ SIGNATURES = {
     "mfa_set_half_list": (C.c_int32, [_P, C.c_int32]),
     "mfa_get_tuning": (C.c_int32, [_P, _ip, _ip, _dp, _ip]),
     "mfa_autotune_skin": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    "mfa_params_set_cells": (C.c_int32, [C.c_void_p, C.c_double]),
+    "mfa_set_cells": (C.c_int32, [_P, C.c_double]),
+    "mfa_autotune_cells": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_int32, _dp, C.c_int32, _dp,
+                                       _dp, _dp, _dp, C.POINTER(C.c_int64)]),
     "mfa_time_ewald": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_time_rebuild": (C.c_int32, [_P, C.c_int32, _dp]),
     "mfa_copy_probe": (C.c_int32, [C.c_int64, _dp]),
@@ -248,6 +252,17 @@ def derive(inputs):
     if rc:
         raise MfaError("mfa_derive: %s (%s)" % (lib.mfa_last_error(None).decode(), ERRORS[rc]))
     return p
+
+
+def with_cells(params, width):
+    """A copy of derived params with another cell grid (mfa_params_set_cells)."""
+    lib = load()
+    q = Params.from_buffer_copy(params)
+    rc = lib.mfa_params_set_cells(C.byref(q), width)
+    if rc:
+        raise MfaError("mfa_params_set_cells: %s (%s)" % (lib.mfa_last_error(None).decode(), ERRORS[rc]),
+                       rc)
+    return q
 
 
 def with_skin(params, skin):
@@ -326,6 +341,22 @@ class Context:
                                              cand.ctypes.data_as(_dp), C.byref(best), C.byref(ms)),
                   "mfa_autotune_skin")
         return best.value, ms.value
+
+    def set_cells(self, width):
+        self._chk(self.lib.mfa_set_cells(self.h, width), "mfa_set_cells")
+
+    def autotune_cells(self, first_step, n_steps, widths, skins):
+        """The fastest (cell width, skin) pair over n_steps whole steps each
+        (advances the state); the context keeps it.  Returns (width, skin,
+        ms per step, the step after the last one run)."""
+        w = np.ascontiguousarray(widths, dtype=np.float64)
+        sk = np.ascontiguousarray(skins, dtype=np.float64)
+        bw, bs, ms, nxt = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        self._chk(self.lib.mfa_autotune_cells(self.h, first_step, n_steps, len(w),
+                                              w.ctypes.data_as(_dp), len(sk), sk.ctypes.data_as(_dp),
+                                              C.byref(bw), C.byref(bs), C.byref(ms), C.byref(nxt)),
+                  "mfa_autotune_cells")
+        return bw.value, bs.value, ms.value, nxt.value
 
     def __init__(self, params, precision=MFA_FP64, list_capacity=0):
         self.lib = load()

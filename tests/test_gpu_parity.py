@@ -106,6 +106,29 @@ def test_forces_mixed(case):
     ctx.close()
 
 
+@pytest.mark.parametrize("case", ["brush_small", "brush_bend"])
+def test_forces_mixed_switch_on_dpd_cut_off(case):
+    """A DPD_CUT_OFF brush at a fresh start (FORCE_SWITCH_ON): early in the
+    ramp the clamped r^2 of every pair inside the LJ range exceeds the DPD
+    range, so those pairs have no DPD term (fluid_fluid.f90:94-96,201); the
+    mixed form must decide as the reference (and the FP64 form) does."""
+    inputs, p, r, v, t, a = make_case(case)
+    inputs.s_time = 1
+    inputs.force_switch_on = 1
+    p = lib.derive(inputs)
+    assert p.dpd_cut_off_on
+    s = oracle_sys(inputs, r, v, a, t)
+    s.binning()
+    for step in (3, 2000):  # clamp above the DPD range, then below it
+        fref, _ = s.forces(step=step)
+        ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+        ctx.energies(step=step)
+        per, rms = force_error(ctx.get_state()["f"], fref)
+        ctx.close()
+        print("%s step %d: mixed max %.2e rms %.2e" % (case, step, per, rms), flush=True)
+        assert rms < 1e-5, (step, per, rms)
+
+
 @pytest.mark.parametrize("precision", [lib.MFA_FP64, lib.MFA_MIXED])
 def test_mixed_threads_per_particle_invariant(precision):
     """Fixed-point sums do not depend on how the list is split across lanes."""

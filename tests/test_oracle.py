@@ -583,6 +583,34 @@ def test_params_set_skin_refuses_lists_wider_than_the_cells():
     assert max(max(row) for row in q.list_2) <= min(p.r_bin) ** 2
 
 
+@pytest.mark.parametrize("name", ["brush9k", "melt100k", "pe_brush27k"])
+def test_params_set_cells(name):
+    """mfa_params_set_cells: at the reference's cell width (r_cut_max + 0.4,
+    make_binning_boxes.f90:42-47) it gives mfa_derive's params byte for byte;
+    a wider width gives n_bin = int(L / width) with r_bin = L / n_bin and
+    leaves every other field alone; widths with fewer than 3 cells on an
+    axis, or cells narrower than the list radius, are refused."""
+    x = configs.WORKLOADS[name]["make"]()
+    p = lib.derive(x)
+    ref_w = p.r_cut_max + float(np.float32(0.4))
+    assert bytes(lib.with_cells(p, ref_w)) == bytes(p)
+    w = max(p.r_bin) + 0.3
+    q = lib.with_cells(p, w)
+    for d in range(3):
+        assert q.n_bin[d] == int(p.boundary[d] / w) and q.n_bin[d] < p.n_bin[d]
+        assert q.r_bin[d] == p.boundary[d] / q.n_bin[d] and q.inv_r_bin[d] == 1.0 / q.r_bin[d]
+    r = lib.Params.from_buffer_copy(q)
+    for d in range(3):
+        r.n_bin[d], r.r_bin[d], r.inv_r_bin[d] = p.n_bin[d], p.r_bin[d], p.inv_r_bin[d]
+    assert bytes(r) == bytes(p)
+    with pytest.raises(lib.MfaError) as ei:
+        lib.with_cells(p, 0.9 * np.sqrt(max(max(row) for row in p.list_2)))
+    assert "narrower than the list radius" in str(ei.value)
+    with pytest.raises(lib.MfaError) as ei:
+        lib.with_cells(p, min(p.boundary) / 2.5)
+    assert "fewer than 3" in str(ei.value)
+
+
 def test_ewald_plan_pe_brush27k_grid():
     w = configs.WORKLOADS["pe_brush27k"]
     x = w["make"]()
