@@ -46,14 +46,14 @@ FORCE_BYTES_PER_PARTICLE = 76
 # list, the skin reference, the rebuild) is this implementation's own.
 STEP_BYTES_PER_PARTICLE = 100
 # Counter figures of the mixed step's force pass (fused with the integration)
-# on this workload, per (threads per particle, pair queue) as the autotuners
-# chose them, measured by `bench.py --counters` under rocprofv3 on the state
+# on this workload, per (threads per particle, pair queue, cell grid) as the
+# autotuners chose them, measured by `bench.py --counters` under rocprofv3 on the state
 # the timed steps leave (profiles/force_pass_melt100k_hbm_traffic.md,
 # profiles/force_and_list_melt100k_sq_counters.md): HBM bytes per launch
 # (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a copy of known
 # size) and VALU wave-instructions per launch (SQ_INSTS_VALU).  The issue
 # peak: 1024 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz.
-FORCE_COUNTERS = {(8, False): (99.481e6 + 10.000e6, 25232977)}
+FORCE_COUNTERS = {(8, False, (15, 15, 15)): (97.283e6 + 9.846e6, 25383487)}
 VALU_PEAK_GINSTS = 1024 * 2.4 / 4
 
 
@@ -279,9 +279,9 @@ def bench_single(args, name):
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
-        measured_choice = not tm_m["half_list"] and tm_m["n_bin"] == list(p.n_bin)
-        traffic, valu = (FORCE_COUNTERS.get((tm_m["tpp"], bool(tm_m["pair_queue"])), (None, None))
-                         if measured_choice else (None, None))
+        key = (tm_m["tpp"], bool(tm_m["pair_queue"]), tuple(tm_m["n_bin"]))
+        traffic, valu = (FORCE_COUNTERS.get(key, (None, None)) if not tm_m["half_list"]
+                         else (None, None))
     else:
         config = {"n_particles": n, "chains": inputs.n_chain, "chain_length": inputs.n_mon,
                   "box": list(p.boundary), "dt": inputs.dt, "dpd_gamma": inputs.friction[0],
