@@ -53,7 +53,7 @@ STEP_BYTES_PER_PARTICLE = 100
 # (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a copy of known
 # size) and VALU wave-instructions per launch (SQ_INSTS_VALU).  The issue
 # peak: 1024 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz.
-FORCE_COUNTERS = {(8, False, (15, 15, 15)): (97.283e6 + 9.846e6, 25383487)}
+FORCE_COUNTERS = {(8, 0, (15, 15, 15)): (97.283e6 + 9.846e6, 25383487)}
 VALU_PEAK_GINSTS = 1024 * 2.4 / 4
 
 
@@ -62,8 +62,10 @@ def log(*a):
 
 
 def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tune_skin=True):
-    """Warm-up (the autotuners' choices of the force pass's width and, where
-    the workload leaves it open, of the list skin, then `warmup` steps), then
+    """Warm-up (the autotuners' choices of the force pass's width and list
+    walk, of the cell width and, where the workload leaves it open, of the
+    list skin, then the width and walk again on that grid; then `warmup`
+    steps), then
     the timed region: `steps` steps in one mfa_run_steps call
     (blocks of 16 steps captured in a hipGraph), state resident in HBM.  Then
     the average kernel duration of the force pass at the final state, alone
@@ -74,13 +76,16 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     ctx.set_graph_steps(graph_steps)
     ctx.set_state(st0["r"], st0["v"], st0["a"], st0["t"], st0["mic"])
     # untimed: the autotuner picks the threads per particle of the force pass
-    tpp, _ = ctx.autotune(first_step, 20)
-    first_step += autotune_trials(p, precision) * (max(graph_steps, 0) + 1 + 20)
+    tpp, _ = ctx.autotune(first_step, 200)
+    first_step += autotune_trials(p, precision) * (max(graph_steps, 0) + 1 + 200)
     # untimed: the cell-size autotuner picks the cell width and, where the
     # workload leaves it open, the list skin (each pair over 200 whole steps)
     skins = list(configs.SKIN_CANDIDATES) if tune_skin else [p.skin]
     width, skin, _, first_step = ctx.autotune_cells(first_step, 200, configs.cell_width_candidates(p),
                                                     skins)
+    # untimed: the width and list walk again, on the chosen grid and skin
+    tpp, _ = ctx.autotune(first_step, 200)
+    first_step += autotune_trials(p, precision) * (max(graph_steps, 0) + 1 + 200)
     ctx.run_steps(first_step, warmup, want_obs=False)
     tuning = ctx.tuning()
     t0 = time.perf_counter()
@@ -102,12 +107,16 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
 
 
 def autotune_trials(p, precision):
-    """Candidates mfa_autotune runs: widths 1, 2, 4, 8, 16; in the mixed
-    form also the half-list form at 4, 8 and 16; for DPD_CUT_OFF systems in
-    the mixed form all of them again with the pair queue."""
+    """Candidates mfa_autotune runs (it advances the state by graph_steps + 1
+    + n steps per candidate): widths 1, 2, 4, 8, 16 directly; in the mixed
+    form also with the skin part queued and, for DPD_CUT_OFF, everything
+    queued (systems without charges, bending or the Langevin force), and the
+    half-list form at 4, 8 and 16 lanes (for DPD_CUT_OFF also queued)."""
     if precision != lib.MFA_MIXED:
         return 5
-    return 8 * (2 if p.dpd_cut_off_on else 1)
+    qok = not p.coulomb and not p.bending and p.thermostat != 1
+    dc = bool(p.dpd_cut_off_on)
+    return 5 + (5 if qok else 0) + (5 if qok and dc else 0) + 3 + (3 if qok and dc else 0)
 
 
 def _fits(p, skin):
@@ -279,7 +288,7 @@ def bench_single(args, name):
                   "parallelism": "single GPU"}
         data = ("synthetic (seeded random-walk melt, Maxwell-Boltzmann velocities, relaxed "
                 "by the switch-on ramp before timing)")
-        key = (tm_m["tpp"], bool(tm_m["pair_queue"]), tuple(tm_m["n_bin"]))
+        key = (tm_m["tpp"], int(tm_m["pair_queue"]), tuple(tm_m["n_bin"]))
         traffic, valu = (FORCE_COUNTERS.get(key, (None, None)) if not tm_m["half_list"]
                          else (None, None))
     else:
@@ -368,9 +377,10 @@ def bench_batch(args, n_rep=256):
         b.set_graph_steps(args.graph_steps)
         for q, (p, r, v, t) in enumerate(reps):
             b.set_replica(q, r, v, np.zeros_like(r), t)
-        tpp, ms_tune = b.autotune(1, 20)
-        log("autotune: %d threads per particle (force pass %.3f ms)" % (tpp, ms_tune))
-        first = 1 + autotune_trials(reps[0][0], prec) * (args.graph_steps + 1 + 20)  # steps it ran
+        tpp, ms_tune = b.autotune(1, 60)
+        log("autotune: %d threads per particle, pair queue %d, half list %s (%.3f ms per step)" % (
+            tpp, b.tuning()["pair_queue"], b.tuning()["half_list"], ms_tune))
+        first = 1 + autotune_trials(reps[0][0], prec) * (args.graph_steps + 1 + 60)  # steps it ran
         # the reference grid (batches keep it); the skin autotuner over the
         # candidates those cells hold
         cand = [x for x in configs.SKIN_CANDIDATES if all(_fits(rp[0], x) for rp in reps)]
@@ -538,7 +548,7 @@ def counter_workload(args):
         if "cell_width" in st:
             c.set_cells(float(st["cell_width"]))
         c.set_skin(float(st["skin"]))  # the autotuners' choices of the timed run
-        c.set_pair_queue(bool(st["pair_queue"]))
+        c.set_pair_queue(int(st["pair_queue"]))
         c.set_half_list(bool(st["half_list"]) if "half_list" in st else False)
         c.set_state(st["r"], st["v"], st["a"], st["t"], st["mic"])
         for _ in range(args.rebuilds):

@@ -397,12 +397,17 @@ int32_t mfa_obs_init(mfa_ctx* ctx, int32_t hist_dim);
 int32_t mfa_observe(mfa_ctx* ctx, const mfa_obs* e, int32_t coul_in_total);
 int32_t mfa_obs_read(mfa_ctx* ctx, mfa_obs_sums* out, double* dens_z, double* f_on_heads);
 
-/* Pair queue of the mixed force pass (DPD_CUT_OFF systems; no counterpart
- * in the reference): on, each wave pushes the list candidates that interact
- * into a queue in LDS and evaluates them 64 at a time, so no lane idles
- * where few candidates interact; the forces are bit-identical either way
- * (exact fixed-point sums).  Off by default; the autotuner chooses. */
-int32_t mfa_set_pair_queue(mfa_ctx* ctx, int32_t on);
+/* Pair queue of the mixed force pass (no counterpart in the reference;
+ * systems without charges, bending or the Langevin force): each wave pushes
+ * the list candidates that interact into a queue in LDS and evaluates them
+ * 64 at a time, so no lane idles where few candidates interact.  mode 0:
+ * no queue; 1: every entry through the queue; 2: near/shell rows (the list
+ * build puts the entries inside the interaction range at the build in
+ * front, those in the skin behind them), the near part walked directly and
+ * the skin part, where few entries interact, through the queue.  The
+ * forces are bit-identical in every mode (exact fixed-point sums).  0 by
+ * default; the autotuner chooses. */
+int32_t mfa_set_pair_queue(mfa_ctx* ctx, int32_t mode);
 
 /* Half-list form of the mixed force pass (no counterpart in the reference,
  * whose binning.f90:81 list is a half list applied with +/- in
@@ -421,14 +426,17 @@ int32_t mfa_set_half_list(mfa_ctx* ctx, int32_t on);
 int32_t mfa_get_tuning(mfa_ctx* ctx, int32_t* tpp, int32_t* pair_queue, double* skin,
                        int32_t* half_list);
 
-/* Autotuner: for each number of threads per particle in {1, 2, 4, 8, 16}
- * (for mixed DPD_CUT_OFF systems with the pair queue off and on, for the
- * mixed form at 4 or more threads with the half-list form off and on) runs
- * n_steps from first_step (advancing the state, so call it during warm-up),
- * then times 20 force passes on the state they left; keeps the fastest and
- * returns it with its time per force pass (ms). */
+/* Autotuner: for each number of threads per particle in {1, 2, 4, 8, 16},
+ * and in the mixed form each way of walking the list (directly; the skin
+ * part of near/shell rows through the pair queue and, for DPD_CUT_OFF, all
+ * of it, for systems without charges, bending or the Langevin force; the
+ * half-list form at 4 or more threads, also with the queue for
+ * DPD_CUT_OFF), runs graph_steps + 1 untimed and then n_steps timed whole
+ * steps from first_step (advancing the state, so call it during warm-up),
+ * rebuilds included; keeps the fastest and returns it with its time per
+ * step (ms). */
 int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
-                     double* ms_per_pass);
+                     double* ms_per_step);
 
 /* Another list skin for every replica of a running context (no counterpart
  * in the reference, whose skin is fixed at 0.4): the constants as

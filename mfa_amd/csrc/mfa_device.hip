@@ -69,6 +69,10 @@ enum { F_REBUILD = 0, F_ERROR = 1, F_MAXNEIGH = 2, F_NREBUILD = 3, F_FILLDONE = 
 // nn[s] = (nup << 16) | (nup + nlo).  A kernel that walks the whole row maps
 // its q-th entry to list_slot(q, nup, cnt, cap).
 #define MFA_CAP_MAX 32767
+// Row layouts (what the front part holds): everything; the upper entries
+// (half-list form); the entries inside the interaction range at the build
+// (near/shell rows, the skin part in the back)
+enum { LK_PLAIN = 0, LK_HALF = 1, LK_NEAR = 2 };
 __device__ __forceinline__ int list_slot(int q, int nup, int cnt, int cap) {
   return q < nup ? q : q + (cap - cnt);
 }
@@ -86,6 +90,7 @@ struct KParams {
   double va[3];
   double inv_r_bin[3], r_bin[3];
   double skin_half_2;
+  double skin_2;  // list_2 - skin_2: the interaction range^2 of a type pair
   double fso_r2_init, fso_time;  // force switch-on ramp (0 = off)
   // Coulomb (SYSTEM 2/3): charge per type, erfc(sqrt(alpha_c) r) real space
   // with the prefactor inv_sqrt_pi (1/pi in the compat form,
@@ -156,13 +161,16 @@ struct mfa_ctx {
   int n;
   int cap;
   bool cap_fixed;
-  bool pair_queue;  // mixed pass with DPD_CUT_OFF: compact the interacting pairs per wave  // the capacity the caller chose at creation (no growth)
+  // mixed pass, how list entries are walked: 0 directly, 1 through the pair
+  // queue (the interacting pairs of a wave compacted), 2 the near part
+  // directly and the skin part through the queue (near/shell rows)
+  int pair_queue;
   // half-list pair passes of the mixed form: each pair evaluated once, from
   // the row of its lower sorted position; its scaled pair scalar kept per
   // upper entry (pair_s), the position of each lower entry's pair in its
   // neighbour's row (mir), and the first pass's fixed-point sums (fpart)
   bool half_list;
-  bool list_split;  // rows built split into upper and lower entries (for the half-list form)
+  int list_kind;  // row layout the builds make: LK_PLAIN, LK_HALF or LK_NEAR
   float* pair_s = nullptr;
   uint16_t* mir = nullptr;
   double* fpart = nullptr;
@@ -907,7 +915,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_cell_sort(
 #define LB_RING 128                  // queued candidates per group (a block of 64 adds at most 64)
 #define LB_MIN_OCTANT 16             // particles in a cell for the octant groups
 #define LB_PIECE 1024                // candidates staged in LDS at a time
-template <bool SPLIT>
+template <int KIND>
 __device__ __forceinline__ void build_cell(
     int cg, const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
@@ -921,6 +929,7 @@ __device__ __forceinline__ void build_cell(
   __shared__ int q_first[28], q_start[27];
   __shared__ float4 q_off[27];
   __shared__ float l2tab[MFA_NTYPE * MFA_NTYPE];
+  __shared__ float n2tab[MFA_NTYPE * MFA_NTYPE];  // interaction range^2 (near/shell rows)
   __shared__ int hlist[LB_WAVES][64];  // the wave's particles, group 0 then group 1
   __shared__ int crowded_s;
   const int lane = threadIdx.x & 63;
@@ -959,13 +968,14 @@ __device__ __forceinline__ void build_cell(
   if (threadIdx.x >= 32 && threadIdx.x < 32 + MFA_NTYPE * MFA_NTYPE) {
     const int q = threadIdx.x - 32;
     l2tab[q] = (float)k.list_2[q / MFA_NTYPE][q % MFA_NTYPE];
+    n2tab[q] = (float)(k.list_2[q / MFA_NTYPE][q % MFA_NTYPE] - k.skin_2);
   }
   if (threadIdx.x == 64) crowded_s = 0;
   for (int t = threadIdx.x; t < m; t += MFA_BLOCK) hpos[t] = pos4[hs0 + t];
   __syncthreads();
   if (c == 0 && threadIdx.x == 0) {
     atomicAdd(&flags[F_NREBUILD], 1);
-    if (SPLIT) atomicExch(&flags[F_MIRROR], 1);  // the half-list pass must find new mirror slots
+    if (KIND == LK_HALF) atomicExch(&flags[F_MIRROR], 1);  // the half-list pass must find new mirror slots
   }
   // the wave's two octants: particle lists and their counts
   const float hx = 0.5f * (float)k.r_bin[0], hy = 0.5f * (float)k.r_bin[1],
@@ -1073,11 +1083,14 @@ __device__ __forceinline__ void build_cell(
       const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
       if (mask) {
         const int base = __builtin_amdgcn_readlane(hc, slot);
-        if constexpr (SPLIT) {
-          // split rows (the half-list form): upper entries (a neighbour at a
-          // higher sorted position) go to the front of the row, lower ones
-          // to its back; both in lane order
-          const bool up = js_c > hs0 + h;
+        if constexpr (KIND != LK_PLAIN) {
+          // split rows: half-list rows put the upper entries (a neighbour at
+          // a higher sorted position) in the front, the lower ones in the
+          // back; near/shell rows put the entries inside the interaction
+          // range in the front, those in the skin in the back (an FP32 test:
+          // the split only steers how the force pass walks the row); both
+          // parts in lane order
+          const bool up = KIND == LK_HALF ? js_c > hs0 + h : d2 < n2tab[ti * MFA_NTYPE + tj];
           const unsigned long long mu = __builtin_amdgcn_ballot_w64(hit && up);
           const int bup = __builtin_amdgcn_readlane(hu, slot);
           const int tot = base + (int)__popcll(mask);
@@ -1176,7 +1189,7 @@ __device__ __forceinline__ void build_cell(
     const int g = lane < g_n[0] ? 0 : 1, t = lane - (g == 0 ? 0 : g_n[0]);
     const int h = crowded ? w + LB_WAVES * t : hlist[w][g_at[g] + t];
     // unsplit rows: every entry counts as upper (list_slot(q) = q)
-    nn[hs0 + h] = hc <= cap ? (hc | ((SPLIT ? hu : hc) << 16)) : cap;
+    nn[hs0 + h] = hc <= cap ? (hc | ((KIND != LK_PLAIN ? hu : hc) << 16)) : cap;
     if (hc > cap) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_LIST_OVERFLOW);
     my_max = hc;
   }
@@ -1194,7 +1207,7 @@ __device__ __forceinline__ void build_cell(
 // that a step without a rebuild costs a small launch even for the hundreds
 // of thousands of cells of a replica batch.
 #define LB_GRID 8192
-template <bool SPLIT>
+template <int KIND>
 __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     int n_cells, int n_rep, const KParams* __restrict__ kb, const int* __restrict__ cell_rep,
     const double* __restrict__ r, const int* __restrict__ cell_count,
@@ -1203,7 +1216,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     int* __restrict__ flags_all, int octants, int piece) {
   __shared__ unsigned long long live[MFA_BLOCK / 64];
   if ((int)gridDim.x >= n_cells) {  // one cell per workgroup: it checks its own replica
-    build_cell<SPLIT>(blockIdx.x, kb, cell_rep, r, cell_count, cell_start, perm, pos4, nl, nn, cap,
+    build_cell<KIND>(blockIdx.x, kb, cell_rep, r, cell_count, cell_start, perm, pos4, nl, nn, cap,
                       flags_all, octants, piece);
     return;
   }
@@ -1214,7 +1227,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_build_list(
     while (b) {
       const int t = q * 64 + __builtin_ctzll(b);
       b &= b - 1;
-      build_cell<SPLIT>(blockIdx.x + t * gridDim.x, kb, cell_rep, r, cell_count, cell_start, perm,
+      build_cell<KIND>(blockIdx.x + t * gridDim.x, kb, cell_rep, r, cell_count, cell_start, perm,
                         pos4, nl, nn, cap, flags_all, octants, piece);
       __syncthreads();  // the next cell reuses the LDS
     }
@@ -1753,7 +1766,7 @@ __device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, in
 // (pairs, bonds, walls, Langevin, bending) left in hf.fpart as fixed-point
 // sums for k_force_lower; after a rebuild it also finds, for each lower
 // entry, the slot of its pair in the neighbour's row (hf.mir).
-template <int TPP, int MODE, int XT, bool QUEUE>
+template <int TPP, int MODE, int XT, int QM>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     const KParams* __restrict__ kb, const int* __restrict__ rmap, const double* __restrict__ r,
     const float4* __restrict__ pos4, const float4* __restrict__ vel4, const int* __restrict__ perm,
@@ -1821,16 +1834,101 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   const bool coul_on = (XT & 1) && k.coulomb != 0;
   const bool coul_rc = (XT & 1) && k.rc_coul_2 > 0.0;  // own real-space cut-off
   bool qbad = false;  // a queued pair's fixed-point terms out of range
-  if constexpr (QUEUE) {
-    // The same pairs, with the wave's work compacted: each lane tests its
-    // candidates (as below) and pushes those with a term to evaluate into
-    // the wave's queue in LDS; whenever 64 are queued (and at the end) the
-    // whole wave evaluates them, one pair per lane, and adds each pair's
-    // three fixed-point terms to its particle's sums in LDS with 64-bit
-    // integer atomics.  The terms are the direct loop's, and integer sums do
-    // not depend on order, so the forces equal the direct loop's bit for
-    // bit; only the lanes are kept busy where few candidates interact (a
-    // DPD cut-off below the list radius).  Everything here is per wave, in
+  // A row's entries lie in two parts (list_slot): the front [0, n_front)
+  // and the back [cap - n_back, cap).  Near/shell rows hold in front the
+  // entries inside the interaction range at the build, in the back those in
+  // the skin; half-list rows hold in front the upper entries (the half-list
+  // first pass walks only those); plain rows have everything in front.  QM
+  // says how each part is walked: 0 both directly, 1 both through the pair
+  // queue, 2 the front directly and the back through the queue.
+  const int wn = active && (mask & TM_PAIR) ? nn[ss] : 0;
+  const int n_front = wn >> 16, n_back = HALF ? 0 : (wn & 0xffff) - (wn >> 16);
+  const unsigned* row = nl + (size_t)ss * cap;
+
+  // Walks entries rw[0, cnt) directly (their row slots slot0 + q).  FM_BATCH
+  // candidates per lane and round: their list entries are loaded together,
+  // then their views together, so each round waits for two memory round
+  // trips instead of two per candidate.  Past the end a slot reads the
+  // particle's own view and is skipped.
+  auto direct_range = [&](const unsigned* rw, int cnt, int slot0) {
+    float* srow = HALF ? hf.pair_s + (size_t)ss * cap + slot0 : nullptr;
+    for (int q0 = sub; q0 < cnt; q0 += TPP * FM_BATCH) {
+      unsigned ev[FM_BATCH];
+      float4 pv[FM_BATCH];
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) {
+        const int q = q0 + u * TPP;
+        ev[u] = q < cnt ? rw[q] : (unsigned)ss;
+      }
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u) {
+        if (q0 + u * TPP >= cnt) continue;
+        const unsigned e = ev[u];
+        const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
+        const float4 o = s_off[e >> MFA_JBITS];
+        const float4 pj = pv[u];
+        const float dx = (pi4.x - pj.x) + o.x;
+        const float dy = (pi4.y - pj.y) + o.y;
+        const float dz = (pi4.z - pj.z) + o.z;
+        const float r2raw = dx * dx + dy * dy + dz * dz;
+        const int wj = __float_as_int(pj.w);
+        const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
+        const float4 tb = s_tab[ti * MFA_NTYPE + tj];
+        bool in_lj = r2raw < tb.x;
+        bool in_dpd = DC ? r2raw < rdpd2 && !(in_lj && lj_no_dpd) : in_lj;
+        bool in_coul = coul_rc && r2raw < rc2f;
+        // distance of r^2 to the nearest cut-off that decides something
+        float dm = fabsf(r2raw - tb.x);
+        if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
+        if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
+        if (__builtin_expect(dm < margin, 0)) {
+          // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
+          const int j = k.p0 + jl;
+          double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
+          double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
+          double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
+          min_image(k, ex, ey, ez);
+          const double r2d = (ex * ex + ey * ey) + ez * ez;
+          in_lj = r2d < k.range_2[ti][tj];
+          in_dpd = DC ? r2d < k.r_cut_dpd_2 && !(in_lj && lj_no_dpd) : in_lj;
+          in_coul = coul_rc && r2d < k.rc_coul_2;
+        }
+        // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
+        const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
+        // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
+        const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
+        if (!in_lj && !dpd && !coul) {
+          if (HALF) srow[q0 + u * TPP] = 0.0f;
+          continue;
+        }
+        const float s = pair_scalar<XT, ENERGY>(
+            k, tb, DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj], in_lj, dpd, coul, r2raw, r2min, dx,
+            dy, dz, vi4, vel4, js, seed, step, il, jl, ti, tj, sigf, gam, sa, alf, pref, e_ff, vir,
+            e_coul);
+        // the pair's three components in fixed point: (s 2^22) d equals
+        // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
+        const float sf = s * MFA_FIX_SCALE;
+        if (HALF) srow[q0 + u * TPP] = sf;
+        ax += (double)truncf(sf * dx);
+        ay += (double)truncf(sf * dy);
+        az += (double)truncf(sf * dz);
+      }
+    }
+  };
+
+  if constexpr (QM != 0) {
+    // The pair queue: the same pairs, with the wave's work compacted.  Each
+    // lane tests its candidates (as the direct walk does) and pushes those
+    // with a term to evaluate into the wave's queue in LDS; whenever 64 are
+    // queued (and at the end of a part) the whole wave evaluates them, one
+    // pair per lane, and adds each pair's three fixed-point terms to its
+    // particle's sums in LDS with 64-bit integer atomics.  The terms are the
+    // direct walk's, and integer sums do not depend on order, so the forces
+    // equal the direct walk's bit for bit; only the lanes are kept busy where
+    // few candidates interact (a DPD cut-off below the list radius, the skin
+    // part of near/shell rows).  Everything here is per wave, in
     // wave-uniform control flow.
     constexpr int NS = 64 / TPP;  // particles (lane groups) per wave
     __shared__ int4 q_rec[MFA_BLOCK / 64][FMQ_CAP][2];
@@ -1845,13 +1943,6 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       q_acc[wv][slot][1] = 0ull;
       q_acc[wv][slot][2] = 0ull;
     }
-    const int wn = active && (mask & TM_PAIR) ? nn[ss] : 0;
-    const int tot = wn & 0xffff, nup = wn >> 16;
-    const int cnt = HALF ? nup : tot;  // the half-list pass: upper entries only
-    int mcnt = cnt;
-    for (int o = 32; o > 0; o >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, o, 64));
-    mcnt = __builtin_amdgcn_readfirstlane(mcnt);
-    const unsigned* row = nl + (size_t)ss * cap;
     int head = 0, tail = 0;  // queue positions, the same in every lane
     // evaluates queued pairs head .. head + n - 1, one per lane
     auto drain = [&](int n) {
@@ -1882,106 +1973,27 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     };
-    for (int q0 = 0; q0 < mcnt; q0 += TPP * FM_BATCH) {
-      unsigned ev[FM_BATCH];
-      float4 pv[FM_BATCH];
-#pragma unroll
-      for (int u = 0; u < FM_BATCH; ++u) {
-        const int q = q0 + sub + u * TPP;
-        ev[u] = q < cnt ? row[ENERGY ? list_slot(q, nup, tot, cap) : q] : (unsigned)ss;
-      }
-#pragma unroll
-      for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
-#pragma unroll
-      for (int u = 0; u < FM_BATCH; ++u) {
-        const bool valid = q0 + sub + u * TPP < cnt;
-        const unsigned e = ev[u];
-        const int js = (int)(e & MFA_JMASK);
-        const float4 o = s_off[e >> MFA_JBITS];
-        const float4 pj = pv[u];
-        const float dx = (pi4.x - pj.x) + o.x;
-        const float dy = (pi4.y - pj.y) + o.y;
-        const float dz = (pi4.z - pj.z) + o.z;
-        const float r2raw = dx * dx + dy * dy + dz * dz;
-        const int wj = __float_as_int(pj.w);
-        const int tj = wj & 3, jl = wj >> 2;
-        const float4 tb = s_tab[ti * MFA_NTYPE + tj];
-        bool in_lj = r2raw < tb.x;
-        bool in_dpd = DC ? r2raw < rdpd2 && !(in_lj && lj_no_dpd) : in_lj;
-        bool in_coul = coul_rc && r2raw < rc2f;
-        float dm = fabsf(r2raw - tb.x);
-        if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
-        if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
-        if (__builtin_expect(valid && dm < margin, 0)) {
-          // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
-          const int j = k.p0 + jl;
-          double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
-          double ey = r[3 * (size_t)i + 1] - r[3 * (size_t)j + 1];
-          double ez = r[3 * (size_t)i + 2] - r[3 * (size_t)j + 2];
-          min_image(k, ex, ey, ez);
-          const double r2d = (ex * ex + ey * ey) + ez * ez;
-          in_lj = r2d < k.range_2[ti][tj];
-          in_dpd = DC ? r2d < k.r_cut_dpd_2 && !(in_lj && lj_no_dpd) : in_lj;
-          in_coul = coul_rc && r2d < k.rc_coul_2;
-        }
-        const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
-        const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
-        const bool push = valid && ((in_lj && tb.y != 0.0f) || dpd || coul);
-        if (HALF && valid && !push) hf.pair_s[(size_t)ss * cap + (q0 + sub + u * TPP)] = 0.0f;
-        const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
-        if (push) {
-          const int at = (tail + (int)__builtin_amdgcn_mbcnt_hi(
-                                     (unsigned)(pm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u))) &
-                         (FMQ_CAP - 1);
-          q_rec[wv][at][0] = make_int4(__float_as_int(dx), __float_as_int(dy), __float_as_int(dz),
-                                       __float_as_int(r2raw));
-          q_rec[wv][at][1] = make_int4(js, jl,
-                                       slot | (tj << 8) | ((in_lj ? 1 : 0) << 10) |
-                                           ((dpd ? 1 : 0) << 11) | ((coul ? 1 : 0) << 12),
-                                       HALF ? ss * cap + (q0 + sub + u * TPP) : 0);
-        }
-        tail += (int)__popcll(pm);
-        if (tail - head >= 64) {
-          drain(64);
-          head += 64;
-        }
-      }
-    }
-    while (tail > head) {
-      const int n = tail - head < 64 ? tail - head : 64;
-      drain(n);
-      head += n;
-    }
-    if (sub == 0) {
-      ax += (double)(long long)q_acc[wv][slot][0];
-      ay += (double)(long long)q_acc[wv][slot][1];
-      az += (double)(long long)q_acc[wv][slot][2];
-    }
-  } else {
-    if (active && (mask & TM_PAIR)) {
-      const int wn = nn[ss], tot = wn & 0xffff, nup = wn >> 16;
-      const int cnt = HALF ? nup : tot;  // the half-list pass: upper entries only
-      const unsigned* row = nl + (size_t)ss * cap;
-      float* srow = HALF ? hf.pair_s + (size_t)ss * cap : nullptr;
-      // FM_BATCH candidates per lane and round: their list entries are loaded
-      // together, then their views together, so each round waits for two
-      // memory round trips instead of two per candidate.  Past the end of the
-      // row a slot reads the particle's own view and is skipped.
-      for (int q0 = sub; q0 < cnt; q0 += TPP * FM_BATCH) {
+    // Tests entries rw[0, cnt) (row slots slot0 + q) and queues the pairs
+    // with a term; every lane of the wave calls it.
+    auto queue_range = [&](const unsigned* rw, int cnt, int slot0) {
+      int mcnt = cnt;
+      for (int o = 32; o > 0; o >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, o, 64));
+      mcnt = __builtin_amdgcn_readfirstlane(mcnt);
+      for (int q0 = 0; q0 < mcnt; q0 += TPP * FM_BATCH) {
         unsigned ev[FM_BATCH];
         float4 pv[FM_BATCH];
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) {
-          const int q = q0 + u * TPP;
-          ev[u] = q < cnt ? row[ENERGY ? list_slot(q, nup, tot, cap) : q] : (unsigned)ss;
+          const int q = q0 + sub + u * TPP;
+          ev[u] = q < cnt ? rw[q] : (unsigned)ss;
         }
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < FM_BATCH; ++u) {
-          if (q0 + u * TPP >= cnt) continue;
+          const bool valid = q0 + sub + u * TPP < cnt;
           const unsigned e = ev[u];
-          const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
+          const int js = (int)(e & MFA_JMASK);
           const float4 o = s_off[e >> MFA_JBITS];
           const float4 pj = pv[u];
           const float dx = (pi4.x - pj.x) + o.x;
@@ -1989,16 +2001,15 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
           const float dz = (pi4.z - pj.z) + o.z;
           const float r2raw = dx * dx + dy * dy + dz * dz;
           const int wj = __float_as_int(pj.w);
-          const int tj = wj & 3, jl = wj >> 2;  // type and id within the replica
+          const int tj = wj & 3, jl = wj >> 2;
           const float4 tb = s_tab[ti * MFA_NTYPE + tj];
           bool in_lj = r2raw < tb.x;
           bool in_dpd = DC ? r2raw < rdpd2 && !(in_lj && lj_no_dpd) : in_lj;
           bool in_coul = coul_rc && r2raw < rc2f;
-          // distance of r^2 to the nearest cut-off that decides something
           float dm = fabsf(r2raw - tb.x);
           if (DC) dm = fminf(dm, fabsf(r2raw - rdpd2));
           if (coul_rc) dm = fminf(dm, fabsf(r2raw - rc2f));
-          if (__builtin_expect(dm < margin, 0)) {
+          if (__builtin_expect(valid && dm < margin, 0)) {
             // within the FP32 error of a cut-off: decide in FP64 (fluid_fluid.f90:84-94)
             const int j = k.p0 + jl;
             double ex = r[3 * (size_t)i + 0] - r[3 * (size_t)j + 0];
@@ -2010,28 +2021,47 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
             in_dpd = DC ? r2d < k.r_cut_dpd_2 && !(in_lj && lj_no_dpd) : in_lj;
             in_coul = coul_rc && r2d < k.rc_coul_2;
           }
-          // DPD_CUT_OFF excludes the heads (fluid_fluid.f90:179-185)
           const bool dpd = dpd_on && in_dpd && !(head_excl && (ti == 0 || tj == 0));
-          // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
           const bool coul = coul_on && (coul_rc ? in_coul : in_lj);
-          if (!in_lj && !dpd && !coul) {
-            if (HALF) srow[q0 + u * TPP] = 0.0f;
-            continue;
+          const bool push = valid && ((in_lj && tb.y != 0.0f) || dpd || coul);
+          if (HALF && valid && !push)
+            hf.pair_s[(size_t)ss * cap + (slot0 + q0 + sub + u * TPP)] = 0.0f;
+          const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
+          if (push) {
+            const int at = (tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                       (unsigned)(pm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u))) &
+                           (FMQ_CAP - 1);
+            q_rec[wv][at][0] = make_int4(__float_as_int(dx), __float_as_int(dy), __float_as_int(dz),
+                                         __float_as_int(r2raw));
+            q_rec[wv][at][1] = make_int4(js, jl,
+                                         slot | (tj << 8) | ((in_lj ? 1 : 0) << 10) |
+                                             ((dpd ? 1 : 0) << 11) | ((coul ? 1 : 0) << 12),
+                                         HALF ? ss * cap + (slot0 + q0 + sub + u * TPP) : 0);
           }
-          const float s = pair_scalar<XT, ENERGY>(
-              k, tb, DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj], in_lj, dpd, coul, r2raw, r2min, dx,
-              dy, dz, vi4, vel4, js, seed, step, il, jl, ti, tj, sigf, gam, sa, alf, pref, e_ff, vir,
-              e_coul);
-          // the pair's three components in fixed point: (s 2^22) d equals
-          // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
-          const float sf = s * MFA_FIX_SCALE;
-          if (HALF) srow[q0 + u * TPP] = sf;
-          ax += (double)truncf(sf * dx);
-          ay += (double)truncf(sf * dy);
-          az += (double)truncf(sf * dz);
+          tail += (int)__popcll(pm);
+          if (tail - head >= 64) {
+            drain(64);
+            head += 64;
+          }
         }
       }
+      while (tail > head) {
+        const int n = tail - head < 64 ? tail - head : 64;
+        drain(n);
+        head += n;
+      }
+    };
+    if (QM == 1) queue_range(row, n_front, 0);
+    else direct_range(row, n_front, 0);
+    queue_range(row + (cap - n_back), n_back, cap - n_back);
+    if (sub == 0) {
+      ax += (double)(long long)q_acc[wv][slot][0];
+      ay += (double)(long long)q_acc[wv][slot][1];
+      az += (double)(long long)q_acc[wv][slot][2];
     }
+  } else {
+    direct_range(row, n_front, 0);
+    direct_range(row + (cap - n_back), n_back, cap - n_back);
   }
 
   // Langevin, bonds and walls: lane 0 of the group, from the FP64 positions.
@@ -2844,6 +2874,7 @@ static void kparams_skin(KParams& k, const mfa_params& p) {
   for (int t = 0; t < MFA_NTYPE; ++t)
     for (int u = 0; u < MFA_NTYPE; ++u) k.list_2[t][u] = p.list_2[t][u];
   k.skin_half_2 = p.skin_half_2;
+  k.skin_2 = p.skin_2;
   {
     // FP32 separation from FP32 positions: the two coordinates, the raw
     // difference (|d| <= L before the image shift) and the FP32 box edge each
@@ -3000,12 +3031,16 @@ static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0
                      c->flags, c->pos4, c->vel4);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   const int lg = strided_grid(c->n_cells, LB_GRID);
-  if (c->list_split)
-    hipLaunchKernelGGL(k_build_list<true>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
+  if (c->list_kind == LK_HALF)
+    hipLaunchKernelGGL(k_build_list<LK_HALF>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
+                       c->n_cells, c->n_rep, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
+                       c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
+  else if (c->list_kind == LK_NEAR)
+    hipLaunchKernelGGL(k_build_list<LK_NEAR>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
                        c->n_cells, c->n_rep, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
                        c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
   else
-    hipLaunchKernelGGL(k_build_list<false>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
+    hipLaunchKernelGGL(k_build_list<LK_PLAIN>, dim3(lg), dim3(MFA_BLOCK), list_lds_bytes(c), c->stream,
                        c->n_cells, c->n_rep, c->d_kp, c->d_cell_rep, c->r, c->cell_count, c->cell_start, c->perm,
                        c->pos4, c->nl, c->nn, c->cap, c->flags, c->lb_octants ? 1 : 0, list_piece(c));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
@@ -3016,11 +3051,11 @@ static int launch_rebuild(mfa_ctx* c, hipEvent_t* ev = nullptr, int inc_step = 0
 // The half-list form applies to the mixed pair passes without energies, at
 // 4 or more threads per particle, once its buffers exist for this capacity.
 static bool half_ready(const mfa_ctx* c, int mask) {
-  return c->list_split && c->precision == MFA_MIXED && c->tpp >= 4 && (mask & TM_PAIR) &&
+  return c->list_kind == LK_HALF && c->precision == MFA_MIXED && c->tpp >= 4 && (mask & TM_PAIR) &&
          !(mask & TM_ENERGY) && c->pair_s && c->half_cap == c->cap;
 }
 
-template <int TPP, int XT, bool Q>
+template <int TPP, int XT, int Q>
 static void launch_force_txq(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
   if (c->precision == MFA_MIXED) {
@@ -3091,17 +3126,22 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
   }
 }
 
-// The mixed pass's pair queue (compiled for DPD_CUT_OFF systems only, where
-// few list candidates interact) when the context has it on.
+// The mixed pass's pair queue modes (compiled for DPD_CUT_OFF systems and
+// for systems with no extra terms, the cases where many list candidates do
+// not interact) when the context has one on.
 template <int TPP, int XT>
 static void launch_force_tx(mfa_ctx* c, int mask) {
   if constexpr ((XT & 4) != 0 || XT == 0) {
-    if (c->pair_queue && c->precision == MFA_MIXED) {
-      launch_force_txq<TPP, XT, true>(c, mask);
+    if (c->pair_queue == 1 && c->precision == MFA_MIXED) {
+      launch_force_txq<TPP, XT, 1>(c, mask);
+      return;
+    }
+    if (c->pair_queue == 2 && c->precision == MFA_MIXED) {
+      launch_force_txq<TPP, XT, 2>(c, mask);
       return;
     }
   }
-  launch_force_txq<TPP, XT, false>(c, mask);
+  launch_force_txq<TPP, XT, 0>(c, mask);
 }
 
 // XT bit 1: Coulomb real space and bending; bit 2: the Langevin force; bit 4
@@ -3852,9 +3892,9 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   c->ps.assign(ps, ps + n_rep);
   c->precision = precision;
   c->tpp = 8;
-  c->pair_queue = false;
+  c->pair_queue = 0;
   c->half_list = false;
-  c->list_split = false;
+  c->list_kind = LK_PLAIN;
   c->need_rebuild = true;
   c->timing = false;
   c->capturing = false;
@@ -4081,16 +4121,19 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
   return MFA_OK;
 }
 
-// Rows are built split exactly when the half-list passes will read them
-// (half-list form on, mixed form, 4 or more threads per particle), so the
-// full-list passes without energies take row entries in place; the energy
-// pass and the FP64, DPD_VV and adjoint passes map split rows with
-// list_slot.  A change of layout takes a fresh build before the next pass
-// (the pair set is the same either way).
+// The row layout the builds make: half-list rows exactly when the half-list
+// passes will read them (half-list form on, mixed form, 4 or more threads
+// per particle), near/shell rows for the pair-queue mode that queues the skin
+// part, plain rows otherwise.  Every pass walks both parts of a row (the
+// FP64, DPD_VV and adjoint passes through list_slot).  A change of layout
+// takes a fresh build before the next pass (the pair set is the same).
 static void sync_split(mfa_ctx* c) {
-  const bool want = c->half_list && c->precision == MFA_MIXED && c->tpp >= 4;
-  if (want != c->list_split) {
-    c->list_split = want;
+  const bool mixed = c->precision == MFA_MIXED;
+  const int want = c->half_list && mixed && c->tpp >= 4 ? LK_HALF
+                   : mixed && c->pair_queue == 2         ? LK_NEAR
+                                                         : LK_PLAIN;
+  if (want != c->list_kind) {
+    c->list_kind = want;
     c->need_rebuild = true;
     drop_graph(c);
   }
@@ -4128,16 +4171,17 @@ extern "C" int32_t mfa_get_tuning(mfa_ctx* c, int32_t* tpp, int32_t* pair_queue,
                                   int32_t* half_list) {
   if (!c) return MFA_ERR_ARG;
   if (tpp) *tpp = c->tpp;
-  if (pair_queue) *pair_queue = c->pair_queue ? 1 : 0;
+  if (pair_queue) *pair_queue = c->pair_queue;
   if (half_list) *half_list = c->half_list ? 1 : 0;
   if (skin) *skin = c->p.skin;
   return MFA_OK;
 }
 
-extern "C" int32_t mfa_set_pair_queue(mfa_ctx* c, int32_t on) {
-  if (!c || (on != 0 && on != 1)) return MFA_ERR_ARG;
-  if ((on != 0) != c->pair_queue) drop_graph(c);
-  c->pair_queue = on != 0;
+extern "C" int32_t mfa_set_pair_queue(mfa_ctx* c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return MFA_ERR_ARG;
+  if (mode != c->pair_queue) drop_graph(c);
+  c->pair_queue = mode;
+  sync_split(c);
   return MFA_OK;
 }
 
@@ -4842,17 +4886,30 @@ extern "C" int32_t mfa_obs_read(mfa_ctx* c, mfa_obs_sums* out, double* dens_z, d
 }
 
 extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
-                                double* ms_per_pass) {
+                                double* ms_per_step) {
   if (!c || n_steps < 1) return MFA_ERR_ARG;
   const int cand[5] = {1, 2, 4, 8, 16};
-  const int reps = 20;
   double best = 1e300;
-  int best_t = c->tpp;
-  bool best_q = c->pair_queue, best_h = c->half_list;
-  // the pair queue is a choice where the kernels have it (mixed, DPD_CUT_OFF),
-  // the half-list form in the mixed form at 4 or more threads per particle
-  const int nq = c->precision == MFA_MIXED && c->p.dpd_cut_off_on ? 2 : 1;
-  const int nh = c->precision == MFA_MIXED ? 2 : 1;
+  int best_t = c->tpp, best_q = c->pair_queue;
+  bool best_h = c->half_list;
+  // How the mixed pass walks the list, tried with every width: directly; the
+  // skin part through the pair queue and the queue for everything (compiled
+  // for systems without extra terms; the latter for DPD_CUT_OFF only, where
+  // few candidates interact); the half-list form (4 or more lanes), also
+  // with the queue for DPD_CUT_OFF.
+  struct Variant {
+    int queue;
+    bool half;
+  };
+  std::vector<Variant> vars = {{0, false}};
+  const bool mixed = c->precision == MFA_MIXED;
+  if (mixed) {
+    const bool qok = !c->extra_terms && !c->lgv_terms, dc = c->p.dpd_cut_off_on != 0;
+    if (qok) vars.push_back({2, false});
+    if (qok && dc) vars.push_back({1, false});
+    vars.push_back({0, true});
+    if (qok && dc) vars.push_back({1, true});
+  }
   int64_t step = first_step;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
@@ -4860,40 +4917,35 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   const bool timing = c->timing;
   c->timing = false;
   int rc = MFA_OK;
-  for (int qq = 0; qq < 5 * nq * nh && rc == MFA_OK; ++qq) {
-    const int q = qq % 5;
-    const bool half = (qq / 5) % nh == 1;
-    if (half && cand[q] < 4) continue;  // the half-list passes need 4 or more lanes
-    c->tpp = cand[q];
-    sync_split(c);
-    c->pair_queue = nq == 2 && qq / (5 * nh) == 1;
-    if (nh == 2) {
-      rc = mfa_set_half_list(c, half ? 1 : 0);
+  for (size_t vi = 0; vi < vars.size() && rc == MFA_OK; ++vi) {
+    for (int q = 0; q < 5 && rc == MFA_OK; ++q) {
+      if (vars[vi].half && cand[q] < 4) continue;  // the half-list passes need 4 or more lanes
+      c->tpp = cand[q];
+      c->pair_queue = vars[vi].queue;
+      if (mixed) rc = mfa_set_half_list(c, vars[vi].half ? 1 : 0);  // also sets the row layout
       if (rc) break;
-    }
-    drop_graph(c);
-    // the candidate's steps (untimed; the first call captures its step block)
-    rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
-    step += c->graph_steps + 1;
-    if (rc) break;
-    rc = mfa_run_steps(c, step, n_steps, nullptr);
-    step += n_steps;
-    if (rc) break;
-    // then its force pass, repeated on the state the steps left (the same
-    // list and step counter, so every repetition writes the same forces):
-    // one comparison on identical work, free of the rebuilds the steps hit
-    for (int w = 0; w < 3 && rc == MFA_OK; ++w) rc = launch_force(c, TM_ALL);
-    HIPCHK(c, hipEventRecord(e0, c->stream));
-    for (int w = 0; w < reps && rc == MFA_OK; ++w) rc = launch_force(c, TM_ALL);
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) {
-      best = ms;
-      best_t = cand[q];
-      best_q = c->pair_queue;
-      best_h = c->half_list;
+      sync_split(c);
+      drop_graph(c);
+      // the candidate's first steps (untimed; they capture its step block),
+      // then n_steps whole steps as the run makes them (the force pass as the
+      // step runs it, and the rebuilds with their row layout), timed
+      rc = mfa_run_steps(c, step, c->graph_steps + 1, nullptr);
+      step += c->graph_steps + 1;
+      if (rc) break;
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      rc = mfa_run_steps(c, step, n_steps, nullptr);
+      step += n_steps;
+      if (rc) break;
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+      HIPCHK(c, hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_t = cand[q];
+        best_q = vars[vi].queue;
+        best_h = vars[vi].half;
+      }
     }
   }
   c->timing = timing;
@@ -4901,15 +4953,15 @@ extern "C" int32_t mfa_autotune(mfa_ctx* c, int64_t first_step, int32_t n_steps,
   hipEventDestroy(e1);
   c->tpp = best_t;
   c->pair_queue = best_q;
-  sync_split(c);
-  if (nh == 2) {
+  if (mixed) {
     const int rh = mfa_set_half_list(c, best_h ? 1 : 0);
     if (!rc) rc = rh;
   }
+  sync_split(c);
   drop_graph(c);
   if (rc) return rc;
   if (best_tpp) *best_tpp = best_t;
-  if (ms_per_pass) *ms_per_pass = best / (double)reps;
+  if (ms_per_step) *ms_per_step = best / (double)n_steps;
   return check_flags(c);
 }
 

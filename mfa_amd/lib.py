@@ -317,8 +317,11 @@ class Context:
     def set_skin(self, skin):
         self._chk(self.lib.mfa_set_skin(self.h, skin), "mfa_set_skin")
 
-    def set_pair_queue(self, on):
-        self._chk(self.lib.mfa_set_pair_queue(self.h, 1 if on else 0), "mfa_set_pair_queue")
+    def set_pair_queue(self, mode):
+        """0 (or False): walk the list directly; 1 (or True): every entry
+        through the pair queue; 2: the near part directly and the skin part
+        through the queue (near/shell rows)."""
+        self._chk(self.lib.mfa_set_pair_queue(self.h, int(mode)), "mfa_set_pair_queue")
 
     def set_half_list(self, on):
         self._chk(self.lib.mfa_set_half_list(self.h, 1 if on else 0), "mfa_set_half_list")
@@ -329,7 +332,7 @@ class Context:
         t, q, sk, hl = C.c_int32(), C.c_int32(), C.c_double(), C.c_int32()
         self._chk(self.lib.mfa_get_tuning(self.h, C.byref(t), C.byref(q), C.byref(sk), C.byref(hl)),
                   "mfa_get_tuning")
-        return {"threads_per_particle": t.value, "pair_queue": bool(q.value), "skin": sk.value,
+        return {"threads_per_particle": t.value, "pair_queue": q.value, "skin": sk.value,
                 "half_list": bool(hl.value)}
 
     def autotune_skin(self, first_step, n_steps, skins):

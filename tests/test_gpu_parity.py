@@ -509,32 +509,64 @@ def test_autotune_skin_keeps_a_candidate():
     ctx.close()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("tpp", [1, 4, 16])
-@pytest.mark.parametrize("case", ["brush_small", "pe_brush_small"])
-def test_pair_queue_forces_bit_identical(case, tpp):
-    """The mixed pass with the wave's interacting pairs queued in LDS gives
-    the direct loop's forces bit for bit (the same fixed-point terms, summed
-    exactly), and the energies to rounding; trajectories of 40 steps with
-    rebuilds stay bit-identical."""
+@pytest.mark.parametrize("case", ["brush_small", "melt_small", "melt_switch_on", "pe_brush_small"])
+def test_pair_queue_forces_bit_identical(case, tpp, mode):
+    """The mixed pass with the wave's interacting pairs queued in LDS (mode
+    1: every entry; mode 2: near/shell rows, the skin part queued) gives the
+    direct walk's forces bit for bit (the same fixed-point terms, summed
+    exactly), and the energies to rounding; forces without energies and
+    trajectories of 40 steps with rebuilds stay bit-identical.  (Charged
+    systems have no queue: the mode is ignored there.)"""
     inputs, p, r, v, t, a = make_case(case)
     out = {}
-    for q in (False, True):
+    for q in (0, mode):
         ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED, tpp=tpp)
         ctx.set_pair_queue(q)
         assert ctx.tuning()["pair_queue"] == q
         e = ctx.energies(step=3)
         f = ctx.get_state()["f"]
+        ctx.time_force(3)  # the passes without energies, at the same step and list
+        fp = ctx.get_state()["f"]
+        if not p.coulomb:
+            assert np.array_equal(f, fp)
         obs = ctx.run_steps(4, 40)
-        out[q] = (f, e, ctx.get_state(), obs)
+        out[q] = (f, e, ctx.get_state(), obs, fp)
         ctx.close()
-    (f0, e0, s0, o0), (f1, e1, s1, o1) = out[False], out[True]
-    print("%s tpp %d: %d rebuilds" % (case, tpp, o0["n_rebuilds"]), flush=True)
-    assert np.array_equal(f0, f1)
+    (f0, e0, s0, o0, g0), (f1, e1, s1, o1, g1) = out[0], out[mode]
+    print("%s tpp %d mode %d: %d rebuilds" % (case, tpp, mode, o0["n_rebuilds"]), flush=True)
+    assert np.array_equal(f0, f1) and np.array_equal(g0, g1)
     for k in ("v_fluid_fluid", "virial", "v_coul"):
         assert rel(e0[k], e1[k]) < 1e-12 or abs(e0[k] - e1[k]) < 1e-12, (k, e0[k], e1[k])
     for k in ("r", "v", "f", "a", "mic"):
         assert np.array_equal(s0[k], s1[k]), k
     assert o0["n_rebuilds"] == o1["n_rebuilds"]
+
+
+def test_near_shell_queue_melt100k():
+    """The headline melt with near/shell rows and the skin part queued: one
+    force pass bit-identical to the direct walk on plain rows, and within
+    the mixed form's bar of the oracle."""
+    w = configs.WORKLOADS["melt100k"]
+    inputs = w["make"]()
+    p, r, v, t = configs.build(inputs, w["kind"], min_dist=w["min_dist"])
+    a = np.zeros_like(r)
+    out = {}
+    for mode in (0, 2):
+        ctx = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED, tpp=8)
+        ctx.set_pair_queue(mode)
+        ctx.energies(step=9)  # the step of the passes below
+        ms = ctx.time_force(3)
+        out[mode] = ctx.get_state()["f"]
+        print("melt100k mode %d: force pass %.4f ms" % (mode, ms), flush=True)
+        ctx.close()
+    assert np.array_equal(out[0], out[2])
+    s = oracle_sys(inputs, r, v, a, t)
+    s.binning()
+    fref, _ = s.forces(step=9)
+    per, rms = force_error(out[2], fref)
+    assert rms < 1e-5, (per, rms)
 
 
 def test_pair_queue_matches_oracle_brush9k():
