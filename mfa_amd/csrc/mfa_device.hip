@@ -928,8 +928,8 @@ __device__ __forceinline__ void build_cell(
   extern __shared__ float4 lds_lb[];
   __shared__ int q_first[28], q_start[27];
   __shared__ float4 q_off[27];
-  __shared__ float l2tab[MFA_NTYPE * MFA_NTYPE];
-  __shared__ float n2tab[MFA_NTYPE * MFA_NTYPE];  // interaction range^2 (near/shell rows)
+  // per type pair: list radius^2 and interaction range^2 (near/shell rows)
+  __shared__ float2 l2tab[MFA_NTYPE * MFA_NTYPE];
   __shared__ int hlist[LB_WAVES][64];  // the wave's particles, group 0 then group 1
   __shared__ int crowded_s;
   const int lane = threadIdx.x & 63;
@@ -967,8 +967,8 @@ __device__ __forceinline__ void build_cell(
   }
   if (threadIdx.x >= 32 && threadIdx.x < 32 + MFA_NTYPE * MFA_NTYPE) {
     const int q = threadIdx.x - 32;
-    l2tab[q] = (float)k.list_2[q / MFA_NTYPE][q % MFA_NTYPE];
-    n2tab[q] = (float)(k.list_2[q / MFA_NTYPE][q % MFA_NTYPE] - k.skin_2);
+    l2tab[q] = make_float2((float)k.list_2[q / MFA_NTYPE][q % MFA_NTYPE],
+                           (float)(k.list_2[q / MFA_NTYPE][q % MFA_NTYPE] - k.skin_2));
   }
   if (threadIdx.x == 64) crowded_s = 0;
   for (int t = threadIdx.x; t < m; t += MFA_BLOCK) hpos[t] = pos4[hs0 + t];
@@ -1023,7 +1023,7 @@ __device__ __forceinline__ void build_cell(
   const int total = q_first[27];
   const float margin = k.lf_margin;
   float l2max = 0.0f;
-  for (int q = 0; q < MFA_NTYPE * MFA_NTYPE; ++q) l2max = fmaxf(l2max, l2tab[q]);
+  for (int q = 0; q < MFA_NTYPE * MFA_NTYPE; ++q) l2max = fmaxf(l2max, l2tab[q].x);
   const float reach2 = l2max + 2.0f * margin;
   // each group's box: the bounding box of its particles' view positions
   float bmin[2][3], bmax[2][3];
@@ -1062,7 +1062,8 @@ __device__ __forceinline__ void build_cell(
       const int ti = __float_as_int(pi.w) & 3;
       const float dx = pi.x - pjx, dy = pi.y - pjy, dz = pi.z - pjz;
       const float d2 = dx * dx + dy * dy + dz * dz;
-      const float l2 = l2tab[ti * MFA_NTYPE + tj];
+      const float2 l2n2 = l2tab[ti * MFA_NTYPE + tj];
+      const float l2 = l2n2.x;
       const bool cand = live && js_c != hs0 + h;
       bool hit = cand && d2 < l2 - margin;
       const bool band = cand && !(d2 < l2 - margin) && d2 < l2 + margin;
@@ -1090,7 +1091,7 @@ __device__ __forceinline__ void build_cell(
           // range in the front, those in the skin in the back (an FP32 test:
           // the split only steers how the force pass walks the row); both
           // parts in lane order
-          const bool up = KIND == LK_HALF ? js_c > hs0 + h : d2 < n2tab[ti * MFA_NTYPE + tj];
+          const bool up = KIND == LK_HALF ? js_c > hs0 + h : d2 < l2n2.y;
           const unsigned long long mu = __builtin_amdgcn_ballot_w64(hit && up);
           const int bup = __builtin_amdgcn_readlane(hu, slot);
           const int tot = base + (int)__popcll(mask);
@@ -1683,10 +1684,14 @@ __device__ __forceinline__ float pair_scalar(const KParams& k, const float4 tb, 
   return s;
 }
 
-// Candidates per lane whose loads the mixed force pass issues together.
-#ifndef FM_BATCH
-#define FM_BATCH 4
-#endif
+// Candidates per lane whose loads the mixed force pass issues together: 2
+// below 16 lanes per particle (72 VGPRs, 7 waves per SIMD: the melt's
+// 6,250 waves at 4 lanes fit the chip in one round), 4 at 16 lanes (waves
+// enough at 5 per SIMD; more loads in flight per wave).
+template <int TPP>
+struct FmBatch {
+  static constexpr int value = TPP >= 16 ? 4 : 2;
+};
 // Pairs a wave's queue holds (a round adds at most 64 to fewer than 64).
 #define FMQ_CAP 128
 
@@ -1778,6 +1783,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
   constexpr bool FUSE = MODE == 2;
   constexpr bool HALF = MODE == 3;
   constexpr bool DC = (XT & 4) != 0;
+  constexpr int FM_BATCH = FmBatch<TPP>::value;
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   int* flags = flags_all + k.rep * F_NFLAGS;
   // cell offsets -(d)*r_bin of the 27 codes, and the FP32 pair tables
@@ -2222,6 +2228,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_lower(
     int* __restrict__ flags_all, FuseArgs fu, HalfArgs hf) {
 #pragma clang fp contract(fast)
   constexpr bool FUSE = MODE == 2;
+  constexpr int FM_BATCH = 4;  // slots per lane whose loads are issued together
   const int sp0 = blockIdx.x * (MFA_BLOCK / TPP);
   MFA_REPLICA(sp0);
   int* flags = flags_all + k.rep * F_NFLAGS;
