@@ -85,6 +85,8 @@ struct KParams {
   double range_2[MFA_NTYPE][MFA_NTYPE], inv_range_2[MFA_NTYPE][MFA_NTYPE];
   double e_shift[MFA_NTYPE][MFA_NTYPE], list_2[MFA_NTYPE][MFA_NTYPE];
   double r_cut_dpd_2, friction1, sig;
+  // 1 / range and 1 / r_cut_dpd (the DPD weight w_r = 1 - r / r_c)
+  double inv_range[MFA_NTYPE][MFA_NTYPE], inv_r_cut_dpd;
   double k_chain, inv_r_chain_2;
   double a_w, sigma_wall[MFA_NTYPE], a_wall[MFA_NTYPE], z_space_wall;
   double va[3];
@@ -1286,7 +1288,9 @@ __device__ __forceinline__ double pair_normal_f64(uint32_t seed, int64_t step, i
   mfa_pair_words(seed, step, i, j, &w0, &w1);
   const double u1 = (double)(w0 >> 8) * 0x1p-24 + 0x1p-25;
   const double u2 = (double)(w1 >> 8) * 0x1p-24;
-  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  // cos(2 pi u2) as cospi(2 u2): the argument is reduced exactly, within an
+  // ulp or two of cos(fl(2 pi) u2)
+  return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
 }
 __device__ __forceinline__ float pair_normal_f32(uint32_t seed, int64_t step, int i, int j) {
   uint32_t w0, w1;
@@ -1473,23 +1477,26 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
       // The clamped value also feeds the DPD terms, as in the reference.
       const double r2 = in_lj ? fmax(r2raw, r2min) : r2raw;
       bool dpd = false;
-      double w_arg = 0.0;  // r^2 * (1/rc^2) of the DPD weight
+      double inv_rc = 0.0;  // 1 / r_c of the DPD weight
       if (k.dpd_on) {
         if (!k.dpd_cut_off_on) {
           dpd = in_lj;
-          w_arg = r2 * k.inv_range_2[ti][tj];
+          inv_rc = k.inv_range[ti][tj];
         } else {
           const bool excl = k.dpd_head_excl && (ti == 0 || tj == 0);
           dpd = !excl && r2 < k.r_cut_dpd_2;
-          w_arg = r2 / k.r_cut_dpd_2;
+          inv_rc = k.inv_r_cut_dpd;
         }
       }
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
       const bool coul = (XT & 1) && k.coulomb && (k.rc_coul_2 > 0.0 ? r2raw < k.rc_coul_2 : in_lj);
       if (!in_lj && !dpd && !coul) continue;
       double fx = 0.0, fy = 0.0, fz = 0.0;
-      const double inv_r_2 = 1.0 / r2;
-      const double inv_sqrt_r_2 = sqrt(1.0 / r2);
+      // 1/r from one reciprocal square root, r and 1/r^2 from it: each within
+      // a few ulp of the reference's sqrt(r2), 1/r2 and sqrt(1/r2)
+      const double inv_sqrt_r_2 = rsqrt(r2);
+      const double inv_r_2 = inv_sqrt_r_2 * inv_sqrt_r_2;
+      const double rr = r2 * inv_sqrt_r_2;
       if (in_lj) {
         const double eps = k.epsil[ti][tj];
         const double r61 = k.sigma_2[ti][tj] * inv_r_2;
@@ -1504,7 +1511,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
       }
       if (dpd) {
         const double xi_n = pair_normal_f64(k.seed, step, il, j - k.p0);
-        const double wr = 1.0 - sqrt(w_arg);
+        const double wr = 1.0 - rr * inv_rc;
         const double wd = wr * wr;
         const double ex = dx * inv_sqrt_r_2, ey = dy * inv_sqrt_r_2, ez = dz * inv_sqrt_r_2;
         const double sr = (k.sig * wr) * xi_n;
@@ -1522,8 +1529,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
       }
       if (coul) {  // ewald_real.f90:15-32 with the clamped r_2
         const double qq = k.q_type[ti] * k.q_type[tj];
-        const double rr = sqrt(r2);
-        const double inv_r = 1.0 / rr;
+        const double inv_r = inv_sqrt_r_2;
         const double ec = erfc(k.sqrt_alpha * rr);
         e_coul += 0.5 * (qq * ec * inv_r);
         const double f_scal = qq * k.sqrt_alpha * k.inv_sqrt_pi * 2.0 * exp(-k.alpha_c * r2) * inv_r +
@@ -2953,10 +2959,12 @@ static KParams make_kparams(const mfa_params& p) {
       k.sigma_2[t][u] = p.sigma_2[t][u];
       k.range_2[t][u] = p.range_2[t][u];
       k.inv_range_2[t][u] = p.inv_range_2[t][u];
+      k.inv_range[t][u] = sqrt(p.inv_range_2[t][u]);
       k.e_shift[t][u] = p.e_shift[t][u];
     }
   }
   k.r_cut_dpd_2 = p.r_cut_dpd_2;
+  k.inv_r_cut_dpd = p.r_cut_dpd_2 > 0.0 ? 1.0 / sqrt(p.r_cut_dpd_2) : 0.0;
   k.friction1 = p.friction1;
   k.sig = p.sig;
   k.k_chain = p.k_chain;
