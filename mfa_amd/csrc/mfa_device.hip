@@ -1651,8 +1651,8 @@ template <int XT, bool ENERGY>
 __device__ __forceinline__ float pair_scalar(const KParams& k, const float4 tb, float inv_rc,
                                              bool in_lj, bool dpd, bool coul, float r2raw,
                                              float r2min, float dx, float dy, float dz,
-                                             const float4 vi4, const float4* __restrict__ vel4,
-                                             int js, uint32_t seed, int64_t step, int il, int jl,
+                                             const float4 vi4, const float4 vj,
+                                             uint32_t seed, int64_t step, int il, int jl,
                                              int ti, int tj, float sigf, float gam, float sa,
                                              float alf, float pref, double& e_ff, double& vir,
                                              double& e_coul) {
@@ -1676,7 +1676,6 @@ __device__ __forceinline__ float pair_scalar(const KParams& k, const float4 tb, 
     const float xi_n = pair_normal_f32(seed, step, il, jl);
     // w_r = 1 - r / rc with r = r^2 / r (the rsq above)
     const float wr = 1.0f - (r2 * inv_r) * inv_rc;
-    const float4 vj = vel4[js];
     const float rv = ((vi4.x - vj.x) * dx + (vi4.y - vj.y) * dy + (vi4.z - vj.z) * dz) * inv_r;
     s += (sigf * wr * xi_n - gam * (wr * wr) * rv) * inv_r;
   }
@@ -1874,11 +1873,17 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
       }
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) pv[u] = pos4[(int)(ev[u] & MFA_JMASK)];
+      // the candidates' velocities with their positions (the DPD term needs
+      // them for each interacting pair), so their latency is not exposed
+      // inside the pair term
+      float4 vv[FM_BATCH];
+#pragma unroll
+      for (int u = 0; u < FM_BATCH; ++u)
+        vv[u] = dpd_on ? vel4[(int)(ev[u] & MFA_JMASK)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
       for (int u = 0; u < FM_BATCH; ++u) {
         if (q0 + u * TPP >= cnt) continue;
         const unsigned e = ev[u];
-        const int js = (int)(e & MFA_JMASK);  // sorted position of the neighbour
         const float4 o = s_off[e >> MFA_JBITS];
         const float4 pj = pv[u];
         const float dx = (pi4.x - pj.x) + o.x;
@@ -1917,7 +1922,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         }
         const float s = pair_scalar<XT, ENERGY>(
             k, tb, DC ? inv_rdpd : s_inv_rc[ti * MFA_NTYPE + tj], in_lj, dpd, coul, r2raw, r2min, dx,
-            dy, dz, vi4, vel4, js, seed, step, il, jl, ti, tj, sigf, gam, sa, alf, pref, e_ff, vir,
+            dy, dz, vi4, vv[u], seed, step, il, jl, ti, tj, sigf, gam, sa, alf, pref, e_ff, vir,
             e_coul);
         // the pair's three components in fixed point: (s 2^22) d equals
         // (s d) 2^22 exactly (a power-of-two scale), truncated toward zero
@@ -1966,10 +1971,11 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
         const int sl = b.z & 0xff, qtj = (b.z >> 8) & 3;
         const int idq = q_id[wv][sl];
         const int qti = idq & 3;
+        const float4 qvj = (b.z >> 11) & 1 ? vel4[b.x] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const float qs = pair_scalar<XT, ENERGY>(
             k, s_tab[qti * MFA_NTYPE + qtj], DC ? inv_rdpd : s_inv_rc[qti * MFA_NTYPE + qtj],
             (b.z >> 10) & 1, (b.z >> 11) & 1, (b.z >> 12) & 1, __int_as_float(a.w), r2min, qdx, qdy,
-            qdz, q_vi[wv][sl], vel4, b.x, seed, step, idq >> 2, b.y, qti, qtj, sigf, gam, sa, alf,
+            qdz, q_vi[wv][sl], qvj, seed, step, idq >> 2, b.y, qti, qtj, sigf, gam, sa, alf,
             pref, e_ff, vir, e_coul);
         const float sf = qs * MFA_FIX_SCALE;
         if (HALF) hf.pair_s[(unsigned)b.w] = sf;  // the pair's slot (row * cap + entry)
