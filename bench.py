@@ -106,6 +106,13 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
                      "rebuild_ms": rebuild, "rebuild_idle_ms": rebuild_idle}, st
 
 
+def kernel_xt(p):
+    """The extra-terms bits of the mixed force kernel a system runs: 1 charges
+    or bending, 2 the Langevin force, 4 DPD_CUT_OFF."""
+    return ((1 if (p.coulomb or p.bending) else 0) | (2 if p.thermostat == 1 else 0) |
+            (4 if p.dpd_cut_off_on else 0))
+
+
 def autotune_trials(p, precision):
     """Candidates mfa_autotune runs (it advances the state by graph_steps + 1
     + n steps per candidate): widths 1, 2, 4, 8, 16 directly; in the mixed
@@ -302,18 +309,18 @@ def bench_single(args, name):
                 "velocities, relaxed by the 40-tau switch-on ramp before timing)")
         traffic = None
         valu = None
-    xt = 1 if (p.coulomb or p.bending) else 0
+    xt = kernel_xt(p)
     # the dominant kernel of the mixed step: the force pass fused with the
     # integration where the step runs it so (no k-space stage), else the
     # force pass alone
     fused = not p.coulomb
     if fused:
         roof = roofline(n, tm_m["step_force_ms"], traffic,
-                        kernel="k_force_mixed<%d,2,%d>" % (tm_m["tpp"], xt),
+                        kernel="k_force_mixed<%d,2,%d,%d>" % (tm_m["tpp"], xt, tm_m["pair_queue"]),
                         bytes_per_particle=STEP_BYTES_PER_PARTICLE)
     else:
         roof = roofline(n, tm_m["force_ms"], None,
-                        kernel="k_force_mixed<%d,0,%d>" % (tm_m["tpp"], xt))
+                        kernel="k_force_mixed<%d,0,%d,%d>" % (tm_m["tpp"], xt, tm_m["pair_queue"]))
     roof["force_alone_ms_avg"] = tm_m["force_ms"]
     roof["force_ms_avg_fp64"] = tm_d["force_ms"]
     if valu:
@@ -436,7 +443,8 @@ def bench_batch(args, n_rep=256):
         "sequential_value": ntot * args.steps / dt_s,
         "batched_over_sequential": dt_s / dt_b,
         "kernel_ms": {"force_in_step": force_ms, "rebuild_idle": rb_idle, "rebuild_all": rb_full},
-        "roofline": roofline(ntot, force_ms, None, kernel="k_force_mixed<%d,2,0>" % tpp,
+        "roofline": roofline(ntot, force_ms, None,
+                             kernel="k_force_mixed<%d,2,%d,%d>" % (tpp, kernel_xt(reps[0][0]), pair_queue),
                              bytes_per_particle=STEP_BYTES_PER_PARTICLE),
         "cpu_baseline": None,
     }
