@@ -1425,7 +1425,7 @@ __device__ double bend_angle(const KParams& k, const double* __restrict__ r, int
   return .5 * f_mod * delta_alpha;
 }
 
-template <int TPP, int XT>
+template <int TPP, int XT, bool ENERGY>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict__ kb,
                                                        const int* __restrict__ rmap,
                                                        const double* __restrict__ r,
@@ -1441,7 +1441,14 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   int* flags = flags_all + k.rep * F_NFLAGS;
   __shared__ int err_s;
+  // per type pair: range^2, epsilon, sigma^2, 1 / r_c of the DPD weight
+  __shared__ double4 s_pair[MFA_NTYPE * MFA_NTYPE];
   if (threadIdx.x == 0) err_s = flags[F_ERROR];
+  if (threadIdx.x >= 64 && threadIdx.x < 64 + MFA_NTYPE * MFA_NTYPE) {
+    const int q = threadIdx.x - 64, a = q / MFA_NTYPE, b = q % MFA_NTYPE;
+    s_pair[q] = make_double4(k.range_2[a][b], k.epsil[a][b], k.sigma_2[a][b],
+                             k.dpd_cut_off_on ? k.inv_r_cut_dpd : k.inv_range[a][b]);
+  }
   __syncthreads();
   if (err_s != 0) return;  // state already invalid: stop, the host reports it
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1472,20 +1479,19 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
       min_image(k, dx, dy, dz);
       const double r2raw = (dx * dx + dy * dy) + dz * dz;
       const int tj = type[j] - 1;
-      const bool in_lj = r2raw < k.range_2[ti][tj];
+      const double4 tp = s_pair[ti * MFA_NTYPE + tj];
+      const bool in_lj = r2raw < tp.x;
       // force switch-on clamp (fluid_fluid.f90:96); r2min = 0 leaves r2 as is.
       // The clamped value also feeds the DPD terms, as in the reference.
       const double r2 = in_lj ? fmax(r2raw, r2min) : r2raw;
       bool dpd = false;
-      double inv_rc = 0.0;  // 1 / r_c of the DPD weight
+      const double inv_rc = tp.w;  // 1 / r_c of the DPD weight
       if (k.dpd_on) {
         if (!k.dpd_cut_off_on) {
           dpd = in_lj;
-          inv_rc = k.inv_range[ti][tj];
         } else {
           const bool excl = k.dpd_head_excl && (ti == 0 || tj == 0);
           dpd = !excl && r2 < k.r_cut_dpd_2;
-          inv_rc = k.inv_r_cut_dpd;
         }
       }
       // ewald_real (fluid_fluid.f90:166): inside the LJ range (compat) or rc_coul
@@ -1498,16 +1504,16 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
       const double inv_r_2 = inv_sqrt_r_2 * inv_sqrt_r_2;
       const double rr = r2 * inv_sqrt_r_2;
       if (in_lj) {
-        const double eps = k.epsil[ti][tj];
-        const double r61 = k.sigma_2[ti][tj] * inv_r_2;
+        const double eps = tp.y;
+        const double r61 = tp.z * inv_r_2;
         const double r6 = (r61 * r61) * r61;
         const double r12 = r6 * r6;
-        e_ff += 0.5 * (eps * ((r12 - r6) - k.e_shift[ti][tj]));
+        if (ENERGY) e_ff += 0.5 * (eps * ((r12 - r6) - k.e_shift[ti][tj]));
         const double rd = (eps * (-12.0 * r12 + 6.0 * r6)) * inv_r_2;
         fx = -(rd * dx);
         fy = -(rd * dy);
         fz = -(rd * dz);
-        vir += 0.5 * (-(rd * r2raw));
+        if (ENERGY) vir += 0.5 * (-(rd * r2raw));
       }
       if (dpd) {
         const double xi_n = pair_normal_f64(k.seed, step, il, j - k.p0);
@@ -1531,7 +1537,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
         const double qq = k.q_type[ti] * k.q_type[tj];
         const double inv_r = inv_sqrt_r_2;
         const double ec = erfc(k.sqrt_alpha * rr);
-        e_coul += 0.5 * (qq * ec * inv_r);
+        if (ENERGY) e_coul += 0.5 * (qq * ec * inv_r);
         const double f_scal = qq * k.sqrt_alpha * k.inv_sqrt_pi * 2.0 * exp(-k.alpha_c * r2) * inv_r +
                               qq * ec * inv_r * inv_r;
         fx = fx + f_scal * dx * inv_r;
@@ -1574,7 +1580,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
         ax += sgn * (rd * dx);
         ay += sgn * (rd * dy);
         az += sgn * (rd * dz);
-        if (side == 1) {
+        if (ENERGY && side == 1) {
           e_bond += -0.5 * log(1.0 - x);
           vir += -(rd * r2);
         }
@@ -1622,7 +1628,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
     f[3 * (size_t)i + 2] = fz;
     if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
   }
-  if (mask & TM_ENERGY) {
+  if (ENERGY) {
     double vals[MFA_NTERMS];
     for (int t = 0; t < MFA_NTERMS; ++t) vals[t] = 0.0;
     if (active) {
@@ -3141,9 +3147,14 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
                          c->d_step, mask, c->part, c->flags, fu, hf);
     }
   } else {
-    hipLaunchKernelGGL((k_force64<TPP, XT & 3>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
-                       c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
-                       mask, c->part, c->flags);
+    if (mask & TM_ENERGY)
+      hipLaunchKernelGGL((k_force64<TPP, XT & 3, true>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
+                         c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
+                         mask, c->part, c->flags);
+    else
+      hipLaunchKernelGGL((k_force64<TPP, XT & 3, false>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                         c->d_kp, c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f,
+                         c->d_step, mask, c->part, c->flags);
   }
 }
 
