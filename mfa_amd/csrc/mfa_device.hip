@@ -1471,14 +1471,34 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   if (active && (mask & TM_PAIR)) {
     const int wn = nn[ss], cnt = wn & 0xffff, nup = wn >> 16;
     const unsigned* row = nl + (size_t)ss * cap;
-    for (int q = sub; q < cnt; q += TPP) {
-      const int j = perm[row[list_slot(q, nup, cnt, cap)] & MFA_JMASK];
-      double dx = xi - r[3 * (size_t)j + 0];
-      double dy = yi - r[3 * (size_t)j + 1];
-      double dz = zi - r[3 * (size_t)j + 2];
+    // two candidates per lane and round: their list entries, then their
+    // particles, then their positions and types are loaded together (the
+    // order of the lane's sums is that of the entries)
+    for (int q0 = sub; q0 < cnt; q0 += 2 * TPP) {
+      int jv[2], tv[2];
+      double xv[2], yv[2], zv[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int q = q0 + u * TPP;
+        jv[u] = q < cnt ? perm[row[list_slot(q, nup, cnt, cap)] & MFA_JMASK] : ii;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        xv[u] = r[3 * (size_t)jv[u] + 0];
+        yv[u] = r[3 * (size_t)jv[u] + 1];
+        zv[u] = r[3 * (size_t)jv[u] + 2];
+        tv[u] = type[jv[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+      if (q0 + u * TPP >= cnt) continue;
+      const int j = jv[u];
+      double dx = xi - xv[u];
+      double dy = yi - yv[u];
+      double dz = zi - zv[u];
       min_image(k, dx, dy, dz);
       const double r2raw = (dx * dx + dy * dy) + dz * dz;
-      const int tj = type[j] - 1;
+      const int tj = tv[u] - 1;
       const double4 tp = s_pair[ti * MFA_NTYPE + tj];
       const bool in_lj = r2raw < tp.x;
       // force switch-on clamp (fluid_fluid.f90:96); r2min = 0 leaves r2 as is.
@@ -1547,6 +1567,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
       ax += fx;
       ay += fy;
       az += fz;
+      }
     }
   }
 
