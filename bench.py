@@ -53,7 +53,7 @@ STEP_BYTES_PER_PARTICLE = 100
 # (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a copy of known
 # size) and VALU wave-instructions per launch (SQ_INSTS_VALU).  The issue
 # peak: 1024 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz.
-FORCE_COUNTERS = {(4, 2, (15, 15, 15)): (94.979e6 + 9.601e6, 16809694)}
+FORCE_COUNTERS = {(4, 2, (15, 15, 15)): (96.917e6 + 9.603e6, 17107282)}
 VALU_PEAK_GINSTS = 1024 * 2.4 / 4
 
 
