@@ -311,21 +311,15 @@ def bench_single(args, name):
         valu = None
     xt = kernel_xt(p)
     # the dominant kernel of the mixed step: the force pass fused with the
-    # integration where the step runs it so (no k-space stage), else the
-    # force pass alone
-    fused = not p.coulomb
-    if fused:
-        roof = roofline(n, tm_m["step_force_ms"], traffic,
-                        kernel="k_force_mixed<%d,2,%d,%d>" % (tm_m["tpp"], xt, tm_m["pair_queue"]),
-                        bytes_per_particle=STEP_BYTES_PER_PARTICLE)
-    else:
-        roof = roofline(n, tm_m["force_ms"], None,
-                        kernel="k_force_mixed<%d,0,%d,%d>" % (tm_m["tpp"], xt, tm_m["pair_queue"]))
+    # integration (charged systems: and the k-space force's addition)
+    roof = roofline(n, tm_m["step_force_ms"], traffic,
+                    kernel="k_force_mixed<%d,2,%d,%d>" % (tm_m["tpp"], xt, tm_m["pair_queue"]),
+                    bytes_per_particle=STEP_BYTES_PER_PARTICLE)
     roof["force_alone_ms_avg"] = tm_m["force_ms"]
     roof["force_ms_avg_fp64"] = tm_d["force_ms"]
     if valu:
         # the kernel is VALU-issue bound: its instruction rate against the issue peak
-        g = valu / ((tm_m["step_force_ms"] if fused else tm_m["force_ms"]) * 1e-3) / 1e9
+        g = valu / (tm_m["step_force_ms"] * 1e-3) / 1e9
         roof["valu_issue"] = {"achieved": g, "peak": VALU_PEAK_GINSTS,
                               "unit": "G wave-instructions/s", "frac": g / VALU_PEAK_GINSTS,
                               "instructions_per_launch": valu}

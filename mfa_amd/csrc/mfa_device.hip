@@ -224,6 +224,7 @@ struct mfa_ctx {
   double2 *d_tx, *d_ty, *d_tz;  // phase tables [entry][charged particle]
   int* d_qidx;
   double* d_qval;
+  int* d_qinv;      // charged index of each particle, -1 for the neutral ones
   double2 *d_rho_part, *d_grho;
   double *d_uk_part, *d_mz_part, *d_mz, *d_fk_part, *d_ek;
   double v_coul_self;
@@ -1745,6 +1746,16 @@ struct FuseArgs {
   int* cell_count;
   int* rebuild_flags;
   int nc_tot;
+  // charged systems: the k-space force partials of this step's positions
+  // (k_ewald_force*) added to F before the integration, as k_ewald_apply
+  // adds them to F otherwise; qinv[i] = charged index of particle i or -1
+  // (qinv null: no k-space term)
+  const int* qinv;
+  const double* qval;
+  const double* fk_part;
+  const double* mz;
+  double inv_V;
+  int nq, n_kgroups, add_dip;
 };
 
 // The half-list form's buffers: per list slot, the scaled pair scalar of an
@@ -1757,25 +1768,107 @@ struct HalfArgs {
   double* fpart;
 };
 
+// The k-space force partials: per charged particle idx, per force group kg
+// (a block row of the force kernel), three components, particle-major, so
+// that one particle's partials lie together.
+__device__ __forceinline__ size_t fk_index(int idx, int kg, int n_kgroups) {
+  return ((size_t)idx * (size_t)n_kgroups + (size_t)kg) * 3;
+}
+
+// Sum of the partials of charged particle idx, component d, in a fixed
+// order that a lane group can share: eight strands (strand l: groups l,
+// l + 8, l + 16, ... added in turn), then ((t0 + t4) + (t2 + t6)) +
+// ((t1 + t5) + (t3 + t7)).
+__device__ __forceinline__ double fk_strand(const double* __restrict__ fk_part, int idx, int n_kgroups,
+                                            int d, int l) {
+  const double* p = fk_part + fk_index(idx, 0, n_kgroups) + d;
+  double t = 0.0;
+  for (int kg = l; kg < n_kgroups; kg += 8) t += p[3 * (size_t)kg];
+  return t;
+}
+__device__ __forceinline__ double fk_sum(const double* __restrict__ fk_part, int idx, int n_kgroups,
+                                         int d) {
+#pragma clang fp contract(off)
+  double t[8];
+  for (int l = 0; l < 8; ++l) t[l] = fk_strand(fk_part, idx, n_kgroups, d, l);
+  return ((t[0] + t[4]) + (t[2] + t[6])) + ((t[1] + t[5]) + (t[3] + t[7]));
+}
+// The same sum by the TPP lanes of a group (every lane gets it; idx < 0:
+// a particle without charge, 0).  Lane sub holds strands sub, sub + TPP, ...
+// (mod 8), pairs them as the fixed order does and the butterfly finishes it.
+template <int TPP>
+__device__ __forceinline__ double fk_sum_group(const double* __restrict__ fk_part, int idx,
+                                               int n_kgroups, int d, int sub) {
+#pragma clang fp contract(off)
+  auto st = [&](int l) { return idx >= 0 ? fk_strand(fk_part, idx, n_kgroups, d, l) : 0.0; };
+  if constexpr (TPP == 1) {
+    return idx >= 0 ? fk_sum(fk_part, idx, n_kgroups, d) : 0.0;
+  } else if constexpr (TPP == 2) {
+    const double b = (st(sub) + st(sub + 4)) + (st(sub + 2) + st(sub + 6));
+    return b + __shfl_xor(b, 1, 64);
+  } else if constexpr (TPP == 4) {
+    const double a = st(sub) + st(sub + 4);
+    const double b = a + __shfl_xor(a, 2, 64);
+    return b + __shfl_xor(b, 1, 64);
+  } else {
+    const double t = st(sub & 7);
+    const double a = t + __shfl_xor(t, 4, 64);
+    const double b = a + __shfl_xor(a, 2, 64);
+    return b + __shfl_xor(b, 1, 64);
+  }
+}
+
+// F_i,d plus the k-space force on a charged particle from its partial sum s:
+// -8 pi q s / V (ewald_k.f90:177-192), and with add_dip the slab dipole force
+// -4 pi q M_z / V on z (dipolar_correction.f90:26-30).  The same arithmetic,
+// uncontracted, wherever it is added.
+__device__ __forceinline__ double kspace_add(double fd, double s, double q, double inv_V, int add_k,
+                                             int add_dip, int d, const double* __restrict__ mz) {
+#pragma clang fp contract(off)
+  const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
+  if (add_k) fd += -8.0 * pi_f * q * s * inv_V;
+  if (add_dip && d == 2) fd -= 4.0 * pi_f * q * (*mz) * inv_V;
+  return fd;
+}
+
 // The end of a force pass that also integrates: verlet_velocities of this
 // step with F_i (k_fused_integrate's order: a = F/m, v += dt/2 a, pinned
 // heads), then verlet_positions and check_skin of the next.  Positions are
 // read from r (this step's buffer, which the whole pass reads) and written
 // to fu.r_out; only the group's lane 0 reads or writes particle i's v and
 // mic.  Every lane of the launch takes part (ballot, count clearing).
+template <int TPP>
 __device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, int sub, int i, int il,
                                                int ti, int ss, double fx, double fy, double fz,
                                                const double* __restrict__ r, const FuseArgs& fu) {
   bool over = false;
-  if (active && sub == 0) {
-    double fi[3] = {fx, fy, fz}, vi[3], ai[3], ri[3], rr[3];
-    int mi[3];
+  double vi[3], ri[3], rr[3];
+  int mi[3];
+  auto load_state = [&]() {
     for (int d = 0; d < 3; ++d) {
       vi[d] = fu.v_in[3 * (size_t)i + d];
       ri[d] = r[3 * (size_t)i + d];
       mi[d] = fu.mic_in[3 * (size_t)i + d];
       rr[d] = fu.r_ref[3 * (size_t)i + d];
     }
+  };
+  // charged systems: the k-space force of this step, summed by the group;
+  // every lane of the group loads the particle's state with its partials, so
+  // that the two wait for memory once
+  const bool charged = fu.qinv != nullptr;  // uniform over the launch
+  int qidx = -1;
+  double fk[3] = {0.0, 0.0, 0.0};
+  if (charged) {
+    qidx = active ? fu.qinv[i] : -1;
+    if (active) load_state();
+    for (int d = 0; d < 3; ++d) fk[d] = fk_sum_group<TPP>(fu.fk_part, qidx, fu.n_kgroups, d, sub);
+  }
+  if (active && sub == 0) {
+    double fi[3] = {fx, fy, fz}, ai[3];
+    if (qidx >= 0)
+      for (int d = 0; d < 3; ++d)
+        fi[d] = kspace_add(fi[d], fk[d], fu.qval[qidx], fu.inv_V, 1, fu.add_dip, d, fu.mz);
+    if (!charged) load_state();
     vv_one(k, il, ti, fi, vi, ai);
     bool wrapped = false;
     vp_one(k, ri, vi, ai, mi, wrapped);
@@ -2245,7 +2338,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     }
     block_partials(vals, part);
   }
-  if (FUSE) fused_epilogue(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
+  if (FUSE) fused_epilogue<TPP>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // Second pass of the half-list form.  Each lower entry of a row (i -> j, j
@@ -2343,7 +2436,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_lower(
     f[3 * (size_t)i + 2] = fz;
   }
   if (bad) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
-  if (FUSE) fused_epilogue(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
+  if (FUSE) fused_epilogue<TPP>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // ---------------------------------------------------------------------------
@@ -2615,7 +2708,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_ewald_force(EwaldK e, const doubl
   sh[w][lane][2] = fz;
   __syncthreads();
   if (w == 0 && live) {
-    double* o = fk_part + 3 * ((size_t)blockIdx.y * e.nq + idx);
+    double* o = fk_part + fk_index(idx, blockIdx.y, gridDim.y);
     for (int d = 0; d < 3; ++d)
       o[d] = (sh[0][lane][d] + sh[1][lane][d]) + (sh[2][lane][d] + sh[3][lane][d]);
   }
@@ -2735,7 +2828,7 @@ __global__ __launch_bounds__(MFA_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
   for (int u = 0; u < EW_PPL; ++u)
     if (idx[u] < e.nq) {
-      double* o = fk_part + 3 * ((size_t)grp * e.nq + idx[u]);
+      double* o = fk_part + fk_index(idx[u], grp, ngroups);
       o[0] = fx[u];
       o[1] = fy[u];
       o[2] = fz[u];
@@ -2776,26 +2869,8 @@ __global__ void k_ewald_apply(EwaldK e, const int* __restrict__ qidx,
   if (t >= 3 * e.nq) return;
   const int idx = t / 3, d = t - 3 * idx;
   const int i = qidx[idx];
-  const double q = qval[idx];
-  const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
-  double fd = f[3 * (size_t)i + d];
-  if (add_k) {
-    const size_t stride = 3 * (size_t)e.nq;
-    const double* p = fk_part + t;
-    double s = 0.0;
-    int kg = 0;
-    for (; kg + 8 <= n_kgroups; kg += 8) {
-      double v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(kg + u) * stride];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) s += v[u];
-    }
-    for (; kg < n_kgroups; ++kg) s += p[(size_t)kg * stride];
-    fd += -8.0 * pi_f * q * s * inv_V;
-  }
-  if (add_dip && d == 2) fd -= 4.0 * pi_f * q * (*mz) * inv_V;
-  f[3 * (size_t)i + d] = fd;
+  const double sk = add_k ? fk_sum(fk_part, idx, n_kgroups, d) : 0.0;
+  f[3 * (size_t)i + d] = kspace_add(f[3 * (size_t)i + d], sk, qval[idx], inv_V, add_k, add_dip, d, mz);
 }
 
 // u_cou_k = 4 pi inv_V sum g_k |rho_k|^2 (ewald_k.f90:160-170) and the dipole
@@ -3103,6 +3178,40 @@ static bool half_ready(const mfa_ctx* c, int mask) {
          !(mask & TM_ENERGY) && c->pair_s && c->half_cap == c->cap;
 }
 
+static bool ewald_active(const mfa_ctx* c);
+
+// What the force pass of a step that also integrates reads and writes
+// besides its inputs: the other position and view buffers, v and mic in
+// place (or, under mfa_time_force_step, the timing copies), the skin flags,
+// the cell counts, and for charged systems the k-space force partials.
+static FuseArgs fuse_args(const mfa_ctx* c) {
+  FuseArgs fu;
+  memset(&fu, 0, sizeof fu);
+  fu.r_out = c->r_alt;
+  fu.v_in = c->v;
+  fu.mic_in = c->mic;
+  fu.v_out = c->probe_v ? c->probe_v : c->v;
+  fu.mic_out = c->probe_mic ? c->probe_mic : c->mic;
+  fu.rebuild_flags = c->probe_flags ? c->probe_flags : c->flags;
+  fu.r_ref = c->r_ref;
+  fu.cell_of = c->cell_of;
+  fu.pos4_out = c->pos4_alt;
+  fu.vel4_out = c->vel4_alt;
+  fu.cell_count = c->probe_count ? c->probe_count : c->cell_count;
+  fu.nc_tot = c->n_cells;
+  if (ewald_active(c)) {
+    fu.qinv = c->d_qinv;
+    fu.qval = c->d_qval;
+    fu.fk_part = c->d_fk_part;
+    fu.mz = c->d_mz;
+    fu.inv_V = c->ew.inv_V;
+    fu.nq = c->ew.e.nq;
+    fu.n_kgroups = c->ew.n_kgroups;
+    fu.add_dip = c->p.symmetry == 0 ? 1 : 0;  // md_main.f90:128-132
+  }
+  return fu;
+}
+
 template <int TPP, int XT, int Q>
 static void launch_force_txq(mfa_ctx* c, int mask) {
   const int g = grid_of(c->n * TPP);
@@ -3118,18 +3227,7 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
                            c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap,
                            c->f, c->d_step, mask, c->part, c->flags, fu, hf);
         if (mask & TM_FUSE) {
-          fu.r_out = c->r_alt;
-          fu.v_in = c->v;
-          fu.mic_in = c->mic;
-          fu.v_out = c->probe_v ? c->probe_v : c->v;
-          fu.mic_out = c->probe_mic ? c->probe_mic : c->mic;
-          fu.rebuild_flags = c->probe_flags ? c->probe_flags : c->flags;
-          fu.r_ref = c->r_ref;
-          fu.cell_of = c->cell_of;
-          fu.pos4_out = c->pos4_alt;
-          fu.vel4_out = c->vel4_alt;
-          fu.cell_count = c->probe_count ? c->probe_count : c->cell_count;
-          fu.nc_tot = c->n_cells;
+          fu = fuse_args(c);
           hipLaunchKernelGGL((k_force_lower<TPP, 2>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
                              c->d_rmap, c->r, c->pos4, c->perm, c->nl, c->nn, c->cap, c->f, mask,
                              c->flags, fu, hf);
@@ -3142,19 +3240,7 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
       }
     }
     if (mask & TM_FUSE) {
-      fu.r_out = c->r_alt;
-      fu.v_in = c->v;
-      fu.mic_in = c->mic;
-      // outputs as the step writes them, or the timing copies
-      fu.v_out = c->probe_v ? c->probe_v : c->v;
-      fu.mic_out = c->probe_mic ? c->probe_mic : c->mic;
-      fu.rebuild_flags = c->probe_flags ? c->probe_flags : c->flags;
-      fu.r_ref = c->r_ref;
-      fu.cell_of = c->cell_of;
-      fu.pos4_out = c->pos4_alt;
-      fu.vel4_out = c->vel4_alt;
-      fu.cell_count = c->probe_count ? c->probe_count : c->cell_count;
-      fu.nc_tot = c->n_cells;
+      fu = fuse_args(c);
       hipLaunchKernelGGL((k_force_mixed<TPP, 2, XT, Q>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->pos4, c->vel4, c->perm, c->nl, c->nn, c->cap, c->f,
                          c->d_step, mask, c->part, c->flags, fu, hf);
@@ -3453,15 +3539,29 @@ static void swap_buffers(mfa_ctx* c) {
 // the mixed form without the k-space Ewald stage (which adds to F after the
 // pair pass) or DPD_VV, and not while kernels are timed one by one.
 static bool fused_ok(const mfa_ctx* c) {
-  return c->precision == MFA_MIXED && !c->dpd_vv && !ewald_active(c) && !c->timing;
+  return c->precision == MFA_MIXED && !c->dpd_vv && !c->timing;
 }
+
+static int ew_tables(mfa_ctx* c);
+static int ew_rho(mfa_ctx* c);
+static int ew_sum(mfa_ctx* c);
+static int ew_force(mfa_ctx* c);
 
 static int enqueue_middle_step(mfa_ctx* c) {
   if (fused_ok(c)) {
-    // rebuild when due (the step counter moves on there), then the force
-    // pass with verlet_velocities and the next verlet_positions in it
+    // rebuild when due (the step counter moves on there), then (charged
+    // systems) the k-space force partials of this step's positions, then
+    // the force pass with the k-space force, verlet_velocities and the next
+    // verlet_positions in it
     int rc = launch_rebuild(c, nullptr, 1);
     if (rc) return rc;
+    if (ewald_active(c)) {
+      rc = ew_tables(c);
+      if (!rc) rc = ew_rho(c);
+      if (!rc) rc = ew_sum(c);
+      if (!rc) rc = ew_force(c);
+      if (rc) return rc;
+    }
     rc = launch_force(c, TM_ALL | TM_FUSE);
     if (rc) return rc;
     swap_buffers(c);
@@ -3728,14 +3828,14 @@ static int ewald_setup(mfa_ctx* c) {
                    (void**)&c->d_qidx, (void**)&c->d_qval, (void**)&c->d_rho_part,
                    (void**)&c->d_grho, (void**)&c->d_uk_part, (void**)&c->d_mz_part,
                    (void**)&c->d_mz, (void**)&c->d_fk_part, (void**)&c->d_ek, (void**)&c->d_tx,
-                   (void**)&c->d_rows, (void**)&c->d_row_k0};
+                   (void**)&c->d_rows, (void**)&c->d_row_k0, (void**)&c->d_qinv};
   const size_t bytes[] = {nk * sizeof(double4), nr * sizeof(int4), nr * sizeof(int),
                           n * sizeof(int), n * sizeof(double), n_chunk_max * nk * sizeof(double2),
                           nk * sizeof(double2), (size_t)(w.sum_blocks + 1) * sizeof(double),
                           (n / MFA_BLOCK + 1) * sizeof(double), sizeof(double),
                           w.fk_cap * 3 * sizeof(double), 2 * sizeof(double),
-                          ntab * n * sizeof(double2), nrw * sizeof(int4), nrw * sizeof(int)};
-  for (int b = 0; b < 15; ++b)
+                          ntab * n * sizeof(double2), nrw * sizeof(int4), nrw * sizeof(int), n * sizeof(int)};
+  for (int b = 0; b < 16; ++b)
     if (hipMalloc(bufs[b], bytes[b]) != hipSuccess)
       return set_err(nullptr, MFA_ERR_NOMEM, "hipMalloc of %zu bytes (Ewald) failed", bytes[b]);
   c->d_ty = c->d_tx + (size_t)e.nex * n;
@@ -3776,6 +3876,9 @@ static int ewald_set_charges(mfa_ctx* c, const int32_t* a_type) {
   if (qi.empty() || c->ew.e.nsegs == 0) return MFA_OK;
   HIPCHK(c, hipMemcpy(c->d_qidx, qi.data(), qi.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_qval, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<int> qinv((size_t)c->n, -1);
+  for (size_t q = 0; q < qi.size(); ++q) qinv[(size_t)qi[q]] = (int)q;
+  HIPCHK(c, hipMemcpy(c->d_qinv, qinv.data(), qinv.size() * sizeof(int), hipMemcpyHostToDevice));
   return MFA_OK;
 }
 
@@ -4160,6 +4263,7 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
     if (q) hipFree(q);
   if (c->d_rows) hipFree(c->d_rows);
   if (c->d_row_k0) hipFree(c->d_row_k0);
+  if (c->d_qinv) hipFree(c->d_qinv);
   void* eptrs[] = {c->d_kvec,    c->d_runs,    c->d_run_k0, c->d_qidx, c->d_qval,   c->d_rho_part,
                    c->d_grho,    c->d_uk_part, c->d_mz_part, c->d_mz,  c->d_fk_part, c->d_ek,
                    c->d_tx};
