@@ -83,9 +83,10 @@ def run_form(p, st0, precision, steps, warmup, first_step=1, graph_steps=16, tun
     skins = list(configs.SKIN_CANDIDATES) if tune_skin else [p.skin]
     width, skin, _, first_step = ctx.autotune_cells(first_step, 200, configs.cell_width_candidates(p),
                                                     skins)
-    # untimed: the width and list walk again, on the chosen grid and skin
-    tpp, _ = ctx.autotune(first_step, 200)
-    first_step += autotune_trials(p, precision) * (max(graph_steps, 0) + 1 + 200)
+    # untimed: the width and list walk again, on the chosen grid and skin,
+    # over longer trials (their rebuilds vary)
+    tpp, _ = ctx.autotune(first_step, 500)
+    first_step += autotune_trials(p, precision) * (max(graph_steps, 0) + 1 + 500)
     ctx.run_steps(first_step, warmup, want_obs=False)
     tuning = ctx.tuning()
     t0 = time.perf_counter()
