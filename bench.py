@@ -435,8 +435,9 @@ def bench_batch(args, n_rep=256):
               "half_list": {"value": half_list, "chosen_by": "autotuner"},
               "parallelism": "single GPU, replicas batched"}
     extra = {
-        "sequential_value": ntot * args.steps / dt_s,
-        "batched_over_sequential": dt_s / dt_b,
+        # (None with --no-seq: no sequential runs)
+        "sequential_value": ntot * args.steps / dt_s if relaxed else None,
+        "batched_over_sequential": dt_s / dt_b if relaxed else None,
         "kernel_ms": {"force_in_step": force_ms, "rebuild_idle": rb_idle, "rebuild_all": rb_full},
         "roofline": roofline(ntot, force_ms, None,
                              kernel="k_force_mixed<%d,2,%d,%d>" % (tpp, kernel_xt(reps[0][0]), pair_queue),
