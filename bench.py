@@ -115,16 +115,16 @@ def kernel_xt(p):
 
 
 def autotune_trials(p, precision):
-    """Candidates mfa_autotune runs (it advances the state by graph_steps + 1
-    + n steps per candidate): widths 1, 2, 4, 8, 16 directly; in the mixed
+    """Trials mfa_autotune runs, two sweeps over its candidates (it advances
+    the state by graph_steps + 1 + n steps per trial): widths 1, 2, 4, 8, 16 directly; in the mixed
     form also with the skin part queued and, for DPD_CUT_OFF, everything
     queued (systems without charges, bending or the Langevin force), and the
     half-list form at 4, 8 and 16 lanes (for DPD_CUT_OFF also queued)."""
     if precision != lib.MFA_MIXED:
-        return 5
+        return 2 * 5
     qok = not p.coulomb and not p.bending and p.thermostat != 1
     dc = bool(p.dpd_cut_off_on)
-    return 5 + (5 if qok else 0) + (5 if qok and dc else 0) + 3 + (3 if qok and dc else 0)
+    return 2 * (5 + (5 if qok else 0) + (5 if qok and dc else 0) + 3 + (3 if qok and dc else 0))
 
 
 def _fits(p, skin):

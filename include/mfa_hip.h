@@ -433,8 +433,9 @@ int32_t mfa_get_tuning(mfa_ctx* ctx, int32_t* tpp, int32_t* pair_queue, double* 
  * half-list form at 4 or more threads, also with the queue for
  * DPD_CUT_OFF), runs graph_steps + 1 untimed and then n_steps timed whole
  * steps from first_step (advancing the state, so call it during warm-up),
- * rebuilds included; keeps the fastest and returns it with its time per
- * step (ms). */
+ * rebuilds included, in two sweeps over the candidates (each keeps its
+ * faster trial); keeps the fastest and returns it with its time per step
+ * (ms). */
 int32_t mfa_autotune(mfa_ctx* ctx, int64_t first_step, int32_t n_steps, int32_t* best_tpp,
                      double* ms_per_step);
 
