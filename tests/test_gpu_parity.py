@@ -436,6 +436,26 @@ def test_fused_step_matches_separate_kernels(case):
     split.close()
 
 
+@pytest.mark.parametrize("tpp", [1, 2, 4, 16])
+def test_fused_charged_step_every_width(tpp):
+    """The fused step of a charged system at each lane-group width: the
+    group sums the k-space force partials in the fixed order k_ewald_apply
+    uses (each width has its own way of sharing the strands), so the state
+    equals the separate kernels' bit for bit."""
+    inputs, p, r, v, t, a = make_case("pe_brush_small")
+    fused = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED, tpp=tpp)
+    split = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED, tpp=tpp)
+    split.timing(True)
+    of = fused.run_steps(1, 30)
+    os_ = split.run_steps(1, 30)
+    sf, ss = fused.get_state(), split.get_state()
+    assert of["n_rebuilds"] == os_["n_rebuilds"]
+    for k in ("r", "v", "f", "a", "mic"):
+        assert np.array_equal(sf[k], ss[k]), (tpp, k)
+    fused.close()
+    split.close()
+
+
 def test_time_force_step_leaves_the_state():
     """mfa_time_force_step times the fused force kernel without changing the
     state: the steps that follow equal those of an untimed context."""
