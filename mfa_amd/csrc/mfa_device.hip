@@ -199,10 +199,12 @@ struct mfa_ctx {
   unsigned* nl;
   int32_t* nn;
   float4 *pos4, *vel4;  // FP32 view of the mixed form
-  // the other buffers of positions and view: a mixed step whose force pass
-  // also integrates reads r, pos4, vel4 and writes these; the host then swaps
-  // the pointers (parity counts the swaps)
-  double* r_alt;
+  // the other buffers of positions, velocities and view: a step whose force
+  // pass also integrates reads r, v, pos4, vel4 and writes these (the FP64
+  // pass reads the velocities of its neighbours, which other blocks of the
+  // same pass integrate); the host then swaps the pointers (parity counts
+  // the swaps)
+  double *r_alt, *v_alt;
   float4 *pos4_alt, *vel4_alt;
   int parity;
   // set only while mfa_time_force_step times that force pass: copies that
@@ -1426,7 +1428,195 @@ __device__ double bend_angle(const KParams& k, const double* __restrict__ r, int
   return .5 * f_mod * delta_alpha;
 }
 
-template <int TPP, int XT, bool ENERGY>
+// What the force pass of a step that also integrates (either form) reads and writes
+// besides r, the view and the list: the next positions and velocities (the
+// other buffers: the FP64 pass reads its neighbours' velocities), wrap counts
+// (in the step: in place, mic_out = mic_in), the next FP32 view (the other
+// view buffer), the skin test's rebuild flags (one F_NFLAGS row per replica)
+// and zeroed cell counts.
+struct FuseArgs {
+  double* r_out;
+  const double* v_in;
+  double* v_out;
+  const int* mic_in;
+  int* mic_out;
+  const double* r_ref;
+  const int* cell_of;
+  float4* pos4_out;
+  float4* vel4_out;
+  int* cell_count;
+  int* rebuild_flags;
+  int nc_tot;
+  // charged systems: the k-space force partials of this step's positions
+  // (k_ewald_force*) added to F before the integration, as k_ewald_apply
+  // adds them to F otherwise; qinv[i] = charged index of particle i or -1
+  // (qinv null: no k-space term)
+  const int* qinv;
+  const double* qval;
+  const double* fk_part;
+  const double* mz;
+  double inv_V;
+  int nq, n_kgroups, add_dip;
+};
+
+// The half-list form's buffers: per list slot, the scaled pair scalar of an
+// upper entry (written by the first pass, 0 where the pair does not
+// interact) and, for a lower entry, the slot of its pair in the neighbour's
+// row; per particle (sorted order), the first pass's fixed-point sums.
+struct HalfArgs {
+  float* pair_s;
+  uint16_t* mir;
+  double* fpart;
+};
+
+// The k-space force partials: per charged particle idx, per force group kg
+// (a block row of the force kernel), three components, particle-major, so
+// that one particle's partials lie together.
+__device__ __forceinline__ size_t fk_index(int idx, int kg, int n_kgroups) {
+  return ((size_t)idx * (size_t)n_kgroups + (size_t)kg) * 3;
+}
+
+// Sum of the partials of charged particle idx, component d, in a fixed
+// order that a lane group can share: eight strands (strand l: groups l,
+// l + 8, l + 16, ... added in turn), then ((t0 + t4) + (t2 + t6)) +
+// ((t1 + t5) + (t3 + t7)).
+__device__ __forceinline__ double fk_strand(const double* __restrict__ fk_part, int idx, int n_kgroups,
+                                            int d, int l) {
+  const double* p = fk_part + fk_index(idx, 0, n_kgroups) + d;
+  double t = 0.0;
+  for (int kg = l; kg < n_kgroups; kg += 8) t += p[3 * (size_t)kg];
+  return t;
+}
+__device__ __forceinline__ double fk_sum(const double* __restrict__ fk_part, int idx, int n_kgroups,
+                                         int d) {
+#pragma clang fp contract(off)
+  double t[8];
+  for (int l = 0; l < 8; ++l) t[l] = fk_strand(fk_part, idx, n_kgroups, d, l);
+  return ((t[0] + t[4]) + (t[2] + t[6])) + ((t[1] + t[5]) + (t[3] + t[7]));
+}
+// The same sum by the TPP lanes of a group (every lane gets it; idx < 0:
+// a particle without charge, 0).  Lane sub holds strands sub, sub + TPP, ...
+// (mod 8), pairs them as the fixed order does and the butterfly finishes it.
+template <int TPP>
+__device__ __forceinline__ double fk_sum_group(const double* __restrict__ fk_part, int idx,
+                                               int n_kgroups, int d, int sub) {
+#pragma clang fp contract(off)
+  auto st = [&](int l) { return idx >= 0 ? fk_strand(fk_part, idx, n_kgroups, d, l) : 0.0; };
+  if constexpr (TPP == 1) {
+    return idx >= 0 ? fk_sum(fk_part, idx, n_kgroups, d) : 0.0;
+  } else if constexpr (TPP == 2) {
+    const double b = (st(sub) + st(sub + 4)) + (st(sub + 2) + st(sub + 6));
+    return b + __shfl_xor(b, 1, 64);
+  } else if constexpr (TPP == 4) {
+    const double a = st(sub) + st(sub + 4);
+    const double b = a + __shfl_xor(a, 2, 64);
+    return b + __shfl_xor(b, 1, 64);
+  } else {
+    const double t = st(sub & 7);
+    const double a = t + __shfl_xor(t, 4, 64);
+    const double b = a + __shfl_xor(a, 2, 64);
+    return b + __shfl_xor(b, 1, 64);
+  }
+}
+
+// F_i,d plus the k-space force on a charged particle from its partial sum s:
+// -8 pi q s / V (ewald_k.f90:177-192), and with add_dip the slab dipole force
+// -4 pi q M_z / V on z (dipolar_correction.f90:26-30).  The same arithmetic,
+// uncontracted, wherever it is added.
+__device__ __forceinline__ double kspace_add(double fd, double s, double q, double inv_V, int add_k,
+                                             int add_dip, int d, const double* __restrict__ mz) {
+#pragma clang fp contract(off)
+  const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
+  if (add_k) fd += -8.0 * pi_f * q * s * inv_V;
+  if (add_dip && d == 2) fd -= 4.0 * pi_f * q * (*mz) * inv_V;
+  return fd;
+}
+
+// The end of a force pass that also integrates: verlet_velocities of this
+// step with F_i (k_fused_integrate's order: a = F/m, v += dt/2 a, pinned
+// heads), then verlet_positions and check_skin of the next.  Positions are
+// read from r (this step's buffer, which the whole pass reads) and written
+// to fu.r_out; only the group's lane 0 reads or writes particle i's v and
+// mic.  Every lane of the launch takes part (ballot, count clearing).
+// CHARGED: compiled with the k-space force's addition (kernels of systems
+// with charges or bending; fu.qinv says whether there are charges).
+template <int TPP, bool CHARGED>
+__device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, int sub, int i, int il,
+                                               int ti, int ss, double fx, double fy, double fz,
+                                               const double* __restrict__ r, const FuseArgs& fu) {
+  bool over = false;
+  if constexpr (CHARGED) {
+    // the k-space force of this step, summed by the lane group; every lane
+    // of the group loads the particle's state with its partials, so that the
+    // two wait for memory once
+    double vi[3], ri[3], rr[3];
+    int mi[3];
+    int qidx = -1;
+    double fk[3] = {0.0, 0.0, 0.0};
+    if (fu.qinv) {  // uniform over the launch
+      qidx = active ? fu.qinv[i] : -1;
+      if (active)
+        for (int d = 0; d < 3; ++d) {
+          vi[d] = fu.v_in[3 * (size_t)i + d];
+          ri[d] = r[3 * (size_t)i + d];
+          mi[d] = fu.mic_in[3 * (size_t)i + d];
+          rr[d] = fu.r_ref[3 * (size_t)i + d];
+        }
+      for (int d = 0; d < 3; ++d) fk[d] = fk_sum_group<TPP>(fu.fk_part, qidx, fu.n_kgroups, d, sub);
+    }
+    if (active && sub == 0) {
+      double fi[3] = {fx, fy, fz}, ai[3];
+      if (!fu.qinv)
+        for (int d = 0; d < 3; ++d) {
+          vi[d] = fu.v_in[3 * (size_t)i + d];
+          ri[d] = r[3 * (size_t)i + d];
+          mi[d] = fu.mic_in[3 * (size_t)i + d];
+          rr[d] = fu.r_ref[3 * (size_t)i + d];
+        }
+      if (qidx >= 0)
+        for (int d = 0; d < 3; ++d)
+          fi[d] = kspace_add(fi[d], fk[d], fu.qval[qidx], fu.inv_V, 1, fu.add_dip, d, fu.mz);
+      vv_one(k, il, ti, fi, vi, ai);
+      bool wrapped = false;
+      vp_one(k, ri, vi, ai, mi, wrapped);
+      for (int d = 0; d < 3; ++d) {
+        fu.r_out[3 * (size_t)i + d] = ri[d];
+        fu.v_out[3 * (size_t)i + d] = vi[d];
+      }
+      if (wrapped)
+        for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
+      write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
+      over = skin_one(k, ri, rr, mi);
+    }
+  } else {
+    if (active && sub == 0) {
+      double fi[3] = {fx, fy, fz}, vi[3], ai[3], ri[3], rr[3];
+      int mi[3];
+      for (int d = 0; d < 3; ++d) {
+        vi[d] = fu.v_in[3 * (size_t)i + d];
+        ri[d] = r[3 * (size_t)i + d];
+        mi[d] = fu.mic_in[3 * (size_t)i + d];
+        rr[d] = fu.r_ref[3 * (size_t)i + d];
+      }
+      vv_one(k, il, ti, fi, vi, ai);
+      bool wrapped = false;
+      vp_one(k, ri, vi, ai, mi, wrapped);
+      for (int d = 0; d < 3; ++d) {
+        fu.r_out[3 * (size_t)i + d] = ri[d];
+        fu.v_out[3 * (size_t)i + d] = vi[d];
+      }
+      if (wrapped)
+        for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
+      write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
+      over = skin_one(k, ri, rr, mi);
+    }
+  }
+  if (__builtin_amdgcn_ballot_w64(over) != 0ull && (threadIdx.x & 63) == 0)
+    atomicOr(&fu.rebuild_flags[k.rep * F_NFLAGS + F_REBUILD], 1);
+  zero_counts(fu.nc_tot, fu.cell_count);
+}
+
+template <int TPP, int XT, bool ENERGY, bool FUSE>
 __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict__ kb,
                                                        const int* __restrict__ rmap,
                                                        const double* __restrict__ r,
@@ -1438,7 +1628,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
                                                        double* __restrict__ f,
                                                        const long long* __restrict__ d_step,
                                                        int mask, double* __restrict__ part,
-                                                       int* __restrict__ flags_all) {
+                                                       int* __restrict__ flags_all, FuseArgs fu) {
   MFA_REPLICA(blockIdx.x * (MFA_BLOCK / TPP));
   int* flags = flags_all + k.rep * F_NFLAGS;
   __shared__ int err_s;
@@ -1639,7 +1829,10 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
 
   // Combine the lane group and write F_i.
   double fx = group_sum<TPP>(ax), fy = group_sum<TPP>(ay), fz = group_sum<TPP>(az);
-  if (active && sub == 0) {
+  if (FUSE) {
+    if (active && sub == 0 && !(isfinite(fx) && isfinite(fy) && isfinite(fz)))
+      atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
+  } else if (active && sub == 0) {
     if (mask & TM_ACCUM) {
       fx = f[3 * (size_t)i + 0] + fx;
       fy = f[3 * (size_t)i + 1] + fy;
@@ -1666,6 +1859,9 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
     }
     block_partials(vals, part);
   }
+  // a step's pass: the k-space force (charged systems), verlet_velocities of
+  // this step, verlet_positions and check_skin of the next
+  if (FUSE) fused_epilogue<TPP, (XT & 1) != 0>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // The scalar s of one interacting pair of the mixed form, F_ij = s (dx, dy,
@@ -1727,193 +1923,6 @@ struct FmBatch {
 };
 // Pairs a wave's queue holds (a round adds at most 64 to fewer than 64).
 #define FMQ_CAP 128
-
-// What the mixed force pass of a step that also integrates reads and writes
-// besides r, the view and the list: the next positions (the other position
-// buffer), velocities and wrap counts (in the step: in place, v_out = v_in,
-// mic_out = mic_in), the next FP32 view (the other view buffer), the skin
-// test's rebuild flags (one F_NFLAGS row per replica) and zeroed cell counts.
-struct FuseArgs {
-  double* r_out;
-  const double* v_in;
-  double* v_out;
-  const int* mic_in;
-  int* mic_out;
-  const double* r_ref;
-  const int* cell_of;
-  float4* pos4_out;
-  float4* vel4_out;
-  int* cell_count;
-  int* rebuild_flags;
-  int nc_tot;
-  // charged systems: the k-space force partials of this step's positions
-  // (k_ewald_force*) added to F before the integration, as k_ewald_apply
-  // adds them to F otherwise; qinv[i] = charged index of particle i or -1
-  // (qinv null: no k-space term)
-  const int* qinv;
-  const double* qval;
-  const double* fk_part;
-  const double* mz;
-  double inv_V;
-  int nq, n_kgroups, add_dip;
-};
-
-// The half-list form's buffers: per list slot, the scaled pair scalar of an
-// upper entry (written by the first pass, 0 where the pair does not
-// interact) and, for a lower entry, the slot of its pair in the neighbour's
-// row; per particle (sorted order), the first pass's fixed-point sums.
-struct HalfArgs {
-  float* pair_s;
-  uint16_t* mir;
-  double* fpart;
-};
-
-// The k-space force partials: per charged particle idx, per force group kg
-// (a block row of the force kernel), three components, particle-major, so
-// that one particle's partials lie together.
-__device__ __forceinline__ size_t fk_index(int idx, int kg, int n_kgroups) {
-  return ((size_t)idx * (size_t)n_kgroups + (size_t)kg) * 3;
-}
-
-// Sum of the partials of charged particle idx, component d, in a fixed
-// order that a lane group can share: eight strands (strand l: groups l,
-// l + 8, l + 16, ... added in turn), then ((t0 + t4) + (t2 + t6)) +
-// ((t1 + t5) + (t3 + t7)).
-__device__ __forceinline__ double fk_strand(const double* __restrict__ fk_part, int idx, int n_kgroups,
-                                            int d, int l) {
-  const double* p = fk_part + fk_index(idx, 0, n_kgroups) + d;
-  double t = 0.0;
-  for (int kg = l; kg < n_kgroups; kg += 8) t += p[3 * (size_t)kg];
-  return t;
-}
-__device__ __forceinline__ double fk_sum(const double* __restrict__ fk_part, int idx, int n_kgroups,
-                                         int d) {
-#pragma clang fp contract(off)
-  double t[8];
-  for (int l = 0; l < 8; ++l) t[l] = fk_strand(fk_part, idx, n_kgroups, d, l);
-  return ((t[0] + t[4]) + (t[2] + t[6])) + ((t[1] + t[5]) + (t[3] + t[7]));
-}
-// The same sum by the TPP lanes of a group (every lane gets it; idx < 0:
-// a particle without charge, 0).  Lane sub holds strands sub, sub + TPP, ...
-// (mod 8), pairs them as the fixed order does and the butterfly finishes it.
-template <int TPP>
-__device__ __forceinline__ double fk_sum_group(const double* __restrict__ fk_part, int idx,
-                                               int n_kgroups, int d, int sub) {
-#pragma clang fp contract(off)
-  auto st = [&](int l) { return idx >= 0 ? fk_strand(fk_part, idx, n_kgroups, d, l) : 0.0; };
-  if constexpr (TPP == 1) {
-    return idx >= 0 ? fk_sum(fk_part, idx, n_kgroups, d) : 0.0;
-  } else if constexpr (TPP == 2) {
-    const double b = (st(sub) + st(sub + 4)) + (st(sub + 2) + st(sub + 6));
-    return b + __shfl_xor(b, 1, 64);
-  } else if constexpr (TPP == 4) {
-    const double a = st(sub) + st(sub + 4);
-    const double b = a + __shfl_xor(a, 2, 64);
-    return b + __shfl_xor(b, 1, 64);
-  } else {
-    const double t = st(sub & 7);
-    const double a = t + __shfl_xor(t, 4, 64);
-    const double b = a + __shfl_xor(a, 2, 64);
-    return b + __shfl_xor(b, 1, 64);
-  }
-}
-
-// F_i,d plus the k-space force on a charged particle from its partial sum s:
-// -8 pi q s / V (ewald_k.f90:177-192), and with add_dip the slab dipole force
-// -4 pi q M_z / V on z (dipolar_correction.f90:26-30).  The same arithmetic,
-// uncontracted, wherever it is added.
-__device__ __forceinline__ double kspace_add(double fd, double s, double q, double inv_V, int add_k,
-                                             int add_dip, int d, const double* __restrict__ mz) {
-#pragma clang fp contract(off)
-  const double pi_f = (double)3.1415926536f;  // mfa_common.f90:332
-  if (add_k) fd += -8.0 * pi_f * q * s * inv_V;
-  if (add_dip && d == 2) fd -= 4.0 * pi_f * q * (*mz) * inv_V;
-  return fd;
-}
-
-// The end of a force pass that also integrates: verlet_velocities of this
-// step with F_i (k_fused_integrate's order: a = F/m, v += dt/2 a, pinned
-// heads), then verlet_positions and check_skin of the next.  Positions are
-// read from r (this step's buffer, which the whole pass reads) and written
-// to fu.r_out; only the group's lane 0 reads or writes particle i's v and
-// mic.  Every lane of the launch takes part (ballot, count clearing).
-// CHARGED: compiled with the k-space force's addition (kernels of systems
-// with charges or bending; fu.qinv says whether there are charges).
-template <int TPP, bool CHARGED>
-__device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, int sub, int i, int il,
-                                               int ti, int ss, double fx, double fy, double fz,
-                                               const double* __restrict__ r, const FuseArgs& fu) {
-  bool over = false;
-  if constexpr (CHARGED) {
-    // the k-space force of this step, summed by the lane group; every lane
-    // of the group loads the particle's state with its partials, so that the
-    // two wait for memory once
-    double vi[3], ri[3], rr[3];
-    int mi[3];
-    int qidx = -1;
-    double fk[3] = {0.0, 0.0, 0.0};
-    if (fu.qinv) {  // uniform over the launch
-      qidx = active ? fu.qinv[i] : -1;
-      if (active)
-        for (int d = 0; d < 3; ++d) {
-          vi[d] = fu.v_in[3 * (size_t)i + d];
-          ri[d] = r[3 * (size_t)i + d];
-          mi[d] = fu.mic_in[3 * (size_t)i + d];
-          rr[d] = fu.r_ref[3 * (size_t)i + d];
-        }
-      for (int d = 0; d < 3; ++d) fk[d] = fk_sum_group<TPP>(fu.fk_part, qidx, fu.n_kgroups, d, sub);
-    }
-    if (active && sub == 0) {
-      double fi[3] = {fx, fy, fz}, ai[3];
-      if (!fu.qinv)
-        for (int d = 0; d < 3; ++d) {
-          vi[d] = fu.v_in[3 * (size_t)i + d];
-          ri[d] = r[3 * (size_t)i + d];
-          mi[d] = fu.mic_in[3 * (size_t)i + d];
-          rr[d] = fu.r_ref[3 * (size_t)i + d];
-        }
-      if (qidx >= 0)
-        for (int d = 0; d < 3; ++d)
-          fi[d] = kspace_add(fi[d], fk[d], fu.qval[qidx], fu.inv_V, 1, fu.add_dip, d, fu.mz);
-      vv_one(k, il, ti, fi, vi, ai);
-      bool wrapped = false;
-      vp_one(k, ri, vi, ai, mi, wrapped);
-      for (int d = 0; d < 3; ++d) {
-        fu.r_out[3 * (size_t)i + d] = ri[d];
-        fu.v_out[3 * (size_t)i + d] = vi[d];
-      }
-      if (wrapped)
-        for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
-      write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
-      over = skin_one(k, ri, rr, mi);
-    }
-  } else {
-    if (active && sub == 0) {
-      double fi[3] = {fx, fy, fz}, vi[3], ai[3], ri[3], rr[3];
-      int mi[3];
-      for (int d = 0; d < 3; ++d) {
-        vi[d] = fu.v_in[3 * (size_t)i + d];
-        ri[d] = r[3 * (size_t)i + d];
-        mi[d] = fu.mic_in[3 * (size_t)i + d];
-        rr[d] = fu.r_ref[3 * (size_t)i + d];
-      }
-      vv_one(k, il, ti, fi, vi, ai);
-      bool wrapped = false;
-      vp_one(k, ri, vi, ai, mi, wrapped);
-      for (int d = 0; d < 3; ++d) {
-        fu.r_out[3 * (size_t)i + d] = ri[d];
-        fu.v_out[3 * (size_t)i + d] = vi[d];
-      }
-      if (wrapped)
-        for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
-      write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
-      over = skin_one(k, ri, rr, mi);
-    }
-  }
-  if (__builtin_amdgcn_ballot_w64(over) != 0ull && (threadIdx.x & 63) == 0)
-    atomicOr(&fu.rebuild_flags[k.rep * F_NFLAGS + F_REBUILD], 1);
-  zero_counts(fu.nc_tot, fu.cell_count);
-}
 
 // XT bits: 1 Coulomb real space and bending, 2 the Langevin force, 4 DPD_CUT_OFF
 // (a DPD range of its own, heads excluded from DPD).  MODE: 0 forces, 1 forces
@@ -3220,7 +3229,7 @@ static FuseArgs fuse_args(const mfa_ctx* c) {
   fu.r_out = c->r_alt;
   fu.v_in = c->v;
   fu.mic_in = c->mic;
-  fu.v_out = c->probe_v ? c->probe_v : c->v;
+  fu.v_out = c->probe_v ? c->probe_v : c->v_alt;
   fu.mic_out = c->probe_mic ? c->probe_mic : c->mic;
   fu.rebuild_flags = c->probe_flags ? c->probe_flags : c->flags;
   fu.r_ref = c->r_ref;
@@ -3284,14 +3293,22 @@ static void launch_force_txq(mfa_ctx* c, int mask) {
                          c->d_step, mask, c->part, c->flags, fu, hf);
     }
   } else {
-    if (mask & TM_ENERGY)
-      hipLaunchKernelGGL((k_force64<TPP, XT & 3, true>), dim3(g), dim3(MFA_BLOCK), 0, c->stream, c->d_kp,
-                         c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f, c->d_step,
-                         mask, c->part, c->flags);
-    else
-      hipLaunchKernelGGL((k_force64<TPP, XT & 3, false>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+    FuseArgs fu;
+    memset(&fu, 0, sizeof fu);
+    if (mask & TM_FUSE) {
+      fu = fuse_args(c);
+      hipLaunchKernelGGL((k_force64<TPP, XT & 3, false, true>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
                          c->d_kp, c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f,
-                         c->d_step, mask, c->part, c->flags);
+                         c->d_step, mask, c->part, c->flags, fu);
+    } else if (mask & TM_ENERGY) {
+      hipLaunchKernelGGL((k_force64<TPP, XT & 3, true, false>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                         c->d_kp, c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f,
+                         c->d_step, mask, c->part, c->flags, fu);
+    } else {
+      hipLaunchKernelGGL((k_force64<TPP, XT & 3, false, false>), dim3(g), dim3(MFA_BLOCK), 0, c->stream,
+                         c->d_kp, c->d_rmap, c->r, c->v, c->type, c->perm, c->nl, c->nn, c->cap, c->f,
+                         c->d_step, mask, c->part, c->flags, fu);
+    }
   }
 }
 
@@ -3560,17 +3577,16 @@ static bool ewald_active(const mfa_ctx* c);
 // the other ones.
 static void swap_buffers(mfa_ctx* c) {
   std::swap(c->r, c->r_alt);
+  std::swap(c->v, c->v_alt);
   std::swap(c->pos4, c->pos4_alt);
   std::swap(c->vel4, c->vel4_alt);
   c->parity ^= 1;
 }
 
 // Whether the middle steps of mfa_run_steps integrate inside the force pass:
-// the mixed form without the k-space Ewald stage (which adds to F after the
-// pair pass) or DPD_VV, and not while kernels are timed one by one.
-static bool fused_ok(const mfa_ctx* c) {
-  return c->precision == MFA_MIXED && !c->dpd_vv && !c->timing;
-}
+// either form without DPD_VV (whose dissipative force needs the velocities
+// of the whole half step), and not while kernels are timed one by one.
+static bool fused_ok(const mfa_ctx* c) { return !c->dpd_vv && !c->timing; }
 
 static int ew_tables(mfa_ctx* c);
 static int ew_rho(mfa_ctx* c);
@@ -4184,6 +4200,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   ALLOC(c->pos4, n * sizeof(float4));
   ALLOC(c->vel4, n * sizeof(float4));
   ALLOC(c->r_alt, n * 3 * sizeof(double));
+  ALLOC(c->v_alt, n * 3 * sizeof(double));
   ALLOC(c->pos4_alt, n * sizeof(float4));
   ALLOC(c->vel4_alt, n * sizeof(float4));
   ALLOC(c->flags, (size_t)n_rep * F_NFLAGS * sizeof(int32_t));
@@ -4237,6 +4254,7 @@ static int32_t create_impl(const mfa_params* ps, int32_t n_rep, int32_t precisio
   hipMemcpyAsync(c->type, ones.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
   hipMemsetAsync(c->r, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->r_alt, 0, n * 3 * sizeof(double), c->stream);
+  hipMemsetAsync(c->v_alt, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->v, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->r_ref, 0, n * 3 * sizeof(double), c->stream);
   hipMemsetAsync(c->cell_count, 0, (size_t)c->n_cells * sizeof(int32_t), c->stream);
@@ -4288,7 +4306,7 @@ extern "C" int32_t mfa_destroy(mfa_ctx* c) {
                   c->pos4, c->vel4,   c->flags,      c->d_step,     c->part,  c->red,
                   c->d_kp, c->d_rmap, c->d_cell_rep, c->cell_start, c->perm,  c->sidx,
                   c->fd,   c->fd_new, c->fill_cnt, c->head_f, c->d_hist, c->d_chain_rg,
-                  c->d_obs, c->r_alt, c->pos4_alt, c->vel4_alt};
+                  c->d_obs, c->r_alt, c->v_alt, c->pos4_alt, c->vel4_alt};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->d_rows) hipFree(c->d_rows);

@@ -411,17 +411,18 @@ def test_langevin_holds_the_temperature():
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [lib.MFA_MIXED, lib.MFA_FP64])
 @pytest.mark.parametrize("case", ["melt_small", "brush_small", "pe_brush_small", "pe_brush_compat"])
-def test_fused_step_matches_separate_kernels(case):
-    """The mixed step whose force pass also integrates (double-buffered
-    positions and view; for charged systems with the k-space force of the
-    step's positions added before the integration) against the same steps
+def test_fused_step_matches_separate_kernels(case, precision):
+    """The step whose force pass also integrates (either form;
+    double-buffered positions and view; for charged systems with the k-space
+    force of the step's positions added before the integration) against the same steps
     with the integration in its own kernel and the k-space force added by
     k_ewald_apply (kernel timing on switches the fusion off): bit-identical
     state and the same rebuilds, over 60 steps with several rebuilds."""
     inputs, p, r, v, t, a = make_case(case)
-    fused = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
-    split = gpu_ctx(p, r, v, a, t, precision=lib.MFA_MIXED)
+    fused = gpu_ctx(p, r, v, a, t, precision=precision)
+    split = gpu_ctx(p, r, v, a, t, precision=precision)
     split.timing(True)
     of = fused.run_steps(1, 60)
     os_ = split.run_steps(1, 60)
