@@ -70,3 +70,28 @@ def test_time_ewald_keeps_forces():
     with pytest.raises(lib.MfaError):
         ctx.time_ewald(5)
     ctx.close()
+
+
+@pytest.mark.parametrize("n_kmax", list(range(1, 14)))
+def test_k_space_every_row_width_matches_oracle(n_kmax):
+    """The k-space force kernel is compiled once per row width nx_max (1 to
+    12, the row kernel; 13 and wider take the segment kernel): a displaced
+    rock salt with k_c = n_kmax dk runs each width, FP64 forces and v_coul
+    within 1e-12 of the oracle."""
+    x, r, t = _rock_salt(8, 1.5, 1.44, n_kmax, 3.0)
+    r = r + np.random.default_rng(40 + n_kmax).uniform(-0.2, 0.2, r.shape)
+    p = lib.derive(x)
+    rc, plan, why = lib.ewald_plan(p, len(t))
+    assert rc == 0, why
+    assert plan["nx_max"] == n_kmax and plan["row_kernel"] == (1 if n_kmax <= 12 else 0), plan
+    print("rock salt n_kmax", n_kmax, "k vectors", plan["k_vectors"], flush=True)
+    s = orc.System(x)
+    s.set_state(r, np.zeros_like(r), np.zeros_like(r), t)
+    s.binning()
+    fref, eref = s.forces(step=1)
+    ctx = gpu_ctx(p, r, np.zeros_like(r), np.zeros_like(r), t, precision=lib.MFA_FP64)
+    obs = ctx.energies(step=1)
+    f = ctx.get_state()["f"]
+    ctx.close()
+    per, rms = force_error(f, fref)
+    assert per < 1e-12 and rel(obs["v_coul"], eref["v_coul"]) < 1e-12, (per, obs["v_coul"], eref["v_coul"])
