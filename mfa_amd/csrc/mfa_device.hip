@@ -1540,7 +1540,7 @@ __device__ __forceinline__ double kspace_add(double fd, double s, double q, doub
 // mic.  Every lane of the launch takes part (ballot, count clearing).
 // CHARGED: compiled with the k-space force's addition (kernels of systems
 // with charges or bending; fu.qinv says whether there are charges).
-template <int TPP, bool CHARGED>
+template <int TPP, bool CHARGED, bool SPLIT>
 __device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, int sub, int i, int il,
                                                int ti, int ss, double fx, double fy, double fz,
                                                const double* __restrict__ r, const FuseArgs& fu) {
@@ -1587,6 +1587,73 @@ __device__ __forceinline__ void fused_epilogue(const KParams& k, bool active, in
         for (int d = 0; d < 3; ++d) fu.mic_out[3 * (size_t)i + d] = mi[d];
       write_view(k, ss, fu.cell_of[i], ti, il, ri, vi, fu.pos4_out, fu.vel4_out);
       over = skin_one(k, ri, rr, mi);
+    }
+  } else if constexpr (TPP >= 4 && SPLIT) {
+    // lanes 0, 1, 2 of the group integrate x, y, z (vv_one, vp_one and
+    // write_view one component at a time, the same operations); lane 0
+    // gathers the skin test's three terms and adds them in skin_one's order.
+    // (The mixed pass only: in the FP64 pass the extra registers cost more
+    // than the shorter tail saves.)
+    double dd = 0.0;
+    if (active && sub < 3) {
+      const int d = sub;
+      const size_t o = 3 * (size_t)i + d;
+      const double Ld = d == 0 ? k.L[0] : (d == 1 ? k.L[1] : k.L[2]);
+      double vd = fu.v_in[o], rd = r[o];
+      int md = fu.mic_in[o];
+      const double rrd = fu.r_ref[o];
+      const int cell = fu.cell_of[i];
+      const double fd = d == 0 ? fx : (d == 1 ? fy : fz);
+      double ad = fd * k.inv_mass_type[ti];
+      vd = vd + k.half_dt * ad;
+      if (is_head(k, il) && !k.dpd_vv) {
+        const int c = il / k.n_mon;
+        const double sgn = c < k.n_chain / 2 ? 1.0 : -1.0;
+        vd = d == 2 ? 0.0 : sgn * (d == 0 ? k.va[0] : k.va[1]);
+        ad = 0.0;
+      }
+      rd = (rd + k.dt * vd) + k.half_dt_2 * ad;
+      vd = vd + k.half_dt * ad;
+      const bool wraps = d < 2 || k.symmetry != 0;
+      bool wrapped = false;
+      if (wraps) {
+        if (rd > Ld) {
+          rd = rd - Ld;
+          md += 1;
+          wrapped = true;
+        } else if (rd < 0.0) {
+          rd = rd + Ld;
+          md -= 1;
+          wrapped = true;
+        }
+      }
+      fu.r_out[o] = rd;
+      fu.v_out[o] = vd;
+      if (wrapped) fu.mic_out[o] = md;
+      const int cx = cell % k.nb[0], cy = (cell / k.nb[0]) % k.nb[1], cz = cell / (k.nb[0] * k.nb[1]);
+      const int cd = d == 0 ? cx : (d == 1 ? cy : cz);
+      const double bd = d == 0 ? k.r_bin[0] : (d == 1 ? k.r_bin[1] : k.r_bin[2]);
+      const double iLd = d == 0 ? k.invL[0] : (d == 1 ? k.invL[1] : k.invL[2]);
+      double ld = rd - (double)cd * bd;
+      if (d < 2 || k.symmetry == 1) ld -= Ld * rint(ld * iLd);
+      float* pv = reinterpret_cast<float*>(fu.pos4_out + ss);
+      float* vv = reinterpret_cast<float*>(fu.vel4_out + ss);
+      pv[d] = (float)ld;
+      vv[d] = (float)vd;
+      if (d == 0) {
+        pv[3] = __int_as_float((il << 2) | ti);
+        vv[3] = 0.0f;
+      }
+      dd = (rrd - rd) - (double)md * Ld;
+    }
+    const int base = (threadIdx.x & 63) & ~(TPP - 1);
+    const double d1 = __shfl(dd, base + 1, 64), d2 = __shfl(dd, base + 2, 64);
+    if (active && sub == 0) {
+      double s = 0.0;
+      s = s + dd * dd;
+      s = s + d1 * d1;
+      s = s + d2 * d2;
+      over = s > k.skin_half_2;
     }
   } else {
     if (active && sub == 0) {
@@ -1861,7 +1928,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force64(const KParams* __restrict
   }
   // a step's pass: the k-space force (charged systems), verlet_velocities of
   // this step, verlet_positions and check_skin of the next
-  if (FUSE) fused_epilogue<TPP, (XT & 1) != 0>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
+  if (FUSE) fused_epilogue<TPP, (XT & 1) != 0, false>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // The scalar s of one interacting pair of the mixed form, F_ij = s (dx, dy,
@@ -2377,7 +2444,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_mixed(
     block_partials(vals, part);
   }
   // (charged systems carry XT bit 1; only they add a k-space force)
-  if (FUSE) fused_epilogue<TPP, (XT & 1) != 0>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
+  if (FUSE) fused_epilogue<TPP, (XT & 1) != 0, true>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // Second pass of the half-list form.  Each lower entry of a row (i -> j, j
@@ -2475,7 +2542,7 @@ __global__ __launch_bounds__(MFA_BLOCK) void k_force_lower(
     f[3 * (size_t)i + 2] = fz;
   }
   if (bad) atomicCAS(&flags[F_ERROR], 0, MFA_ERR_NAN);
-  if (FUSE) fused_epilogue<TPP, true>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
+  if (FUSE) fused_epilogue<TPP, true, false>(k, active, sub, i, il, ti, ss, fx, fy, fz, r, fu);
 }
 
 // ---------------------------------------------------------------------------
